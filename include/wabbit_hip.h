@@ -1,0 +1,183 @@
+/*
+ * wabbit_hip.h -- C ABI of the MI355X (gfx950) block time-step library.
+ *
+ * Drop-in for the per-block time-step path of WABBIT.  The host keeps the
+ * tree, treecodes, neighbour lists and refinement flags; the library owns
+ * the heavy data (hvy_block, hvy_work, hvy_tmp, hvy_mask) in device memory
+ * and runs the ACM right-hand side, ghost-node synchronisation, Runge-Kutta
+ * stage updates and the dt reduction as HIP kernels in FP64.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative WH_E* code on
+ *    failure; wh_last_error() returns the message of the last failure on the
+ *    calling thread.  The reference aborts instead of returning codes
+ *    (LIB/MODULE/module_globals.f90:131-162); a Fortran binding maps non-zero
+ *    to abort().
+ *  - Heavy data layout is the reference's, Fortran column-major
+ *    hvy_block(nx, ny, nz, neq, Nb) (LIB/MESH/allocate_forest.f90:228), i.e.
+ *    C order [Nb][neq][nz][ny][nx], nx = Bs1+2g, nz = 1 in 2D.  Work slots are
+ *    hvy_work(nx, ny, nz, neq, Nb, nslots): slot is the outermost index.
+ *  - Block ids ("hvy ids") are 0-based here (the reference's are 1-based).
+ *  - One host thread per pool; calls are stream-ordered on the pool's stream.
+ */
+#ifndef WABBIT_HIP_H
+#define WABBIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    WH_OK = 0,
+    WH_E_ARG = -1,      /* invalid argument / size out of range            */
+    WH_E_HIP = -2,      /* HIP runtime error                               */
+    WH_E_STATE = -3,    /* call made before its prerequisite (e.g. no active list) */
+    WH_E_DIVERGED = -4, /* |u| > 1e12 in the divergence guard              */
+    WH_E_NOMEM = -5
+};
+
+/* heavy-data arrays of a pool */
+enum { WH_BLOCK = 0, WH_TMP = 1, WH_MASK = 2, WH_WORK = 16 /* + slot */ };
+
+/* order_discretization (rhs_ACM.f90:374-870) */
+enum { WH_FD2 = 2, WH_FD4 = 4, WH_FD4_TW = 40, WH_FD6 = 6 };
+
+/* ghost-patch operations */
+enum { WH_PATCH_COPY = 0, WH_PATCH_RESTRICT = 1, WH_PATCH_PREDICT = 2 };
+
+typedef struct wh_pool wh_pool;
+
+typedef struct {
+    int dim;            /* 2 or 3 */
+    int bs[3];          /* interior points per direction (bs[2] ignored in 2D) */
+    int g;              /* stored ghost width */
+    int neq;            /* equations in hvy_block (n_eqn) */
+    int nb_max;         /* blocks in the pool (params%number_blocks) */
+    int n_work_slots;   /* hvy_work slots (nrhs_slots) */
+    int n_tmp_eqn;      /* components of hvy_tmp (0: none) */
+    int n_mask;         /* components of hvy_mask (6 for ACM, 0: none) */
+    int mask_per_block; /* 1: hvy_mask(..., Nb); 0: a single shared mask block */
+} wh_pool_desc;
+
+/* type_params_acm subset (LIB/EQUATION/ACMnew/module_ACM.f90:56-149) */
+typedef struct {
+    int dim;
+    int scheme;         /* WH_FD* */
+    int skew_symmetry;
+    int penalization;
+    int use_sponge;
+    double c_0, nu, gamma_p, C_eta, C_sponge;
+    double u_mean_set[3];
+    double CFL, CFL_nu, CFL_eta;
+} wh_acm_params;
+
+/* One ghost patch: copy / restrict / predict a box of the sender into a box
+ * of the receiver.  Indices are 0-based and ghost-inclusive.  For COPY the
+ * sender box has the receiver box's extent; for RESTRICT the sender is read
+ * at stride 2 from send_lo; PREDICT is evaluated from the coarser sender
+ * (see wh_sync_ghosts).  The table is the device form of the reference's
+ * (sender, receiver, relation, level-diff) list and ijkPatches boxes
+ * (LIB/MPI/synchronize_ghosts_generic.f90:352-694, module_mpi.f90:318-396). */
+typedef struct {
+    int32_t recv, send;        /* hvy ids */
+    int32_t recv_lo[3];
+    int32_t send_lo[3];
+    int32_t n[3];
+    int32_t op;                /* WH_PATCH_* */
+    int32_t stage;             /* 1..3: sync stage the patch belongs to */
+    int32_t aux;               /* op-specific (PREDICT: parity/offset code) */
+} wh_patch;
+
+const char *wh_last_error(void);
+int wh_version(void);
+
+/* ---- pool: allocate_forest (LIB/MESH/allocate_forest.f90:1-381) -------- */
+int wh_pool_create(wh_pool **out, const wh_pool_desc *desc);
+int wh_pool_destroy(wh_pool *pool);
+/* bytes of one block of array `array` */
+int wh_pool_block_bytes(const wh_pool *pool, int array, size_t *bytes);
+/* copy `count` consecutive hvy ids starting at `first` between host and device */
+int wh_pool_upload(wh_pool *pool, int array, int first, int count, const double *host);
+int wh_pool_download(wh_pool *pool, int array, int first, int count, double *host);
+/* device address of an array (for zero-copy interop on the same device) */
+int wh_pool_device_ptr(wh_pool *pool, int array, void **dptr);
+int wh_pool_fill(wh_pool *pool, int array, double value);
+int wh_synchronize(wh_pool *pool);
+
+/* ---- topology ----------------------------------------------------------- */
+/* Active list hvy_active(1:hvy_n) (0-based ids) and, per listed block, its
+ * origin and spacing (get_block_spacing_origin, get_block_spacing_origin.f90:6-19):
+ * x0dx[6*k + 0..2] = x0, x0dx[6*k + 3..5] = dx. */
+int wh_set_active(wh_pool *pool, const int32_t *hvy_active, int n, const double *x0dx);
+/* Ghost patch table of one sync case (0: sync_ghosts_RHS_tree "full_leaf",
+ * 1: sync_ghosts_tree "full") for ghost width gs. */
+int wh_set_patches(wh_pool *pool, int sync_case, const wh_patch *patches, int npatch);
+
+/* ---- hot path ----------------------------------------------------------- */
+/* sync_ghosts_RHS_tree / sync_ghosts_tree (synchronize_ghosts_generic.f90:121-174)
+ * on array `array` (WH_BLOCK, WH_TMP or WH_WORK+slot) using the patch table
+ * of `sync_case`. */
+int wh_sync_ghosts(wh_pool *pool, int array, int sync_case);
+
+/* RHS_wrapper local stage (RHS_wrapper.f90:124-142 -> RHS_ACM rhs_ACM.f90:236-277):
+ * rhs of every active block of `in_array` into work slot `out_slot`. */
+int wh_rhs_acm(wh_pool *pool, const wh_acm_params *p, double time, int in_array, int out_slot);
+
+/* integral-stage divergence guard (rhs_ACM.f90:133-146): returns
+ * WH_E_DIVERGED if any interior |u| exceeds 1e12; *maxabs gets the max. */
+int wh_divergence_guard(wh_pool *pool, int in_array, double *maxabs);
+
+/* GET_DT_BLOCK_ACM over all active blocks and its minimum
+ * (module_ACM.f90:617-691, calculate_time_step.f90:28-48). */
+int wh_dt_reduce(wh_pool *pool, const wh_acm_params *p, double *dt_out);
+
+/* RungeKuttaGeneric building block (runge_kutta_generic.f90:63-67, 90-112,
+ * 136-154): interior of `out_array` = work[base_slot] + sum_i coef[i]*work[slot[i]]
+ * over all active blocks, terms added in the given order; with base_slot < 0
+ * the base is `out_array` itself.  Also used for the u0 copy (nterms = 0). */
+int wh_rk_combine(wh_pool *pool, int out_array, int base_array, int nterms,
+                  const int *term_arrays, const double *coefs);
+
+/* One full RungeKuttaGeneric step: sync + (dt given) + stages + final update.
+ * `path` 0: reference-structured (one kernel per operation, every active
+ * block in one launch); 1: one launch per block ("plain", as the reference
+ * loops over blocks); 2: fused stage kernels (RHS + RK update in one launch
+ * per stage, needs a tableau whose stage rows have only a sub-diagonal
+ * entry, e.g. classic RK4).  butcher is (nrows x nrows) row-major. */
+int wh_rk_step(wh_pool *pool, const wh_acm_params *p, const double *butcher, int nrows,
+               double time, double dt, int path);
+
+/* Time controls of calculate_time_step (calculate_time_step.f90:22-118) and
+ * the main-loop stop time (main.f90:293), for the device-side time loop. */
+typedef struct {
+    double time_max, dt_fixed, dt_max;
+    double write_time, write_time_first, tsave_stats, write_backup_time;
+    int write_fixed_time; /* write_method == "fixed_time" */
+    int write_backup_n;
+} wh_time_params;
+
+int wh_time_set(wh_pool *pool, double time, int iteration);
+int wh_time_get(wh_pool *pool, double *time, int *iteration, double *dt_last);
+
+/* `nsteps` main-loop time steps with dt, its clamps and the time advance kept
+ * on the device: per step the block-dt reduction, the clamps, a sync and the
+ * fused stage kernels (path 2 of wh_rk_step).  use_graph = 1 captures one
+ * step into a hipGraph once and replays it (re-captured when the active list,
+ * the patch tables or the parameters change).  Steps at or beyond time_max
+ * take dt = 0 and leave the state and iteration unchanged. */
+int wh_run_steps(wh_pool *pool, const wh_acm_params *p, const double *butcher, int nrows,
+                 const wh_time_params *tp, int nsteps, int use_graph);
+
+/* HIP-event timing of kernel kinds (0: RHS or fused stage kernel, 1: ghost
+ * sync) on the pool's stream: average ms per launch since the last reset.
+ * Timing serialises host and device; enable it only for measurement. */
+int wh_kernel_timing_enable(wh_pool *pool, int on);
+int wh_kernel_timing(wh_pool *pool, int which, double *avg_ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,97 @@
+"""Convert the reference's own regression data (HDF5) into small .npz fixtures.
+
+Runs only where the reference tree is mounted (read-only, at
+/root/reference); the GPU machine only sees the .npz files this writes.  The
+data are the reference's committed golden inputs/outputs, unchanged:
+
+  vortex2d_ic.npz          TESTING/acm/3vortices/{ux,uy,p}_000010000000.h5
+                           (t=10, iteration 3054; shared input of the three
+                           3vorticesEqui* tests)
+  vortex2d_fd{2,4,6}.npz   TESTING/acm/3vortices/3vorticesEquiFD{2,4,6}_CDF{20,40,60}/
+                           {ux,uy,p}_000020000000.h5 (t=20, iteration 6127)
+  tg3d_fd{2,4,6}.npz       TESTING/acm/taylorGreen/taylorGreenEqui_FD{2,4,6}_CDF{20,40,60}/
+                           {ux,uy,uz,p}_00000{0,10}000000.h5 (t=0 input, t=10 output)
+
+Each array keeps the file's (N, Bs+1, ..) storage: the interior plus the
+first upper ghost point.  Parameters of each run are copied from its
+PARAMS_*.ini into the fixture's ``params_json`` entry.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from wabbit_amd.h5io import read_wabbit_file  # noqa: E402
+
+REF = "/root/reference/TESTING/acm"
+
+
+def _fields(dirpath, names, tag):
+    out = {}
+    meta = None
+    for n in names:
+        f = read_wabbit_file(os.path.join(dirpath, f"{n}_{tag}.h5"))
+        out[n] = f.blocks
+        if meta is None:
+            meta = f
+        else:
+            assert np.array_equal(meta.treecode, f.treecode)
+    return out, meta
+
+
+def _save(path, fields, meta, params):
+    np.savez_compressed(
+        path,
+        **{f"field_{k}": v for k, v in fields.items()},
+        treecode=meta.treecode, level=meta.level, origin=meta.origin,
+        spacing=meta.spacing, time=meta.time, iteration=meta.iteration,
+        dim=meta.dim, domain_size=meta.domain_size, bs=meta.bs,
+        max_level=meta.max_level,
+        params_json=json.dumps(params),
+    )
+    print("wrote", path, os.path.getsize(path))
+
+
+def vortex2d():
+    names = ["ux", "uy", "p"]
+    ic, meta = _fields(os.path.join(REF, "3vortices"), names, "000010000000")
+    _save(os.path.join(HERE, "vortex2d_ic.npz"), ic, meta, {})
+    for order, wav in ((2, "CDF20"), (4, "CDF40"), (6, "CDF60")):
+        d = os.path.join(REF, "3vortices", f"3vorticesEquiFD{order}_{wav}")
+        # all three runs start from the shared input
+        ic2, _ = _fields(d, names, "000010000000")
+        for n in names:
+            assert np.array_equal(ic2[n], ic[n])
+        out, m = _fields(d, names, "000020000000")
+        assert np.array_equal(m.treecode, meta.treecode)
+        params = dict(scheme=order, wavelet=wav, c_0=7.0, nu=5.0e-5, gamma_p=1.0,
+                      CFL=1.0, skew_symmetry=1, write_time=10.0, time_max=20.0,
+                      Bs=32, Jmax=3, L=float(meta.domain_size[0]))
+        _save(os.path.join(HERE, f"vortex2d_fd{order}.npz"), out, m, params)
+
+
+def taylor_green():
+    names = ["ux", "uy", "uz", "p"]
+    for order, wav in ((2, "CDF20"), (4, "CDF40"), (6, "CDF60")):
+        d = os.path.join(REF, "taylorGreen", f"taylorGreenEqui_FD{order}_{wav}")
+        ic, meta = _fields(d, names, "000000000000")
+        out, m = _fields(d, names, "000010000000")
+        assert np.array_equal(m.treecode, meta.treecode)
+        params = dict(scheme=order, wavelet=wav, c_0=10.0, nu=3.125e-3, gamma_p=0.0,
+                      CFL=1.0, skew_symmetry=1, write_time=10.0, time_max=10.0,
+                      tsave_stats=0.2, Bs=20, Jmax=1, L=float(meta.domain_size[0]),
+                      t0=meta.time, it0=int(meta.iteration), t1=m.time, it1=int(m.iteration))
+        fields = {f"ic_{k}": v for k, v in ic.items()}
+        fields.update({f"out_{k}": v for k, v in out.items()})
+        _save(os.path.join(HERE, f"tg3d_fd{order}.npz"), fields, m, params)
+
+
+if __name__ == "__main__":
+    vortex2d()
+    taylor_green()

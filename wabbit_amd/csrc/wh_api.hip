@@ -1,0 +1,661 @@
+// wh_api.hip -- extern "C" entry points of libwabbit_hip.so (see include/wabbit_hip.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wh_internal.h"
+
+namespace wh {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+double *array_ptr(wh_pool *P, int array, size_t *blk_out, int *ncomp)
+{
+    if (array == WH_BLOCK) {
+        if (blk_out) *blk_out = P->blk;
+        if (ncomp) *ncomp = P->d.neq;
+        return P->block;
+    }
+    if (array == WH_TMP) {
+        if (blk_out) *blk_out = P->tblk;
+        if (ncomp) *ncomp = P->d.n_tmp_eqn;
+        return P->tmp;
+    }
+    if (array == WH_MASK) {
+        if (blk_out) *blk_out = P->mblk;
+        if (ncomp) *ncomp = P->d.n_mask;
+        return P->mask;
+    }
+    if (array >= WH_WORK && array < WH_WORK + P->d.n_work_slots) {
+        if (blk_out) *blk_out = P->blk;
+        if (ncomp) *ncomp = P->d.neq;
+        return P->work + (size_t)(array - WH_WORK) * P->d.nb_max * P->blk;
+    }
+    return nullptr;
+}
+
+// Time one kernel-kind launch sequence with events on the pool stream.
+template <class F>
+static int timed(wh_pool *P, int which, F &&f)
+{
+    if (!P->timing) return f();
+    WH_HIP(hipEventRecord(P->ev_a, P->stream));
+    int rc = f();
+    if (rc) return rc;
+    WH_HIP(hipEventRecord(P->ev_b, P->stream));
+    WH_HIP(hipEventSynchronize(P->ev_b));
+    float ms = 0.f;
+    WH_HIP(hipEventElapsedTime(&ms, P->ev_a, P->ev_b));
+    P->t_ms[which] += ms;
+    P->t_cnt[which] += 1;
+    return WH_OK;
+}
+
+static int read_scalar(wh_pool *P, int which, double *out)
+{
+    WH_HIP(hipMemcpyAsync(P->h_scal + which, P->scal + which, sizeof(double),
+                          hipMemcpyDeviceToHost, P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    *out = P->h_scal[which];
+    return WH_OK;
+}
+
+static TimeDev make_time_dev(const wh_time_params *tp)
+{
+    TimeDev t{};
+    t.time_max = tp->time_max;
+    t.dt_fixed = tp->dt_fixed;
+    t.dt_max = tp->dt_max;
+    t.write_time = tp->write_time;
+    t.write_time_first = tp->write_time_first;
+    t.tsave_stats = tp->tsave_stats;
+    t.write_backup_time = tp->write_backup_time;
+    t.write_fixed_time = tp->write_fixed_time;
+    t.write_backup_n = tp->write_backup_n;
+    t.use_stats = std::fabs(tp->tsave_stats - 9999999.9) > 1e-3;
+    return t;
+}
+
+// The fused path needs every stage row j>=2 of the tableau to carry only its
+// sub-diagonal coefficient a(j, j-1) (classic RK4, midpoint rule, ...).
+static bool fused_ok(const double *bt, int nrows)
+{
+    if (nrows < 3) return false;
+    for (int j = 1; j < nrows - 1; ++j)
+        for (int l = 1; l < nrows; ++l) {
+            if (l == j) continue;  // a(j+1, j) in 1-based terms
+            if (std::fabs(bt[j * nrows + l]) >= 1.0e-8) return false;
+        }
+    return true;
+}
+
+// one step of the fused path; dt is taken from the device scalar DS_DT
+static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows)
+{
+    const int s = nrows - 1;  // number of stages
+    double *S = P->block;
+    double *acc = array_ptr(P, WH_WORK + 0, nullptr, nullptr);
+    double *T[2] = {array_ptr(P, WH_WORK + 1, nullptr, nullptr),
+                    array_ptr(P, WH_WORK + 2, nullptr, nullptr)};
+    int rc;
+    if ((rc = timed(P, 1, [&] { return launch_sync(P, 0, S, -1); }))) return rc;
+    // stage 1: input S, writes acc and T[0]
+    if ((rc = timed(P, 0, [&] {
+             return launch_fused_stage(P, a, S, T[0], nullptr, acc, nullptr, 0, bt[1 * nrows + 1],
+                                       bt[s * nrows + 1], 1);
+         })))
+        return rc;
+    int cur = 0;
+    for (int j = 2; j <= s; ++j) {
+        double *tin = T[cur];
+        if ((rc = timed(P, 1, [&] { return launch_sync(P, 0, tin, -1); }))) return rc;
+        if (j < s) {
+            double *tout = T[cur ^ 1];
+            if ((rc = timed(P, 0, [&] {
+                     return launch_fused_stage(P, a, tin, tout, S, acc, nullptr, 1,
+                                               bt[j * nrows + j], bt[s * nrows + j], 1);
+                 })))
+                return rc;
+            cur ^= 1;
+        } else {
+            if ((rc = timed(P, 0, [&] {
+                     return launch_fused_stage(P, a, tin, nullptr, nullptr, acc, S, 2, 0.0,
+                                               bt[s * nrows + j], 1);
+                 })))
+                return rc;
+        }
+    }
+    return WH_OK;
+}
+
+// reference-structured step; per_block = 1 launches every operation once per block
+static int generic_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows, double dt,
+                        int per_block)
+{
+    const int ncol = nrows;
+    double *S = P->block;
+    auto W = [&](int slot) { return array_ptr(P, WH_WORK + slot, nullptr, nullptr); };
+    auto sync = [&](double *arr) -> int {
+        if (!per_block) return timed(P, 1, [&] { return launch_sync(P, 0, arr, -1); });
+        for (int r = 0; r < P->ps[0].nrecv; ++r) {
+            int rc = timed(P, 1, [&] { return launch_sync(P, 0, arr, r); });
+            if (rc) return rc;
+        }
+        return WH_OK;
+    };
+    auto rhs = [&](double *in, double *out) -> int {
+        if (!per_block)
+            return timed(P, 0, [&] {
+                return launch_rhs(P, a, in, out, P->d_active, P->d_x0dx, P->nact);
+            });
+        for (int k = 0; k < P->nact; ++k) {
+            int rc = timed(P, 0, [&] {
+                return launch_rhs(P, a, in, out, P->d_active + k, P->d_x0dx + 6 * k, 1);
+            });
+            if (rc) return rc;
+        }
+        return WH_OK;
+    };
+    auto combine = [&](double *out, const double *base, int nt, const double *const *terms,
+                       const double *coefs) -> int {
+        if (!per_block) return launch_combine(P, out, base, nt, terms, coefs, P->d_active, P->nact);
+        for (int k = 0; k < P->nact; ++k) {
+            int rc = launch_combine(P, out, base, nt, terms, coefs, P->d_active + k, 1);
+            if (rc) return rc;
+        }
+        return WH_OK;
+    };
+    int rc;
+    // runge_kutta_generic.f90:52 sync, :63-67 u0 copy, :72 first RHS
+    if ((rc = sync(S))) return rc;
+    if ((rc = combine(W(0), S, 0, nullptr, nullptr))) return rc;
+    if ((rc = rhs(S, W(1)))) return rc;
+    for (int j = 2; j <= nrows - 1; ++j) {  // :80-127
+        std::vector<const double *> terms;
+        std::vector<double> coefs;
+        for (int l = 2; l <= j; ++l) {
+            const double c = bt[(j - 1) * ncol + (l - 1)];
+            if (std::fabs(c) < 1.0e-8) continue;
+            terms.push_back(W(l - 1));
+            coefs.push_back(dt * c);
+        }
+        if ((rc = combine(S, W(0), (int)terms.size(), terms.data(), coefs.data()))) return rc;
+        if ((rc = sync(S))) return rc;
+        if ((rc = rhs(S, W(j)))) return rc;
+    }
+    // :136-154 final stage
+    std::vector<const double *> terms;
+    std::vector<double> coefs;
+    for (int j = 2; j <= ncol; ++j) {
+        const double c = bt[(nrows - 1) * ncol + (j - 1)];
+        if (std::fabs(c) < 1.0e-8) continue;
+        terms.push_back(W(j - 1));
+        coefs.push_back(dt * c);
+    }
+    return combine(S, W(0), (int)terms.size(), terms.data(), coefs.data());
+}
+
+// Checks that the kernels' reads stay inside the pool for these parameters.
+static int check_acm(const wh_pool *P, const wh_acm_params *p)
+{
+    WH_CHECK(p->dim == P->d.dim, WH_E_ARG, "params dim does not match the pool");
+    WH_CHECK(P->d.neq >= p->dim + 1, WH_E_ARG, "pool has fewer equations than the ACM needs");
+    WH_CHECK(P->d.dim == 3 || P->d.neq == 3, WH_E_ARG, "2D ACM pools hold exactly 3 equations");
+    const int reach = p->scheme == WH_FD2 ? 1 : p->scheme == WH_FD4 ? 2
+                      : (p->scheme == WH_FD4_TW || p->scheme == WH_FD6) ? 3 : -1;
+    WH_CHECK(reach > 0, WH_E_ARG, "unknown discretization scheme");
+    WH_CHECK(P->d.g >= reach, WH_E_ARG, "ghost width smaller than the stencil reach");
+    WH_CHECK(!((p->penalization || p->use_sponge) && (P->d.n_mask < 6 || !P->mask)), WH_E_ARG,
+             "penalization/sponge need a 6-component mask");
+    WH_CHECK(p->C_eta > 0.0 && p->C_sponge > 0.0, WH_E_ARG, "C_eta and C_sponge must be > 0");
+    return WH_OK;
+}
+
+static void free_patchset(PatchSet &ps)
+{
+    if (ps.d_patches) (void)hipFree(ps.d_patches);
+    if (ps.d_recv_off) (void)hipFree(ps.d_recv_off);
+    if (ps.d_recv_ids) (void)hipFree(ps.d_recv_ids);
+    ps = PatchSet{};
+}
+
+static void destroy_graph(wh_pool *P)
+{
+    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    P->gexec = nullptr;
+    P->graph = nullptr;
+    P->graph_topo = ~0ull;
+}
+
+}  // namespace wh
+
+using namespace wh;
+
+extern "C" {
+
+const char *wh_last_error(void) { return g_err.c_str(); }
+
+int wh_version(void) { return 100; }
+
+int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
+{
+    WH_CHECK(out && desc, WH_E_ARG, "null argument");
+    const wh_pool_desc &d = *desc;
+    WH_CHECK(d.dim == 2 || d.dim == 3, WH_E_ARG, "dim must be 2 or 3");
+    WH_CHECK(d.bs[0] >= 4 && d.bs[1] >= 4 && (d.dim == 2 || d.bs[2] >= 4), WH_E_ARG,
+             "Bs must be >= 4");
+    WH_CHECK(d.g >= 1 && d.neq >= 1 && d.nb_max >= 1 && d.n_work_slots >= 0, WH_E_ARG,
+             "bad g/neq/nb_max/n_work_slots");
+    WH_CHECK(d.n_tmp_eqn >= 0 && d.n_mask >= 0, WH_E_ARG, "bad n_tmp_eqn/n_mask");
+    wh_pool *P = new wh_pool();
+    P->d = d;
+    P->nx = d.bs[0] + 2 * d.g;
+    P->ny = d.bs[1] + 2 * d.g;
+    P->nz = d.dim == 3 ? d.bs[2] + 2 * d.g : 1;
+    P->cs = (size_t)P->nx * P->ny * P->nz;
+    P->blk = P->cs * d.neq;
+    P->tblk = P->cs * d.n_tmp_eqn;
+    P->mblk = P->cs * d.n_mask;
+    auto fail = [&](const char *what, hipError_t e) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        wh_pool_destroy(P);
+        return e == hipErrorOutOfMemory ? WH_E_NOMEM : WH_E_HIP;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail("hipStreamCreate", e);
+    if ((e = hipMalloc(&P->block, sizeof(double) * P->blk * d.nb_max)) != hipSuccess)
+        return fail("hipMalloc(hvy_block)", e);
+    if (d.n_work_slots > 0 &&
+        (e = hipMalloc(&P->work, sizeof(double) * P->blk * d.nb_max * d.n_work_slots)) != hipSuccess)
+        return fail("hipMalloc(hvy_work)", e);
+    if (P->tblk && (e = hipMalloc(&P->tmp, sizeof(double) * P->tblk * d.nb_max)) != hipSuccess)
+        return fail("hipMalloc(hvy_tmp)", e);
+    if (P->mblk &&
+        (e = hipMalloc(&P->mask, sizeof(double) * P->mblk * (d.mask_per_block ? d.nb_max : 1))) !=
+            hipSuccess)
+        return fail("hipMalloc(hvy_mask)", e);
+    if ((e = hipMalloc(&P->scal, sizeof(double) * DS_COUNT)) != hipSuccess)
+        return fail("hipMalloc(scalars)", e);
+    if ((e = hipHostMalloc((void **)&P->h_scal, sizeof(double) * DS_COUNT, hipHostMallocDefault)) !=
+        hipSuccess)
+        return fail("hipHostMalloc", e);
+    if ((e = hipMemsetAsync(P->scal, 0, sizeof(double) * DS_COUNT, P->stream)) != hipSuccess)
+        return fail("hipMemset", e);
+    if ((e = hipMemsetAsync(P->block, 0, sizeof(double) * P->blk * d.nb_max, P->stream)) != hipSuccess)
+        return fail("hipMemset", e);
+    if (P->work && (e = hipMemsetAsync(P->work, 0, sizeof(double) * P->blk * d.nb_max * d.n_work_slots,
+                                       P->stream)) != hipSuccess)
+        return fail("hipMemset", e);
+    if ((e = hipEventCreate(&P->ev_a)) != hipSuccess) return fail("hipEventCreate", e);
+    if ((e = hipEventCreate(&P->ev_b)) != hipSuccess) return fail("hipEventCreate", e);
+    if ((e = hipStreamSynchronize(P->stream)) != hipSuccess) return fail("sync", e);
+    *out = P;
+    return WH_OK;
+}
+
+int wh_pool_destroy(wh_pool *P)
+{
+    if (!P) return WH_OK;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    destroy_graph(P);
+    free_patchset(P->ps[0]);
+    free_patchset(P->ps[1]);
+    for (double *p : {P->block, P->work, P->tmp, P->mask, P->scal, P->d_x0dx})
+        if (p) (void)hipFree(p);
+    if (P->d_active) (void)hipFree(P->d_active);
+    if (P->h_scal) (void)hipHostFree(P->h_scal);
+    if (P->ev_a) (void)hipEventDestroy(P->ev_a);
+    if (P->ev_b) (void)hipEventDestroy(P->ev_b);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+    return WH_OK;
+}
+
+int wh_pool_block_bytes(const wh_pool *P, int array, size_t *bytes)
+{
+    WH_CHECK(P && bytes, WH_E_ARG, "null argument");
+    size_t blk = 0;
+    double *ptr = array_ptr(const_cast<wh_pool *>(P), array, &blk, nullptr);
+    WH_CHECK(ptr || blk == 0, WH_E_ARG, "unknown array");
+    *bytes = blk * sizeof(double);
+    return WH_OK;
+}
+
+static int xfer(wh_pool *P, int array, int first, int count, void *host, bool up)
+{
+    WH_CHECK(P && host, WH_E_ARG, "null argument");
+    size_t blk = 0;
+    double *base = array_ptr(P, array, &blk, nullptr);
+    WH_CHECK(base, WH_E_ARG, "unknown or unallocated array");
+    const int nb = (array == WH_MASK && !P->d.mask_per_block) ? 1 : P->d.nb_max;
+    WH_CHECK(first >= 0 && count >= 0 && first + count <= nb, WH_E_ARG, "block range out of pool");
+    double *dev = base + (size_t)first * blk;
+    const size_t bytes = sizeof(double) * blk * count;
+    if (up)
+        WH_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, P->stream));
+    else
+        WH_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+int wh_pool_upload(wh_pool *P, int array, int first, int count, const double *host)
+{
+    return xfer(P, array, first, count, const_cast<double *>(host), true);
+}
+
+int wh_pool_download(wh_pool *P, int array, int first, int count, double *host)
+{
+    return xfer(P, array, first, count, host, false);
+}
+
+int wh_pool_device_ptr(wh_pool *P, int array, void **dptr)
+{
+    WH_CHECK(P && dptr, WH_E_ARG, "null argument");
+    double *p = array_ptr(P, array, nullptr, nullptr);
+    WH_CHECK(p, WH_E_ARG, "unknown or unallocated array");
+    *dptr = p;
+    return WH_OK;
+}
+
+int wh_pool_fill(wh_pool *P, int array, double value)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    size_t blk = 0;
+    double *p = array_ptr(P, array, &blk, nullptr);
+    WH_CHECK(p, WH_E_ARG, "unknown or unallocated array");
+    const int nb = (array == WH_MASK && !P->d.mask_per_block) ? 1 : P->d.nb_max;
+    std::vector<double> h(blk * nb, value);
+    WH_HIP(hipMemcpyAsync(p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+int wh_synchronize(wh_pool *P)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
+{
+    WH_CHECK(P && (n == 0 || (act && x0dx)), WH_E_ARG, "null argument");
+    WH_CHECK(n >= 0 && n <= P->d.nb_max, WH_E_ARG, "active list longer than the pool");
+    std::vector<char> seen(P->d.nb_max, 0);
+    for (int k = 0; k < n; ++k) {
+        WH_CHECK(act[k] >= 0 && act[k] < P->d.nb_max, WH_E_ARG, "hvy id out of pool range");
+        WH_CHECK(!seen[act[k]], WH_E_ARG, "duplicate hvy id in active list");
+        seen[act[k]] = 1;
+        for (int d = 0; d < P->d.dim; ++d)
+            WH_CHECK(x0dx[6 * k + 3 + d] > 0.0, WH_E_ARG, "block spacing must be positive");
+    }
+    WH_HIP(hipStreamSynchronize(P->stream));
+    if (n > P->act_cap) {
+        if (P->d_active) WH_HIP(hipFree(P->d_active));
+        if (P->d_x0dx) WH_HIP(hipFree(P->d_x0dx));
+        P->d_active = nullptr;
+        P->d_x0dx = nullptr;
+        WH_HIP(hipMalloc(&P->d_active, sizeof(int32_t) * n));
+        WH_HIP(hipMalloc(&P->d_x0dx, sizeof(double) * 6 * n));
+        P->act_cap = n;
+    }
+    P->h_active.assign(act, act + n);
+    P->h_x0dx.assign(x0dx, x0dx + 6 * n);
+    if (n) {
+        WH_HIP(hipMemcpyAsync(P->d_active, act, sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
+        WH_HIP(hipMemcpyAsync(P->d_x0dx, x0dx, sizeof(double) * 6 * n, hipMemcpyHostToDevice,
+                              P->stream));
+        WH_HIP(hipStreamSynchronize(P->stream));
+    }
+    P->nact = n;
+    P->topo_version++;
+    return WH_OK;
+}
+
+int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatch)
+{
+    WH_CHECK(P && (npatch == 0 || patches), WH_E_ARG, "null argument");
+    WH_CHECK(sync_case == 0 || sync_case == 1, WH_E_ARG, "sync_case must be 0 or 1");
+    WH_CHECK(npatch >= 0, WH_E_ARG, "negative patch count");
+    const int ext[3] = {P->nx, P->ny, P->nz};
+    for (int i = 0; i < npatch; ++i) {
+        const wh_patch &q = patches[i];
+        WH_CHECK(q.recv >= 0 && q.recv < P->d.nb_max && q.send >= 0 && q.send < P->d.nb_max,
+                 WH_E_ARG, "patch block id out of pool range");
+        WH_CHECK(q.op == WH_PATCH_COPY || q.op == WH_PATCH_RESTRICT, WH_E_ARG,
+                 "patch op not supported by this build");
+        WH_CHECK(q.stage >= 1 && q.stage <= 3, WH_E_ARG, "patch stage must be 1..3");
+        const int st = q.op == WH_PATCH_RESTRICT ? 2 : 1;
+        for (int d = 0; d < 3; ++d) {
+            WH_CHECK(q.n[d] >= 1, WH_E_ARG, "empty patch");
+            WH_CHECK(q.recv_lo[d] >= 0 && q.recv_lo[d] + q.n[d] <= ext[d], WH_E_ARG,
+                     "receiver box outside the block");
+            WH_CHECK(q.send_lo[d] >= 0 && q.send_lo[d] + st * (q.n[d] - 1) < ext[d], WH_E_ARG,
+                     "sender box outside the block");
+        }
+    }
+    WH_HIP(hipStreamSynchronize(P->stream));
+    PatchSet &ps = P->ps[sync_case];
+    free_patchset(ps);
+    ps.host.assign(patches, patches + npatch);
+    std::stable_sort(ps.host.begin(), ps.host.end(),
+                     [](const wh_patch &a, const wh_patch &b) { return a.recv < b.recv; });
+    ps.h_recv_off.clear();
+    ps.h_recv_ids.clear();
+    for (int i = 0; i < npatch; ++i) {
+        if (i == 0 || ps.host[i].recv != ps.host[i - 1].recv) {
+            ps.h_recv_off.push_back(i);
+            ps.h_recv_ids.push_back(ps.host[i].recv);
+        }
+        ps.max_stage = std::max(ps.max_stage, (int)ps.host[i].stage);
+    }
+    ps.h_recv_off.push_back(npatch);
+    ps.n = npatch;
+    ps.nrecv = (int)ps.h_recv_ids.size();
+    if (npatch) {
+        WH_HIP(hipMalloc(&ps.d_patches, sizeof(wh_patch) * npatch));
+        WH_HIP(hipMalloc(&ps.d_recv_off, sizeof(int32_t) * ps.h_recv_off.size()));
+        WH_HIP(hipMalloc(&ps.d_recv_ids, sizeof(int32_t) * ps.h_recv_ids.size()));
+        WH_HIP(hipMemcpyAsync(ps.d_patches, ps.host.data(), sizeof(wh_patch) * npatch,
+                              hipMemcpyHostToDevice, P->stream));
+        WH_HIP(hipMemcpyAsync(ps.d_recv_off, ps.h_recv_off.data(),
+                              sizeof(int32_t) * ps.h_recv_off.size(), hipMemcpyHostToDevice, P->stream));
+        WH_HIP(hipMemcpyAsync(ps.d_recv_ids, ps.h_recv_ids.data(),
+                              sizeof(int32_t) * ps.h_recv_ids.size(), hipMemcpyHostToDevice, P->stream));
+        WH_HIP(hipStreamSynchronize(P->stream));
+    }
+    P->topo_version++;
+    return WH_OK;
+}
+
+int wh_sync_ghosts(wh_pool *P, int array, int sync_case)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    WH_CHECK(sync_case == 0 || sync_case == 1, WH_E_ARG, "sync_case must be 0 or 1");
+    WH_CHECK(array == WH_BLOCK || array >= WH_WORK, WH_E_ARG, "sync works on hvy_block/hvy_work");
+    double *arr = array_ptr(P, array, nullptr, nullptr);
+    WH_CHECK(arr, WH_E_ARG, "unknown or unallocated array");
+    return timed(P, 1, [&] { return launch_sync(P, sync_case, arr, -1); });
+}
+
+int wh_rhs_acm(wh_pool *P, const wh_acm_params *p, double /*time*/, int in_array, int out_slot)
+{
+    WH_CHECK(P && p, WH_E_ARG, "null argument");
+    if (int rc = check_acm(P, p)) return rc;
+    double *in = array_ptr(P, in_array, nullptr, nullptr);
+    double *out = array_ptr(P, WH_WORK + out_slot, nullptr, nullptr);
+    WH_CHECK(in && out && (in_array == WH_BLOCK || in_array >= WH_WORK), WH_E_ARG, "bad array");
+    WH_CHECK(P->nact > 0 || P->h_active.empty(), WH_E_STATE, "no active list");
+    const AcmDev a = make_acm_dev(p);
+    return timed(P, 0, [&] { return launch_rhs(P, a, in, out, P->d_active, P->d_x0dx, P->nact); });
+}
+
+int wh_divergence_guard(wh_pool *P, int in_array, double *maxabs)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    double *in = array_ptr(P, in_array, nullptr, nullptr);
+    WH_CHECK(in, WH_E_ARG, "bad array");
+    int rc = launch_maxabs(P, in);
+    if (rc) return rc;
+    double m = 0;
+    if ((rc = read_scalar(P, DS_MAXABS_BITS, &m))) return rc;
+    if (maxabs) *maxabs = m;
+    if (!(m <= 1.0e12)) {  // LIM_DIVERGED, module_globals.f90:52
+        set_error("ACM fail: very very large values in state vector");
+        return WH_E_DIVERGED;
+    }
+    return WH_OK;
+}
+
+int wh_dt_reduce(wh_pool *P, const wh_acm_params *p, double *dt_out)
+{
+    WH_CHECK(P && p && dt_out, WH_E_ARG, "null argument");
+    if (int rc = check_acm(P, p)) return rc;
+    const AcmDev a = make_acm_dev(p);
+    int rc = launch_dt(P, a);
+    if (rc) return rc;
+    return read_scalar(P, DS_DTMIN_BITS, dt_out);
+}
+
+int wh_rk_combine(wh_pool *P, int out_array, int base_array, int nterms, const int *term_arrays,
+                  const double *coefs)
+{
+    WH_CHECK(P && (nterms == 0 || (term_arrays && coefs)), WH_E_ARG, "null argument");
+    double *out = array_ptr(P, out_array, nullptr, nullptr);
+    const double *base = array_ptr(P, base_array < 0 ? out_array : base_array, nullptr, nullptr);
+    WH_CHECK(out && base, WH_E_ARG, "bad array");
+    std::vector<const double *> terms(nterms);
+    for (int t = 0; t < nterms; ++t) {
+        terms[t] = array_ptr(P, term_arrays[t], nullptr, nullptr);
+        WH_CHECK(terms[t], WH_E_ARG, "bad term array");
+    }
+    return launch_combine(P, out, base, nterms, terms.data(), coefs, P->d_active, P->nact);
+}
+
+int wh_rk_step(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows, double /*time*/,
+               double dt, int path)
+{
+    WH_CHECK(P && p && bt, WH_E_ARG, "null argument");
+    WH_CHECK(nrows >= 2 && nrows <= 9, WH_E_ARG, "butcher tableau size");
+    if (int rc = check_acm(P, p)) return rc;
+    const AcmDev a = make_acm_dev(p);
+    if (path == 2) {
+        WH_CHECK(fused_ok(bt, nrows), WH_E_ARG, "tableau not supported by the fused path");
+        WH_CHECK(P->d.n_work_slots >= 3, WH_E_ARG, "fused path needs 3 work slots");
+        int rc = launch_set_scalar(P, DS_DT, dt);
+        if (rc) return rc;
+        return fused_step(P, a, bt, nrows);
+    }
+    WH_CHECK(path == 0 || path == 1, WH_E_ARG, "path must be 0, 1 or 2");
+    WH_CHECK(P->d.n_work_slots >= nrows, WH_E_ARG, "need nrows work slots");
+    return generic_step(P, a, bt, nrows, dt, path);
+}
+
+int wh_time_set(wh_pool *P, double time, int iteration)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    int rc = launch_set_scalar(P, DS_TIME, time);
+    if (!rc) rc = launch_set_scalar(P, DS_ITER, (double)iteration);
+    if (!rc) rc = launch_set_scalar(P, DS_DT, 0.0);
+    if (rc) return rc;
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+int wh_time_get(wh_pool *P, double *time, int *iteration, double *dt_last)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    WH_HIP(hipMemcpyAsync(P->h_scal, P->scal, sizeof(double) * DS_COUNT, hipMemcpyDeviceToHost,
+                          P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    if (time) *time = P->h_scal[DS_TIME];
+    if (iteration) *iteration = (int)P->h_scal[DS_ITER];
+    if (dt_last) *dt_last = P->h_scal[DS_DT];
+    return WH_OK;
+}
+
+static int one_device_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows,
+                           const TimeDev &tc)
+{
+    int rc;
+    if ((rc = launch_dt(P, a))) return rc;
+    if ((rc = launch_dt_finalize(P, tc))) return rc;
+    if ((rc = fused_step(P, a, bt, nrows))) return rc;
+    return launch_time_advance(P);
+}
+
+int wh_run_steps(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows,
+                 const wh_time_params *tp, int nsteps, int use_graph)
+{
+    WH_CHECK(P && p && bt && tp, WH_E_ARG, "null argument");
+    WH_CHECK(nrows >= 3 && nrows <= 9 && fused_ok(bt, nrows), WH_E_ARG,
+             "tableau not supported by the fused path");
+    WH_CHECK(P->d.n_work_slots >= 3, WH_E_ARG, "device time loop needs 3 work slots");
+    if (int rc = check_acm(P, p)) return rc;
+    WH_CHECK(nsteps >= 0, WH_E_ARG, "negative step count");
+    const AcmDev a = make_acm_dev(p);
+    const TimeDev tc = make_time_dev(tp);
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; ++s) {
+            int rc = one_device_step(P, a, bt, nrows, tc);
+            if (rc) return rc;
+        }
+        return WH_OK;
+    }
+    WH_CHECK(!P->timing, WH_E_STATE, "kernel timing cannot be captured into a graph");
+    const bool same = P->gexec && P->graph_topo == P->topo_version &&
+                      std::memcmp(&P->graph_p, p, sizeof(*p)) == 0 &&
+                      std::memcmp(&P->graph_tc, &tc, sizeof(tc)) == 0 &&
+                      P->graph_bt == std::vector<double>(bt, bt + nrows * nrows);
+    if (!same) {
+        destroy_graph(P);
+        WH_HIP(hipStreamSynchronize(P->stream));
+        WH_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+        int rc = one_device_step(P, a, bt, nrows, tc);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(P->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        WH_HIP(e);
+        P->graph = g;
+        WH_HIP(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
+        P->graph_topo = P->topo_version;
+        P->graph_p = *p;
+        P->graph_tc = tc;
+        P->graph_bt.assign(bt, bt + nrows * nrows);
+    }
+    for (int s = 0; s < nsteps; ++s) WH_HIP(hipGraphLaunch(P->gexec, P->stream));
+    return WH_OK;
+}
+
+int wh_kernel_timing_enable(wh_pool *P, int on)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    P->timing = on != 0;
+    return WH_OK;
+}
+
+int wh_kernel_timing(wh_pool *P, int which, double *avg_ms, int *count)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    if (which < 0) {
+        P->t_ms[0] = P->t_ms[1] = 0.0;
+        P->t_cnt[0] = P->t_cnt[1] = 0;
+        return WH_OK;
+    }
+    WH_CHECK(which <= 1, WH_E_ARG, "which must be 0 or 1");
+    if (avg_ms) *avg_ms = P->t_cnt[which] ? P->t_ms[which] / P->t_cnt[which] : 0.0;
+    if (count) *count = P->t_cnt[which];
+    return WH_OK;
+}
+
+}  // extern "C"

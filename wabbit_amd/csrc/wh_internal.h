@@ -1,0 +1,124 @@
+// wh_internal.h -- shared declarations of the gfx950 block time-step library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wabbit_hip.h"
+
+namespace wh {
+
+void set_error(const std::string &msg);
+
+#define WH_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            ::wh::set_error(std::string(#call) + ": " + hipGetErrorString(e_));      \
+            return WH_E_HIP;                                                         \
+        }                                                                            \
+    } while (0)
+
+#define WH_CHECK(cond, code, msg)                                                    \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            ::wh::set_error(msg);                                                    \
+            return code;                                                             \
+        }                                                                            \
+    } while (0)
+
+// Scalars the kernels read/write in device memory (one array per pool).
+enum DevScalar {
+    DS_DTMIN_BITS = 0,   // uint64 bit pattern of min block dt (atomicMin)
+    DS_MAXABS_BITS = 1,  // uint64 bit pattern of max |u| (atomicMax)
+    DS_TIME = 2,         // current time (device-side time loop)
+    DS_DT = 3,           // dt of the current step
+    DS_ITER = 4,         // iteration counter (stored as double)
+    DS_COUNT = 8
+};
+
+// Kernel-side copy of the ACM parameters.
+struct AcmDev {
+    double c02, nu, gamma_p, Cinv, Csp_inv;
+    double ums[3];
+    double CFL, CFL_nu, CFL_eta, C_eta, C_sponge, c_0;
+    int dim, scheme, skew, penal, sponge;
+};
+
+// Host-side time controls (calculate_time_step.f90:58-118) evaluated on device.
+struct TimeDev {
+    double time_max, dt_fixed, dt_max, write_time, write_time_first, tsave_stats,
+        write_backup_time;
+    int write_fixed_time, write_backup_n, use_stats;
+};
+
+struct PatchSet {
+    std::vector<wh_patch> host;     // sorted by receiver
+    wh_patch *d_patches = nullptr;
+    int32_t *d_recv_off = nullptr;  // CSR over distinct receivers
+    int32_t *d_recv_ids = nullptr;  // receiver hvy id per CSR row
+    std::vector<int32_t> h_recv_off, h_recv_ids;
+    int n = 0, nrecv = 0;
+    int max_stage = 0;
+};
+
+}  // namespace wh
+
+struct wh_pool {
+    wh_pool_desc d{};
+    int nx = 0, ny = 0, nz = 0;
+    size_t cs = 0;        // points per component (nx*ny*nz)
+    size_t blk = 0;       // doubles per block of hvy_block / hvy_work
+    size_t tblk = 0;      // doubles per block of hvy_tmp
+    size_t mblk = 0;      // doubles per block of hvy_mask
+    double *block = nullptr, *work = nullptr, *tmp = nullptr, *mask = nullptr;
+    double *scal = nullptr;       // wh::DS_COUNT doubles
+    double *h_scal = nullptr;     // pinned mirror
+    int32_t *d_active = nullptr;
+    double *d_x0dx = nullptr;
+    std::vector<int32_t> h_active;
+    std::vector<double> h_x0dx;
+    int nact = 0, act_cap = 0;
+    wh::PatchSet ps[2];
+    hipStream_t stream = nullptr;
+    // timing
+    bool timing = false;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    double t_ms[2] = {0, 0};
+    int t_cnt[2] = {0, 0};
+    // graph
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t topo_version = 0, graph_topo = ~0ull;
+    wh_acm_params graph_p{};
+    std::vector<double> graph_bt;
+    wh::TimeDev graph_tc{};
+};
+
+namespace wh {
+
+double *array_ptr(wh_pool *p, int array, size_t *blk_out, int *ncomp);
+
+// launchers (wh_kernels.hip)
+int launch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *out, const int32_t *act,
+               const double *x0dx, int nact);
+int launch_sync(wh_pool *P, int sync_case, double *arr, int one_receiver);
+int launch_combine(wh_pool *P, double *out, const double *base, int nterms,
+                   const double *const *terms, const double *coefs, const int32_t *act,
+                   int nact);
+int launch_dt(wh_pool *P, const AcmDev &a);
+int launch_maxabs(wh_pool *P, const double *in);
+int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *tout,
+                       const double *u0, double *acc, double *sout, int mode, double acoef,
+                       double bcoef, int coef_from_dev);
+int launch_dt_finalize(wh_pool *P, const TimeDev &tc);
+int launch_time_advance(wh_pool *P);
+int launch_set_scalar(wh_pool *P, int which, double v);
+
+AcmDev make_acm_dev(const wh_acm_params *p);
+
+}  // namespace wh

@@ -1,0 +1,638 @@
+// wh_kernels.hip -- FP64 HIP kernels of the block time step for gfx950.
+//
+// Every kernel works on the device block pool in the reference's heavy-data
+// layout [Nb][neq][nz][ny][nx] (allocate_forest.f90:228) and on the active
+// list hvy_active.  Arithmetic follows the reference's term order and is
+// compiled with -ffp-contract=off, so each product and sum rounds on its own
+// exactly as the CPU code does.
+#include "wh_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace wh {
+
+// ---------------------------------------------------------------------------
+// FD stencils (rhs_ACM.f90:335-344 and :958-965)
+// ---------------------------------------------------------------------------
+__constant__ double cA4[5] = {1.0 / 12.0, -2.0 / 3.0, 0.0, 2.0 / 3.0, -1.0 / 12.0};
+__constant__ double cB4[5] = {-1.0 / 12.0, 4.0 / 3.0, -5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0};
+__constant__ double cTW[7] = {-0.02651995, 0.18941314, -0.79926643, 0.0,
+                              0.79926643, -0.18941314, 0.02651995};
+__constant__ double cA6[7] = {-1.0 / 60.0, 3.0 / 20.0, -3.0 / 4.0, 0.0,
+                              3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
+__constant__ double cB6[7] = {1.0 / 90.0, -3.0 / 20.0, 3.0 / 2.0, -49.0 / 18.0,
+                              3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
+
+template <int S> struct Reach;
+template <> struct Reach<WH_FD2> { static constexpr int R = 1; };
+template <> struct Reach<WH_FD4> { static constexpr int R = 2; };
+template <> struct Reach<WH_FD4_TW> { static constexpr int R = 3; };
+template <> struct Reach<WH_FD6> { static constexpr int R = 3; };
+
+// first derivative along stride s
+template <int S>
+__device__ __forceinline__ double d1(const double *f, ptrdiff_t s, double inv)
+{
+    if constexpr (S == WH_FD2) {
+        return (f[s] - f[-s]) * inv * 0.5;
+    } else if constexpr (S == WH_FD4) {
+        return (cA4[0] * f[-2 * s] + cA4[1] * f[-s] + cA4[3] * f[s] + cA4[4] * f[2 * s]) * inv;
+    } else if constexpr (S == WH_FD4_TW) {
+        return (cTW[0] * f[-3 * s] + cTW[1] * f[-2 * s] + cTW[2] * f[-s] + cTW[4] * f[s] +
+                cTW[5] * f[2 * s] + cTW[6] * f[3 * s]) * inv;
+    } else {
+        return (cA6[0] * f[-3 * s] + cA6[1] * f[-2 * s] + cA6[2] * f[-s] + cA6[4] * f[s] +
+                cA6[5] * f[2 * s] + cA6[6] * f[3 * s]) * inv;
+    }
+}
+
+// first derivative of the product fa*fb (skew-symmetric form); terms (c*fa)*fb
+template <int S>
+__device__ __forceinline__ double d1p(const double *fa, const double *fb, ptrdiff_t s, double inv)
+{
+    if constexpr (S == WH_FD2) {
+        return (fa[s] * fb[s] - fa[-s] * fb[-s]) * inv * 0.5;
+    } else if constexpr (S == WH_FD4) {
+        return (cA4[0] * fa[-2 * s] * fb[-2 * s] + cA4[1] * fa[-s] * fb[-s] +
+                cA4[3] * fa[s] * fb[s] + cA4[4] * fa[2 * s] * fb[2 * s]) * inv;
+    } else {
+        const double *c = (S == WH_FD4_TW) ? cTW : cA6;
+        return (c[0] * fa[-3 * s] * fb[-3 * s] + c[1] * fa[-2 * s] * fb[-2 * s] +
+                c[2] * fa[-s] * fb[-s] + c[4] * fa[s] * fb[s] + c[5] * fa[2 * s] * fb[2 * s] +
+                c[6] * fa[3 * s] * fb[3 * s]) * inv;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ double d2(const double *f, ptrdiff_t s, double inv2)
+{
+    if constexpr (S == WH_FD2) {
+        return (f[-s] - 2.0 * f[0] + f[s]) * inv2;
+    } else if constexpr (S == WH_FD4 || S == WH_FD4_TW) {
+        return (cB4[0] * f[-2 * s] + cB4[1] * f[-s] + cB4[2] * f[0] + cB4[3] * f[s] +
+                cB4[4] * f[2 * s]) * inv2;
+    } else {
+        return (cB6[0] * f[-3 * s] + cB6[1] * f[-2 * s] + cB6[2] * f[-s] + cB6[3] * f[0] +
+                cB6[4] * f[s] + cB6[5] * f[2 * s] + cB6[6] * f[3 * s]) * inv2;
+    }
+}
+
+// ACM right-hand side at one point (RHS_2D_acm / RHS_3D_acm).  uc points at
+// component 0 of the point; components are cs apart; s1/s2 are the y/z
+// strides.  mk: mask component 0 at this point (components mcs apart) or null.
+template <int DIM, int S, bool SKEW>
+__device__ __forceinline__ void acm_point(const double *__restrict__ uc, ptrdiff_t cs, ptrdiff_t s1,
+                                          ptrdiff_t s2, const double *inv, const double *inv2,
+                                          const AcmDev &a, const double *__restrict__ mk,
+                                          ptrdiff_t mcs, double *r)
+{
+    const ptrdiff_t st[3] = {1, s1, s2};
+    double uv[DIM + 1];
+    double du[DIM + 1][DIM];
+#pragma unroll
+    for (int c = 0; c <= DIM; ++c) {
+        uv[c] = uc[c * cs];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) du[c][d] = d1<S>(uc + c * cs, st[d], inv[d]);
+    }
+    double chi = 0.0;
+    if (a.penal && mk) {
+        const int color = (int)mk[4 * mcs];
+        chi = mk[0] * (color == 0 ? 0.0 : a.Cinv);
+    }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        double lap = d2<S>(uc + c * cs, st[0], inv2[0]);
+#pragma unroll
+        for (int d = 1; d < DIM; ++d) lap = lap + d2<S>(uc + c * cs, st[d], inv2[d]);
+        double t;
+        if constexpr (SKEW) {
+            double sum = d1p<S>(uc + c * cs, uc, st[0], inv[0]);
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) sum = sum + d1p<S>(uc + c * cs, uc + d * cs, st[d], inv[d]);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) sum = sum + uv[d] * du[c][d];
+            t = -(0.5 * sum);
+        } else {
+            t = -(uv[0] * du[c][0]);
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) t = t - uv[d] * du[c][d];
+        }
+        t = t - du[DIM][c];
+        t = t + a.nu * lap;
+        if (a.penal && mk) t = t + (-(chi * (uv[c] - mk[(1 + c) * mcs])));
+        r[c] = t;
+    }
+    double div = du[0][0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) div = div + du[d][d];
+    r[DIM] = -(a.c02 * div) - a.gamma_p * uv[DIM];
+    if (a.sponge && mk) {
+        const double spo = mk[5 * mcs] * a.Csp_inv;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) r[c] = r[c] - (uv[c] - a.ums[c]) * spo;
+        r[DIM] = r[DIM] - uv[DIM] * spo;
+    }
+}
+
+struct Geo {
+    int bs0, bs1, bs2, g, nx, ny;
+    size_t blk, cs, mblk;
+    int mask_per_block;
+};
+
+// Runge-Kutta epilogue of the fused stage kernel.
+//   mode 0 (first stage): u0 = input state; acc = u0 + (dt*b)*k; tout = u0 + (dt*a)*k
+//   mode 1 (middle):      acc = acc + (dt*b)*k;  tout = u0 + (dt*a)*k
+//   mode 2 (last):        sout = acc + (dt*b)*k
+// b == 0 skips the acc term as the reference skips |coef| < 1e-8
+// (runge_kutta_generic.f90:99,144).
+struct StageArgs {
+    const double *u0;
+    double *acc, *tout, *sout;
+    const double *scal;
+    double a, b;
+    int mode, use_b, use_a;
+};
+
+struct NoEpilogue {};
+
+template <int NC>
+__device__ __forceinline__ void stage_epilogue(const StageArgs &sa, size_t o, size_t cs,
+                                               const double *r, const double *ucen,
+                                               ptrdiff_t ucs)
+{
+    const double dt = sa.scal[DS_DT];
+    const double ca = dt * sa.a, cb = dt * sa.b;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const size_t i = o + c * cs;
+        if (sa.mode == 0) {
+            const double u0 = ucen[c * ucs];
+            sa.acc[i] = sa.use_b ? u0 + cb * r[c] : u0;
+            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+        } else if (sa.mode == 1) {
+            const double u0 = sa.u0[i];
+            if (sa.use_b) sa.acc[i] = sa.acc[i] + cb * r[c];
+            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+        } else {
+            sa.sout[i] = sa.use_b ? sa.acc[i] + cb * r[c] : sa.acc[i];
+        }
+    }
+}
+
+// 2D: one workgroup per block, the (Bs+2R)^2 x 3 halo tile staged in LDS.
+template <int S, bool SKEW, bool FUSED>
+__global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, double *__restrict__ out,
+                                               const double *__restrict__ mask,
+                                               const int32_t *__restrict__ act,
+                                               const double *__restrict__ x0dx, Geo G, AcmDev a,
+                                               StageArgs sa)
+{
+    constexpr int R = Reach<S>::R;
+    extern __shared__ double tile[];
+    const int k = blockIdx.x;
+    const int b = act[k];
+    const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R, tcs = tx * ty;
+    const double *u = in + (size_t)b * G.blk;
+    for (int i = threadIdx.x; i < 3 * tcs; i += blockDim.x) {
+        const int c = i / tcs, rr = i - c * tcs;
+        const int iy = rr / tx, ix = rr - iy * tx;
+        tile[i] = u[c * G.cs + (size_t)(G.g - R + iy) * G.nx + (G.g - R + ix)];
+    }
+    __syncthreads();
+    const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4];
+    const double inv[2] = {1.0 / dx0, 1.0 / dx1};
+    const double inv2[2] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1)};
+    const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
+    for (int pnt = threadIdx.x; pnt < G.bs0 * G.bs1; pnt += blockDim.x) {
+        const int iy = pnt / G.bs0, ix = pnt - iy * G.bs0;
+        const double *uc = tile + (iy + R) * tx + (ix + R);
+        const size_t gi = (size_t)(G.g + iy) * G.nx + (G.g + ix);
+        double r[3];
+        acm_point<2, S, SKEW>(uc, tcs, tx, 0, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
+        const size_t o = (size_t)b * G.blk + gi;
+        if constexpr (FUSED) {
+            stage_epilogue<3>(sa, o, G.cs, r, uc, tcs);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[o + c * G.cs] = r[c];
+        }
+    }
+}
+
+// 3D: 256 interior points per workgroup, stencil reads served by L1/L2.
+template <int S, bool SKEW, bool FUSED>
+__global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, double *__restrict__ out,
+                                               const double *__restrict__ mask,
+                                               const int32_t *__restrict__ act,
+                                               const double *__restrict__ x0dx, Geo G, AcmDev a,
+                                               StageArgs sa)
+{
+    const int npts = G.bs0 * G.bs1 * G.bs2;
+    const int nchunk = (npts + 255) / 256;
+    const int k = blockIdx.x / nchunk;
+    const int pnt = (blockIdx.x - k * nchunk) * 256 + threadIdx.x;
+    if (pnt >= npts) return;
+    const int b = act[k];
+    const int ix = pnt % G.bs0, iy = (pnt / G.bs0) % G.bs1, iz = pnt / (G.bs0 * G.bs1);
+    const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4], dx2 = x0dx[6 * k + 5];
+    const double inv[3] = {1.0 / dx0, 1.0 / dx1, 1.0 / dx2};
+    const double inv2[3] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1), 1.0 / (dx2 * dx2)};
+    const size_t gi = ((size_t)(G.g + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
+    const double *uc = in + (size_t)b * G.blk + gi;
+    const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) + gi : nullptr;
+    double r[4];
+    acm_point<3, S, SKEW>(uc, G.cs, G.nx, (ptrdiff_t)G.nx * G.ny, inv, inv2, a, mk, G.cs, r);
+    const size_t o = (size_t)b * G.blk + gi;
+    if constexpr (FUSED) {
+        stage_epilogue<4>(sa, o, G.cs, r, uc, G.cs);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
+    }
+}
+
+static Geo make_geo(const wh_pool *P)
+{
+    Geo G;
+    G.bs0 = P->d.bs[0];
+    G.bs1 = P->d.bs[1];
+    G.bs2 = P->d.dim == 3 ? P->d.bs[2] : 1;
+    G.g = P->d.g;
+    G.nx = P->nx;
+    G.ny = P->ny;
+    G.blk = P->blk;
+    G.cs = P->cs;
+    G.mblk = P->mblk;
+    G.mask_per_block = P->d.mask_per_block;
+    return G;
+}
+
+template <bool FUSED>
+static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *out,
+                        const int32_t *act, const double *x0dx, int nact, const StageArgs &sa)
+{
+    if (nact <= 0) return WH_OK;
+    const Geo G = make_geo(P);
+    const double *mask = (P->mask && (a.penal || a.sponge)) ? P->mask : nullptr;
+    if (P->d.dim == 2) {
+        int R = a.scheme == WH_FD2 ? 1 : a.scheme == WH_FD4 ? 2 : 3;
+        size_t lds = sizeof(double) * 3 * (G.bs0 + 2 * R) * (G.bs1 + 2 * R);
+        WH_CHECK(lds <= 160 * 1024, WH_E_ARG, "2D block tile exceeds LDS");
+        dim3 grid(nact), blockd(256);
+#define L2D(SC, SK)                                                                          \
+    hipLaunchKernelGGL((k_rhs2d<SC, SK, FUSED>), grid, blockd, lds, P->stream, in, out, mask, \
+                       act, x0dx, G, a, sa)
+        switch (a.scheme * 2 + a.skew) {
+        case WH_FD2 * 2: L2D(WH_FD2, false); break;
+        case WH_FD2 * 2 + 1: L2D(WH_FD2, true); break;
+        case WH_FD4 * 2: L2D(WH_FD4, false); break;
+        case WH_FD4 * 2 + 1: L2D(WH_FD4, true); break;
+        case WH_FD4_TW * 2: L2D(WH_FD4_TW, false); break;
+        case WH_FD4_TW * 2 + 1: L2D(WH_FD4_TW, true); break;
+        case WH_FD6 * 2: L2D(WH_FD6, false); break;
+        case WH_FD6 * 2 + 1: L2D(WH_FD6, true); break;
+        default: set_error("unknown scheme"); return WH_E_ARG;
+        }
+#undef L2D
+    } else {
+        const int npts = G.bs0 * G.bs1 * G.bs2;
+        dim3 grid(nact * ((npts + 255) / 256)), blockd(256);
+#define L3D(SC, SK)                                                                           \
+    hipLaunchKernelGGL((k_rhs3d<SC, SK, FUSED>), grid, blockd, 0, P->stream, in, out, mask, act, \
+                       x0dx, G, a, sa)
+        switch (a.scheme * 2 + a.skew) {
+        case WH_FD2 * 2: L3D(WH_FD2, false); break;
+        case WH_FD2 * 2 + 1: L3D(WH_FD2, true); break;
+        case WH_FD4 * 2: L3D(WH_FD4, false); break;
+        case WH_FD4 * 2 + 1: L3D(WH_FD4, true); break;
+        case WH_FD4_TW * 2: L3D(WH_FD4_TW, false); break;
+        case WH_FD4_TW * 2 + 1: L3D(WH_FD4_TW, true); break;
+        case WH_FD6 * 2: L3D(WH_FD6, false); break;
+        case WH_FD6 * 2 + 1: L3D(WH_FD6, true); break;
+        default: set_error("unknown scheme"); return WH_E_ARG;
+        }
+#undef L3D
+    }
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int launch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *out, const int32_t *act,
+               const double *x0dx, int nact)
+{
+    StageArgs sa{};
+    return dispatch_rhs<false>(P, a, in, out, act, x0dx, nact, sa);
+}
+
+int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *tout,
+                       const double *u0, double *acc, double *sout, int mode, double acoef,
+                       double bcoef, int /*coef_from_dev*/)
+{
+    StageArgs sa{};
+    sa.u0 = u0;
+    sa.acc = acc;
+    sa.tout = tout;
+    sa.sout = sout;
+    sa.scal = P->scal;
+    sa.a = acoef;
+    sa.b = bcoef;
+    sa.mode = mode;
+    sa.use_a = std::abs(acoef) >= 1.0e-8;
+    sa.use_b = std::abs(bcoef) >= 1.0e-8;
+    return dispatch_rhs<true>(P, a, tin, nullptr, P->d_active, P->d_x0dx, P->nact, sa);
+}
+
+// ---------------------------------------------------------------------------
+// ghost patches: one workgroup per receiving block (CSR rows of the table)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sync(double *__restrict__ arr,
+                                              const wh_patch *__restrict__ pt,
+                                              const int32_t *__restrict__ off, int row0,
+                                              int stage, size_t blk, size_t cs, int nx, int ny,
+                                              int neq)
+{
+    const int row = row0 + blockIdx.x;
+    for (int q = off[row]; q < off[row + 1]; ++q) {
+        const wh_patch P = pt[q];
+        if (stage > 0 && P.stage != stage) continue;
+        const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
+        const int npt = n0 * n1 * n2;
+        const int st = (P.op == WH_PATCH_RESTRICT) ? 2 : 1;
+        double *dst = arr + (size_t)P.recv * blk;
+        const double *src = arr + (size_t)P.send * blk;
+        for (int e = threadIdx.x; e < npt * neq; e += blockDim.x) {
+            const int c = e / npt, r = e - c * npt;
+            const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
+            const size_t ri = ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx +
+                              (P.recv_lo[0] + i);
+            const size_t si = ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
+                              (P.send_lo[0] + st * i);
+            dst[c * cs + ri] = src[c * cs + si];
+        }
+    }
+}
+
+int launch_sync(wh_pool *P, int sync_case, double *arr, int one_receiver)
+{
+    PatchSet &ps = P->ps[sync_case];
+    if (ps.nrecv == 0) return WH_OK;
+    for (int stage = 1; stage <= std::max(1, ps.max_stage); ++stage) {
+        const int st = ps.max_stage > 1 ? stage : 0;
+        if (one_receiver >= 0) {
+            hipLaunchKernelGGL(k_sync, dim3(1), dim3(256), 0, P->stream, arr, ps.d_patches,
+                               ps.d_recv_off, one_receiver, st, P->blk, P->cs, P->nx, P->ny,
+                               P->d.neq);
+        } else {
+            hipLaunchKernelGGL(k_sync, dim3(ps.nrecv), dim3(256), 0, P->stream, arr,
+                               ps.d_patches, ps.d_recv_off, 0, st, P->blk, P->cs, P->nx, P->ny,
+                               P->d.neq);
+        }
+    }
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// RK combination over block interiors: out = base + sum coef_i * term_i
+// ---------------------------------------------------------------------------
+struct CombineArgs {
+    const double *base;
+    const double *terms[8];
+    double coefs[8];
+    int nterms;
+};
+
+__global__ __launch_bounds__(256) void k_combine(double *__restrict__ out, CombineArgs ca,
+                                                 const int32_t *__restrict__ act, Geo G, int neq)
+{
+    const int npts = G.bs0 * G.bs1 * G.bs2;
+    const int nchunk = (npts * neq + 255) / 256;
+    const int k = blockIdx.x / nchunk;
+    const int e = (blockIdx.x - k * nchunk) * 256 + threadIdx.x;
+    if (e >= npts * neq) return;
+    const int b = act[k];
+    const int c = e / npts, pnt = e - c * npts;
+    const int ix = pnt % G.bs0, iy = (pnt / G.bs0) % G.bs1, iz = pnt / (G.bs0 * G.bs1);
+    const int zoff = (G.bs2 > 1) ? G.g : 0;
+    const size_t i = (size_t)b * G.blk + c * G.cs +
+                     ((size_t)(zoff + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
+    double v = ca.base[i];
+    for (int t = 0; t < ca.nterms; ++t) v = v + ca.coefs[t] * ca.terms[t][i];
+    out[i] = v;
+}
+
+int launch_combine(wh_pool *P, double *out, const double *base, int nterms,
+                   const double *const *terms, const double *coefs, const int32_t *act, int nact)
+{
+    WH_CHECK(nterms >= 0 && nterms <= 8, WH_E_ARG, "rk_combine: at most 8 terms");
+    if (nact <= 0) return WH_OK;
+    CombineArgs ca{};
+    ca.base = base;
+    ca.nterms = nterms;
+    for (int t = 0; t < nterms; ++t) {
+        ca.terms[t] = terms[t];
+        ca.coefs[t] = coefs[t];
+    }
+    const Geo G = make_geo(P);
+    const int npts = G.bs0 * G.bs1 * G.bs2;
+    const int nchunk = (npts * P->d.neq + 255) / 256;
+    hipLaunchKernelGGL(k_combine, dim3(nact * nchunk), dim3(256), 0, P->stream, out, ca, act, G,
+                       P->d.neq);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reductions: per-block dt (GET_DT_BLOCK_ACM) and max |u| (divergence guard)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ double block_max(double v)
+{
+    __shared__ double red[16];
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = (l < (int)(blockDim.x >> 6)) ? red[l] : -1.0;
+        v = wave_max(v);
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_dt(const double *__restrict__ in,
+                                            const int32_t *__restrict__ act,
+                                            const double *__restrict__ x0dx, Geo G, AcmDev a,
+                                            unsigned long long *dtmin)
+{
+    const int k = blockIdx.x, b = act[k];
+    const int npts = G.bs0 * G.bs1 * G.bs2;
+    const int zoff = (G.bs2 > 1) ? G.g : 0;
+    const double *u = in + (size_t)b * G.blk;
+    double m = -1.0;
+    for (int p = threadIdx.x; p < npts; p += blockDim.x) {
+        const int ix = p % G.bs0, iy = (p / G.bs0) % G.bs1, iz = p / (G.bs0 * G.bs1);
+        const size_t i = ((size_t)(zoff + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
+        double v = u[i] * u[i] + u[G.cs + i] * u[G.cs + i];
+        if (a.dim == 3) v = v + u[2 * G.cs + i] * u[2 * G.cs + i];
+        m = fmax(m, v);
+    }
+    m = block_max(m);
+    if (threadIdx.x == 0) {
+        double dxmin = x0dx[6 * k + 3];
+        for (int d = 1; d < a.dim; ++d) dxmin = fmin(dxmin, x0dx[6 * k + 3 + d]);
+        const double ueig = sqrt(m) + sqrt(a.c_0 * a.c_0 + m);
+        double dt = (ueig >= 1.0e-6) ? a.CFL * dxmin / ueig : 1.0e-2;
+        if (a.nu > 1.0e-13) dt = fmin(dt, a.CFL_nu * (dxmin * dxmin) / a.nu);
+        if (a.gamma_p > 0.0) dt = fmin(dt, a.CFL_eta * a.gamma_p);
+        if (a.penal) dt = fmin(dt, a.CFL_eta * a.C_eta);
+        if (a.sponge) dt = fmin(dt, a.CFL_eta * a.C_sponge);
+        atomicMin(dtmin, (unsigned long long)__double_as_longlong(dt));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_maxabs(const double *__restrict__ in,
+                                                const int32_t *__restrict__ act, Geo G, int neq,
+                                                unsigned long long *mx)
+{
+    const int k = blockIdx.x, b = act[k];
+    const int npts = G.bs0 * G.bs1 * G.bs2;
+    const int zoff = (G.bs2 > 1) ? G.g : 0;
+    const double *u = in + (size_t)b * G.blk;
+    double m = 0.0;
+    for (int e = threadIdx.x; e < npts * neq; e += blockDim.x) {
+        const int c = e / npts, p = e - c * npts;
+        const int ix = p % G.bs0, iy = (p / G.bs0) % G.bs1, iz = p / (G.bs0 * G.bs1);
+        const size_t i = c * G.cs + ((size_t)(zoff + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
+        m = fmax(m, fabs(u[i]));
+    }
+    m = block_max(m);
+    if (threadIdx.x == 0) atomicMax(mx, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void k_scalar_init(double *scal, int which, double v)
+{
+    if (threadIdx.x == 0) scal[which] = v;
+}
+
+int launch_dt(wh_pool *P, const AcmDev &a)
+{
+    hipLaunchKernelGGL(k_scalar_init, dim3(1), dim3(64), 0, P->stream, P->scal, (int)DS_DTMIN_BITS,
+                       9.0e9);
+    if (P->nact > 0) {
+        const Geo G = make_geo(P);
+        hipLaunchKernelGGL(k_dt, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
+                           P->d_x0dx, G, a, (unsigned long long *)(P->scal + DS_DTMIN_BITS));
+    }
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int launch_maxabs(wh_pool *P, const double *in)
+{
+    hipLaunchKernelGGL(k_scalar_init, dim3(1), dim3(64), 0, P->stream, P->scal,
+                       (int)DS_MAXABS_BITS, 0.0);
+    if (P->nact > 0) {
+        const Geo G = make_geo(P);
+        hipLaunchKernelGGL(k_maxabs, dim3(P->nact), dim3(256), 0, P->stream, in, P->d_active, G,
+                           P->d.neq, (unsigned long long *)(P->scal + DS_MAXABS_BITS));
+    }
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+// dt clamps of calculate_time_step.f90:58-118 on the device, so that a whole
+// step can be replayed from a hipGraph without a host round trip.
+__global__ void k_dt_finalize(double *scal, TimeDev tc)
+{
+    if (threadIdx.x != 0) return;
+    const double time = scal[DS_TIME];
+    double dt;
+    if (tc.dt_fixed > 0.0) {
+        dt = tc.dt_fixed;
+    } else {
+        dt = scal[DS_DTMIN_BITS];  // the min was taken on the bit pattern of positive doubles
+        if (tc.dt_max > 0.0) dt = fmin(tc.dt_max, dt);
+    }
+    if (tc.write_fixed_time) {
+        if (fmod(time + dt, tc.write_time) < fmod(time + 1e-12, tc.write_time) &&
+            !(fabs(fmod(time, tc.write_time)) < 1e-12) && time + 1e-12 > tc.write_time_first)
+            dt = tc.write_time - fmod(time, tc.write_time);
+    }
+    if (tc.write_backup_n > 0) {
+        const double wb = tc.write_backup_time;
+        if (fmod(time + dt, wb) < fmod(time + 1e-12, wb) && !(fabs(fmod(time, wb)) < 1e-12))
+            dt = wb - fmod(time, wb);
+    }
+    if (tc.use_stats) {
+        const double ts = tc.tsave_stats;
+        if (fmod(time + dt, ts) < fmod(time + 1e-12, ts) && !(fabs(fmod(time, ts)) < 1e-12))
+            dt = ts - fmod(time, ts);
+    }
+    if (time + dt > tc.time_max && time <= tc.time_max) dt = tc.time_max - time;
+    if (!(time < tc.time_max)) dt = 0.0;  // main.f90:293 loop condition
+    scal[DS_DT] = dt;
+}
+
+__global__ void k_time_advance(double *scal)
+{
+    if (threadIdx.x == 0 && scal[DS_DT] > 0.0) {
+        scal[DS_TIME] = scal[DS_TIME] + scal[DS_DT];
+        scal[DS_ITER] = scal[DS_ITER] + 1.0;
+    }
+}
+
+int launch_dt_finalize(wh_pool *P, const TimeDev &tc)
+{
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, P->stream, P->scal, tc);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int launch_time_advance(wh_pool *P)
+{
+    hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, P->stream, P->scal);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int launch_set_scalar(wh_pool *P, int which, double v)
+{
+    hipLaunchKernelGGL(k_scalar_init, dim3(1), dim3(64), 0, P->stream, P->scal, which, v);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+AcmDev make_acm_dev(const wh_acm_params *p)
+{
+    AcmDev a{};
+    a.c02 = p->c_0 * p->c_0;
+    a.c_0 = p->c_0;
+    a.nu = p->nu;
+    a.gamma_p = p->gamma_p;
+    a.C_eta = p->C_eta;
+    a.Cinv = 1.0 / p->C_eta;
+    a.C_sponge = p->C_sponge;
+    a.Csp_inv = 1.0 / p->C_sponge;
+    for (int i = 0; i < 3; ++i) a.ums[i] = p->u_mean_set[i];
+    a.CFL = p->CFL;
+    a.CFL_nu = p->CFL_nu;
+    a.CFL_eta = p->CFL_eta;
+    a.dim = p->dim;
+    a.scheme = p->scheme;
+    a.skew = p->skew_symmetry ? 1 : 0;
+    a.penal = p->penalization ? 1 : 0;
+    a.sponge = p->use_sponge ? 1 : 0;
+    return a;
+}
+
+}  // namespace wh
