@@ -91,6 +91,30 @@ def cpu_baseline(w, hvy, forest, seconds=10.0):
                        f"OMP_NUM_THREADS={threads}")
 
 
+def calibrate_copy(name, reps=10):
+    """A plain copy of known size for scaling the FETCH_SIZE / WRITE_SIZE
+    counters: wh_rk_combine with no terms copies the interiors of all blocks
+    of hvy_block into work slot 1 (reads and writes q*N doubles per block)."""
+    import ctypes
+
+    from wabbit_amd import lib as L
+    from wabbit_amd.forest import EquidistantForest
+    w = workload(name)
+    dim = w["dim"]
+    forest = EquidistantForest(dim, w["J"], [w["bs"]] * dim, w["g"], [w["L"]] * dim)
+    pool = L.Pool(dim, forest.bs, forest.g, dim + 1, forest.n, n_work_slots=2)
+    pool.set_active(forest.active, forest.x0dx)
+    pool.fill(L.WH_BLOCK, 1.0)
+    for _ in range(reps):
+        L.check(L.load().wh_rk_combine(pool.h, L.WH_WORK + 1, L.WH_BLOCK, 0, None, None))
+    pool.synchronize()
+    nbytes = forest.n * (dim + 1) * w["bs"] ** dim * 8
+    print(json.dumps({"calibration": "interior copy", "launches": reps,
+                      "read_bytes_per_launch": nbytes, "write_bytes_per_launch": nbytes}))
+    pool.close()
+    del ctypes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +123,12 @@ def main():
     ap.add_argument("--workload", default="c1", choices=["c1", "c3u"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain interior copy of known size (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.workload)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
@@ -128,7 +157,12 @@ def main():
     except Exception:  # torch is only used for the contract's extra barrier
         torch_sync = lambda: None  # noqa: E731
 
-    # warmup (also captures the graph)
+    # autotune the stage kernel for this (Bs, g, neq), then warm up (also
+    # captures the graph)
+    tuned, tuned_ms = pool.autotune(solver.acm_c, reps=3)
+    solver.load(hvy)
+    print(f"# autotune: variant {tuned} ({tuned_ms:.4f} ms per RHS launch)", file=sys.stderr,
+          flush=True)
     solver.run_device(args.warmup, use_graph=True)
     pool.synchronize()
     torch_sync()
@@ -154,7 +188,7 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
                     kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(k_ms, 5),
                     launches=k_n, bytes_per_block_stage=bpb, blocks_per_launch=forest.n,
-                    sync_kernel_avg_ms=round(s_ms, 5))
+                    sync_kernel_avg_ms=round(s_ms, 5), tuned_variant=tuned)
 
     paths = {"fused_graph": round(value, 1)}
     if not args.no_paths:

@@ -171,6 +171,18 @@ int wh_time_get(wh_pool *pool, double *time, int *iteration, double *dt_last);
 int wh_run_steps(wh_pool *pool, const wh_acm_params *p, const double *butcher, int nrows,
                  const wh_time_params *tp, int nsteps, int use_graph);
 
+/* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
+ * stage kernel (each owns Bs2/ysplit rows of (x,y) columns and marches in z);
+ * 0 selects the flat one-point-per-thread kernel, -1 automatic. */
+enum { WH_TUNE_K3D_YSPLIT = 1 };
+int wh_set_tuning(wh_pool *pool, int key, int value);
+int wh_get_tuning(wh_pool *pool, int key, int *value);
+
+/* Autotuner: times every valid variant of the RHS/stage kernel for this
+ * pool's (Bs, g, neq) and parameters on the current active blocks, keeps the
+ * fastest and returns it.  Work slot 1 is used as scratch (overwritten). */
+int wh_autotune(wh_pool *pool, const wh_acm_params *p, int reps, int *best_value, double *best_ms);
+
 /* HIP-event timing of kernel kinds (0: RHS or fused stage kernel, 1: ghost
  * sync) on the pool's stream: average ms per launch since the last reset.
  * Timing serialises host and device; enable it only for measurement. */

@@ -220,3 +220,28 @@ def test_divergence_guard_and_errors():
     with pytest.raises(L.WabbitHipError):
         pool.set_patches(0, p)
     pool.close()
+
+
+@pytest.mark.parametrize("scheme", [4, 6])
+@pytest.mark.parametrize("ysplit", [0, 1, 2, 3])
+def test_3d_kernel_variants_identical(scheme, ysplit):
+    """Every tuning variant of the 3D stage kernel gives the oracle's RHS
+    bit for bit (the autotuner may pick any of them)."""
+    po = _oracle()
+    forest, acm, hvy, mask = _setup(3, scheme, True, True, True, J=1, bs=24)
+    hs = _oracle_sync(forest, acm, hvy, REACH[scheme])
+    pool = L.Pool(3, forest.bs, forest.g, acm.neq, forest.n, n_work_slots=2, n_mask=6)
+    pool.set_active(forest.active, forest.x0dx)
+    pool.upload(L.WH_BLOCK, 0, hs)
+    pool.upload(L.WH_MASK, 0, mask)
+    pool.set_tuning(L.WH_TUNE_K3D_YSPLIT, ysplit)
+    pool.rhs(L.acm_params_c(acm), 0.0, L.WH_BLOCK, 1)
+    got = interior(pool.download(L.WH_WORK + 1, 0, forest.n), 24, forest.g, 3)
+    p = po.make_params(acm)
+    want = interior(np.stack([po.rhs_block(p, [24] * 3, forest.g, forest.x0dx[k, 3:], hs[k], mask[k])
+                              for k in range(forest.n)]), 24, forest.g, 3)
+    assert rel_l2(got, want) <= 1e-12
+    assert np.array_equal(got, want)
+    best, ms = pool.autotune(L.acm_params_c(acm), reps=2)
+    assert best in (0, 1, 2, 3, 4, 6, 8) and ms > 0
+    pool.close()

@@ -220,6 +220,7 @@ static void free_patchset(PatchSet &ps)
     if (ps.d_patches) (void)hipFree(ps.d_patches);
     if (ps.d_recv_off) (void)hipFree(ps.d_recv_off);
     if (ps.d_recv_ids) (void)hipFree(ps.d_recv_ids);
+    if (ps.d_pstart) (void)hipFree(ps.d_pstart);
     ps = PatchSet{};
 }
 
@@ -445,16 +446,29 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
     PatchSet &ps = P->ps[sync_case];
     free_patchset(ps);
     ps.host.assign(patches, patches + npatch);
-    std::stable_sort(ps.host.begin(), ps.host.end(),
-                     [](const wh_patch &a, const wh_patch &b) { return a.recv < b.recv; });
+    std::stable_sort(ps.host.begin(), ps.host.end(), [](const wh_patch &a, const wh_patch &b) {
+        return a.stage != b.stage ? a.stage < b.stage : a.recv < b.recv;
+    });
     ps.h_recv_off.clear();
     ps.h_recv_ids.clear();
+    ps.h_pstart.assign(npatch + 1, 0);
+    for (int s = 0; s < 5; ++s) ps.stage_begin[s] = 0;
     for (int i = 0; i < npatch; ++i) {
-        if (i == 0 || ps.host[i].recv != ps.host[i - 1].recv) {
+        const wh_patch &q = ps.host[i];
+        if (i == 0 || q.recv != ps.host[i - 1].recv || q.stage != ps.host[i - 1].stage) {
             ps.h_recv_off.push_back(i);
-            ps.h_recv_ids.push_back(ps.host[i].recv);
+            ps.h_recv_ids.push_back(q.recv);
         }
-        ps.max_stage = std::max(ps.max_stage, (int)ps.host[i].stage);
+        ps.max_stage = std::max(ps.max_stage, (int)q.stage);
+        ps.h_pstart[i + 1] = ps.h_pstart[i] + (int64_t)q.n[0] * q.n[1] * q.n[2];
+    }
+    {  // stage ranges over the (stage, receiver)-sorted table
+        int i = 0;
+        for (int s = 1; s <= 3; ++s) {
+            ps.stage_begin[s] = i;
+            while (i < npatch && ps.host[i].stage == s) ++i;
+        }
+        ps.stage_begin[4] = npatch;
     }
     ps.h_recv_off.push_back(npatch);
     ps.n = npatch;
@@ -463,6 +477,9 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
         WH_HIP(hipMalloc(&ps.d_patches, sizeof(wh_patch) * npatch));
         WH_HIP(hipMalloc(&ps.d_recv_off, sizeof(int32_t) * ps.h_recv_off.size()));
         WH_HIP(hipMalloc(&ps.d_recv_ids, sizeof(int32_t) * ps.h_recv_ids.size()));
+        WH_HIP(hipMalloc(&ps.d_pstart, sizeof(int64_t) * ps.h_pstart.size()));
+        WH_HIP(hipMemcpyAsync(ps.d_pstart, ps.h_pstart.data(), sizeof(int64_t) * ps.h_pstart.size(),
+                              hipMemcpyHostToDevice, P->stream));
         WH_HIP(hipMemcpyAsync(ps.d_patches, ps.host.data(), sizeof(wh_patch) * npatch,
                               hipMemcpyHostToDevice, P->stream));
         WH_HIP(hipMemcpyAsync(ps.d_recv_off, ps.h_recv_off.data(),
@@ -655,6 +672,80 @@ int wh_kernel_timing(wh_pool *P, int which, double *avg_ms, int *count)
     WH_CHECK(which <= 1, WH_E_ARG, "which must be 0 or 1");
     if (avg_ms) *avg_ms = P->t_cnt[which] ? P->t_ms[which] / P->t_cnt[which] : 0.0;
     if (count) *count = P->t_cnt[which];
+    return WH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int wh_set_tuning(wh_pool *P, int key, int value)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    WH_CHECK(key == WH_TUNE_K3D_YSPLIT, WH_E_ARG, "unknown tuning key");
+    WH_CHECK(value >= -1 && value <= 64, WH_E_ARG, "ysplit out of range");
+    // a split that is not valid for a launch's scheme falls back to the
+    // automatic choice at that launch (zm_split)
+    P->k3d_ysplit = value;
+    P->topo_version++;  // captured graphs hold the old launch shape
+    return WH_OK;
+}
+
+int wh_get_tuning(wh_pool *P, int key, int *value)
+{
+    WH_CHECK(P && value, WH_E_ARG, "null argument");
+    WH_CHECK(key == WH_TUNE_K3D_YSPLIT, WH_E_ARG, "unknown tuning key");
+    *value = P->k3d_ysplit;
+    return WH_OK;
+}
+
+int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, double *best_ms)
+{
+    WH_CHECK(P && p, WH_E_ARG, "null argument");
+    if (int rc = check_acm(P, p)) return rc;
+    WH_CHECK(P->d.n_work_slots >= 2, WH_E_ARG, "autotune needs work slot 1 as scratch");
+    WH_CHECK(P->nact > 0, WH_E_STATE, "autotune needs an active list");
+    WH_CHECK(!P->timing, WH_E_STATE, "disable kernel timing before autotuning");
+    reps = std::max(1, std::min(reps, 100));
+    std::vector<int> cand;
+    if (P->d.dim == 3) {
+        cand.push_back(0);
+        for (int ys = 1; ys <= 8; ++ys)
+            if (zm_valid(P->d.bs[0], P->d.bs[1], p->scheme, ys)) cand.push_back(ys);
+    } else {
+        cand.push_back(P->k3d_ysplit);  // 2D has a single variant
+    }
+    const AcmDev a = make_acm_dev(p);
+    double *out = array_ptr(P, WH_WORK + 1, nullptr, nullptr);
+    const int saved = P->k3d_ysplit;
+    int best = saved;
+    double bms = 1e300;
+    for (int v : cand) {
+        P->k3d_ysplit = v;
+        int rc = launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact);  // warm
+        if (rc) {
+            P->k3d_ysplit = saved;
+            return rc;
+        }
+        WH_HIP(hipEventRecord(P->ev_a, P->stream));
+        for (int r = 0; r < reps; ++r)
+            if ((rc = launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact))) {
+                P->k3d_ysplit = saved;
+                return rc;
+            }
+        WH_HIP(hipEventRecord(P->ev_b, P->stream));
+        WH_HIP(hipEventSynchronize(P->ev_b));
+        float ms = 0.f;
+        WH_HIP(hipEventElapsedTime(&ms, P->ev_a, P->ev_b));
+        if (ms / reps < bms) {
+            bms = ms / reps;
+            best = v;
+        }
+    }
+    P->k3d_ysplit = best;
+    P->topo_version++;
+    if (best_value) *best_value = best;
+    if (best_ms) *best_ms = bms;
     return WH_OK;
 }
 

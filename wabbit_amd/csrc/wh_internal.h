@@ -57,11 +57,14 @@ struct TimeDev {
 };
 
 struct PatchSet {
-    std::vector<wh_patch> host;     // sorted by receiver
+    std::vector<wh_patch> host;     // sorted by (stage, receiver)
     wh_patch *d_patches = nullptr;
-    int32_t *d_recv_off = nullptr;  // CSR over distinct receivers
+    int32_t *d_recv_off = nullptr;  // CSR over (stage, receiver) rows
     int32_t *d_recv_ids = nullptr;  // receiver hvy id per CSR row
+    int64_t *d_pstart = nullptr;    // prefix sum of patch point counts
     std::vector<int32_t> h_recv_off, h_recv_ids;
+    std::vector<int64_t> h_pstart;
+    int stage_begin[5] = {0, 0, 0, 0, 0};  // stage s (1..3): patches [stage_begin[s], stage_begin[s+1])
     int n = 0, nrecv = 0;
     int max_stage = 0;
 };
@@ -97,6 +100,8 @@ struct wh_pool {
     wh_acm_params graph_p{};
     std::vector<double> graph_bt;
     wh::TimeDev graph_tc{};
+    // tuning: y-split of the 3D stage kernel (-1 auto, 0 flat kernel)
+    int k3d_ysplit = -1;
 };
 
 namespace wh {
@@ -120,5 +125,7 @@ int launch_time_advance(wh_pool *P);
 int launch_set_scalar(wh_pool *P, int which, double v);
 
 AcmDev make_acm_dev(const wh_acm_params *p);
+int reach_of(int scheme);
+bool zm_valid(int bs0, int bs1, int scheme, int ys);
 
 }  // namespace wh

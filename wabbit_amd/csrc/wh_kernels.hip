@@ -14,16 +14,20 @@
 namespace wh {
 
 // ---------------------------------------------------------------------------
-// FD stencils (rhs_ACM.f90:335-344 and :958-965)
+// FD stencils (rhs_ACM.f90:335-344 and :958-965).  Coefficients are compile-
+// time constants; each stencil takes a value accessor f(o) -> field at offset
+// o along one direction, so the same code reads global memory, an LDS tile or
+// a register queue (constant offsets keep the queue in registers).
 // ---------------------------------------------------------------------------
-__constant__ double cA4[5] = {1.0 / 12.0, -2.0 / 3.0, 0.0, 2.0 / 3.0, -1.0 / 12.0};
-__constant__ double cB4[5] = {-1.0 / 12.0, 4.0 / 3.0, -5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0};
-__constant__ double cTW[7] = {-0.02651995, 0.18941314, -0.79926643, 0.0,
-                              0.79926643, -0.18941314, 0.02651995};
-__constant__ double cA6[7] = {-1.0 / 60.0, 3.0 / 20.0, -3.0 / 4.0, 0.0,
-                              3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
-__constant__ double cB6[7] = {1.0 / 90.0, -3.0 / 20.0, 3.0 / 2.0, -49.0 / 18.0,
-                              3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
+constexpr double A4_m2 = 1.0 / 12.0, A4_m1 = -2.0 / 3.0, A4_p1 = 2.0 / 3.0, A4_p2 = -1.0 / 12.0;
+constexpr double B4_m2 = -1.0 / 12.0, B4_m1 = 4.0 / 3.0, B4_0 = -5.0 / 2.0, B4_p1 = 4.0 / 3.0,
+                 B4_p2 = -1.0 / 12.0;
+constexpr double TW_m3 = -0.02651995, TW_m2 = 0.18941314, TW_m1 = -0.79926643,
+                 TW_p1 = 0.79926643, TW_p2 = -0.18941314, TW_p3 = 0.02651995;
+constexpr double A6_m3 = -1.0 / 60.0, A6_m2 = 3.0 / 20.0, A6_m1 = -3.0 / 4.0, A6_p1 = 3.0 / 4.0,
+                 A6_p2 = -3.0 / 20.0, A6_p3 = 1.0 / 60.0;
+constexpr double B6_m3 = 1.0 / 90.0, B6_m2 = -3.0 / 20.0, B6_m1 = 3.0 / 2.0, B6_0 = -49.0 / 18.0,
+                 B6_p1 = 3.0 / 2.0, B6_p2 = -3.0 / 20.0, B6_p3 = 1.0 / 90.0;
 
 template <int S> struct Reach;
 template <> struct Reach<WH_FD2> { static constexpr int R = 1; };
@@ -31,71 +35,70 @@ template <> struct Reach<WH_FD4> { static constexpr int R = 2; };
 template <> struct Reach<WH_FD4_TW> { static constexpr int R = 3; };
 template <> struct Reach<WH_FD6> { static constexpr int R = 3; };
 
-// first derivative along stride s
-template <int S>
-__device__ __forceinline__ double d1(const double *f, ptrdiff_t s, double inv)
+// first derivative; terms summed from the most negative offset up
+template <int S, class F>
+__device__ __forceinline__ double d1(const F &f, double inv)
 {
     if constexpr (S == WH_FD2) {
-        return (f[s] - f[-s]) * inv * 0.5;
+        return (f(1) - f(-1)) * inv * 0.5;
     } else if constexpr (S == WH_FD4) {
-        return (cA4[0] * f[-2 * s] + cA4[1] * f[-s] + cA4[3] * f[s] + cA4[4] * f[2 * s]) * inv;
+        return (A4_m2 * f(-2) + A4_m1 * f(-1) + A4_p1 * f(1) + A4_p2 * f(2)) * inv;
     } else if constexpr (S == WH_FD4_TW) {
-        return (cTW[0] * f[-3 * s] + cTW[1] * f[-2 * s] + cTW[2] * f[-s] + cTW[4] * f[s] +
-                cTW[5] * f[2 * s] + cTW[6] * f[3 * s]) * inv;
+        return (TW_m3 * f(-3) + TW_m2 * f(-2) + TW_m1 * f(-1) + TW_p1 * f(1) + TW_p2 * f(2) +
+                TW_p3 * f(3)) * inv;
     } else {
-        return (cA6[0] * f[-3 * s] + cA6[1] * f[-2 * s] + cA6[2] * f[-s] + cA6[4] * f[s] +
-                cA6[5] * f[2 * s] + cA6[6] * f[3 * s]) * inv;
+        return (A6_m3 * f(-3) + A6_m2 * f(-2) + A6_m1 * f(-1) + A6_p1 * f(1) + A6_p2 * f(2) +
+                A6_p3 * f(3)) * inv;
     }
 }
 
 // first derivative of the product fa*fb (skew-symmetric form); terms (c*fa)*fb
-template <int S>
-__device__ __forceinline__ double d1p(const double *fa, const double *fb, ptrdiff_t s, double inv)
+template <int S, class F, class G>
+__device__ __forceinline__ double d1p(const F &fa, const G &fb, double inv)
 {
     if constexpr (S == WH_FD2) {
-        return (fa[s] * fb[s] - fa[-s] * fb[-s]) * inv * 0.5;
+        return (fa(1) * fb(1) - fa(-1) * fb(-1)) * inv * 0.5;
     } else if constexpr (S == WH_FD4) {
-        return (cA4[0] * fa[-2 * s] * fb[-2 * s] + cA4[1] * fa[-s] * fb[-s] +
-                cA4[3] * fa[s] * fb[s] + cA4[4] * fa[2 * s] * fb[2 * s]) * inv;
+        return (A4_m2 * fa(-2) * fb(-2) + A4_m1 * fa(-1) * fb(-1) + A4_p1 * fa(1) * fb(1) +
+                A4_p2 * fa(2) * fb(2)) * inv;
+    } else if constexpr (S == WH_FD4_TW) {
+        return (TW_m3 * fa(-3) * fb(-3) + TW_m2 * fa(-2) * fb(-2) + TW_m1 * fa(-1) * fb(-1) +
+                TW_p1 * fa(1) * fb(1) + TW_p2 * fa(2) * fb(2) + TW_p3 * fa(3) * fb(3)) * inv;
     } else {
-        const double *c = (S == WH_FD4_TW) ? cTW : cA6;
-        return (c[0] * fa[-3 * s] * fb[-3 * s] + c[1] * fa[-2 * s] * fb[-2 * s] +
-                c[2] * fa[-s] * fb[-s] + c[4] * fa[s] * fb[s] + c[5] * fa[2 * s] * fb[2 * s] +
-                c[6] * fa[3 * s] * fb[3 * s]) * inv;
+        return (A6_m3 * fa(-3) * fb(-3) + A6_m2 * fa(-2) * fb(-2) + A6_m1 * fa(-1) * fb(-1) +
+                A6_p1 * fa(1) * fb(1) + A6_p2 * fa(2) * fb(2) + A6_p3 * fa(3) * fb(3)) * inv;
     }
 }
 
-template <int S>
-__device__ __forceinline__ double d2(const double *f, ptrdiff_t s, double inv2)
+template <int S, class F>
+__device__ __forceinline__ double d2(const F &f, double inv2)
 {
     if constexpr (S == WH_FD2) {
-        return (f[-s] - 2.0 * f[0] + f[s]) * inv2;
+        return (f(-1) - 2.0 * f(0) + f(1)) * inv2;
     } else if constexpr (S == WH_FD4 || S == WH_FD4_TW) {
-        return (cB4[0] * f[-2 * s] + cB4[1] * f[-s] + cB4[2] * f[0] + cB4[3] * f[s] +
-                cB4[4] * f[2 * s]) * inv2;
+        return (B4_m2 * f(-2) + B4_m1 * f(-1) + B4_0 * f(0) + B4_p1 * f(1) + B4_p2 * f(2)) * inv2;
     } else {
-        return (cB6[0] * f[-3 * s] + cB6[1] * f[-2 * s] + cB6[2] * f[-s] + cB6[3] * f[0] +
-                cB6[4] * f[s] + cB6[5] * f[2 * s] + cB6[6] * f[3 * s]) * inv2;
+        return (B6_m3 * f(-3) + B6_m2 * f(-2) + B6_m1 * f(-1) + B6_0 * f(0) + B6_p1 * f(1) +
+                B6_p2 * f(2) + B6_p3 * f(3)) * inv2;
     }
 }
 
-// ACM right-hand side at one point (RHS_2D_acm / RHS_3D_acm).  uc points at
-// component 0 of the point; components are cs apart; s1/s2 are the y/z
-// strides.  mk: mask component 0 at this point (components mcs apart) or null.
-template <int DIM, int S, bool SKEW>
-__device__ __forceinline__ void acm_point(const double *__restrict__ uc, ptrdiff_t cs, ptrdiff_t s1,
-                                          ptrdiff_t s2, const double *inv, const double *inv2,
+// ACM right-hand side at one point (RHS_2D_acm / RHS_3D_acm).  A(c, d, o) is
+// component c of the state at offset o along direction d from the point.
+// mk: mask component 0 at this point (components mcs apart) or null.
+template <int DIM, int S, bool SKEW, class ACC>
+__device__ __forceinline__ void acm_point(const ACC &A, const double *inv, const double *inv2,
                                           const AcmDev &a, const double *__restrict__ mk,
                                           ptrdiff_t mcs, double *r)
 {
-    const ptrdiff_t st[3] = {1, s1, s2};
     double uv[DIM + 1];
     double du[DIM + 1][DIM];
 #pragma unroll
     for (int c = 0; c <= DIM; ++c) {
-        uv[c] = uc[c * cs];
+        uv[c] = A(c, 0, 0);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) du[c][d] = d1<S>(uc + c * cs, st[d], inv[d]);
+        for (int d = 0; d < DIM; ++d)
+            du[c][d] = d1<S>([&](int o) { return A(c, d, o); }, inv[d]);
     }
     double chi = 0.0;
     if (a.penal && mk) {
@@ -104,14 +107,18 @@ __device__ __forceinline__ void acm_point(const double *__restrict__ uc, ptrdiff
     }
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-        double lap = d2<S>(uc + c * cs, st[0], inv2[0]);
+        double lap = d2<S>([&](int o) { return A(c, 0, o); }, inv2[0]);
 #pragma unroll
-        for (int d = 1; d < DIM; ++d) lap = lap + d2<S>(uc + c * cs, st[d], inv2[d]);
+        for (int d = 1; d < DIM; ++d)
+            lap = lap + d2<S>([&](int o) { return A(c, d, o); }, inv2[d]);
         double t;
         if constexpr (SKEW) {
-            double sum = d1p<S>(uc + c * cs, uc, st[0], inv[0]);
+            double sum = d1p<S>([&](int o) { return A(c, 0, o); },
+                                [&](int o) { return A(0, 0, o); }, inv[0]);
 #pragma unroll
-            for (int d = 1; d < DIM; ++d) sum = sum + d1p<S>(uc + c * cs, uc + d * cs, st[d], inv[d]);
+            for (int d = 1; d < DIM; ++d)
+                sum = sum + d1p<S>([&](int o) { return A(c, d, o); },
+                                   [&](int o) { return A(d, d, o); }, inv[d]);
 #pragma unroll
             for (int d = 0; d < DIM; ++d) sum = sum + uv[d] * du[c][d];
             t = -(0.5 * sum);
@@ -136,6 +143,19 @@ __device__ __forceinline__ void acm_point(const double *__restrict__ uc, ptrdiff
         r[DIM] = r[DIM] - uv[DIM] * spo;
     }
 }
+
+// accessor over one strided storage: base = component 0 of the point,
+// components cs apart, strides s[d] along direction d
+template <int DIM>
+struct StridedAcc {
+    const double *base;
+    ptrdiff_t cs;
+    ptrdiff_t s[3];
+    __device__ __forceinline__ double operator()(int c, int d, int o) const
+    {
+        return base[c * cs + o * s[d]];
+    }
+};
 
 struct Geo {
     int bs0, bs1, bs2, g, nx, ny;
@@ -212,7 +232,8 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
         const double *uc = tile + (iy + R) * tx + (ix + R);
         const size_t gi = (size_t)(G.g + iy) * G.nx + (G.g + ix);
         double r[3];
-        acm_point<2, S, SKEW>(uc, tcs, tx, 0, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
+        const StridedAcc<2> A{uc, tcs, {1, tx, 0}};
+        acm_point<2, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
         const size_t o = (size_t)b * G.blk + gi;
         if constexpr (FUSED) {
             stage_epilogue<3>(sa, o, G.cs, r, uc, tcs);
@@ -245,7 +266,8 @@ __global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, do
     const double *uc = in + (size_t)b * G.blk + gi;
     const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) + gi : nullptr;
     double r[4];
-    acm_point<3, S, SKEW>(uc, G.cs, G.nx, (ptrdiff_t)G.nx * G.ny, inv, inv2, a, mk, G.cs, r);
+    const StridedAcc<3> A{uc, (ptrdiff_t)G.cs, {1, G.nx, (ptrdiff_t)G.nx * G.ny}};
+    acm_point<3, S, SKEW>(A, inv, inv2, a, mk, G.cs, r);
     const size_t o = (size_t)b * G.blk + gi;
     if constexpr (FUSED) {
         stage_epilogue<4>(sa, o, G.cs, r, uc, G.cs);
@@ -253,6 +275,148 @@ __global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, do
 #pragma unroll
         for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
     }
+}
+
+// 3D, z-marching: a workgroup owns a slab of ty0 rows of (x, y) columns of
+// one block (ysplit workgroups per block, one thread per column) and marches
+// through z.  The x/y stencils read an LDS copy of the current z-plane slab
+// (interior from the threads' registers, the R-wide x/y ring from global
+// memory); the z stencil reads a per-thread register queue of 2R+1 planes.
+// The next plane's values are fetched before the current one is computed.
+// Every input value is read from memory once (plus the slab ring).
+template <int S, bool SKEW, bool FUSED>
+__global__ __launch_bounds__(512) void k_rhs3d_zm(const double *__restrict__ in,
+                                                  double *__restrict__ out,
+                                                  const double *__restrict__ mask,
+                                                  const int32_t *__restrict__ act,
+                                                  const double *__restrict__ x0dx, Geo G, AcmDev a,
+                                                  StageArgs sa, int ysplit)
+{
+    constexpr int R = Reach<S>::R;
+    constexpr int W = 2 * R + 1;
+    constexpr int MAXRING = 4;  // ring elements per thread held in registers
+    extern __shared__ double plane[];
+    const int k = blockIdx.x / ysplit;
+    const int part = blockIdx.x - k * ysplit;
+    const int b = act[k];
+    const int ty0 = G.bs1 / ysplit;
+    const int y0 = part * ty0;
+    const int tid = threadIdx.x;
+    const int ix = tid % G.bs0, iyl = tid / G.bs0, iy = y0 + iyl;
+    const int tx = G.bs0 + 2 * R, ty = ty0 + 2 * R, tcs = tx * ty;
+    const size_t zs = (size_t)G.nx * G.ny;
+    const double *u = in + (size_t)b * G.blk;
+    const size_t col = (size_t)(G.g + iy) * G.nx + (G.g + ix);
+    const int nthr = G.bs0 * ty0;
+    const int ring = 2 * R * (G.bs0 + ty0);  // x/y ring of the slab, without corners
+    const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4], dx2 = x0dx[6 * k + 5];
+    const double inv[3] = {1.0 / dx0, 1.0 / dx1, 1.0 / dx2};
+    const double inv2[3] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1), 1.0 / (dx2 * dx2)};
+    const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
+
+    // ring element h -> (component, offset in LDS, offset in the block plane)
+    int r_lds[MAXRING];
+    size_t r_glb[MAXRING];
+    int nring = 0;
+    for (int h = tid; h < 4 * ring && nring < MAXRING; h += nthr, ++nring) {
+        const int c = h / ring;
+        const int e = h - c * ring;
+        int px, py;
+        if (e < 2 * R * G.bs0) {
+            const int row = e / G.bs0;
+            px = R + (e - row * G.bs0);
+            py = row < R ? row : ty0 + row;
+        } else {
+            const int e2 = e - 2 * R * G.bs0;
+            const int kk = e2 % (2 * R);
+            py = R + e2 / (2 * R);
+            px = kk < R ? kk : G.bs0 + kk;
+        }
+        r_lds[nring] = c * tcs + py * tx + px;
+        r_glb[nring] = c * G.cs + (size_t)(G.g + y0 - R + py) * G.nx + (G.g - R + px);
+    }
+
+    double q[4][W];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int j = 0; j < 2 * R; ++j) q[c][j] = u[c * G.cs + (size_t)(G.g - R + j) * zs + col];
+    double nq[4], nr[MAXRING];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nq[c] = u[c * G.cs + (size_t)(G.g + R) * zs + col];
+#pragma unroll
+    for (int j = 0; j < MAXRING; ++j)
+        if (j < nring) nr[j] = u[(size_t)G.g * zs + r_glb[j]];
+
+    const int lc = (iyl + R) * tx + (ix + R);
+    for (int iz = 0; iz < G.bs2; ++iz) {
+        const size_t zoff = (size_t)(G.g + iz) * zs;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            q[c][2 * R] = nq[c];
+            plane[c * tcs + lc] = q[c][R];
+        }
+#pragma unroll
+        for (int j = 0; j < MAXRING; ++j)
+            if (j < nring) plane[r_lds[j]] = nr[j];
+        if (iz + 1 < G.bs2) {  // prefetch the next plane
+            const size_t zn = zoff + zs;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) nq[c] = u[c * G.cs + zn + R * zs + col];
+#pragma unroll
+            for (int j = 0; j < MAXRING; ++j)
+                if (j < nring) nr[j] = u[zn + r_glb[j]];
+        }
+        __syncthreads();
+        // x/y from the LDS plane, z from the register queue
+        const auto A = [&](int c, int d, int o) -> double {
+            if (d == 2) return q[c][R + o];
+            return plane[c * tcs + lc + o * (d == 0 ? 1 : tx)];
+        };
+        const size_t gi = zoff + col;
+        double r[4];
+        acm_point<3, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
+        const size_t o = (size_t)b * G.blk + gi;
+        if constexpr (FUSED) {
+            const double u0c[4] = {q[0][R], q[1][R], q[2][R], q[3][R]};
+            stage_epilogue<4>(sa, o, G.cs, r, u0c, 1);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int j = 0; j < 2 * R; ++j) q[c][j] = q[c][j + 1];
+    }
+}
+
+int reach_of(int scheme)
+{
+    return scheme == WH_FD2 ? 1 : scheme == WH_FD4 ? 2 : 3;
+}
+
+// Is the z-marching kernel usable with this y-split?  Threads per workgroup
+// in [64, 512], the slab ring fits the per-thread ring registers, LDS <= 64 KB.
+bool zm_valid(int bs0, int bs1, int scheme, int ys)
+{
+    if (ys < 1 || bs1 % ys) return false;
+    const int R = reach_of(scheme), ty0 = bs1 / ys, nthr = bs0 * ty0;
+    if (nthr < 64 || nthr > 512) return false;
+    if (2 * R * (bs0 + ty0) > nthr) return false;
+    return sizeof(double) * 4 * (bs0 + 2 * R) * (ty0 + 2 * R) <= 64 * 1024;
+}
+
+// y-split of the 3D stage kernel: the pool's tuned value, or the smallest
+// valid split; 0 selects the flat kernel.
+static int zm_split(const wh_pool *P, const Geo &G, int scheme)
+{
+    if (P->k3d_ysplit == 0) return 0;
+    if (P->k3d_ysplit > 0 && zm_valid(G.bs0, G.bs1, scheme, P->k3d_ysplit)) return P->k3d_ysplit;
+    for (int ys = 1; ys <= 8; ++ys)
+        if (zm_valid(G.bs0, G.bs1, scheme, ys)) return ys;
+    return 0;
 }
 
 static Geo make_geo(const wh_pool *P)
@@ -298,6 +462,25 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
         default: set_error("unknown scheme"); return WH_E_ARG;
         }
 #undef L2D
+    } else if (const int ys = zm_split(P, G, a.scheme); ys > 0) {
+        const int R = reach_of(a.scheme);
+        const size_t lds = sizeof(double) * 4 * (G.bs0 + 2 * R) * (G.bs1 / ys + 2 * R);
+        dim3 grid(nact * ys), blockd(G.bs0 * (G.bs1 / ys));
+#define L3D(SC, SK)                                                                             \
+    hipLaunchKernelGGL((k_rhs3d_zm<SC, SK, FUSED>), grid, blockd, lds, P->stream, in, out, mask, \
+                       act, x0dx, G, a, sa, ys)
+        switch (a.scheme * 2 + a.skew) {
+        case WH_FD2 * 2: L3D(WH_FD2, false); break;
+        case WH_FD2 * 2 + 1: L3D(WH_FD2, true); break;
+        case WH_FD4 * 2: L3D(WH_FD4, false); break;
+        case WH_FD4 * 2 + 1: L3D(WH_FD4, true); break;
+        case WH_FD4_TW * 2: L3D(WH_FD4_TW, false); break;
+        case WH_FD4_TW * 2 + 1: L3D(WH_FD4_TW, true); break;
+        case WH_FD6 * 2: L3D(WH_FD6, false); break;
+        case WH_FD6 * 2 + 1: L3D(WH_FD6, true); break;
+        default: set_error("unknown scheme"); return WH_E_ARG;
+        }
+#undef L3D
     } else {
         const int npts = G.bs0 * G.bs1 * G.bs2;
         dim3 grid(nact * ((npts + 255) / 256)), blockd(256);
@@ -376,21 +559,54 @@ __global__ __launch_bounds__(256) void k_sync(double *__restrict__ arr,
     }
 }
 
+// Flat ghost fill: one thread per ghost point of the patches [p0, p1); the
+// patch of a point is found by binary search in the point-count prefix sum.
+__global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
+                                                   const wh_patch *__restrict__ pt,
+                                                   const int64_t *__restrict__ pstart, int p0,
+                                                   int p1, size_t blk, size_t cs, int nx, int ny,
+                                                   int neq)
+{
+    const int64_t e = pstart[p0] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= pstart[p1]) return;
+    int lo = p0, hi = p1 - 1;
+    while (lo < hi) {  // last patch with pstart <= e
+        const int mid = (lo + hi + 1) >> 1;
+        if (pstart[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const wh_patch P = pt[lo];
+    const int r = (int)(e - pstart[lo]);
+    const int n0 = P.n[0], n1 = P.n[1];
+    const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
+    const int st = (P.op == WH_PATCH_RESTRICT) ? 2 : 1;
+    const size_t ri =
+        ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx + (P.recv_lo[0] + i);
+    const size_t si = ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
+                      (P.send_lo[0] + st * i);
+    double *dst = arr + (size_t)P.recv * blk + ri;
+    const double *src = arr + (size_t)P.send * blk + si;
+    for (int c = 0; c < neq; ++c) dst[c * cs] = src[c * cs];
+}
+
 int launch_sync(wh_pool *P, int sync_case, double *arr, int one_receiver)
 {
     PatchSet &ps = P->ps[sync_case];
-    if (ps.nrecv == 0) return WH_OK;
-    for (int stage = 1; stage <= std::max(1, ps.max_stage); ++stage) {
-        const int st = ps.max_stage > 1 ? stage : 0;
-        if (one_receiver >= 0) {
-            hipLaunchKernelGGL(k_sync, dim3(1), dim3(256), 0, P->stream, arr, ps.d_patches,
-                               ps.d_recv_off, one_receiver, st, P->blk, P->cs, P->nx, P->ny,
-                               P->d.neq);
-        } else {
-            hipLaunchKernelGGL(k_sync, dim3(ps.nrecv), dim3(256), 0, P->stream, arr,
-                               ps.d_patches, ps.d_recv_off, 0, st, P->blk, P->cs, P->nx, P->ny,
-                               P->d.neq);
-        }
+    if (ps.n == 0) return WH_OK;
+    if (one_receiver >= 0) {
+        // one launch per (stage, receiver) row: the reference's per-block loop
+        hipLaunchKernelGGL(k_sync, dim3(1), dim3(256), 0, P->stream, arr, ps.d_patches,
+                           ps.d_recv_off, one_receiver, 0, P->blk, P->cs, P->nx, P->ny, P->d.neq);
+        WH_HIP(hipGetLastError());
+        return WH_OK;
+    }
+    for (int stage = 1; stage <= 3; ++stage) {
+        const int p0 = ps.stage_begin[stage], p1 = ps.stage_begin[stage + 1];
+        if (p1 <= p0) continue;
+        const int64_t npts = ps.h_pstart[p1] - ps.h_pstart[p0];
+        const unsigned nblk = (unsigned)((npts + 255) / 256);
+        hipLaunchKernelGGL(k_sync_flat, dim3(nblk), dim3(256), 0, P->stream, arr, ps.d_patches,
+                           ps.d_pstart, p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq);
     }
     WH_HIP(hipGetLastError());
     return WH_OK;

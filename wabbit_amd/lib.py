@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "wabbit_hip.h")
 
 WH_BLOCK, WH_TMP, WH_MASK, WH_WORK = 0, 1, 2, 16
 WH_PATCH_COPY, WH_PATCH_RESTRICT, WH_PATCH_PREDICT = 0, 1, 2
+WH_TUNE_K3D_YSPLIT = 1
 ERRORS = {-1: "WH_E_ARG", -2: "WH_E_HIP", -3: "WH_E_STATE", -4: "WH_E_DIVERGED", -5: "WH_E_NOMEM"}
 
 
@@ -100,6 +101,10 @@ def load(path: str = LIB_PATH):
                         ctypes.POINTER(ctypes.c_double)],
         "wh_run_steps": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int,
                          ctypes.POINTER(TimeParamsC), ctypes.c_int, ctypes.c_int],
+        "wh_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
+        "wh_get_tuning": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+        "wh_autotune": [_vp, ctypes.POINTER(AcmParamsC), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                        ctypes.POINTER(ctypes.c_double)],
         "wh_kernel_timing_enable": [_vp, ctypes.c_int],
         "wh_kernel_timing": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_int)],
@@ -241,6 +246,14 @@ class Pool:
 
     def synchronize(self):
         check(load().wh_synchronize(self.h))
+
+    def set_tuning(self, key, value):
+        check(load().wh_set_tuning(self.h, key, value))
+
+    def autotune(self, acm_c, reps=5):
+        v, ms = ctypes.c_int(), ctypes.c_double()
+        check(load().wh_autotune(self.h, ctypes.byref(acm_c), reps, ctypes.byref(v), ctypes.byref(ms)))
+        return v.value, ms.value
 
     def timing(self, on):
         check(load().wh_kernel_timing_enable(self.h, int(on)))
