@@ -171,6 +171,42 @@ int wh_time_get(wh_pool *pool, double *time, int *iteration, double *dt_last);
 int wh_run_steps(wh_pool *pool, const wh_acm_params *p, const double *butcher, int nrows,
                  const wh_time_params *tp, int nsteps, int use_graph);
 
+/* refineBlock (LIB/MESH/refinementExecute.f90:4-116) for a list of
+ * (mother, daughter, digit) triples: daughter `digit` of a mother takes its
+ * quadrant/octant of the mother's interpolated grid (prediction of order
+ * `order` = 2/4/6 over the whole ghost-inclusive block, so the mothers' ghost
+ * nodes must be synchronised first).  Daughters and mothers must be distinct
+ * pool slots. */
+int wh_refine_blocks(wh_pool *pool, int array, const int32_t *triples, int ntriples, int order);
+
+/* ---- host forest (metadata of one tree) --------------------------------
+ * The reference keeps this on the host (lgt_block, hvy_neighbor, ...); these
+ * helpers build the device tables from a block list (treecode, level, hvy id).
+ * Treecodes as in LIB/TREE/module_treelib.f90 (dim bits per level, level l's
+ * digit at bit offset (jmax-l)*dim, digit bits x=2, y=1, z=4). */
+typedef struct wh_forest wh_forest;
+int wh_forest_create(wh_forest **out, int dim, int jmax, const int *bs, int g,
+                     const double *domain, const int *periodic, int nb_max);
+int wh_forest_destroy(wh_forest *forest);
+int wh_forest_set_blocks(wh_forest *forest, int n, const int64_t *treecodes,
+                         const int32_t *levels, const int32_t *hvy_ids);
+int wh_forest_size(const wh_forest *forest, int *n);
+/* blocks in list order; x0dx (6 per block) as get_block_spacing_origin */
+int wh_forest_get_blocks(const wh_forest *forest, int cap, int64_t *treecodes, int32_t *levels,
+                         int32_t *hvy_ids, double *x0dx);
+/* Ghost patches of a 2:1-balanced leaf forest for ghost depth gminus/gplus
+ * (replaces prepare_ghost_synch_metadata + ghosts_setup_patches for the
+ * "full_leaf"/"full" cases, synchronize_ghosts_generic.f90:352-694,
+ * calc_data_bounds.f90:46-189): each ghost box is filled from the leaf that
+ * covers it -- COPY (stage 1), RESTRICT by decimation (stage 2) or PREDICT of
+ * `order` (stage 3).  out may be NULL to query *npatch. */
+int wh_forest_patches(const wh_forest *forest, int gminus, int gplus, int order, wh_patch *out,
+                      int cap, int *npatch);
+/* Refine blocks (by hvy id): daughters on free pool slots, mothers leave the
+ * list; writes (mother, daughter, digit) triples for wh_refine_blocks. */
+int wh_forest_refine(wh_forest *forest, int n, const int32_t *hvy_ids, int32_t *triples, int cap,
+                     int *ntriples);
+
 /* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
  * stage kernel (each owns Bs2/ysplit rows of (x,y) columns and marches in z);
  * 0 selects the flat one-point-per-thread kernel, -1 automatic. */

@@ -49,8 +49,67 @@ def lib():
         L.or_dt_min.argtypes = [ctypes.POINTER(OrAcmParams), _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 _dp, _dp]
         L.or_dt_min.restype = ctypes.c_double
+        L.or_sync_leaf.argtypes = [ctypes.POINTER(OrForest), _dp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int]
+        L.or_refine_block.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp,
+                                      ctypes.POINTER(_dp), ctypes.c_int]
+        L.or_prediction.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
         _LIB = L
     return _LIB
+
+
+class OrForest(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("jmax", ctypes.c_int), ("g", ctypes.c_int),
+                ("bs", ctypes.c_int * 3), ("n", ctypes.c_int),
+                ("tc", ctypes.POINTER(ctypes.c_int64)), ("level", ctypes.POINTER(ctypes.c_int))]
+
+
+class LeafForest:
+    """A leaf forest for the oracle: block b <-> row b of the heavy array."""
+
+    def __init__(self, dim, jmax, bs, g, treecodes, levels):
+        self.dim, self.jmax, self.g = dim, jmax, g
+        self.bs = list(bs) + [1] * (3 - len(bs))
+        self.tc = np.ascontiguousarray(treecodes, np.int64)
+        self.level = np.ascontiguousarray(levels, np.int32)
+        f = OrForest()
+        f.dim, f.jmax, f.g = dim, jmax, g
+        for i in range(3):
+            f.bs[i] = self.bs[i]
+        f.n = self.tc.shape[0]
+        f.tc = self.tc.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        f.level = self.level.ctypes.data_as(_ip)
+        self.c = f
+
+    def sync(self, hvy, gm, gp, order):
+        """Ghost fill of depth gm/gp in place; hvy (n, neq, nz, ny, nx)."""
+        assert hvy.flags.c_contiguous and hvy.dtype == np.float64
+        rc = lib().or_sync_leaf(ctypes.byref(self.c), _d(hvy), hvy.shape[1], gm, gp, order)
+        if rc:
+            raise RuntimeError(f"or_sync_leaf failed ({rc})")
+
+    def refine_all(self, hvy, order):
+        """refineBlock of every block below jmax; returns (treecodes, levels, hvy)."""
+        dim, g, bs = self.dim, self.g, self.bs
+        out_tc, out_lev, out = [], [], []
+        nd = 1 << dim
+        bsa = np.asarray(bs, np.int32)
+        for b in range(self.tc.shape[0]):
+            lev = int(self.level[b])
+            if lev >= self.jmax:
+                out_tc.append(int(self.tc[b]))
+                out_lev.append(lev)
+                out.append(hvy[b].copy())
+                continue
+            daughters = [np.zeros_like(hvy[b]) for _ in range(nd)]
+            arr = (_dp * nd)(*[_d(d) for d in daughters])
+            m = np.ascontiguousarray(hvy[b])
+            lib().or_refine_block(dim, _i(bsa), g, hvy.shape[1], _d(m), arr, order)
+            for k in range(nd):
+                out_tc.append(int(self.tc[b]) | (k << ((self.jmax - lev - 1) * dim)))
+                out_lev.append(lev + 1)
+                out.append(daughters[k])
+        return np.array(out_tc, np.int64), np.array(out_lev, np.int32), np.ascontiguousarray(np.stack(out))
 
 
 def _d(a):

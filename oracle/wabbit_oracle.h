@@ -95,6 +95,30 @@ void or_rk_step_equidistant(const or_acm_params *p, const int Bs[3], int g,
 double or_dt_min(const or_acm_params *p, const int Bs[3], int g, int neq,
                  int nblocks, const double *dxs, const double *hvy);
 
+/* A leaf forest: block b (0..n-1) has treecode tc[b] at level level[b] and
+ * lives in slot b of the heavy array (layout as above). */
+typedef struct {
+    int dim, jmax, g;
+    int bs[3];
+    int n;
+    const int64_t *tc;
+    const int *level;
+} or_forest;
+
+/* `prediction` (module_wavelets.f90:96-284) as whole-array passes:
+ * coarse (nx,ny,nz) -> fine (2nx-1, 2ny-1, 2nz-1); nz = 1 in 2D. */
+void or_prediction(const double *coarse, int nx, int ny, int nz, double *fine, int order);
+
+/* Ghost fill of depth gm/gp on a 2:1-balanced periodic leaf forest by point
+ * location (see adapt.c).  Returns 0, -1 duplicate block, -2 not balanced,
+ * -3 prediction outside the coarse block's interpolated grid. */
+int or_sync_leaf(const or_forest *f, double *hvy, int neq, int gm, int gp, int order);
+
+/* refineBlock (refinementExecute.f90:4-116) of one mother into 2^dim
+ * daughter arrays (ghost-inclusive, same layout). */
+void or_refine_block(int dim, const int bs[3], int g, int neq, const double *mother,
+                     double *const *daughters, int order);
+
 #ifdef __cplusplus
 }
 #endif

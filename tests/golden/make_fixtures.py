@@ -92,6 +92,39 @@ def taylor_green():
         _save(os.path.join(HERE, f"tg3d_fd{order}.npz"), fields, m, params)
 
 
+def wavelets():
+    """TESTING/wavelets: input vor_000020000000.h5 (112 blocks, levels 2-5,
+    Bs=32, old digit-array treecodes), the --refine-everywhere outputs
+    (vor_00100.h5, kept by the reference for CDF20/40/60; CDF20 and CDF60
+    subsampled to every 8th block) and the --coarsen-everywhere outputs of the
+    lifted wavelets (vor_00200.h5; for CDF20/40/60 it equals the input bit for
+    bit, asserted here)."""
+    W = "/root/reference/TESTING/wavelets"
+    src = read_wabbit_file(os.path.join(W, "vor_000020000000.h5"))
+    _save(os.path.join(HERE, "wavelet_input.npz"), {"vor": src.blocks}, src, {})
+    for wav in ("CDF20", "CDF40", "CDF60"):
+        f = read_wabbit_file(os.path.join(W, f"adaptive_{wav}", "vor_00100.h5"))
+        keep = np.arange(f.blocks.shape[0])
+        if wav != "CDF40":
+            keep = keep[::8]
+        sub = type(f)(**{**f.__dict__})
+        sub.blocks, sub.treecode, sub.level = f.blocks[keep], f.treecode[keep], f.level[keep]
+        sub.origin, sub.spacing = f.origin[keep], f.spacing[keep]
+        _save(os.path.join(HERE, f"wavelet_refine_{wav}.npz"), {"vor": sub.blocks}, sub,
+              {"wavelet": wav, "subsample": int(wav != "CDF40") * 8, "n_total": int(f.blocks.shape[0])})
+    key = {(int(t), int(l)): n for n, (t, l) in enumerate(zip(src.treecode, src.level))}
+    for wav in ("CDF20", "CDF22", "CDF40", "CDF42", "CDF44", "CDF60", "CDF62"):
+        f = read_wabbit_file(os.path.join(W, f"adaptive_{wav}", "vor_00200.h5"))
+        idx = [key[(int(t), int(l))] for t, l in zip(f.treecode, f.level)]
+        if wav in ("CDF20", "CDF40", "CDF60"):
+            # interiors identical to the input (the stored extra point is a
+            # synced ghost value and may differ)
+            assert np.array_equal(f.blocks[:, :-1, :-1], src.blocks[idx][:, :-1, :-1])
+            continue
+        _save(os.path.join(HERE, f"wavelet_coarsen_{wav}.npz"), {"vor": f.blocks}, f, {"wavelet": wav})
+
+
 if __name__ == "__main__":
     vortex2d()
     taylor_green()
+    wavelets()

@@ -190,3 +190,114 @@ def random_mask(forest, seed):
     m = rng.uniform(0.0, 1.0, size=(forest.n, 6, nz, ny, nx))
     m[:, 4] = rng.integers(0, 3, size=m[:, 4].shape).astype(np.float64)
     return m
+
+
+# ---------------------------------------------------------------------------
+# random 2:1-balanced leaf forests and a host-side patch applier
+# ---------------------------------------------------------------------------
+def _tc(ijk, level, jmax, dim):
+    from wabbit_amd.forest import treecode_from_ijk
+    return treecode_from_ijk(list(ijk) + [0] * (3 - len(ijk)), level, jmax, dim)
+
+
+def random_graded_forest(dim, jmin, jmax, seed, nref):
+    """Leaves (level, ijk) of a periodic forest: uniform at jmin, nref random
+    refinements, then refined until every pair of touching leaves (faces,
+    edges, corners, across the periodic boundary) differs by at most one
+    level.  Returns (treecodes, levels) sorted by (level, treecode)."""
+    rng = np.random.default_rng(seed)
+    nb = 2 ** jmin
+    leaves = set()
+    for idx in np.ndindex(*([nb] * dim)):
+        leaves.add((jmin, tuple(int(v) for v in idx)))
+
+    def children(l, ijk):
+        out = []
+        for c in np.ndindex(*([2] * dim)):
+            out.append((l + 1, tuple(2 * ijk[d] + int(c[d]) for d in range(dim))))
+        return out
+
+    for _ in range(nref):
+        cand = sorted(x for x in leaves if x[0] < jmax)
+        if not cand:
+            break
+        l, ijk = cand[rng.integers(len(cand))]
+        leaves.remove((l, ijk))
+        leaves.update(children(l, ijk))
+    ext = 2 ** jmax
+
+    def box(l, ijk):
+        s = 2 ** (jmax - l)
+        return [(ijk[d] * s, (ijk[d] + 1) * s) for d in range(dim)]
+
+    def touch(a, b):
+        for d in range(dim):
+            ok = False
+            for sh in (-ext, 0, ext):
+                if a[d][0] <= b[d][1] + sh and b[d][0] + sh <= a[d][1]:
+                    ok = True
+            if not ok:
+                return False
+        return True
+
+    changed = True
+    while changed:
+        changed = False
+        lst = sorted(leaves)
+        boxes = {x: box(*x) for x in lst}
+        for a in lst:
+            for b in lst:
+                if b[0] >= a[0] + 2 and touch(boxes[a], boxes[b]):
+                    leaves.remove(a)
+                    leaves.update(children(*a))
+                    changed = True
+                    break
+            if changed:
+                break
+    lst = sorted(leaves)
+    tcs = np.array([_tc(ijk, l, jmax, dim) for l, ijk in lst], np.int64)
+    lev = np.array([l for l, _ in lst], np.int32)
+    return tcs, lev
+
+
+def apply_patches_host(hvy, patches, order):
+    """Apply a ghost-patch table to hvy (n, neq, nz, ny, nx) in stage order;
+    prediction from the oracle's whole-array prediction of the sender block
+    taken after stages 1-2.  Test helper (host)."""
+    from oracle import pyoracle as po
+    out = hvy.copy()
+    n, neq, nz, ny, nx = hvy.shape
+    dim = 3 if nz > 1 else 2
+    for stage in (1, 2, 3):
+        snap = out.copy()
+        pred_cache = {}
+        for p in patches[patches["stage"] == stage]:
+            r, s = int(p["recv"]), int(p["send"])
+            rl, sl, nn = p["recv_lo"], p["send_lo"], p["n"]
+            rsl = (slice(rl[2], rl[2] + nn[2]), slice(rl[1], rl[1] + nn[1]), slice(rl[0], rl[0] + nn[0]))
+            if p["op"] == 0:
+                ssl = (slice(sl[2], sl[2] + nn[2]), slice(sl[1], sl[1] + nn[1]), slice(sl[0], sl[0] + nn[0]))
+                out[r][(slice(None),) + rsl] = snap[s][(slice(None),) + ssl]
+            elif p["op"] == 1:
+                ssl = tuple(slice(sl[d], sl[d] + 2 * (nn[d] - 1) + 1, 2) for d in (2, 1, 0))
+                out[r][(slice(None),) + rsl] = snap[s][(slice(None),) + ssl]
+            else:
+                if s not in pred_cache:
+                    fz = 2 * nz - 1 if dim == 3 else 1
+                    fine = np.zeros((neq, fz, 2 * ny - 1, 2 * nx - 1))
+                    for c in range(neq):
+                        src = np.ascontiguousarray(snap[s, c])
+                        dst = np.zeros((fz, 2 * ny - 1, 2 * nx - 1))
+                        po.lib().or_prediction(po._d(src), nx, ny, nz, po._d(dst), order)
+                        fine[c] = dst
+                    pred_cache[s] = fine
+                fsl = (slice(sl[2], sl[2] + nn[2]), slice(sl[1], sl[1] + nn[1]), slice(sl[0], sl[0] + nn[0]))
+                out[r][(slice(None),) + rsl] = pred_cache[s][(slice(None),) + fsl]
+    return out
+
+
+def random_block_data(n, neq, dim, bs, g, seed):
+    rng = np.random.default_rng(seed)
+    nx = bs + 2 * g
+    nz = nx if dim == 3 else 1
+    return rng.standard_normal((n, neq, nz, nx, nx))

@@ -1,0 +1,54 @@
+"""Host forest (C++ wh_forest_*): the ghost-patch tables it builds, applied on
+the host, fill every ghost point exactly as the oracle's point-location fill
+does (random 2:1-balanced periodic forests, 2D and 3D, predictor orders 2/4/6,
+full and RHS ghost depths).  Only host metadata calls: no GPU needed."""
+import numpy as np
+import pytest
+
+from helpers import apply_patches_host, random_block_data, random_graded_forest
+from oracle import pyoracle as po
+from wabbit_amd.tree import Forest
+
+
+CASES = [
+    # dim, jmin, jmax, bs, g, order, depth
+    (2, 2, 5, 8, 3, 4, 3),
+    (2, 2, 5, 8, 3, 4, 2),
+    (2, 1, 4, 8, 1, 2, 1),
+    (2, 2, 5, 12, 5, 6, 5),
+    (3, 1, 3, 6, 3, 4, 3),
+    (3, 1, 3, 6, 3, 4, 2),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_patch_tables_match_point_location(case, seed):
+    dim, jmin, jmax, bs, g, order, depth = case
+    tcs, lev = random_graded_forest(dim, jmin, jmax, seed, nref=6 if dim == 2 else 3)
+    n = tcs.shape[0]
+    hvy = random_block_data(n, 2, dim, bs, g, seed)
+    f = Forest(dim, jmax, [bs] * dim, g, [1.0, 2.0, 1.5][:dim], nb_max=n + 8)
+    f.set_blocks(tcs, lev, np.arange(n, dtype=np.int32))
+    pt = f.patches(depth, depth, order)
+    got = apply_patches_host(hvy, pt, order)
+    want = hvy.copy()
+    po.LeafForest(dim, jmax, [bs] * dim, g, tcs, lev).sync(want, depth, depth, order)
+    assert np.array_equal(got, want)
+    assert set(np.unique(pt["op"])) <= {0, 1, 2}
+    if len(np.unique(lev)) > 1:
+        assert (pt["op"] == 1).any() and (pt["op"] == 2).any()
+    f.close()
+
+
+def test_refine_bookkeeping():
+    f = Forest(2, 4, [8, 8], 3, [1.0, 1.0], nb_max=40)
+    tcs, lev = random_graded_forest(2, 1, 1, 0, 0)
+    f.set_blocks(tcs, lev, np.arange(4, dtype=np.int32))
+    tri = f.refine(np.array([0, 2], np.int32))
+    assert tri.shape == (8, 3)
+    tc, lv, hv, x = f.blocks()
+    assert f.n == 4 - 2 + 8 and (lv == 2).sum() == 8
+    assert len(set(hv.tolist())) == f.n
+    assert not set(tri[:, 1].tolist()) & {0, 2}
+    f.close()

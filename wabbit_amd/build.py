@@ -35,7 +35,7 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = True) -> str:
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     out = os.path.join(PKG, LIBNAME)
     tmp = out + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs

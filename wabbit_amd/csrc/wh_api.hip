@@ -199,6 +199,19 @@ static int generic_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows
     return combine(S, W(0), (int)terms.size(), terms.data(), coefs.data());
 }
 
+// Prediction reads, for the refined-grid indices [fmin, fmax] along one
+// direction: even f copies coarse point f/2, odd f reads coarse points
+// f/2-N+1 .. f/2+N.  True when all of them lie in [0, ext).
+static bool pred_range_ok(int fmin, int fmax, int N, int ext)
+{
+    if (fmin < 0 || fmax < fmin) return false;
+    if ((fmax >> 1) >= ext) return false;
+    const int omin = (fmin & 1) ? fmin : fmin + 1;
+    const int omax = (fmax & 1) ? fmax : fmax - 1;
+    if (omin > omax) return true;  // only copies
+    return (omin >> 1) - N + 1 >= 0 && (omax >> 1) + N < ext;
+}
+
 // Checks that the kernels' reads stay inside the pool for these parameters.
 static int check_acm(const wh_pool *P, const wh_acm_params *p)
 {
@@ -430,16 +443,27 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
         const wh_patch &q = patches[i];
         WH_CHECK(q.recv >= 0 && q.recv < P->d.nb_max && q.send >= 0 && q.send < P->d.nb_max,
                  WH_E_ARG, "patch block id out of pool range");
-        WH_CHECK(q.op == WH_PATCH_COPY || q.op == WH_PATCH_RESTRICT, WH_E_ARG,
-                 "patch op not supported by this build");
+        WH_CHECK(q.op == WH_PATCH_COPY || q.op == WH_PATCH_RESTRICT || q.op == WH_PATCH_PREDICT,
+                 WH_E_ARG, "unknown patch op");
         WH_CHECK(q.stage >= 1 && q.stage <= 3, WH_E_ARG, "patch stage must be 1..3");
+        WH_CHECK(q.op != WH_PATCH_PREDICT || q.aux == 2 || q.aux == 4 || q.aux == 6, WH_E_ARG,
+                 "predict patch order must be 2, 4 or 6");
         const int st = q.op == WH_PATCH_RESTRICT ? 2 : 1;
         for (int d = 0; d < 3; ++d) {
             WH_CHECK(q.n[d] >= 1, WH_E_ARG, "empty patch");
             WH_CHECK(q.recv_lo[d] >= 0 && q.recv_lo[d] + q.n[d] <= ext[d], WH_E_ARG,
                      "receiver box outside the block");
-            WH_CHECK(q.send_lo[d] >= 0 && q.send_lo[d] + st * (q.n[d] - 1) < ext[d], WH_E_ARG,
-                     "sender box outside the block");
+            if (q.op == WH_PATCH_PREDICT) {
+                // coarse stencil of every point must stay inside the sender block
+                const int N = q.aux / 2;
+                const int fmin = q.send_lo[d], fmax = q.send_lo[d] + q.n[d] - 1;
+                const bool interp = (d < P->d.dim);
+                WH_CHECK(interp ? pred_range_ok(fmin, fmax, N, ext[d]) : (fmin == 0 && fmax == 0),
+                         WH_E_ARG, "prediction stencil outside the sender block");
+            } else {
+                WH_CHECK(q.send_lo[d] >= 0 && q.send_lo[d] + st * (q.n[d] - 1) < ext[d], WH_E_ARG,
+                         "sender box outside the block");
+            }
         }
     }
     WH_HIP(hipStreamSynchronize(P->stream));
@@ -750,3 +774,48 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
 }
 
 }  // extern "C"
+
+namespace wh {
+int launch_refine(wh_pool *P, double *arr, const int32_t *d_pairs, int npairs, int order);
+}
+
+extern "C" int wh_refine_blocks(wh_pool *P, int array, const int32_t *triples, int ntriples,
+                                int order)
+{
+    WH_CHECK(P && (ntriples == 0 || triples), WH_E_ARG, "null argument");
+    WH_CHECK(order == 2 || order == 4 || order == 6, WH_E_ARG, "order must be 2, 4 or 6");
+    WH_CHECK(ntriples >= 0, WH_E_ARG, "negative count");
+    double *arr = array_ptr(P, array, nullptr, nullptr);
+    WH_CHECK(arr && (array == WH_BLOCK || array >= WH_WORK), WH_E_ARG, "bad array");
+    const int N = order / 2;
+    // the daughters' points reach the mother's interpolated grid at fine index
+    // g .. g + 2*Bs + g - 1 + ...: every stencil must stay in [0, nx)
+    for (int d = 0; d < P->d.dim; ++d) {
+        const int ext = d == 0 ? P->nx : d == 1 ? P->ny : P->nz;
+        const int fmin = P->d.g, fmax = P->d.g + P->d.bs[d] + ext - 1;
+        WH_CHECK(pred_range_ok(fmin, fmax, N, ext), WH_E_ARG,
+                 "too few ghost nodes for this predictor order");
+    }
+    std::vector<char> mother(P->d.nb_max, 0), daughter(P->d.nb_max, 0);
+    for (int i = 0; i < ntriples; ++i) {
+        const int m = triples[3 * i], dgh = triples[3 * i + 1], k = triples[3 * i + 2];
+        WH_CHECK(m >= 0 && m < P->d.nb_max && dgh >= 0 && dgh < P->d.nb_max, WH_E_ARG,
+                 "hvy id out of pool range");
+        WH_CHECK(k >= 0 && k < (1 << P->d.dim), WH_E_ARG, "daughter digit out of range");
+        WH_CHECK(!daughter[dgh], WH_E_ARG, "daughter slot written twice");
+        mother[m] = 1;
+        daughter[dgh] = 1;
+    }
+    for (int h = 0; h < P->d.nb_max; ++h)
+        WH_CHECK(!(mother[h] && daughter[h]), WH_E_ARG, "a slot is both mother and daughter");
+    if (ntriples == 0) return WH_OK;
+    int32_t *d_tr = nullptr;
+    WH_HIP(hipMallocAsync((void **)&d_tr, sizeof(int32_t) * 3 * ntriples, P->stream));
+    WH_HIP(hipMemcpyAsync(d_tr, triples, sizeof(int32_t) * 3 * ntriples, hipMemcpyHostToDevice,
+                          P->stream));
+    int rc = launch_refine(P, arr, d_tr, ntriples, order);
+    (void)hipFreeAsync(d_tr, P->stream);
+    if (rc) return rc;
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}

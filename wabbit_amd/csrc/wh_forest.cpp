@@ -1,0 +1,453 @@
+// wh_forest.cpp -- host-side forest metadata for the device pool: block
+// list (treecode, level, hvy id), neighbour search and ghost-patch tables,
+// refinement bookkeeping.  Host code only (no kernels).
+//
+// Treecode layout (LIB/TREE/module_treelib.f90:107-283): int64 with dim bits
+// per level; the digit of level l sits at bit offset (Jmax-l)*dim; digit bits
+// (x, y, z) = (2, 1, 4) (refinementExecute.f90:98-105).
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/wabbit_hip.h"
+#include <cmath>
+
+namespace wh {
+void set_error(const std::string &msg);
+}
+
+namespace {
+
+struct Block {
+    int64_t tc;
+    int level;
+    int hvy;
+    int status;  // refinement status (module_globals.f90:25-32)
+};
+
+inline uint64_t key(int level, int64_t tc) { return ((uint64_t)level << 58) ^ (uint64_t)tc; }
+
+}  // namespace
+
+struct wh_forest {
+    int dim = 2, jmax = 0, g = 0;
+    int bs[3] = {1, 1, 1};
+    double domain[3] = {1, 1, 1};
+    int periodic[3] = {1, 1, 1};
+    int nb_max = 0;
+    std::vector<Block> blocks;              // active blocks (leaf list)
+    std::unordered_map<uint64_t, int> map;  // (level, tc) -> index in blocks
+    std::vector<int> free_hvy;              // free pool slots
+
+    void rebuild_map()
+    {
+        map.clear();
+        for (size_t i = 0; i < blocks.size(); ++i) map[key(blocks[i].level, blocks[i].tc)] = (int)i;
+    }
+    int digit_shift(int level) const { return (jmax - level) * dim; }
+    void ijk_of(int64_t tc, int level, int ijk[3]) const
+    {
+        ijk[0] = ijk[1] = ijk[2] = 0;
+        for (int l = 1; l <= level; ++l) {
+            const int dgt = (int)((tc >> digit_shift(l)) & ((1 << dim) - 1));
+            ijk[0] = 2 * ijk[0] + ((dgt >> 1) & 1);
+            ijk[1] = 2 * ijk[1] + (dgt & 1);
+            ijk[2] = 2 * ijk[2] + (dim == 3 ? ((dgt >> 2) & 1) : 0);
+        }
+    }
+    int64_t tc_of(const int ijk[3], int level) const
+    {
+        int64_t tc = 0;
+        for (int l = 1; l <= level; ++l) {
+            const int sh = level - l;
+            const int bx = (ijk[0] >> sh) & 1, by = (ijk[1] >> sh) & 1;
+            const int bz = dim == 3 ? ((ijk[2] >> sh) & 1) : 0;
+            tc |= (int64_t)(by + 2 * bx + 4 * bz) << digit_shift(l);
+        }
+        return tc;
+    }
+    // block at grid index ijk on level (periodic wrap); -1 if not active.
+    // out_shift: periodic shift in block units applied (ijk_wrapped = ijk - shift*n)
+    int find(int level, const int ijk_in[3], int wrapped[3]) const
+    {
+        const int n = 1 << level;
+        for (int d = 0; d < 3; ++d) {
+            int v = ijk_in[d];
+            if (d < dim) {
+                if (v < 0 || v >= n) {
+                    if (!periodic[d]) return -1;
+                    v = ((v % n) + n) % n;
+                }
+            } else {
+                v = 0;
+            }
+            wrapped[d] = v;
+        }
+        auto it = map.find(key(level, tc_of(wrapped, level)));
+        return it == map.end() ? -1 : it->second;
+    }
+    void x0dx(const Block &b, double out[6]) const
+    {
+        int ijk[3];
+        ijk_of(b.tc, b.level, ijk);
+        for (int d = 0; d < 3; ++d) {
+            if (d < dim) {
+                // get_block_spacing_origin_b: dx = 2^-J * L / Bs, x0 = ((ijk)*Bs) * dx
+                const double dx = std::ldexp(1.0, -b.level) * domain[d] / (double)bs[d];
+                out[3 + d] = dx;
+                out[d] = (double)(ijk[d] * bs[d]) * dx;
+            } else {
+                out[d] = 0.0;
+                out[3 + d] = 0.0;
+            }
+        }
+    }
+};
+
+namespace {
+
+// Ghost box of receiver b in direction o (per dim -1/0/+1): ghost-inclusive
+// index ranges [lo, lo+n) with depth gm (lower) / gp (upper).
+void ghost_box(const wh_forest &F, const int o[3], int gm, int gp, int lo[3], int n[3])
+{
+    for (int d = 0; d < 3; ++d) {
+        if (d >= F.dim) {
+            lo[d] = 0;
+            n[d] = 1;
+        } else if (o[d] < 0) {
+            lo[d] = F.g - gm;
+            n[d] = gm;
+        } else if (o[d] > 0) {
+            lo[d] = F.g + F.bs[d];
+            n[d] = gp;
+        } else {
+            lo[d] = F.g;
+            n[d] = F.bs[d];
+        }
+    }
+}
+
+// order of a predictor "multiresolution_<2N>th": stencil half-width N
+}  // namespace
+
+extern "C" {
+
+int wh_forest_create(wh_forest **out, int dim, int jmax, const int *bs, int g,
+                     const double *domain, const int *periodic, int nb_max)
+{
+    if (!out || !bs || !domain || (dim != 2 && dim != 3) || jmax < 0 || jmax > 18 || g < 1 ||
+        nb_max < 1) {
+        wh::set_error("wh_forest_create: bad argument");
+        return WH_E_ARG;
+    }
+    wh_forest *F = new wh_forest();
+    F->dim = dim;
+    F->jmax = jmax;
+    F->g = g;
+    F->nb_max = nb_max;
+    for (int d = 0; d < 3; ++d) {
+        F->bs[d] = d < dim ? bs[d] : 1;
+        F->domain[d] = d < dim ? domain[d] : 1.0;
+        F->periodic[d] = periodic ? periodic[d] : 1;
+    }
+    for (int h = nb_max - 1; h >= 0; --h) F->free_hvy.push_back(h);
+    *out = F;
+    return WH_OK;
+}
+
+int wh_forest_destroy(wh_forest *F)
+{
+    delete F;
+    return WH_OK;
+}
+
+int wh_forest_set_blocks(wh_forest *F, int n, const int64_t *tc, const int32_t *level,
+                         const int32_t *hvy)
+{
+    if (!F || n < 0 || (n && (!tc || !level || !hvy)) || n > F->nb_max) {
+        wh::set_error("wh_forest_set_blocks: bad argument");
+        return WH_E_ARG;
+    }
+    std::vector<char> used(F->nb_max, 0);
+    F->blocks.clear();
+    for (int i = 0; i < n; ++i) {
+        if (hvy[i] < 0 || hvy[i] >= F->nb_max || used[hvy[i]] || level[i] < 0 || level[i] > F->jmax) {
+            wh::set_error("wh_forest_set_blocks: hvy id / level out of range or duplicate");
+            return WH_E_ARG;
+        }
+        used[hvy[i]] = 1;
+        F->blocks.push_back(Block{tc[i], level[i], hvy[i], 0});
+    }
+    F->free_hvy.clear();
+    for (int h = F->nb_max - 1; h >= 0; --h)
+        if (!used[h]) F->free_hvy.push_back(h);
+    F->rebuild_map();
+    if ((int)F->map.size() != n) {
+        wh::set_error("wh_forest_set_blocks: duplicate (level, treecode)");
+        return WH_E_ARG;
+    }
+    return WH_OK;
+}
+
+int wh_forest_size(const wh_forest *F, int *n)
+{
+    if (!F || !n) return WH_E_ARG;
+    *n = (int)F->blocks.size();
+    return WH_OK;
+}
+
+int wh_forest_get_blocks(const wh_forest *F, int cap, int64_t *tc, int32_t *level, int32_t *hvy,
+                         double *x0dx)
+{
+    if (!F || cap < (int)F->blocks.size()) {
+        wh::set_error("wh_forest_get_blocks: capacity too small");
+        return WH_E_ARG;
+    }
+    for (size_t i = 0; i < F->blocks.size(); ++i) {
+        const Block &b = F->blocks[i];
+        if (tc) tc[i] = b.tc;
+        if (level) level[i] = b.level;
+        if (hvy) hvy[i] = b.hvy;
+        if (x0dx) F->x0dx(b, x0dx + 6 * i);
+    }
+    return WH_OK;
+}
+
+// Ghost patches of a leaf forest (sync_ghosts_RHS_tree "full_leaf" and
+// sync_ghosts_tree "full" on leaf grids, synchronize_ghosts_generic.f90:352-694):
+// for every block and each of the 3^dim-1 directions the ghost box of depth
+// gminus/gplus is filled from the leaf that covers it: a same-level neighbour
+// (COPY, stage 1), the finer neighbours adjacent to the box (RESTRICT, stage 2:
+// decimation), or the coarser neighbour (PREDICT, stage 3: interpolation of
+// order `order`, 2/4/6).  aux of a PREDICT patch carries the order.
+int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_patch *out,
+                      int cap, int *npatch)
+{
+    if (!F || !npatch || gminus < 0 || gplus < 0 || gminus > F->g || gplus > F->g ||
+        (order != 2 && order != 4 && order != 6)) {
+        wh::set_error("wh_forest_patches: bad argument");
+        return WH_E_ARG;
+    }
+    const int dim = F->dim, g = F->g;
+    const int N = order / 2;
+    std::vector<wh_patch> P;
+    int dirs[27][3], nd = 0;
+    for (int oz = (dim == 3 ? -1 : 0); oz <= (dim == 3 ? 1 : 0); ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                if (!ox && !oy && !oz) continue;
+                dirs[nd][0] = ox;
+                dirs[nd][1] = oy;
+                dirs[nd][2] = oz;
+                ++nd;
+            }
+    const int nxyz[3] = {F->bs[0] + 2 * g, F->bs[1] + 2 * g, dim == 3 ? F->bs[2] + 2 * g : 1};
+    for (const Block &b : F->blocks) {
+        int ijk[3];
+        F->ijk_of(b.tc, b.level, ijk);
+        for (int q = 0; q < nd; ++q) {
+            const int *o = dirs[q];
+            int rlo[3], rn[3];
+            ghost_box(*F, o, gminus, gplus, rlo, rn);
+            if (rn[0] <= 0 || rn[1] <= 0 || rn[2] <= 0) continue;
+            int nb[3] = {ijk[0] + o[0], ijk[1] + o[1], ijk[2] + o[2]}, w[3];
+            // 1) same level
+            int s = F->find(b.level, nb, w);
+            if (s >= 0) {
+                wh_patch p{};
+                p.recv = b.hvy;
+                p.send = F->blocks[s].hvy;
+                for (int d = 0; d < 3; ++d) {
+                    p.recv_lo[d] = rlo[d];
+                    p.n[d] = rn[d];
+                    // receiver index i <-> sender index i - o*Bs
+                    p.send_lo[d] = d < dim ? rlo[d] - o[d] * F->bs[d] : 0;
+                }
+                p.op = WH_PATCH_COPY;
+                p.stage = 1;
+                P.push_back(p);
+                continue;
+            }
+            // 2) finer neighbours: children of the virtual same-level neighbour
+            //    that touch the receiver box
+            bool finer = false;
+            if (b.level < F->jmax) {
+                int lo_c[3], hi_c[3];  // child index range (0/1) per dim
+                for (int d = 0; d < 3; ++d) {
+                    if (d >= dim) {
+                        lo_c[d] = hi_c[d] = 0;
+                    } else if (o[d] < 0) {
+                        lo_c[d] = hi_c[d] = 1;  // the child adjacent to the receiver
+                    } else if (o[d] > 0) {
+                        lo_c[d] = hi_c[d] = 0;
+                    } else {
+                        lo_c[d] = 0;
+                        hi_c[d] = 1;
+                    }
+                }
+                std::vector<wh_patch> fp;
+                bool all = true;
+                for (int cz = lo_c[2]; cz <= hi_c[2] && all; ++cz)
+                    for (int cy = lo_c[1]; cy <= hi_c[1] && all; ++cy)
+                        for (int cx = lo_c[0]; cx <= hi_c[0] && all; ++cx) {
+                            const int cc[3] = {cx, cy, cz};
+                            int cij[3];
+                            for (int d = 0; d < 3; ++d) cij[d] = d < dim ? 2 * nb[d] + cc[d] : 0;
+                            int cw[3];
+                            const int f = F->find(b.level + 1, cij, cw);
+                            if (f < 0) {
+                                all = false;
+                                break;
+                            }
+                            wh_patch p{};
+                            p.recv = b.hvy;
+                            p.send = F->blocks[f].hvy;
+                            bool ok = true;
+                            for (int d = 0; d < 3; ++d) {
+                                if (d >= dim) {
+                                    p.recv_lo[d] = 0;
+                                    p.n[d] = 1;
+                                    p.send_lo[d] = 0;
+                                    continue;
+                                }
+                                int lo = rlo[d], n = rn[d];
+                                if (o[d] == 0) {  // half of the face covered by this child
+                                    const int h = F->bs[d] / 2;
+                                    lo = g + cc[d] * h;
+                                    n = h;
+                                }
+                                p.recv_lo[d] = lo;
+                                p.n[d] = n;
+                                // receiver ghost-incl index i -> position (i-g)*dxc relative to
+                                // receiver origin; child origin relative to receiver origin in
+                                // fine units: (2*nb - 2*ijk + c)*Bs ... expressed per dim
+                                const int child_off_f = (2 * o[d] + cc[d]) * F->bs[d];  // fine pts
+                                // fine index in child (ghost-incl) = 2*(i-g) - child_off_f + g
+                                p.send_lo[d] = 2 * (lo - g) - child_off_f + g;
+                                if (p.send_lo[d] < 0 || p.send_lo[d] + 2 * (n - 1) >= nxyz[d]) ok = false;
+                            }
+                            if (!ok) {
+                                all = false;
+                                break;
+                            }
+                            p.op = WH_PATCH_RESTRICT;
+                            p.stage = 2;
+                            fp.push_back(p);
+                        }
+                if (all && !fp.empty()) {
+                    P.insert(P.end(), fp.begin(), fp.end());
+                    finer = true;
+                }
+            }
+            if (finer) continue;
+            // 3) coarser neighbour
+            if (b.level > 0) {
+                int cij[3];
+                for (int d = 0; d < 3; ++d) cij[d] = d < dim ? (int)std::floor(nb[d] / 2.0) : 0;
+                int cw[3];
+                const int c = F->find(b.level - 1, cij, cw);
+                if (c >= 0) {
+                    wh_patch p{};
+                    p.recv = b.hvy;
+                    p.send = F->blocks[c].hvy;
+                    bool ok = true;
+                    for (int d = 0; d < 3; ++d) {
+                        p.recv_lo[d] = rlo[d];
+                        p.n[d] = rn[d];
+                        if (d >= dim) {
+                            p.send_lo[d] = 0;
+                            continue;
+                        }
+                        // receiver origin relative to coarse origin, in fine points
+                        const int off_f = ijk[d] * F->bs[d] - 2 * cij[d] * F->bs[d];
+                        // fine index in the coarse block's doubled ghost-incl index space
+                        p.send_lo[d] = (rlo[d] - g) + off_f + 2 * g;
+                        const int fmin = p.send_lo[d], fmax = p.send_lo[d] + rn[d] - 1;
+                        // even f copies coarse f/2, odd f reads f/2-N+1 .. f/2+N
+                        const int omin = (fmin & 1) ? fmin : fmin + 1;
+                        const int omax = (fmax & 1) ? fmax : fmax - 1;
+                        if (fmin < 0 || (fmax >> 1) >= nxyz[d]) ok = false;
+                        if (omin <= omax && ((omin >> 1) - N + 1 < 0 || (omax >> 1) + N >= nxyz[d]))
+                            ok = false;
+                    }
+                    if (!ok) {
+                        wh::set_error("wh_forest_patches: prediction stencil leaves the coarse block "
+                                      "(too few ghost nodes for this predictor)");
+                        return WH_E_ARG;
+                    }
+                    p.op = WH_PATCH_PREDICT;
+                    p.stage = 3;
+                    p.aux = order;
+                    P.push_back(p);
+                }
+            }
+            // no cover: non-periodic domain boundary, left to boundary conditions
+        }
+    }
+    *npatch = (int)P.size();
+    if (out) {
+        if (cap < (int)P.size()) {
+            wh::set_error("wh_forest_patches: capacity too small");
+            return WH_E_ARG;
+        }
+        std::copy(P.begin(), P.end(), out);
+    }
+    return WH_OK;
+}
+
+// Refine the listed blocks (refinementExecute.f90:4-116 bookkeeping): each
+// gets 2^dim daughters on fresh hvy ids (daughter k has digit k at level+1);
+// the mother leaves the active list and its slot returns to the free list.
+// pairs_out[3*i] = mother hvy, [3*i+1] = daughter hvy, [3*i+2] = daughter digit.
+int wh_forest_refine(wh_forest *F, int n, const int32_t *hvy_list, int32_t *pairs_out, int cap,
+                     int *npairs)
+{
+    if (!F || n < 0 || (n && !hvy_list) || !npairs) return WH_E_ARG;
+    const int nd = 1 << F->dim;
+    if ((int)F->free_hvy.size() < n * nd || cap < n * nd) {
+        wh::set_error("wh_forest_refine: pool out of memory (or output too small)");
+        return WH_E_NOMEM;
+    }
+    std::unordered_map<int, int> by_hvy;
+    for (size_t i = 0; i < F->blocks.size(); ++i) by_hvy[F->blocks[i].hvy] = (int)i;
+    std::vector<char> remove(F->blocks.size(), 0);
+    std::vector<Block> added;
+    int np = 0;
+    for (int i = 0; i < n; ++i) {
+        auto it = by_hvy.find(hvy_list[i]);
+        if (it == by_hvy.end() || remove[it->second]) {
+            wh::set_error("wh_forest_refine: block not active");
+            return WH_E_ARG;
+        }
+        const Block m = F->blocks[it->second];
+        if (m.level >= F->jmax) {
+            wh::set_error("wh_forest_refine: block already at Jmax");
+            return WH_E_ARG;
+        }
+        remove[it->second] = 1;
+        for (int k = 0; k < nd; ++k) {
+            const int h = F->free_hvy.back();
+            F->free_hvy.pop_back();
+            const int64_t tc = m.tc | ((int64_t)k << F->digit_shift(m.level + 1));
+            added.push_back(Block{tc, m.level + 1, h, 34});  // REF_FRESHLY_REFINED
+            pairs_out[3 * np] = m.hvy;
+            pairs_out[3 * np + 1] = h;
+            pairs_out[3 * np + 2] = k;
+            ++np;
+        }
+    }
+    std::vector<Block> keep;
+    for (size_t i = 0; i < F->blocks.size(); ++i)
+        if (!remove[i]) keep.push_back(F->blocks[i]);
+        else F->free_hvy.push_back(F->blocks[i].hvy);
+    keep.insert(keep.end(), added.begin(), added.end());
+    F->blocks.swap(keep);
+    F->rebuild_map();
+    *npairs = np;
+    return WH_OK;
+}
+
+}  // extern "C"

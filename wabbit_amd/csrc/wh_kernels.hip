@@ -529,34 +529,54 @@ int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *t
     return dispatch_rhs<true>(P, a, tin, nullptr, P->d_active, P->d_x0dx, P->nact, sa);
 }
 
-// ---------------------------------------------------------------------------
-// ghost patches: one workgroup per receiving block (CSR rows of the table)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sync(double *__restrict__ arr,
-                                              const wh_patch *__restrict__ pt,
-                                              const int32_t *__restrict__ off, int row0,
-                                              int stage, size_t blk, size_t cs, int nx, int ny,
-                                              int neq)
+
+// Interpolating prediction (module_wavelets.f90:96-284, `prediction`) at one
+// point of the refined grid.  f[d] is the point's index in the doubled
+// ghost-inclusive index space of the coarse block u (coarse point c sits at
+// fine index 2c).  Even indices copy, odd ones interpolate with the centred
+// stencil of `order` (2: (1,1)/2, 4: (-1,9,9,-1)/16, 6: (3,-25,150,150,-25,3)/256)
+// separably, x innermost, then y, then z, each sum left to right -- the
+// reference's separable passes evaluated at one point.
+template <int N>
+__device__ __forceinline__ double pred_point(const double *__restrict__ u, int nx, int ny,
+                                             int dim, const int f[3])
 {
-    const int row = row0 + blockIdx.x;
-    for (int q = off[row]; q < off[row + 1]; ++q) {
-        const wh_patch P = pt[q];
-        if (stage > 0 && P.stage != stage) continue;
-        const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
-        const int npt = n0 * n1 * n2;
-        const int st = (P.op == WH_PATCH_RESTRICT) ? 2 : 1;
-        double *dst = arr + (size_t)P.recv * blk;
-        const double *src = arr + (size_t)P.send * blk;
-        for (int e = threadIdx.x; e < npt * neq; e += blockDim.x) {
-            const int c = e / npt, r = e - c * npt;
-            const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
-            const size_t ri = ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx +
-                              (P.recv_lo[0] + i);
-            const size_t si = ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
-                              (P.send_lo[0] + st * i);
-            dst[c * cs + ri] = src[c * cs + si];
-        }
-    }
+    constexpr double c2[2] = {0.5, 0.5};
+    constexpr double c4[4] = {-1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0};
+    constexpr double c6[6] = {3.0 / 256.0, -25.0 / 256.0, 150.0 / 256.0,
+                              150.0 / 256.0, -25.0 / 256.0, 3.0 / 256.0};
+    const double *cf = (N == 1) ? c2 : (N == 2) ? c4 : c6;
+    const int cx = f[0] >> 1, px = f[0] & 1;
+    const int cy = f[1] >> 1, py = f[1] & 1;
+    const int cz = dim == 3 ? f[2] >> 1 : 0, pz = dim == 3 ? (f[2] & 1) : 0;
+    auto vx = [&](int jy, int kz) -> double {
+        const double *row = u + ((size_t)kz * ny + jy) * nx;
+        if (!px) return row[cx];
+        double s = cf[0] * row[cx - N + 1];
+#pragma unroll
+        for (int t = 1; t < 2 * N; ++t) s = s + cf[t] * row[cx - N + 1 + t];
+        return s;
+    };
+    auto vy = [&](int kz) -> double {
+        if (!py) return vx(cy, kz);
+        double s = cf[0] * vx(cy - N + 1, kz);
+#pragma unroll
+        for (int t = 1; t < 2 * N; ++t) s = s + cf[t] * vx(cy - N + 1 + t, kz);
+        return s;
+    };
+    if (!pz) return vy(cz);
+    double s = cf[0] * vy(cz - N + 1);
+#pragma unroll
+    for (int t = 1; t < 2 * N; ++t) s = s + cf[t] * vy(cz - N + 1 + t);
+    return s;
+}
+
+__device__ __forceinline__ double pred_any(const double *u, int nx, int ny, int dim, const int f[3],
+                                           int order)
+{
+    if (order == 4) return pred_point<2>(u, nx, ny, dim, f);
+    if (order == 2) return pred_point<1>(u, nx, ny, dim, f);
+    return pred_point<3>(u, nx, ny, dim, f);
 }
 
 // Flat ghost fill: one thread per ghost point of the patches [p0, p1); the
@@ -579,14 +599,91 @@ __global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
     const int r = (int)(e - pstart[lo]);
     const int n0 = P.n[0], n1 = P.n[1];
     const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
-    const int st = (P.op == WH_PATCH_RESTRICT) ? 2 : 1;
     const size_t ri =
         ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx + (P.recv_lo[0] + i);
+    double *dst = arr + (size_t)P.recv * blk + ri;
+    if (P.op == WH_PATCH_PREDICT) {
+        const int f[3] = {P.send_lo[0] + i, P.send_lo[1] + j, P.send_lo[2] + kk};
+        const double *src = arr + (size_t)P.send * blk;
+        const int dim = (cs == (size_t)nx * ny) ? 2 : 3;
+        for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(src + c * cs, nx, ny, dim, f, P.aux);
+        return;
+    }
+    const int st = (P.op == WH_PATCH_RESTRICT) ? 2 : 1;
     const size_t si = ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
                       (P.send_lo[0] + st * i);
-    double *dst = arr + (size_t)P.recv * blk + ri;
     const double *src = arr + (size_t)P.send * blk + si;
     for (int c = 0; c < neq; ++c) dst[c * cs] = src[c * cs];
+}
+
+// Per-receiver ghost fill (the plain one-launch-per-block path): one
+// workgroup walks the patches of one (stage, receiver) row of the table.
+__global__ __launch_bounds__(256) void k_sync(double *__restrict__ arr,
+                                              const wh_patch *__restrict__ pt,
+                                              const int32_t *__restrict__ off, int row0,
+                                              int stage, size_t blk, size_t cs, int nx, int ny,
+                                              int neq)
+{
+    const int row = row0 + blockIdx.x;
+    const int dim = (cs == (size_t)nx * ny) ? 2 : 3;
+    for (int q = off[row]; q < off[row + 1]; ++q) {
+        const wh_patch P = pt[q];
+        if (stage > 0 && P.stage != stage) continue;
+        const int n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
+        const int npt = n0 * n1 * n2;
+        const int st = (P.op == WH_PATCH_RESTRICT) ? 2 : 1;
+        double *dst = arr + (size_t)P.recv * blk;
+        const double *src = arr + (size_t)P.send * blk;
+        for (int e = threadIdx.x; e < npt * neq; e += blockDim.x) {
+            const int c = e / npt, r = e - c * npt;
+            const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
+            const size_t ri = ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx +
+                              (P.recv_lo[0] + i);
+            if (P.op == WH_PATCH_PREDICT) {
+                const int f[3] = {P.send_lo[0] + i, P.send_lo[1] + j, P.send_lo[2] + kk};
+                dst[c * cs + ri] = pred_any(src + c * cs, nx, ny, dim, f, P.aux);
+                continue;
+            }
+            const size_t si = ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
+                              (P.send_lo[0] + st * i);
+            dst[c * cs + ri] = src[c * cs + si];
+        }
+    }
+}
+
+// refineBlock (refinementExecute.f90:4-116): daughter k of a mother takes
+// points g + Bs*off_k + i (i over its ghost-inclusive block) of the mother's
+// interpolated grid; one thread per daughter point.
+__global__ __launch_bounds__(256) void k_refine(double *__restrict__ arr,
+                                                const int32_t *__restrict__ pairs, int npairs,
+                                                int bs0, int bs1, int bs2, int g, int nx, int ny,
+                                                int nz, size_t blk, size_t cs, int neq, int dim,
+                                                int order)
+{
+    const size_t npt = (size_t)nx * ny * nz;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npt * npairs) return;
+    const int q = (int)(e / npt);
+    const int r = (int)(e - (size_t)q * npt);
+    const int i = r % nx, j = (r / nx) % ny, k = r / (nx * ny);
+    const int mother = pairs[3 * q], daughter = pairs[3 * q + 1], dg = pairs[3 * q + 2];
+    const int off[3] = {(dg >> 1) & 1, dg & 1, (dg >> 2) & 1};
+    const int f[3] = {g + bs0 * off[0] + i, g + bs1 * off[1] + j,
+                      dim == 3 ? g + bs2 * off[2] + k : 0};
+    const double *src = arr + (size_t)mother * blk;
+    double *dst = arr + (size_t)daughter * blk + r;
+    for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(src + c * cs, nx, ny, dim, f, order);
+}
+
+int launch_refine(wh_pool *P, double *arr, const int32_t *d_pairs, int npairs, int order)
+{
+    if (npairs <= 0) return WH_OK;
+    const size_t npt = (size_t)P->nx * P->ny * P->nz * npairs;
+    hipLaunchKernelGGL(k_refine, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, P->stream, arr,
+                       d_pairs, npairs, P->d.bs[0], P->d.bs[1], P->d.dim == 3 ? P->d.bs[2] : 1,
+                       P->d.g, P->nx, P->ny, P->nz, P->blk, P->cs, P->d.neq, P->d.dim, order);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
 }
 
 int launch_sync(wh_pool *P, int sync_case, double *arr, int one_receiver)

@@ -51,6 +51,9 @@ def _lib():
         raise OSError(f"libhdf5 not found ({err})")
     lib.H5open.restype = _herr_t
     lib.H5open()
+    # optional datasets/attributes are probed; keep HDF5's error stack quiet
+    lib.H5Eset_auto2.argtypes = [_hid_t, ctypes.c_void_p, ctypes.c_void_p]
+    lib.H5Eset_auto2(0, None, None)
     lib.H5Fopen.restype = _hid_t
     lib.H5Fopen.argtypes = [ctypes.c_char_p, ctypes.c_uint, _hid_t]
     lib.H5Fcreate.restype = _hid_t
@@ -162,17 +165,33 @@ def read_wabbit_file(path: str) -> WabbitFile:
         raise IOError(f"cannot open {path}")
     try:
         blocks = _read_dset(fid, "blocks", np.float64)
-        tc = _read_dset(fid, "block_treecode_num", np.int64)
-        lev = _read_dset(fid, "level", np.int32)
+        dim = blocks.ndim - 1
+        try:
+            dim = int(_read_attr(fid, "blocks", "dim", np.int32)[0])
+        except KeyError:
+            pass
+        try:
+            tc = _read_dset(fid, "block_treecode_num", np.int64)
+            lev = _read_dset(fid, "level", np.int32)
+            ml = int(_read_attr(fid, "blocks", "max_level", np.int32)[0])
+        except KeyError:
+            # files before version 20240410 store the treecode as a digit array
+            # (N, tc_length), -1 past the block's level (InputOutput.f90:491-558,
+            # treecode_size / array2tcb)
+            digits = _read_dset(fid, "block_treecode", np.int32)
+            ml = digits.shape[1]
+            lev = (digits >= 0).sum(axis=1).astype(np.int32)
+            tc = np.zeros(digits.shape[0], np.int64)
+            for l in range(ml):
+                d = digits[:, l].astype(np.int64)
+                tc |= np.where(d >= 0, d, 0) << ((ml - 1 - l) * dim)
         rs = _read_dset(fid, "refinement_status", np.int32)
         org = _read_dset(fid, "coords_origin", np.float64)[:, ::-1].copy()
         spc = _read_dset(fid, "coords_spacing", np.float64)[:, ::-1].copy()
         time = float(_read_attr(fid, "blocks", "time", np.float64)[0])
         it = int(_read_attr(fid, "blocks", "iteration", np.int32)[0])
-        dim = int(_read_attr(fid, "blocks", "dim", np.int32)[0])
         dom = _read_attr(fid, "blocks", "domain-size", np.float64)
         bs = _read_attr(fid, "blocks", "block-size", np.int32)
-        ml = int(_read_attr(fid, "blocks", "max_level", np.int32)[0])
         try:
             per = _read_attr(fid, "blocks", "periodic_BC", np.int32)
         except KeyError:
