@@ -190,3 +190,150 @@ class EquidistantRun:
                                      self.nb, _i(self.block_at), self.nblocks, _d(self.dxs),
                                      _d(self.hvy), _d(self.work), _d(self.mask), _d(self.butcher),
                                      self.butcher.shape[0], dt)
+
+
+# ---- wavelets and adapt_tree (wavelet.c, tree.c) ----------------------------
+_WMAX = 16
+
+
+class OrWavelet(ctypes.Structure):
+    _fields_ = [("X", ctypes.c_int), ("Y", ctypes.c_int), ("lifted", ctypes.c_int),
+                ("hd_lo", ctypes.c_int), ("hd_hi", ctypes.c_int), ("gd_lo", ctypes.c_int),
+                ("gd_hi", ctypes.c_int), ("hr_lo", ctypes.c_int), ("hr_hi", ctypes.c_int),
+                ("gr_lo", ctypes.c_int), ("gr_hi", ctypes.c_int),
+                ("HD", ctypes.c_double * (2 * _WMAX + 1)), ("GD", ctypes.c_double * (2 * _WMAX + 1)),
+                ("HR", ctypes.c_double * (2 * _WMAX + 1)), ("GR", ctypes.c_double * (2 * _WMAX + 1)),
+                ("g", ctypes.c_int), ("g_rhs", ctypes.c_int), ("order", ctypes.c_int),
+                ("Nscl", ctypes.c_int), ("Nscr", ctypes.c_int), ("Nwcl", ctypes.c_int),
+                ("Nwcr", ctypes.c_int), ("Nreconl", ctypes.c_int), ("Nreconr", ctypes.c_int),
+                ("Ndep2l", ctypes.c_int), ("Ndep2r", ctypes.c_int), ("specialised", ctypes.c_int)]
+
+    def filt(self, name):
+        lo, hi = getattr(self, name.lower() + "_lo"), getattr(self, name.lower() + "_hi")
+        return lo, np.array([getattr(self, name)[i] for i in range(hi - lo + 1)])
+
+
+class OrAdaptParams(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("jmin", ctypes.c_int), ("jmax", ctypes.c_int),
+                ("g", ctypes.c_int), ("neq", ctypes.c_int), ("bs", ctypes.c_int * 3),
+                ("indicator", ctypes.c_int), ("use_ce", ctypes.c_int),
+                ("force_maxlevel_dealiasing", ctypes.c_int), ("eps", ctypes.c_double),
+                ("norm", ctypes.c_double * 8), ("groups", ctypes.c_int * 8)]
+
+
+def _wlib():
+    L = lib()
+    if not getattr(L, "_wavelet_bound", False):
+        vp = ctypes.c_void_p
+        L.or_wavelet_setup.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(OrWavelet)]
+        L.or_wavelet_setup.restype = ctypes.c_int
+        for fn in ("or_fwt_block", "or_iwt_block", "or_filter_block"):
+            getattr(L, fn).argtypes = [ctypes.POINTER(OrWavelet), ctypes.c_int, _ip, ctypes.c_int,
+                                       ctypes.c_int, _dp, _dp]
+        L.or_detail_block.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
+        L.or_tree_create.argtypes = [ctypes.POINTER(OrWavelet), ctypes.POINTER(OrAdaptParams), ctypes.c_int,
+                                     vp, _ip, _dp]
+        L.or_tree_create.restype = vp
+        L.or_tree_destroy.argtypes = [vp]
+        L.or_tree_size.argtypes = [vp]
+        L.or_tree_get.argtypes = [vp, vp, _ip, _dp]
+        for fn in ("or_tree_sync_full", "or_tree_refine_everywhere", "or_tree_adapt"):
+            getattr(L, fn).argtypes = [vp]
+            getattr(L, fn).restype = ctypes.c_int
+        L._wavelet_bound = True
+    return L
+
+
+def wavelet(name, fd_max_size=0) -> OrWavelet:
+    w = OrWavelet()
+    if _wlib().or_wavelet_setup(name.encode(), fd_max_size, ctypes.byref(w)):
+        raise ValueError(f"unsupported wavelet {name}")
+    return w
+
+
+def _blk(dim, bs):
+    return np.asarray(list(bs) + [1] * (3 - len(bs)), np.int32)
+
+
+def fwt_block(w, dim, bs, g, src, dst=None):
+    src = np.ascontiguousarray(src, np.float64)
+    dst = src.copy() if dst is None else np.ascontiguousarray(dst, np.float64).copy()
+    _wlib().or_fwt_block(ctypes.byref(w), dim, _i(_blk(dim, bs)), g, src.shape[0], _d(src), _d(dst))
+    return dst
+
+
+def iwt_block(w, dim, bs, g, src, dst=None):
+    src = np.ascontiguousarray(src, np.float64)
+    dst = src.copy() if dst is None else np.ascontiguousarray(dst, np.float64).copy()
+    _wlib().or_iwt_block(ctypes.byref(w), dim, _i(_blk(dim, bs)), g, src.shape[0], _d(src), _d(dst))
+    return dst
+
+
+def filter_block(w, dim, bs, g, src):
+    src = np.ascontiguousarray(src, np.float64)
+    dst = np.empty_like(src)
+    _wlib().or_filter_block(ctypes.byref(w), dim, _i(_blk(dim, bs)), g, src.shape[0], _d(src), _d(dst))
+    return dst
+
+
+def detail_block(dim, bs, g, u, groups):
+    u = np.ascontiguousarray(u, np.float64)
+    gr = np.ascontiguousarray(groups, np.int32)
+    out = np.zeros(u.shape[0])
+    _wlib().or_detail_block(dim, _i(_blk(dim, bs)), g, u.shape[0], _i(gr), _d(u), _d(out))
+    return out
+
+
+class Tree:
+    """A periodic forest owned by the oracle (adapt_tree, syncs, refinement)."""
+
+    def __init__(self, w, dim, jmin, jmax, bs, g, treecodes, levels, hvy, indicator=0, use_ce=None,
+                 eps=1e-3, norm=None, groups=None, force_maxlevel_dealiasing=False):
+        p = OrAdaptParams()
+        p.dim, p.jmin, p.jmax, p.g = dim, jmin, jmax, g
+        hvy = np.ascontiguousarray(hvy, np.float64)
+        p.neq = hvy.shape[1]
+        for i, b in enumerate(_blk(dim, bs)):
+            p.bs[i] = int(b)
+        p.indicator = indicator
+        p.use_ce = int(w.lifted if use_ce is None else use_ce)
+        p.force_maxlevel_dealiasing = int(force_maxlevel_dealiasing)
+        p.eps = eps
+        for c in range(p.neq):
+            p.norm[c] = 1.0 if norm is None else norm[c]
+            p.groups[c] = 1 if groups is None else groups[c]
+        self.w, self.p = w, p
+        self.dim, self.g, self.bs = dim, g, list(_blk(dim, bs))
+        self.shape = hvy.shape[1:]
+        tc = np.ascontiguousarray(treecodes, np.int64)
+        lv = np.ascontiguousarray(levels, np.int32)
+        self.h = _wlib().or_tree_create(ctypes.byref(w), ctypes.byref(p), tc.shape[0], tc.ctypes.data,
+                                        _i(lv), _d(hvy))
+
+    def close(self):
+        if self.h:
+            _wlib().or_tree_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _ck(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    def sync(self):
+        self._ck(_wlib().or_tree_sync_full(self.h), "or_tree_sync_full")
+
+    def refine_everywhere(self):
+        self._ck(_wlib().or_tree_refine_everywhere(self.h), "or_tree_refine_everywhere")
+
+    def adapt(self):
+        self._ck(_wlib().or_tree_adapt(self.h), "or_tree_adapt")
+
+    def get(self):
+        n = _wlib().or_tree_size(self.h)
+        tc = np.zeros(n, np.int64)
+        lv = np.zeros(n, np.int32)
+        hv = np.zeros((n,) + tuple(self.shape))
+        _wlib().or_tree_get(self.h, tc.ctypes.data, _i(lv), _d(hv))
+        return tc, lv, hv

@@ -1,0 +1,107 @@
+/*
+ * wavelet_oracle.h -- CPU restatement of WABBIT's CDF wavelet machinery and
+ * of adapt_tree (coarsening on the full tree).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load liboracle.so.
+ *
+ * Block layout as in wabbit_oracle.h: u[c][iz][iy][ix], x fastest,
+ * nx = Bs+2g (nz = 1 in 2D).  Index conventions below are 0-based; the
+ * reference's 1-based index i is i-1 here.
+ *
+ * Pinning: tests/test_oracle_wavelet.py
+ *   or_wavelet_setup     closed-form tables (CDF40/CDF44 HD/GD/HR/GR and CE
+ *                        widths restated from the formulas), sum rules
+ *                        (sum HD = 1, sum HR = 2, vanishing moments of GD).
+ *   or_fwt/or_iwt_block  perfect reconstruction IWT(FWT(u)) = u on the
+ *                        interior for every wavelet; SC of polynomials.
+ *   or_adapt_tree        the reference's --coarsen-everywhere outputs
+ *                        (TESTING/wavelets/adaptive_CDF{20,22,40,42,44,60,62}/
+ *                        vor_00200.h5) bit for bit.
+ */
+#ifndef WAVELET_ORACLE_H
+#define WAVELET_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_WMAX 16 /* filter support bound: taps -OR_WMAX..OR_WMAX */
+
+/* setup_wavelet (module_wavelets.f90:1031-1430) for "CDFXY", X in {2,4,6},
+ * Y in {0,2,4,6}, Y <= X.  Filters are stored on [lo, hi] with coef[i - lo]. */
+typedef struct {
+    int X, Y, lifted;
+    int hd_lo, hd_hi, gd_lo, gd_hi, hr_lo, hr_hi, gr_lo, gr_hi;
+    double HD[2 * OR_WMAX + 1], GD[2 * OR_WMAX + 1], HR[2 * OR_WMAX + 1], GR[2 * OR_WMAX + 1];
+    int g;           /* g_min: largest filter half-width */
+    int g_rhs;       /* max(X/2, FD_max_size) */
+    int order;       /* predictor order (multiresolution_<X>th) */
+    int Nscl, Nscr, Nwcl, Nwcr, Nreconl, Nreconr, Ndep2l, Ndep2r;
+    int specialised; /* 1: CDF20/22/40/42/44/60/62 unrolled forms, 0: generic */
+} or_wavelet;
+
+/* returns 0, or -1 for an unsupported name. fd_max_size: reach of the FD
+ * scheme (0 when no discretization is set, as in wabbit-post). */
+int or_wavelet_setup(const char *name, int fd_max_size, or_wavelet *w);
+
+/* waveletDecomposition_optimized_block (wavelet_decomposition_reconstruction.f90:23-389):
+ * src ghost-inclusive block -> dst spaghetti form.  Only the region the
+ * reference computes is written; the caller gives dst its prior content. */
+void or_fwt_block(const or_wavelet *w, int dim, const int bs[3], int g, int neq,
+                  const double *src, double *dst);
+
+/* waveletReconstruction_optimized_block (:426-841), same conventions. */
+void or_iwt_block(const or_wavelet *w, int dim, const int bs[3], int g, int neq,
+                  const double *src, double *dst);
+
+/* blockFilterXYZ_vct with do_restriction (module_wavelets.f90:307-401) with
+ * the filter HD: dst = src, then SC interior points replaced by the
+ * separable filter. */
+void or_filter_block(const or_wavelet *w, int dim, const int bs[3], int g, int neq,
+                     const double *src, double *dst);
+
+/* threshold_block's detail for eps_norm = Linfty (threshold_block.f90:1-130,
+ * wavelet_renorm_block module_wavelets.f90:1848-1960): per component the
+ * max |WC| over the interior (SCs zeroed); components with group id l >= 2
+ * take max over the group of sqrt(wc^2); group id 0 gives 0. */
+void or_detail_block(int dim, const int bs[3], int g, int neq, const int *groups,
+                     const double *u, double *detail);
+
+/* ---- adapt_tree on a periodic leaf forest ----------------------------- */
+enum { OR_IND_EVERYWHERE = 0, OR_IND_THRESHOLD = 1 };
+
+typedef struct {
+    int dim, jmin, jmax, g, neq;
+    int bs[3];
+    int indicator;                  /* OR_IND_* */
+    int use_ce;                     /* params%useCoarseExtension */
+    int force_maxlevel_dealiasing;
+    double eps;
+    double norm[8];                 /* eps * norm per component */
+    int groups[8];                  /* threshold_state_vector_component */
+} or_adapt_params;
+
+/* A forest with heavy data that the oracle owns.  n blocks in list order. */
+typedef struct or_tree or_tree;
+
+or_tree *or_tree_create(const or_wavelet *w, const or_adapt_params *p, int n, const int64_t *tc,
+                        const int *level, const double *hvy);
+void or_tree_destroy(or_tree *T);
+int or_tree_size(const or_tree *T);
+/* copies out the leaf list (sorted by level, then treecode) and data */
+void or_tree_get(const or_tree *T, int64_t *tc, int *level, double *hvy);
+/* sync_ghosts_tree (full sync of all ghosts, filtered restriction for lifted
+ * wavelets) */
+int or_tree_sync_full(or_tree *T);
+/* refine_tree "everywhere" execution (refineBlock of every leaf below jmax) */
+int or_tree_refine_everywhere(or_tree *T);
+/* adapt_tree (adapt_tree.f90:11-255) without the final load balancing */
+int or_tree_adapt(or_tree *T);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
