@@ -207,6 +207,83 @@ int wh_forest_patches(const wh_forest *forest, int gminus, int gplus, int order,
 int wh_forest_refine(wh_forest *forest, int n, const int32_t *hvy_ids, int32_t *triples, int cap,
                      int *ntriples);
 
+/* ---- CDF wavelets (LIB/WAVELETS/module_wavelets.f90) ---------------------
+ * Filters on [lo, hi], the coefficient of tap i at [i - lo]. */
+#define WH_WAVELET_TAPS 33
+typedef struct {
+    int X, Y, lifted, specialised;
+    int hd_lo, hd_hi, gd_lo, gd_hi, hr_lo, hr_hi, gr_lo, gr_hi;
+    double HD[WH_WAVELET_TAPS], GD[WH_WAVELET_TAPS], HR[WH_WAVELET_TAPS], GR[WH_WAVELET_TAPS];
+    int g;       /* ghost nodes the filters need (g_min) */
+    int g_rhs;   /* max(X/2, FD reach) */
+    int order;   /* predictor order (multiresolution_<X>th) */
+    int Nscl, Nscr, Nwcl, Nwcr, Nreconl, Nreconr, Ndep2l, Ndep2r;  /* coarse extension */
+} wh_wavelet;
+
+/* setup_wavelet (module_wavelets.f90:1031-1430) for "CDFXY" with X in
+ * {2,4,6}, Y in {0,2,4,6}, Y <= X; fd_max_size is the reach of the FD scheme
+ * (0 when none is set).  Host only. */
+int wh_wavelet_setup(const char *name, int fd_max_size, wh_wavelet *out);
+/* The wavelet the pool's transform kernels use. */
+int wh_pool_set_wavelet(wh_pool *pool, const wh_wavelet *w);
+
+/* Per-block transforms of the listed hvy ids, src and dst distinct arrays;
+ * only the region the reference computes is written (dst keeps the rest):
+ * waveletDecomposition_optimized_block (wavelet_decomposition_reconstruction.f90:23-389),
+ * waveletReconstruction_optimized_block (:426-841),
+ * blockFilterXYZ_vct with HD and do_restriction (module_wavelets.f90:307-401).
+ * Work slot 0 is used as scratch. */
+int wh_fwt_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *hvy_ids, int n);
+int wh_iwt_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *hvy_ids, int n);
+int wh_filter_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *hvy_ids, int n);
+/* whole-block copies (hvy_tmp = hvy_block and back) */
+int wh_copy_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *hvy_ids, int n);
+/* threshold_block's detail with eps_norm = Linfty (threshold_block.f90:1-130):
+ * detail[k*neq + c] = max over the interior of |WC| (SCs zeroed); components
+ * with groups[c] = l >= 2 take the max over the group, groups[c] = 0 gives 0. */
+int wh_detail_blocks(wh_pool *pool, int array, const int32_t *hvy_ids, int n,
+                     const int32_t *groups, double *detail);
+
+/* Coarse-extension boxes (coarseExtensionManipulateWC/SC_block,
+ * module_wavelets.f90:877-1027): inclusive ghost-inclusive index box of block
+ * hvy; WH_BOX_ZERO_WC zeroes the points that are not SC in every direction,
+ * WH_BOX_COPY_SC copies the SC points from src_array. */
+enum { WH_BOX_ZERO_WC = 0, WH_BOX_COPY_SC = 1 };
+typedef struct {
+    int32_t hvy;
+    int32_t lo[3], hi[3];
+    int32_t op;
+} wh_box;
+int wh_apply_boxes(wh_pool *pool, int array, int src_array, const wh_box *boxes, int n);
+
+/* Run an explicit patch list on `array` (stages in order).  Bits 8-9 of a
+ * patch's op select where its sender is read: 0 `array`, 1 `alt1`, 2 `alt2`
+ * (the source-array choice of xfer_block_data.f90:321-437). */
+int wh_run_patches(wh_pool *pool, int array, int alt1, int alt2, const wh_patch *patches, int n);
+
+/* adapt_tree (LIB/MESH/adapt_tree.f90:11-255) on the forest's leaf blocks in
+ * WH_BLOCK: full tree down to jmin, level-wise decomposition, coarsening
+ * indicator, gradedness, coarse extension, reconstruction, pruning and the
+ * closing sync_ghosts_tree.  The forest's block list is replaced by the new
+ * leaves.  WH_TMP must hold >= neq components; work slots 0 and 1 are used
+ * as scratch.  The pool needs free slots for the mother blocks. */
+enum { WH_IND_EVERYWHERE = 0, WH_IND_THRESHOLD = 1 };
+typedef struct {
+    int jmin, jmax;
+    int indicator;                  /* WH_IND_* */
+    int use_ce;                     /* useCoarseExtension */
+    int force_maxlevel_dealiasing;
+    double eps;
+    double norm[8];                 /* per component: threshold eps * norm */
+    int32_t groups[8];              /* threshold_state_vector_component */
+} wh_adapt_params;
+int wh_adapt_tree(wh_pool *pool, wh_forest *forest, const wh_adapt_params *p);
+
+/* sync_ghosts_tree (synchronize_ghosts_generic.f90:121-144) on a leaf forest
+ * with all ghosts, restriction filtered with HD for lifted wavelets
+ * (restrict_copy_at_CE, restrict_predict_data.f90:101-170). */
+int wh_sync_tree(wh_pool *pool, wh_forest *forest, int array);
+
 /* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
  * stage kernel (each owns Bs2/ysplit rows of (x,y) columns and marches in z);
  * 0 selects the flat one-point-per-thread kernel, -1 automatic. */

@@ -847,8 +847,8 @@ static void ensure_gradedness(or_tree *T, int stay)
     }
 }
 
-/* wavelet_reconstruct_full_tree_CEoptimized, leaf path (adapt_tree.f90:686-790) */
-static int reconstruct_leaf(or_tree *T)
+/* wavelet_reconstruct_full_tree_CEoptimized (adapt_tree.f90:686-987) */
+static int reconstruct_tree(or_tree *T)
 {
     const or_wavelet *w = &T->w;
     int dirs[26][3];
@@ -893,11 +893,33 @@ static int reconstruct_leaf(or_tree *T)
     S.src_mode = SRC_DECOMPOSED;
     S.filter = 1;
     S.gm = S.gp = T->g;
-    const int rc = tree_sync(T, &S);
-    if (T->p.use_ce) ce_modify(T, CE_TREE, 0, 0, 1, 0);
-    for (int b = 0; b < T->n; ++b) {
-        if (T->status[b] == -1) or_iwt_block(w, T->dim, T->p.bs, T->g, T->neq, T->u[b], T->t[b]);
-        memcpy(T->u[b], T->t[b], sizeof(double) * T->bsz);
+    int rc = 0;
+    int leaf_only = 1;
+    for (int d = 0; d < T->dim; ++d)
+        if (T->p.bs[d] < w->Ndep2l || T->p.bs[d] < w->Ndep2r) leaf_only = 0;
+    if (leaf_only) {
+        rc = tree_sync(T, &S);
+        if (T->p.use_ce) ce_modify(T, CE_TREE, 0, 0, 1, 0);
+        for (int b = 0; b < T->n; ++b) {
+            if (T->status[b] == -1) or_iwt_block(w, T->dim, T->p.bs, T->g, T->neq, T->u[b], T->t[b]);
+            memcpy(T->u[b], T->t[b], sizeof(double) * T->bsz);
+        }
+    } else {
+        /* level by level from Jmin (adapt_tree.f90:880-980); sync_M2D only
+         * when reconstruct_neighbors, which the supported sizes exclude */
+        if (reconstruct_neighbors) rc = -11;
+        const int lmax = max_level(T);
+        for (int L = T->p.jmin; L <= lmax && !rc; ++L) {
+            S.recv_level_on = 1;
+            S.recv_level = L;
+            rc = tree_sync(T, &S);
+            if (T->p.use_ce) ce_modify(T, CE_LEVEL, L, 0, 1, 0);
+            for (int b = 0; b < T->n; ++b) {
+                if (T->level[b] != L) continue;
+                if (T->status[b] == -1) or_iwt_block(w, T->dim, T->p.bs, T->g, T->neq, T->u[b], T->t[b]);
+                memcpy(T->u[b], T->t[b], sizeof(double) * T->bsz);
+            }
+        }
     }
     for (int b = 0; b < T->n; ++b) T->status[b] = backup[b];
     free(backup);
@@ -945,11 +967,7 @@ int or_tree_adapt(or_tree *T)
     if (!T->p.use_ce) {
         for (int b = 0; b < T->n; ++b) memcpy(T->u[b], T->t[b], sizeof(double) * T->bsz);
     } else {
-        int leaf_only = 1;
-        for (int d = 0; d < T->dim; ++d)
-            if (T->p.bs[d] < w->Ndep2l || T->p.bs[d] < w->Ndep2r) leaf_only = 0;
-        if (!leaf_only) return -10; /* level-wise reconstruction not restated */
-        rc = reconstruct_leaf(T);
+        rc = reconstruct_tree(T);
         if (rc) return rc;
     }
     /* prune_fulltree2leafs */

@@ -1,0 +1,816 @@
+// wh_adapt.cpp -- adapt_tree (LIB/MESH/adapt_tree.f90:11-255) and
+// sync_ghosts_tree with filtered restriction, orchestrated on the host over
+// the pool's device operations (wavelet transforms, coarse-extension boxes,
+// ghost-patch lists).  Host code only.
+//
+// The full tree (leaves plus their mothers down to Jmin) lives in a Tree
+// built from the forest's leaf list; mothers take free pool slots.  Ghost
+// fills are generated per receiver and direction from the block covering
+// the ghost box: the same-level block (COPY), else the finer blocks touching
+// the box (RESTRICT), else the coarser leaf (PREDICT), gated by the sender
+// and receiver rules of each sync case (prepare_ghost_synch_metadata,
+// synchronize_ghosts_generic.f90:352-694) and reading the source array the
+// reference's xfer_block_data.f90:321-437 picks (hvy_block or hvy_tmp).
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <cstdlib>
+#include <unordered_map>
+#include <vector>
+
+#include "wh_forest_impl.h"
+#include "wh_internal.h"
+
+namespace wh {
+namespace {
+
+constexpr int REF_TMP_EMPTY = 10;          // module_globals.f90:25
+constexpr int REF_UNSIGNIFICANT_STAY = 9;  // module_globals.f90:28
+constexpr int A_U = WH_BLOCK, A_T = WH_TMP, A_SCR = WH_WORK + 0, A_FILT = WH_WORK + 1;
+
+struct TNode {
+    int64_t tc;
+    int level;
+    int hvy;
+    int status;
+};
+
+enum SrcMode { SRC_U = 0, SRC_NOT_FLAG = 1, SRC_DECOMPOSED = 2 };
+struct SyncSpec {
+    bool same = true, restrict_ = true, predict = true;
+    bool recv_ref_on = false, recv_level_on = false;
+    int recv_ref = 0, recv_level = 0;
+    SrcMode src = SRC_U;
+    int src_ref = -1;
+    bool filter = true;
+    int gm = 0, gp = 0;
+};
+
+struct Tree {
+    wh_forest *F;
+    wh_pool *P;
+    wh_wavelet w;
+    int dim, g, bs[3], jmax;
+    std::vector<TNode> b;
+    std::unordered_map<uint64_t, int> map;
+
+    Tree(wh_forest *f, wh_pool *p) : F(f), P(p), w(p->ws.w), dim(f->dim), g(f->g), jmax(f->jmax)
+    {
+        for (int d = 0; d < 3; ++d) bs[d] = f->bs[d];
+        for (const FBlock &x : f->blocks) b.push_back(TNode{x.tc, x.level, x.hvy, 0});
+        rehash();
+    }
+    void rehash()
+    {
+        map.clear();
+        for (size_t i = 0; i < b.size(); ++i) map[fkey(b[i].level, b[i].tc)] = (int)i;
+    }
+    void ijk(int i, int out[3]) const { F->ijk_of(b[i].tc, b[i].level, out); }
+    int find(int level, const int q[3]) const
+    {
+        if (level < 0) return -1;
+        const int m = 1 << level;
+        int wq[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+            if (!F->periodic[d] && (q[d] < 0 || q[d] >= m)) return -1;
+            wq[d] = ((q[d] % m) + m) % m;
+        }
+        auto it = map.find(fkey(level, F->tc_of(wq, level)));
+        return it == map.end() ? -1 : it->second;
+    }
+    bool valid(int i) const { return i >= 0 && b[i].status != REF_TMP_EMPTY; }
+    int mother(int i) const
+    {
+        if (b[i].level == 0) return -1;
+        int q[3];
+        ijk(i, q);
+        const int m[3] = {q[0] >> 1, q[1] >> 1, q[2] >> 1};
+        return find(b[i].level - 1, m);
+    }
+    int daughter(int i, int k) const
+    {
+        int q[3];
+        ijk(i, q);
+        const int d[3] = {2 * q[0] + ((k >> 1) & 1), 2 * q[1] + (k & 1), dim == 3 ? 2 * q[2] + ((k >> 2) & 1) : 0};
+        return find(b[i].level + 1, d);
+    }
+    // block_is_leaf (block_relations.f90); check_empty: REF_TMP_EMPTY daughters don't count
+    bool is_leaf(int i, bool check_empty) const
+    {
+        for (int k = 0; k < (1 << dim); ++k) {
+            const int d = daughter(i, k);
+            if (d >= 0 && (!check_empty || b[d].status != REF_TMP_EMPTY)) return false;
+        }
+        return true;
+    }
+    int nb_same(int i, const int o[3]) const
+    {
+        int q[3];
+        ijk(i, q);
+        const int n[3] = {q[0] + o[0], q[1] + o[1], q[2] + o[2]};
+        return find(b[i].level, n);
+    }
+    // the finer blocks adjacent to block i in direction o (children of the
+    // virtual same-level neighbour); child cc per dimension in out
+    void finer_children(int i, const int o[3], std::vector<std::pair<int, std::array<int, 3>>> &out) const
+    {
+        out.clear();
+        int q[3];
+        ijk(i, q);
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            if (d >= dim) lo[d] = hi[d] = 0;
+            else if (o[d] < 0) lo[d] = hi[d] = 1;
+            else if (o[d] > 0) lo[d] = hi[d] = 0;
+            else {
+                lo[d] = 0;
+                hi[d] = 1;
+            }
+        }
+        for (int cz = lo[2]; cz <= hi[2]; ++cz)
+            for (int cy = lo[1]; cy <= hi[1]; ++cy)
+                for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+                    const int c[3] = {2 * (q[0] + o[0]) + cx, 2 * (q[1] + o[1]) + cy,
+                                      dim == 3 ? 2 * (q[2] + o[2]) + cz : 0};
+                    out.push_back({find(b[i].level + 1, c), {cx, cy, cz}});
+                }
+    }
+    bool any_finer(int i, const int o[3], bool need_valid) const
+    {
+        std::vector<std::pair<int, std::array<int, 3>>> ch;
+        finer_children(i, o, ch);
+        for (auto &c : ch)
+            if (c.first >= 0 && (!need_valid || valid(c.first))) return true;
+        return false;
+    }
+    // coarse relation in direction o (find_neighbors.f90): exists only on the
+    // mother's boundary in that direction
+    int nb_coarse(int i, const int o[3]) const
+    {
+        if (b[i].level == 0) return -1;
+        int q[3];
+        ijk(i, q);
+        int c[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+            if (o[d] > 0 && (q[d] & 1) != 1) return -1;
+            if (o[d] < 0 && (q[d] & 1) != 0) return -1;
+            const int v = q[d] + o[d];
+            c[d] = v >= 0 ? v / 2 : -((-v + 1) / 2);
+        }
+        return find(b[i].level - 1, c);
+    }
+    int add(int64_t tc, int level, int status)
+    {
+        WH_CHECK(!F->free_hvy.empty(), -1, "adapt_tree: pool out of slots for the full tree");
+        const int h = F->free_hvy.back();
+        F->free_hvy.pop_back();
+        b.push_back(TNode{tc, level, h, status});
+        map[fkey(level, tc)] = (int)b.size() - 1;
+        return (int)b.size() - 1;
+    }
+    void remove_marked(const std::vector<char> &del)
+    {
+        std::vector<TNode> keep;
+        for (size_t i = 0; i < b.size(); ++i) {
+            if (del[i]) F->free_hvy.push_back(b[i].hvy);
+            else keep.push_back(b[i]);
+        }
+        b.swap(keep);
+        rehash();
+    }
+    int max_level() const
+    {
+        int m = -1;
+        for (const TNode &x : b) m = std::max(m, x.level);
+        return m;
+    }
+};
+
+int dirs_of(int dim, int dirs[26][3])
+{
+    int nd = 0;
+    for (int oz = (dim == 3 ? -1 : 0); oz <= (dim == 3 ? 1 : 0); ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                if (!ox && !oy && !oz) continue;
+                dirs[nd][0] = ox;
+                dirs[nd][1] = oy;
+                dirs[nd][2] = oz;
+                ++nd;
+            }
+    return nd;
+}
+
+// ---- coarse extension boxes ------------------------------------------------
+// get_indices_of_modify_patch (neighborhood.f90) with lvl_diff = +1: the
+// first Nl / last Nr interior points toward direction o, full interior across
+void modify_box(const Tree &T, const int o[3], int Nl, int Nr, wh_box &bx)
+{
+    for (int d = 0; d < 3; ++d) {
+        if (d >= T.dim) {
+            bx.lo[d] = bx.hi[d] = 0;
+            continue;
+        }
+        bx.lo[d] = T.g;
+        bx.hi[d] = T.g + T.bs[d] - 1;
+        if (o[d] < 0) bx.hi[d] = T.g + Nl - 1;
+        if (o[d] > 0) bx.lo[d] = T.g + T.bs[d] - Nr;
+    }
+}
+// get_indices_of_ghost_patch with lvl_diff = +1 for the coarse relation of
+// block i in direction o: the ghost slab, extended across each free direction
+// toward the sister side of the block
+void ce_ghost_box(const Tree &T, int i, const int o[3], wh_box &bx)
+{
+    int q[3];
+    T.ijk(i, q);
+    for (int d = 0; d < 3; ++d) {
+        if (d >= T.dim) {
+            bx.lo[d] = bx.hi[d] = 0;
+            continue;
+        }
+        const int B = T.bs[d], g = T.g;
+        if (o[d] < 0) {
+            bx.lo[d] = 0;
+            bx.hi[d] = g - 1;
+        } else if (o[d] > 0) {
+            bx.lo[d] = g + B;
+            bx.hi[d] = B + 2 * g - 1;
+        } else if (q[d] & 1) {
+            bx.lo[d] = 0;
+            bx.hi[d] = g + B - 1;
+        } else {
+            bx.lo[d] = g;
+            bx.hi[d] = B + 2 * g - 1;
+        }
+    }
+}
+
+enum CeCase { CE_TREE = 0, CE_LEVEL = 1, CE_REF = 2 };
+// coarse_extension_modify (reconstruction_step.f90:3-100) on WH_BLOCK
+int ce_modify(Tree &T, CeCase ce, int s_level, int s_ref, bool clear_wc, bool copy_sc)
+{
+    int dirs[26][3];
+    const int nd = dirs_of(T.dim, dirs);
+    std::vector<wh_box> boxes;
+    for (int i = 0; i < (int)T.b.size(); ++i) {
+        if (ce == CE_LEVEL && T.b[i].level != s_level) continue;
+        if (ce == CE_REF && T.b[i].status != s_ref) continue;
+        if (!T.is_leaf(i, false)) continue;
+        for (int q = 0; q < nd; ++q) {
+            const int *o = dirs[q];
+            if (T.nb_coarse(i, o) < 0) continue;
+            if (T.valid(T.nb_same(i, o)) || T.any_finer(i, o, true)) continue;
+            wh_box bx{};
+            bx.hvy = T.b[i].hvy;
+            if (clear_wc) {
+                bx.op = WH_BOX_ZERO_WC;
+                modify_box(T, o, T.w.Nwcl, T.w.Nwcr, bx);
+                boxes.push_back(bx);
+                ce_ghost_box(T, i, o, bx);
+                boxes.push_back(bx);
+            }
+            if (copy_sc) {
+                bx.op = WH_BOX_COPY_SC;
+                modify_box(T, o, T.w.Nscl, T.w.Nscr, bx);
+                boxes.push_back(bx);
+            }
+        }
+    }
+    return run_boxes(T.P, A_U, A_T, boxes);
+}
+
+// ---- ghost syncs -------------------------------------------------------------
+int src_sel(const Tree &T, const SyncSpec &S, int sender, bool res_pred)
+{
+    if (S.src == SRC_NOT_FLAG && T.b[sender].status != S.src_ref) return 1;
+    if (S.src == SRC_DECOMPOSED && res_pred) return 1;
+    return 0;
+}
+
+int tree_sync(Tree &T, const SyncSpec &S)
+{
+    const int dim = T.dim, g = T.g, order = T.w.order;
+    const bool filt = S.filter && T.w.lifted;
+    int dirs[26][3];
+    const int nd = dirs_of(dim, dirs);
+    const int nxyz[3] = {T.P->nx, T.P->ny, T.P->nz};
+    std::vector<wh_patch> st1, st23;
+    std::vector<int> restrict_senders;  // tree indices
+    std::vector<std::pair<int, std::array<int, 3>>> ch;
+    for (int r = 0; r < (int)T.b.size(); ++r) {
+        const TNode &R = T.b[r];
+        if (R.status == REF_TMP_EMPTY) continue;
+        if (S.recv_ref_on && R.status != S.recv_ref) continue;
+        if (S.recv_level_on && R.level != S.recv_level) continue;
+        int q[3];
+        T.ijk(r, q);
+        for (int k = 0; k < nd; ++k) {
+            const int *o = dirs[k];
+            int rlo[3], rn[3];
+            for (int d = 0; d < 3; ++d) {
+                if (d >= dim) {
+                    rlo[d] = 0;
+                    rn[d] = 1;
+                } else if (o[d] < 0) {
+                    rlo[d] = g - S.gm;
+                    rn[d] = S.gm;
+                } else if (o[d] > 0) {
+                    rlo[d] = g + T.bs[d];
+                    rn[d] = S.gp;
+                } else {
+                    rlo[d] = g;
+                    rn[d] = T.bs[d];
+                }
+            }
+            if (rn[0] <= 0 || rn[1] <= 0 || rn[2] <= 0) continue;
+            // 1) same level
+            const int ns = T.nb_same(r, o);
+            if (T.valid(ns)) {
+                if (!S.same) continue;
+                wh_patch p{};
+                p.recv = R.hvy;
+                p.send = T.b[ns].hvy;
+                for (int d = 0; d < 3; ++d) {
+                    p.recv_lo[d] = rlo[d];
+                    p.n[d] = rn[d];
+                    p.send_lo[d] = d < dim ? rlo[d] - o[d] * T.bs[d] : 0;
+                }
+                p.op = WH_PATCH_COPY | (src_sel(T, S, ns, false) << 8);
+                p.stage = 1;
+                st1.push_back(p);
+                continue;
+            }
+            // 2) finer blocks touching the box
+            T.finer_children(r, o, ch);
+            bool finer = !ch.empty();
+            for (auto &c : ch) finer = finer && T.valid(c.first);
+            if (finer) {
+                if (!S.restrict_) continue;
+                for (auto &c : ch) {
+                    const int f = c.first;
+                    wh_patch p{};
+                    p.recv = R.hvy;
+                    p.send = T.b[f].hvy;
+                    for (int d = 0; d < 3; ++d) {
+                        if (d >= dim) {
+                            p.recv_lo[d] = 0;
+                            p.n[d] = 1;
+                            p.send_lo[d] = 0;
+                            continue;
+                        }
+                        int lo = rlo[d], n = rn[d];
+                        if (o[d] == 0) {
+                            const int h = T.bs[d] / 2;
+                            lo = g + c.second[d] * h;
+                            n = h;
+                        }
+                        p.recv_lo[d] = lo;
+                        p.n[d] = n;
+                        p.send_lo[d] = 2 * (lo - g) - (2 * o[d] + c.second[d]) * T.bs[d] + g;
+                        WH_CHECK(p.send_lo[d] >= 0 && p.send_lo[d] + 2 * (n - 1) < nxyz[d], WH_E_ARG,
+                                 "sync: restriction box outside the finer block (too few points per block)");
+                    }
+                    p.op = WH_PATCH_RESTRICT | ((filt ? 2 : src_sel(T, S, f, true)) << 8);
+                    p.stage = 2;
+                    st23.push_back(p);
+                    restrict_senders.push_back(f);
+                }
+                continue;
+            }
+            // 3) the coarser leaf covering the box
+            if (!S.predict || R.level == 0) continue;
+            int cu[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) {
+                const int v = q[d] + o[d];
+                cu[d] = v >= 0 ? v / 2 : -((-v + 1) / 2);
+            }
+            const int c = T.find(R.level - 1, cu);
+            if (!T.valid(c) || !T.is_leaf(c, true)) continue;
+            wh_patch p{};
+            p.recv = R.hvy;
+            p.send = T.b[c].hvy;
+            for (int d = 0; d < 3; ++d) {
+                p.recv_lo[d] = rlo[d];
+                p.n[d] = rn[d];
+                if (d >= dim) {
+                    p.send_lo[d] = 0;
+                    continue;
+                }
+                // fine index in the coarse block's doubled ghost-inclusive index space
+                p.send_lo[d] = q[d] * T.bs[d] + (rlo[d] - g) - 2 * cu[d] * T.bs[d] + 2 * g;
+            }
+            p.op = WH_PATCH_PREDICT | (src_sel(T, S, c, true) << 8);
+            p.stage = 3;
+            p.aux = order;
+            st23.push_back(p);
+        }
+    }
+    if (int rc = run_patch_list(T.P, A_U, A_T, A_FILT, st1)) return rc;
+    if (filt && !restrict_senders.empty()) {
+        // restrict_copy_at_CE: HD-filtered copy of every restriction sender,
+        // original SC points put back near level jumps of leaf senders
+        std::sort(restrict_senders.begin(), restrict_senders.end());
+        restrict_senders.erase(std::unique(restrict_senders.begin(), restrict_senders.end()),
+                               restrict_senders.end());
+        for (int arr : {A_U, A_T}) {
+            std::vector<int32_t> ids;
+            std::vector<wh_box> boxes;
+            for (int s : restrict_senders) {
+                if ((src_sel(T, S, s, true) == 1 ? A_T : A_U) != arr) continue;
+                ids.push_back(T.b[s].hvy);
+                if (!T.is_leaf(s, false)) continue;
+                for (int k = 0; k < nd; ++k) {
+                    const int *o = dirs[k];
+                    if (T.valid(T.nb_same(s, o))) continue;
+                    if (T.nb_coarse(s, o) < 0 && !T.any_finer(s, o, false)) continue;
+                    wh_box bx{};
+                    bx.hvy = T.b[s].hvy;
+                    bx.op = WH_BOX_COPY_SC;
+                    modify_box(T, o, T.w.Nscl, T.w.Nscr, bx);
+                    boxes.push_back(bx);
+                }
+            }
+            if (int rc = run_blocks_op(T.P, WOP_FILTER, arr, A_FILT, ids)) return rc;
+            if (int rc = run_boxes(T.P, A_FILT, arr, boxes)) return rc;
+        }
+    }
+    return run_patch_list(T.P, A_U, A_T, A_FILT, st23);
+}
+
+// ---- adapt_tree pieces ----------------------------------------------------------
+// init_full_tree (adapt_tree.f90:995-1084): mothers down to jmin, REF_TMP_EMPTY
+int init_full_tree(Tree &T, int jmin)
+{
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        const int n0 = (int)T.b.size();
+        for (int i = 0; i < n0; ++i) {
+            const TNode x = T.b[i];
+            if (x.level == jmin || x.level == 0 || T.mother(i) >= 0) continue;
+            const int64_t mask = (((int64_t)1 << T.dim) - 1) << T.F->digit_shift(x.level);
+            if (T.add(x.tc & ~mask, x.level - 1, REF_TMP_EMPTY) < 0) return WH_E_NOMEM;
+            changed = true;
+        }
+    }
+    return WH_OK;
+}
+
+// sync_D2M (executeCoarsening_tree.f90:125-227): daughters on `level` hand
+// their SC (interior points g, g+2, ...) to the mother's quadrant/octant
+int d2m(Tree &T, int level)
+{
+    std::vector<wh_patch> p;
+    for (int i = 0; i < (int)T.b.size(); ++i) {
+        if (T.b[i].level != level) continue;
+        const int m = T.mother(i);
+        if (m < 0) continue;
+        int q[3];
+        T.ijk(i, q);
+        wh_patch x{};
+        x.recv = T.b[m].hvy;
+        x.send = T.b[i].hvy;
+        for (int d = 0; d < 3; ++d) {
+            if (d >= T.dim) {
+                x.recv_lo[d] = x.send_lo[d] = 0;
+                x.n[d] = 1;
+                continue;
+            }
+            const int h = T.bs[d] / 2;
+            x.recv_lo[d] = T.g + (q[d] & 1) * h;
+            x.send_lo[d] = T.g;
+            x.n[d] = h;
+        }
+        x.op = WH_PATCH_RESTRICT;
+        x.stage = 1;
+        p.push_back(x);
+    }
+    return run_patch_list(T.P, A_U, -1, -1, p);
+}
+
+std::vector<int32_t> hvy_of(const Tree &T, const std::vector<int> &idx)
+{
+    std::vector<int32_t> h;
+    h.reserve(idx.size());
+    for (int i : idx) h.push_back(T.b[i].hvy);
+    return h;
+}
+
+// wavelet_decompose_full_tree (adapt_tree.f90:264-545)
+int decompose_full_tree(Tree &T, const wh_adapt_params &p)
+{
+    const wh_wavelet &w = T.w;
+    const int n_leaf = (int)T.b.size();
+    std::vector<int> backup(n_leaf);
+    for (int i = 0; i < n_leaf; ++i) backup[i] = T.b[i].status;
+    const int hmax = std::max(std::abs(w.hd_lo), w.hd_hi);
+    bool leaf_first = true;
+    for (int d = 0; d < T.dim; ++d)
+        if (T.bs[d] < 3 * hmax) leaf_first = false;
+    int level = T.max_level();
+    std::vector<int> all(n_leaf);
+    for (int i = 0; i < n_leaf; ++i) {
+        all[i] = i;
+        if (T.b[i].level == level || leaf_first) T.b[i].status = -1;
+    }
+    if (int rc = run_blocks_op(T.P, WOP_COPY, A_U, A_T, hvy_of(T, all))) return rc;
+    if (int rc = init_full_tree(T, p.jmin)) return rc;
+    const int g_this = std::max(std::abs(w.hd_lo), w.gd_hi);
+    int it = 0;
+    while (level >= p.jmin) {
+        SyncSpec S;
+        S.predict = !(p.use_ce && (!leaf_first || it > 0));  // MoF_ref vs full_ref
+        S.recv_ref_on = true;
+        S.recv_ref = -1;
+        S.src = SRC_NOT_FLAG;
+        S.src_ref = -1;
+        S.gm = S.gp = g_this;
+        if (int rc = tree_sync(T, S)) return rc;
+        std::vector<int> dec, dea;
+        for (int i = 0; i < (int)T.b.size(); ++i) {
+            if (T.b[i].status != -1) continue;
+            if (T.b[i].level == p.jmax && p.force_maxlevel_dealiasing) dea.push_back(i);
+            else dec.push_back(i);
+        }
+        std::vector<int> both(dec);
+        both.insert(both.end(), dea.begin(), dea.end());
+        if (int rc = run_blocks_op(T.P, WOP_COPY, A_U, A_T, hvy_of(T, both))) return rc;
+        if (int rc = run_blocks_op(T.P, WOP_FWT, A_T, A_U, hvy_of(T, dec))) return rc;
+        if (int rc = run_blocks_op(T.P, WOP_FILTER, A_T, A_U, hvy_of(T, dea))) return rc;
+        if (p.use_ce)
+            if (int rc = ce_modify(T, CE_REF, 0, -1, true, true)) return rc;
+        for (TNode &x : T.b) {
+            if (x.status == -1) x.status = 0;
+            if ((x.status == REF_TMP_EMPTY || !leaf_first) && x.level == level - 1) x.status = -1;
+        }
+        if (int rc = d2m(T, level)) return rc;
+        --level;
+        ++it;
+    }
+    for (int i = 0; i < (int)T.b.size(); ++i) T.b[i].status = i < n_leaf ? backup[i] : 0;
+    return WH_OK;
+}
+
+// ensureGradedness_tree (ensureGradedness_tree.f90:13-219) with
+// ensure_completeness_block(check_daughters) (ensure_completeness_block.f90)
+void ensure_gradedness(Tree &T, int stay)
+{
+    int dirs[26][3];
+    const int nd = dirs_of(T.dim, dirs);
+    const int nsis = 1 << T.dim;
+    std::vector<std::pair<int, std::array<int, 3>>> ch;
+    bool changed = true;
+    int counter = 0;
+    while (changed && counter < 10 * T.jmax + 10) {
+        changed = false;
+        ++counter;
+        for (int i = 0; i < (int)T.b.size(); ++i) {
+            int me = T.b[i].status;
+            if (me == -1) {
+                const int m = T.mother(i);
+                int sis[8];
+                bool all = true;
+                for (int k = 0; k < nsis; ++k) {
+                    sis[k] = m >= 0 ? T.daughter(m, k) : -1;
+                    if (sis[k] < 0) all = false;
+                }
+                if (all) {
+                    bool any_empty = false, any_stay = false, all_neg = true;
+                    for (int k = 0; k < nsis; ++k) {
+                        const int s = T.b[sis[k]].status;
+                        if (s == REF_TMP_EMPTY) any_empty = true;
+                        if (s == 0 || s == stay) any_stay = true;
+                        if (s >= 0) all_neg = false;
+                    }
+                    if (any_empty || any_stay) {
+                        for (int k = 0; k < nsis; ++k)
+                            if (T.b[sis[k]].status < 0) T.b[sis[k]].status = stay;
+                    } else if (all_neg) {
+                        for (int k = 0; k < nsis; ++k) T.b[sis[k]].status = -1;
+                    }
+                    if (!T.is_leaf(i, false))
+                        for (int k = 0; k < nsis; ++k) {
+                            const int d = T.daughter(i, k);
+                            if (d >= 0 && (T.b[d].status == 0 || T.b[d].status == stay))
+                                for (int l = 0; l < nsis; ++l)
+                                    if (T.b[sis[l]].status < 0) T.b[sis[l]].status = stay;
+                        }
+                } else {
+                    for (int k = 0; k < nsis; ++k)
+                        if (sis[k] >= 0 && T.b[sis[k]].status < -1) T.b[sis[k]].status = stay;
+                }
+                if (T.b[i].status != me) {
+                    me = T.b[i].status;
+                    changed = true;
+                }
+            }
+            if (me == -1) {
+                for (int q = 0; q < nd; ++q) {
+                    const int *o = dirs[q];
+                    const int s = T.nb_same(i, o);
+                    if (s >= 0 && T.b[s].status == 1) {
+                        T.b[i].status = stay;
+                        changed = true;
+                    }
+                    T.finer_children(i, o, ch);
+                    for (auto &c : ch) {
+                        const int f = c.first;
+                        if (f < 0) continue;
+                        if (T.b[f].status == 1) {
+                            T.b[i].status = 1;
+                            changed = true;
+                        } else if (T.b[f].status == 0 || T.b[f].status == stay) {
+                            if (T.b[i].status != 1) T.b[i].status = stay;
+                            changed = true;
+                        }
+                    }
+                }
+            } else if (me == 0 || me == stay) {
+                for (int q = 0; q < nd; ++q) {
+                    T.finer_children(i, dirs[q], ch);
+                    for (auto &c : ch)
+                        if (c.first >= 0 && T.b[c.first].status == 1) {
+                            T.b[i].status = 1;
+                            changed = true;
+                        }
+                }
+            }
+        }
+    }
+}
+
+// wavelet_reconstruct_full_tree_CEoptimized (adapt_tree.f90:686-987)
+int reconstruct_tree(Tree &T, const wh_adapt_params &p)
+{
+    const wh_wavelet &w = T.w;
+    int dirs[26][3];
+    const int nd = dirs_of(T.dim, dirs);
+    std::vector<int> backup(T.b.size());
+    for (size_t i = 0; i < T.b.size(); ++i) {
+        backup[i] = T.b[i].status;
+        T.b[i].status = 0;
+    }
+    for (int i = 0; i < (int)T.b.size(); ++i) {
+        if (!T.is_leaf(i, false)) continue;
+        for (int q = 0; q < nd; ++q) {
+            const int *o = dirs[q];
+            if (T.nb_coarse(i, o) < 0) continue;
+            if (T.valid(T.nb_same(i, o)) || T.any_finer(i, o, true)) continue;
+            T.b[i].status = -1;
+            break;
+        }
+    }
+    bool reconstruct_neighbors = false, leaf_only = true;
+    for (int d = 0; d < T.dim; ++d) {
+        if (T.bs[d] < w.Nreconl || T.bs[d] < w.Nreconr) reconstruct_neighbors = true;
+        if (T.bs[d] < w.Ndep2l || T.bs[d] < w.Ndep2r) leaf_only = false;
+    }
+    WH_CHECK(!reconstruct_neighbors, WH_E_ARG,
+             "adapt_tree: blocks smaller than the reconstruction reach are not supported");
+    SyncSpec S;
+    S.restrict_ = false;
+    S.recv_ref_on = true;
+    S.recv_ref = -1;
+    S.src = SRC_DECOMPOSED;
+    S.gm = S.gp = T.g;
+    auto finish = [&](int level_only) -> int {
+        std::vector<int> rec, all;
+        for (int i = 0; i < (int)T.b.size(); ++i) {
+            if (level_only >= 0 && T.b[i].level != level_only) continue;
+            all.push_back(i);
+            if (T.b[i].status == -1) rec.push_back(i);
+        }
+        if (int rc = run_blocks_op(T.P, WOP_IWT, A_U, A_T, hvy_of(T, rec))) return rc;
+        return run_blocks_op(T.P, WOP_COPY, A_T, A_U, hvy_of(T, all));
+    };
+    if (leaf_only) {
+        if (int rc = tree_sync(T, S)) return rc;
+        if (p.use_ce)
+            if (int rc = ce_modify(T, CE_TREE, 0, 0, true, false)) return rc;
+        if (int rc = finish(-1)) return rc;
+    } else {
+        const int lmax = T.max_level();
+        S.recv_level_on = true;
+        for (int L = p.jmin; L <= lmax; ++L) {
+            S.recv_level = L;
+            if (int rc = tree_sync(T, S)) return rc;
+            if (p.use_ce)
+                if (int rc = ce_modify(T, CE_LEVEL, L, 0, true, false)) return rc;
+            if (int rc = finish(L)) return rc;
+        }
+    }
+    for (size_t i = 0; i < T.b.size(); ++i) T.b[i].status = backup[i];
+    return WH_OK;
+}
+
+// leaves of the tree back into the forest's block list
+void store_leaves(Tree &T)
+{
+    T.F->blocks.clear();
+    for (const TNode &x : T.b) T.F->blocks.push_back(FBlock{x.tc, x.level, x.hvy, 0});
+    T.F->rebuild_map();
+}
+
+int check_pool(wh_pool *P, wh_forest *F)
+{
+    WH_CHECK(P && F, WH_E_ARG, "adapt: null argument");
+    WH_CHECK(P->ws.set, WH_E_STATE, "adapt: set the pool's wavelet first (wh_pool_set_wavelet)");
+    WH_CHECK(F->dim == P->d.dim && F->g == P->d.g && F->nb_max <= P->d.nb_max, WH_E_ARG,
+             "adapt: forest and pool disagree on dim / ghost nodes / capacity");
+    for (int d = 0; d < F->dim; ++d)
+        WH_CHECK(F->bs[d] == P->d.bs[d] && F->bs[d] % 2 == 0 && 2 * F->g <= F->bs[d], WH_E_ARG,
+                 "adapt: block size must match the pool, be even and at least 2g");
+    WH_CHECK(P->d.n_work_slots >= 2 && P->tmp && P->d.n_tmp_eqn >= P->d.neq, WH_E_ARG,
+             "adapt: needs two work slots and hvy_tmp with neq components");
+    return WH_OK;
+}
+
+}  // namespace
+}  // namespace wh
+
+extern "C" {
+
+int wh_sync_tree(wh_pool *P, wh_forest *F, int array)
+{
+    using namespace wh;
+    if (int rc = check_pool(P, F)) return rc;
+    WH_CHECK(array == WH_BLOCK, WH_E_ARG, "wh_sync_tree: only WH_BLOCK is supported");
+    Tree T(F, P);
+    SyncSpec S;
+    S.gm = S.gp = T.g;
+    return tree_sync(T, S);
+}
+
+int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
+{
+    using namespace wh;
+    if (int rc = check_pool(P, F)) return rc;
+    WH_CHECK(p && p->jmin >= 0 && p->jmax <= F->jmax && p->jmin <= p->jmax &&
+                 (p->indicator == WH_IND_EVERYWHERE || p->indicator == WH_IND_THRESHOLD),
+             WH_E_ARG, "wh_adapt_tree: bad parameters");
+    for (const FBlock &b : F->blocks)
+        WH_CHECK(b.level >= p->jmin, WH_E_ARG, "wh_adapt_tree: a block is below jmin");
+    const int stay = REF_UNSIGNIFICANT_STAY;
+    Tree T(F, P);
+    const std::vector<int> free0 = F->free_hvy;
+    auto fail = [&](int rc) {
+        // restore the forest's slots; device data of the leaves is not restored
+        F->free_hvy = free0;
+        return rc;
+    };
+    if (int rc = decompose_full_tree(T, *p)) return fail(rc);
+    // coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331)
+    for (TNode &x : T.b) x.status = 0;
+    if (p->indicator == WH_IND_EVERYWHERE) {
+        for (int i = 0; i < (int)T.b.size(); ++i)
+            if (T.is_leaf(i, false)) T.b[i].status = -1;
+    } else {
+        std::vector<int> idx;
+        for (int i = 0; i < (int)T.b.size(); ++i) {
+            if (p->force_maxlevel_dealiasing && T.b[i].level == p->jmax) T.b[i].status = -1;
+            else idx.push_back(i);
+        }
+        std::vector<double> det;
+        if (int rc = run_detail(P, A_U, hvy_of(T, idx), p->groups, det)) return fail(rc);
+        const int neq = P->d.neq;
+        for (size_t k = 0; k < idx.size(); ++k) {
+            bool all = true;
+            for (int c = 0; c < neq; ++c)
+                if (!(det[k * neq + c] <= p->eps * p->norm[c])) all = false;
+            T.b[idx[k]].status = all ? -1 : 0;
+        }
+    }
+    // respectJmaxJmin_tree (respectJmaxJmin_tree.f90:1-39)
+    for (TNode &x : T.b) {
+        if (x.status == 1 && x.level >= p->jmax) x.status = stay;
+        if (x.status == -1 && x.level <= p->jmin) x.status = stay;
+    }
+    ensure_gradedness(T, stay);
+    {
+        std::vector<char> del(T.b.size(), 0);
+        for (size_t i = 0; i < T.b.size(); ++i) del[i] = T.b[i].status == -1;
+        T.remove_marked(del);
+    }
+    if (p->use_ce) {
+        if (int rc = ce_modify(T, CE_TREE, 0, 0, true, false)) return fail(rc);
+        if (int rc = reconstruct_tree(T, *p)) return fail(rc);
+    } else {
+        std::vector<int> all(T.b.size());
+        for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+        if (int rc = run_blocks_op(P, WOP_COPY, A_T, A_U, hvy_of(T, all))) return fail(rc);
+    }
+    // prune_fulltree2leafs
+    {
+        std::vector<char> del(T.b.size(), 0);
+        for (size_t i = 0; i < T.b.size(); ++i) del[i] = !T.is_leaf((int)i, false);
+        T.remove_marked(del);
+    }
+    for (TNode &x : T.b) x.status = 0;
+    store_leaves(T);
+    SyncSpec S;
+    S.gm = S.gp = T.g;
+    return tree_sync(T, S);
+}
+
+}  // extern "C"

@@ -320,6 +320,7 @@ int wh_pool_destroy(wh_pool *P)
     destroy_graph(P);
     free_patchset(P->ps[0]);
     free_patchset(P->ps[1]);
+    wh::free_wavelet_ws(P);
     for (double *p : {P->block, P->work, P->tmp, P->mask, P->scal, P->d_x0dx})
         if (p) (void)hipFree(p);
     if (P->d_active) (void)hipFree(P->d_active);

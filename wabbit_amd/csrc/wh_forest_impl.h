@@ -1,0 +1,104 @@
+// wh_forest_impl.h -- the host forest (block list of one tree) shared by the
+// forest helpers (wh_forest.cpp) and the adapt_tree orchestration
+// (wh_adapt.cpp).  Host code only.
+//
+// Treecode layout (LIB/TREE/module_treelib.f90:107-283): int64 with dim bits
+// per level; the digit of level l sits at bit offset (Jmax-l)*dim; digit bits
+// (x, y, z) = (2, 1, 4) (refinementExecute.f90:98-105).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/wabbit_hip.h"
+
+namespace wh {
+void set_error(const std::string &msg);
+
+struct FBlock {
+    int64_t tc;
+    int level;
+    int hvy;
+    int status;  // refinement status (module_globals.f90:25-32)
+};
+
+inline uint64_t fkey(int level, int64_t tc) { return ((uint64_t)level << 58) ^ (uint64_t)tc; }
+}  // namespace wh
+
+struct wh_forest {
+    int dim = 2, jmax = 0, g = 0;
+    int bs[3] = {1, 1, 1};
+    double domain[3] = {1, 1, 1};
+    int periodic[3] = {1, 1, 1};
+    int nb_max = 0;
+    std::vector<wh::FBlock> blocks;         // active blocks (leaf list)
+    std::unordered_map<uint64_t, int> map;  // (level, tc) -> index in blocks
+    std::vector<int> free_hvy;              // free pool slots
+
+    void rebuild_map()
+    {
+        map.clear();
+        for (size_t i = 0; i < blocks.size(); ++i) map[wh::fkey(blocks[i].level, blocks[i].tc)] = (int)i;
+    }
+    int digit_shift(int level) const { return (jmax - level) * dim; }
+    void ijk_of(int64_t tc, int level, int ijk[3]) const
+    {
+        ijk[0] = ijk[1] = ijk[2] = 0;
+        for (int l = 1; l <= level; ++l) {
+            const int dgt = (int)((tc >> digit_shift(l)) & ((1 << dim) - 1));
+            ijk[0] = 2 * ijk[0] + ((dgt >> 1) & 1);
+            ijk[1] = 2 * ijk[1] + (dgt & 1);
+            ijk[2] = 2 * ijk[2] + (dim == 3 ? ((dgt >> 2) & 1) : 0);
+        }
+    }
+    int64_t tc_of(const int ijk[3], int level) const
+    {
+        int64_t tc = 0;
+        for (int l = 1; l <= level; ++l) {
+            const int sh = level - l;
+            const int bx = (ijk[0] >> sh) & 1, by = (ijk[1] >> sh) & 1;
+            const int bz = dim == 3 ? ((ijk[2] >> sh) & 1) : 0;
+            tc |= (int64_t)(by + 2 * bx + 4 * bz) << digit_shift(l);
+        }
+        return tc;
+    }
+    // block at grid index ijk on level (periodic wrap); -1 if not active.
+    int find(int level, const int ijk_in[3], int wrapped[3]) const
+    {
+        if (level < 0) return -1;
+        const int n = 1 << level;
+        for (int d = 0; d < 3; ++d) {
+            int v = ijk_in[d];
+            if (d < dim) {
+                if (v < 0 || v >= n) {
+                    if (!periodic[d]) return -1;
+                    v = ((v % n) + n) % n;
+                }
+            } else {
+                v = 0;
+            }
+            wrapped[d] = v;
+        }
+        auto it = map.find(wh::fkey(level, tc_of(wrapped, level)));
+        return it == map.end() ? -1 : it->second;
+    }
+    void x0dx(const wh::FBlock &b, double out[6]) const
+    {
+        int ijk[3];
+        ijk_of(b.tc, b.level, ijk);
+        for (int d = 0; d < 3; ++d) {
+            if (d < dim) {
+                // get_block_spacing_origin_b: dx = 2^-J * L / Bs, x0 = ((ijk)*Bs) * dx
+                const double dx = std::ldexp(1.0, -b.level) * domain[d] / (double)bs[d];
+                out[3 + d] = dx;
+                out[d] = (double)(ijk[d] * bs[d]) * dx;
+            } else {
+                out[d] = 0.0;
+                out[3 + d] = 0.0;
+            }
+        }
+    }
+};

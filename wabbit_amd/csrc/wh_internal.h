@@ -49,6 +49,28 @@ struct AcmDev {
     int dim, scheme, skew, penal, sponge;
 };
 
+// Alternate source arrays of explicit patch lists (bits 8-9 of wh_patch::op).
+struct SyncSrc {
+    const double *alt[2] = {nullptr, nullptr};
+    size_t blk_alt[2] = {0, 0};
+};
+
+// Device workspace of the wavelet and adapt operations (grown on demand).
+struct WaveletWS {
+    bool set = false;
+    wh_wavelet w{};
+    int32_t *d_ids = nullptr;
+    size_t ids_cap = 0;
+    wh_patch *d_patches = nullptr;
+    size_t patch_cap = 0;
+    int64_t *d_pstart = nullptr;
+    size_t pstart_cap = 0;
+    wh_box *d_boxes = nullptr;
+    size_t box_cap = 0;
+    double *d_out = nullptr;
+    size_t out_cap = 0;
+};
+
 // Host-side time controls (calculate_time_step.f90:58-118) evaluated on device.
 struct TimeDev {
     double time_max, dt_fixed, dt_max, write_time, write_time_first, tsave_stats,
@@ -102,6 +124,8 @@ struct wh_pool {
     wh::TimeDev graph_tc{};
     // tuning: y-split of the 3D stage kernel (-1 auto, 0 flat kernel)
     int k3d_ysplit = -1;
+    // wavelet transforms / adapt_tree
+    wh::WaveletWS ws;
 };
 
 namespace wh {
@@ -123,6 +147,20 @@ int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *t
 int launch_dt_finalize(wh_pool *P, const TimeDev &tc);
 int launch_time_advance(wh_pool *P);
 int launch_set_scalar(wh_pool *P, int which, double v);
+
+int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const int64_t *d_pstart,
+                     int p0, int p1, int64_t npts, const SyncSrc &S);
+
+// wavelet workspace (wh_wavelet.hip)
+void free_wavelet_ws(wh_pool *P);
+// device-side steps used by the adapt orchestration (wh_adapt.cpp); the
+// host vectors are uploaded on the pool's stream before use
+int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<wh_patch> &patches);
+int run_blocks_op(wh_pool *P, int op, int src_array, int dst_array, const std::vector<int32_t> &ids);
+int run_boxes(wh_pool *P, int array, int src_array, const std::vector<wh_box> &boxes);
+int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
+               std::vector<double> &detail);
+enum { WOP_FWT = 0, WOP_IWT = 1, WOP_FILTER = 2, WOP_COPY = 3 };
 
 AcmDev make_acm_dev(const wh_acm_params *p);
 int reach_of(int scheme);

@@ -581,11 +581,13 @@ __device__ __forceinline__ double pred_any(const double *u, int nx, int ny, int 
 
 // Flat ghost fill: one thread per ghost point of the patches [p0, p1); the
 // patch of a point is found by binary search in the point-count prefix sum.
+// Bits 8-9 of a patch's op select the array its sender is read from (0: the
+// synced array itself, 1/2: the alternates in S).
 __global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
                                                    const wh_patch *__restrict__ pt,
                                                    const int64_t *__restrict__ pstart, int p0,
                                                    int p1, size_t blk, size_t cs, int nx, int ny,
-                                                   int neq)
+                                                   int neq, SyncSrc S)
 {
     const int64_t e = pstart[p0] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= pstart[p1]) return;
@@ -602,18 +604,31 @@ __global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
     const size_t ri =
         ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx + (P.recv_lo[0] + i);
     double *dst = arr + (size_t)P.recv * blk + ri;
-    if (P.op == WH_PATCH_PREDICT) {
+    const int op = P.op & 0xff, sel = (P.op >> 8) & 3;
+    const double *sbase = sel == 0 ? arr + (size_t)P.send * blk
+                                   : S.alt[sel - 1] + (size_t)P.send * S.blk_alt[sel - 1];
+    if (op == WH_PATCH_PREDICT) {
         const int f[3] = {P.send_lo[0] + i, P.send_lo[1] + j, P.send_lo[2] + kk};
-        const double *src = arr + (size_t)P.send * blk;
         const int dim = (cs == (size_t)nx * ny) ? 2 : 3;
-        for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(src + c * cs, nx, ny, dim, f, P.aux);
+        for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(sbase + c * cs, nx, ny, dim, f, P.aux);
         return;
     }
-    const int st = (P.op == WH_PATCH_RESTRICT) ? 2 : 1;
+    const int st = (op == WH_PATCH_RESTRICT) ? 2 : 1;
     const size_t si = ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
                       (P.send_lo[0] + st * i);
-    const double *src = arr + (size_t)P.send * blk + si;
+    const double *src = sbase + si;
     for (int c = 0; c < neq; ++c) dst[c * cs] = src[c * cs];
+}
+
+int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const int64_t *d_pstart,
+                     int p0, int p1, int64_t npts, const SyncSrc &S)
+{
+    if (p1 <= p0 || npts <= 0) return WH_OK;
+    const unsigned nblk = (unsigned)((npts + 255) / 256);
+    hipLaunchKernelGGL(k_sync_flat, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
+                       p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
 }
 
 // Per-receiver ghost fill (the plain one-launch-per-block path): one
@@ -703,7 +718,7 @@ int launch_sync(wh_pool *P, int sync_case, double *arr, int one_receiver)
         const int64_t npts = ps.h_pstart[p1] - ps.h_pstart[p0];
         const unsigned nblk = (unsigned)((npts + 255) / 256);
         hipLaunchKernelGGL(k_sync_flat, dim3(nblk), dim3(256), 0, P->stream, arr, ps.d_patches,
-                           ps.d_pstart, p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq);
+                           ps.d_pstart, p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, SyncSrc{});
     }
     WH_HIP(hipGetLastError());
     return WH_OK;

@@ -1,0 +1,717 @@
+// wh_wavelet.hip -- CDF wavelet machinery on the device pool: filter setup
+// (host), one-level FWT / IWT and the restriction filter of listed blocks,
+// Linfty details for the coarsening indicator, coarse-extension boxes and
+// explicit ghost-patch lists with selectable source arrays.
+//
+// Layout: block b of an array at base + b*blk, component c at + c*cs, point
+// (ix, iy, iz) at ((iz*ny + iy)*nx + ix), x fastest (Fortran hvy_block).
+//
+// Summation order.  Every filter output is a chain of products added left
+// to right, the order of the reference's loops (gfortran, no reassociation):
+// the unrolled cases of wavelet_decomposition_reconstruction.f90 (CDF20/22/
+// 40/42/44/60/62) skip zero taps and, in the reconstruction, start with the
+// point's own coefficient; the generic case adds two separate sums;
+// blockFilterXYZ_vct starts from zero and adds every tap.  A TermTable holds
+// that chain per output parity, so results match the CPU restatement bit for
+// bit (compiled with -ffp-contract=off).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "wh_internal.h"
+
+namespace wh {
+
+// ---------------------------------------------------------------------------
+// setup_wavelet (module_wavelets.f90:1031-1430)
+// ---------------------------------------------------------------------------
+namespace {
+
+// Stencil_Int_<order>th (module_wavelets.f90:14-22) on -(order-1)..order-1
+void interp_stencil(int order, double *c)
+{
+    if (order == 2) {
+        const double v[3] = {1.0, 2.0, 1.0};
+        for (int i = 0; i < 3; ++i) c[i] = v[i] / 2.0;
+    } else if (order == 4) {
+        const double v[7] = {-1.0, 0.0, 9.0, 16.0, 9.0, 0.0, -1.0};
+        for (int i = 0; i < 7; ++i) c[i] = v[i] / 16.0;
+    } else {
+        const double v[11] = {3.0, 0.0, -25.0, 0.0, 150.0, 256.0, 150.0, 0.0, -25.0, 0.0, 3.0};
+        for (int i = 0; i < 11; ++i) c[i] = v[i] / 256.0;
+    }
+}
+
+inline double &tap(double *f, int lo, int i) { return f[i - lo]; }
+inline double tapc(const double *f, int lo, int i) { return f[i - lo]; }
+
+}  // namespace
+}  // namespace wh
+
+extern "C" int wh_wavelet_setup(const char *name, int fd_max_size, wh_wavelet *w)
+{
+    using namespace wh;
+    WH_CHECK(w && name && std::strlen(name) == 5 && std::strncmp(name, "CDF", 3) == 0, WH_E_ARG,
+             "wh_wavelet_setup: expected a name CDFXY");
+    const int X = name[3] - '0', Y = name[4] - '0';
+    WH_CHECK((X == 2 || X == 4 || X == 6) && (Y == 0 || Y == 2 || Y == 4 || Y == 6) && Y <= X,
+             WH_E_ARG, "wh_wavelet_setup: supported are CDFXY with X in {2,4,6}, Y in {0,2,4,6}, Y <= X");
+    WH_CHECK(fd_max_size >= 0 && fd_max_size <= 8, WH_E_ARG, "wh_wavelet_setup: bad fd_max_size");
+    std::memset(w, 0, sizeof(*w));
+    w->X = X;
+    w->Y = Y;
+    w->lifted = Y != 0;
+    w->hr_lo = -(X - 1);
+    w->hr_hi = X - 1;
+    interp_stencil(X, w->HR);
+    if (Y == 0) {
+        w->hd_lo = w->hd_hi = 0;
+        w->HD[0] = 1.0;
+    } else {
+        // HD(i) = delta_i0 + 1/2 sum_j (-1)^j HR(j) h(i-j), h = order-Y stencil
+        // with its centre tap removed (:1182-1234)
+        double h[11];
+        interp_stencil(Y, h);
+        const int h_lo = -(Y - 1), h_hi = Y - 1;
+        h[-h_lo] = 0.0;
+        w->hd_lo = w->hr_lo + h_lo;
+        w->hd_hi = w->hr_hi + h_hi;
+        for (int i = w->hd_lo; i <= w->hd_hi; ++i) {
+            double v = (i == 0) ? 1.0 : 0.0;
+            for (int j = w->hr_lo; j <= w->hr_hi; ++j) {
+                const int k = i - j;
+                if (k < h_lo || k > h_hi) continue;
+                const double sgn = (j % 2 == 0) ? 1.0 : -1.0;
+                v = v + sgn * tapc(w->HR, w->hr_lo, j) * h[k - h_lo] / 2.0;
+            }
+            tap(w->HD, w->hd_lo, i) = v;
+        }
+    }
+    // GD(i) = (-1)^i HR(i), GR(i) = (-1)^i HD(i) (:1275-1288)
+    w->gd_lo = w->hr_lo;
+    w->gd_hi = w->hr_hi;
+    for (int i = w->gd_lo; i <= w->gd_hi; ++i)
+        tap(w->GD, w->gd_lo, i) = ((i % 2 == 0) ? 1.0 : -1.0) * tapc(w->HR, w->hr_lo, i);
+    w->gr_lo = w->hd_lo;
+    w->gr_hi = w->hd_hi;
+    for (int i = w->gr_lo; i <= w->gr_hi; ++i)
+        tap(w->GR, w->gr_lo, i) = ((i % 2 == 0) ? 1.0 : -1.0) * tapc(w->HD, w->hd_lo, i);
+    w->order = X;
+    const int bounds[8] = {w->hd_lo, w->hd_hi, w->gd_lo, w->gd_hi,
+                           w->hr_lo, w->hr_hi, w->gr_lo, w->gr_hi};
+    w->g = 0;
+    for (int b : bounds) w->g = std::max(w->g, std::abs(b));
+    w->g_rhs = std::max(X / 2, fd_max_size);
+    // coarse-extension widths (:1368-1420) with the FD minimum 2*reach
+    w->Nscl = std::max(std::abs(w->hd_lo) - 1, 0);
+    w->Nwcl = w->Nscl + std::abs(w->gd_lo);
+    w->Nreconl = w->Nwcl + std::abs(w->gr_lo);
+    w->Ndep2l = w->Nreconl + std::max((std::abs(w->hr_lo) + 1) / 2 - 1, 0);
+    w->Nscr = w->hd_hi;
+    w->Nwcr = w->Nscr + w->gd_hi;
+    w->Nreconr = w->Nwcr + w->gr_hi;
+    w->Ndep2r = w->Nreconr + (w->hr_hi + 1) / 2;
+    const int m = 2 * fd_max_size;
+    const int dl = std::max(m - w->Nwcl, 0), dr = std::max(m - w->Nwcr, 0);
+    w->Nwcl += dl;
+    w->Nreconl += dl;
+    w->Ndep2l += dl;
+    w->Nwcr += dr;
+    w->Nreconr += dr;
+    w->Ndep2r += dr;
+    w->specialised = (Y == 0) || (X == 2 && Y == 2) || (X == 4 && (Y == 2 || Y == 4)) ||
+                     (X == 6 && Y == 2);
+    return WH_OK;
+}
+
+namespace wh {
+namespace {
+
+// ---------------------------------------------------------------------------
+// term chains
+// ---------------------------------------------------------------------------
+constexpr int TMAX = 24;
+struct TermTable {
+    int n[2];  // [0]: output at an SC point of the pass direction, [1]: WC point
+    int off[2][TMAX];
+    double c[2][TMAX];
+    int brk[2][TMAX];  // 1: term starts a new sum added to the running total
+    int from_zero;     // accumulate from 0.0 (blockFilterXYZ_vct)
+};
+
+void add_term(TermTable &T, int par, int off, double c, int brk = 0)
+{
+    const int k = T.n[par]++;
+    T.off[par][k] = off;
+    T.c[par][k] = c;
+    T.brk[par][k] = brk;
+}
+
+TermTable fwt_terms(const wh_wavelet &w)
+{
+    TermTable T{};
+    for (int j = w.hd_lo; j <= w.hd_hi; ++j)
+        if (tapc(w.HD, w.hd_lo, j) != 0.0) add_term(T, 0, j, tapc(w.HD, w.hd_lo, j));
+    for (int j = w.gd_lo; j <= w.gd_hi; ++j)
+        if (tapc(w.GD, w.gd_lo, j) != 0.0) add_term(T, 1, j, tapc(w.GD, w.gd_lo, j));
+    return T;
+}
+
+TermTable iwt_terms(const wh_wavelet &w)
+{
+    TermTable T{};
+    auto hr = [&](int j) { return tapc(w.HR, w.hr_lo, j); };
+    auto gr = [&](int j) { return tapc(w.GR, w.gr_lo, j); };
+    if (!w.lifted) {
+        // u_r = u; WC points: u_r(ix) + HR(odd taps) (:483-537)
+        add_term(T, 0, 0, 1.0);
+        add_term(T, 1, 0, 1.0);
+        for (int j = w.hr_lo; j <= w.hr_hi; ++j)
+            if ((j & 1) && hr(j) != 0.0) add_term(T, 1, j, hr(j));
+    } else if (w.specialised) {
+        // SC: u(ix) + GR(odd); WC: HR(odd), GR(0), GR(even != 0), one chain
+        add_term(T, 0, 0, 1.0);
+        for (int j = w.gr_lo; j <= w.gr_hi; ++j)
+            if ((j & 1) && gr(j) != 0.0) add_term(T, 0, j, gr(j));
+        for (int j = w.hr_lo; j <= w.hr_hi; ++j)
+            if ((j & 1) && hr(j) != 0.0) add_term(T, 1, j, hr(j));
+        add_term(T, 1, 0, gr(0));
+        for (int j = w.gr_lo; j <= w.gr_hi; ++j)
+            if (j != 0 && !(j & 1) && gr(j) != 0.0) add_term(T, 1, j, gr(j));
+    } else {
+        // sum(HR * SC-buffer) + sum(GR * WC-buffer) (:797-838)
+        add_term(T, 0, 0, 1.0);
+        int first = 1;
+        for (int j = w.gr_lo; j <= w.gr_hi; ++j)
+            if ((j & 1) && gr(j) != 0.0) {
+                add_term(T, 0, j, gr(j), first);
+                first = 0;
+            }
+        for (int j = w.hr_lo; j <= w.hr_hi; ++j)
+            if ((j & 1) && hr(j) != 0.0) add_term(T, 1, j, hr(j));
+        first = 1;
+        for (int j = w.gr_lo; j <= w.gr_hi; ++j)
+            if (!(j & 1) && gr(j) != 0.0) {
+                add_term(T, 1, j, gr(j), first);
+                first = 0;
+            }
+    }
+    return T;
+}
+
+TermTable filter_terms(const wh_wavelet &w)
+{
+    TermTable T{};
+    T.from_zero = 1;
+    for (int j = w.hd_lo; j <= w.hd_hi; ++j) add_term(T, 0, j, tapc(w.HD, w.hd_lo, j));
+    return T;
+}
+
+__device__ __forceinline__ double chain(const TermTable &T, int par, const double *p, long stride)
+{
+    const int n = T.n[par];
+    if (T.from_zero) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s = s + p[T.off[par][k] * stride] * T.c[par][k];
+        return s;
+    }
+    double total = 0.0, cur = 0.0;
+    bool have = false;
+    for (int k = 0; k < n; ++k) {
+        const double v = p[T.off[par][k] * stride] * T.c[par][k];
+        if (k == 0) {
+            cur = v;
+        } else if (T.brk[par][k]) {
+            total = have ? total + cur : cur;
+            have = true;
+            cur = v;
+        } else {
+            cur = cur + v;
+        }
+    }
+    return have ? total + cur : cur;
+}
+
+// Output points of a pass: lo + step*i, i < n per direction.
+struct Region {
+    int lo[3], n[3], step[3];
+};
+
+struct ArrGeo {
+    size_t blk_in, blk_out, cs;
+    int nx, ny, neq;
+};
+
+// One separable pass over the listed blocks: out = chain(in) along `axis`;
+// the parity of the output point (relative to g) picks the SC/WC chain.
+__global__ __launch_bounds__(256) void k_wpass(const double *__restrict__ in,
+                                               double *__restrict__ out,
+                                               const int32_t *__restrict__ ids, int nb,
+                                               ArrGeo A, Region R, int axis, long astride, int g,
+                                               TermTable T)
+{
+    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
+    const long per_b = npts * A.neq;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_b * nb) return;
+    const int k = (int)(e / per_b);
+    long r = e - (long)k * per_b;
+    const int c = (int)(r / npts);
+    r -= (long)c * npts;
+    const int i0 = (int)(r % R.n[0]), i1 = (int)((r / R.n[0]) % R.n[1]),
+              i2 = (int)(r / ((long)R.n[0] * R.n[1]));
+    const int p0 = R.lo[0] + R.step[0] * i0, p1 = R.lo[1] + R.step[1] * i1,
+              p2 = R.lo[2] + R.step[2] * i2;
+    const size_t q = ((size_t)p2 * A.ny + p1) * A.nx + p0;
+    const int pa = axis == 0 ? p0 : (axis == 1 ? p1 : p2);
+    const int par = (pa - g) & 1;
+    const int b = ids[k];
+    out[(size_t)b * A.blk_out + c * A.cs + q] =
+        chain(T, par, in + (size_t)b * A.blk_in + c * A.cs + q, astride);
+}
+
+__global__ __launch_bounds__(256) void k_copy_region(const double *__restrict__ in,
+                                                     double *__restrict__ out,
+                                                     const int32_t *__restrict__ ids, int nb,
+                                                     ArrGeo A, Region R)
+{
+    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
+    const long per_b = npts * A.neq;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_b * nb) return;
+    const int k = (int)(e / per_b);
+    long r = e - (long)k * per_b;
+    const int c = (int)(r / npts);
+    r -= (long)c * npts;
+    const int i0 = (int)(r % R.n[0]), i1 = (int)((r / R.n[0]) % R.n[1]),
+              i2 = (int)(r / ((long)R.n[0] * R.n[1]));
+    const size_t q = ((size_t)(R.lo[2] + R.step[2] * i2) * A.ny + (R.lo[1] + R.step[1] * i1)) * A.nx +
+                     (R.lo[0] + R.step[0] * i0);
+    const int b = ids[k];
+    out[(size_t)b * A.blk_out + c * A.cs + q] = in[(size_t)b * A.blk_in + c * A.cs + q];
+}
+
+// Linfty detail: per listed block and component, max over the interior of
+// |v| (sqrt(v*v) for grouped components), SC points counted as 0.
+struct DetailArgs {
+    int g, bs[3], dim, neq;
+    int groups[8];
+};
+__global__ __launch_bounds__(256) void k_detail(const double *__restrict__ arr,
+                                                const int32_t *__restrict__ ids, ArrGeo A,
+                                                DetailArgs D, double *__restrict__ out)
+{
+    __shared__ unsigned long long smax[8];
+    if (threadIdx.x < 8) smax[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int b = ids[blockIdx.x];
+    const int npts = D.bs[0] * D.bs[1] * D.bs[2];
+    const int zoff = D.dim == 3 ? D.g : 0;
+    for (int c = 0; c < D.neq; ++c) {
+        double m = 0.0;
+        const double *u = arr + (size_t)b * A.blk_in + c * A.cs;
+        for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+            const int ix = e % D.bs[0], iy = (e / D.bs[0]) % D.bs[1], iz = e / (D.bs[0] * D.bs[1]);
+            const bool sc = ((ix & 1) == 0) && ((iy & 1) == 0) && (D.dim == 2 || (iz & 1) == 0);
+            const double v = sc ? 0.0
+                                : u[((size_t)(zoff + iz) * A.ny + (D.g + iy)) * A.nx + (D.g + ix)];
+            const double a = D.groups[c] >= 2 ? sqrt(v * v) : fabs(v);
+            m = a > m ? a : m;
+        }
+        // non-negative doubles order like their bit patterns
+        atomicMax(&smax[c], (unsigned long long)__double_as_longlong(m));
+    }
+    __syncthreads();
+    if (threadIdx.x < D.neq)
+        out[(size_t)blockIdx.x * D.neq + threadIdx.x] = __longlong_as_double((long long)smax[threadIdx.x]);
+}
+
+// Coarse-extension boxes: one workgroup per box.
+__global__ __launch_bounds__(256) void k_boxes(double *__restrict__ arr, const double *__restrict__ src,
+                                               size_t blk_src, const wh_box *__restrict__ boxes,
+                                               ArrGeo A, int g, int dim)
+{
+    const wh_box B = boxes[blockIdx.x];
+    const int n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1, n2 = B.hi[2] - B.lo[2] + 1;
+    const int npts = n0 * n1 * n2;
+    for (int e = threadIdx.x; e < npts * A.neq; e += blockDim.x) {
+        const int c = e / npts, r = e - c * npts;
+        const int ix = B.lo[0] + r % n0, iy = B.lo[1] + (r / n0) % n1, iz = B.lo[2] + r / (n0 * n1);
+        const bool sc = (((ix - g) & 1) == 0) && (((iy - g) & 1) == 0) && (dim == 2 || ((iz - g) & 1) == 0);
+        const size_t q = c * A.cs + ((size_t)iz * A.ny + iy) * A.nx + ix;
+        if (B.op == WH_BOX_ZERO_WC) {
+            if (!sc) arr[(size_t)B.hvy * A.blk_out + q] = 0.0;
+        } else if (sc) {
+            arr[(size_t)B.hvy * A.blk_out + q] = src[(size_t)B.hvy * blk_src + q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+template <class T>
+int ensure(T *&ptr, size_t &cap, size_t n)
+{
+    if (n <= cap) return WH_OK;
+    if (ptr) WH_HIP(hipFree(ptr));
+    ptr = nullptr;
+    size_t c = std::max<size_t>(n, 2 * cap);
+    WH_HIP(hipMalloc(&ptr, sizeof(T) * c));
+    cap = c;
+    return WH_OK;
+}
+
+int upload_ids(wh_pool *P, const int32_t *ids, int n)
+{
+    WaveletWS &W = P->ws;
+    if (int rc = ensure(W.d_ids, W.ids_cap, (size_t)std::max(n, 1))) return rc;
+    WH_HIP(hipMemcpyAsync(W.d_ids, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
+    // the host buffer may be released after return
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+int check_ids(wh_pool *P, const int32_t *ids, int n)
+{
+    WH_CHECK(n >= 0 && (n == 0 || ids), WH_E_ARG, "block list: bad arguments");
+    for (int k = 0; k < n; ++k)
+        WH_CHECK(ids[k] >= 0 && ids[k] < P->d.nb_max, WH_E_ARG, "block list: hvy id out of range");
+    return WH_OK;
+}
+
+int launch_pass(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
+                const Region &R, int axis, int g, const TermTable &T)
+{
+    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
+    if (npts <= 0 || nb <= 0) return WH_OK;
+    const long tot = npts * P->d.neq * nb;
+    const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
+    const long astride = axis == 0 ? 1 : (axis == 1 ? P->nx : (long)P->nx * P->ny);
+    hipLaunchKernelGGL(k_wpass, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, in, out,
+                       P->ws.d_ids, nb, A, R, axis, astride, g, T);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int launch_copy_region(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
+                       const Region &R)
+{
+    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
+    if (npts <= 0 || nb <= 0) return WH_OK;
+    const long tot = npts * P->d.neq * nb;
+    const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
+    hipLaunchKernelGGL(k_copy_region, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, in,
+                       out, P->ws.d_ids, nb, A, R);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+Region full_region(const wh_pool *P)
+{
+    return Region{{0, 0, 0}, {P->nx, P->ny, P->nz}, {1, 1, 1}};
+}
+
+// Three separable passes (x over an extended row range, then y, then z) of
+// the chains T: x reads `in` and writes `out`; y and z read `out` and go
+// through work slot 0 (in-place along the pass direction).
+int separable(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
+              const TermTable &T, int ext)
+{
+    const int g = P->d.g, dim = P->d.dim;
+    const int B0 = P->d.bs[0], B1 = P->d.bs[1], B2 = dim == 3 ? P->d.bs[2] : 1;
+    const int zlo = dim == 3 ? g - ext : 0, zn = dim == 3 ? B2 + 2 * ext : 1;
+    double *scratch = P->work;  // slot 0
+    Region Rx{{g, g - ext, zlo}, {B0, B1 + 2 * ext, zn}, {1, 1, 1}};
+    if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
+    Region Ry{{g, g, zlo}, {B0, B1, zn}, {1, 1, 1}};
+    if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Ry, 1, g, T)) return rc;
+    if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Ry)) return rc;
+    if (dim == 3) {
+        Region Rz{{g, g, g}, {B0, B1, B2}, {1, 1, 1}};
+        if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Rz, 2, g, T)) return rc;
+        if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Rz)) return rc;
+    }
+    return WH_OK;
+}
+
+// blockFilterXYZ_vct(HD, do_restriction): out = in, then the SC interior
+// points are filtered along x (rows g+lo..g+Bs-1+hi), y, z.
+int filter_passes(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb)
+{
+    const wh_wavelet &w = P->ws.w;
+    const int g = P->d.g, dim = P->d.dim;
+    const int B0 = P->d.bs[0], B1 = P->d.bs[1], B2 = dim == 3 ? P->d.bs[2] : 1;
+    if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
+    if (w.hd_lo == 0 && w.hd_hi == 0 && std::fabs(w.HD[0] - 1.0) <= 1.0e-10) return WH_OK;
+    const TermTable T = filter_terms(w);
+    const int lo = w.hd_lo, hi = w.hd_hi;
+    const int zlo = dim == 3 ? g + lo : 0, zn = dim == 3 ? B2 - lo + hi : 1;
+    double *scratch = P->work;
+    // the parity of every filtered point is even, so chain 0 applies
+    Region Rx{{g, g + lo, zlo}, {B0 / 2, B1 - lo + hi, zn}, {2, 1, 1}};
+    if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
+    Region Ry{{g, g, zlo}, {B0 / 2, B1 / 2, zn}, {2, 2, 1}};
+    if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Ry, 1, g, T)) return rc;
+    if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Ry)) return rc;
+    if (dim == 3) {
+        Region Rz{{g, g, g}, {B0 / 2, B1 / 2, B2 / 2}, {2, 2, 2}};
+        if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Rz, 2, g, T)) return rc;
+        if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Rz)) return rc;
+    }
+    return WH_OK;
+}
+
+}  // namespace
+
+void free_wavelet_ws(wh_pool *P)
+{
+    WaveletWS &W = P->ws;
+    if (W.d_ids) (void)hipFree(W.d_ids);
+    if (W.d_patches) (void)hipFree(W.d_patches);
+    if (W.d_pstart) (void)hipFree(W.d_pstart);
+    if (W.d_boxes) (void)hipFree(W.d_boxes);
+    if (W.d_out) (void)hipFree(W.d_out);
+    W = WaveletWS{};
+}
+
+int run_blocks_op(wh_pool *P, int op, int src_array, int dst_array, const std::vector<int32_t> &ids)
+{
+    const int n = (int)ids.size();
+    if (n == 0) return WH_OK;
+    size_t blk_in = 0, blk_out = 0;
+    int nc_in = 0, nc_out = 0;
+    const double *in = array_ptr(P, src_array, &blk_in, &nc_in);
+    double *out = array_ptr(P, dst_array, &blk_out, &nc_out);
+    WH_CHECK(in && out, WH_E_ARG, "wavelet op: unknown array");
+    WH_CHECK(src_array != dst_array, WH_E_ARG, "wavelet op: source and destination must differ");
+    WH_CHECK(nc_in >= P->d.neq && nc_out >= P->d.neq, WH_E_ARG,
+             "wavelet op: arrays need neq components");
+    WH_CHECK(op == WOP_COPY || P->ws.set, WH_E_STATE, "wavelet op: no wavelet set on the pool");
+    WH_CHECK(op == WOP_COPY || (P->d.n_work_slots >= 1 && src_array != WH_WORK && dst_array != WH_WORK),
+             WH_E_ARG, "wavelet op: work slot 0 is scratch and needed");
+    if (int rc = check_ids(P, ids.data(), n)) return rc;
+    if (int rc = upload_ids(P, ids.data(), n)) return rc;
+    const wh_wavelet &w = P->ws.w;
+    WH_CHECK(op == WOP_COPY || P->d.g >= w.g, WH_E_ARG, "wavelet op: too few ghost nodes for the wavelet");
+    if (op == WOP_COPY) return launch_copy_region(P, in, blk_in, out, blk_out, n, full_region(P));
+    if (op == WOP_FWT)
+        return separable(P, in, blk_in, out, blk_out, n, fwt_terms(w), std::max(w.hd_hi, w.gd_hi));
+    if (op == WOP_IWT) {
+        if (!w.lifted)  // CDFX0 copies the whole block first
+            if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, n, full_region(P))) return rc;
+        return separable(P, in, blk_in, out, blk_out, n, iwt_terms(w), std::max(w.hr_hi, w.gr_hi));
+    }
+    if (op == WOP_FILTER) return filter_passes(P, in, blk_in, out, blk_out, n);
+    set_error("wavelet op: unknown op");
+    return WH_E_ARG;
+}
+
+int run_boxes(wh_pool *P, int array, int src_array, const std::vector<wh_box> &boxes)
+{
+    const int n = (int)boxes.size();
+    if (n == 0) return WH_OK;
+    size_t blk = 0, blk_src = 0;
+    int nc = 0, nc_src = 0;
+    double *arr = array_ptr(P, array, &blk, &nc);
+    const double *src = array_ptr(P, src_array, &blk_src, &nc_src);
+    WH_CHECK(arr && src && nc >= P->d.neq && nc_src >= P->d.neq, WH_E_ARG, "boxes: bad arrays");
+    const int lim[3] = {P->nx, P->ny, P->nz};
+    for (const wh_box &b : boxes) {
+        WH_CHECK(b.hvy >= 0 && b.hvy < P->d.nb_max && (b.op == WH_BOX_ZERO_WC || b.op == WH_BOX_COPY_SC),
+                 WH_E_ARG, "boxes: bad hvy id or op");
+        for (int d = 0; d < 3; ++d)
+            WH_CHECK(b.lo[d] >= 0 && b.hi[d] < lim[d] && b.lo[d] <= b.hi[d], WH_E_ARG,
+                     "boxes: box outside the block");
+    }
+    WaveletWS &W = P->ws;
+    if (int rc = ensure(W.d_boxes, W.box_cap, (size_t)n)) return rc;
+    WH_HIP(hipMemcpyAsync(W.d_boxes, boxes.data(), sizeof(wh_box) * n, hipMemcpyHostToDevice, P->stream));
+    const ArrGeo A{blk_src, blk, P->cs, P->nx, P->ny, P->d.neq};
+    hipLaunchKernelGGL(k_boxes, dim3(n), dim3(256), 0, P->stream, arr, src, blk_src, W.d_boxes, A,
+                       P->d.g, P->d.dim);
+    WH_HIP(hipGetLastError());
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
+               std::vector<double> &detail)
+{
+    const int n = (int)ids.size(), neq = P->d.neq;
+    detail.assign((size_t)n * neq, 0.0);
+    if (n == 0) return WH_OK;
+    WH_CHECK(neq <= 8, WH_E_ARG, "detail: at most 8 components");
+    size_t blk = 0;
+    int nc = 0;
+    const double *arr = array_ptr(P, array, &blk, &nc);
+    WH_CHECK(arr && nc >= neq, WH_E_ARG, "detail: bad array");
+    if (int rc = check_ids(P, ids.data(), n)) return rc;
+    if (int rc = upload_ids(P, ids.data(), n)) return rc;
+    WaveletWS &W = P->ws;
+    if (int rc = ensure(W.d_out, W.out_cap, (size_t)n * neq)) return rc;
+    DetailArgs D{};
+    D.g = P->d.g;
+    D.dim = P->d.dim;
+    D.neq = neq;
+    for (int d = 0; d < 3; ++d) D.bs[d] = d < P->d.dim ? P->d.bs[d] : 1;
+    for (int c = 0; c < 8; ++c) D.groups[c] = (groups && c < neq) ? groups[c] : 1;
+    const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
+    hipLaunchKernelGGL(k_detail, dim3(n), dim3(256), 0, P->stream, arr, W.d_ids, A, D, W.d_out);
+    WH_HIP(hipGetLastError());
+    WH_HIP(hipMemcpyAsync(detail.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost,
+                          P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    // groups: the max over the group's components (threshold_block.f90:86-91)
+    int maxg = 0;
+    for (int c = 0; c < neq; ++c) maxg = std::max(maxg, D.groups[c]);
+    for (int k = 0; k < n; ++k) {
+        double *dk = &detail[(size_t)k * neq];
+        for (int l = 2; l <= maxg; ++l) {
+            double m = -1.0;
+            for (int c = 0; c < neq; ++c)
+                if (D.groups[c] == l) m = std::max(m, dk[c]);
+            for (int c = 0; c < neq; ++c)
+                if (D.groups[c] == l) dk[c] = m;
+        }
+        for (int c = 0; c < neq; ++c)
+            if (D.groups[c] == 0) dk[c] = 0.0;
+    }
+    return WH_OK;
+}
+
+int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<wh_patch> &patches_in)
+{
+    const int n = (int)patches_in.size();
+    if (n == 0) return WH_OK;
+    size_t blk = 0;
+    int nc = 0;
+    double *arr = array_ptr(P, array, &blk, &nc);
+    WH_CHECK(arr && nc >= P->d.neq && blk == P->blk, WH_E_ARG, "patch list: bad target array");
+    SyncSrc S;
+    const int alts[2] = {alt1, alt2};
+    for (int a = 0; a < 2; ++a) {
+        if (alts[a] < 0) continue;
+        int nca = 0;
+        S.alt[a] = array_ptr(P, alts[a], &S.blk_alt[a], &nca);
+        WH_CHECK(S.alt[a] && nca >= P->d.neq, WH_E_ARG, "patch list: bad alternate array");
+    }
+    // validate and order by stage
+    const int lim[3] = {P->nx, P->ny, P->nz};
+    std::vector<wh_patch> p(patches_in);
+    for (const wh_patch &q : p) {
+        const int op = q.op & 0xff, sel = (q.op >> 8) & 3;
+        WH_CHECK(q.recv >= 0 && q.recv < P->d.nb_max && q.send >= 0 && q.send < P->d.nb_max, WH_E_ARG,
+                 "patch list: hvy id out of range");
+        WH_CHECK(op <= WH_PATCH_PREDICT && q.stage >= 1 && q.stage <= 3 && sel <= 2 &&
+                     (sel == 0 || S.alt[sel - 1]),
+                 WH_E_ARG, "patch list: bad op/stage/source");
+        for (int d = 0; d < 3; ++d) {
+            WH_CHECK(q.n[d] >= 1 && q.recv_lo[d] >= 0 && q.recv_lo[d] + q.n[d] <= lim[d], WH_E_ARG,
+                     "patch list: receiver box outside the block");
+            if (op == WH_PATCH_PREDICT) continue;
+            const int st = op == WH_PATCH_RESTRICT ? 2 : 1;
+            WH_CHECK(q.send_lo[d] >= 0 && q.send_lo[d] + st * (q.n[d] - 1) < lim[d], WH_E_ARG,
+                     "patch list: sender box outside the block");
+        }
+        if (op == WH_PATCH_PREDICT) {
+            WH_CHECK(q.aux == 2 || q.aux == 4 || q.aux == 6, WH_E_ARG, "patch list: predictor order");
+            const int N = q.aux / 2;
+            for (int d = 0; d < P->d.dim; ++d) {
+                const int fmin = q.send_lo[d], fmax = q.send_lo[d] + q.n[d] - 1;
+                WH_CHECK(fmin >= 0 && (fmax >> 1) < lim[d], WH_E_ARG, "patch list: prediction box");
+                const int omin = (fmin & 1) ? fmin : fmin + 1, omax = (fmax & 1) ? fmax : fmax - 1;
+                WH_CHECK(omin > omax || ((omin >> 1) - N + 1 >= 0 && (omax >> 1) + N < lim[d]), WH_E_ARG,
+                         "patch list: prediction stencil leaves the coarse block");
+            }
+        }
+    }
+    std::stable_sort(p.begin(), p.end(), [](const wh_patch &a, const wh_patch &b) { return a.stage < b.stage; });
+    std::vector<int64_t> pstart(n + 1, 0);
+    for (int i = 0; i < n; ++i) pstart[i + 1] = pstart[i] + (int64_t)p[i].n[0] * p[i].n[1] * p[i].n[2];
+    WaveletWS &W = P->ws;
+    if (int rc = ensure(W.d_patches, W.patch_cap, (size_t)n)) return rc;
+    if (int rc = ensure(W.d_pstart, W.pstart_cap, (size_t)n + 1)) return rc;
+    WH_HIP(hipMemcpyAsync(W.d_patches, p.data(), sizeof(wh_patch) * n, hipMemcpyHostToDevice, P->stream));
+    WH_HIP(hipMemcpyAsync(W.d_pstart, pstart.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
+                          P->stream));
+    int s0 = 0;
+    while (s0 < n) {
+        int s1 = s0;
+        while (s1 < n && p[s1].stage == p[s0].stage) ++s1;
+        if (int rc = launch_sync_flat(P, arr, W.d_patches, W.d_pstart, s0, s1, pstart[s1] - pstart[s0], S))
+            return rc;
+        s0 = s1;
+    }
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+}  // namespace wh
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int wh_pool_set_wavelet(wh_pool *pool, const wh_wavelet *w)
+{
+    WH_CHECK(pool && w, WH_E_ARG, "wh_pool_set_wavelet: null argument");
+    WH_CHECK(w->hd_lo >= -16 && w->hd_hi <= 16 && w->gd_lo >= -16 && w->gd_hi <= 16 &&
+                 w->hr_lo >= -16 && w->hr_hi <= 16 && w->gr_lo >= -16 && w->gr_hi <= 16 &&
+                 w->hd_hi - w->hd_lo + 1 <= 23 && w->gr_hi - w->gr_lo + 1 <= 23,
+             WH_E_ARG, "wh_pool_set_wavelet: filter support too large");
+    WH_CHECK(pool->d.g >= w->g, WH_E_ARG, "wh_pool_set_wavelet: the pool has fewer ghost nodes than the wavelet needs");
+    pool->ws.w = *w;
+    pool->ws.set = true;
+    return WH_OK;
+}
+
+int wh_fwt_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *ids, int n)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || ids), WH_E_ARG, "wh_fwt_blocks: bad argument");
+    return wh::run_blocks_op(pool, wh::WOP_FWT, src_array, dst_array, std::vector<int32_t>(ids, ids + n));
+}
+
+int wh_iwt_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *ids, int n)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || ids), WH_E_ARG, "wh_iwt_blocks: bad argument");
+    return wh::run_blocks_op(pool, wh::WOP_IWT, src_array, dst_array, std::vector<int32_t>(ids, ids + n));
+}
+
+int wh_filter_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *ids, int n)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || ids), WH_E_ARG, "wh_filter_blocks: bad argument");
+    return wh::run_blocks_op(pool, wh::WOP_FILTER, src_array, dst_array,
+                             std::vector<int32_t>(ids, ids + n));
+}
+
+int wh_copy_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *ids, int n)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || ids), WH_E_ARG, "wh_copy_blocks: bad argument");
+    return wh::run_blocks_op(pool, wh::WOP_COPY, src_array, dst_array, std::vector<int32_t>(ids, ids + n));
+}
+
+int wh_detail_blocks(wh_pool *pool, int array, const int32_t *ids, int n, const int32_t *groups,
+                     double *detail)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || (ids && detail)), WH_E_ARG, "wh_detail_blocks: bad argument");
+    std::vector<double> d;
+    if (int rc = wh::run_detail(pool, array, std::vector<int32_t>(ids, ids + n), groups, d)) return rc;
+    std::copy(d.begin(), d.end(), detail);
+    return WH_OK;
+}
+
+int wh_apply_boxes(wh_pool *pool, int array, int src_array, const wh_box *boxes, int n)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || boxes), WH_E_ARG, "wh_apply_boxes: bad argument");
+    return wh::run_boxes(pool, array, src_array, std::vector<wh_box>(boxes, boxes + n));
+}
+
+int wh_run_patches(wh_pool *pool, int array, int alt1, int alt2, const wh_patch *patches, int n)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || patches), WH_E_ARG, "wh_run_patches: bad argument");
+    return wh::run_patch_list(pool, array, alt1, alt2, std::vector<wh_patch>(patches, patches + n));
+}
+
+}  // extern "C"

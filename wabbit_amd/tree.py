@@ -121,11 +121,17 @@ class Forest:
         return tc, lv, hv, x
 
 
-def refine_on_device(pool, forest, hvy_ids, order, g_rhs, array=L.WH_BLOCK):
+def refine_on_device(pool, forest, hvy_ids, order, g_rhs, array=L.WH_BLOCK, presync="table"):
     """refine_tree's execution step for the given blocks: sync the full ghost
     layer (sync_ghosts_tree), refine on the device, then update the pool's
-    tables."""
-    pool.sync(array, 1)
+    tables.  presync "table" runs the pool's full-depth patch table (plain
+    restriction), "tree" the forest sync with HD-filtered restriction of
+    lifted wavelets (wh_sync_tree), None skips the sync."""
+    if presync == "table":
+        pool.sync(array, 1)
+    elif presync == "tree":
+        from .wavelets import sync_tree
+        sync_tree(pool, forest, array)
     ids = [int(h) for h in hvy_ids]
     nd = 1 << forest.dim
     done = []

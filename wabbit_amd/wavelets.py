@@ -1,0 +1,168 @@
+"""CDF wavelets and adapt_tree over the C ABI (wh_wavelet_setup, wh_*_blocks,
+wh_apply_boxes, wh_run_patches, wh_adapt_tree, wh_sync_tree).
+
+Mirrors the reference's setup_wavelet (LIB/WAVELETS/module_wavelets.f90:1031-1430),
+the per-block transforms of wavelet_decomposition_reconstruction.f90 and
+adapt_tree (LIB/MESH/adapt_tree.f90:11-255).  There is no CPU fallback: every
+call goes to the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import lib as L
+
+_TAPS = 33
+_vp = ctypes.c_void_p
+_BOUND = False
+
+WH_BOX_ZERO_WC, WH_BOX_COPY_SC = 0, 1
+WH_IND_EVERYWHERE, WH_IND_THRESHOLD = 0, 1
+
+
+class WaveletC(ctypes.Structure):
+    _fields_ = [("X", ctypes.c_int), ("Y", ctypes.c_int), ("lifted", ctypes.c_int),
+                ("specialised", ctypes.c_int),
+                ("hd_lo", ctypes.c_int), ("hd_hi", ctypes.c_int), ("gd_lo", ctypes.c_int),
+                ("gd_hi", ctypes.c_int), ("hr_lo", ctypes.c_int), ("hr_hi", ctypes.c_int),
+                ("gr_lo", ctypes.c_int), ("gr_hi", ctypes.c_int),
+                ("HD", ctypes.c_double * _TAPS), ("GD", ctypes.c_double * _TAPS),
+                ("HR", ctypes.c_double * _TAPS), ("GR", ctypes.c_double * _TAPS),
+                ("g", ctypes.c_int), ("g_rhs", ctypes.c_int), ("order", ctypes.c_int),
+                ("Nscl", ctypes.c_int), ("Nscr", ctypes.c_int), ("Nwcl", ctypes.c_int),
+                ("Nwcr", ctypes.c_int), ("Nreconl", ctypes.c_int), ("Nreconr", ctypes.c_int),
+                ("Ndep2l", ctypes.c_int), ("Ndep2r", ctypes.c_int)]
+
+    def filt(self, name):
+        lo, hi = getattr(self, name.lower() + "_lo"), getattr(self, name.lower() + "_hi")
+        return lo, np.array([getattr(self, name)[i] for i in range(hi - lo + 1)])
+
+
+class BoxC(ctypes.Structure):
+    _fields_ = [("hvy", ctypes.c_int32), ("lo", ctypes.c_int32 * 3), ("hi", ctypes.c_int32 * 3),
+                ("op", ctypes.c_int32)]
+
+
+class AdaptParamsC(ctypes.Structure):
+    _fields_ = [("jmin", ctypes.c_int), ("jmax", ctypes.c_int), ("indicator", ctypes.c_int),
+                ("use_ce", ctypes.c_int), ("force_maxlevel_dealiasing", ctypes.c_int),
+                ("eps", ctypes.c_double), ("norm", ctypes.c_double * 8),
+                ("groups", ctypes.c_int32 * 8)]
+
+
+BOX_DTYPE = np.dtype([("hvy", np.int32), ("lo", np.int32, 3), ("hi", np.int32, 3), ("op", np.int32)])
+
+
+def _bind():
+    global _BOUND
+    if _BOUND:
+        return
+    lib = L.load()
+    ip = ctypes.c_int
+    sig = {
+        "wh_wavelet_setup": [ctypes.c_char_p, ip, ctypes.POINTER(WaveletC)],
+        "wh_pool_set_wavelet": [_vp, ctypes.POINTER(WaveletC)],
+        "wh_fwt_blocks": [_vp, ip, ip, _vp, ip],
+        "wh_iwt_blocks": [_vp, ip, ip, _vp, ip],
+        "wh_filter_blocks": [_vp, ip, ip, _vp, ip],
+        "wh_copy_blocks": [_vp, ip, ip, _vp, ip],
+        "wh_detail_blocks": [_vp, ip, _vp, ip, _vp, _vp],
+        "wh_apply_boxes": [_vp, ip, ip, _vp, ip],
+        "wh_run_patches": [_vp, ip, ip, ip, _vp, ip],
+        "wh_adapt_tree": [_vp, _vp, ctypes.POINTER(AdaptParamsC)],
+        "wh_sync_tree": [_vp, _vp, ip],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _BOUND = True
+
+
+def setup(name: str, fd_max_size: int = 0) -> WaveletC:
+    """setup_wavelet for "CDFXY"."""
+    _bind()
+    w = WaveletC()
+    L.check(L.load().wh_wavelet_setup(name.encode(), fd_max_size, ctypes.byref(w)))
+    return w
+
+
+def set_wavelet(pool, w: WaveletC):
+    _bind()
+    L.check(L.load().wh_pool_set_wavelet(pool.h, ctypes.byref(w)))
+
+
+def _ids(ids):
+    return np.ascontiguousarray(ids, np.int32)
+
+
+def _blocks_op(fn, pool, src, dst, ids):
+    _bind()
+    a = _ids(ids)
+    L.check(getattr(L.load(), fn)(pool.h, src, dst, a.ctypes.data if a.size else None, a.size))
+
+
+def fwt_blocks(pool, src, dst, ids):
+    _blocks_op("wh_fwt_blocks", pool, src, dst, ids)
+
+
+def iwt_blocks(pool, src, dst, ids):
+    _blocks_op("wh_iwt_blocks", pool, src, dst, ids)
+
+
+def filter_blocks(pool, src, dst, ids):
+    _blocks_op("wh_filter_blocks", pool, src, dst, ids)
+
+
+def copy_blocks(pool, src, dst, ids):
+    _blocks_op("wh_copy_blocks", pool, src, dst, ids)
+
+
+def detail_blocks(pool, array, ids, groups, neq):
+    _bind()
+    a = _ids(ids)
+    g = np.ascontiguousarray(groups, np.int32)
+    out = np.zeros((a.size, neq))
+    L.check(L.load().wh_detail_blocks(pool.h, array, a.ctypes.data if a.size else None, a.size,
+                                      g.ctypes.data, out.ctypes.data))
+    return out
+
+
+def apply_boxes(pool, array, src_array, boxes):
+    _bind()
+    b = np.ascontiguousarray(boxes, BOX_DTYPE)
+    L.check(L.load().wh_apply_boxes(pool.h, array, src_array, b.ctypes.data if b.size else None, b.size))
+
+
+def run_patches(pool, array, patches, alt1=-1, alt2=-1):
+    _bind()
+    p = np.ascontiguousarray(patches, L.PATCH_DTYPE)
+    L.check(L.load().wh_run_patches(pool.h, array, alt1, alt2, p.ctypes.data if p.size else None, p.size))
+
+
+def adapt_params(jmin, jmax, indicator=WH_IND_THRESHOLD, use_ce=True, eps=1e-3, norm=None,
+                 groups=None, force_maxlevel_dealiasing=False, neq=1) -> AdaptParamsC:
+    p = AdaptParamsC()
+    p.jmin, p.jmax, p.indicator = jmin, jmax, indicator
+    p.use_ce = int(use_ce)
+    p.force_maxlevel_dealiasing = int(force_maxlevel_dealiasing)
+    p.eps = eps
+    for c in range(8):
+        p.norm[c] = 1.0 if norm is None or c >= len(norm) else float(norm[c])
+        p.groups[c] = 1 if groups is None or c >= len(groups) else int(groups[c])
+    return p
+
+
+def adapt_tree(pool, forest, params: AdaptParamsC):
+    """adapt_tree on the forest's leaves in WH_BLOCK; the forest's block list
+    becomes the new leaf list (re-attach the pool's tables afterwards)."""
+    _bind()
+    L.check(L.load().wh_adapt_tree(pool.h, forest.h, ctypes.byref(params)))
+
+
+def sync_tree(pool, forest, array=L.WH_BLOCK):
+    """sync_ghosts_tree with filtered restriction for lifted wavelets."""
+    _bind()
+    L.check(L.load().wh_sync_tree(pool.h, forest.h, array))
