@@ -213,12 +213,47 @@ class OrWavelet(ctypes.Structure):
         return lo, np.array([getattr(self, name)[i] for i in range(hi - lo + 1)])
 
 
+class OrMaskParams(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_int), ("x_cntr", ctypes.c_double * 3), ("R_cyl", ctypes.c_double),
+                ("smoothing_width", ctypes.c_double), ("smoothing_safety", ctypes.c_double),
+                ("use_sponge", ctypes.c_int), ("sponge_type", ctypes.c_int),
+                ("L_sponge", ctypes.c_double), ("p_sponge", ctypes.c_double),
+                ("domain", ctypes.c_double * 3)]
+
+
 class OrAdaptParams(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("jmin", ctypes.c_int), ("jmax", ctypes.c_int),
                 ("g", ctypes.c_int), ("neq", ctypes.c_int), ("bs", ctypes.c_int * 3),
                 ("indicator", ctypes.c_int), ("use_ce", ctypes.c_int),
                 ("force_maxlevel_dealiasing", ctypes.c_int), ("eps", ctypes.c_double),
-                ("norm", ctypes.c_double * 8), ("groups", ctypes.c_int * 8)]
+                ("norm", ctypes.c_double * 8), ("groups", ctypes.c_int * 8),
+                ("eps_normalized", ctypes.c_int), ("threshold_mask", ctypes.c_int),
+                ("security_zone", ctypes.c_int), ("domain", ctypes.c_double * 3),
+                ("mask", OrMaskParams)]
+
+
+def mask_params(geometry=0, x_cntr=(0, 0, 0), R_cyl=0.5, smoothing_width=0.0, smoothing_safety=0.0,
+                use_sponge=False, sponge_type=0, L_sponge=0.0, p_sponge=20.0, domain=(1, 1, 1)):
+    m = OrMaskParams()
+    m.geometry = geometry
+    for i in range(3):
+        m.x_cntr[i] = x_cntr[i]
+        m.domain[i] = domain[i]
+    m.R_cyl, m.smoothing_width, m.smoothing_safety = R_cyl, smoothing_width, smoothing_safety
+    m.use_sponge, m.sponge_type = int(use_sponge), sponge_type
+    m.L_sponge, m.p_sponge = L_sponge, p_sponge
+    return m
+
+
+def mask_block(mp, dim, bs, g, x0, dx):
+    _wlib()
+    nx = bs[0] + 2 * g
+    nz = bs[2] + 2 * g if dim == 3 else 1
+    out = np.zeros((6, nz, bs[1] + 2 * g, nx))
+    x0a = np.asarray(list(x0) + [0.0] * (3 - len(x0)), np.float64)
+    dxa = np.asarray(list(dx) + [0.0] * (3 - len(dx)), np.float64)
+    lib().or_mask_block(ctypes.byref(mp), dim, _i(_blk(dim, bs)), g, _d(x0a), _d(dxa), _d(out))
+    return out
 
 
 def _wlib():
@@ -240,6 +275,13 @@ def _wlib():
         for fn in ("or_tree_sync_full", "or_tree_refine_everywhere", "or_tree_adapt"):
             getattr(L, fn).argtypes = [vp]
             getattr(L, fn).restype = ctypes.c_int
+        L.or_mask_block.argtypes = [ctypes.POINTER(OrMaskParams), ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp]
+        L.or_tree_dt.argtypes = [vp, ctypes.POINTER(OrAcmParams)]
+        L.or_tree_dt.restype = ctypes.c_double
+        L.or_tree_rk_step.argtypes = [vp, ctypes.POINTER(OrAcmParams), ctypes.POINTER(OrMaskParams), _dp,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.or_tree_rk_step.restype = ctypes.c_int
+        L.or_tree_get_mask.argtypes = [vp, ctypes.POINTER(OrMaskParams), _dp]
         L._wavelet_bound = True
     return L
 
@@ -288,9 +330,17 @@ class Tree:
     """A periodic forest owned by the oracle (adapt_tree, syncs, refinement)."""
 
     def __init__(self, w, dim, jmin, jmax, bs, g, treecodes, levels, hvy, indicator=0, use_ce=None,
-                 eps=1e-3, norm=None, groups=None, force_maxlevel_dealiasing=False):
+                 eps=1e-3, norm=None, groups=None, force_maxlevel_dealiasing=False, eps_normalized=False,
+                 threshold_mask=False, security_zone=None, domain=(1.0, 1.0, 1.0), mask=None):
         p = OrAdaptParams()
         p.dim, p.jmin, p.jmax, p.g = dim, jmin, jmax, g
+        p.eps_normalized = int(eps_normalized)
+        p.threshold_mask = int(threshold_mask)
+        p.security_zone = int(w.lifted if security_zone is None else security_zone)
+        for i in range(3):
+            p.domain[i] = domain[i] if i < len(domain) else 1.0
+        if mask is not None:
+            p.mask = mask
         hvy = np.ascontiguousarray(hvy, np.float64)
         p.neq = hvy.shape[1]
         for i, b in enumerate(_blk(dim, bs)):
@@ -331,9 +381,25 @@ class Tree:
         self._ck(_wlib().or_tree_adapt(self.h), "or_tree_adapt")
 
     def get(self):
+        """(treecodes, levels, data) sorted by (level, treecode)."""
         n = _wlib().or_tree_size(self.h)
         tc = np.zeros(n, np.int64)
         lv = np.zeros(n, np.int32)
         hv = np.zeros((n,) + tuple(self.shape))
         _wlib().or_tree_get(self.h, tc.ctypes.data, _i(lv), _d(hv))
         return tc, lv, hv
+
+    def get_mask(self, mp):
+        """Mask of the blocks, in get()'s order, shape (n, 6, nz, ny, nx)."""
+        n = _wlib().or_tree_size(self.h)
+        out = np.zeros((n, 6) + tuple(self.shape[1:]))
+        _wlib().or_tree_get_mask(self.h, ctypes.byref(mp), _d(out))
+        return out
+
+    def dt(self, acm_p):
+        return _wlib().or_tree_dt(self.h, ctypes.byref(acm_p))
+
+    def rk_step(self, acm_p, mp, butcher, dt, g_rhs):
+        bt = np.ascontiguousarray(butcher, np.float64)
+        self._ck(_wlib().or_tree_rk_step(self.h, ctypes.byref(acm_p), ctypes.byref(mp) if mp is not None else None,
+                                         _d(bt), bt.shape[0], dt, g_rhs), "or_tree_rk_step")

@@ -926,18 +926,171 @@ static int reconstruct_tree(or_tree *T)
     return rc;
 }
 
+/* get_block_spacing_origin: dx = 2^-J L / Bs, x0 = ijk Bs dx */
+static void block_x0dx(const or_tree *T, int b, double x0[3], double dx[3])
+{
+    int ijk[3];
+    ijk_of(T, T->tc[b], T->level[b], ijk);
+    for (int d = 0; d < 3; ++d) {
+        x0[d] = dx[d] = 0.0;
+        if (d >= T->dim) continue;
+        dx[d] = ldexp(1.0, -T->level[b]) * T->p.domain[d] / (double)T->p.bs[d];
+        x0[d] = (double)(ijk[d] * T->p.bs[d]) * dx[d];
+    }
+}
+
+/* Linfty detail of a box (threshold_block with indices), SCs zeroed by the
+ * global parity, grouped components as in or_detail_block */
+static void detail_box(const or_tree *T, const double *u, const int lo[3], const int hi[3], double *det)
+{
+    const int g = T->g;
+    int maxgrp = 0;
+    for (int c = 0; c < T->neq; ++c) maxgrp = T->p.groups[c] > maxgrp ? T->p.groups[c] : maxgrp;
+    double m[8];
+    for (int c = 0; c < T->neq; ++c) {
+        m[c] = 0.0;
+        for (int iz = lo[2]; iz <= hi[2]; ++iz)
+            for (int iy = lo[1]; iy <= hi[1]; ++iy)
+                for (int ix = lo[0]; ix <= hi[0]; ++ix) {
+                    if (is_sc(T, ix, iy, iz)) continue;
+                    const double v = u[c * T->cs + ((size_t)iz * T->ny + iy) * T->nx + ix];
+                    const double a = T->p.groups[c] >= 2 ? sqrt(v * v) : fabs(v);
+                    if (a > m[c]) m[c] = a;
+                }
+        (void)g;
+    }
+    for (int c = 0; c < T->neq; ++c) det[c] = T->p.groups[c] == 1 ? m[c] : 0.0;
+    for (int l = 2; l <= maxgrp; ++l) {
+        double mm = -1.0;
+        for (int c = 0; c < T->neq; ++c)
+            if (T->p.groups[c] == l && m[c] > mm) mm = m[c];
+        for (int c = 0; c < T->neq; ++c)
+            if (T->p.groups[c] == l) det[c] = mm;
+    }
+}
+
+/* componentWiseNorm_tree(hvy_tmp, "Linfty") over the leaves
+ * (componentWiseNorm_tree.f90), then norm <= 1e-9 -> 1
+ * (coarseningIndicator_tree.f90:160-163) */
+static void leaf_linfty(const or_tree *T, double *norm)
+{
+    const int g = T->g;
+    const int z0 = T->dim == 3 ? g : 0, z1 = T->dim == 3 ? g + T->p.bs[2] : 1;
+    for (int c = 0; c < T->neq; ++c) norm[c] = -1.0;
+    int maxgrp = 0;
+    for (int c = 0; c < T->neq; ++c) maxgrp = T->p.groups[c] > maxgrp ? T->p.groups[c] : maxgrp;
+    for (int b = 0; b < T->n; ++b) {
+        if (!is_leaf(T, b, 0)) continue;
+        double m[8];
+        for (int c = 0; c < T->neq; ++c) {
+            m[c] = 0.0;
+            for (int iz = z0; iz < z1; ++iz)
+                for (int iy = g; iy < g + T->p.bs[1]; ++iy)
+                    for (int ix = g; ix < g + T->p.bs[0]; ++ix) {
+                        const double a = fabs(T->t[b][c * T->cs + ((size_t)iz * T->ny + iy) * T->nx + ix]);
+                        if (a > m[c]) m[c] = a;
+                    }
+        }
+        for (int c = 0; c < T->neq; ++c)
+            if (T->p.groups[c] == 1 && m[c] > norm[c]) norm[c] = m[c];
+        for (int l = 2; l <= maxgrp; ++l) {
+            double mm = 0.0;
+            for (int c = 0; c < T->neq; ++c)
+                if (T->p.groups[c] == l && m[c] > mm) mm = m[c];
+            for (int c = 0; c < T->neq; ++c)
+                if (T->p.groups[c] == l && mm > norm[c]) norm[c] = mm;
+        }
+    }
+    for (int c = 0; c < T->neq; ++c)
+        if (norm[c] <= 1.0e-9) norm[c] = 1.0;
+}
+
+/* coarseningIndicatorMask_tree (coarseningIndicator_tree.f90:338-422): a
+ * block whose chi is partly in (0,1) or not constant stays */
+static void mask_criterion(or_tree *T)
+{
+    const int g = T->g;
+    double *mk = (double *)malloc(sizeof(double) * T->cs * 6);
+    const int z0 = T->dim == 3 ? g : 0, z1 = T->dim == 3 ? g + T->p.bs[2] : 1;
+    for (int b = 0; b < T->n; ++b) {
+        if (T->p.force_maxlevel_dealiasing && T->level[b] == T->p.jmax) continue;
+        double x0[3], dx[3];
+        block_x0dx(T, b, x0, dx);
+        or_mask_block(&T->p.mask, T->dim, T->p.bs, g, x0, dx, mk);
+        int stay = 0;
+        double mx = -1.0e300, mn = 1.0e300;
+        for (int iz = z0; iz < z1; ++iz)
+            for (int iy = g; iy < g + T->p.bs[1]; ++iy)
+                for (int ix = g; ix < g + T->p.bs[0]; ++ix) {
+                    const double v = mk[((size_t)iz * T->ny + iy) * T->nx + ix];
+                    if (v > 1.0e-12 && v < 1.0 - 1.0e-12) stay = 1;
+                    mx = v > mx ? v : mx;
+                    mn = v < mn ? v : mn;
+                }
+        if (mx - mn > 1.0e-12) stay = 1;
+        const int ms = stay ? 0 : -1;
+        if (ms > T->status[b]) T->status[b] = ms;
+    }
+    free(mk);
+}
+
+/* addSecurityZone_CE_tree (securityZone_tree.f90): a block that would
+ * coarsen stays if a same-level neighbour that stays has significant
+ * details within Nwc of their common boundary */
+static void security_zone(or_tree *T, const double *norm)
+{
+    int dirs[26][3];
+    const int nd = nd_dirs(T, dirs);
+    char *flag = (char *)calloc((size_t)T->n * 26, 1);
+    for (int b = 0; b < T->n; ++b) {
+        if (T->level[b] == T->p.jmax && T->p.force_maxlevel_dealiasing) continue;
+        if (T->status[b] != 0) continue;
+        for (int q = 0; q < nd; ++q) {
+            const int s = nb_same(T, b, dirs[q]);
+            if (s < 0 || T->status[s] != -1) continue;
+            int lo[3], hi[3];
+            modify_box(T, dirs[q], T->w.Nwcl, T->w.Nwcr, lo, hi);
+            double det[8];
+            detail_box(T, T->u[b], lo, hi, det);
+            int all = 1;
+            for (int c = 0; c < T->neq; ++c)
+                if (!(det[c] <= T->p.eps * norm[c])) all = 0;
+            if (!all) flag[(size_t)b * 26 + q] = 1;
+        }
+    }
+    char *keep = (char *)calloc((size_t)T->n, 1);
+    for (int b = 0; b < T->n; ++b) {
+        if (T->status[b] != -1) continue;
+        for (int q = 0; q < nd; ++q) {
+            const int s = nb_same(T, b, dirs[q]);
+            if (s < 0) continue;
+            /* the neighbour's patch toward b is its direction -o */
+            int inv = -1;
+            for (int r = 0; r < nd; ++r)
+                if (dirs[r][0] == -dirs[q][0] && dirs[r][1] == -dirs[q][1] && dirs[r][2] == -dirs[q][2]) inv = r;
+            if (flag[(size_t)s * 26 + inv]) keep[b] = 1;
+        }
+    }
+    for (int b = 0; b < T->n; ++b)
+        if (keep[b]) T->status[b] = 0;
+    free(keep);
+    free(flag);
+}
+
 int or_tree_adapt(or_tree *T)
 {
-    const or_wavelet *w = &T->w;
     const int stay = REF_UNSIGNIFICANT_STAY;
     int rc = decompose_full_tree(T);
     if (rc) return rc;
-    /* coarseningIndicator_tree */
+    /* coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331) */
     for (int b = 0; b < T->n; ++b) T->status[b] = 0;
+    double norm[8];
+    for (int c = 0; c < 8; ++c) norm[c] = T->p.norm[c];
     if (T->p.indicator == OR_IND_EVERYWHERE) {
         for (int b = 0; b < T->n; ++b)
             if (is_leaf(T, b, 0)) T->status[b] = -1;
     } else {
+        if (T->p.eps_normalized) leaf_linfty(T, norm);
         double det[8];
         for (int b = 0; b < T->n; ++b) {
             if (T->p.force_maxlevel_dealiasing && T->level[b] == T->p.jmax) {
@@ -947,9 +1100,11 @@ int or_tree_adapt(or_tree *T)
             or_detail_block(T->dim, T->p.bs, T->g, T->neq, T->p.groups, T->u[b], det);
             int all = 1;
             for (int c = 0; c < T->neq; ++c)
-                if (!(det[c] <= T->p.eps * T->p.norm[c])) all = 0;
+                if (!(det[c] <= T->p.eps * norm[c])) all = 0;
             T->status[b] = all ? -1 : 0;
         }
+        if (T->p.threshold_mask) mask_criterion(T);
+        if (T->p.security_zone) security_zone(T, norm);
     }
     /* respectJmaxJmin_tree */
     for (int b = 0; b < T->n; ++b) {
@@ -979,4 +1134,122 @@ int or_tree_adapt(or_tree *T)
     }
     for (int b = 0; b < T->n; ++b) T->status[b] = 0;
     return or_tree_sync_full(T);
+}
+
+/* ---- ACM time step on the leaves ------------------------------------------------ */
+double or_tree_dt(or_tree *T, const or_acm_params *acm)
+{
+    double dt = 9.0e9;
+    for (int b = 0; b < T->n; ++b) {
+        double x0[3], dx[3];
+        block_x0dx(T, b, x0, dx);
+        const double d = or_dt_block(acm, T->p.bs, T->g, dx, T->u[b]);
+        if (d < dt) dt = d;
+    }
+    return dt;
+}
+
+static void masks_in_list_order(or_tree *T, const or_mask_params *mp, double *mask)
+{
+    for (int b = 0; b < T->n; ++b) {
+        double x0[3], dx[3];
+        block_x0dx(T, b, x0, dx);
+        or_mask_block(mp, T->dim, T->p.bs, T->g, x0, dx, mask + (size_t)b * T->cs * 6);
+    }
+}
+
+/* in or_tree_get's order (level, treecode) */
+void or_tree_get_mask(or_tree *T, const or_mask_params *mp, double *mask)
+{
+    int *ord = (int *)malloc(sizeof(int) * (T->n ? T->n : 1));
+    for (int b = 0; b < T->n; ++b) ord[b] = b;
+    for (int i = 1; i < T->n; ++i) {
+        const int v = ord[i];
+        int j = i - 1;
+        while (j >= 0 && cmp_idx_T_level_tc(T, ord[j], v) > 0) {
+            ord[j + 1] = ord[j];
+            --j;
+        }
+        ord[j + 1] = v;
+    }
+    for (int i = 0; i < T->n; ++i) {
+        double x0[3], dx[3];
+        block_x0dx(T, ord[i], x0, dx);
+        or_mask_block(mp, T->dim, T->p.bs, T->g, x0, dx, mask + (size_t)i * T->cs * 6);
+    }
+    free(ord);
+}
+
+/* RungeKuttaGeneric (runge_kutta_generic.f90:1-156) on the leaves, the RHS
+ * ghosts filled by sync_ghosts_RHS_tree (plain restriction) of depth g_rhs */
+int or_tree_rk_step(or_tree *T, const or_acm_params *acm, const or_mask_params *mp, const double *bt,
+                    int nrows, double dt, int g_rhs)
+{
+    const int n = T->n, g = T->g;
+    const size_t bsz = T->bsz, cs = T->cs;
+    const int z0 = T->dim == 3 ? g : 0, z1 = T->dim == 3 ? g + T->p.bs[2] : 1;
+    double *work = (double *)malloc(sizeof(double) * bsz * (size_t)n * (size_t)nrows);
+    double *mask = mp ? (double *)malloc(sizeof(double) * cs * 6 * (size_t)n) : NULL;
+    double *dxs = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    for (int b = 0; b < n; ++b) {
+        double x0[3];
+        block_x0dx(T, b, x0, dxs + 3 * b);
+    }
+    if (mp) masks_in_list_order(T, mp, mask);
+    sync_spec S = {0};
+    S.same = S.restrict_ = S.predict = 1;
+    S.src_mode = SRC_U;
+    S.filter = 0;
+    S.gm = S.gp = g_rhs;
+#define SLOT(j, b) (work + ((size_t)(j) * n + (b)) * bsz)
+#define BT(r, c) bt[(r) * nrows + (c)]
+#define FOR_INT(body)                                                              \
+    for (int c = 0; c < T->neq; ++c)                                                \
+        for (int iz = z0; iz < z1; ++iz)                                            \
+            for (int iy = g; iy < g + T->p.bs[1]; ++iy)                             \
+                for (int ix = g; ix < g + T->p.bs[0]; ++ix) {                       \
+                    const size_t i = c * cs + ((size_t)iz * T->ny + iy) * T->nx + ix; \
+                    body;                                                           \
+                }
+    int rc = tree_sync(T, &S);
+    for (int b = 0; b < n && !rc; ++b) {
+        double *u = T->u[b], *w1 = SLOT(0, b);
+        FOR_INT(w1[i] = u[i]);
+        or_rhs_acm_block(acm, T->p.bs, g, dxs + 3 * b, u, SLOT(1, b), mask ? mask + (size_t)b * cs * 6 : NULL);
+    }
+    for (int j = 2; j <= nrows - 1 && !rc; ++j) {
+        for (int b = 0; b < n; ++b) {
+            double *u = T->u[b], *w1 = SLOT(0, b);
+            FOR_INT(u[i] = w1[i]);
+            for (int l = 2; l <= j; ++l) {
+                const double a = BT(j - 1, l - 1);
+                if (fabs(a) < 1.0e-8) continue;
+                const double *k = SLOT(l - 1, b);
+                const double da = dt * a;
+                FOR_INT(u[i] = u[i] + da * k[i]);
+            }
+        }
+        rc = tree_sync(T, &S);
+        for (int b = 0; b < n && !rc; ++b)
+            or_rhs_acm_block(acm, T->p.bs, g, dxs + 3 * b, T->u[b], SLOT(j, b),
+                             mask ? mask + (size_t)b * cs * 6 : NULL);
+    }
+    for (int b = 0; b < n && !rc; ++b) {
+        double *u = T->u[b], *w1 = SLOT(0, b);
+        FOR_INT(u[i] = w1[i]);
+        for (int j = 2; j <= nrows; ++j) {
+            const double bb = BT(nrows - 1, j - 1);
+            if (fabs(bb) < 1.0e-8) continue;
+            const double *k = SLOT(j - 1, b);
+            const double db = dt * bb;
+            FOR_INT(u[i] = u[i] + db * k[i]);
+        }
+    }
+#undef SLOT
+#undef BT
+#undef FOR_INT
+    free(work);
+    free(mask);
+    free(dxs);
+    return rc;
 }

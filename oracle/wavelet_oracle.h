@@ -24,6 +24,8 @@
 
 #include <stdint.h>
 
+#include "wabbit_oracle.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -70,6 +72,22 @@ void or_filter_block(const or_wavelet *w, int dim, const int bs[3], int g, int n
 void or_detail_block(int dim, const int bs[3], int g, int neq, const int *groups,
                      const double *u, double *detail);
 
+/* ---- penalisation mask (create_mask_2D_ACM, create_mask.f90:183-320) --- */
+typedef struct {
+    int geometry;                   /* 0 none, 1 cylinder (a circle in 2D) */
+    double x_cntr[3], R_cyl;
+    double smoothing_width, smoothing_safety;   /* cosine smoothing */
+    int use_sponge, sponge_type;    /* sponge_type 0: rect, 1: p-norm */
+    double L_sponge, p_sponge;
+    double domain[3];
+} or_mask_params;
+
+/* CREATE_MASK_meta "all-parts" for one block: mask[6][nz][ny][nx] = chi,
+ * u_s (3), colour, sponge (module_physics_metamodule.f90:38-84,
+ * draw_circle module_geometry.f90:315-381, sponge_2D sponge.f90:4-58). */
+void or_mask_block(const or_mask_params *mp, int dim, const int bs[3], int g, const double x0[3],
+                   const double dx[3], double *mask);
+
 /* ---- adapt_tree on a periodic leaf forest ----------------------------- */
 enum { OR_IND_EVERYWHERE = 0, OR_IND_THRESHOLD = 1 };
 
@@ -82,6 +100,11 @@ typedef struct {
     double eps;
     double norm[8];                 /* eps * norm per component */
     int groups[8];                  /* threshold_state_vector_component */
+    int eps_normalized;             /* norm = Linfty of the leaves (componentWiseNorm_tree) */
+    int threshold_mask;             /* coarseningIndicatorMask_tree */
+    int security_zone;              /* addSecurityZone_CE_tree */
+    double domain[3];
+    or_mask_params mask;
 } or_adapt_params;
 
 /* A forest with heavy data that the oracle owns.  n blocks in list order. */
@@ -100,6 +123,14 @@ int or_tree_sync_full(or_tree *T);
 int or_tree_refine_everywhere(or_tree *T);
 /* adapt_tree (adapt_tree.f90:11-255) without the final load balancing */
 int or_tree_adapt(or_tree *T);
+/* the ACM time step on the leaves: min block dt (before the host clamps)
+ * and one RungeKuttaGeneric step with sync_ghosts_RHS_tree of depth g_rhs
+ * and the mask of every leaf rebuilt from mp (mp NULL: no mask) */
+double or_tree_dt(or_tree *T, const or_acm_params *acm);
+int or_tree_rk_step(or_tree *T, const or_acm_params *acm, const or_mask_params *mp,
+                    const double *butcher, int nrows, double dt, int g_rhs);
+/* the mask (chi) of the leaves, layout as the data: [n][6][...] */
+void or_tree_get_mask(or_tree *T, const or_mask_params *mp, double *mask);
 
 #ifdef __cplusplus
 }

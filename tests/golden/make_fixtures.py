@@ -124,7 +124,26 @@ def wavelets():
         _save(os.path.join(HERE, f"wavelet_coarsen_{wav}.npz"), {"vor": f.blocks}, f, {"wavelet": wav})
 
 
+def acm_cylinder():
+    """TESTING/acm/acm_CDF44: adaptive penalised cylinder with a p-norm sponge
+    (acm_cyl.ini: Bs=26, Jmax=6, CDF44, eps=1e-3 normalised, mask
+    thresholding, max-level dealiasing).  The saved state at t = 0, 0.05 and
+    0.1 (ux, uy, p and the mask chi), one fixture per time."""
+    d = "/root/reference/TESTING/acm/acm_CDF44"
+    params = dict(Bs=26, Jmax=6, Jmin=1, domain=20.0, wavelet="CDF44", eps=1e-3, eps_normalized=True,
+                  eps_norm="Linfty", threshold_mask=True, force_maxlevel_dealiasing=True,
+                  c_0=12.5, nu=0.0, gamma_p=0.0, u_mean_set=[0.0, -1.0, 0.0], C_eta=1.34e-3,
+                  C_sponge=8.0e-3, L_sponge=2.0, p_sponge=8.0, sponge_type="p-norm",
+                  x_cntr=[10.0, 10.0], R_cyl=0.5, C_smooth=1.5, CFL=1.5, CFL_eta=0.99,
+                  time_max=0.1, write_time=0.05, tsave_stats=0.2, skew_symmetry=False,
+                  scheme="FD_4th_central")
+    for tag, name in (("000000000000", "t0"), ("000000050000", "t005"), ("000000100000", "t010")):
+        fields, meta = _fields(d, ["ux", "uy", "p", "mask"], tag)
+        _save(os.path.join(HERE, f"acm_cyl_{name}.npz"), fields, meta, params)
+
+
 if __name__ == "__main__":
     vortex2d()
     taylor_green()
     wavelets()
+    acm_cylinder()
