@@ -1,28 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the block time step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c3u]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3u]
 
-A "step" is one RK4 time step (4 RK stages over every block of the forest):
-device-side dt reduction and clamps, ghost sync and the fused RHS+RK stage
-kernels, replayed from a hipGraph.  Inputs are resident in HBM before the
-timed region starts.  The metric is block-RK-stages/s = 4 * N_blocks * K / t.
+A "step" is one time step of the workload's main loop.  The metric is
+block-RK-stages/s: the RK4 stages times the blocks each stage swept, summed
+over the K timed steps, divided by the wall time of those steps.  Inputs are
+resident in HBM before the timed region starts.
 
-Workloads
-  c1   2D ACM, periodic unit square, uniform level 4 (256 blocks), Bs=16,
-       CDF40 ghosts (g=3, g_rhs=2), FD_4th_central, RK4, CFL 0.5, c0=20,
-       nu=1e-3; seeded Fourier velocity (|k|<=8, amplitude |k|^-2, u_rms=1)
-       and a Gaussian pressure pulse (sigma 0.05, amplitude 0.1).
-  c3u  3D ACM, periodic unit cube, uniform level 4 (4096 blocks), Bs=24,
-       CDF44 ghosts (g=6, g_rhs=2), FD_4th_central, c0=15, nu=1e-3; seeded
-       solenoidal velocity (1<=|k|<=6, E(k)~k^-5/3, u_rms=1), p=0.  The
-       block size, ghosts and physics of the 3D adaptive workload on a
-       uniform grid sized to exceed the 256 MB last-level cache.
+c2 (default): the adaptive penalised-cylinder channel (2D ACM, Bs=16,
+Jmin=2, Jmax=6, CDF44, eps=1e-3, adapt every step, 500 steps).  A step is
+the reference's main-loop iteration: sync_ghosts_tree, refine everywhere,
+the mask, dt, four fused RHS+RK stages, adapt_tree.  The blocks of a step
+are those of the refined grid the RK stages run on.
 
-Printed: one JSON line with the contract fields plus "roofline" (dominant
-kernel: the fused RHS+RK stage kernel, HIP-event timed on the library's
-stream) and "cpu_baseline" (the oracle's C restatement on the host cores, a
-bounded sample of the same workload).
+c1 / c3u: uniform periodic grids (no adaptivity); a step is device-side
+dt + four ghost syncs and fused RHS+RK stages, replayed from a hipGraph.
+
+The workloads are defined in wabbit_amd/workloads.py.
+
+Printed: one JSON line with the contract fields plus "roofline" (the fused
+RHS+RK stage kernel, HIP-event timed on the library's stream) and
+"cpu_baseline" (the oracle's C restatement on the host cores, a bounded
+sample of the same workload).
 """
 from __future__ import annotations
 
@@ -48,21 +48,8 @@ def b_min(q, bs, dim, g_r, m):
 
 
 def workload(name):
-    from wabbit_amd.params import ACMParams, TimeParams
-    if name == "c1":
-        return dict(name="c1-2d-periodic-J4-Bs16", dim=2, J=4, bs=16, g=3, g_rhs=2, L=1.0,
-                    acm=ACMParams(dim=2, scheme=4, skew_symmetry=False, penalization=False,
-                                  c_0=20.0, nu=1e-3, gamma_p=1.0, CFL=0.5,
-                                  domain_size=[1.0, 1.0, 1.0]),
-                    ic=dict(kmax=8, kmin=1, amp_exponent=-2.0), pressure=True, nsteps_cfg=100)
-    if name == "c3u":
-        return dict(name="c3u-3d-periodic-J4-Bs24", dim=3, J=4, bs=24, g=6, g_rhs=2, L=1.0,
-                    acm=ACMParams(dim=3, scheme=4, skew_symmetry=False, penalization=False,
-                                  c_0=15.0, nu=1e-3, gamma_p=1.0, CFL=0.5,
-                                  domain_size=[1.0, 1.0, 1.0]),
-                    ic=dict(kmax=6, kmin=1, energy_exponent=-5.0 / 3.0, solenoidal=True),
-                    pressure=False, nsteps_cfg=200)
-    raise SystemExit(f"unknown workload {name}")
+    from wabbit_amd import workloads
+    return workloads.get(name)
 
 
 def cpu_baseline(w, hvy, forest, seconds=10.0):
@@ -115,24 +102,154 @@ def calibrate_copy(name, reps=10):
     del ctypes
 
 
+def _torch_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.synchronize
+    except Exception:  # torch is only used for the contract's extra barrier
+        pass
+    return lambda: None
+
+
+def cpu_baseline_adaptive(w, state, seconds=20.0):
+    """The oracle's tree path (serial C) on the host for a bounded number of
+    main-loop iterations of the adaptive workload from the same initial grid."""
+    from oracle import pyoracle as po
+    from wabbit_amd import workloads
+    from wabbit_amd.params import butcher_rk4, clamp_dt
+    cfg = workloads.channel_adapt_config(w)
+    tc, lv, hvy = state
+    wo = po.wavelet(w["wavelet"], fd_max_size=2)
+    tree = po.Tree(wo, w["dim"], w["jmin"], w["jmax"], [w["bs"]] * w["dim"], wo.g, tc, lv, hvy, indicator=1,
+                   eps=w["eps"], groups=cfg.groups, force_maxlevel_dealiasing=cfg.force_maxlevel_dealiasing,
+                   eps_normalized=cfg.eps_normalized, threshold_mask=cfg.threshold_mask,
+                   domain=w["domain"], mask=po.mask_params(**w["mask"]))
+    ap = po.make_params(w["acm"])
+    mp = po.mask_params(**w["mask"])
+    bt = butcher_rk4()
+    t, n, stages = 0.0, 0, 0
+    t0 = time.perf_counter()
+    while True:
+        tree.sync()
+        tree.refine_everywhere()
+        nb = tree.size()
+        dt = clamp_dt(tree.dt(ap), t, w["tp"])
+        tree.rk_step(ap, mp, bt, dt, 2)
+        t += dt
+        tree.adapt()
+        n += 1
+        stages += 4 * nb
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    tree.close()
+    return dict(value=stages / el, unit="block-RK-stages/s", cores=1, kind="port",
+                sample=f"{n} main-loop iterations of the same workload from its initial grid "
+                       f"({len(tc)} blocks), {el:.1f} s, serial")
+
+
+def run_adaptive(args, w):
+    from wabbit_amd import workloads
+    from wabbit_amd.timestep import PATH_FUSED
+    torch_sync = _torch_sync()
+    steps = args.steps if args.steps is not None else w["nsteps_cfg"]
+    warmup = args.warmup
+    t0 = time.perf_counter()
+    s = workloads.channel_solver(w, path=PATH_FUSED, nb_max=16384)
+    print(f"# workload {w['name']}: initial grid {s.nblocks} blocks ({time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
+    init_state = s.state()
+    for _ in range(warmup):
+        s.step()
+    s.pool.synchronize()
+    torch_sync()
+    stages, nb_min, nb_max = 0, 1 << 30, 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.step()
+        stages += 4 * s.rk_blocks
+        nb_min, nb_max = min(nb_min, s.rk_blocks), max(nb_max, s.rk_blocks)
+        if (k + 1) % 100 == 0:
+            print(f"# step {k + 1}: {s.rk_blocks} blocks, t={s.time:.5f}", file=sys.stderr, flush=True)
+    s.pool.synchronize()
+    torch_sync()
+    el = time.perf_counter() - t0
+    value = stages / el
+    print(f"# adaptive loop: {steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+
+    # kernel timing + per-phase pass over a few more steps
+    nprof = max(3, min(20, steps))
+    s.pool.timing(True)
+    s.profile(True)
+    st2 = 0
+    for _ in range(nprof):
+        s.step()
+        st2 += 4 * s.rk_blocks
+    k_ms, k_n = s.pool.kernel_time(0)
+    phases = {k: round(v / nprof, 4) for k, v in s.phase_ms.items()}
+    s.profile(False)
+    s.pool.timing(False)
+    q, dim = w["dim"] + 1, w["dim"]
+    bpb = b_min(q, w["bs"], dim, 2, 5)       # m = 5 mask components read (chi, u_s, colour, sponge)
+    achieved = bpb * st2 / (k_ms * k_n * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_rhs2d<FD4,fused>", avg_launch_ms=round(k_ms, 5), launches=k_n,
+                    bytes_per_block_stage=bpb, blocks_per_launch=round(st2 / max(k_n, 1), 1))
+
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_adaptive(w, init_state, seconds=20.0)
+
+    out = {
+        "metric": "block-RK-stages/s",
+        "value": round(value, 1),
+        "unit": "block-RK-stages/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(el / steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w["name"], "dim": dim, "Bs": w["bs"], "Jmin": w["jmin"], "Jmax": w["jmax"],
+                   "wavelet": w["wavelet"], "eps": w["eps"], "scheme": "FD_4th_central", "rk": "RK4",
+                   "neq": q, "initial_blocks": int(len(init_state[0])),
+                   "rk_blocks_min": nb_min, "rk_blocks_max": nb_max, "time_end": round(s.time, 6)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "phase_ms_per_step": phases,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c1", choices=["c1", "c3u"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3u"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain interior copy of known size (counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
-        calibrate_copy(args.workload)
+        calibrate_copy("c1" if args.workload == "c2" else args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
+              "(there is no multi-GPU decomposition)", file=sys.stderr)
         sys.exit(2)
+    w = workload(args.workload)
+    if w["kind"] == "adaptive":
+        run_adaptive(args, w)
+        return
 
     from wabbit_amd import lib as L
     from wabbit_amd.forest import EquidistantForest
@@ -140,7 +257,6 @@ def main():
     from wabbit_amd.params import TimeParams
     from wabbit_amd.timestep import PATH_FUSED, PATH_PER_BLOCK, PATH_REFERENCE, AcmSolver
 
-    w = workload(args.workload)
     steps = args.steps if args.steps is not None else w["nsteps_cfg"]
     dim = w["dim"]
     forest = EquidistantForest(dim, w["J"], [w["bs"]] * dim, w["g"], [w["L"]] * dim)

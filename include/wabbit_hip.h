@@ -261,6 +261,22 @@ int wh_apply_boxes(wh_pool *pool, int array, int src_array, const wh_box *boxes,
  * (the source-array choice of xfer_block_data.f90:321-437). */
 int wh_run_patches(wh_pool *pool, int array, int alt1, int alt2, const wh_patch *patches, int n);
 
+/* ---- penalisation mask (create_mask_2D_ACM, LIB/EQUATION/ACMnew/create_mask.f90:183-320) */
+typedef struct {
+    int geometry;                    /* 0 none, 1 cylinder (a circle in 2D) */
+    double x_cntr[3], R_cyl;
+    double smoothing_width, smoothing_safety;  /* cosine smoothing (module_ACM.f90:467-471) */
+    int use_sponge, sponge_type;     /* sponge_type 0: "rect", 1: "p-norm" (sponge.f90:4-58) */
+    double L_sponge, p_sponge;
+    double domain[3];
+} wh_mask_params;
+/* CREATE_MASK_meta "all-parts" (module_physics_metamodule.f90:38-84) for the
+ * listed blocks into WH_MASK (6 components: chi, u_s, colour, sponge); x0dx
+ * per listed block as in wh_set_active.  n < 0: all active blocks with the
+ * pool's own origins and spacings. */
+int wh_create_mask(wh_pool *pool, const wh_mask_params *mp, const int32_t *hvy_ids, int n,
+                   const double *x0dx);
+
 /* adapt_tree (LIB/MESH/adapt_tree.f90:11-255) on the forest's leaf blocks in
  * WH_BLOCK: full tree down to jmin, level-wise decomposition, coarsening
  * indicator, gradedness, coarse extension, reconstruction, pruning and the
@@ -276,6 +292,11 @@ typedef struct {
     double eps;
     double norm[8];                 /* per component: threshold eps * norm */
     int32_t groups[8];              /* threshold_state_vector_component */
+    int eps_normalized;             /* norm = Linfty of the leaves (componentWiseNorm_tree) */
+    int threshold_mask;             /* coarseningIndicatorMask_tree, with `mask` */
+    int security_zone;              /* addSecurityZone_CE_tree */
+    double domain[3];
+    wh_mask_params mask;
 } wh_adapt_params;
 int wh_adapt_tree(wh_pool *pool, wh_forest *forest, const wh_adapt_params *p);
 

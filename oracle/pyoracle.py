@@ -380,6 +380,9 @@ class Tree:
     def adapt(self):
         self._ck(_wlib().or_tree_adapt(self.h), "or_tree_adapt")
 
+    def size(self):
+        return _wlib().or_tree_size(self.h)
+
     def get(self):
         """(treecodes, levels, data) sorted by (level, treecode)."""
         n = _wlib().or_tree_size(self.h)

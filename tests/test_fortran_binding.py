@@ -9,6 +9,7 @@ import subprocess
 import pytest
 
 from wabbit_amd import lib as L
+from wabbit_amd import wavelets as W
 
 FLANG = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -28,3 +29,8 @@ def test_fortran_interface_links_and_runs(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     assert "wh_version=100" in out
     assert f"sizeof(wh_pool_desc)={ctypes.sizeof(L.PoolDesc)}" in out
+    assert f"sizeof(wh_wavelet)={ctypes.sizeof(W.WaveletC)}" in out
+    assert f"sizeof(wh_adapt_params)={ctypes.sizeof(W.AdaptParamsC)}" in out
+    assert f"sizeof(wh_mask_params)={ctypes.sizeof(W.MaskParamsC)}" in out
+    assert "CDF44 g=6 g_rhs=2 lifted=1" in out
+    assert "forest_n=4" in out

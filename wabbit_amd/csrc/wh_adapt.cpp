@@ -704,6 +704,143 @@ int reconstruct_tree(Tree &T, const wh_adapt_params &p)
     return WH_OK;
 }
 
+// the max over each group l >= 2 of the per-component values, 0 for group 0
+void combine_groups(double *v, int neq, const int32_t *groups)
+{
+    int maxg = 0;
+    for (int c = 0; c < neq; ++c) maxg = std::max(maxg, (int)groups[c]);
+    for (int l = 2; l <= maxg; ++l) {
+        double m = -1.0;
+        for (int c = 0; c < neq; ++c)
+            if (groups[c] == l) m = std::max(m, v[c]);
+        for (int c = 0; c < neq; ++c)
+            if (groups[c] == l) v[c] = m;
+    }
+    for (int c = 0; c < neq; ++c)
+        if (groups[c] == 0) v[c] = 0.0;
+}
+
+wh_box interior_box(const Tree &T, int i)
+{
+    wh_box b{};
+    b.hvy = T.b[i].hvy;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = d < T.dim ? T.g : 0;
+        b.hi[d] = d < T.dim ? T.g + T.bs[d] - 1 : 0;
+    }
+    return b;
+}
+
+// componentWiseNorm_tree(hvy_tmp, "Linfty") over the leaves
+// (OPERATORS/componentWiseNorm_tree.f90), then norm <= 1e-9 -> 1
+// (coarseningIndicator_tree.f90:160-163)
+int leaf_linfty(Tree &T, const int32_t *groups, double *norm)
+{
+    const int neq = T.P->d.neq;
+    std::vector<wh_box> boxes;
+    for (int i = 0; i < (int)T.b.size(); ++i)
+        if (T.is_leaf(i, false)) boxes.push_back(interior_box(T, i));
+    std::vector<double> m;
+    if (int rc = run_region_max(T.P, A_T, boxes, false, false, groups, m)) return rc;
+    for (int c = 0; c < neq; ++c) norm[c] = -1.0;
+    for (size_t k = 0; k < boxes.size(); ++k) {
+        double *v = &m[k * neq];
+        for (int c = 0; c < neq; ++c)
+            if (groups[c] == 1) norm[c] = std::max(norm[c], v[c]);
+        int maxg = 0;
+        for (int c = 0; c < neq; ++c) maxg = std::max(maxg, (int)groups[c]);
+        for (int l = 2; l <= maxg; ++l) {
+            double mm = 0.0;
+            for (int c = 0; c < neq; ++c)
+                if (groups[c] == l) mm = std::max(mm, v[c]);
+            for (int c = 0; c < neq; ++c)
+                if (groups[c] == l) norm[c] = std::max(norm[c], mm);
+        }
+    }
+    for (int c = 0; c < neq; ++c)
+        if (norm[c] <= 1.0e-9) norm[c] = 1.0;
+    return WH_OK;
+}
+
+// coarseningIndicatorMask_tree (coarseningIndicator_tree.f90:338-422): the
+// mask of every block of the full tree; a block whose chi is partly inside
+// (0,1) or not constant stays
+int mask_criterion(Tree &T, const wh_adapt_params &p)
+{
+    std::vector<int32_t> ids;
+    std::vector<double> x0dx;
+    std::vector<int> idx;
+    for (int i = 0; i < (int)T.b.size(); ++i) {
+        if (p.force_maxlevel_dealiasing && T.b[i].level == p.jmax) continue;
+        idx.push_back(i);
+        ids.push_back(T.b[i].hvy);
+        double xd[6];
+        T.F->x0dx(FBlock{T.b[i].tc, T.b[i].level, T.b[i].hvy, 0}, xd);
+        x0dx.insert(x0dx.end(), xd, xd + 6);
+    }
+    if (int rc = run_mask(T.P, p.mask, ids, x0dx)) return rc;
+    std::vector<double> st;
+    if (int rc = run_mask_stats(T.P, ids, st)) return rc;
+    for (size_t k = 0; k < idx.size(); ++k) {
+        const bool stay = st[3 * k] != 0.0 || (st[3 * k + 1] - st[3 * k + 2]) > 1.0e-12;
+        const int ms = stay ? 0 : -1;
+        int &s = T.b[idx[k]].status;
+        s = std::max(s, ms);
+    }
+    return WH_OK;
+}
+
+// addSecurityZone_CE_tree (securityZone_tree.f90): a block that would coarsen
+// stays if a same-level neighbour that stays has significant details within
+// Nwc of their common boundary
+int security_zone(Tree &T, const wh_adapt_params &p, const double *norm)
+{
+    int dirs[26][3];
+    const int nd = dirs_of(T.dim, dirs);
+    const int neq = T.P->d.neq;
+    std::vector<wh_box> boxes;
+    std::vector<std::pair<int, int>> who;  // (block, direction)
+    for (int i = 0; i < (int)T.b.size(); ++i) {
+        if (T.b[i].level == p.jmax && p.force_maxlevel_dealiasing) continue;
+        if (T.b[i].status != 0) continue;
+        for (int q = 0; q < nd; ++q) {
+            const int s = T.nb_same(i, dirs[q]);
+            if (s < 0 || T.b[s].status != -1) continue;
+            wh_box bx{};
+            bx.hvy = T.b[i].hvy;
+            modify_box(T, dirs[q], T.w.Nwcl, T.w.Nwcr, bx);
+            boxes.push_back(bx);
+            who.push_back({i, q});
+        }
+    }
+    std::vector<double> m;
+    if (int rc = run_region_max(T.P, A_U, boxes, true, true, p.groups, m)) return rc;
+    std::vector<char> flag(T.b.size() * 26, 0);
+    for (size_t k = 0; k < boxes.size(); ++k) {
+        double *v = &m[k * neq];
+        combine_groups(v, neq, p.groups);
+        bool all = true;
+        for (int c = 0; c < neq; ++c)
+            if (!(v[c] <= p.eps * norm[c])) all = false;
+        if (!all) flag[(size_t)who[k].first * 26 + who[k].second] = 1;
+    }
+    std::vector<char> keep(T.b.size(), 0);
+    for (int i = 0; i < (int)T.b.size(); ++i) {
+        if (T.b[i].status != -1) continue;
+        for (int q = 0; q < nd; ++q) {
+            const int s = T.nb_same(i, dirs[q]);
+            if (s < 0) continue;
+            int inv = -1;
+            for (int r = 0; r < nd; ++r)
+                if (dirs[r][0] == -dirs[q][0] && dirs[r][1] == -dirs[q][1] && dirs[r][2] == -dirs[q][2]) inv = r;
+            if (flag[(size_t)s * 26 + inv]) keep[i] = 1;
+        }
+    }
+    for (size_t i = 0; i < T.b.size(); ++i)
+        if (keep[i]) T.b[i].status = 0;
+    return WH_OK;
+}
+
 // leaves of the tree back into the forest's block list
 void store_leaves(Tree &T)
 {
@@ -766,6 +903,10 @@ int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
         for (int i = 0; i < (int)T.b.size(); ++i)
             if (T.is_leaf(i, false)) T.b[i].status = -1;
     } else {
+        double norm[8];
+        for (int c = 0; c < 8; ++c) norm[c] = p->norm[c];
+        if (p->eps_normalized)
+            if (int rc = leaf_linfty(T, p->groups, norm)) return fail(rc);
         std::vector<int> idx;
         for (int i = 0; i < (int)T.b.size(); ++i) {
             if (p->force_maxlevel_dealiasing && T.b[i].level == p->jmax) T.b[i].status = -1;
@@ -777,9 +918,13 @@ int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
         for (size_t k = 0; k < idx.size(); ++k) {
             bool all = true;
             for (int c = 0; c < neq; ++c)
-                if (!(det[k * neq + c] <= p->eps * p->norm[c])) all = false;
+                if (!(det[k * neq + c] <= p->eps * norm[c])) all = false;
             T.b[idx[k]].status = all ? -1 : 0;
         }
+        if (p->threshold_mask)
+            if (int rc = mask_criterion(T, *p)) return fail(rc);
+        if (p->security_zone)
+            if (int rc = security_zone(T, *p, norm)) return fail(rc);
     }
     // respectJmaxJmin_tree (respectJmaxJmin_tree.f90:1-39)
     for (TNode &x : T.b) {

@@ -161,6 +161,14 @@ int run_boxes(wh_pool *P, int array, int src_array, const std::vector<wh_box> &b
 int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
                std::vector<double> &detail);
 enum { WOP_FWT = 0, WOP_IWT = 1, WOP_FILTER = 2, WOP_COPY = 3 };
+// per-box component maxima (|v|, SCs zeroed with zero_sc, sqrt(v*v) for
+// grouped components with sqrt_groups): out[k*neq + c]
+int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
+                   const int32_t *groups, std::vector<double> &out);
+// penalisation mask (wh_mask.hip)
+int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &ids,
+             const std::vector<double> &x0dx);
+int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<double> &stats);
 
 AcmDev make_acm_dev(const wh_acm_params *p);
 int reach_of(int scheme);

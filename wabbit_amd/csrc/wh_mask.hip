@@ -1,0 +1,232 @@
+// wh_mask.hip -- the ACM penalisation mask of listed blocks on the device
+// (CREATE_MASK_meta "all-parts", LIB/EQUATION/module_physics_metamodule.f90:
+// 38-84, with create_mask_2D_ACM, LIB/EQUATION/ACMnew/create_mask.f90:183-320):
+//   component 0: chi of the cylinder (draw_circle, LIB/EQUATION/insects/
+//                module_geometry.f90:315-381), written over the interior and
+//                the first upper ghost point within the circle's bounding box
+//                widened by the smoothing safety, 0 elsewhere;
+//   components 1-3: u_s = 0;  component 4: colour 1;
+//   component 5: sponge on the interior (sponge_2D, sponge.f90:4-58), "rect"
+//                or "p-norm", 0 on the ghosts.
+// Smoothing: step_cosine (LIB/HELPER/module_helpers.f90:436-487).
+// Also the per-block statistics of chi the coarsening mask criterion needs
+// (coarseningIndicatorMask_tree, coarseningIndicator_tree.f90:338-422).
+#include <cmath>
+
+#include "wh_internal.h"
+
+namespace wh {
+namespace {
+
+struct MaskDev {
+    int geometry, use_sponge, sponge_type;
+    double cx, cy, R, h, safety, Lh, p, pinv, off, Lx, Ly, pi;
+};
+
+__device__ __forceinline__ double step_cos(double x, double t, double h, double pi)
+{
+    const double xr = x - t;
+    if (xr <= -h) return 1.0;
+    if (xr >= h) return 0.0;
+    return 0.5 * (1.0 + cos((xr + h) * pi / (2.0 * h)));
+}
+
+__global__ __launch_bounds__(256) void k_mask(double *__restrict__ mask, size_t mblk, size_t cs,
+                                              const int32_t *__restrict__ ids,
+                                              const double *__restrict__ x0dx, int nb, int nx, int ny,
+                                              int g, int bs0, int bs1, MaskDev M)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nb * (long)cs) return;
+    const int k = (int)(e / (long)cs);
+    const int q = (int)(e - (long)k * cs);
+    const int ix = q % nx, iy = q / nx;  // 2D: nz = 1
+    const double x0 = x0dx[6 * k + 0], y0 = x0dx[6 * k + 1], dx = x0dx[6 * k + 3], dy = x0dx[6 * k + 4];
+    double *m = mask + (size_t)ids[k] * mblk + q;
+    double chi = 0.0;
+    if (M.geometry == 1) {
+        // draw_circle: 1-based indices, bounds from int()/ceiling() of the
+        // circle's box in grid units, widened by ceiling(safety/min dx)
+        const int nsafe = (int)ceil(M.safety / fmin(dx, dy));
+        const int bxmin = (int)((M.cx - M.R - x0) / dx) - nsafe + (g + 1);
+        const int bymin = (int)((M.cy - M.R - y0) / dy) - nsafe + (g + 1);
+        const int bxmax = (int)ceil((M.cx + M.R - x0) / dx) + nsafe + (g + 1);
+        const int bymax = (int)ceil((M.cy + M.R - y0) / dy) + nsafe + (g + 1);
+        const int i1 = ix + 1, j1 = iy + 1;
+        if (i1 >= max(bxmin, g + 1) && i1 <= min(bxmax, nx - g + 1) && j1 >= max(bymin, g + 1) &&
+            j1 <= min(bymax, ny - g + 1)) {
+            const double y = (double)(j1 - (g + 1)) * dy + y0;
+            const double x = (double)(i1 - (g + 1)) * dx + x0;
+            const double ax = x - M.cx, ay = y - M.cy;
+            const double dist = sqrt(ax * ax + ay * ay) - M.R;
+            chi = step_cos(dist, 0.0, M.h, M.pi);
+        }
+    }
+    double sp = 0.0;
+    if (M.use_sponge && ix >= g && ix < g + bs0 && iy >= g && iy < g + bs1) {
+        double tmp;
+        if (M.sponge_type == 0) {
+            const double x = (double)(ix - g) * dx + x0, y = (double)(iy - g) * dy + y0;
+            tmp = fmin(fmin(fmin(x, y), -(x - M.Lx)), -(y - M.Ly));
+        } else {
+            const double y = (double)(iy - g) * dy + y0 - M.off;
+            const double x = (double)(ix - g) * dx + x0 - M.off;
+            tmp = -(pow(pow(x, M.p) + pow(y, M.p), M.pinv) - M.off);
+        }
+        sp = step_cos(tmp, M.Lh, M.Lh, M.pi);
+    }
+    m[0] = chi;
+    m[cs] = 0.0;
+    m[2 * cs] = 0.0;
+    m[3 * cs] = 0.0;
+    m[4 * cs] = 1.0;
+    m[5 * cs] = sp;
+}
+
+// chi statistics over the interior of listed blocks: [between, max, min]
+__global__ __launch_bounds__(256) void k_mask_stats(const double *__restrict__ mask, size_t mblk, size_t cs,
+                                                    const int32_t *__restrict__ ids, int nx, int g, int bs0,
+                                                    int bs1, double *__restrict__ out)
+{
+    __shared__ double smax[256], smin[256];
+    __shared__ int sbetw[256];
+    const double *m = mask + (size_t)ids[blockIdx.x] * mblk;
+    double mx = -1.0e300, mn = 1.0e300;
+    int betw = 0;
+    for (int e = threadIdx.x; e < bs0 * bs1; e += blockDim.x) {
+        const int ix = g + e % bs0, iy = g + e / bs0;
+        const double v = m[(size_t)iy * nx + ix];
+        if (v > 1.0e-12 && v < 1.0 - 1.0e-12) betw = 1;
+        mx = fmax(mx, v);
+        mn = fmin(mn, v);
+    }
+    smax[threadIdx.x] = mx;
+    smin[threadIdx.x] = mn;
+    sbetw[threadIdx.x] = betw;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+            sbetw[threadIdx.x] |= sbetw[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x + 0] = (double)sbetw[0];
+        out[3 * blockIdx.x + 1] = smax[0];
+        out[3 * blockIdx.x + 2] = smin[0];
+    }
+}
+
+MaskDev make_mask_dev(const wh_mask_params &mp)
+{
+    MaskDev M{};
+    M.geometry = mp.geometry;
+    M.use_sponge = mp.use_sponge;
+    M.sponge_type = mp.sponge_type;
+    M.cx = mp.x_cntr[0];
+    M.cy = mp.x_cntr[1];
+    M.R = mp.R_cyl;
+    M.h = mp.smoothing_width;
+    M.safety = mp.smoothing_safety;
+    M.Lh = 0.5 * mp.L_sponge;
+    M.p = mp.p_sponge;
+    M.pinv = 1.0 / mp.p_sponge;
+    M.off = 0.5 * mp.domain[0];
+    M.Lx = mp.domain[0];
+    M.Ly = mp.domain[1];
+    M.pi = 4.0 * std::atan(1.0);
+    return M;
+}
+
+}  // namespace
+
+int check_mask_pool(wh_pool *P)
+{
+    WH_CHECK(P->d.dim == 2, WH_E_ARG, "mask: only the 2D ACM mask is implemented");
+    WH_CHECK(P->mask && P->d.n_mask >= 6 && P->d.mask_per_block, WH_E_ARG,
+             "mask: the pool needs a per-block mask with 6 components");
+    return WH_OK;
+}
+
+int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &ids,
+             const std::vector<double> &x0dx)
+{
+    if (int rc = check_mask_pool(P)) return rc;
+    WH_CHECK(mp.geometry == 0 || mp.geometry == 1, WH_E_ARG, "mask: unknown geometry");
+    WH_CHECK(mp.sponge_type == 0 || mp.sponge_type == 1, WH_E_ARG, "mask: unknown sponge type");
+    WH_CHECK(!(mp.use_sponge && mp.sponge_type == 1) || std::fabs(mp.domain[1] - mp.domain[0]) <= 1.0e-10,
+             WH_E_ARG, "mask: the p-norm sponge needs a square domain");
+    WH_CHECK(mp.geometry == 0 || (mp.smoothing_width > 0.0), WH_E_ARG, "mask: smoothing width must be > 0");
+    const int n = (int)ids.size();
+    if (n == 0) return WH_OK;
+    for (int32_t h : ids) WH_CHECK(h >= 0 && h < P->d.nb_max, WH_E_ARG, "mask: hvy id out of range");
+    WaveletWS &W = P->ws;
+    // ids then x0dx in the workspace buffers
+    if (W.ids_cap < (size_t)n) {
+        if (W.d_ids) WH_HIP(hipFree(W.d_ids));
+        W.d_ids = nullptr;
+        WH_HIP(hipMalloc(&W.d_ids, sizeof(int32_t) * n));
+        W.ids_cap = n;
+    }
+    if (W.out_cap < (size_t)n * 6) {
+        if (W.d_out) WH_HIP(hipFree(W.d_out));
+        W.d_out = nullptr;
+        WH_HIP(hipMalloc(&W.d_out, sizeof(double) * n * 6));
+        W.out_cap = (size_t)n * 6;
+    }
+    WH_HIP(hipMemcpyAsync(W.d_ids, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
+    WH_HIP(hipMemcpyAsync(W.d_out, x0dx.data(), sizeof(double) * n * 6, hipMemcpyHostToDevice, P->stream));
+    const long tot = (long)n * (long)P->cs;
+    hipLaunchKernelGGL(k_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask, P->mblk,
+                       P->cs, W.d_ids, W.d_out, n, P->nx, P->ny, P->d.g, P->d.bs[0], P->d.bs[1],
+                       make_mask_dev(mp));
+    WH_HIP(hipGetLastError());
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<double> &stats)
+{
+    if (int rc = check_mask_pool(P)) return rc;
+    const int n = (int)ids.size();
+    stats.assign((size_t)n * 3, 0.0);
+    if (n == 0) return WH_OK;
+    WaveletWS &W = P->ws;
+    if (W.ids_cap < (size_t)n) {
+        if (W.d_ids) WH_HIP(hipFree(W.d_ids));
+        W.d_ids = nullptr;
+        WH_HIP(hipMalloc(&W.d_ids, sizeof(int32_t) * n));
+        W.ids_cap = n;
+    }
+    if (W.out_cap < (size_t)n * 3) {
+        if (W.d_out) WH_HIP(hipFree(W.d_out));
+        W.d_out = nullptr;
+        WH_HIP(hipMalloc(&W.d_out, sizeof(double) * n * 3));
+        W.out_cap = (size_t)n * 3;
+    }
+    WH_HIP(hipMemcpyAsync(W.d_ids, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
+    hipLaunchKernelGGL(k_mask_stats, dim3(n), dim3(256), 0, P->stream, P->mask, P->mblk, P->cs, W.d_ids, P->nx,
+                       P->d.g, P->d.bs[0], P->d.bs[1], W.d_out);
+    WH_HIP(hipGetLastError());
+    WH_HIP(hipMemcpyAsync(stats.data(), W.d_out, sizeof(double) * n * 3, hipMemcpyDeviceToHost, P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    return WH_OK;
+}
+
+}  // namespace wh
+
+extern "C" int wh_create_mask(wh_pool *pool, const wh_mask_params *mp, const int32_t *ids, int n,
+                              const double *x0dx)
+{
+    using namespace wh;
+    WH_CHECK(pool && mp, WH_E_ARG, "wh_create_mask: null argument");
+    if (n < 0) {
+        std::vector<int32_t> act(pool->h_active.begin(), pool->h_active.begin() + pool->nact);
+        std::vector<double> xd(pool->h_x0dx.begin(), pool->h_x0dx.begin() + 6 * (size_t)pool->nact);
+        return run_mask(pool, *mp, act, xd);
+    }
+    WH_CHECK(n == 0 || (ids && x0dx), WH_E_ARG, "wh_create_mask: null list");
+    return run_mask(pool, *mp, std::vector<int32_t>(ids, ids + n), std::vector<double>(x0dx, x0dx + 6 * (size_t)n));
+}

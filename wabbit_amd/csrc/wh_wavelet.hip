@@ -326,6 +326,41 @@ __global__ __launch_bounds__(256) void k_detail(const double *__restrict__ arr,
         out[(size_t)blockIdx.x * D.neq + threadIdx.x] = __longlong_as_double((long long)smax[threadIdx.x]);
 }
 
+// Per-box maxima (one workgroup per box): per component the max over the box
+// of |v| (zero_sc: SC points count as 0; sqrt_groups: components of group
+// >= 2 take sqrt(v*v)), as threshold_block with `indices` and the Linfty
+// norm of componentWiseNorm_tree.
+struct RegionArgs {
+    int g, dim, neq, zero_sc, sqrt_groups;
+    int groups[8];
+};
+__global__ __launch_bounds__(256) void k_region_max(const double *__restrict__ arr, const wh_box *__restrict__ boxes,
+                                                    ArrGeo A, RegionArgs R, double *__restrict__ out)
+{
+    __shared__ unsigned long long smax[8];
+    if (threadIdx.x < 8) smax[threadIdx.x] = 0ull;
+    __syncthreads();
+    const wh_box B = boxes[blockIdx.x];
+    const int n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1, n2 = B.hi[2] - B.lo[2] + 1;
+    const int npts = n0 * n1 * n2;
+    for (int c = 0; c < R.neq; ++c) {
+        double m = 0.0;
+        const double *u = arr + (size_t)B.hvy * A.blk_in + c * A.cs;
+        for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+            const int ix = B.lo[0] + e % n0, iy = B.lo[1] + (e / n0) % n1, iz = B.lo[2] + e / (n0 * n1);
+            const bool sc = (((ix - R.g) & 1) == 0) && (((iy - R.g) & 1) == 0) &&
+                            (R.dim == 2 || ((iz - R.g) & 1) == 0);
+            const double v = (R.zero_sc && sc) ? 0.0 : u[((size_t)iz * A.ny + iy) * A.nx + ix];
+            const double a = (R.sqrt_groups && R.groups[c] >= 2) ? sqrt(v * v) : fabs(v);
+            m = a > m ? a : m;
+        }
+        atomicMax(&smax[c], (unsigned long long)__double_as_longlong(m));
+    }
+    __syncthreads();
+    if (threadIdx.x < R.neq)
+        out[(size_t)blockIdx.x * R.neq + threadIdx.x] = __longlong_as_double((long long)smax[threadIdx.x]);
+}
+
 // Coarse-extension boxes: one workgroup per box.
 __global__ __launch_bounds__(256) void k_boxes(double *__restrict__ arr, const double *__restrict__ src,
                                                size_t blk_src, const wh_box *__restrict__ boxes,
@@ -577,6 +612,42 @@ int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int
         for (int c = 0; c < neq; ++c)
             if (D.groups[c] == 0) dk[c] = 0.0;
     }
+    return WH_OK;
+}
+
+int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
+                   const int32_t *groups, std::vector<double> &out)
+{
+    const int n = (int)boxes.size(), neq = P->d.neq;
+    out.assign((size_t)n * neq, 0.0);
+    if (n == 0) return WH_OK;
+    WH_CHECK(neq <= 8, WH_E_ARG, "region max: at most 8 components");
+    size_t blk = 0;
+    int nc = 0;
+    const double *arr = array_ptr(P, array, &blk, &nc);
+    WH_CHECK(arr && nc >= neq, WH_E_ARG, "region max: bad array");
+    const int lim[3] = {P->nx, P->ny, P->nz};
+    for (const wh_box &b : boxes) {
+        WH_CHECK(b.hvy >= 0 && b.hvy < P->d.nb_max, WH_E_ARG, "region max: bad hvy id");
+        for (int d = 0; d < 3; ++d)
+            WH_CHECK(b.lo[d] >= 0 && b.hi[d] < lim[d] && b.lo[d] <= b.hi[d], WH_E_ARG, "region max: bad box");
+    }
+    WaveletWS &W = P->ws;
+    if (int rc = ensure(W.d_boxes, W.box_cap, (size_t)n)) return rc;
+    if (int rc = ensure(W.d_out, W.out_cap, (size_t)n * neq)) return rc;
+    WH_HIP(hipMemcpyAsync(W.d_boxes, boxes.data(), sizeof(wh_box) * n, hipMemcpyHostToDevice, P->stream));
+    RegionArgs R{};
+    R.g = P->d.g;
+    R.dim = P->d.dim;
+    R.neq = neq;
+    R.zero_sc = zero_sc;
+    R.sqrt_groups = sqrt_groups;
+    for (int c = 0; c < 8; ++c) R.groups[c] = (groups && c < neq) ? groups[c] : 1;
+    const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
+    hipLaunchKernelGGL(k_region_max, dim3(n), dim3(256), 0, P->stream, arr, W.d_boxes, A, R, W.d_out);
+    WH_HIP(hipGetLastError());
+    WH_HIP(hipMemcpyAsync(out.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost, P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
     return WH_OK;
 }
 

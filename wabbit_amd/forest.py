@@ -108,3 +108,31 @@ class EquidistantForest:
                 out[q] = (hid, nid, rlo, slo, n, WH_PATCH_COPY, 1, 0)
                 q += 1
         return out
+
+
+def equidistant_treecodes(level: int, dim: int, jmax: int):
+    """Treecodes and levels of createEquidistantGrid_tree at ``level``
+    (createEquidistantGrid_tree.f90), x fastest."""
+    n1 = 2 ** level
+    tcs = []
+    for k in range(n1 ** dim):
+        ijk = [k % n1, (k // n1) % n1, (k // (n1 * n1)) if dim == 3 else 0]
+        tcs.append(treecode_from_ijk(ijk, level, jmax, dim))
+    return np.asarray(tcs, np.int64), np.full(len(tcs), level, np.int32)
+
+
+def refined_treecodes(treecodes, levels, jmax_tc: int, dim: int, level_max: int):
+    """The block list after refine_tree "everywhere" (refine_tree.f90 with
+    respectJmaxJmin_tree): every block below level_max is replaced by its
+    2^dim daughters, digit d at the daughter's level (refinementExecute.f90)."""
+    tc_out, lv_out = [], []
+    for tc, lv in zip(treecodes, levels):
+        tc, lv = int(tc), int(lv)
+        if lv >= level_max:
+            tc_out.append(tc)
+            lv_out.append(lv)
+            continue
+        for d in range(1 << dim):
+            tc_out.append(tc | (d << ((jmax_tc - lv - 1) * dim)))
+            lv_out.append(lv + 1)
+    return np.asarray(tc_out, np.int64), np.asarray(lv_out, np.int32)

@@ -28,6 +28,36 @@ module wabbit_hip
         integer(c_int32_t) :: op, stage, aux
     end type
 
+    integer(c_int), parameter :: WH_WAVELET_TAPS = 33
+    integer(c_int), parameter :: WH_IND_EVERYWHERE = 0, WH_IND_THRESHOLD = 1
+
+    type, bind(C) :: wh_wavelet
+        integer(c_int) :: X, Y, lifted, specialised
+        integer(c_int) :: hd_lo, hd_hi, gd_lo, gd_hi, hr_lo, hr_hi, gr_lo, gr_hi
+        real(c_double) :: HD(WH_WAVELET_TAPS), GD(WH_WAVELET_TAPS), HR(WH_WAVELET_TAPS), GR(WH_WAVELET_TAPS)
+        integer(c_int) :: g, g_rhs, order
+        integer(c_int) :: Nscl, Nscr, Nwcl, Nwcr, Nreconl, Nreconr, Ndep2l, Ndep2r
+    end type
+
+    type, bind(C) :: wh_mask_params
+        integer(c_int) :: geometry
+        real(c_double) :: x_cntr(3), R_cyl
+        real(c_double) :: smoothing_width, smoothing_safety
+        integer(c_int) :: use_sponge, sponge_type
+        real(c_double) :: L_sponge, p_sponge
+        real(c_double) :: domain(3)
+    end type
+
+    type, bind(C) :: wh_adapt_params
+        integer(c_int) :: jmin, jmax, indicator, use_ce, force_maxlevel_dealiasing
+        real(c_double) :: eps
+        real(c_double) :: norm(8)
+        integer(c_int32_t) :: groups(8)
+        integer(c_int) :: eps_normalized, threshold_mask, security_zone
+        real(c_double) :: domain(3)
+        type(wh_mask_params) :: mask
+    end type
+
     interface
         function wh_version() bind(C, name="wh_version") result(v)
             import :: c_int
@@ -105,6 +135,96 @@ module wabbit_hip
             real(c_double), intent(in) :: butcher(*)
             integer(c_int), value :: nrows, path
             real(c_double), value :: time, dt
+            integer(c_int) :: rc
+        end function
+        ! ---- adaptive path: forest, wavelets, mask, adapt_tree ----
+        function wh_forest_create(forest, dim, jmax, bs, g, domain, periodic, nb_max) &
+                bind(C, name="wh_forest_create") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), intent(out) :: forest
+            integer(c_int), value :: dim, jmax, g, nb_max
+            integer(c_int), intent(in) :: bs(3), periodic(3)
+            real(c_double), intent(in) :: domain(3)
+            integer(c_int) :: rc
+        end function
+        function wh_forest_destroy(forest) bind(C, name="wh_forest_destroy") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: forest
+            integer(c_int) :: rc
+        end function
+        function wh_forest_set_blocks(forest, n, treecodes, levels, hvy_ids) &
+                bind(C, name="wh_forest_set_blocks") result(rc)
+            import :: c_ptr, c_int, c_int32_t, c_int64_t
+            type(c_ptr), value :: forest
+            integer(c_int), value :: n
+            integer(c_int64_t), intent(in) :: treecodes(*)
+            integer(c_int32_t), intent(in) :: levels(*), hvy_ids(*)
+            integer(c_int) :: rc
+        end function
+        function wh_forest_size(forest, n) bind(C, name="wh_forest_size") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: forest
+            integer(c_int), intent(out) :: n
+            integer(c_int) :: rc
+        end function
+        function wh_forest_get_blocks(forest, cap, treecodes, levels, hvy_ids, x0dx) &
+                bind(C, name="wh_forest_get_blocks") result(rc)
+            import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double
+            type(c_ptr), value :: forest
+            integer(c_int), value :: cap
+            integer(c_int64_t), intent(out) :: treecodes(*)
+            integer(c_int32_t), intent(out) :: levels(*), hvy_ids(*)
+            real(c_double), intent(out) :: x0dx(*)
+            integer(c_int) :: rc
+        end function
+        function wh_wavelet_setup(name, fd_max_size, w) bind(C, name="wh_wavelet_setup") result(rc)
+            import :: c_int, c_char, wh_wavelet
+            character(kind=c_char), intent(in) :: name(*)
+            integer(c_int), value :: fd_max_size
+            type(wh_wavelet), intent(out) :: w
+            integer(c_int) :: rc
+        end function
+        function wh_pool_set_wavelet(pool, w) bind(C, name="wh_pool_set_wavelet") result(rc)
+            import :: c_ptr, c_int, wh_wavelet
+            type(c_ptr), value :: pool
+            type(wh_wavelet), intent(in) :: w
+            integer(c_int) :: rc
+        end function
+        function wh_create_mask(pool, mp, hvy_ids, n, x0dx) bind(C, name="wh_create_mask") result(rc)
+            import :: c_ptr, c_int, wh_mask_params
+            type(c_ptr), value :: pool
+            type(wh_mask_params), intent(in) :: mp
+            type(c_ptr), value :: hvy_ids, x0dx
+            integer(c_int), value :: n
+            integer(c_int) :: rc
+        end function
+        function wh_sync_tree(pool, forest, array) bind(C, name="wh_sync_tree") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: pool, forest
+            integer(c_int), value :: array
+            integer(c_int) :: rc
+        end function
+        function wh_adapt_tree(pool, forest, p) bind(C, name="wh_adapt_tree") result(rc)
+            import :: c_ptr, c_int, wh_adapt_params
+            type(c_ptr), value :: pool, forest
+            type(wh_adapt_params), intent(in) :: p
+            integer(c_int) :: rc
+        end function
+        function wh_forest_refine(forest, n, hvy_ids, triples, cap, ntriples) &
+                bind(C, name="wh_forest_refine") result(rc)
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: forest
+            integer(c_int), value :: n, cap
+            integer(c_int32_t), intent(in) :: hvy_ids(*)
+            integer(c_int32_t), intent(out) :: triples(*)
+            integer(c_int), intent(out) :: ntriples
+            integer(c_int) :: rc
+        end function
+        function wh_refine_blocks(pool, array, triples, ntriples, order) bind(C, name="wh_refine_blocks") result(rc)
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: pool
+            integer(c_int), value :: array, ntriples, order
+            integer(c_int32_t), intent(in) :: triples(*)
             integer(c_int) :: rc
         end function
     end interface

@@ -171,6 +171,7 @@ class Pool:
         self.ny = bs[1] + 2 * g
         self.nz = bs[2] + 2 * g if dim == 3 else 1
         self.n_work_slots = n_work_slots
+        self.n_tmp_eqn = n_tmp_eqn
         self.n_mask = n_mask
         self.mask_per_block = mask_per_block
 
@@ -185,16 +186,27 @@ class Pool:
         except Exception:
             pass
 
+    def ncomp(self, array):
+        """Components per block of a pool array."""
+        if array == WH_TMP:
+            return self.n_tmp_eqn
+        if array == WH_MASK:
+            return self.n_mask
+        return self.neq
+
     def block_shape(self, ncomp=None):
         return (self.neq if ncomp is None else ncomp, self.nz, self.ny, self.nx)
 
     def upload(self, array, first, data):
         data = np.ascontiguousarray(data, dtype=np.float64)
         n = data.shape[0]
+        if data.shape[1:] != self.block_shape(self.ncomp(array)):
+            raise ValueError(f"upload: blocks of shape {data.shape[1:]}, the array holds "
+                             f"{self.block_shape(self.ncomp(array))}")
         check(load().wh_pool_upload(self.h, array, first, n, data.ctypes.data))
 
-    def download(self, array, first, count, ncomp=None):
-        out = np.empty((count,) + self.block_shape(ncomp), np.float64)
+    def download(self, array, first, count):
+        out = np.empty((count,) + self.block_shape(self.ncomp(array)), np.float64)
         check(load().wh_pool_download(self.h, array, first, count, out.ctypes.data))
         return out
 

@@ -45,11 +45,49 @@ class BoxC(ctypes.Structure):
                 ("op", ctypes.c_int32)]
 
 
+class MaskParamsC(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_int), ("x_cntr", ctypes.c_double * 3), ("R_cyl", ctypes.c_double),
+                ("smoothing_width", ctypes.c_double), ("smoothing_safety", ctypes.c_double),
+                ("use_sponge", ctypes.c_int), ("sponge_type", ctypes.c_int),
+                ("L_sponge", ctypes.c_double), ("p_sponge", ctypes.c_double),
+                ("domain", ctypes.c_double * 3)]
+
+
 class AdaptParamsC(ctypes.Structure):
     _fields_ = [("jmin", ctypes.c_int), ("jmax", ctypes.c_int), ("indicator", ctypes.c_int),
                 ("use_ce", ctypes.c_int), ("force_maxlevel_dealiasing", ctypes.c_int),
                 ("eps", ctypes.c_double), ("norm", ctypes.c_double * 8),
-                ("groups", ctypes.c_int32 * 8)]
+                ("groups", ctypes.c_int32 * 8), ("eps_normalized", ctypes.c_int),
+                ("threshold_mask", ctypes.c_int), ("security_zone", ctypes.c_int),
+                ("domain", ctypes.c_double * 3), ("mask", MaskParamsC)]
+
+
+def mask_params(geometry=0, x_cntr=(0.0, 0.0, 0.0), R_cyl=0.5, smoothing_width=0.0, smoothing_safety=0.0,
+                use_sponge=False, sponge_type=0, L_sponge=0.0, p_sponge=20.0,
+                domain=(1.0, 1.0, 1.0)) -> MaskParamsC:
+    """The [VPM]/[Sponge] settings create_mask_2D_ACM reads (module_ACM.f90:300-481):
+    geometry 1 = cylinder, sponge_type 0 = rect, 1 = p-norm."""
+    m = MaskParamsC()
+    m.geometry = geometry
+    for i in range(3):
+        m.x_cntr[i] = x_cntr[i] if i < len(x_cntr) else 0.0
+        m.domain[i] = domain[i] if i < len(domain) else 1.0
+    m.R_cyl, m.smoothing_width, m.smoothing_safety = R_cyl, smoothing_width, smoothing_safety
+    m.use_sponge, m.sponge_type = int(use_sponge), sponge_type
+    m.L_sponge, m.p_sponge = L_sponge, p_sponge
+    return m
+
+
+def create_mask(pool, mp: MaskParamsC, ids=None, x0dx=None):
+    """The penalisation mask of the listed blocks (all active blocks when ids
+    is None) into WH_MASK."""
+    _bind()
+    if ids is None:
+        L.check(L.load().wh_create_mask(pool.h, ctypes.byref(mp), None, -1, None))
+        return
+    a = _ids(ids)
+    x = np.ascontiguousarray(x0dx, np.float64)
+    L.check(L.load().wh_create_mask(pool.h, ctypes.byref(mp), a.ctypes.data, a.size, x.ctypes.data))
 
 
 BOX_DTYPE = np.dtype([("hvy", np.int32), ("lo", np.int32, 3), ("hi", np.int32, 3), ("op", np.int32)])
@@ -73,6 +111,7 @@ def _bind():
         "wh_run_patches": [_vp, ip, ip, ip, _vp, ip],
         "wh_adapt_tree": [_vp, _vp, ctypes.POINTER(AdaptParamsC)],
         "wh_sync_tree": [_vp, _vp, ip],
+        "wh_create_mask": [_vp, ctypes.POINTER(MaskParamsC), _vp, ip, _vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -143,10 +182,22 @@ def run_patches(pool, array, patches, alt1=-1, alt2=-1):
 
 
 def adapt_params(jmin, jmax, indicator=WH_IND_THRESHOLD, use_ce=True, eps=1e-3, norm=None,
-                 groups=None, force_maxlevel_dealiasing=False, neq=1) -> AdaptParamsC:
+                 groups=None, force_maxlevel_dealiasing=False, neq=1, eps_normalized=False,
+                 threshold_mask=False, security_zone=None, domain=(1.0, 1.0, 1.0),
+                 mask=None) -> AdaptParamsC:
+    """adapt_tree settings; security_zone defaults to use_ce, as the
+    reference's useSecurityZone defaults to the lifted-wavelet flag
+    (ini_file_to_params.f90:543-546)."""
     p = AdaptParamsC()
     p.jmin, p.jmax, p.indicator = jmin, jmax, indicator
     p.use_ce = int(use_ce)
+    p.eps_normalized = int(eps_normalized)
+    p.threshold_mask = int(threshold_mask)
+    p.security_zone = int(use_ce if security_zone is None else security_zone)
+    for i in range(3):
+        p.domain[i] = domain[i] if i < len(domain) else 1.0
+    if mask is not None:
+        p.mask = mask
     p.force_maxlevel_dealiasing = int(force_maxlevel_dealiasing)
     p.eps = eps
     for c in range(8):
