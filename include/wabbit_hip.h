@@ -171,12 +171,34 @@ int wh_time_get(wh_pool *pool, double *time, int *iteration, double *dt_last);
 int wh_run_steps(wh_pool *pool, const wh_acm_params *p, const double *butcher, int nrows,
                  const wh_time_params *tp, int nsteps, int use_graph);
 
+/* ---- block groups: independent forests sharing one pool ----------------
+ * The reference runs one forest per process; a batch of independent solves
+ * (an ensemble, or the requests of a serving loop) can instead share one pool
+ * and one launch per operation.  Each active block belongs to a group (its
+ * forest); groups never exchange data (their ghost patches stay inside the
+ * group) and each has its own dt.
+ *
+ * wh_set_block_groups: the group of every active block, in active-list order
+ * (call after wh_set_active, which clears the groups). */
+int wh_set_block_groups(wh_pool *pool, const int32_t *group, int n, int ngroups);
+/* calculate_time_step's block-dt reduction per group (calculate_time_step.f90:
+ * 28-48, GET_DT_BLOCK_ACM module_ACM.f90:617-691): dt_out[ngroups], before
+ * the host clamps. */
+int wh_dt_reduce_groups(wh_pool *pool, const wh_acm_params *p, double *dt_out);
+/* RungeKuttaGeneric (runge_kutta_generic.f90:1-156) for every group at once,
+ * group i with its own dt_groups[i]; path 2 (fused stages) or 1 (one launch
+ * per block).  Each block's arithmetic is that of wh_rk_step with its
+ * group's dt, so a group's result does not depend on the other groups. */
+int wh_rk_step_groups(wh_pool *pool, const wh_acm_params *p, const double *butcher, int nrows,
+                      const double *dt_groups, int path);
+
 /* refineBlock (LIB/MESH/refinementExecute.f90:4-116) for a list of
  * (mother, daughter, digit) triples: daughter `digit` of a mother takes its
  * quadrant/octant of the mother's interpolated grid (prediction of order
  * `order` = 2/4/6 over the whole ghost-inclusive block, so the mothers' ghost
  * nodes must be synchronised first).  Daughters and mothers must be distinct
- * pool slots. */
+ * pool slots.  Ordered on the pool's stream; may return before the device
+ * has finished. */
 int wh_refine_blocks(wh_pool *pool, int array, const int32_t *triples, int ntriples, int order);
 
 /* ---- host forest (metadata of one tree) --------------------------------
@@ -188,6 +210,9 @@ typedef struct wh_forest wh_forest;
 int wh_forest_create(wh_forest **out, int dim, int jmax, const int *bs, int g,
                      const double *domain, const int *periodic, int nb_max);
 int wh_forest_destroy(wh_forest *forest);
+/* The forest's pool slots become [first, first + nb_max) (default first = 0),
+ * so that several forests can share one pool; only while it has no blocks. */
+int wh_forest_set_slot_base(wh_forest *forest, int first);
 int wh_forest_set_blocks(wh_forest *forest, int n, const int64_t *treecodes,
                          const int32_t *levels, const int32_t *hvy_ids);
 int wh_forest_size(const wh_forest *forest, int *n);
@@ -227,7 +252,13 @@ int wh_wavelet_setup(const char *name, int fd_max_size, wh_wavelet *out);
 /* The wavelet the pool's transform kernels use. */
 int wh_pool_set_wavelet(wh_pool *pool, const wh_wavelet *w);
 
-/* Per-block transforms of the listed hvy ids, src and dst distinct arrays;
+/* The transform, box, patch-list and mask calls below are ordered on the
+ * pool's stream and may return before the device has finished (the host
+ * lists are staged through pinned memory, so the caller's buffers are free
+ * on return); downloads, wh_detail_blocks, wh_sync_tree and wh_adapt_tree
+ * return with the stream synchronised.
+ *
+ * Per-block transforms of the listed hvy ids, src and dst distinct arrays;
  * only the region the reference computes is written (dst keeps the rest):
  * waveletDecomposition_optimized_block (wavelet_decomposition_reconstruction.f90:23-389),
  * waveletReconstruction_optimized_block (:426-841),

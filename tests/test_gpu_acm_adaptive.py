@@ -42,7 +42,13 @@ def test_mask_kernel_matches_oracle():
     tree = A.oracle_tree(tc, lv, hvy, A.JMAX)
     want = tree.get_mask(A.oracle_mask_params())
     assert got.shape == want.shape
-    assert np.max(np.abs(got - want)) <= 4e-16          # chi, colour, sponge: ulp of libm at most
+    # chi and colour within an ulp; the p-norm sponge step_cos(tmp) with
+    # tmp = -(pow(pow(x,8)+pow(y,8),1/8) - L/2): the device and host pow may
+    # differ by an ulp or two of |tmp| <= L/2 = 10 (ulp 1.8e-15), scaled by
+    # the cosine step's slope pi/(2h) = pi/2 for h = L_sponge/2 = 1
+    d = np.abs(got - want)
+    assert np.max(d[:, [0, 1, 2, 3, 4]]) <= 4e-16
+    assert np.max(d[:, 5]) <= 1e-14
     # and chi on the reference's saved grid
     ref = load_npz("acm_cyl_t0.npz")
     key = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(tc, lv))}

@@ -81,12 +81,19 @@ class AdaptiveACM:
         self.time, self.iteration = 0.0, 0
         self.rk_blocks = 0
         self.phase_ms = None
+        self._stale = True
         self._attach()
 
     def _attach(self):
-        self.forest.attach(self.pool, self.g_rhs, self.w.order)
+        self.forest.attach(self.pool, self.g_rhs, self.w.order, full=False)
         if self.mask_p is not None:
             W.create_mask(self.pool, self.mask_p)
+        self._stale = False
+
+    def _fresh(self):
+        """The pool's tables and mask describe the current leaves."""
+        if self._stale:
+            self._attach()
 
     @property
     def nblocks(self):
@@ -98,9 +105,10 @@ class AdaptiveACM:
         tc, lv, hv, _ = self.forest.blocks()
         ids = hv[lv < self.cfg.jmax]
         if ids.size:
-            refine_on_device(self.pool, self.forest, ids, self.w.order, self.g_rhs, presync=None)
+            refine_on_device(self.pool, self.forest, ids, self.w.order, self.g_rhs, presync=None, full=False)
         else:
-            self.forest.attach(self.pool, self.g_rhs, self.w.order)
+            self.forest.attach(self.pool, self.g_rhs, self.w.order, full=False)
+        self._stale = False
 
     def set_blocks(self, treecodes, levels, data):
         """Replace the leaves and their data (hvy ids 0..n-1), as
@@ -156,9 +164,8 @@ class AdaptiveACM:
         self.time += dt
         self.iteration += 1
         W.adapt_tree(self.pool, self.forest, self.adapt_p)
-        t = self._tick(t, "adapt")
-        self._attach()
-        self._tick(t, "attach")
+        self._stale = True   # the next refine re-attaches; state()/mask() refresh
+        self._tick(t, "adapt")
         return dt
 
     def profile(self, on: bool):
@@ -184,6 +191,7 @@ class AdaptiveACM:
         return tc[order], lv[order], data
 
     def mask(self):
+        self._fresh()
         tc, lv, hv, _ = self.forest.blocks()
         order = np.lexsort((tc, lv))
         return np.stack([self.pool.download(L.WH_MASK, int(h), 1)[0] for h in hv[order]])

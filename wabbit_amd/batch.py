@@ -1,0 +1,278 @@
+"""Many independent adaptive ACM solves on shared device pools.
+
+The reference runs one forest per process (LIB/MAIN/main.f90); here a batch of
+independent solves -- an ensemble, or the requests of a serving loop -- share
+one block pool per block shape (dim, Bs, wavelet, physics constants) and one
+launch per RK-stage operation: every solve's blocks form one active list
+with a group index per block (wh_set_block_groups), the ghost-patch tables of
+all solves are concatenated (patches never cross solves), the block-dt
+reduction runs per group (wh_dt_reduce_groups) and each solve steps with its
+own dt (wh_rk_step_groups).  Mesh work (sync_ghosts_tree, refine_tree,
+createMask_tree, adapt_tree) runs per solve on the shared pool, each solve
+inside its own range of pool slots (wh_forest_set_slot_base).
+
+Each solve's arithmetic is exactly that of AdaptiveACM on its own, so a
+solve's result does not depend on what else is in the batch.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import lib as L
+from . import wavelets as W
+from .adaptive import AdaptConfig, fd_reach
+from .forest import equidistant_treecodes, refined_treecodes
+from .params import ACMParams, TimeParams, butcher_rk4, clamp_dt
+from .timestep import PATH_FUSED
+from .tree import Forest, refine_on_device
+
+
+@dataclass
+class SolveSpec:
+    """One independent adaptive ACM solve."""
+    acm: ACMParams
+    cfg: AdaptConfig
+    inicond: Callable            # (treecodes, levels, jmax_treecode) -> (n, neq, nz, ny, nx)
+    nsteps: int
+    mask: Optional[dict] = None  # wavelets.mask_params keywords (penalised solves)
+    tp: TimeParams = field(default_factory=lambda: TimeParams(time_max=1e9))
+    slots: int = 4096            # pool slots reserved for the solve (its --memory)
+    jini: Optional[int] = None   # initial equidistant level (default Jmin)
+    tag: str = ""
+
+
+class SlotRanges:
+    """First-fit allocator of contiguous slot ranges of one pool."""
+
+    def __init__(self, n):
+        self.n = n
+        self.used: Dict[int, int] = {}   # base -> count
+
+    def alloc(self, count):
+        pos = 0
+        for b in sorted(self.used):
+            if b - pos >= count:
+                break
+            pos = b + self.used[b]
+        if pos + count > self.n:
+            return None
+        self.used[pos] = count
+        return pos
+
+    def free(self, base):
+        del self.used[base]
+
+    @property
+    def in_use(self):
+        return sum(self.used.values())
+
+
+def _acm_key(a: ACMParams):
+    return (a.dim, a.scheme, bool(a.skew_symmetry), bool(a.penalization), bool(a.use_sponge), a.c_0, a.nu,
+            a.gamma_p, a.C_eta, a.C_sponge, tuple(a.u_mean_set), a.CFL, a.CFL_nu, a.CFL_eta)
+
+
+class Solve:
+    """A solve admitted to a pool group."""
+
+    def __init__(self, rid, spec: SolveSpec, group: "PoolGroup", base: int):
+        self.rid, self.spec, self.group, self.base = rid, spec, group, base
+        self.time, self.iteration = 0.0, 0
+        self.rk_blocks = 0
+        cfg, acm = spec.cfg, spec.acm
+        self.forest = Forest(acm.dim, cfg.jmax, [cfg.bs] * acm.dim, group.w.g, acm.domain_size[:acm.dim],
+                             spec.slots, slot_base=base)
+        self.mask_p = W.mask_params(**spec.mask) if spec.mask is not None else None
+        lifted = bool(group.w.lifted)
+        self.adapt_p = W.adapt_params(
+            cfg.jmin, cfg.jmax, W.WH_IND_THRESHOLD,
+            use_ce=lifted if cfg.use_ce is None else cfg.use_ce, eps=cfg.eps, groups=cfg.groups,
+            force_maxlevel_dealiasing=cfg.force_maxlevel_dealiasing, eps_normalized=cfg.eps_normalized,
+            threshold_mask=cfg.threshold_mask and self.mask_p is not None,
+            security_zone=lifted if cfg.security_zone is None else cfg.security_zone,
+            domain=acm.domain_size, mask=self.mask_p)
+
+    @property
+    def done(self):
+        return self.iteration >= self.spec.nsteps
+
+    def _set_blocks(self, tc, lv, data):
+        n = len(tc)
+        if n > self.spec.slots:
+            raise L.WabbitHipError(-5, f"solve {self.rid}: {n} blocks exceed its {self.spec.slots} slots")
+        self.forest.set_blocks(tc, lv, self.base + np.arange(n, dtype=np.int32))
+        self.group.pool.upload(L.WH_BLOCK, self.base, data)
+
+    def initial_grid(self):
+        """setInitialCondition_tree.f90:75-108 for an analytic initial condition."""
+        cfg, dim, ic = self.spec.cfg, self.spec.acm.dim, self.spec.inicond
+        jtc = self.forest.jmax
+        tc, lv = equidistant_treecodes(cfg.jmin if self.spec.jini is None else self.spec.jini, dim, jtc)
+        self._set_blocks(tc, lv, ic(tc, lv, jtc))
+        n_old, it = -1, 0
+        while self.forest.n != n_old and it < cfg.jmax - cfg.jmin:
+            n_old = self.forest.n
+            tc, lv, _, _ = self.forest.blocks()
+            tc, lv = refined_treecodes(tc, lv, jtc, dim, cfg.jmax)
+            self._set_blocks(tc, lv, ic(tc, lv, jtc))
+            W.adapt_tree(self.group.pool, self.forest, self.adapt_p)
+            it += 1
+
+    def state(self):
+        """(treecodes, levels, data) of the leaves, sorted by (level, treecode)."""
+        tc, lv, hv, _ = self.forest.blocks()
+        order = np.lexsort((tc, lv))
+        pool = self.group.pool
+        data = np.stack([pool.download(L.WH_BLOCK, int(h), 1)[0] for h in hv[order]])
+        return tc[order], lv[order], data
+
+    def close(self):
+        self.forest.close()
+
+
+class PoolGroup:
+    """One shared pool for the solves of one block shape and physics."""
+
+    def __init__(self, acm: ACMParams, bs: int, wavelet: str, n_slots: int, masked: bool,
+                 n_work_slots: int = 5):
+        """n_work_slots: 3 suffice for the fused path and adapt_tree, the
+        per-block path needs 5 (RK4)."""
+        self.acm = acm
+        self.w = W.setup(wavelet, fd_reach(acm.scheme))
+        self.pool = L.Pool(acm.dim, [bs] * acm.dim, self.w.g, acm.neq, n_slots, n_work_slots=n_work_slots,
+                           n_tmp_eqn=acm.neq, n_mask=6 if masked else 0, mask_per_block=True)
+        W.set_wavelet(self.pool, self.w)
+        self.slots = SlotRanges(n_slots)
+        self.acm_c = L.acm_params_c(acm)
+        self.solves: List[Solve] = []
+
+    def admit(self, rid, spec: SolveSpec) -> Optional[Solve]:
+        base = self.slots.alloc(spec.slots)
+        if base is None:
+            return None
+        s = Solve(rid, spec, self, base)
+        try:
+            s.initial_grid()
+        except Exception:
+            s.close()
+            self.slots.free(base)
+            raise
+        self.solves.append(s)
+        return s
+
+    def retire(self, s: Solve):
+        self.solves.remove(s)
+        s.close()
+        self.slots.free(s.base)
+
+    def step(self, path=PATH_FUSED, butcher=None):
+        """One main-loop iteration of every solve of the group: per solve
+        sync_ghosts_tree + refine everywhere + mask, then one RK step of all
+        solves together (one launch per stage operation), then per solve
+        the time advance and adapt_tree.  Returns the block-RK-stages done."""
+        if not self.solves:
+            return 0
+        bt = butcher_rk4() if butcher is None else butcher
+        pool, w = self.pool, self.w
+        hvs, xs, grp, pats = [], [], [], []
+        for i, s in enumerate(self.solves):
+            W.sync_tree(pool, s.forest)
+            tc, lv, hv, _ = s.forest.blocks()
+            ids = hv[lv < s.spec.cfg.jmax]
+            if ids.size:
+                refine_on_device(pool, s.forest, ids, w.order, w.g_rhs, presync=None, attach=False)
+            tc, lv, hv, x = s.forest.blocks()
+            s.rk_blocks = len(hv)
+            if s.mask_p is not None:
+                W.create_mask(pool, s.mask_p, hv, x)
+            hvs.append(hv)
+            xs.append(x)
+            grp.append(np.full(len(hv), i, np.int32))
+            pats.append(s.forest.patches(w.g_rhs, w.g_rhs, w.order))
+        pool.set_active(np.concatenate(hvs), np.concatenate(xs))
+        pool.set_patches(0, np.concatenate(pats))
+        pool.set_block_groups(np.concatenate(grp), len(self.solves))
+        dts = pool.dt_groups(self.acm_c)
+        dts = np.array([clamp_dt(d, s.time, s.spec.tp) for d, s in zip(dts, self.solves)])
+        pool.rk_step_groups(self.acm_c, bt, dts, path)
+        stages = 0
+        for s, dt in zip(self.solves, dts):
+            s.time += float(dt)
+            s.iteration += 1
+            stages += 4 * s.rk_blocks
+            W.adapt_tree(pool, s.forest, s.adapt_p)
+        return stages
+
+    def close(self):
+        for s in list(self.solves):
+            s.close()
+        self.solves = []
+        self.pool.close()
+
+
+class Batch:
+    """Independent solves on shared pools, one pool per block shape."""
+
+    def __init__(self, slots_per_pool: Dict[tuple, int], n_work_slots: int = 5):
+        """slots_per_pool: {(dim, Bs): pool slots} -- the device memory set
+        aside per shape (the reference's --memory)."""
+        self.slots_per_pool = dict(slots_per_pool)
+        self.n_work_slots = n_work_slots
+        self.groups: Dict[tuple, PoolGroup] = {}
+        self.next_rid = 0
+
+    def key(self, spec: SolveSpec):
+        return (spec.acm.dim, spec.cfg.bs, spec.cfg.wavelet, spec.mask is not None, _acm_key(spec.acm))
+
+    def group_for(self, spec: SolveSpec) -> PoolGroup:
+        k = self.key(spec)
+        if k not in self.groups:
+            n = self.slots_per_pool[(spec.acm.dim, spec.cfg.bs)]
+            self.groups[k] = PoolGroup(spec.acm, spec.cfg.bs, spec.cfg.wavelet, n, spec.mask is not None,
+                                       self.n_work_slots)
+        return self.groups[k]
+
+    def admit(self, spec: SolveSpec, rid=None) -> Optional[Solve]:
+        """Admit a solve if its pool has a free slot range; None otherwise."""
+        if rid is None:
+            rid = self.next_rid
+            self.next_rid += 1
+        return self.group_for(spec).admit(rid, spec)
+
+    @property
+    def solves(self):
+        return [s for g in self.groups.values() for s in g.solves]
+
+    def step(self, path=PATH_FUSED):
+        """One iteration of every admitted solve; returns block-RK-stages."""
+        return sum(g.step(path) for g in self.groups.values())
+
+    def retire_done(self):
+        """Remove finished solves; returns [(rid, tag, state)]."""
+        out = []
+        for g in self.groups.values():
+            for s in [s for s in g.solves if s.done]:
+                out.append((s.rid, s.spec.tag, s.state()))
+                g.retire(s)
+        return out
+
+    def close(self):
+        for g in self.groups.values():
+            g.close()
+        self.groups = {}
+
+
+def run_alone(spec: SolveSpec, slots: int, path=PATH_FUSED):
+    """The solve on a batch of its own (the reference's one-forest run)."""
+    b = Batch({(spec.acm.dim, spec.cfg.bs): slots})
+    try:
+        s = b.admit(spec)
+        while not s.done:
+            b.step(path)
+        return s.state()
+    finally:
+        b.close()
+

@@ -15,11 +15,13 @@
 #include <array>
 #include <cstdint>
 #include <cstdlib>
+#include <memory>
 #include <unordered_map>
 #include <vector>
 
 #include "wh_forest_impl.h"
 #include "wh_internal.h"
+#include "wh_prof.h"
 
 namespace wh {
 namespace {
@@ -250,6 +252,7 @@ enum CeCase { CE_TREE = 0, CE_LEVEL = 1, CE_REF = 2 };
 // coarse_extension_modify (reconstruction_step.f90:3-100) on WH_BLOCK
 int ce_modify(Tree &T, CeCase ce, int s_level, int s_ref, bool clear_wc, bool copy_sc)
 {
+    WH_PROF("ce_modify");
     int dirs[26][3];
     const int nd = dirs_of(T.dim, dirs);
     std::vector<wh_box> boxes;
@@ -290,6 +293,7 @@ int src_sel(const Tree &T, const SyncSpec &S, int sender, bool res_pred)
 
 int tree_sync(Tree &T, const SyncSpec &S)
 {
+    WH_PROF("tree_sync");
     const int dim = T.dim, g = T.g, order = T.w.order;
     const bool filt = S.filter && T.w.lifted;
     int dirs[26][3];
@@ -298,6 +302,7 @@ int tree_sync(Tree &T, const SyncSpec &S)
     std::vector<wh_patch> st1, st23;
     std::vector<int> restrict_senders;  // tree indices
     std::vector<std::pair<int, std::array<int, 3>>> ch;
+    std::unique_ptr<ProfScope> gen(new ProfScope("tree_sync.gen"));
     for (int r = 0; r < (int)T.b.size(); ++r) {
         const TNode &R = T.b[r];
         if (R.status == REF_TMP_EMPTY) continue;
@@ -406,6 +411,7 @@ int tree_sync(Tree &T, const SyncSpec &S)
             st23.push_back(p);
         }
     }
+    gen.reset();
     if (int rc = run_patch_list(T.P, A_U, A_T, A_FILT, st1)) return rc;
     if (filt && !restrict_senders.empty()) {
         // restrict_copy_at_CE: HD-filtered copy of every restriction sender,
@@ -442,6 +448,7 @@ int tree_sync(Tree &T, const SyncSpec &S)
 // init_full_tree (adapt_tree.f90:995-1084): mothers down to jmin, REF_TMP_EMPTY
 int init_full_tree(Tree &T, int jmin)
 {
+    WH_PROF("init_full_tree");
     bool changed = true;
     while (changed) {
         changed = false;
@@ -461,6 +468,7 @@ int init_full_tree(Tree &T, int jmin)
 // their SC (interior points g, g+2, ...) to the mother's quadrant/octant
 int d2m(Tree &T, int level)
 {
+    WH_PROF("d2m");
     std::vector<wh_patch> p;
     for (int i = 0; i < (int)T.b.size(); ++i) {
         if (T.b[i].level != level) continue;
@@ -500,6 +508,7 @@ std::vector<int32_t> hvy_of(const Tree &T, const std::vector<int> &idx)
 // wavelet_decompose_full_tree (adapt_tree.f90:264-545)
 int decompose_full_tree(Tree &T, const wh_adapt_params &p)
 {
+    WH_PROF("decompose_full_tree");
     const wh_wavelet &w = T.w;
     const int n_leaf = (int)T.b.size();
     std::vector<int> backup(n_leaf);
@@ -556,6 +565,7 @@ int decompose_full_tree(Tree &T, const wh_adapt_params &p)
 // ensure_completeness_block(check_daughters) (ensure_completeness_block.f90)
 void ensure_gradedness(Tree &T, int stay)
 {
+    WH_PROF("ensure_gradedness");
     int dirs[26][3];
     const int nd = dirs_of(T.dim, dirs);
     const int nsis = 1 << T.dim;
@@ -643,6 +653,7 @@ void ensure_gradedness(Tree &T, int stay)
 // wavelet_reconstruct_full_tree_CEoptimized (adapt_tree.f90:686-987)
 int reconstruct_tree(Tree &T, const wh_adapt_params &p)
 {
+    WH_PROF("reconstruct_tree");
     const wh_wavelet &w = T.w;
     int dirs[26][3];
     const int nd = dirs_of(T.dim, dirs);
@@ -736,6 +747,7 @@ wh_box interior_box(const Tree &T, int i)
 // (coarseningIndicator_tree.f90:160-163)
 int leaf_linfty(Tree &T, const int32_t *groups, double *norm)
 {
+    WH_PROF("leaf_linfty");
     const int neq = T.P->d.neq;
     std::vector<wh_box> boxes;
     for (int i = 0; i < (int)T.b.size(); ++i)
@@ -767,6 +779,7 @@ int leaf_linfty(Tree &T, const int32_t *groups, double *norm)
 // (0,1) or not constant stays
 int mask_criterion(Tree &T, const wh_adapt_params &p)
 {
+    WH_PROF("mask_criterion");
     std::vector<int32_t> ids;
     std::vector<double> x0dx;
     std::vector<int> idx;
@@ -795,6 +808,7 @@ int mask_criterion(Tree &T, const wh_adapt_params &p)
 // Nwc of their common boundary
 int security_zone(Tree &T, const wh_adapt_params &p, const double *norm)
 {
+    WH_PROF("security_zone");
     int dirs[26][3];
     const int nd = dirs_of(T.dim, dirs);
     const int neq = T.P->d.neq;
@@ -853,7 +867,7 @@ int check_pool(wh_pool *P, wh_forest *F)
 {
     WH_CHECK(P && F, WH_E_ARG, "adapt: null argument");
     WH_CHECK(P->ws.set, WH_E_STATE, "adapt: set the pool's wavelet first (wh_pool_set_wavelet)");
-    WH_CHECK(F->dim == P->d.dim && F->g == P->d.g && F->nb_max <= P->d.nb_max, WH_E_ARG,
+    WH_CHECK(F->dim == P->d.dim && F->g == P->d.g && F->slot0 + F->nb_max <= P->d.nb_max, WH_E_ARG,
              "adapt: forest and pool disagree on dim / ghost nodes / capacity");
     for (int d = 0; d < F->dim; ++d)
         WH_CHECK(F->bs[d] == P->d.bs[d] && F->bs[d] % 2 == 0 && 2 * F->g <= F->bs[d], WH_E_ARG,
@@ -870,17 +884,20 @@ extern "C" {
 
 int wh_sync_tree(wh_pool *P, wh_forest *F, int array)
 {
+    WH_PROF("api.sync_tree");
     using namespace wh;
     if (int rc = check_pool(P, F)) return rc;
     WH_CHECK(array == WH_BLOCK, WH_E_ARG, "wh_sync_tree: only WH_BLOCK is supported");
     Tree T(F, P);
     SyncSpec S;
     S.gm = S.gp = T.g;
-    return tree_sync(T, S);
+    if (int rc = tree_sync(T, S)) return rc;
+    return stage_sync(P);  // the data is final when the call returns
 }
 
 int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
 {
+    WH_PROF("api.adapt_tree");
     using namespace wh;
     if (int rc = check_pool(P, F)) return rc;
     WH_CHECK(p && p->jmin >= 0 && p->jmax <= F->jmax && p->jmin <= p->jmax &&
@@ -894,6 +911,7 @@ int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
     auto fail = [&](int rc) {
         // restore the forest's slots; device data of the leaves is not restored
         F->free_hvy = free0;
+        (void)stage_sync(P);
         return rc;
     };
     if (int rc = decompose_full_tree(T, *p)) return fail(rc);
@@ -955,7 +973,8 @@ int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
     store_leaves(T);
     SyncSpec S;
     S.gm = S.gp = T.g;
-    return tree_sync(T, S);
+    if (int rc = tree_sync(T, S)) return rc;
+    return stage_sync(P);  // the data is final when the call returns
 }
 
 }  // extern "C"

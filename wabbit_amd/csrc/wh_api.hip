@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "wh_internal.h"
+#include "wh_prof.h"
 
 namespace wh {
 
@@ -93,8 +94,10 @@ static bool fused_ok(const double *bt, int nrows)
     return true;
 }
 
-// one step of the fused path; dt is taken from the device scalar DS_DT
-static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows)
+// one step of the fused path; dt is taken from the device scalar DS_DT, or
+// per block from dtb (block groups)
+static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows,
+                      const double *dtb = nullptr)
 {
     const int s = nrows - 1;  // number of stages
     double *S = P->block;
@@ -106,7 +109,7 @@ static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows)
     // stage 1: input S, writes acc and T[0]
     if ((rc = timed(P, 0, [&] {
              return launch_fused_stage(P, a, S, T[0], nullptr, acc, nullptr, 0, bt[1 * nrows + 1],
-                                       bt[s * nrows + 1], 1);
+                                       bt[s * nrows + 1], dtb);
          })))
         return rc;
     int cur = 0;
@@ -117,14 +120,14 @@ static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows)
             double *tout = T[cur ^ 1];
             if ((rc = timed(P, 0, [&] {
                      return launch_fused_stage(P, a, tin, tout, S, acc, nullptr, 1,
-                                               bt[j * nrows + j], bt[s * nrows + j], 1);
+                                               bt[j * nrows + j], bt[s * nrows + j], dtb);
                  })))
                 return rc;
             cur ^= 1;
         } else {
             if ((rc = timed(P, 0, [&] {
                      return launch_fused_stage(P, a, tin, nullptr, nullptr, acc, S, 2, 0.0,
-                                               bt[s * nrows + j], 1);
+                                               bt[s * nrows + j], dtb);
                  })))
                 return rc;
         }
@@ -132,9 +135,11 @@ static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows)
     return WH_OK;
 }
 
-// reference-structured step; per_block = 1 launches every operation once per block
+// reference-structured step; per_block = 1 launches every operation once per
+// block, then dt_pos (host, per active-list position) may give each block its
+// own dt (block groups)
 static int generic_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows, double dt,
-                        int per_block)
+                        int per_block, const double *dt_pos = nullptr)
 {
     const int ncol = nrows;
     double *S = P->block;
@@ -160,11 +165,17 @@ static int generic_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows
         }
         return WH_OK;
     };
+    // raw holds the tableau entries a; the kernels get dt * a, with dt_pos
+    // block k's own dt_pos[k] * a
     auto combine = [&](double *out, const double *base, int nt, const double *const *terms,
-                       const double *coefs) -> int {
-        if (!per_block) return launch_combine(P, out, base, nt, terms, coefs, P->d_active, P->nact);
+                       const double *raw) -> int {
+        std::vector<double> ck(nt > 0 ? nt : 1);
+        for (int t = 0; t < nt; ++t) ck[t] = dt * raw[t];
+        if (!per_block) return launch_combine(P, out, base, nt, terms, ck.data(), P->d_active, P->nact);
         for (int k = 0; k < P->nact; ++k) {
-            int rc = launch_combine(P, out, base, nt, terms, coefs, P->d_active + k, 1);
+            if (dt_pos)
+                for (int t = 0; t < nt; ++t) ck[t] = dt_pos[k] * raw[t];
+            int rc = launch_combine(P, out, base, nt, terms, ck.data(), P->d_active + k, 1);
             if (rc) return rc;
         }
         return WH_OK;
@@ -181,7 +192,7 @@ static int generic_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows
             const double c = bt[(j - 1) * ncol + (l - 1)];
             if (std::fabs(c) < 1.0e-8) continue;
             terms.push_back(W(l - 1));
-            coefs.push_back(dt * c);
+            coefs.push_back(c);
         }
         if ((rc = combine(S, W(0), (int)terms.size(), terms.data(), coefs.data()))) return rc;
         if ((rc = sync(S))) return rc;
@@ -194,7 +205,7 @@ static int generic_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows
         const double c = bt[(nrows - 1) * ncol + (j - 1)];
         if (std::fabs(c) < 1.0e-8) continue;
         terms.push_back(W(j - 1));
-        coefs.push_back(dt * c);
+        coefs.push_back(c);
     }
     return combine(S, W(0), (int)terms.size(), terms.data(), coefs.data());
 }
@@ -324,6 +335,9 @@ int wh_pool_destroy(wh_pool *P)
     for (double *p : {P->block, P->work, P->tmp, P->mask, P->scal, P->d_x0dx})
         if (p) (void)hipFree(p);
     if (P->d_active) (void)hipFree(P->d_active);
+    if (P->d_grp) (void)hipFree(P->d_grp);
+    if (P->d_dtb) (void)hipFree(P->d_dtb);
+    if (P->d_dtg) (void)hipFree(P->d_dtg);
     if (P->h_scal) (void)hipHostFree(P->h_scal);
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
@@ -362,11 +376,13 @@ static int xfer(wh_pool *P, int array, int first, int count, void *host, bool up
 
 int wh_pool_upload(wh_pool *P, int array, int first, int count, const double *host)
 {
+    WH_PROF("wh_pool_upload");
     return xfer(P, array, first, count, const_cast<double *>(host), true);
 }
 
 int wh_pool_download(wh_pool *P, int array, int first, int count, double *host)
 {
+    WH_PROF("wh_pool_download");
     return xfer(P, array, first, count, host, false);
 }
 
@@ -401,6 +417,7 @@ int wh_synchronize(wh_pool *P)
 
 int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
 {
+    WH_PROF("wh_set_active");
     WH_CHECK(P && (n == 0 || (act && x0dx)), WH_E_ARG, "null argument");
     WH_CHECK(n >= 0 && n <= P->d.nb_max, WH_E_ARG, "active list longer than the pool");
     std::vector<char> seen(P->d.nb_max, 0);
@@ -430,12 +447,14 @@ int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
         WH_HIP(hipStreamSynchronize(P->stream));
     }
     P->nact = n;
+    P->ngrp = 0;  // block groups refer to the previous active list
     P->topo_version++;
     return WH_OK;
 }
 
 int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatch)
 {
+    WH_PROF("wh_set_patches");
     WH_CHECK(P && (npatch == 0 || patches), WH_E_ARG, "null argument");
     WH_CHECK(sync_case == 0 || sync_case == 1, WH_E_ARG, "sync_case must be 0 or 1");
     WH_CHECK(npatch >= 0, WH_E_ARG, "negative patch count");
@@ -558,6 +577,7 @@ int wh_divergence_guard(wh_pool *P, int in_array, double *maxabs)
 
 int wh_dt_reduce(wh_pool *P, const wh_acm_params *p, double *dt_out)
 {
+    WH_PROF("wh_dt_reduce");
     WH_CHECK(P && p && dt_out, WH_E_ARG, "null argument");
     if (int rc = check_acm(P, p)) return rc;
     const AcmDev a = make_acm_dev(p);
@@ -584,6 +604,7 @@ int wh_rk_combine(wh_pool *P, int out_array, int base_array, int nterms, const i
 int wh_rk_step(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows, double /*time*/,
                double dt, int path)
 {
+    WH_PROF("wh_rk_step");
     WH_CHECK(P && p && bt, WH_E_ARG, "null argument");
     WH_CHECK(nrows >= 2 && nrows <= 9, WH_E_ARG, "butcher tableau size");
     if (int rc = check_acm(P, p)) return rc;
@@ -598,6 +619,81 @@ int wh_rk_step(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows, 
     WH_CHECK(path == 0 || path == 1, WH_E_ARG, "path must be 0, 1 or 2");
     WH_CHECK(P->d.n_work_slots >= nrows, WH_E_ARG, "need nrows work slots");
     return generic_step(P, a, bt, nrows, dt, path);
+}
+
+int wh_set_block_groups(wh_pool *P, const int32_t *group, int n, int ngroups)
+{
+    WH_CHECK(P && (n == 0 || group), WH_E_ARG, "null argument");
+    WH_CHECK(n == P->nact, WH_E_ARG, "one group per active block (set the active list first)");
+    WH_CHECK(ngroups >= 1 && ngroups <= (1 << 20), WH_E_ARG, "ngroups out of range");
+    for (int k = 0; k < n; ++k)
+        WH_CHECK(group[k] >= 0 && group[k] < ngroups, WH_E_ARG, "group index out of range");
+    WH_HIP(hipStreamSynchronize(P->stream));
+    if (n > P->grp_cap) {
+        if (P->d_grp) WH_HIP(hipFree(P->d_grp));
+        if (P->d_dtb) WH_HIP(hipFree(P->d_dtb));
+        P->d_grp = nullptr;
+        P->d_dtb = nullptr;
+        WH_HIP(hipMalloc(&P->d_grp, sizeof(int32_t) * n));
+        WH_HIP(hipMalloc(&P->d_dtb, sizeof(double) * n));
+        P->grp_cap = n;
+    }
+    if (ngroups > P->dtg_cap) {
+        if (P->d_dtg) WH_HIP(hipFree(P->d_dtg));
+        P->d_dtg = nullptr;
+        WH_HIP(hipMalloc(&P->d_dtg, sizeof(unsigned long long) * ngroups));
+        P->dtg_cap = ngroups;
+    }
+    P->h_grp.assign(group, group + n);
+    if (n) {
+        WH_HIP(hipMemcpyAsync(P->d_grp, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
+        WH_HIP(hipStreamSynchronize(P->stream));
+    }
+    P->ngrp = ngroups;
+    return WH_OK;
+}
+
+int wh_dt_reduce_groups(wh_pool *P, const wh_acm_params *p, double *dt_out)
+{
+    WH_CHECK(P && p && dt_out, WH_E_ARG, "null argument");
+    WH_CHECK(P->ngrp > 0, WH_E_STATE, "no block groups set");
+    if (int rc = check_acm(P, p)) return rc;
+    const AcmDev a = make_acm_dev(p);
+    if (int rc = launch_dt_groups(P, a)) return rc;
+    std::vector<unsigned long long> bits(P->ngrp);
+    WH_HIP(hipMemcpyAsync(bits.data(), P->d_dtg, sizeof(unsigned long long) * P->ngrp,
+                          hipMemcpyDeviceToHost, P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    for (int i = 0; i < P->ngrp; ++i) std::memcpy(dt_out + i, &bits[i], sizeof(double));
+    return WH_OK;
+}
+
+int wh_rk_step_groups(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows,
+                      const double *dt_groups, int path)
+{
+    WH_CHECK(P && p && bt && dt_groups, WH_E_ARG, "null argument");
+    WH_CHECK(nrows >= 2 && nrows <= 9, WH_E_ARG, "butcher tableau size");
+    WH_CHECK(P->ngrp > 0, WH_E_STATE, "no block groups set");
+    for (int i = 0; i < P->ngrp; ++i)
+        WH_CHECK(dt_groups[i] >= 0.0, WH_E_ARG, "negative dt");
+    if (int rc = check_acm(P, p)) return rc;
+    const AcmDev a = make_acm_dev(p);
+    std::vector<double> dt_pos(P->nact);
+    for (int k = 0; k < P->nact; ++k) dt_pos[k] = dt_groups[P->h_grp[k]];
+    if (path == 2) {
+        WH_CHECK(fused_ok(bt, nrows), WH_E_ARG, "tableau not supported by the fused path");
+        WH_CHECK(P->d.n_work_slots >= 3, WH_E_ARG, "fused path needs 3 work slots");
+        if (P->nact)
+            WH_HIP(hipMemcpyAsync(P->d_dtb, dt_pos.data(), sizeof(double) * P->nact,
+                                  hipMemcpyHostToDevice, P->stream));
+        int rc = fused_step(P, a, bt, nrows, P->d_dtb);
+        if (rc) return rc;
+        WH_HIP(hipStreamSynchronize(P->stream));  // dt_pos is read by the copy above
+        return WH_OK;
+    }
+    WH_CHECK(path == 1, WH_E_ARG, "block groups run on path 1 (per block) or 2 (fused)");
+    WH_CHECK(P->d.n_work_slots >= nrows, WH_E_ARG, "need nrows work slots");
+    return generic_step(P, a, bt, nrows, 1.0, 1, dt_pos.data());
 }
 
 int wh_time_set(wh_pool *P, double time, int iteration)
@@ -783,6 +879,7 @@ int launch_refine(wh_pool *P, double *arr, const int32_t *d_pairs, int npairs, i
 extern "C" int wh_refine_blocks(wh_pool *P, int array, const int32_t *triples, int ntriples,
                                 int order)
 {
+    WH_PROF("wh_refine_blocks");
     WH_CHECK(P && (ntriples == 0 || triples), WH_E_ARG, "null argument");
     WH_CHECK(order == 2 || order == 4 || order == 6, WH_E_ARG, "order must be 2, 4 or 6");
     WH_CHECK(ntriples >= 0, WH_E_ARG, "negative count");
@@ -810,13 +907,9 @@ extern "C" int wh_refine_blocks(wh_pool *P, int array, const int32_t *triples, i
     for (int h = 0; h < P->d.nb_max; ++h)
         WH_CHECK(!(mother[h] && daughter[h]), WH_E_ARG, "a slot is both mother and daughter");
     if (ntriples == 0) return WH_OK;
-    int32_t *d_tr = nullptr;
-    WH_HIP(hipMallocAsync((void **)&d_tr, sizeof(int32_t) * 3 * ntriples, P->stream));
-    WH_HIP(hipMemcpyAsync(d_tr, triples, sizeof(int32_t) * 3 * ntriples, hipMemcpyHostToDevice,
-                          P->stream));
-    int rc = launch_refine(P, arr, d_tr, ntriples, order);
-    (void)hipFreeAsync(d_tr, P->stream);
-    if (rc) return rc;
-    WH_HIP(hipStreamSynchronize(P->stream));
-    return WH_OK;
+    const size_t bytes = sizeof(int32_t) * 3 * ntriples;
+    if (int rc = stage_begin(P, bytes)) return rc;
+    const int32_t *d_tr = (const int32_t *)stage_put(P, triples, bytes);
+    if (!d_tr) return WH_E_HIP;
+    return launch_refine(P, arr, d_tr, ntriples, order);
 }

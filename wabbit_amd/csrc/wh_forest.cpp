@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "wh_forest_impl.h"
+#include "wh_prof.h"
 
 using Block = wh::FBlock;
 
@@ -63,6 +64,18 @@ int wh_forest_create(wh_forest **out, int dim, int jmax, const int *bs, int g,
     return WH_OK;
 }
 
+int wh_forest_set_slot_base(wh_forest *F, int first)
+{
+    if (!F || first < 0 || !F->blocks.empty()) {
+        wh::set_error("wh_forest_set_slot_base: bad argument or the forest has blocks");
+        return WH_E_ARG;
+    }
+    F->slot0 = first;
+    F->free_hvy.clear();
+    for (int h = F->nb_max - 1; h >= 0; --h) F->free_hvy.push_back(first + h);
+    return WH_OK;
+}
+
 int wh_forest_destroy(wh_forest *F)
 {
     delete F;
@@ -79,16 +92,17 @@ int wh_forest_set_blocks(wh_forest *F, int n, const int64_t *tc, const int32_t *
     std::vector<char> used(F->nb_max, 0);
     F->blocks.clear();
     for (int i = 0; i < n; ++i) {
-        if (hvy[i] < 0 || hvy[i] >= F->nb_max || used[hvy[i]] || level[i] < 0 || level[i] > F->jmax) {
+        const int h = hvy[i] - F->slot0;
+        if (h < 0 || h >= F->nb_max || used[h] || level[i] < 0 || level[i] > F->jmax) {
             wh::set_error("wh_forest_set_blocks: hvy id / level out of range or duplicate");
             return WH_E_ARG;
         }
-        used[hvy[i]] = 1;
+        used[h] = 1;
         F->blocks.push_back(Block{tc[i], level[i], hvy[i], 0});
     }
     F->free_hvy.clear();
     for (int h = F->nb_max - 1; h >= 0; --h)
-        if (!used[h]) F->free_hvy.push_back(h);
+        if (!used[h]) F->free_hvy.push_back(F->slot0 + h);
     F->rebuild_map();
     if ((int)F->map.size() != n) {
         wh::set_error("wh_forest_set_blocks: duplicate (level, treecode)");
@@ -131,6 +145,7 @@ int wh_forest_get_blocks(const wh_forest *F, int cap, int64_t *tc, int32_t *leve
 int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_patch *out,
                       int cap, int *npatch)
 {
+    WH_PROF("forest_patches");
     if (!F || !npatch || gminus < 0 || gplus < 0 || gminus > F->g || gplus > F->g ||
         (order != 2 && order != 4 && order != 6)) {
         wh::set_error("wh_forest_patches: bad argument");
@@ -310,6 +325,7 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
 int wh_forest_refine(wh_forest *F, int n, const int32_t *hvy_list, int32_t *pairs_out, int cap,
                      int *npairs)
 {
+    WH_PROF("forest_refine");
     if (!F || n < 0 || (n && !hvy_list) || !npairs) return WH_E_ARG;
     const int nd = 1 << F->dim;
     if ((int)F->free_hvy.size() < n * nd || cap < n * nd) {

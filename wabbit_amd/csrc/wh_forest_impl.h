@@ -34,6 +34,7 @@ struct wh_forest {
     double domain[3] = {1, 1, 1};
     int periodic[3] = {1, 1, 1};
     int nb_max = 0;
+    int slot0 = 0;                          // the forest's pool slots: [slot0, slot0 + nb_max)
     std::vector<wh::FBlock> blocks;         // active blocks (leaf list)
     std::unordered_map<uint64_t, int> map;  // (level, tc) -> index in blocks
     std::vector<int> free_hvy;              // free pool slots

@@ -69,6 +69,13 @@ struct WaveletWS {
     size_t box_cap = 0;
     double *d_out = nullptr;
     size_t out_cap = 0;
+    // staging of the host-built lists (block ids, boxes, patches): copied to
+    // a pinned buffer and from there to the same offset of a device buffer,
+    // so that the copies run asynchronously on the pool stream; the space is
+    // reclaimed only after a stream synchronisation
+    char *h_stage = nullptr, *d_stage = nullptr;
+    size_t stage_cap = 0, stage_off = 0;
+    const int32_t *cur_ids = nullptr;  // device block list of the current wavelet op
 };
 
 // Host-side time controls (calculate_time_step.f90:58-118) evaluated on device.
@@ -126,6 +133,13 @@ struct wh_pool {
     int k3d_ysplit = -1;
     // wavelet transforms / adapt_tree
     wh::WaveletWS ws;
+    // block groups (independent forests in one pool): group of each active
+    // block, per-group dt and its per-position copy for the stage kernels
+    int ngrp = 0, grp_cap = 0, dtg_cap = 0;
+    std::vector<int32_t> h_grp;
+    int32_t *d_grp = nullptr;
+    double *d_dtb = nullptr;
+    unsigned long long *d_dtg = nullptr;
 };
 
 namespace wh {
@@ -141,9 +155,12 @@ int launch_combine(wh_pool *P, double *out, const double *base, int nterms,
                    int nact);
 int launch_dt(wh_pool *P, const AcmDev &a);
 int launch_maxabs(wh_pool *P, const double *in);
+// dtb: per active-list position dt (block groups), or null for the pool's DS_DT
 int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *tout,
                        const double *u0, double *acc, double *sout, int mode, double acoef,
-                       double bcoef, int coef_from_dev);
+                       double bcoef, const double *dtb);
+// per-group minimum block dt into P->d_dtg (bit patterns) for the P->ngrp groups
+int launch_dt_groups(wh_pool *P, const AcmDev &a);
 int launch_dt_finalize(wh_pool *P, const TimeDev &tc);
 int launch_time_advance(wh_pool *P);
 int launch_set_scalar(wh_pool *P, int which, double v);
@@ -153,6 +170,14 @@ int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const i
 
 // wavelet workspace (wh_wavelet.hip)
 void free_wavelet_ws(wh_pool *P);
+// staging: stage_begin reserves `bytes` (aligned pieces included) for the
+// following stage_put calls, synchronising the stream and reusing the
+// buffer from its start when it is full; stage_put copies host data and
+// returns its device address; stage_sync waits for the stream and frees all
+// staged space
+int stage_begin(wh_pool *P, size_t bytes);
+void *stage_put(wh_pool *P, const void *src, size_t bytes);
+int stage_sync(wh_pool *P);
 // device-side steps used by the adapt orchestration (wh_adapt.cpp); the
 // host vectors are uploaded on the pool's stream before use
 int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<wh_patch> &patches);

@@ -173,6 +173,7 @@ struct StageArgs {
     const double *u0;
     double *acc, *tout, *sout;
     const double *scal;
+    const double *dtb;  // per active-list position dt (independent forests), or null
     double a, b;
     int mode, use_b, use_a;
 };
@@ -180,11 +181,11 @@ struct StageArgs {
 struct NoEpilogue {};
 
 template <int NC>
-__device__ __forceinline__ void stage_epilogue(const StageArgs &sa, size_t o, size_t cs,
+__device__ __forceinline__ void stage_epilogue(const StageArgs &sa, int k, size_t o, size_t cs,
                                                const double *r, const double *ucen,
                                                ptrdiff_t ucs)
 {
-    const double dt = sa.scal[DS_DT];
+    const double dt = sa.dtb ? sa.dtb[k] : sa.scal[DS_DT];
     const double ca = dt * sa.a, cb = dt * sa.b;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
         acm_point<2, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
         const size_t o = (size_t)b * G.blk + gi;
         if constexpr (FUSED) {
-            stage_epilogue<3>(sa, o, G.cs, r, uc, tcs);
+            stage_epilogue<3>(sa, k, o, G.cs, r, uc, tcs);
         } else {
 #pragma unroll
             for (int c = 0; c < 3; ++c) out[o + c * G.cs] = r[c];
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, do
     acm_point<3, S, SKEW>(A, inv, inv2, a, mk, G.cs, r);
     const size_t o = (size_t)b * G.blk + gi;
     if constexpr (FUSED) {
-        stage_epilogue<4>(sa, o, G.cs, r, uc, G.cs);
+        stage_epilogue<4>(sa, k, o, G.cs, r, uc, G.cs);
     } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(512) void k_rhs3d_zm(const double *__restrict__ in,
         const size_t o = (size_t)b * G.blk + gi;
         if constexpr (FUSED) {
             const double u0c[4] = {q[0][R], q[1][R], q[2][R], q[3][R]};
-            stage_epilogue<4>(sa, o, G.cs, r, u0c, 1);
+            stage_epilogue<4>(sa, k, o, G.cs, r, u0c, 1);
         } else {
 #pragma unroll
             for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
@@ -513,9 +514,10 @@ int launch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *out, const
 
 int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *tout,
                        const double *u0, double *acc, double *sout, int mode, double acoef,
-                       double bcoef, int /*coef_from_dev*/)
+                       double bcoef, const double *dtb)
 {
     StageArgs sa{};
+    sa.dtb = dtb;
     sa.u0 = u0;
     sa.acc = acc;
     sa.tout = tout;
@@ -800,7 +802,7 @@ __device__ double block_max(double v)
 __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ in,
                                             const int32_t *__restrict__ act,
                                             const double *__restrict__ x0dx, Geo G, AcmDev a,
-                                            unsigned long long *dtmin)
+                                            unsigned long long *dtmin, const int32_t *__restrict__ grp)
 {
     const int k = blockIdx.x, b = act[k];
     const int npts = G.bs0 * G.bs1 * G.bs2;
@@ -824,7 +826,7 @@ __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ in,
         if (a.gamma_p > 0.0) dt = fmin(dt, a.CFL_eta * a.gamma_p);
         if (a.penal) dt = fmin(dt, a.CFL_eta * a.C_eta);
         if (a.sponge) dt = fmin(dt, a.CFL_eta * a.C_sponge);
-        atomicMin(dtmin, (unsigned long long)__double_as_longlong(dt));
+        atomicMin(grp ? dtmin + grp[k] : dtmin, (unsigned long long)__double_as_longlong(dt));
     }
 }
 
@@ -859,7 +861,31 @@ int launch_dt(wh_pool *P, const AcmDev &a)
     if (P->nact > 0) {
         const Geo G = make_geo(P);
         hipLaunchKernelGGL(k_dt, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
-                           P->d_x0dx, G, a, (unsigned long long *)(P->scal + DS_DTMIN_BITS));
+                           P->d_x0dx, G, a, (unsigned long long *)(P->scal + DS_DTMIN_BITS),
+                           (const int32_t *)nullptr);
+    }
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_dt_groups(wh_pool *P, const AcmDev &a)
+{
+    const int n = P->ngrp;
+    const double init = 9.0e9;
+    unsigned long long bits;
+    std::memcpy(&bits, &init, sizeof bits);
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, P->stream, P->d_dtg, n,
+                       bits);
+    if (P->nact > 0) {
+        const Geo G = make_geo(P);
+        hipLaunchKernelGGL(k_dt, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
+                           P->d_x0dx, G, a, P->d_dtg, (const int32_t *)P->d_grp);
     }
     WH_HIP(hipGetLastError());
     return WH_OK;

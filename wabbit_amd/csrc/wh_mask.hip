@@ -162,28 +162,16 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
     const int n = (int)ids.size();
     if (n == 0) return WH_OK;
     for (int32_t h : ids) WH_CHECK(h >= 0 && h < P->d.nb_max, WH_E_ARG, "mask: hvy id out of range");
-    WaveletWS &W = P->ws;
-    // ids then x0dx in the workspace buffers
-    if (W.ids_cap < (size_t)n) {
-        if (W.d_ids) WH_HIP(hipFree(W.d_ids));
-        W.d_ids = nullptr;
-        WH_HIP(hipMalloc(&W.d_ids, sizeof(int32_t) * n));
-        W.ids_cap = n;
-    }
-    if (W.out_cap < (size_t)n * 6) {
-        if (W.d_out) WH_HIP(hipFree(W.d_out));
-        W.d_out = nullptr;
-        WH_HIP(hipMalloc(&W.d_out, sizeof(double) * n * 6));
-        W.out_cap = (size_t)n * 6;
-    }
-    WH_HIP(hipMemcpyAsync(W.d_ids, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
-    WH_HIP(hipMemcpyAsync(W.d_out, x0dx.data(), sizeof(double) * n * 6, hipMemcpyHostToDevice, P->stream));
+    WH_CHECK(x0dx.size() == (size_t)n * 6, WH_E_ARG, "mask: six origin/spacing values per block");
+    if (int rc = stage_begin(P, sizeof(int32_t) * n + sizeof(double) * n * 6)) return rc;
+    const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n);
+    const double *d_x0dx = (const double *)stage_put(P, x0dx.data(), sizeof(double) * n * 6);
+    if (!d_ids || !d_x0dx) return WH_E_HIP;
     const long tot = (long)n * (long)P->cs;
     hipLaunchKernelGGL(k_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask, P->mblk,
-                       P->cs, W.d_ids, W.d_out, n, P->nx, P->ny, P->d.g, P->d.bs[0], P->d.bs[1],
+                       P->cs, d_ids, d_x0dx, n, P->nx, P->ny, P->d.g, P->d.bs[0], P->d.bs[1],
                        make_mask_dev(mp));
     WH_HIP(hipGetLastError());
-    WH_HIP(hipStreamSynchronize(P->stream));
     return WH_OK;
 }
 
@@ -194,25 +182,21 @@ int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<doub
     stats.assign((size_t)n * 3, 0.0);
     if (n == 0) return WH_OK;
     WaveletWS &W = P->ws;
-    if (W.ids_cap < (size_t)n) {
-        if (W.d_ids) WH_HIP(hipFree(W.d_ids));
-        W.d_ids = nullptr;
-        WH_HIP(hipMalloc(&W.d_ids, sizeof(int32_t) * n));
-        W.ids_cap = n;
-    }
     if (W.out_cap < (size_t)n * 3) {
+        // the buffer is only used by downloads, which synchronise: nothing in flight reads it
         if (W.d_out) WH_HIP(hipFree(W.d_out));
         W.d_out = nullptr;
         WH_HIP(hipMalloc(&W.d_out, sizeof(double) * n * 3));
         W.out_cap = (size_t)n * 3;
     }
-    WH_HIP(hipMemcpyAsync(W.d_ids, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
-    hipLaunchKernelGGL(k_mask_stats, dim3(n), dim3(256), 0, P->stream, P->mask, P->mblk, P->cs, W.d_ids, P->nx,
+    if (int rc = stage_begin(P, sizeof(int32_t) * n)) return rc;
+    const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n);
+    if (!d_ids) return WH_E_HIP;
+    hipLaunchKernelGGL(k_mask_stats, dim3(n), dim3(256), 0, P->stream, P->mask, P->mblk, P->cs, d_ids, P->nx,
                        P->d.g, P->d.bs[0], P->d.bs[1], W.d_out);
     WH_HIP(hipGetLastError());
     WH_HIP(hipMemcpyAsync(stats.data(), W.d_out, sizeof(double) * n * 3, hipMemcpyDeviceToHost, P->stream));
-    WH_HIP(hipStreamSynchronize(P->stream));
-    return WH_OK;
+    return stage_sync(P);
 }
 
 }  // namespace wh

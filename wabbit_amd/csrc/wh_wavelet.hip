@@ -19,6 +19,7 @@
 #include <cstring>
 
 #include "wh_internal.h"
+#include "wh_prof.h"
 
 namespace wh {
 
@@ -399,12 +400,9 @@ int ensure(T *&ptr, size_t &cap, size_t n)
 
 int upload_ids(wh_pool *P, const int32_t *ids, int n)
 {
-    WaveletWS &W = P->ws;
-    if (int rc = ensure(W.d_ids, W.ids_cap, (size_t)std::max(n, 1))) return rc;
-    WH_HIP(hipMemcpyAsync(W.d_ids, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
-    // the host buffer may be released after return
-    WH_HIP(hipStreamSynchronize(P->stream));
-    return WH_OK;
+    if (int rc = stage_begin(P, sizeof(int32_t) * n)) return rc;
+    P->ws.cur_ids = (const int32_t *)stage_put(P, ids, sizeof(int32_t) * n);
+    return P->ws.cur_ids ? WH_OK : WH_E_HIP;
 }
 
 int check_ids(wh_pool *P, const int32_t *ids, int n)
@@ -424,7 +422,7 @@ int launch_pass(wh_pool *P, const double *in, size_t blk_in, double *out, size_t
     const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
     const long astride = axis == 0 ? 1 : (axis == 1 ? P->nx : (long)P->nx * P->ny);
     hipLaunchKernelGGL(k_wpass, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, in, out,
-                       P->ws.d_ids, nb, A, R, axis, astride, g, T);
+                       P->ws.cur_ids, nb, A, R, axis, astride, g, T);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -437,7 +435,7 @@ int launch_copy_region(wh_pool *P, const double *in, size_t blk_in, double *out,
     const long tot = npts * P->d.neq * nb;
     const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
     hipLaunchKernelGGL(k_copy_region, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, in,
-                       out, P->ws.d_ids, nb, A, R);
+                       out, P->ws.cur_ids, nb, A, R);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -499,9 +497,61 @@ int filter_passes(wh_pool *P, const double *in, size_t blk_in, double *out, size
 
 }  // namespace
 
+static constexpr size_t STAGE_ALIGN = 256;
+
+static size_t stage_round(size_t b) { return (b + STAGE_ALIGN - 1) / STAGE_ALIGN * STAGE_ALIGN; }
+
+int stage_sync(wh_pool *P)
+{
+    WH_PROF("stage_sync");
+    WH_HIP(hipStreamSynchronize(P->stream));
+    P->ws.stage_off = 0;
+    return WH_OK;
+}
+
+int stage_begin(wh_pool *P, size_t bytes)
+{
+    WaveletWS &W = P->ws;
+    const size_t need = stage_round(bytes) + 4 * STAGE_ALIGN;  // room for a few aligned pieces
+    if (W.stage_off + need <= W.stage_cap) return WH_OK;
+    if (int rc = stage_sync(P)) return rc;
+    if (need <= W.stage_cap) return WH_OK;
+    if (W.h_stage) WH_HIP(hipHostFree(W.h_stage));
+    if (W.d_stage) WH_HIP(hipFree(W.d_stage));
+    W.h_stage = W.d_stage = nullptr;
+    W.stage_cap = 0;
+    const size_t cap = std::max<size_t>(need, std::max<size_t>(2 * W.stage_cap, (size_t)1 << 20));
+    WH_HIP(hipHostMalloc((void **)&W.h_stage, cap, hipHostMallocDefault));
+    WH_HIP(hipMalloc((void **)&W.d_stage, cap));
+    W.stage_cap = cap;
+    return WH_OK;
+}
+
+void *stage_put(wh_pool *P, const void *src, size_t bytes)
+{
+    WaveletWS &W = P->ws;
+    const size_t r = stage_round(bytes ? bytes : 1);
+    if (W.stage_off + r > W.stage_cap) {
+        set_error("staging: stage_put beyond the reserved space");
+        return nullptr;
+    }
+    char *h = W.h_stage + W.stage_off, *d = W.d_stage + W.stage_off;
+    if (bytes) {
+        std::memcpy(h, src, bytes);
+        if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, P->stream) != hipSuccess) {
+            set_error("staging: hipMemcpyAsync failed");
+            return nullptr;
+        }
+    }
+    W.stage_off += r;
+    return d;
+}
+
 void free_wavelet_ws(wh_pool *P)
 {
     WaveletWS &W = P->ws;
+    if (W.h_stage) (void)hipHostFree(W.h_stage);
+    if (W.d_stage) (void)hipFree(W.d_stage);
     if (W.d_ids) (void)hipFree(W.d_ids);
     if (W.d_patches) (void)hipFree(W.d_patches);
     if (W.d_pstart) (void)hipFree(W.d_pstart);
@@ -512,6 +562,7 @@ void free_wavelet_ws(wh_pool *P)
 
 int run_blocks_op(wh_pool *P, int op, int src_array, int dst_array, const std::vector<int32_t> &ids)
 {
+    WH_PROF("run_blocks_op");
     const int n = (int)ids.size();
     if (n == 0) return WH_OK;
     size_t blk_in = 0, blk_out = 0;
@@ -544,6 +595,7 @@ int run_blocks_op(wh_pool *P, int op, int src_array, int dst_array, const std::v
 
 int run_boxes(wh_pool *P, int array, int src_array, const std::vector<wh_box> &boxes)
 {
+    WH_PROF("run_boxes");
     const int n = (int)boxes.size();
     if (n == 0) return WH_OK;
     size_t blk = 0, blk_src = 0;
@@ -559,20 +611,20 @@ int run_boxes(wh_pool *P, int array, int src_array, const std::vector<wh_box> &b
             WH_CHECK(b.lo[d] >= 0 && b.hi[d] < lim[d] && b.lo[d] <= b.hi[d], WH_E_ARG,
                      "boxes: box outside the block");
     }
-    WaveletWS &W = P->ws;
-    if (int rc = ensure(W.d_boxes, W.box_cap, (size_t)n)) return rc;
-    WH_HIP(hipMemcpyAsync(W.d_boxes, boxes.data(), sizeof(wh_box) * n, hipMemcpyHostToDevice, P->stream));
+    if (int rc = stage_begin(P, sizeof(wh_box) * n)) return rc;
+    const wh_box *d_boxes = (const wh_box *)stage_put(P, boxes.data(), sizeof(wh_box) * n);
+    if (!d_boxes) return WH_E_HIP;
     const ArrGeo A{blk_src, blk, P->cs, P->nx, P->ny, P->d.neq};
-    hipLaunchKernelGGL(k_boxes, dim3(n), dim3(256), 0, P->stream, arr, src, blk_src, W.d_boxes, A,
+    hipLaunchKernelGGL(k_boxes, dim3(n), dim3(256), 0, P->stream, arr, src, blk_src, d_boxes, A,
                        P->d.g, P->d.dim);
     WH_HIP(hipGetLastError());
-    WH_HIP(hipStreamSynchronize(P->stream));
     return WH_OK;
 }
 
 int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
                std::vector<double> &detail)
 {
+    WH_PROF("run_detail");
     const int n = (int)ids.size(), neq = P->d.neq;
     detail.assign((size_t)n * neq, 0.0);
     if (n == 0) return WH_OK;
@@ -592,11 +644,11 @@ int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int
     for (int d = 0; d < 3; ++d) D.bs[d] = d < P->d.dim ? P->d.bs[d] : 1;
     for (int c = 0; c < 8; ++c) D.groups[c] = (groups && c < neq) ? groups[c] : 1;
     const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
-    hipLaunchKernelGGL(k_detail, dim3(n), dim3(256), 0, P->stream, arr, W.d_ids, A, D, W.d_out);
+    hipLaunchKernelGGL(k_detail, dim3(n), dim3(256), 0, P->stream, arr, W.cur_ids, A, D, W.d_out);
     WH_HIP(hipGetLastError());
     WH_HIP(hipMemcpyAsync(detail.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost,
                           P->stream));
-    WH_HIP(hipStreamSynchronize(P->stream));
+    if (int rc = stage_sync(P)) return rc;
     // groups: the max over the group's components (threshold_block.f90:86-91)
     int maxg = 0;
     for (int c = 0; c < neq; ++c) maxg = std::max(maxg, D.groups[c]);
@@ -633,9 +685,10 @@ int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool
             WH_CHECK(b.lo[d] >= 0 && b.hi[d] < lim[d] && b.lo[d] <= b.hi[d], WH_E_ARG, "region max: bad box");
     }
     WaveletWS &W = P->ws;
-    if (int rc = ensure(W.d_boxes, W.box_cap, (size_t)n)) return rc;
     if (int rc = ensure(W.d_out, W.out_cap, (size_t)n * neq)) return rc;
-    WH_HIP(hipMemcpyAsync(W.d_boxes, boxes.data(), sizeof(wh_box) * n, hipMemcpyHostToDevice, P->stream));
+    if (int rc = stage_begin(P, sizeof(wh_box) * n)) return rc;
+    const wh_box *d_boxes = (const wh_box *)stage_put(P, boxes.data(), sizeof(wh_box) * n);
+    if (!d_boxes) return WH_E_HIP;
     RegionArgs R{};
     R.g = P->d.g;
     R.dim = P->d.dim;
@@ -644,15 +697,15 @@ int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool
     R.sqrt_groups = sqrt_groups;
     for (int c = 0; c < 8; ++c) R.groups[c] = (groups && c < neq) ? groups[c] : 1;
     const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
-    hipLaunchKernelGGL(k_region_max, dim3(n), dim3(256), 0, P->stream, arr, W.d_boxes, A, R, W.d_out);
+    hipLaunchKernelGGL(k_region_max, dim3(n), dim3(256), 0, P->stream, arr, d_boxes, A, R, W.d_out);
     WH_HIP(hipGetLastError());
     WH_HIP(hipMemcpyAsync(out.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost, P->stream));
-    WH_HIP(hipStreamSynchronize(P->stream));
-    return WH_OK;
+    return stage_sync(P);
 }
 
 int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<wh_patch> &patches_in)
 {
+    WH_PROF("run_patch_list");
     const int n = (int)patches_in.size();
     if (n == 0) return WH_OK;
     size_t blk = 0;
@@ -700,21 +753,18 @@ int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<
     std::stable_sort(p.begin(), p.end(), [](const wh_patch &a, const wh_patch &b) { return a.stage < b.stage; });
     std::vector<int64_t> pstart(n + 1, 0);
     for (int i = 0; i < n; ++i) pstart[i + 1] = pstart[i] + (int64_t)p[i].n[0] * p[i].n[1] * p[i].n[2];
-    WaveletWS &W = P->ws;
-    if (int rc = ensure(W.d_patches, W.patch_cap, (size_t)n)) return rc;
-    if (int rc = ensure(W.d_pstart, W.pstart_cap, (size_t)n + 1)) return rc;
-    WH_HIP(hipMemcpyAsync(W.d_patches, p.data(), sizeof(wh_patch) * n, hipMemcpyHostToDevice, P->stream));
-    WH_HIP(hipMemcpyAsync(W.d_pstart, pstart.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
-                          P->stream));
+    if (int rc = stage_begin(P, sizeof(wh_patch) * n + sizeof(int64_t) * (n + 1))) return rc;
+    const wh_patch *d_patches = (const wh_patch *)stage_put(P, p.data(), sizeof(wh_patch) * n);
+    const int64_t *d_pstart = (const int64_t *)stage_put(P, pstart.data(), sizeof(int64_t) * (n + 1));
+    if (!d_patches || !d_pstart) return WH_E_HIP;
     int s0 = 0;
     while (s0 < n) {
         int s1 = s0;
         while (s1 < n && p[s1].stage == p[s0].stage) ++s1;
-        if (int rc = launch_sync_flat(P, arr, W.d_patches, W.d_pstart, s0, s1, pstart[s1] - pstart[s0], S))
+        if (int rc = launch_sync_flat(P, arr, d_patches, d_pstart, s0, s1, pstart[s1] - pstart[s0], S))
             return rc;
         s0 = s1;
     }
-    WH_HIP(hipStreamSynchronize(P->stream));
     return WH_OK;
 }
 

@@ -47,7 +47,7 @@ def _synth(dim, N, ks, coef):
         if kx == 0:
             idx2 = (((-kz) % N, (-ky) % N, 0) if dim == 3 else ((-ky) % N, 0))
             F[idx2] += np.conj(c)
-    return np.fft.irfftn(F, s=shape) * float(N ** dim)
+    return np.fft.irfftn(F, s=shape, axes=tuple(range(dim))) * float(N ** dim)
 
 
 def fourier_velocity_grid(dim, N, seed, kmax=8, kmin=1, amp_exponent=-2.0,

@@ -108,6 +108,9 @@ def load(path: str = LIB_PATH):
         "wh_kernel_timing_enable": [_vp, ctypes.c_int],
         "wh_kernel_timing": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_int)],
+        "wh_set_block_groups": [_vp, _vp, ctypes.c_int, ctypes.c_int],
+        "wh_dt_reduce_groups": [_vp, ctypes.POINTER(AcmParamsC), _vp],
+        "wh_rk_step_groups": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int, _vp, ctypes.c_int],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -238,6 +241,24 @@ class Pool:
         out = ctypes.c_double()
         check(load().wh_divergence_guard(self.h, array, ctypes.byref(out)))
         return out.value
+
+    def set_block_groups(self, groups, ngroups):
+        """Group (independent forest) of every active block, in active-list order."""
+        g = np.ascontiguousarray(groups, np.int32)
+        self.ngroups = int(ngroups)
+        check(load().wh_set_block_groups(self.h, g.ctypes.data if g.size else None, g.size, int(ngroups)))
+
+    def dt_groups(self, acm_c):
+        """Per-group minimum block dt (before the host clamps)."""
+        out = np.zeros(self.ngroups)
+        check(load().wh_dt_reduce_groups(self.h, ctypes.byref(acm_c), out.ctypes.data))
+        return out
+
+    def rk_step_groups(self, acm_c, butcher, dts, path=2):
+        bt = np.ascontiguousarray(butcher, np.float64)
+        d = np.ascontiguousarray(dts, np.float64)
+        check(load().wh_rk_step_groups(self.h, ctypes.byref(acm_c), bt.ctypes.data, bt.shape[0],
+                                       d.ctypes.data, path))
 
     def rk_step(self, acm_c, butcher, time, dt, path=0):
         bt = np.ascontiguousarray(butcher, np.float64)

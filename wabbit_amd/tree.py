@@ -28,6 +28,7 @@ def _bind():
         "wh_forest_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, _vp,
                              ctypes.c_int, _vp, _vp, ctypes.c_int],
         "wh_forest_destroy": [_vp],
+        "wh_forest_set_slot_base": [_vp, ctypes.c_int],
         "wh_forest_set_blocks": [_vp, ctypes.c_int, _vp, _vp, _vp],
         "wh_forest_size": [_vp, ip],
         "wh_forest_get_blocks": [_vp, ctypes.c_int, _vp, _vp, _vp, _vp],
@@ -45,9 +46,11 @@ def _bind():
 class Forest:
     """One tree's leaf blocks (treecode, level, hvy id) on a pool of nb_max slots."""
 
-    def __init__(self, dim, jmax, bs, g, domain, nb_max, periodic=(1, 1, 1)):
+    def __init__(self, dim, jmax, bs, g, domain, nb_max, periodic=(1, 1, 1), slot_base=0):
+        """slot_base: the forest uses pool slots [slot_base, slot_base + nb_max)."""
         _bind()
         self.dim, self.jmax, self.g, self.nb_max = dim, jmax, g, nb_max
+        self.slot_base = slot_base
         self.bs = list(bs) + [1] * (3 - len(bs))
         self.domain = list(domain) + [1.0] * (3 - len(domain))
         b = np.asarray(self.bs, np.int32)
@@ -57,6 +60,8 @@ class Forest:
         L.check(L.load().wh_forest_create(ctypes.byref(h), dim, jmax, b.ctypes.data, g, d.ctypes.data,
                                           p.ctypes.data, nb_max))
         self.h = h
+        if slot_base:
+            L.check(L.load().wh_forest_set_slot_base(h, slot_base))
 
     def close(self):
         if getattr(self, "h", None):
@@ -111,22 +116,26 @@ class Forest:
                                           ctypes.byref(n)))
         return tri[:3 * n.value].reshape(-1, 3)
 
-    def attach(self, pool, g_rhs, order):
-        """Upload the active list, origins/spacings and both patch tables
-        (case 0: RHS depth g_rhs, case 1: full depth g) to a device pool."""
+    def attach(self, pool, g_rhs, order, full=True):
+        """Upload the active list, origins/spacings and the patch tables
+        (case 0: RHS depth g_rhs; case 1: full depth g, unless full=False)
+        to a device pool."""
         tc, lv, hv, x = self.blocks()
         pool.set_active(hv, x)
         pool.set_patches(0, self.patches(g_rhs, g_rhs, order))
-        pool.set_patches(1, self.patches(self.g, self.g, order))
+        if full:
+            pool.set_patches(1, self.patches(self.g, self.g, order))
         return tc, lv, hv, x
 
 
-def refine_on_device(pool, forest, hvy_ids, order, g_rhs, array=L.WH_BLOCK, presync="table"):
+def refine_on_device(pool, forest, hvy_ids, order, g_rhs, array=L.WH_BLOCK, presync="table", attach=True,
+                     full=True):
     """refine_tree's execution step for the given blocks: sync the full ghost
     layer (sync_ghosts_tree), refine on the device, then update the pool's
     tables.  presync "table" runs the pool's full-depth patch table (plain
     restriction), "tree" the forest sync with HD-filtered restriction of
-    lifted wavelets (wh_sync_tree), None skips the sync."""
+    lifted wavelets (wh_sync_tree), None skips the sync.  attach=False leaves
+    the pool's tables alone (forests sharing a pool)."""
     if presync == "table":
         pool.sync(array, 1)
     elif presync == "tree":
@@ -150,5 +159,6 @@ def refine_on_device(pool, forest, hvy_ids, order, g_rhs, array=L.WH_BLOCK, pres
         L.check(L.load().wh_refine_blocks(pool.h, array, t.ctypes.data, t.shape[0], order))
         done.append(tri)
         ids = ids[m:]
-    forest.attach(pool, g_rhs, order)
+    if attach:
+        forest.attach(pool, g_rhs, order, full)
     return np.concatenate(done) if done else np.zeros((0, 3), np.int32)

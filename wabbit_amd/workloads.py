@@ -104,11 +104,13 @@ class ChannelInicond:
 
     def __init__(self, w: dict, g: int):
         self.w, self.g = w, g
-        N = 2 ** w["jmax"] * w["bs"]
-        self.comps = fourier_velocity_grid(2, N, w["seed"], **w["perturb"])
+        self.comps = None
 
     def __call__(self, treecodes, levels, jmax_tc):
         w = self.w
+        if self.comps is None:
+            N = 2 ** w["jmax"] * w["bs"]
+            self.comps = fourier_velocity_grid(2, N, w["seed"], **w["perturb"])
         u = [sample_tree(treecodes, levels, jmax_tc, 2, w["bs"], self.g, c, w["jmax"]) + w["inflow"][d]
              for d, c in enumerate(self.comps)]
         return np.stack(u + [np.zeros_like(u[0])], axis=1)
@@ -140,3 +142,28 @@ def channel_solver(w: dict, path=None, nb_max: int = 16384):
                     nb_max=nb_max, path=PATH_FUSED if path is None else path, jmax_treecode=w["jmax"])
     s.set_initial_grid(ic)
     return s
+
+
+def channel_spec(seed: int, jmax: int, bs: int, nsteps: int, slots: int, tag: str = ""):
+    """The channel workload with its seeded cylinder as one solve of a batch."""
+    from . import wavelets as W
+    from .adaptive import fd_reach
+    from .batch import SolveSpec
+    w = channel(seed=seed, jmax=jmax, bs=bs)
+    g = W.setup(w["wavelet"], fd_reach(w["acm"].scheme)).g
+    return SolveSpec(acm=w["acm"], cfg=channel_adapt_config(w), inicond=ChannelInicond(w, g), nsteps=nsteps,
+                     mask=w["mask"], tp=w["tp"], slots=slots, tag=tag or w["name"] + f"-seed{seed}")
+
+
+def ensemble(n: int = 64, seed: int = 4321, nsteps: int = 200):
+    """The batched ensemble: n channel solves, each with its own seeded
+    cylinder, Jmax drawn from {4,5,6,7} and Bs from {16,24}; slots per solve
+    sized from Jmax."""
+    rng = np.random.default_rng(seed)
+    slots_of = {4: 1024, 5: 2560, 6: 6144, 7: 12288}
+    out = []
+    for i in range(n):
+        jmax = int(rng.choice([4, 5, 6, 7]))
+        bs = int(rng.choice([16, 24]))
+        out.append(channel_spec(seed + 17 * i, jmax, bs, nsteps, slots_of[jmax], tag=f"member{i}-J{jmax}-Bs{bs}"))
+    return out
