@@ -38,6 +38,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# bytes beyond L2 per block-stage of the fused 2D stage kernel on c2, from
+# the FETCH_SIZE / WRITE_SIZE passes scaled by the copy calibration
+# (profiles/c2_fused_stage_hbm_traffic.md)
+C2_STAGE_TRAFFIC_PER_BLOCK = 40638
 
 
 def b_min(q, bs, dim, g_r, m):
@@ -118,15 +122,21 @@ def cpu_baseline_adaptive(w, state, seconds=20.0):
     from oracle import pyoracle as po
     from wabbit_amd import workloads
     from wabbit_amd.params import butcher_rk4, clamp_dt
-    cfg = workloads.channel_adapt_config(w)
     tc, lv, hvy = state
     wo = po.wavelet(w["wavelet"], fd_max_size=2)
-    tree = po.Tree(wo, w["dim"], w["jmin"], w["jmax"], [w["bs"]] * w["dim"], wo.g, tc, lv, hvy, indicator=1,
-                   eps=w["eps"], groups=cfg.groups, force_maxlevel_dealiasing=cfg.force_maxlevel_dealiasing,
-                   eps_normalized=cfg.eps_normalized, threshold_mask=cfg.threshold_mask,
-                   domain=w["domain"], mask=po.mask_params(**w["mask"]))
+    if w["mask"] is not None:
+        cfg = workloads.channel_adapt_config(w)
+        tree = po.Tree(wo, w["dim"], w["jmin"], w["jmax"], [w["bs"]] * w["dim"], wo.g, tc, lv, hvy,
+                       indicator=1, eps=w["eps"], groups=cfg.groups,
+                       force_maxlevel_dealiasing=cfg.force_maxlevel_dealiasing,
+                       eps_normalized=cfg.eps_normalized, threshold_mask=cfg.threshold_mask,
+                       domain=w["domain"], mask=po.mask_params(**w["mask"]))
+        mp = po.mask_params(**w["mask"])
+    else:
+        tree = po.Tree(wo, w["dim"], w["jmin"], w["jmax"], [w["bs"]] * w["dim"], wo.g, tc, lv, hvy,
+                       indicator=1, eps=w["eps"], groups=[1] * (w["dim"] + 1), domain=w["domain"])
+        mp = None
     ap = po.make_params(w["acm"])
-    mp = po.mask_params(**w["mask"])
     bt = butcher_rk4()
     t, n, stages = 0.0, 0, 0
     t0 = time.perf_counter()
@@ -156,7 +166,7 @@ def run_adaptive(args, w):
     steps = args.steps if args.steps is not None else w["nsteps_cfg"]
     warmup = args.warmup
     t0 = time.perf_counter()
-    s = workloads.channel_solver(w, path=PATH_FUSED, nb_max=16384)
+    s = workloads.adaptive_solver(w, path=PATH_FUSED, nb_max=16384 if w["dim"] == 2 else 12288)
     print(f"# workload {w['name']}: initial grid {s.nblocks} blocks ({time.perf_counter() - t0:.1f} s)",
           file=sys.stderr, flush=True)
     init_state = s.state()
@@ -191,12 +201,18 @@ def run_adaptive(args, w):
     s.profile(False)
     s.pool.timing(False)
     q, dim = w["dim"] + 1, w["dim"]
-    bpb = b_min(q, w["bs"], dim, 2, 5)       # m = 5 mask components read (chi, u_s, colour, sponge)
+    # m: mask components the RHS reads (2D penalised + sponge: chi, u_s, colour, sponge)
+    m = 5 if w["mask"] is not None else 0
+    bpb = b_min(q, w["bs"], dim, 2, m)
     achieved = bpb * st2 / (k_ms * k_n * 1e-3) / 1e9
+    bpl = st2 / max(k_n, 1)
+    traffic = round(C2_STAGE_TRAFFIC_PER_BLOCK * bpl) if w["kind"] == "adaptive" else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_rhs2d<FD4,fused>", avg_launch_ms=round(k_ms, 5), launches=k_n,
-                    bytes_per_block_stage=bpb, blocks_per_launch=round(st2 / max(k_n, 1), 1))
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(k_ms, 5), launches=k_n,
+                    bytes_per_block_stage=bpb, blocks_per_launch=round(bpl, 1))
+    if traffic is not None:
+        roofline["traffic_per_block_stage"] = C2_STAGE_TRAFFIC_PER_BLOCK
 
     cpu = None
     if not args.no_cpu:
@@ -228,26 +244,163 @@ def run_adaptive(args, w):
     s.close()
 
 
+def _group_roofline(groups, stages_by_group):
+    """Fused stage kernel of the batched path: algorithmic bytes of all
+    timed launches / their summed durations (HIP events per pool stream)."""
+    tot_bytes, tot_ms, launches = 0.0, 0.0, 0
+    for g, st in zip(groups, stages_by_group):
+        ms, n = g.pool.kernel_time(0)
+        if n == 0:
+            continue
+        a = g.acm
+        m = 5 if a.penalization or a.use_sponge else 0
+        tot_bytes += b_min(a.dim + 1, g.pool.bs[0], a.dim, 2, m) * st
+        tot_ms += ms * n
+        launches += n
+    achieved = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_rhs2d<FD4,fused>",
+                launches=launches, avg_launch_ms=round(tot_ms / max(launches, 1), 5),
+                blocks_per_launch=round(sum(stages_by_group) / max(launches, 1), 1))
+
+
+def run_ensemble(args):
+    """c4: 64 independent channel solves (seeded cylinders, Jmax in
+    {4,5,6,7}, Bs in {16,24}) stepped together: per pool one launch per RK
+    stage operation over all members' blocks; W untimed + K timed steps."""
+    from wabbit_amd import workloads
+    from wabbit_amd.batch import Batch
+    torch_sync = _torch_sync()
+    steps = args.steps if args.steps is not None else 200
+    warmup = args.warmup
+    specs = workloads.ensemble(args.members, nsteps=warmup + steps + 2)
+    slots = {}
+    for sp in specs:
+        k = (2, sp.cfg.bs)
+        slots[k] = slots.get(k, 0) + sp.slots
+    lanes = args.lanes
+    slots = {k: v + v // 8 for k, v in slots.items()}   # slack for the split over lanes
+    t0 = time.perf_counter()
+    b = Batch(slots, n_work_slots=3, lanes=lanes)
+    for sp in specs:
+        if b.admit(sp) is None:
+            raise SystemExit(f"ensemble member {sp.tag} does not fit")
+    print(f"# ensemble: {len(specs)} members admitted, {sum(len(s.forest.blocks()[0]) for s in b.solves)} "
+          f"blocks ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        b.step()
+    torch_sync()
+    t0 = time.perf_counter()
+    stages = 0
+    for k in range(steps):
+        stages += b.step()
+        if (k + 1) % 50 == 0:
+            print(f"# step {k + 1}: {sum(s.rk_blocks for s in b.solves)} blocks", file=sys.stderr, flush=True)
+    for g in b.groups.values():
+        g.pool.synchronize()
+    torch_sync()
+    el = time.perf_counter() - t0
+    value = stages / el
+    groups = list(b.groups.values())
+    for g in groups:
+        g.pool.timing(True)
+    st_by = [0] * len(groups)
+    for _ in range(2):
+        b.step()
+        for i, g in enumerate(groups):
+            st_by[i] += g.last_stages
+    roofline = _group_roofline(groups, st_by)
+    for g in groups:
+        g.pool.timing(False)
+    nblocks = sum(s.rk_blocks for s in b.solves)
+    b.close()
+
+    # the plain path (one launch per block, as the reference loops over
+    # blocks) against the batched fused path on the same 4 members, 2 steps
+    plain = {}
+    sub = workloads.ensemble(4, nsteps=4)
+    for path in (1, 2):
+        bb = Batch({(2, 16): 40000, (2, 24): 40000}, n_work_slots=5)
+        for sp in sub:
+            bb.admit(sp)
+        bb.step(path)
+        for g in bb.groups.values():
+            g.pool.synchronize()
+        t1 = time.perf_counter()
+        st = bb.step(path) + bb.step(path)
+        for g in bb.groups.values():
+            g.pool.synchronize()
+        plain["fused" if path == 2 else "per_block"] = round(st / (time.perf_counter() - t1), 1)
+        bb.close()
+    out = {
+        "metric": "block-RK-stages/s", "value": round(value, 1), "unit": "block-RK-stages/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(el / steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"c4-ensemble-{len(specs)}x-channel-J4-7-Bs16-24", "members": len(specs),
+                   "lanes": lanes, "blocks_at_end": nblocks},
+        "roofline": roofline, "cpu_baseline": None,
+        "paths_block_RK_stages_per_s_4_members": plain,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_serving(args):
+    """c5: a seeded Poisson stream of requests (70% 2D channel, 30% 3D
+    periodic) on shared pools with continuous batching; the first
+    --requests of the stream."""
+    from wabbit_amd import workloads
+    from wabbit_amd.serve import Request, Server
+    mix = workloads.serving_mix(n=args.requests, rate=args.rate)
+    reqs = [Request(rid, spec, t) for rid, spec, t in mix]
+    # pools: 2D Bs16, 2D Bs24, 3D Bs24 -- about 60% of the 288 GB
+    slots = {(2, 16): 48000, (2, 24): 48000, (3, 24): 18000}
+    srv = Server(slots, n_work_slots=3, lanes=args.lanes)
+    st = srv.run(reqs, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    srv.close()
+    out = {
+        "metric": "solves/s", "value": round(st.solves_per_s, 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"c5-serving-{len(reqs)}req-rate{args.rate}", "requests": len(reqs),
+                   "lanes": args.lanes, "pool_slots": {f"{k[0]}d-Bs{k[1]}": v for k, v in slots.items()}},
+        "p50_latency_s": round(st.p50, 4), "p99_latency_s": round(st.p99, 4),
+        "mean_queue_wait_s": round(st.mean_wait, 4), "completed": st.completed, "failed": st.failed,
+        "wall_s": round(st.wall, 3), "block_RK_stages_per_s": round(st.block_rk_stages / st.wall, 1),
+        "max_concurrent": st.max_concurrent,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3u"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c1", "c3u"])
+    ap.add_argument("--members", type=int, default=64, help="c4: ensemble members")
+    ap.add_argument("--requests", type=int, default=24, help="c5: requests taken from the stream")
+    ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (1/s)")
+    ap.add_argument("--lanes", type=int, default=4, help="c4/c5: host threads (pool sets) stepping solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain interior copy of known size (counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
-        calibrate_copy("c1" if args.workload == "c2" else args.workload)
+        calibrate_copy(args.workload if args.workload in ("c1", "c3u") else "c1")
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(there is no multi-GPU decomposition)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4":
+        run_ensemble(args)
+        return
+    if args.workload == "c5":
+        run_serving(args)
+        return
     w = workload(args.workload)
-    if w["kind"] == "adaptive":
+    if w["kind"] in ("adaptive", "adaptive3d"):
         run_adaptive(args, w)
         return
 

@@ -85,7 +85,7 @@ def test_slot_ranges_are_reused():
         while not s1.done:
             b.step()
         res = b.retire_done()
-        assert len(res) == 1 and res[0][0] == s1.rid
+        assert len(res) == 1 and res[0][0] == s1.rid and res[0][3] is None
         s2 = b.admit(sp)
         assert s2 is not None and s2.base == s1.base
         while not s2.done:

@@ -82,6 +82,7 @@ class Solve:
         self.rid, self.spec, self.group, self.base = rid, spec, group, base
         self.time, self.iteration = 0.0, 0
         self.rk_blocks = 0
+        self.error: Optional[str] = None   # set when the solve failed (e.g. out of its slots)
         cfg, acm = spec.cfg, spec.acm
         self.forest = Forest(acm.dim, cfg.jmax, [cfg.bs] * acm.dim, group.w.g, acm.domain_size[:acm.dim],
                              spec.slots, slot_base=base)
@@ -97,7 +98,7 @@ class Solve:
 
     @property
     def done(self):
-        return self.iteration >= self.spec.nsteps
+        return self.error is not None or self.iteration >= self.spec.nsteps
 
     def _set_blocks(self, tc, lv, data):
         n = len(tc)
@@ -148,6 +149,7 @@ class PoolGroup:
         self.slots = SlotRanges(n_slots)
         self.acm_c = L.acm_params_c(acm)
         self.solves: List[Solve] = []
+        self.last_stages = 0
 
     def admit(self, rid, spec: SolveSpec) -> Optional[Solve]:
         base = self.slots.alloc(spec.slots)
@@ -177,33 +179,49 @@ class PoolGroup:
             return 0
         bt = butcher_rk4() if butcher is None else butcher
         pool, w = self.pool, self.w
-        hvs, xs, grp, pats = [], [], [], []
-        for i, s in enumerate(self.solves):
-            W.sync_tree(pool, s.forest)
-            tc, lv, hv, _ = s.forest.blocks()
-            ids = hv[lv < s.spec.cfg.jmax]
-            if ids.size:
-                refine_on_device(pool, s.forest, ids, w.order, w.g_rhs, presync=None, attach=False)
-            tc, lv, hv, x = s.forest.blocks()
+        hvs, xs, grp, pats, live = [], [], [], [], []
+        for s in self.solves:
+            if s.done:
+                continue
+            try:
+                W.sync_tree(pool, s.forest)
+                tc, lv, hv, _ = s.forest.blocks()
+                ids = hv[lv < s.spec.cfg.jmax]
+                if ids.size:
+                    refine_on_device(pool, s.forest, ids, w.order, w.g_rhs, presync=None, attach=False)
+                tc, lv, hv, x = s.forest.blocks()
+                if s.mask_p is not None:
+                    W.create_mask(pool, s.mask_p, hv, x)
+                pat = s.forest.patches(w.g_rhs, w.g_rhs, w.order)
+            except L.WabbitHipError as e:    # the solve alone fails (e.g. out of its slots)
+                s.error = str(e)
+                continue
             s.rk_blocks = len(hv)
-            if s.mask_p is not None:
-                W.create_mask(pool, s.mask_p, hv, x)
             hvs.append(hv)
             xs.append(x)
-            grp.append(np.full(len(hv), i, np.int32))
-            pats.append(s.forest.patches(w.g_rhs, w.g_rhs, w.order))
+            grp.append(np.full(len(hv), len(live), np.int32))
+            pats.append(pat)
+            live.append(s)
+        self.last_stages = 0
+        if not live:
+            return 0
         pool.set_active(np.concatenate(hvs), np.concatenate(xs))
         pool.set_patches(0, np.concatenate(pats))
-        pool.set_block_groups(np.concatenate(grp), len(self.solves))
+        pool.set_block_groups(np.concatenate(grp), len(live))
         dts = pool.dt_groups(self.acm_c)
-        dts = np.array([clamp_dt(d, s.time, s.spec.tp) for d, s in zip(dts, self.solves)])
+        dts = np.array([clamp_dt(d, s.time, s.spec.tp) for d, s in zip(dts, live)])
         pool.rk_step_groups(self.acm_c, bt, dts, path)
         stages = 0
-        for s, dt in zip(self.solves, dts):
+        for s, dt in zip(live, dts):
             s.time += float(dt)
             s.iteration += 1
             stages += 4 * s.rk_blocks
-            W.adapt_tree(pool, s.forest, s.adapt_p)
+        self.last_stages = stages
+        for s in live:
+            try:
+                W.adapt_tree(pool, s.forest, s.adapt_p)
+            except L.WabbitHipError as e:
+                s.error = str(e)
         return stages
 
     def close(self):
@@ -214,33 +232,53 @@ class PoolGroup:
 
 
 class Batch:
-    """Independent solves on shared pools, one pool per block shape."""
+    """Independent solves on shared pools, one pool per block shape and lane.
 
-    def __init__(self, slots_per_pool: Dict[tuple, int], n_work_slots: int = 5):
+    With lanes > 1 the solves are spread over that many independent sets of
+    pools, each with its own stream, stepped by one host thread per lane:
+    the per-solve mesh work runs in the library with the interpreter lock
+    released, so the lanes' host orchestration and their kernels overlap."""
+
+    def __init__(self, slots_per_pool: Dict[tuple, int], n_work_slots: int = 5, lanes: int = 1):
         """slots_per_pool: {(dim, Bs): pool slots} -- the device memory set
-        aside per shape (the reference's --memory)."""
+        aside per shape (the reference's --memory), split evenly over the
+        lanes."""
         self.slots_per_pool = dict(slots_per_pool)
         self.n_work_slots = n_work_slots
+        self.lanes = max(1, int(lanes))
         self.groups: Dict[tuple, PoolGroup] = {}
         self.next_rid = 0
+        self._exec = None
+        if self.lanes > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            self._exec = ThreadPoolExecutor(self.lanes)
 
-    def key(self, spec: SolveSpec):
-        return (spec.acm.dim, spec.cfg.bs, spec.cfg.wavelet, spec.mask is not None, _acm_key(spec.acm))
+    def key(self, spec: SolveSpec, lane: int = 0):
+        return (lane, spec.acm.dim, spec.cfg.bs, spec.cfg.wavelet, spec.mask is not None, _acm_key(spec.acm))
 
-    def group_for(self, spec: SolveSpec) -> PoolGroup:
-        k = self.key(spec)
+    def group_for(self, spec: SolveSpec, lane: int = 0) -> PoolGroup:
+        k = self.key(spec, lane)
         if k not in self.groups:
-            n = self.slots_per_pool[(spec.acm.dim, spec.cfg.bs)]
+            n = self.slots_per_pool[(spec.acm.dim, spec.cfg.bs)] // self.lanes
             self.groups[k] = PoolGroup(spec.acm, spec.cfg.bs, spec.cfg.wavelet, n, spec.mask is not None,
                                        self.n_work_slots)
         return self.groups[k]
 
     def admit(self, spec: SolveSpec, rid=None) -> Optional[Solve]:
-        """Admit a solve if its pool has a free slot range; None otherwise."""
+        """Admit a solve into the least loaded lane whose pool has a free
+        slot range; None if none has."""
         if rid is None:
             rid = self.next_rid
             self.next_rid += 1
-        return self.group_for(spec).admit(rid, spec)
+
+        def load(lane):
+            g = self.groups.get(self.key(spec, lane))
+            return g.slots.in_use if g is not None else 0
+        for lane in sorted(range(self.lanes), key=load):
+            s = self.group_for(spec, lane).admit(rid, spec)
+            if s is not None:
+                return s
+        return None
 
     @property
     def solves(self):
@@ -248,14 +286,22 @@ class Batch:
 
     def step(self, path=PATH_FUSED):
         """One iteration of every admitted solve; returns block-RK-stages."""
-        return sum(g.step(path) for g in self.groups.values())
+        if self._exec is None:
+            return sum(g.step(path) for g in self.groups.values())
+        by_lane: Dict[int, List[PoolGroup]] = {}
+        for k, g in self.groups.items():
+            by_lane.setdefault(k[0], []).append(g)
+        futs = [self._exec.submit(lambda gs: sum(g.step(path) for g in gs), gs) for gs in by_lane.values()]
+        return sum(f.result() for f in futs)
 
-    def retire_done(self):
-        """Remove finished solves; returns [(rid, tag, state)]."""
+    def retire_done(self, keep_state=True):
+        """Remove finished (or failed) solves; returns [(rid, tag, state or
+        None, error or None)]."""
         out = []
         for g in self.groups.values():
             for s in [s for s in g.solves if s.done]:
-                out.append((s.rid, s.spec.tag, s.state()))
+                st = s.state() if keep_state and s.error is None else None
+                out.append((s.rid, s.spec.tag, st, s.error))
                 g.retire(s)
         return out
 
@@ -263,6 +309,9 @@ class Batch:
         for g in self.groups.values():
             g.close()
         self.groups = {}
+        if self._exec is not None:
+            self._exec.shutdown()
+            self._exec = None
 
 
 def run_alone(spec: SolveSpec, slots: int, path=PATH_FUSED):

@@ -28,14 +28,21 @@ namespace {
 
 constexpr int REF_TMP_EMPTY = 10;          // module_globals.f90:25
 constexpr int REF_UNSIGNIFICANT_STAY = 9;  // module_globals.f90:28
-constexpr int A_U = WH_BLOCK, A_T = WH_TMP, A_SCR = WH_WORK + 0, A_FILT = WH_WORK + 1;
+constexpr int A_U = WH_BLOCK, A_T = WH_TMP, A_FILT = WH_WORK + 1;  // work slot 0: scratch of the transforms
 
 struct TNode {
     int64_t tc;
     int level;
     int hvy;
     int status;
+    int q[3];  // grid index of the block at its level (from tc)
 };
+
+// hash key of (level, grid index); indices < 2^19 per direction
+inline uint64_t qkey(int level, const int q[3])
+{
+    return (uint64_t)level | ((uint64_t)q[0] << 5) | ((uint64_t)q[1] << 24) | ((uint64_t)q[2] << 43);
+}
 
 enum SrcMode { SRC_U = 0, SRC_NOT_FLAG = 1, SRC_DECOMPOSED = 2 };
 struct SyncSpec {
@@ -48,26 +55,76 @@ struct SyncSpec {
     int gm = 0, gp = 0;
 };
 
+int dirs_of(int dim, int dirs[26][3])
+{
+    int nd = 0;
+    for (int oz = (dim == 3 ? -1 : 0); oz <= (dim == 3 ? 1 : 0); ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                if (!ox && !oy && !oz) continue;
+                dirs[nd][0] = ox;
+                dirs[nd][1] = oy;
+                dirs[nd][2] = oz;
+                ++nd;
+            }
+    return nd;
+}
+
 struct Tree {
     wh_forest *F;
     wh_pool *P;
     wh_wavelet w;
     int dim, g, bs[3], jmax;
     std::vector<TNode> b;
+    // (level, grid index) -> node: a dense table per level while it has at
+    // most 2^20 cells, the hash map beyond
     std::unordered_map<uint64_t, int> map;
+    std::vector<std::vector<int>> dense;
+    static constexpr int DENSE_BITS = 20;
 
     Tree(wh_forest *f, wh_pool *p) : F(f), P(p), w(p->ws.w), dim(f->dim), g(f->g), jmax(f->jmax)
     {
         for (int d = 0; d < 3; ++d) bs[d] = f->bs[d];
-        for (const FBlock &x : f->blocks) b.push_back(TNode{x.tc, x.level, x.hvy, 0});
+        nd = dirs_of(dim, dirs);
+        b.reserve(2 * f->blocks.size());
+        for (const FBlock &x : f->blocks) b.push_back(node(x.tc, x.level, x.hvy, 0));
         rehash();
+    }
+    TNode node(int64_t tc, int level, int hvy, int status) const
+    {
+        TNode n{tc, level, hvy, status, {0, 0, 0}};
+        F->ijk_of(tc, level, n.q);
+        return n;
+    }
+    size_t lin(int level, const int q[3]) const
+    {
+        return ((size_t)q[2] << (2 * level)) | ((size_t)q[1] << level) | (size_t)q[0];
+    }
+    void insert(int i)
+    {
+        const int L = b[i].level;
+        if (L < (int)dense.size() && !dense[L].empty()) dense[L][lin(L, b[i].q)] = i;
+        else map[qkey(L, b[i].q)] = i;
     }
     void rehash()
     {
         map.clear();
-        for (size_t i = 0; i < b.size(); ++i) map[fkey(b[i].level, b[i].tc)] = (int)i;
+        int lmax = 0;
+        for (const TNode &x : b) lmax = std::max(lmax, x.level);
+        dense.resize(lmax + 1);
+        for (int L = 0; L <= lmax; ++L) {
+            if (dim * L <= DENSE_BITS) dense[L].assign((size_t)1 << (dim * L), -1);
+            else dense[L].clear();
+        }
+        for (size_t i = 0; i < b.size(); ++i) insert((int)i);
+        topo_ok = false;
     }
-    void ijk(int i, int out[3]) const { F->ijk_of(b[i].tc, b[i].level, out); }
+    void ijk(int i, int out[3]) const
+    {
+        out[0] = b[i].q[0];
+        out[1] = b[i].q[1];
+        out[2] = b[i].q[2];
+    }
     int find(int level, const int q[3]) const
     {
         if (level < 0) return -1;
@@ -77,81 +134,32 @@ struct Tree {
             if (!F->periodic[d] && (q[d] < 0 || q[d] >= m)) return -1;
             wq[d] = ((q[d] % m) + m) % m;
         }
-        auto it = map.find(fkey(level, F->tc_of(wq, level)));
+        if (level < (int)dense.size() && !dense[level].empty()) return dense[level][lin(level, wq)];
+        auto it = map.find(qkey(level, wq));
         return it == map.end() ? -1 : it->second;
     }
     bool valid(int i) const { return i >= 0 && b[i].status != REF_TMP_EMPTY; }
-    int mother(int i) const
+
+    // ---- relations, uncached (used while the block set changes) ----
+    int mother_raw(int i) const
     {
         if (b[i].level == 0) return -1;
-        int q[3];
-        ijk(i, q);
+        const int *q = b[i].q;
         const int m[3] = {q[0] >> 1, q[1] >> 1, q[2] >> 1};
         return find(b[i].level - 1, m);
     }
-    int daughter(int i, int k) const
+    int daughter_raw(int i, int k) const
     {
-        int q[3];
-        ijk(i, q);
+        const int *q = b[i].q;
         const int d[3] = {2 * q[0] + ((k >> 1) & 1), 2 * q[1] + (k & 1), dim == 3 ? 2 * q[2] + ((k >> 2) & 1) : 0};
         return find(b[i].level + 1, d);
     }
-    // block_is_leaf (block_relations.f90); check_empty: REF_TMP_EMPTY daughters don't count
-    bool is_leaf(int i, bool check_empty) const
-    {
-        for (int k = 0; k < (1 << dim); ++k) {
-            const int d = daughter(i, k);
-            if (d >= 0 && (!check_empty || b[d].status != REF_TMP_EMPTY)) return false;
-        }
-        return true;
-    }
-    int nb_same(int i, const int o[3]) const
-    {
-        int q[3];
-        ijk(i, q);
-        const int n[3] = {q[0] + o[0], q[1] + o[1], q[2] + o[2]};
-        return find(b[i].level, n);
-    }
-    // the finer blocks adjacent to block i in direction o (children of the
-    // virtual same-level neighbour); child cc per dimension in out
-    void finer_children(int i, const int o[3], std::vector<std::pair<int, std::array<int, 3>>> &out) const
-    {
-        out.clear();
-        int q[3];
-        ijk(i, q);
-        int lo[3], hi[3];
-        for (int d = 0; d < 3; ++d) {
-            if (d >= dim) lo[d] = hi[d] = 0;
-            else if (o[d] < 0) lo[d] = hi[d] = 1;
-            else if (o[d] > 0) lo[d] = hi[d] = 0;
-            else {
-                lo[d] = 0;
-                hi[d] = 1;
-            }
-        }
-        for (int cz = lo[2]; cz <= hi[2]; ++cz)
-            for (int cy = lo[1]; cy <= hi[1]; ++cy)
-                for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-                    const int c[3] = {2 * (q[0] + o[0]) + cx, 2 * (q[1] + o[1]) + cy,
-                                      dim == 3 ? 2 * (q[2] + o[2]) + cz : 0};
-                    out.push_back({find(b[i].level + 1, c), {cx, cy, cz}});
-                }
-    }
-    bool any_finer(int i, const int o[3], bool need_valid) const
-    {
-        std::vector<std::pair<int, std::array<int, 3>>> ch;
-        finer_children(i, o, ch);
-        for (auto &c : ch)
-            if (c.first >= 0 && (!need_valid || valid(c.first))) return true;
-        return false;
-    }
     // coarse relation in direction o (find_neighbors.f90): exists only on the
     // mother's boundary in that direction
-    int nb_coarse(int i, const int o[3]) const
+    int nb_coarse_raw(int i, const int o[3]) const
     {
         if (b[i].level == 0) return -1;
-        int q[3];
-        ijk(i, q);
+        const int *q = b[i].q;
         int c[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) {
             if (o[d] > 0 && (q[d] & 1) != 1) return -1;
@@ -161,13 +169,137 @@ struct Tree {
         }
         return find(b[i].level - 1, c);
     }
+
+    // ---- relations, cached per block set ----
+    // Tables built on first use after the block set changed (add, removal):
+    // per block and direction the same-level neighbour, the coarse relation,
+    // the coarser block covering the ghost box (floor((q+o)/2) one level up)
+    // and the finer blocks adjacent in that direction; mother and daughters.
+    int nd = 0;
+    int dirs[26][3];
+    mutable bool topo_ok = false;
+    mutable std::vector<int> t_same, t_coarse, t_cover, t_mom, t_dau, t_fin_off;
+    mutable std::vector<std::pair<int, std::array<int, 3>>> t_fin;
+    int dir_index(const int o[3]) const
+    {
+        const int t = dim == 3 ? (o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1) : (o[1] + 1) * 3 + (o[0] + 1);
+        const int c = dim == 3 ? 13 : 4;
+        return t > c ? t - 1 : t;
+    }
+    void topo() const
+    {
+        if (topo_ok) return;
+        const int n = (int)b.size(), ns = 1 << dim;
+        t_same.assign((size_t)n * nd, -1);
+        t_coarse.assign((size_t)n * nd, -1);
+        t_cover.assign((size_t)n * nd, -1);
+        t_mom.assign(n, -1);
+        t_dau.assign((size_t)n * ns, -1);
+        t_fin_off.assign((size_t)n * nd + 1, 0);
+        t_fin.clear();
+        for (int i = 0; i < n; ++i) {
+            const int *q = b[i].q, L = b[i].level;
+            t_mom[i] = mother_raw(i);
+            for (int k = 0; k < ns; ++k) t_dau[(size_t)i * ns + k] = daughter_raw(i, k);
+            for (int k = 0; k < nd; ++k) {
+                const int *o = dirs[k];
+                const size_t e = (size_t)i * nd + k;
+                const int nq[3] = {q[0] + o[0], q[1] + o[1], q[2] + o[2]};
+                t_same[e] = find(L, nq);
+                t_coarse[e] = nb_coarse_raw(i, o);
+                if (L > 0) {
+                    int cu[3] = {0, 0, 0};
+                    for (int d = 0; d < dim; ++d) {
+                        const int v = q[d] + o[d];
+                        cu[d] = v >= 0 ? v / 2 : -((-v + 1) / 2);
+                    }
+                    t_cover[e] = find(L - 1, cu);
+                }
+                int lo[3], hi[3];
+                for (int d = 0; d < 3; ++d) {
+                    if (d >= dim) lo[d] = hi[d] = 0;
+                    else if (o[d] < 0) lo[d] = hi[d] = 1;
+                    else if (o[d] > 0) lo[d] = hi[d] = 0;
+                    else {
+                        lo[d] = 0;
+                        hi[d] = 1;
+                    }
+                }
+                for (int cz = lo[2]; cz <= hi[2]; ++cz)
+                    for (int cy = lo[1]; cy <= hi[1]; ++cy)
+                        for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+                            const int c[3] = {2 * (q[0] + o[0]) + cx, 2 * (q[1] + o[1]) + cy,
+                                              dim == 3 ? 2 * (q[2] + o[2]) + cz : 0};
+                            t_fin.push_back({find(L + 1, c), {cx, cy, cz}});
+                        }
+                t_fin_off[e + 1] = (int)t_fin.size();
+            }
+        }
+        topo_ok = true;
+    }
+    int mother(int i) const
+    {
+        topo();
+        return t_mom[i];
+    }
+    int daughter(int i, int k) const
+    {
+        topo();
+        return t_dau[((size_t)i << dim) + k];
+    }
+    // block_is_leaf (block_relations.f90); check_empty: REF_TMP_EMPTY daughters don't count
+    bool is_leaf(int i, bool check_empty) const
+    {
+        topo();
+        const int ns = 1 << dim;
+        for (int k = 0; k < ns; ++k) {
+            const int d = t_dau[(size_t)i * ns + k];
+            if (d >= 0 && (!check_empty || b[d].status != REF_TMP_EMPTY)) return false;
+        }
+        return true;
+    }
+    int nb_same(int i, const int o[3]) const
+    {
+        topo();
+        return t_same[(size_t)i * nd + dir_index(o)];
+    }
+    // the coarser block covering the ghost box of block i in direction o
+    int cover(int i, const int o[3]) const
+    {
+        topo();
+        return t_cover[(size_t)i * nd + dir_index(o)];
+    }
+    // the finer blocks adjacent to block i in direction o (children of the
+    // virtual same-level neighbour); child cc per dimension in out
+    void finer_children(int i, const int o[3], std::vector<std::pair<int, std::array<int, 3>>> &out) const
+    {
+        topo();
+        const size_t e = (size_t)i * nd + dir_index(o);
+        out.assign(t_fin.begin() + t_fin_off[e], t_fin.begin() + t_fin_off[e + 1]);
+    }
+    bool any_finer(int i, const int o[3], bool need_valid) const
+    {
+        topo();
+        const size_t e = (size_t)i * nd + dir_index(o);
+        for (int j = t_fin_off[e]; j < t_fin_off[e + 1]; ++j) {
+            const int f = t_fin[j].first;
+            if (f >= 0 && (!need_valid || valid(f))) return true;
+        }
+        return false;
+    }
+    int nb_coarse(int i, const int o[3]) const
+    {
+        topo();
+        return t_coarse[(size_t)i * nd + dir_index(o)];
+    }
     int add(int64_t tc, int level, int status)
     {
         WH_CHECK(!F->free_hvy.empty(), -1, "adapt_tree: pool out of slots for the full tree");
         const int h = F->free_hvy.back();
         F->free_hvy.pop_back();
-        b.push_back(TNode{tc, level, h, status});
-        map[fkey(level, tc)] = (int)b.size() - 1;
+        b.push_back(node(tc, level, h, status));
+        insert((int)b.size() - 1);
+        topo_ok = false;
         return (int)b.size() - 1;
     }
     void remove_marked(const std::vector<char> &del)
@@ -188,20 +320,6 @@ struct Tree {
     }
 };
 
-int dirs_of(int dim, int dirs[26][3])
-{
-    int nd = 0;
-    for (int oz = (dim == 3 ? -1 : 0); oz <= (dim == 3 ? 1 : 0); ++oz)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int ox = -1; ox <= 1; ++ox) {
-                if (!ox && !oy && !oz) continue;
-                dirs[nd][0] = ox;
-                dirs[nd][1] = oy;
-                dirs[nd][2] = oz;
-                ++nd;
-            }
-    return nd;
-}
 
 // ---- coarse extension boxes ------------------------------------------------
 // get_indices_of_modify_patch (neighborhood.f90) with lvl_diff = +1: the
@@ -390,7 +508,7 @@ int tree_sync(Tree &T, const SyncSpec &S)
                 const int v = q[d] + o[d];
                 cu[d] = v >= 0 ? v / 2 : -((-v + 1) / 2);
             }
-            const int c = T.find(R.level - 1, cu);
+            const int c = T.cover(r, o);
             if (!T.valid(c) || !T.is_leaf(c, true)) continue;
             wh_patch p{};
             p.recv = R.hvy;
@@ -455,7 +573,7 @@ int init_full_tree(Tree &T, int jmin)
         const int n0 = (int)T.b.size();
         for (int i = 0; i < n0; ++i) {
             const TNode x = T.b[i];
-            if (x.level == jmin || x.level == 0 || T.mother(i) >= 0) continue;
+            if (x.level == jmin || x.level == 0 || T.mother_raw(i) >= 0) continue;
             const int64_t mask = (((int64_t)1 << T.dim) - 1) << T.F->digit_shift(x.level);
             if (T.add(x.tc & ~mask, x.level - 1, REF_TMP_EMPTY) < 0) return WH_E_NOMEM;
             changed = true;

@@ -239,6 +239,21 @@ static int check_acm(const wh_pool *P, const wh_acm_params *p)
     return WH_OK;
 }
 
+// grow a device buffer to at least n elements (contents are not kept);
+// callers synchronise the stream first
+template <class T>
+static int grow_dev(T *&ptr, size_t &cap, size_t n)
+{
+    if (n <= cap) return WH_OK;
+    if (ptr) WH_HIP(hipFree(ptr));
+    ptr = nullptr;
+    const size_t c = std::max(n, 2 * cap);
+    cap = 0;
+    WH_HIP(hipMalloc(&ptr, sizeof(T) * c));
+    cap = c;
+    return WH_OK;
+}
+
 static void free_patchset(PatchSet &ps)
 {
     if (ps.d_patches) (void)hipFree(ps.d_patches);
@@ -486,9 +501,9 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
             }
         }
     }
-    WH_HIP(hipStreamSynchronize(P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));  // nothing in flight reads the old table
     PatchSet &ps = P->ps[sync_case];
-    free_patchset(ps);
+    ps.max_stage = 0;
     ps.host.assign(patches, patches + npatch);
     std::stable_sort(ps.host.begin(), ps.host.end(), [](const wh_patch &a, const wh_patch &b) {
         return a.stage != b.stage ? a.stage < b.stage : a.recv < b.recv;
@@ -518,10 +533,12 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
     ps.n = npatch;
     ps.nrecv = (int)ps.h_recv_ids.size();
     if (npatch) {
-        WH_HIP(hipMalloc(&ps.d_patches, sizeof(wh_patch) * npatch));
-        WH_HIP(hipMalloc(&ps.d_recv_off, sizeof(int32_t) * ps.h_recv_off.size()));
-        WH_HIP(hipMalloc(&ps.d_recv_ids, sizeof(int32_t) * ps.h_recv_ids.size()));
-        WH_HIP(hipMalloc(&ps.d_pstart, sizeof(int64_t) * ps.h_pstart.size()));
+        if (int rc = grow_dev(ps.d_patches, ps.cap_patches, (size_t)npatch)) return rc;
+        if (int rc = grow_dev(ps.d_recv_off, ps.cap_recv_off, ps.h_recv_off.size())) return rc;
+        if (int rc = grow_dev(ps.d_recv_ids, ps.cap_recv_ids, ps.h_recv_ids.size())) return rc;
+        if (int rc = grow_dev(ps.d_pstart, ps.cap_pstart, ps.h_pstart.size())) return rc;
+        // the host vectors stay untouched until the next call, which
+        // synchronises before changing them
         WH_HIP(hipMemcpyAsync(ps.d_pstart, ps.h_pstart.data(), sizeof(int64_t) * ps.h_pstart.size(),
                               hipMemcpyHostToDevice, P->stream));
         WH_HIP(hipMemcpyAsync(ps.d_patches, ps.host.data(), sizeof(wh_patch) * npatch,
@@ -530,7 +547,6 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
                               sizeof(int32_t) * ps.h_recv_off.size(), hipMemcpyHostToDevice, P->stream));
         WH_HIP(hipMemcpyAsync(ps.d_recv_ids, ps.h_recv_ids.data(),
                               sizeof(int32_t) * ps.h_recv_ids.size(), hipMemcpyHostToDevice, P->stream));
-        WH_HIP(hipStreamSynchronize(P->stream));
     }
     P->topo_version++;
     return WH_OK;

@@ -151,9 +151,22 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
         wh::set_error("wh_forest_patches: bad argument");
         return WH_E_ARG;
     }
+    if (F->pc_version == F->version && F->pc_gm == gminus && F->pc_gp == gplus && F->pc_order == order) {
+        const int n = (int)F->pcache.size();
+        *npatch = n;
+        if (out) {
+            if (cap < n) {
+                wh::set_error("wh_forest_patches: output too small");
+                return WH_E_ARG;
+            }
+            std::copy(F->pcache.begin(), F->pcache.end(), out);
+        }
+        return WH_OK;
+    }
     const int dim = F->dim, g = F->g;
     const int N = order / 2;
     std::vector<wh_patch> P;
+    std::vector<wh_patch> fp;
     int dirs[27][3], nd = 0;
     for (int oz = (dim == 3 ? -1 : 0); oz <= (dim == 3 ? 1 : 0); ++oz)
         for (int oy = -1; oy <= 1; ++oy)
@@ -208,7 +221,7 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
                         hi_c[d] = 1;
                     }
                 }
-                std::vector<wh_patch> fp;
+                fp.clear();
                 bool all = true;
                 for (int cz = lo_c[2]; cz <= hi_c[2] && all; ++cz)
                     for (int cy = lo_c[1]; cy <= hi_c[1] && all; ++cy)
@@ -315,6 +328,11 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
         }
         std::copy(P.begin(), P.end(), out);
     }
+    F->pcache.swap(P);
+    F->pc_gm = gminus;
+    F->pc_gp = gplus;
+    F->pc_order = order;
+    F->pc_version = F->version;
     return WH_OK;
 }
 

@@ -39,8 +39,17 @@ struct wh_forest {
     std::unordered_map<uint64_t, int> map;  // (level, tc) -> index in blocks
     std::vector<int> free_hvy;              // free pool slots
 
+    // bumped by every change of the block list (all changes end in rebuild_map)
+    uint64_t version = 0;
+    // the last generated ghost-patch list: a count query and the fill call
+    // that follows it generate once
+    mutable std::vector<wh_patch> pcache;
+    mutable int pc_gm = -1, pc_gp = -1, pc_order = -1;
+    mutable uint64_t pc_version = ~0ull;
+
     void rebuild_map()
     {
+        ++version;
         map.clear();
         for (size_t i = 0; i < blocks.size(); ++i) map[wh::fkey(blocks[i].level, blocks[i].tc)] = (int)i;
     }

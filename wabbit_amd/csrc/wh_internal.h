@@ -96,6 +96,8 @@ struct PatchSet {
     int stage_begin[5] = {0, 0, 0, 0, 0};  // stage s (1..3): patches [stage_begin[s], stage_begin[s+1])
     int n = 0, nrecv = 0;
     int max_stage = 0;
+    // device capacities (element counts); buffers are reused until they must grow
+    size_t cap_patches = 0, cap_recv_off = 0, cap_recv_ids = 0, cap_pstart = 0;
 };
 
 }  // namespace wh

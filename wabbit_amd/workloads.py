@@ -41,6 +41,8 @@ def get(name: str) -> dict:
                     ic=dict(kmax=8, kmin=1, amp_exponent=-2.0), pressure=True, nsteps_cfg=100)
     if name == "c2":
         return channel()
+    if name == "c3":
+        return periodic3d()
     if name == "c3u":
         return dict(name="c3u-3d-periodic-J4-Bs24", kind="uniform", dim=3, J=4, bs=24, g=6, g_rhs=2,
                     L=1.0,
@@ -166,4 +168,108 @@ def ensemble(n: int = 64, seed: int = 4321, nsteps: int = 200):
         jmax = int(rng.choice([4, 5, 6, 7]))
         bs = int(rng.choice([16, 24]))
         out.append(channel_spec(seed + 17 * i, jmax, bs, nsteps, slots_of[jmax], tag=f"member{i}-J{jmax}-Bs{bs}"))
+    return out
+
+
+def periodic3d(seed: int = 1234, jmax: int = 5, bs: int = 24, jmin: int = 2) -> dict:
+    """The adaptive 3D periodic workload (c3): unit cube, Bs=24, Jmin=2,
+    Jmax=5, CDF44, eps=1e-3; seeded solenoidal velocity (projected Fourier
+    modes 1<=|k|<=6 with a k^-5/3 energy envelope, u_rms=1), p=0; nu=1e-3,
+    c0=15, CFL 0.5, 200 RK4 steps."""
+    acm = ACMParams(dim=3, scheme=4, skew_symmetry=False, penalization=False, use_sponge=False,
+                    c_0=15.0, nu=1e-3, gamma_p=1.0, CFL=0.5, domain_size=[1.0, 1.0, 1.0])
+    return dict(name=f"c3-3d-periodic-J{jmin}-{jmax}-Bs{bs}", kind="adaptive3d", dim=3, bs=bs, jmin=jmin,
+                jmax=jmax, wavelet="CDF44", eps=1e-3, domain=(1.0, 1.0, 1.0), acm=acm,
+                tp=TimeParams(time_max=1e9), mask=None, seed=seed,
+                modes=dict(kmax=6, kmin=1, energy_exponent=-5.0 / 3.0, solenoidal=True), nsteps_cfg=200)
+
+
+class PeriodicInicond:
+    """Seeded Fourier velocity and p = 0 as a function of the block list.
+    The field of a block at level J is sampled from the periodic grid of
+    level J (2^J Bs points per direction), built once per level on first
+    use: the modes (and the u_rms normalisation, exact on any grid finer
+    than the highest mode) are the same on every level."""
+
+    def __init__(self, w: dict, g: int):
+        self.w, self.g = w, g
+        self.grids = {}
+
+    def grid(self, level):
+        if level not in self.grids:
+            w = self.w
+            N = 2 ** level * w["bs"]
+            self.grids[level] = fourier_velocity_grid(w["dim"], N, w["seed"], **w["modes"])
+        return self.grids[level]
+
+    def __call__(self, treecodes, levels, jmax_tc):
+        w, dim = self.w, self.w["dim"]
+        tc = np.asarray(treecodes)
+        lv = np.asarray(levels)
+        nz = w["bs"] + 2 * self.g if dim == 3 else 1
+        n1 = w["bs"] + 2 * self.g
+        out = np.zeros((len(tc), dim + 1, nz, n1, n1))
+        for L in sorted(set(int(x) for x in lv)):
+            sel = np.nonzero(lv == L)[0]
+            comps = self.grid(L)
+            for d in range(dim):
+                out[sel, d] = sample_tree(tc[sel], lv[sel], jmax_tc, dim, w["bs"], self.g, comps[d], L)
+        return out
+
+
+def adaptive_solver(w: dict, path=None, nb_max: int = 16384):
+    """The device solver of an adaptive workload (c2 or c3) on its initial grid."""
+    if w["kind"] == "adaptive":
+        return channel_solver(w, path=path, nb_max=nb_max)
+    from . import wavelets as W
+    from .adaptive import AdaptConfig, AdaptiveACM, fd_reach
+    from .forest import equidistant_treecodes
+    from .timestep import PATH_FUSED
+    cfg = AdaptConfig(wavelet=w["wavelet"], bs=w["bs"], jmin=w["jmin"], jmax=w["jmax"], eps=w["eps"],
+                      groups=[1] * (w["dim"] + 1))
+    g = W.setup(w["wavelet"], fd_reach(w["acm"].scheme)).g
+    ic = PeriodicInicond(w, g)
+    tc, lv = equidistant_treecodes(w["jmin"], w["dim"], w["jmax"])
+    s = AdaptiveACM(w["acm"], w["tp"], cfg, tc, lv, ic(tc, lv, w["jmax"]), mask=None, nb_max=nb_max,
+                    path=PATH_FUSED if path is None else path, jmax_treecode=w["jmax"])
+    s.set_initial_grid(ic)
+    return s
+
+
+def periodic3d_spec(seed: int, jmax: int, bs: int, nsteps: int, slots: int, tag: str = ""):
+    """The 3D periodic workload with its own seeded field as one solve of a batch."""
+    from . import wavelets as W
+    from .adaptive import AdaptConfig, fd_reach
+    from .batch import SolveSpec
+    w = periodic3d(seed=seed, jmax=jmax, bs=bs)
+    g = W.setup(w["wavelet"], fd_reach(w["acm"].scheme)).g
+    cfg = AdaptConfig(wavelet=w["wavelet"], bs=bs, jmin=w["jmin"], jmax=jmax, eps=w["eps"], groups=[1, 1, 1, 1])
+    return SolveSpec(acm=w["acm"], cfg=cfg, inicond=PeriodicInicond(w, g), nsteps=nsteps, mask=None, tp=w["tp"],
+                     slots=slots, tag=tag or w["name"] + f"-seed{seed}")
+
+
+# pool slots reserved per solve, by dimension and Jmax
+SLOTS_2D = {4: 1024, 5: 2560, 6: 6144, 7: 12288}
+SLOTS_3D = {3: 768, 4: 2048, 5: 4096}
+
+
+def serving_mix(n: int = 256, seed: int = 2024, rate: float = 4.0, frac_2d: float = 0.7):
+    """The serving stream: n requests with Poisson arrivals of rate `rate`
+    per second; 70% 2D penalised channel solves (Jmax 4-7, Bs 16/24, 100-500
+    steps), 30% 3D periodic solves (Jmax 4-5, Bs 24, 50-200 steps).  Returns
+    [(rid, spec, arrival seconds)]."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        s = seed * 7919 + 13 * i
+        if rng.random() < frac_2d:
+            jmax, bs = int(rng.choice([4, 5, 6, 7])), int(rng.choice([16, 24]))
+            steps = int(rng.integers(100, 501))
+            spec = channel_spec(s, jmax, bs, steps, SLOTS_2D[jmax], tag=f"req{i}-2d-J{jmax}-Bs{bs}-{steps}st")
+        else:
+            jmax = int(rng.choice([4, 5]))
+            steps = int(rng.integers(50, 201))
+            spec = periodic3d_spec(s, jmax, 24, steps, SLOTS_3D[jmax], tag=f"req{i}-3d-J{jmax}-Bs24-{steps}st")
+        out.append((i, spec, t))
     return out
