@@ -9,15 +9,28 @@
  * nx = Bs+2g (nz = 1 in 2D).  Index conventions below are 0-based; the
  * reference's 1-based index i is i-1 here.
  *
- * Pinning: tests/test_oracle_wavelet.py
+ * Pinning:
  *   or_wavelet_setup     closed-form tables (CDF40/CDF44 HD/GD/HR/GR and CE
  *                        widths restated from the formulas), sum rules
- *                        (sum HD = 1, sum HR = 2, vanishing moments of GD).
+ *                        (sum HD = 1, sum HR = 2, vanishing moments of GD)
+ *                        (tests/test_oracle_wavelet.py).
  *   or_fwt/or_iwt_block  perfect reconstruction IWT(FWT(u)) = u on the
  *                        interior for every wavelet; SC of polynomials.
- *   or_adapt_tree        the reference's --coarsen-everywhere outputs
- *                        (TESTING/wavelets/adaptive_CDF{20,22,40,42,44,60,62}/
- *                        vor_00200.h5) bit for bit.
+ *   or_tree_adapt        coarsening: the reference's --coarsen-everywhere
+ *                        outputs (TESTING/wavelets/adaptive_CDF{20,22,40,42,
+ *                        44,60,62}/vor_00200.h5), bit for bit for CDFX0;
+ *                        threshold indicator with normalised eps, mask
+ *                        criterion and security zone: the adaptive cylinder
+ *                        run TESTING/acm/acm_CDF44 (grids and iteration
+ *                        counts identical at t = 0.05 and 0.1,
+ *                        tests/test_oracle_acm.py).
+ *   or_tree_refine_everywhere / or_tree_sync_full / or_tree_rk_step
+ *                        the same acm_CDF44 run (every step refines, syncs
+ *                        and steps the whole tree).
+ *   or_tree_dt           same run (the dt sequence fixes the iteration count).
+ *   or_mask_block        the run's saved masks (p-norm sponge, cylinder);
+ *                        the rect sponge by symmetry/support properties on a
+ *                        non-square domain (tests/test_oracle_mask.py).
  */
 #ifndef WAVELET_ORACLE_H
 #define WAVELET_ORACLE_H

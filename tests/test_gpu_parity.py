@@ -2,7 +2,7 @@
 against the reference's own regression outputs.
 
 Bars: relative L2 <= 1e-12 after one RHS evaluation and one RK step,
-<= 1e-9 after the configured step count (north-star tolerances); block dt,
+<= 1e-9 after the configured step count; block dt,
 periodic ghost fill and the step count to t_end must agree exactly.  The
 kernels are built with -ffp-contract=off and follow the reference's term
 order, so the tests also record whether results are bit-identical.

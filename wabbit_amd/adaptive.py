@@ -64,8 +64,9 @@ class AdaptiveACM:
         self.forest = Forest(dim, jtc, bs, self.g, acm.domain_size[:dim], self.nb_max)
         self.forest.set_blocks(treecodes, levels, np.arange(n, dtype=np.int32))
         use_mask = acm.penalization or acm.use_sponge
-        self.pool = L.Pool(dim, bs, self.g, acm.neq, self.nb_max, n_work_slots=5, n_tmp_eqn=acm.neq,
-                           n_mask=6 if use_mask else 0, mask_per_block=True)
+        # the fused RK path and adapt_tree need 3 work slots, the others one per RK4 stage
+        self.pool = L.Pool(dim, bs, self.g, acm.neq, self.nb_max, n_work_slots=3 if path == PATH_FUSED else 5,
+                           n_tmp_eqn=acm.neq, n_mask=6 if use_mask else 0, mask_per_block=True)
         W.set_wavelet(self.pool, self.w)
         self.pool.upload(L.WH_BLOCK, 0, data)
         lifted = bool(self.w.lifted)

@@ -280,6 +280,32 @@ extern "C" {
 
 const char *wh_last_error(void) { return g_err.c_str(); }
 
+// marks the end of queued work that reads the pool's tables (on every return path)
+struct TablesUse {
+    wh_pool *P;
+    ~TablesUse()
+    {
+        if (P && P->ev_tables && hipEventRecord(P->ev_tables, P->stream) == hipSuccess) P->tables_recorded = true;
+    }
+};
+
+static int wait_tables(wh_pool *P)
+{
+    if (P->tables_recorded) WH_HIP(hipEventSynchronize(P->ev_tables));
+    return WH_OK;
+}
+
+// host -> pinned staging -> device buffer `dst`, ordered on the pool stream
+static int upload_staged(wh_pool *P, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return WH_OK;
+    if (int rc = stage_begin(P, bytes)) return rc;
+    void *d = stage_put(P, src, bytes);
+    if (!d) return WH_E_HIP;
+    WH_HIP(hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToDevice, P->stream));
+    return WH_OK;
+}
+
 int wh_version(void) { return 100; }
 
 int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
@@ -333,6 +359,8 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
                                        P->stream)) != hipSuccess)
         return fail("hipMemset", e);
     if ((e = hipEventCreate(&P->ev_a)) != hipSuccess) return fail("hipEventCreate", e);
+    if ((e = hipEventCreateWithFlags(&P->ev_tables, hipEventDisableTiming)) != hipSuccess)
+        return fail("hipEventCreate", e);
     if ((e = hipEventCreate(&P->ev_b)) != hipSuccess) return fail("hipEventCreate", e);
     if ((e = hipStreamSynchronize(P->stream)) != hipSuccess) return fail("sync", e);
     *out = P;
@@ -355,6 +383,7 @@ int wh_pool_destroy(wh_pool *P)
     if (P->d_dtg) (void)hipFree(P->d_dtg);
     if (P->h_scal) (void)hipHostFree(P->h_scal);
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
+    if (P->ev_tables) (void)hipEventDestroy(P->ev_tables);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -443,8 +472,9 @@ int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
         for (int d = 0; d < P->d.dim; ++d)
             WH_CHECK(x0dx[6 * k + 3 + d] > 0.0, WH_E_ARG, "block spacing must be positive");
     }
-    WH_HIP(hipStreamSynchronize(P->stream));
+    if (int rc = wait_tables(P)) return rc;
     if (n > P->act_cap) {
+        WH_HIP(hipStreamSynchronize(P->stream));  // before freeing buffers queued work may read
         if (P->d_active) WH_HIP(hipFree(P->d_active));
         if (P->d_x0dx) WH_HIP(hipFree(P->d_x0dx));
         P->d_active = nullptr;
@@ -455,12 +485,8 @@ int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
     }
     P->h_active.assign(act, act + n);
     P->h_x0dx.assign(x0dx, x0dx + 6 * n);
-    if (n) {
-        WH_HIP(hipMemcpyAsync(P->d_active, act, sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
-        WH_HIP(hipMemcpyAsync(P->d_x0dx, x0dx, sizeof(double) * 6 * n, hipMemcpyHostToDevice,
-                              P->stream));
-        WH_HIP(hipStreamSynchronize(P->stream));
-    }
+    if (int rc = upload_staged(P, P->d_active, act, sizeof(int32_t) * n)) return rc;
+    if (int rc = upload_staged(P, P->d_x0dx, x0dx, sizeof(double) * 6 * n)) return rc;
     P->nact = n;
     P->ngrp = 0;  // block groups refer to the previous active list
     P->topo_version++;
@@ -501,7 +527,7 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
             }
         }
     }
-    WH_HIP(hipStreamSynchronize(P->stream));  // nothing in flight reads the old table
+    if (int rc = wait_tables(P)) return rc;  // nothing queued reads the old table any more
     PatchSet &ps = P->ps[sync_case];
     ps.max_stage = 0;
     ps.host.assign(patches, patches + npatch);
@@ -533,20 +559,20 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
     ps.n = npatch;
     ps.nrecv = (int)ps.h_recv_ids.size();
     if (npatch) {
+        if ((size_t)npatch > ps.cap_patches || ps.h_recv_off.size() > ps.cap_recv_off ||
+            ps.h_recv_ids.size() > ps.cap_recv_ids || ps.h_pstart.size() > ps.cap_pstart)
+            WH_HIP(hipStreamSynchronize(P->stream));  // before freeing buffers queued work may read
         if (int rc = grow_dev(ps.d_patches, ps.cap_patches, (size_t)npatch)) return rc;
         if (int rc = grow_dev(ps.d_recv_off, ps.cap_recv_off, ps.h_recv_off.size())) return rc;
         if (int rc = grow_dev(ps.d_recv_ids, ps.cap_recv_ids, ps.h_recv_ids.size())) return rc;
         if (int rc = grow_dev(ps.d_pstart, ps.cap_pstart, ps.h_pstart.size())) return rc;
-        // the host vectors stay untouched until the next call, which
-        // synchronises before changing them
-        WH_HIP(hipMemcpyAsync(ps.d_pstart, ps.h_pstart.data(), sizeof(int64_t) * ps.h_pstart.size(),
-                              hipMemcpyHostToDevice, P->stream));
-        WH_HIP(hipMemcpyAsync(ps.d_patches, ps.host.data(), sizeof(wh_patch) * npatch,
-                              hipMemcpyHostToDevice, P->stream));
-        WH_HIP(hipMemcpyAsync(ps.d_recv_off, ps.h_recv_off.data(),
-                              sizeof(int32_t) * ps.h_recv_off.size(), hipMemcpyHostToDevice, P->stream));
-        WH_HIP(hipMemcpyAsync(ps.d_recv_ids, ps.h_recv_ids.data(),
-                              sizeof(int32_t) * ps.h_recv_ids.size(), hipMemcpyHostToDevice, P->stream));
+        if (int rc = upload_staged(P, ps.d_pstart, ps.h_pstart.data(), sizeof(int64_t) * ps.h_pstart.size()))
+            return rc;
+        if (int rc = upload_staged(P, ps.d_patches, ps.host.data(), sizeof(wh_patch) * npatch)) return rc;
+        if (int rc = upload_staged(P, ps.d_recv_off, ps.h_recv_off.data(), sizeof(int32_t) * ps.h_recv_off.size()))
+            return rc;
+        if (int rc = upload_staged(P, ps.d_recv_ids, ps.h_recv_ids.data(), sizeof(int32_t) * ps.h_recv_ids.size()))
+            return rc;
     }
     P->topo_version++;
     return WH_OK;
@@ -554,6 +580,7 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
 
 int wh_sync_ghosts(wh_pool *P, int array, int sync_case)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P, WH_E_ARG, "null argument");
     WH_CHECK(sync_case == 0 || sync_case == 1, WH_E_ARG, "sync_case must be 0 or 1");
     WH_CHECK(array == WH_BLOCK || array >= WH_WORK, WH_E_ARG, "sync works on hvy_block/hvy_work");
@@ -564,6 +591,7 @@ int wh_sync_ghosts(wh_pool *P, int array, int sync_case)
 
 int wh_rhs_acm(wh_pool *P, const wh_acm_params *p, double /*time*/, int in_array, int out_slot)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P && p, WH_E_ARG, "null argument");
     if (int rc = check_acm(P, p)) return rc;
     double *in = array_ptr(P, in_array, nullptr, nullptr);
@@ -576,6 +604,7 @@ int wh_rhs_acm(wh_pool *P, const wh_acm_params *p, double /*time*/, int in_array
 
 int wh_divergence_guard(wh_pool *P, int in_array, double *maxabs)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P, WH_E_ARG, "null argument");
     double *in = array_ptr(P, in_array, nullptr, nullptr);
     WH_CHECK(in, WH_E_ARG, "bad array");
@@ -593,6 +622,7 @@ int wh_divergence_guard(wh_pool *P, int in_array, double *maxabs)
 
 int wh_dt_reduce(wh_pool *P, const wh_acm_params *p, double *dt_out)
 {
+    TablesUse tables_use{P};
     WH_PROF("wh_dt_reduce");
     WH_CHECK(P && p && dt_out, WH_E_ARG, "null argument");
     if (int rc = check_acm(P, p)) return rc;
@@ -605,6 +635,7 @@ int wh_dt_reduce(wh_pool *P, const wh_acm_params *p, double *dt_out)
 int wh_rk_combine(wh_pool *P, int out_array, int base_array, int nterms, const int *term_arrays,
                   const double *coefs)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P && (nterms == 0 || (term_arrays && coefs)), WH_E_ARG, "null argument");
     double *out = array_ptr(P, out_array, nullptr, nullptr);
     const double *base = array_ptr(P, base_array < 0 ? out_array : base_array, nullptr, nullptr);
@@ -620,6 +651,7 @@ int wh_rk_combine(wh_pool *P, int out_array, int base_array, int nterms, const i
 int wh_rk_step(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows, double /*time*/,
                double dt, int path)
 {
+    TablesUse tables_use{P};
     WH_PROF("wh_rk_step");
     WH_CHECK(P && p && bt, WH_E_ARG, "null argument");
     WH_CHECK(nrows >= 2 && nrows <= 9, WH_E_ARG, "butcher tableau size");
@@ -644,7 +676,9 @@ int wh_set_block_groups(wh_pool *P, const int32_t *group, int n, int ngroups)
     WH_CHECK(ngroups >= 1 && ngroups <= (1 << 20), WH_E_ARG, "ngroups out of range");
     for (int k = 0; k < n; ++k)
         WH_CHECK(group[k] >= 0 && group[k] < ngroups, WH_E_ARG, "group index out of range");
-    WH_HIP(hipStreamSynchronize(P->stream));
+    if (int rc = wait_tables(P)) return rc;
+    if (n > P->grp_cap || ngroups > P->dtg_cap)
+        WH_HIP(hipStreamSynchronize(P->stream));  // before freeing buffers queued work may read
     if (n > P->grp_cap) {
         if (P->d_grp) WH_HIP(hipFree(P->d_grp));
         if (P->d_dtb) WH_HIP(hipFree(P->d_dtb));
@@ -661,16 +695,14 @@ int wh_set_block_groups(wh_pool *P, const int32_t *group, int n, int ngroups)
         P->dtg_cap = ngroups;
     }
     P->h_grp.assign(group, group + n);
-    if (n) {
-        WH_HIP(hipMemcpyAsync(P->d_grp, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
-        WH_HIP(hipStreamSynchronize(P->stream));
-    }
+    if (int rc = upload_staged(P, P->d_grp, group, sizeof(int32_t) * n)) return rc;
     P->ngrp = ngroups;
     return WH_OK;
 }
 
 int wh_dt_reduce_groups(wh_pool *P, const wh_acm_params *p, double *dt_out)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P && p && dt_out, WH_E_ARG, "null argument");
     WH_CHECK(P->ngrp > 0, WH_E_STATE, "no block groups set");
     if (int rc = check_acm(P, p)) return rc;
@@ -687,6 +719,7 @@ int wh_dt_reduce_groups(wh_pool *P, const wh_acm_params *p, double *dt_out)
 int wh_rk_step_groups(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows,
                       const double *dt_groups, int path)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P && p && bt && dt_groups, WH_E_ARG, "null argument");
     WH_CHECK(nrows >= 2 && nrows <= 9, WH_E_ARG, "butcher tableau size");
     WH_CHECK(P->ngrp > 0, WH_E_STATE, "no block groups set");
@@ -699,13 +732,8 @@ int wh_rk_step_groups(wh_pool *P, const wh_acm_params *p, const double *bt, int 
     if (path == 2) {
         WH_CHECK(fused_ok(bt, nrows), WH_E_ARG, "tableau not supported by the fused path");
         WH_CHECK(P->d.n_work_slots >= 3, WH_E_ARG, "fused path needs 3 work slots");
-        if (P->nact)
-            WH_HIP(hipMemcpyAsync(P->d_dtb, dt_pos.data(), sizeof(double) * P->nact,
-                                  hipMemcpyHostToDevice, P->stream));
-        int rc = fused_step(P, a, bt, nrows, P->d_dtb);
-        if (rc) return rc;
-        WH_HIP(hipStreamSynchronize(P->stream));  // dt_pos is read by the copy above
-        return WH_OK;
+        if (int rc = upload_staged(P, P->d_dtb, dt_pos.data(), sizeof(double) * P->nact)) return rc;
+        return fused_step(P, a, bt, nrows, P->d_dtb);
     }
     WH_CHECK(path == 1, WH_E_ARG, "block groups run on path 1 (per block) or 2 (fused)");
     WH_CHECK(P->d.n_work_slots >= nrows, WH_E_ARG, "need nrows work slots");
@@ -748,6 +776,7 @@ static int one_device_step(wh_pool *P, const AcmDev &a, const double *bt, int nr
 int wh_run_steps(wh_pool *P, const wh_acm_params *p, const double *bt, int nrows,
                  const wh_time_params *tp, int nsteps, int use_graph)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P && p && bt && tp, WH_E_ARG, "null argument");
     WH_CHECK(nrows >= 3 && nrows <= 9 && fused_ok(bt, nrows), WH_E_ARG,
              "tableau not supported by the fused path");
@@ -838,6 +867,7 @@ int wh_get_tuning(wh_pool *P, int key, int *value)
 
 int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, double *best_ms)
 {
+    TablesUse tables_use{P};
     WH_CHECK(P && p, WH_E_ARG, "null argument");
     if (int rc = check_acm(P, p)) return rc;
     WH_CHECK(P->d.n_work_slots >= 2, WH_E_ARG, "autotune needs work slot 1 as scratch");

@@ -122,6 +122,11 @@ struct wh_pool {
     // timing
     bool timing = false;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // recorded after the last queued work that reads the active list, the
+    // patch tables or the block groups: replacing them waits for this event
+    // only, not for everything queued since
+    hipEvent_t ev_tables = nullptr;
+    bool tables_recorded = false;
     double t_ms[2] = {0, 0};
     int t_cnt[2] = {0, 0};
     // graph

@@ -220,6 +220,34 @@ module wabbit_hip
             integer(c_int), intent(out) :: ntriples
             integer(c_int) :: rc
         end function
+        function wh_forest_set_slot_base(forest, first) bind(C, name="wh_forest_set_slot_base") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: forest
+            integer(c_int), value :: first
+            integer(c_int) :: rc
+        end function
+        function wh_set_block_groups(pool, group, n, ngroups) bind(C, name="wh_set_block_groups") result(rc)
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: pool
+            integer(c_int32_t), intent(in) :: group(*)
+            integer(c_int), value :: n, ngroups
+            integer(c_int) :: rc
+        end function
+        function wh_dt_reduce_groups(pool, p, dt_out) bind(C, name="wh_dt_reduce_groups") result(rc)
+            import :: c_ptr, c_int, c_double, wh_acm_params
+            type(c_ptr), value :: pool
+            type(wh_acm_params), intent(in) :: p
+            real(c_double), intent(out) :: dt_out(*)
+            integer(c_int) :: rc
+        end function
+        function wh_rk_step_groups(pool, p, butcher, nrows, dt_groups, path) bind(C, name="wh_rk_step_groups") result(rc)
+            import :: c_ptr, c_int, c_double, wh_acm_params
+            type(c_ptr), value :: pool
+            type(wh_acm_params), intent(in) :: p
+            real(c_double), intent(in) :: butcher(*), dt_groups(*)
+            integer(c_int), value :: nrows, path
+            integer(c_int) :: rc
+        end function
         function wh_refine_blocks(pool, array, triples, ntriples, order) bind(C, name="wh_refine_blocks") result(rc)
             import :: c_ptr, c_int, c_int32_t
             type(c_ptr), value :: pool
