@@ -274,15 +274,17 @@ def run_ensemble(args):
     steps = args.steps if args.steps is not None else 200
     warmup = args.warmup
     specs = workloads.ensemble(args.members, nsteps=warmup + steps + 2)
-    slots = {}
+    lanes = args.lanes
+    tot, big = {}, {}
     for sp in specs:
         k = (2, sp.cfg.bs)
-        slots[k] = slots.get(k, 0) + sp.slots
-    lanes = args.lanes
-    slots = {k: v + v // 8 for k, v in slots.items()}   # slack for the split over lanes
+        tot[k] = tot.get(k, 0) + sp.slots
+        big[k] = max(big.get(k, 0), sp.slots)
+    # per lane: an even share plus room for the largest member (first-fit slack)
+    slots = {k: lanes * (-(-tot[k] // lanes) + big[k]) for k in tot}
     t0 = time.perf_counter()
     b = Batch(slots, n_work_slots=3, lanes=lanes)
-    for sp in specs:
+    for sp in sorted(specs, key=lambda x: -x.slots):      # largest first: packs the lanes
         if b.admit(sp) is None:
             raise SystemExit(f"ensemble member {sp.tag} does not fit")
     print(f"# ensemble: {len(specs)} members admitted, {sum(len(s.forest.blocks()[0]) for s in b.solves)} "
@@ -352,9 +354,12 @@ def run_serving(args):
     from wabbit_amd.serve import Request, Server
     mix = workloads.serving_mix(n=args.requests, rate=args.rate)
     reqs = [Request(rid, spec, t) for rid, spec, t in mix]
-    # pools: 2D Bs16, 2D Bs24, 3D Bs24 -- about 60% of the 288 GB
-    slots = {(2, 16): 48000, (2, 24): 48000, (3, 24): 18000}
-    srv = Server(slots, n_work_slots=3, lanes=args.lanes)
+    # pools: 2D Bs16, 2D Bs24, 3D Bs24 -- about 55% of the 288 GB; each of the
+    # lanes gets 1/lanes of every pool, enough for the largest request (2D
+    # Jmax 7: 12288 slots, 3D Jmax 5: 4096 slots)
+    lanes = args.lanes
+    slots = {(2, 16): 16000 * lanes, (2, 24): 16000 * lanes, (3, 24): 4500 * lanes}
+    srv = Server(slots, n_work_slots=3, lanes=lanes)
     st = srv.run(reqs, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     srv.close()
     out = {
@@ -367,6 +372,7 @@ def run_serving(args):
         "mean_queue_wait_s": round(st.mean_wait, 4), "completed": st.completed, "failed": st.failed,
         "wall_s": round(st.wall, 3), "block_RK_stages_per_s": round(st.block_rk_stages / st.wall, 1),
         "max_concurrent": st.max_concurrent,
+        "failures": [dict(tag=r["tag"], error=r["error"]) for r in st.per_request if r["error"]],
     }
     print(json.dumps(out), flush=True)
 

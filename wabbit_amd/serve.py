@@ -81,6 +81,8 @@ class Server:
                 except L.WabbitHipError as e:   # its initial grid does not fit its slots
                     r.error, r.finished = str(e), clock() - t0
                     waiting.remove(r)
+                    if log:
+                        log(f"request {r.rid} ({r.spec.tag}) failed at admission: {e}")
                     continue
                 if s is not None:
                     r.admitted = clock() - t0
@@ -93,6 +95,8 @@ class Server:
                     for r in waiting:
                         r.error = "does not fit in its pool"
                         r.finished = clock() - t0
+                        if log:
+                            log(f"request {r.rid} ({r.spec.tag}) does not fit in its pool")
                     waiting = []
                 continue
             max_conc = max(max_conc, len(self.batch.solves))
