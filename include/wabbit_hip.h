@@ -252,11 +252,12 @@ int wh_wavelet_setup(const char *name, int fd_max_size, wh_wavelet *out);
 /* The wavelet the pool's transform kernels use. */
 int wh_pool_set_wavelet(wh_pool *pool, const wh_wavelet *w);
 
-/* The transform, box, patch-list and mask calls below are ordered on the
- * pool's stream and may return before the device has finished (the host
- * lists are staged through pinned memory, so the caller's buffers are free
- * on return); downloads, wh_detail_blocks, wh_sync_tree and wh_adapt_tree
- * return with the stream synchronised.
+/* The transform, box, patch-list, mask, wh_sync_tree and wh_adapt_tree calls
+ * below are ordered on the pool's stream and may return before the device
+ * has finished (the host lists are staged through pinned memory, so the
+ * caller's buffers are free on return; wh_adapt_tree's new leaf list is in
+ * the forest on return); downloads and wh_detail_blocks return with the
+ * stream synchronised.
  *
  * Per-block transforms of the listed hvy ids, src and dst distinct arrays;
  * only the region the reference computes is written (dst keeps the rest):
@@ -338,8 +339,12 @@ int wh_sync_tree(wh_pool *pool, wh_forest *forest, int array);
 
 /* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
  * stage kernel (each owns Bs2/ysplit rows of (x,y) columns and marches in z);
- * 0 selects the flat one-point-per-thread kernel, -1 automatic. */
-enum { WH_TUNE_K3D_YSPLIT = 1 };
+ * 0 selects the flat one-point-per-thread kernel, -1 automatic.
+ * WH_TUNE_K2D_STAGE: 1 (default) has the 2D kernel load the mask and stage
+ * operands of each thread's first point before the halo-tile barrier, 0
+ * reads them after it.
+ * Every variant computes the same values bit for bit. */
+enum { WH_TUNE_K3D_YSPLIT = 1, WH_TUNE_K2D_STAGE = 2 };
 int wh_set_tuning(wh_pool *pool, int key, int value);
 int wh_get_tuning(wh_pool *pool, int key, int *value);
 

@@ -245,3 +245,22 @@ def test_3d_kernel_variants_identical(scheme, ysplit):
     best, ms = pool.autotune(L.acm_params_c(acm), reps=2)
     assert best in (0, 1, 2, 3, 4, 6, 8) and ms > 0
     pool.close()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("path", [PATH_FUSED, PATH_REFERENCE])
+def test_2d_kernel_staging_variants_identical(scheme, path):
+    """The 2D stage kernel with the mask and stage operands staged in LDS
+    (WH_TUNE_K2D_STAGE 1, where they fit) gives the same state bit for bit
+    as with direct reads (0), penalised + sponge, one RK4 step."""
+    forest, acm, hvy, mask = _setup(2, scheme, False, True, True)
+    out = {}
+    for stage in (0, 1):
+        solver = AcmSolver(forest, acm, TimeParams(time_max=10.0), REACH[scheme], path=path, mask=mask,
+                           n_mask=6)
+        L.check(L.load().wh_set_tuning(solver.pool.h, L.WH_TUNE_K2D_STAGE, stage))
+        solver.load(hvy)
+        solver.step()
+        out[stage] = solver.state()
+        solver.pool.close()
+    assert np.array_equal(out[0], out[1])

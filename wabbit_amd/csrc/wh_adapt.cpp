@@ -1009,8 +1009,7 @@ int wh_sync_tree(wh_pool *P, wh_forest *F, int array)
     Tree T(F, P);
     SyncSpec S;
     S.gm = S.gp = T.g;
-    if (int rc = tree_sync(T, S)) return rc;
-    return stage_sync(P);  // the data is final when the call returns
+    return tree_sync(T, S);
 }
 
 int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
@@ -1091,8 +1090,7 @@ int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
     store_leaves(T);
     SyncSpec S;
     S.gm = S.gp = T.g;
-    if (int rc = tree_sync(T, S)) return rc;
-    return stage_sync(P);  // the data is final when the call returns
+    return tree_sync(T, S);
 }
 
 }  // extern "C"

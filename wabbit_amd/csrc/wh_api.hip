@@ -848,7 +848,13 @@ extern "C" {
 int wh_set_tuning(wh_pool *P, int key, int value)
 {
     WH_CHECK(P, WH_E_ARG, "null argument");
-    WH_CHECK(key == WH_TUNE_K3D_YSPLIT, WH_E_ARG, "unknown tuning key");
+    WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE, WH_E_ARG, "unknown tuning key");
+    if (key == WH_TUNE_K2D_STAGE) {
+        WH_CHECK(value == 0 || value == 1, WH_E_ARG, "K2D_STAGE is 0 or 1");
+        P->k2d_stage = value;
+        P->topo_version++;
+        return WH_OK;
+    }
     WH_CHECK(value >= -1 && value <= 64, WH_E_ARG, "ysplit out of range");
     // a split that is not valid for a launch's scheme falls back to the
     // automatic choice at that launch (zm_split)
@@ -860,8 +866,8 @@ int wh_set_tuning(wh_pool *P, int key, int value)
 int wh_get_tuning(wh_pool *P, int key, int *value)
 {
     WH_CHECK(P && value, WH_E_ARG, "null argument");
-    WH_CHECK(key == WH_TUNE_K3D_YSPLIT, WH_E_ARG, "unknown tuning key");
-    *value = P->k3d_ysplit;
+    WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE, WH_E_ARG, "unknown tuning key");
+    *value = key == WH_TUNE_K2D_STAGE ? P->k2d_stage : P->k3d_ysplit;
     return WH_OK;
 }
 

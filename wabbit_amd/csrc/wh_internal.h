@@ -138,6 +138,8 @@ struct wh_pool {
     wh::TimeDev graph_tc{};
     // tuning: y-split of the 3D stage kernel (-1 auto, 0 flat kernel)
     int k3d_ysplit = -1;
+    // tuning: 2D kernel loads mask and epilogue operands before the tile barrier (1) or after (0)
+    int k2d_stage = 1;
     // wavelet transforms / adapt_tree
     wh::WaveletWS ws;
     // block groups (independent forests in one pool): group of each active

@@ -204,8 +204,37 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &sa, int k, size_
     }
 }
 
+// the fused epilogue with the acc / u0 values of the point already loaded
+// (accv, u0v: components npts apart)
+template <int NC>
+__device__ __forceinline__ void stage_epilogue_pre(const StageArgs &sa, int k, size_t o, size_t cs,
+                                                   const double *r, const double *ucen, ptrdiff_t ucs,
+                                                   const double *accv, const double *u0v, int npts)
+{
+    const double dt = sa.dtb ? sa.dtb[k] : sa.scal[DS_DT];
+    const double ca = dt * sa.a, cb = dt * sa.b;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const size_t i = o + c * cs;
+        if (sa.mode == 0) {
+            const double u0 = ucen[c * ucs];
+            sa.acc[i] = sa.use_b ? u0 + cb * r[c] : u0;
+            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+        } else if (sa.mode == 1) {
+            const double u0 = u0v[c * npts];
+            if (sa.use_b) sa.acc[i] = accv[c * npts] + cb * r[c];
+            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+        } else {
+            sa.sout[i] = sa.use_b ? accv[c * npts] + cb * r[c] : accv[c * npts];
+        }
+    }
+}
+
 // 2D: one workgroup per block, the (Bs+2R)^2 x 3 halo tile staged in LDS.
-template <int S, bool SKEW, bool FUSED>
+// With PRE each thread loads the mask and, for the fused stages, the acc /
+// u0 values of its first point into registers before the tile barrier, so
+// that these reads overlap the tile load instead of following it.
+template <int S, bool SKEW, bool FUSED, bool PRE>
 __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, double *__restrict__ out,
                                                const double *__restrict__ mask,
                                                const int32_t *__restrict__ act,
@@ -217,7 +246,31 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
     const int k = blockIdx.x;
     const int b = act[k];
     const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R, tcs = tx * ty;
+    const int npts = G.bs0 * G.bs1;
     const double *u = in + (size_t)b * G.blk;
+    const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
+    const size_t base = (size_t)b * G.blk;
+    const int p0 = threadIdx.x;
+    double mpre[6] = {0, 0, 0, 0, 0, 0}, apre[3] = {0, 0, 0}, upre[3] = {0, 0, 0};
+    if constexpr (PRE) {
+        if (p0 < npts) {
+            const int iy = p0 / G.bs0, ix = p0 - iy * G.bs0;
+            const size_t gi = (size_t)(G.g + iy) * G.nx + (G.g + ix);
+            if (mk) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    if (c != 3) mpre[c] = mk[c * G.cs + gi];  // component 3 (u_s,z) is 3D only
+            }
+            if (FUSED && sa.mode >= 1) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) apre[c] = sa.acc[base + c * G.cs + gi];
+                if (sa.mode == 1) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) upre[c] = sa.u0[base + c * G.cs + gi];
+                }
+            }
+        }
+    }
     for (int i = threadIdx.x; i < 3 * tcs; i += blockDim.x) {
         const int c = i / tcs, rr = i - c * tcs;
         const int iy = rr / tx, ix = rr - iy * tx;
@@ -227,18 +280,23 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
     const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4];
     const double inv[2] = {1.0 / dx0, 1.0 / dx1};
     const double inv2[2] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1)};
-    const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
-    for (int pnt = threadIdx.x; pnt < G.bs0 * G.bs1; pnt += blockDim.x) {
+    for (int pnt = threadIdx.x; pnt < npts; pnt += blockDim.x) {
         const int iy = pnt / G.bs0, ix = pnt - iy * G.bs0;
         const double *uc = tile + (iy + R) * tx + (ix + R);
         const size_t gi = (size_t)(G.g + iy) * G.nx + (G.g + ix);
         double r[3];
         const StridedAcc<2> A{uc, tcs, {1, tx, 0}};
-        acm_point<2, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
-        const size_t o = (size_t)b * G.blk + gi;
-        if constexpr (FUSED) {
-            stage_epilogue<3>(sa, k, o, G.cs, r, uc, tcs);
+        const size_t o = base + gi;
+        if (PRE && pnt == p0) {
+            // separate calls (not a pointer select) keep mpre in registers
+            if (mk) acm_point<2, S, SKEW>(A, inv, inv2, a, mpre, 1, r);
+            else acm_point<2, S, SKEW>(A, inv, inv2, a, nullptr, 1, r);
+            if constexpr (FUSED) stage_epilogue_pre<3>(sa, k, o, G.cs, r, uc, tcs, apre, upre, 1);
         } else {
+            acm_point<2, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
+            if constexpr (FUSED) stage_epilogue<3>(sa, k, o, G.cs, r, uc, tcs);
+        }
+        if constexpr (!FUSED) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) out[o + c * G.cs] = r[c];
         }
@@ -447,10 +505,17 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
         int R = a.scheme == WH_FD2 ? 1 : a.scheme == WH_FD4 ? 2 : 3;
         size_t lds = sizeof(double) * 3 * (G.bs0 + 2 * R) * (G.bs1 + 2 * R);
         WH_CHECK(lds <= 160 * 1024, WH_E_ARG, "2D block tile exceeds LDS");
+        const int pre = P->k2d_stage != 0;
         dim3 grid(nact), blockd(256);
-#define L2D(SC, SK)                                                                          \
-    hipLaunchKernelGGL((k_rhs2d<SC, SK, FUSED>), grid, blockd, lds, P->stream, in, out, mask, \
-                       act, x0dx, G, a, sa)
+#define L2D(SC, SK)                                                                              \
+    do {                                                                                         \
+        if (pre)                                                                                 \
+            hipLaunchKernelGGL((k_rhs2d<SC, SK, FUSED, true>), grid, blockd, lds, P->stream, in, out, \
+                               mask, act, x0dx, G, a, sa);                                       \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_rhs2d<SC, SK, FUSED, false>), grid, blockd, lds, P->stream, in,     \
+                               out, mask, act, x0dx, G, a, sa);                                  \
+    } while (0)
         switch (a.scheme * 2 + a.skew) {
         case WH_FD2 * 2: L2D(WH_FD2, false); break;
         case WH_FD2 * 2 + 1: L2D(WH_FD2, true); break;
