@@ -170,7 +170,9 @@ def run_adaptive(args, w):
     print(f"# workload {w['name']}: initial grid {s.nblocks} blocks ({time.perf_counter() - t0:.1f} s)",
           file=sys.stderr, flush=True)
     init_state = s.state()
-    for _ in range(warmup):
+    for k in range(warmup):
+        if k == warmup - 1:
+            s.autotune_next_step()   # stage-kernel variant on a refined grid
         s.step()
     s.pool.synchronize()
     torch_sync()
@@ -214,6 +216,8 @@ def run_adaptive(args, w):
     if traffic is not None:
         roofline["traffic_per_block_stage"] = C2_STAGE_TRAFFIC_PER_BLOCK
 
+    if getattr(s, "tuned", None) is not None:
+        roofline["tuned_variant"] = s.tuned[0]
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
@@ -386,7 +390,9 @@ def main():
     ap.add_argument("--members", type=int, default=64, help="c4: ensemble members")
     ap.add_argument("--requests", type=int, default=24, help="c5: requests taken from the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (1/s)")
-    ap.add_argument("--lanes", type=int, default=4, help="c4/c5: host threads (pool sets) stepping solves")
+    ap.add_argument("--lanes", type=int, default=None,
+                    help="c4/c5: host threads (pool sets) stepping solves (default c4: 8, c5: 4 -- c5's "
+                         "3D pool share per lane must hold a Jmax-5 request within ~55%% of HBM)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -400,9 +406,11 @@ def main():
               "(there is no multi-GPU decomposition)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c4":
+        args.lanes = args.lanes or 8
         run_ensemble(args)
         return
     if args.workload == "c5":
+        args.lanes = args.lanes or 4
         run_serving(args)
         return
     w = workload(args.workload)

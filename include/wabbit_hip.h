@@ -350,7 +350,8 @@ int wh_get_tuning(wh_pool *pool, int key, int *value);
 
 /* Autotuner: times every valid variant of the RHS/stage kernel for this
  * pool's (Bs, g, neq) and parameters on the current active blocks, keeps the
- * fastest and returns it.  Work slot 1 is used as scratch (overwritten). */
+ * fastest and returns it (3D: the WH_TUNE_K3D_YSPLIT value, 2D: the
+ * WH_TUNE_K2D_STAGE value).  Work slot 1 is used as scratch (overwritten). */
 int wh_autotune(wh_pool *pool, const wh_acm_params *p, int reps, int *best_value, double *best_ms);
 
 /* HIP-event timing of kernel kinds (0: RHS or fused stage kernel, 1: ghost

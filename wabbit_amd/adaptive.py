@@ -147,6 +147,11 @@ class AdaptiveACM:
             it += 1
         return it
 
+    def autotune_next_step(self):
+        """Pick the stage-kernel variant on the next step's refined grid
+        (the live blocks of a real RK stage; all variants are bit-identical)."""
+        self._tune = True
+
     def step(self):
         """One main-loop iteration; returns dt.  ``rk_blocks`` holds the
         number of blocks the RK stages ran on (the refined grid)."""
@@ -155,6 +160,9 @@ class AdaptiveACM:
         t = self._tick(t, "sync_tree")
         self.refine()
         self.rk_blocks = self.forest.n
+        if getattr(self, "_tune", False):
+            self._tune = False
+            self.tuned = self.pool.autotune(self.acm_c, reps=3)
         t = self._tick(t, "refine")
         if self.mask_p is not None:
             W.create_mask(self.pool, self.mask_p)

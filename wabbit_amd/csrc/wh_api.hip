@@ -880,30 +880,34 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
     WH_CHECK(P->nact > 0, WH_E_STATE, "autotune needs an active list");
     WH_CHECK(!P->timing, WH_E_STATE, "disable kernel timing before autotuning");
     reps = std::max(1, std::min(reps, 100));
+    // 3D: the flat kernel and the z-marching splits; 2D: the load order of
+    // the mask and stage operands.  The variant value is reported in best_value.
     std::vector<int> cand;
-    if (P->d.dim == 3) {
+    const bool three = P->d.dim == 3;
+    if (three) {
         cand.push_back(0);
         for (int ys = 1; ys <= 8; ++ys)
             if (zm_valid(P->d.bs[0], P->d.bs[1], p->scheme, ys)) cand.push_back(ys);
     } else {
-        cand.push_back(P->k3d_ysplit);  // 2D has a single variant
+        cand = {0, 1};
     }
+    int &knob = three ? P->k3d_ysplit : P->k2d_stage;
     const AcmDev a = make_acm_dev(p);
     double *out = array_ptr(P, WH_WORK + 1, nullptr, nullptr);
-    const int saved = P->k3d_ysplit;
+    const int saved = knob;
     int best = saved;
     double bms = 1e300;
     for (int v : cand) {
-        P->k3d_ysplit = v;
+        knob = v;
         int rc = launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact);  // warm
         if (rc) {
-            P->k3d_ysplit = saved;
+            knob = saved;
             return rc;
         }
         WH_HIP(hipEventRecord(P->ev_a, P->stream));
         for (int r = 0; r < reps; ++r)
             if ((rc = launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact))) {
-                P->k3d_ysplit = saved;
+                knob = saved;
                 return rc;
             }
         WH_HIP(hipEventRecord(P->ev_b, P->stream));
@@ -915,7 +919,7 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
             best = v;
         }
     }
-    P->k3d_ysplit = best;
+    knob = best;
     P->topo_version++;
     if (best_value) *best_value = best;
     if (best_ms) *best_ms = bms;
