@@ -340,9 +340,11 @@ int wh_sync_tree(wh_pool *pool, wh_forest *forest, int array);
 /* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
  * stage kernel (each owns Bs2/ysplit rows of (x,y) columns and marches in z);
  * 0 selects the flat one-point-per-thread kernel, -1 automatic.
- * WH_TUNE_K2D_STAGE: 1 (default) has the 2D kernel load the mask and stage
- * operands of each thread's first point before the halo-tile barrier, 0
- * reads them after it.
+ * WH_TUNE_K2D_STAGE: 2D stage kernel variant: 0 one workgroup per block;
+ * 1 (default) the same, loading the mask and stage operands of each
+ * thread's first point before the halo-tile barrier; 2, 3, 4 a persistent
+ * kernel with 2, 4, 8 workgroups per compute unit that loads the next
+ * block's halo tile while computing the current one (where the tile fits).
  * Every variant computes the same values bit for bit. */
 enum { WH_TUNE_K3D_YSPLIT = 1, WH_TUNE_K2D_STAGE = 2 };
 int wh_set_tuning(wh_pool *pool, int key, int value);

@@ -249,13 +249,13 @@ def test_3d_kernel_variants_identical(scheme, ysplit):
 
 @pytest.mark.parametrize("scheme", SCHEMES)
 @pytest.mark.parametrize("path", [PATH_FUSED, PATH_REFERENCE])
-def test_2d_kernel_staging_variants_identical(scheme, path):
-    """The 2D stage kernel with the mask and stage operands staged in LDS
-    (WH_TUNE_K2D_STAGE 1, where they fit) gives the same state bit for bit
-    as with direct reads (0), penalised + sponge, one RK4 step."""
+def test_2d_kernel_variants_identical(scheme, path):
+    """Every 2D stage-kernel variant (WH_TUNE_K2D_STAGE 0..4: plain, early
+    loads, persistent pipelined with 2/4/8 workgroups per CU) gives the same
+    state bit for bit, penalised + sponge, one RK4 step."""
     forest, acm, hvy, mask = _setup(2, scheme, False, True, True)
     out = {}
-    for stage in (0, 1):
+    for stage in (0, 1, 2, 3, 4):
         solver = AcmSolver(forest, acm, TimeParams(time_max=10.0), REACH[scheme], path=path, mask=mask,
                            n_mask=6)
         L.check(L.load().wh_set_tuning(solver.pool.h, L.WH_TUNE_K2D_STAGE, stage))
@@ -263,4 +263,5 @@ def test_2d_kernel_staging_variants_identical(scheme, path):
         solver.step()
         out[stage] = solver.state()
         solver.pool.close()
-    assert np.array_equal(out[0], out[1])
+    for stage in (1, 2, 3, 4):
+        assert np.array_equal(out[0], out[stage]), stage

@@ -335,6 +335,13 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking)) != hipSuccess)
         return fail("hipStreamCreate", e);
+    {
+        int dev = 0, ncu = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return fail("hipGetDevice", e);
+        if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+            return fail("hipDeviceGetAttribute", e);
+        P->n_cu = ncu > 0 ? ncu : 256;
+    }
     if ((e = hipMalloc(&P->block, sizeof(double) * P->blk * d.nb_max)) != hipSuccess)
         return fail("hipMalloc(hvy_block)", e);
     if (d.n_work_slots > 0 &&
@@ -850,7 +857,7 @@ int wh_set_tuning(wh_pool *P, int key, int value)
     WH_CHECK(P, WH_E_ARG, "null argument");
     WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE, WH_E_ARG, "unknown tuning key");
     if (key == WH_TUNE_K2D_STAGE) {
-        WH_CHECK(value == 0 || value == 1, WH_E_ARG, "K2D_STAGE is 0 or 1");
+        WH_CHECK(value >= 0 && value <= 4, WH_E_ARG, "K2D_STAGE is 0..4");
         P->k2d_stage = value;
         P->topo_version++;
         return WH_OK;
@@ -889,7 +896,7 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
         for (int ys = 1; ys <= 8; ++ys)
             if (zm_valid(P->d.bs[0], P->d.bs[1], p->scheme, ys)) cand.push_back(ys);
     } else {
-        cand = {0, 1};
+        cand = {0, 1, 2, 3, 4};
     }
     int &knob = three ? P->k3d_ysplit : P->k2d_stage;
     const AcmDev a = make_acm_dev(p);

@@ -138,8 +138,11 @@ struct wh_pool {
     wh::TimeDev graph_tc{};
     // tuning: y-split of the 3D stage kernel (-1 auto, 0 flat kernel)
     int k3d_ysplit = -1;
-    // tuning: 2D kernel loads mask and epilogue operands before the tile barrier (1) or after (0)
+    // tuning of the 2D stage kernel: 0 plain, 1 mask and epilogue operands
+    // loaded before the tile barrier, 2..4 persistent pipelined kernel with
+    // 2, 4, 8 workgroups per CU
     int k2d_stage = 1;
+    int n_cu = 256;  // compute units of the pool's device
     // wavelet transforms / adapt_tree
     wh::WaveletWS ws;
     // block groups (independent forests in one pool): group of each active

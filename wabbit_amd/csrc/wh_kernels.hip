@@ -303,6 +303,83 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
     }
 }
 
+// 2D, persistent and software-pipelined: workgroup w handles the blocks at
+// active positions w, w + G, w + 2G, ... (G workgroups); the halo tile of its
+// next block is loaded into registers while the current block is computed
+// from LDS, then stored to the other of two LDS buffers.  Per point the
+// arithmetic is that of k_rhs2d.
+constexpr int PIPE_MAXL = 6;  // tile elements per thread held in registers
+template <int S, bool SKEW, bool FUSED>
+__global__ __launch_bounds__(256) void k_rhs2d_pipe(const double *__restrict__ in, double *__restrict__ out,
+                                                    const double *__restrict__ mask,
+                                                    const int32_t *__restrict__ act,
+                                                    const double *__restrict__ x0dx, Geo G, AcmDev a,
+                                                    StageArgs sa, int nact)
+{
+    constexpr int R = Reach<S>::R;
+    extern __shared__ double sm[];
+    const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R, tcs = tx * ty, ntile = 3 * tcs;
+    const int npts = G.bs0 * G.bs1;
+    double reg[PIPE_MAXL];
+    int off[PIPE_MAXL];  // block-independent source offsets of this thread's tile elements
+#pragma unroll
+    for (int j = 0; j < PIPE_MAXL; ++j) {
+        const int i = threadIdx.x + j * blockDim.x;
+        const int c = i / tcs, rr = i - c * tcs;
+        const int iy = rr / tx, ix = rr - iy * tx;
+        off[j] = i < ntile ? (int)(c * G.cs + (size_t)(G.g - R + iy) * G.nx + (G.g - R + ix)) : 0;
+    }
+    int k = blockIdx.x;
+    if (k >= nact) return;  // uniform over the workgroup
+    auto load = [&](int kk) {
+        const double *u = in + (size_t)act[kk] * G.blk;
+#pragma unroll
+        for (int j = 0; j < PIPE_MAXL; ++j)
+            if (threadIdx.x + j * blockDim.x < (unsigned)ntile) reg[j] = u[off[j]];
+    };
+    auto store = [&](double *dst) {
+#pragma unroll
+        for (int j = 0; j < PIPE_MAXL; ++j) {
+            const int i = threadIdx.x + j * blockDim.x;
+            if (i < ntile) dst[i] = reg[j];
+        }
+    };
+    load(k);
+    store(sm);
+    __syncthreads();
+    int cur = 0;
+    while (true) {
+        const int kn = k + (int)gridDim.x;
+        if (kn < nact) load(kn);  // in flight while block k is computed
+        const double *tile = sm + cur * ntile;
+        const int b = act[k];
+        const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4];
+        const double inv[2] = {1.0 / dx0, 1.0 / dx1};
+        const double inv2[2] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1)};
+        const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
+        for (int pnt = threadIdx.x; pnt < npts; pnt += blockDim.x) {
+            const int iy = pnt / G.bs0, ix = pnt - iy * G.bs0;
+            const double *uc = tile + (iy + R) * tx + (ix + R);
+            const size_t gi = (size_t)(G.g + iy) * G.nx + (G.g + ix);
+            double r[3];
+            const StridedAcc<2> A{uc, tcs, {1, tx, 0}};
+            acm_point<2, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
+            const size_t o = (size_t)b * G.blk + gi;
+            if constexpr (FUSED) {
+                stage_epilogue<3>(sa, k, o, G.cs, r, uc, tcs);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) out[o + c * G.cs] = r[c];
+            }
+        }
+        if (kn >= nact) break;  // uniform
+        store(sm + (cur ^ 1) * ntile);
+        __syncthreads();
+        cur ^= 1;
+        k = kn;
+    }
+}
+
 // 3D: 256 interior points per workgroup, stencil reads served by L1/L2.
 template <int S, bool SKEW, bool FUSED>
 __global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, double *__restrict__ out,
@@ -505,7 +582,31 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
         int R = a.scheme == WH_FD2 ? 1 : a.scheme == WH_FD4 ? 2 : 3;
         size_t lds = sizeof(double) * 3 * (G.bs0 + 2 * R) * (G.bs1 + 2 * R);
         WH_CHECK(lds <= 160 * 1024, WH_E_ARG, "2D block tile exceeds LDS");
-        const int pre = P->k2d_stage != 0;
+        const int pre = P->k2d_stage == 1;
+        // the persistent pipelined variant: two tiles in LDS, each thread
+        // holds at most PIPE_MAXL tile elements of the next block
+        const int wpc = P->k2d_stage == 2 ? 2 : P->k2d_stage == 3 ? 4 : P->k2d_stage == 4 ? 8 : 0;
+        if (wpc && 3 * (G.bs0 + 2 * R) * (G.bs1 + 2 * R) <= PIPE_MAXL * 256 && 2 * lds <= 160 * 1024) {
+            const int nwg = std::min(nact, wpc * P->n_cu);
+            dim3 pgrid(nwg), pblock(256);
+#define L2P(SC, SK)                                                                                 \
+    hipLaunchKernelGGL((k_rhs2d_pipe<SC, SK, FUSED>), pgrid, pblock, 2 * lds, P->stream, in, out, mask, \
+                       act, x0dx, G, a, sa, nact)
+            switch (a.scheme * 2 + a.skew) {
+            case WH_FD2 * 2: L2P(WH_FD2, false); break;
+            case WH_FD2 * 2 + 1: L2P(WH_FD2, true); break;
+            case WH_FD4 * 2: L2P(WH_FD4, false); break;
+            case WH_FD4 * 2 + 1: L2P(WH_FD4, true); break;
+            case WH_FD4_TW * 2: L2P(WH_FD4_TW, false); break;
+            case WH_FD4_TW * 2 + 1: L2P(WH_FD4_TW, true); break;
+            case WH_FD6 * 2: L2P(WH_FD6, false); break;
+            case WH_FD6 * 2 + 1: L2P(WH_FD6, true); break;
+            default: set_error("unknown scheme"); return WH_E_ARG;
+            }
+#undef L2P
+            WH_HIP(hipGetLastError());
+            return WH_OK;
+        }
         dim3 grid(nact), blockd(256);
 #define L2D(SC, SK)                                                                              \
     do {                                                                                         \
