@@ -52,3 +52,41 @@ def test_refine_bookkeeping():
     assert len(set(hv.tolist())) == f.n
     assert not set(tri[:, 1].tolist()) & {0, 2}
     f.close()
+
+
+def test_slot_base_and_errors():
+    """A forest restricted to a slot range of a shared pool: ids outside the
+    range, duplicates and levels above Jmax are rejected; refinement takes
+    only slots of the range and reports running out of them."""
+    from wabbit_amd import lib as L
+    f = Forest(2, 4, [8, 8], 3, [1.0, 1.0], nb_max=8, slot_base=100)
+    tcs, lev = random_graded_forest(2, 1, 1, 0, 0)           # 4 blocks on level 1
+    with pytest.raises(L.WabbitHipError):
+        f.set_blocks(tcs, lev, np.arange(4, dtype=np.int32))      # below the range
+    with pytest.raises(L.WabbitHipError):
+        f.set_blocks(tcs, lev, np.array([100, 101, 101, 102], np.int32))  # duplicate
+    with pytest.raises(L.WabbitHipError):
+        f.set_blocks(tcs, np.full(4, 5, np.int32), np.arange(100, 104, dtype=np.int32))  # level > jmax
+    f.set_blocks(tcs, lev, np.arange(100, 104, dtype=np.int32))
+    tri = f.refine(np.array([100], np.int32))
+    assert set(tri[:, 1].tolist()) <= set(range(100, 108))
+    with pytest.raises(L.WabbitHipError):
+        f.refine(np.array([101, 102], np.int32))                  # 8 more slots needed, 1 free + 1 mother
+    # the failed call changed nothing
+    assert f.n == 3 + 4
+    # the slot base can only change while the forest is empty
+    with pytest.raises(L.WabbitHipError):
+        L.check(L.load().wh_forest_set_slot_base(f.h, 0))
+    f.close()
+
+
+def test_empty_and_single_block_forests():
+    """An empty forest has no patches; a single periodic block is its own
+    neighbour in every direction (same-level copies only)."""
+    f = Forest(2, 3, [8, 8], 3, [1.0, 1.0], nb_max=4)
+    f.set_blocks(np.zeros(0, np.int64), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    assert f.n == 0 and len(f.patches(3, 3, 4)) == 0
+    f.set_blocks(np.zeros(1, np.int64), np.zeros(1, np.int32), np.zeros(1, np.int32))
+    pt = f.patches(3, 3, 4)
+    assert len(pt) == 8 and np.all(pt["op"] == 0) and np.all(pt["send"] == 0) and np.all(pt["recv"] == 0)
+    f.close()

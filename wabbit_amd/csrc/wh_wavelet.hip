@@ -271,6 +271,55 @@ __global__ __launch_bounds__(256) void k_wpass(const double *__restrict__ in,
         chain(T, par, in + (size_t)b * A.blk_in + c * A.cs + q, astride);
 }
 
+// 2D: both separable passes of one component of one block in one workgroup.
+// The block's source and destination planes are staged in LDS; the x pass
+// writes Rx of the destination from the source, the y pass computes Ry from
+// the destination into the (no longer needed) source buffer, which is copied
+// back; the whole destination plane is written out, so everything outside
+// Rx and Ry keeps its previous value (init_copy: the source's) -- the values
+// of the two-launch version bit for bit.
+__global__ __launch_bounds__(256) void k_sep2d(const double *__restrict__ in, double *__restrict__ out,
+                                               const int32_t *__restrict__ ids, ArrGeo A, Region Rx,
+                                               Region Ry, int g, int init_copy, TermTable T)
+{
+    extern __shared__ double sp[];
+    const int nxy = A.nx * A.ny;
+    double *S = sp, *D = sp + nxy;
+    const int k = blockIdx.x / A.neq, c = blockIdx.x - k * A.neq;
+    const int b = ids[k];
+    const double *src = in + (size_t)b * A.blk_in + (size_t)c * A.cs;
+    double *dst = out + (size_t)b * A.blk_out + (size_t)c * A.cs;
+    for (int i = threadIdx.x; i < nxy; i += blockDim.x) {
+        const double v = src[i];
+        S[i] = v;
+        D[i] = init_copy ? v : dst[i];
+    }
+    __syncthreads();
+    const int nx_ = Rx.n[0] * Rx.n[1];
+    for (int e = threadIdx.x; e < nx_; e += blockDim.x) {
+        const int i1 = e / Rx.n[0], i0 = e - i1 * Rx.n[0];
+        const int p0 = Rx.lo[0] + Rx.step[0] * i0, p1 = Rx.lo[1] + Rx.step[1] * i1;
+        const int q = p1 * A.nx + p0;
+        D[q] = chain(T, (p0 - g) & 1, S + q, 1);
+    }
+    __syncthreads();
+    const int ny_ = Ry.n[0] * Ry.n[1];
+    for (int e = threadIdx.x; e < ny_; e += blockDim.x) {
+        const int i1 = e / Ry.n[0], i0 = e - i1 * Ry.n[0];
+        const int p0 = Ry.lo[0] + Ry.step[0] * i0, p1 = Ry.lo[1] + Ry.step[1] * i1;
+        const int q = p1 * A.nx + p0;
+        S[q] = chain(T, (p1 - g) & 1, D + q, A.nx);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ny_; e += blockDim.x) {
+        const int i1 = e / Ry.n[0], i0 = e - i1 * Ry.n[0];
+        const int q = (Ry.lo[1] + Ry.step[1] * i1) * A.nx + (Ry.lo[0] + Ry.step[0] * i0);
+        D[q] = S[q];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nxy; i += blockDim.x) dst[i] = D[i];
+}
+
 __global__ __launch_bounds__(256) void k_copy_region(const double *__restrict__ in,
                                                      double *__restrict__ out,
                                                      const int32_t *__restrict__ ids, int nb,
@@ -445,19 +494,37 @@ Region full_region(const wh_pool *P)
     return Region{{0, 0, 0}, {P->nx, P->ny, P->nz}, {1, 1, 1}};
 }
 
+// the fused 2D pair of passes, where a block plane pair fits in 64 KB of LDS
+bool sep2d_ok(const wh_pool *P) { return P->d.dim == 2 && 2 * sizeof(double) * P->nx * P->ny <= 64 * 1024; }
+
+int launch_sep2d(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
+                 const Region &Rx, const Region &Ry, int init_copy, const TermTable &T)
+{
+    if (nb <= 0) return WH_OK;
+    const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
+    const size_t lds = 2 * sizeof(double) * P->nx * P->ny;
+    hipLaunchKernelGGL(k_sep2d, dim3((unsigned)(nb * P->d.neq)), dim3(256), lds, P->stream, in, out,
+                       P->ws.cur_ids, A, Rx, Ry, P->d.g, init_copy, T);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
 // Three separable passes (x over an extended row range, then y, then z) of
 // the chains T: x reads `in` and writes `out`; y and z read `out` and go
 // through work slot 0 (in-place along the pass direction).
 int separable(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
-              const TermTable &T, int ext)
+              const TermTable &T, int ext, bool init_copy = false)
 {
     const int g = P->d.g, dim = P->d.dim;
     const int B0 = P->d.bs[0], B1 = P->d.bs[1], B2 = dim == 3 ? P->d.bs[2] : 1;
     const int zlo = dim == 3 ? g - ext : 0, zn = dim == 3 ? B2 + 2 * ext : 1;
     double *scratch = P->work;  // slot 0
     Region Rx{{g, g - ext, zlo}, {B0, B1 + 2 * ext, zn}, {1, 1, 1}};
-    if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
     Region Ry{{g, g, zlo}, {B0, B1, zn}, {1, 1, 1}};
+    if (sep2d_ok(P)) return launch_sep2d(P, in, blk_in, out, blk_out, nb, Rx, Ry, init_copy ? 1 : 0, T);
+    if (init_copy)
+        if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
+    if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
     if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Ry, 1, g, T)) return rc;
     if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Ry)) return rc;
     if (dim == 3) {
@@ -475,16 +542,18 @@ int filter_passes(wh_pool *P, const double *in, size_t blk_in, double *out, size
     const wh_wavelet &w = P->ws.w;
     const int g = P->d.g, dim = P->d.dim;
     const int B0 = P->d.bs[0], B1 = P->d.bs[1], B2 = dim == 3 ? P->d.bs[2] : 1;
-    if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
-    if (w.hd_lo == 0 && w.hd_hi == 0 && std::fabs(w.HD[0] - 1.0) <= 1.0e-10) return WH_OK;
+    const bool trivial = w.hd_lo == 0 && w.hd_hi == 0 && std::fabs(w.HD[0] - 1.0) <= 1.0e-10;
     const TermTable T = filter_terms(w);
     const int lo = w.hd_lo, hi = w.hd_hi;
     const int zlo = dim == 3 ? g + lo : 0, zn = dim == 3 ? B2 - lo + hi : 1;
     double *scratch = P->work;
     // the parity of every filtered point is even, so chain 0 applies
     Region Rx{{g, g + lo, zlo}, {B0 / 2, B1 - lo + hi, zn}, {2, 1, 1}};
-    if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
     Region Ry{{g, g, zlo}, {B0 / 2, B1 / 2, zn}, {2, 2, 1}};
+    if (!trivial && sep2d_ok(P)) return launch_sep2d(P, in, blk_in, out, blk_out, nb, Rx, Ry, 1, T);
+    if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
+    if (trivial) return WH_OK;
+    if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
     if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Ry, 1, g, T)) return rc;
     if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Ry)) return rc;
     if (dim == 3) {
@@ -583,11 +652,8 @@ int run_blocks_op(wh_pool *P, int op, int src_array, int dst_array, const std::v
     if (op == WOP_COPY) return launch_copy_region(P, in, blk_in, out, blk_out, n, full_region(P));
     if (op == WOP_FWT)
         return separable(P, in, blk_in, out, blk_out, n, fwt_terms(w), std::max(w.hd_hi, w.gd_hi));
-    if (op == WOP_IWT) {
-        if (!w.lifted)  // CDFX0 copies the whole block first
-            if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, n, full_region(P))) return rc;
-        return separable(P, in, blk_in, out, blk_out, n, iwt_terms(w), std::max(w.hr_hi, w.gr_hi));
-    }
+    if (op == WOP_IWT)  // CDFX0 starts from a copy of the whole block
+        return separable(P, in, blk_in, out, blk_out, n, iwt_terms(w), std::max(w.hr_hi, w.gr_hi), !w.lifted);
     if (op == WOP_FILTER) return filter_passes(P, in, blk_in, out, blk_out, n);
     set_error("wavelet op: unknown op");
     return WH_E_ARG;
