@@ -132,7 +132,7 @@ struct Tree {
         int wq[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d) {
             if (!F->periodic[d] && (q[d] < 0 || q[d] >= m)) return -1;
-            wq[d] = ((q[d] % m) + m) % m;
+            wq[d] = q[d] & (m - 1);  // periodic wrap; m is a power of two
         }
         if (level < (int)dense.size() && !dense[level].empty()) return dense[level][lin(level, wq)];
         auto it = map.find(qkey(level, wq));
@@ -171,86 +171,95 @@ struct Tree {
     }
 
     // ---- relations, cached per block set ----
-    // Tables built on first use after the block set changed (add, removal):
-    // per block and direction the same-level neighbour, the coarse relation,
+    // Per block, built on its first use after the block set changed (add,
+    // removal): per direction the same-level neighbour, the coarse relation,
     // the coarser block covering the ghost box (floor((q+o)/2) one level up)
     // and the finer blocks adjacent in that direction; mother and daughters.
     int nd = 0;
     int dirs[26][3];
     mutable bool topo_ok = false;
-    mutable std::vector<int> t_same, t_coarse, t_cover, t_mom, t_dau, t_fin_off;
-    mutable std::vector<std::pair<int, std::array<int, 3>>> t_fin;
+    mutable std::vector<char> t_done;
+    mutable std::vector<int> t_same, t_coarse, t_cover, t_mom, t_dau, t_fcnt;
+    mutable std::vector<std::pair<int, std::array<int, 3>>> t_fin;  // maxc slots per block and direction
     int dir_index(const int o[3]) const
     {
         const int t = dim == 3 ? (o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1) : (o[1] + 1) * 3 + (o[0] + 1);
         const int c = dim == 3 ? 13 : 4;
         return t > c ? t - 1 : t;
     }
+    int maxc() const { return 1 << (dim - 1); }
     void topo() const
     {
         if (topo_ok) return;
-        const int n = (int)b.size(), ns = 1 << dim;
-        t_same.assign((size_t)n * nd, -1);
-        t_coarse.assign((size_t)n * nd, -1);
-        t_cover.assign((size_t)n * nd, -1);
-        t_mom.assign(n, -1);
-        t_dau.assign((size_t)n * ns, -1);
-        t_fin_off.assign((size_t)n * nd + 1, 0);
-        t_fin.clear();
-        for (int i = 0; i < n; ++i) {
-            const int *q = b[i].q, L = b[i].level;
-            t_mom[i] = mother_raw(i);
-            for (int k = 0; k < ns; ++k) t_dau[(size_t)i * ns + k] = daughter_raw(i, k);
-            for (int k = 0; k < nd; ++k) {
-                const int *o = dirs[k];
-                const size_t e = (size_t)i * nd + k;
-                const int nq[3] = {q[0] + o[0], q[1] + o[1], q[2] + o[2]};
-                t_same[e] = find(L, nq);
-                t_coarse[e] = nb_coarse_raw(i, o);
-                if (L > 0) {
-                    int cu[3] = {0, 0, 0};
-                    for (int d = 0; d < dim; ++d) {
-                        const int v = q[d] + o[d];
-                        cu[d] = v >= 0 ? v / 2 : -((-v + 1) / 2);
-                    }
-                    t_cover[e] = find(L - 1, cu);
-                }
-                int lo[3], hi[3];
-                for (int d = 0; d < 3; ++d) {
-                    if (d >= dim) lo[d] = hi[d] = 0;
-                    else if (o[d] < 0) lo[d] = hi[d] = 1;
-                    else if (o[d] > 0) lo[d] = hi[d] = 0;
-                    else {
-                        lo[d] = 0;
-                        hi[d] = 1;
-                    }
-                }
-                for (int cz = lo[2]; cz <= hi[2]; ++cz)
-                    for (int cy = lo[1]; cy <= hi[1]; ++cy)
-                        for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-                            const int c[3] = {2 * (q[0] + o[0]) + cx, 2 * (q[1] + o[1]) + cy,
-                                              dim == 3 ? 2 * (q[2] + o[2]) + cz : 0};
-                            t_fin.push_back({find(L + 1, c), {cx, cy, cz}});
-                        }
-                t_fin_off[e + 1] = (int)t_fin.size();
-            }
-        }
+        const size_t n = b.size();
+        t_done.assign(n, 0);
+        t_same.resize(n * nd);
+        t_coarse.resize(n * nd);
+        t_cover.resize(n * nd);
+        t_mom.resize(n);
+        t_dau.resize(n << dim);
+        t_fcnt.resize(n * nd);
+        t_fin.resize(n * nd * maxc());
         topo_ok = true;
+    }
+    void node_topo(int i) const
+    {
+        topo();
+        if (t_done[i]) return;
+        const int *q = b[i].q, L = b[i].level, ns = 1 << dim, mc = maxc();
+        t_mom[i] = mother_raw(i);
+        for (int k = 0; k < ns; ++k) t_dau[(size_t)i * ns + k] = daughter_raw(i, k);
+        for (int k = 0; k < nd; ++k) {
+            const int *o = dirs[k];
+            const size_t e = (size_t)i * nd + k;
+            const int nq[3] = {q[0] + o[0], q[1] + o[1], q[2] + o[2]};
+            t_same[e] = find(L, nq);
+            t_coarse[e] = nb_coarse_raw(i, o);
+            t_cover[e] = -1;
+            if (L > 0) {
+                int cu[3] = {0, 0, 0};
+                for (int d = 0; d < dim; ++d) {
+                    const int v = q[d] + o[d];
+                    cu[d] = v >= 0 ? v / 2 : -((-v + 1) / 2);
+                }
+                t_cover[e] = find(L - 1, cu);
+            }
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+                if (d >= dim) lo[d] = hi[d] = 0;
+                else if (o[d] < 0) lo[d] = hi[d] = 1;
+                else if (o[d] > 0) lo[d] = hi[d] = 0;
+                else {
+                    lo[d] = 0;
+                    hi[d] = 1;
+                }
+            }
+            int cnt = 0;
+            for (int cz = lo[2]; cz <= hi[2]; ++cz)
+                for (int cy = lo[1]; cy <= hi[1]; ++cy)
+                    for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+                        const int c[3] = {2 * (q[0] + o[0]) + cx, 2 * (q[1] + o[1]) + cy,
+                                          dim == 3 ? 2 * (q[2] + o[2]) + cz : 0};
+                        t_fin[e * mc + cnt++] = {find(L + 1, c), {cx, cy, cz}};
+                    }
+            t_fcnt[e] = cnt;
+        }
+        t_done[i] = 1;
     }
     int mother(int i) const
     {
-        topo();
+        node_topo(i);
         return t_mom[i];
     }
     int daughter(int i, int k) const
     {
-        topo();
+        node_topo(i);
         return t_dau[((size_t)i << dim) + k];
     }
     // block_is_leaf (block_relations.f90); check_empty: REF_TMP_EMPTY daughters don't count
     bool is_leaf(int i, bool check_empty) const
     {
-        topo();
+        node_topo(i);
         const int ns = 1 << dim;
         for (int k = 0; k < ns; ++k) {
             const int d = t_dau[(size_t)i * ns + k];
@@ -260,36 +269,41 @@ struct Tree {
     }
     int nb_same(int i, const int o[3]) const
     {
-        topo();
+        node_topo(i);
         return t_same[(size_t)i * nd + dir_index(o)];
     }
     // the coarser block covering the ghost box of block i in direction o
     int cover(int i, const int o[3]) const
     {
-        topo();
+        node_topo(i);
         return t_cover[(size_t)i * nd + dir_index(o)];
+    }
+    // the finer blocks of block i in direction index k, as a range of the table
+    std::pair<const std::pair<int, std::array<int, 3>> *, const std::pair<int, std::array<int, 3>> *>
+    fin_range(int i, int k) const
+    {
+        node_topo(i);
+        const size_t e = (size_t)i * nd + k;
+        const auto *p = t_fin.data() + e * maxc();
+        return {p, p + t_fcnt[e]};
     }
     // the finer blocks adjacent to block i in direction o (children of the
     // virtual same-level neighbour); child cc per dimension in out
     void finer_children(int i, const int o[3], std::vector<std::pair<int, std::array<int, 3>>> &out) const
     {
-        topo();
-        const size_t e = (size_t)i * nd + dir_index(o);
-        out.assign(t_fin.begin() + t_fin_off[e], t_fin.begin() + t_fin_off[e + 1]);
+        const auto r = fin_range(i, dir_index(o));
+        out.assign(r.first, r.second);
     }
     bool any_finer(int i, const int o[3], bool need_valid) const
     {
-        topo();
-        const size_t e = (size_t)i * nd + dir_index(o);
-        for (int j = t_fin_off[e]; j < t_fin_off[e + 1]; ++j) {
-            const int f = t_fin[j].first;
-            if (f >= 0 && (!need_valid || valid(f))) return true;
-        }
+        const auto r = fin_range(i, dir_index(o));
+        for (auto *p = r.first; p != r.second; ++p)
+            if (p->first >= 0 && (!need_valid || valid(p->first))) return true;
         return false;
     }
     int nb_coarse(int i, const int o[3]) const
     {
-        topo();
+        node_topo(i);
         return t_coarse[(size_t)i * nd + dir_index(o)];
     }
     int add(int64_t tc, int level, int status)
@@ -419,8 +433,10 @@ int tree_sync(Tree &T, const SyncSpec &S)
     const int nxyz[3] = {T.P->nx, T.P->ny, T.P->nz};
     std::vector<wh_patch> st1, st23;
     std::vector<int> restrict_senders;  // tree indices
-    std::vector<std::pair<int, std::array<int, 3>>> ch;
     std::unique_ptr<ProfScope> gen(new ProfScope("tree_sync.gen"));
+    T.topo();
+    st1.reserve(T.b.size() * nd);
+    st23.reserve(T.b.size() * 2);
     for (int r = 0; r < (int)T.b.size(); ++r) {
         const TNode &R = T.b[r];
         if (R.status == REF_TMP_EMPTY) continue;
@@ -465,12 +481,14 @@ int tree_sync(Tree &T, const SyncSpec &S)
                 continue;
             }
             // 2) finer blocks touching the box
-            T.finer_children(r, o, ch);
-            bool finer = !ch.empty();
-            for (auto &c : ch) finer = finer && T.valid(c.first);
+            const auto fr = T.fin_range(r, k);
+            const auto *ch0 = fr.first, *ch1 = fr.second;
+            bool finer = ch1 != ch0;
+            for (auto *cp = ch0; cp != ch1; ++cp) finer = finer && T.valid(cp->first);
             if (finer) {
                 if (!S.restrict_) continue;
-                for (auto &c : ch) {
+                for (auto *cp = ch0; cp != ch1; ++cp) {
+                    const auto &c = *cp;
                     const int f = c.first;
                     wh_patch p{};
                     p.recv = R.hvy;
@@ -687,7 +705,6 @@ void ensure_gradedness(Tree &T, int stay)
     int dirs[26][3];
     const int nd = dirs_of(T.dim, dirs);
     const int nsis = 1 << T.dim;
-    std::vector<std::pair<int, std::array<int, 3>>> ch;
     bool changed = true;
     int counter = 0;
     while (changed && counter < 10 * T.jmax + 10) {
@@ -741,9 +758,9 @@ void ensure_gradedness(Tree &T, int stay)
                         T.b[i].status = stay;
                         changed = true;
                     }
-                    T.finer_children(i, o, ch);
-                    for (auto &c : ch) {
-                        const int f = c.first;
+                    const auto fr = T.fin_range(i, q);
+                    for (auto *cp = fr.first; cp != fr.second; ++cp) {
+                        const int f = cp->first;
                         if (f < 0) continue;
                         if (T.b[f].status == 1) {
                             T.b[i].status = 1;
@@ -756,9 +773,9 @@ void ensure_gradedness(Tree &T, int stay)
                 }
             } else if (me == 0 || me == stay) {
                 for (int q = 0; q < nd; ++q) {
-                    T.finer_children(i, dirs[q], ch);
-                    for (auto &c : ch)
-                        if (c.first >= 0 && T.b[c.first].status == 1) {
+                    const auto fr = T.fin_range(i, q);
+                    for (auto *cp = fr.first; cp != fr.second; ++cp)
+                        if (cp->first >= 0 && T.b[cp->first].status == 1) {
                             T.b[i].status = 1;
                             changed = true;
                         }

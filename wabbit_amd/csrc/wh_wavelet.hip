@@ -786,9 +786,16 @@ int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<
         S.alt[a] = array_ptr(P, alts[a], &S.blk_alt[a], &nca);
         WH_CHECK(S.alt[a] && nca >= P->d.neq, WH_E_ARG, "patch list: bad alternate array");
     }
-    // validate and order by stage
+    // validate and order by stage (lists that are already ordered are used as they are)
     const int lim[3] = {P->nx, P->ny, P->nz};
-    std::vector<wh_patch> p(patches_in);
+    const auto by_stage = [](const wh_patch &a, const wh_patch &b) { return a.stage < b.stage; };
+    std::vector<wh_patch> sorted_copy;
+    const bool ordered = std::is_sorted(patches_in.begin(), patches_in.end(), by_stage);
+    if (!ordered) {
+        sorted_copy = patches_in;
+        std::stable_sort(sorted_copy.begin(), sorted_copy.end(), by_stage);
+    }
+    const std::vector<wh_patch> &p = ordered ? patches_in : sorted_copy;
     for (const wh_patch &q : p) {
         const int op = q.op & 0xff, sel = (q.op >> 8) & 3;
         WH_CHECK(q.recv >= 0 && q.recv < P->d.nb_max && q.send >= 0 && q.send < P->d.nb_max, WH_E_ARG,
@@ -816,7 +823,6 @@ int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<
             }
         }
     }
-    std::stable_sort(p.begin(), p.end(), [](const wh_patch &a, const wh_patch &b) { return a.stage < b.stage; });
     std::vector<int64_t> pstart(n + 1, 0);
     for (int i = 0; i < n; ++i) pstart[i + 1] = pstart[i] + (int64_t)p[i].n[0] * p[i].n[1] * p[i].n[2];
     if (int rc = stage_begin(P, sizeof(wh_patch) * n + sizeof(int64_t) * (n + 1))) return rc;
