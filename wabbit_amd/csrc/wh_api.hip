@@ -537,10 +537,15 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
     if (int rc = wait_tables(P)) return rc;  // nothing queued reads the old table any more
     PatchSet &ps = P->ps[sync_case];
     ps.max_stage = 0;
-    ps.host.assign(patches, patches + npatch);
-    std::stable_sort(ps.host.begin(), ps.host.end(), [](const wh_patch &a, const wh_patch &b) {
-        return a.stage != b.stage ? a.stage < b.stage : a.recv < b.recv;
-    });
+    // stable counting sort by stage (1..3); the receivers keep their order
+    // within a stage (rows of the per-receiver table are runs of one receiver)
+    {
+        int cnt[5] = {0, 0, 0, 0, 0};
+        for (int i = 0; i < npatch; ++i) ++cnt[patches[i].stage + 1];
+        for (int st = 1; st < 5; ++st) cnt[st] += cnt[st - 1];
+        ps.host.resize(npatch);
+        for (int i = 0; i < npatch; ++i) ps.host[cnt[patches[i].stage]++] = patches[i];
+    }
     ps.h_recv_off.clear();
     ps.h_recv_ids.clear();
     ps.h_pstart.assign(npatch + 1, 0);
