@@ -757,14 +757,24 @@ __global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
                                                    int p1, size_t blk, size_t cs, int nx, int ny,
                                                    int neq, SyncSrc S)
 {
-    const int64_t e = pstart[p0] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= pstart[p1]) return;
-    int lo = p0, hi = p1 - 1;
-    while (lo < hi) {  // last patch with pstart <= e
-        const int mid = (lo + hi + 1) >> 1;
-        if (pstart[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
+    // last patch in [lo, hi] with pstart <= x
+    auto find = [&](int64_t x, int lo, int hi) {
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pstart[mid] <= x) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    // the workgroup's first and last points and their patches (uniform),
+    // then each lane's patch between those two
+    const int64_t w0 = pstart[p0] + (int64_t)blockIdx.x * blockDim.x;
+    const int64_t w1 = min(w0 + (int64_t)blockDim.x, pstart[p1]) - 1;
+    const int pa = find(w0, p0, p1 - 1);
+    const int pb = find(w1, pa, p1 - 1);
+    const int64_t e = w0 + threadIdx.x;
+    if (e > w1) return;
+    const int lo = find(e, pa, pb);
     const wh_patch P = pt[lo];
     const int r = (int)(e - pstart[lo]);
     const int n0 = P.n[0], n1 = P.n[1];
@@ -858,9 +868,97 @@ __global__ __launch_bounds__(256) void k_refine(double *__restrict__ arr,
     for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(src + c * cs, nx, ny, dim, f, order);
 }
 
+// 3D refineBlock, one thread per (x, y) column of a daughter component,
+// marching in z: the x-then-y interpolated values of the mother's coarse
+// planes (pred_point's vy) pass through a register window of 2N planes, and
+// each fine z point is either the window's centre plane (even index) or the
+// left-to-right z sum over the window (odd index) -- pred_point's arithmetic
+// in pred_point's order, each vy computed once per column instead of once
+// per output point.  Window planes outside the mother are set to zero; the
+// host's stencil-range check guarantees no output reads them.
+template <int N>
+__global__ __launch_bounds__(256) void k_refine3d(double *__restrict__ arr,
+                                                  const int32_t *__restrict__ pairs, int npairs,
+                                                  int wpc, int bs0, int bs1, int bs2, int g, int nx,
+                                                  int ny, int nz, size_t blk, size_t cs, int neq)
+{
+    constexpr double c2[2] = {0.5, 0.5};
+    constexpr double c4[4] = {-1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0};
+    constexpr double c6[6] = {3.0 / 256.0, -25.0 / 256.0, 150.0 / 256.0,
+                              150.0 / 256.0, -25.0 / 256.0, 3.0 / 256.0};
+    const double *cf = (N == 1) ? c2 : (N == 2) ? c4 : c6;
+    // `wpc` workgroups per (pair, component): the pair and component are a
+    // scalar division of the workgroup index
+    const int ncol = nx * ny;
+    const int qc = (int)blockIdx.x / wpc;
+    const int q = qc / neq, c = qc - q * neq;
+    const int r = ((int)blockIdx.x - qc * wpc) * (int)blockDim.x + (int)threadIdx.x;
+    if (r >= ncol) return;
+    const int j = r / nx, i = r - j * nx;
+    const int mother = pairs[3 * q], daughter = pairs[3 * q + 1], dg = pairs[3 * q + 2];
+    const int f0 = g + bs0 * ((dg >> 1) & 1) + i, f1 = g + bs1 * (dg & 1) + j;
+    const int fz0 = g + bs2 * ((dg >> 2) & 1);
+    const int cx = f0 >> 1, px = f0 & 1, cy = f1 >> 1, py = f1 & 1;
+    const double *u = arr + (size_t)mother * blk + (size_t)c * cs;
+    double *dst = arr + (size_t)daughter * blk + (size_t)c * cs + (size_t)j * nx + i;
+    auto vx = [&](int jy, int kz) -> double {
+        const double *row = u + ((size_t)kz * ny + jy) * nx;
+        if (!px) return row[cx];
+        double s = cf[0] * row[cx - N + 1];
+#pragma unroll
+        for (int t = 1; t < 2 * N; ++t) s = s + cf[t] * row[cx - N + 1 + t];
+        return s;
+    };
+    auto vy = [&](int kz) -> double {
+        if (kz < 0 || kz >= nz) return 0.0;
+        if (!py) return vx(cy, kz);
+        double s = cf[0] * vx(cy - N + 1, kz);
+#pragma unroll
+        for (int t = 1; t < 2 * N; ++t) s = s + cf[t] * vx(cy - N + 1 + t, kz);
+        return s;
+    };
+    int cz = fz0 >> 1;
+    double w[2 * N];  // vy(cz - N + 1 .. cz + N)
+#pragma unroll
+    for (int t = 0; t < 2 * N; ++t) w[t] = vy(cz - N + 1 + t);
+    const size_t pstride = (size_t)ncol;
+    for (int k = 0; k < nz; ++k) {
+        const int f2 = fz0 + k;
+        if ((f2 >> 1) != cz) {
+            ++cz;
+#pragma unroll
+            for (int t = 0; t < 2 * N - 1; ++t) w[t] = w[t + 1];
+            w[2 * N - 1] = vy(cz + N);
+        }
+        double v;
+        if (f2 & 1) {
+            v = cf[0] * w[0];
+#pragma unroll
+            for (int t = 1; t < 2 * N; ++t) v = v + cf[t] * w[t];
+        } else {
+            v = w[N - 1];
+        }
+        dst[(size_t)k * pstride] = v;
+    }
+}
+
 int launch_refine(wh_pool *P, double *arr, const int32_t *d_pairs, int npairs, int order)
 {
     if (npairs <= 0) return WH_OK;
+    if (P->d.dim == 3) {
+        const int wpc = (P->nx * P->ny + 255) / 256;
+        const dim3 grid((unsigned)((size_t)wpc * P->d.neq * npairs));
+#define LR3(NN)                                                                                    \
+    hipLaunchKernelGGL(k_refine3d<NN>, grid, dim3(256), 0, P->stream, arr, d_pairs, npairs, wpc,   \
+                       P->d.bs[0], P->d.bs[1], P->d.bs[2], P->d.g, P->nx, P->ny, P->nz, P->blk,    \
+                       P->cs, P->d.neq)
+        if (order == 4) LR3(2);
+        else if (order == 2) LR3(1);
+        else LR3(3);
+#undef LR3
+        WH_HIP(hipGetLastError());
+        return WH_OK;
+    }
     const size_t npt = (size_t)P->nx * P->ny * P->nz * npairs;
     hipLaunchKernelGGL(k_refine, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, P->stream, arr,
                        d_pairs, npairs, P->d.bs[0], P->d.bs[1], P->d.dim == 3 ? P->d.bs[2] : 1,

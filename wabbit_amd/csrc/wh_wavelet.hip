@@ -243,102 +243,162 @@ struct ArrGeo {
     int nx, ny, neq;
 };
 
+// Per-point launches over listed blocks: `wpb` workgroups per (block,
+// component) pair, so the pair is a scalar division of the workgroup index
+// and a lane's point index t stays in 32 bits.
+struct PtIdx {
+    int k, c, t;
+};
+__device__ __forceinline__ PtIdx pt_index(int wpb, int neq)
+{
+    const int bc = (int)blockIdx.x / wpb;
+    PtIdx r;
+    r.k = bc / neq;
+    r.c = bc - r.k * neq;
+    r.t = ((int)blockIdx.x - bc * wpb) * (int)blockDim.x + (int)threadIdx.x;
+    return r;
+}
+
 // One separable pass over the listed blocks: out = chain(in) along `axis`;
 // the parity of the output point (relative to g) picks the SC/WC chain.
 __global__ __launch_bounds__(256) void k_wpass(const double *__restrict__ in,
                                                double *__restrict__ out,
-                                               const int32_t *__restrict__ ids, int nb,
-                                               ArrGeo A, Region R, int axis, long astride, int g,
+                                               const int32_t *__restrict__ ids, int wpb,
+                                               ArrGeo A, Region R, int axis, int astride, int g,
                                                TermTable T)
 {
-    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
-    const long per_b = npts * A.neq;
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= per_b * nb) return;
-    const int k = (int)(e / per_b);
-    long r = e - (long)k * per_b;
-    const int c = (int)(r / npts);
-    r -= (long)c * npts;
-    const int i0 = (int)(r % R.n[0]), i1 = (int)((r / R.n[0]) % R.n[1]),
-              i2 = (int)(r / ((long)R.n[0] * R.n[1]));
+    const PtIdx I = pt_index(wpb, A.neq);
+    if (I.t >= R.n[0] * R.n[1] * R.n[2]) return;
+    const int t1 = I.t / R.n[0];
+    const int i0 = I.t - t1 * R.n[0], i2 = t1 / R.n[1], i1 = t1 - i2 * R.n[1];
     const int p0 = R.lo[0] + R.step[0] * i0, p1 = R.lo[1] + R.step[1] * i1,
               p2 = R.lo[2] + R.step[2] * i2;
-    const size_t q = ((size_t)p2 * A.ny + p1) * A.nx + p0;
+    const int q = (p2 * A.ny + p1) * A.nx + p0;
     const int pa = axis == 0 ? p0 : (axis == 1 ? p1 : p2);
     const int par = (pa - g) & 1;
-    const int b = ids[k];
-    out[(size_t)b * A.blk_out + c * A.cs + q] =
-        chain(T, par, in + (size_t)b * A.blk_in + c * A.cs + q, astride);
+    const size_t b = (size_t)ids[I.k];
+    out[b * A.blk_out + (size_t)I.c * A.cs + q] =
+        chain(T, par, in + b * A.blk_in + (size_t)I.c * A.cs + q, astride);
 }
 
-// 2D: both separable passes of one component of one block in one workgroup.
-// The block's source and destination planes are staged in LDS; the x pass
-// writes Rx of the destination from the source, the y pass computes Ry from
-// the destination into the (no longer needed) source buffer, which is copied
-// back; the whole destination plane is written out, so everything outside
-// Rx and Ry keeps its previous value (init_copy: the source's) -- the values
-// of the two-launch version bit for bit.
-__global__ __launch_bounds__(256) void k_sep2d(const double *__restrict__ in, double *__restrict__ out,
+// Region membership along one direction, for steps 1 and 2 (checked on the
+// host before launch).
+__device__ __forceinline__ bool in_dir(const Region &R, int d, int p)
+{
+    const int r = p - R.lo[d];
+    return r >= 0 && (r & (R.step[d] - 1)) == 0 && (r >> (R.step[d] - 1)) < R.n[d];
+}
+
+// The x and y passes of one z-plane of one component of one block in one
+// workgroup (the whole pass in 2D).  The source plane is staged in LDS (S);
+// the x pass writes Rx into D; the y pass reads D along y and writes Ry into
+// S (which the x pass no longer needs); the plane's Rx points are written
+// out (Ry points from S, the rest of Rx from D), and with init_copy the
+// points outside Rx take the source's values.  The y chains read only rows
+// of Rx and Ry's x positions are Rx's (checked on the host), so every value
+// is the one of the pass-by-pass version bit for bit.  The chain table is
+// copied to LDS: lanes of one wavefront alternate parity in the x pass.
+__global__ __launch_bounds__(256) void k_sepxy(const double *__restrict__ in, double *__restrict__ out,
                                                const int32_t *__restrict__ ids, ArrGeo A, Region Rx,
-                                               Region Ry, int g, int init_copy, TermTable T)
+                                               Region Ry, int nzp, int g, int init_copy, TermTable T)
 {
     extern __shared__ double sp[];
+    __shared__ TermTable sT;
     const int nxy = A.nx * A.ny;
     double *S = sp, *D = sp + nxy;
-    const int k = blockIdx.x / A.neq, c = blockIdx.x - k * A.neq;
-    const int b = ids[k];
-    const double *src = in + (size_t)b * A.blk_in + (size_t)c * A.cs;
-    double *dst = out + (size_t)b * A.blk_out + (size_t)c * A.cs;
-    for (int i = threadIdx.x; i < nxy; i += blockDim.x) {
-        const double v = src[i];
-        S[i] = v;
-        D[i] = init_copy ? v : dst[i];
+    const int bc = (int)blockIdx.x / nzp, z = (int)blockIdx.x - bc * nzp;
+    const int k = bc / A.neq, c = bc - k * A.neq;
+    const size_t b = (size_t)ids[k];
+    const double *src = in + b * A.blk_in + (size_t)c * A.cs + (size_t)z * nxy;
+    double *dst = out + b * A.blk_out + (size_t)c * A.cs + (size_t)z * nxy;
+    if (!in_dir(Rx, 2, z)) {  // a plane outside the passes (uniform)
+        if (init_copy)
+            for (int i = threadIdx.x; i < nxy; i += blockDim.x) dst[i] = src[i];
+        return;
     }
+    {
+        const int *tw = reinterpret_cast<const int *>(&T);
+        int *sw = reinterpret_cast<int *>(&sT);
+        for (int i = threadIdx.x; i < (int)(sizeof(TermTable) / sizeof(int)); i += blockDim.x) sw[i] = tw[i];
+    }
+    for (int i = threadIdx.x; i < nxy; i += blockDim.x) S[i] = src[i];
     __syncthreads();
-    const int nx_ = Rx.n[0] * Rx.n[1];
-    for (int e = threadIdx.x; e < nx_; e += blockDim.x) {
+    const int nrx = Rx.n[0] * Rx.n[1];
+    for (int e = threadIdx.x; e < nrx; e += blockDim.x) {
         const int i1 = e / Rx.n[0], i0 = e - i1 * Rx.n[0];
         const int p0 = Rx.lo[0] + Rx.step[0] * i0, p1 = Rx.lo[1] + Rx.step[1] * i1;
         const int q = p1 * A.nx + p0;
-        D[q] = chain(T, (p0 - g) & 1, S + q, 1);
+        D[q] = chain(sT, (p0 - g) & 1, S + q, 1);
     }
     __syncthreads();
-    const int ny_ = Ry.n[0] * Ry.n[1];
-    for (int e = threadIdx.x; e < ny_; e += blockDim.x) {
+    const int nry = Ry.n[0] * Ry.n[1];
+    for (int e = threadIdx.x; e < nry; e += blockDim.x) {
         const int i1 = e / Ry.n[0], i0 = e - i1 * Ry.n[0];
         const int p0 = Ry.lo[0] + Ry.step[0] * i0, p1 = Ry.lo[1] + Ry.step[1] * i1;
         const int q = p1 * A.nx + p0;
-        S[q] = chain(T, (p1 - g) & 1, D + q, A.nx);
+        S[q] = chain(sT, (p1 - g) & 1, D + q, A.nx);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < ny_; e += blockDim.x) {
-        const int i1 = e / Ry.n[0], i0 = e - i1 * Ry.n[0];
-        const int q = (Ry.lo[1] + Ry.step[1] * i1) * A.nx + (Ry.lo[0] + Ry.step[0] * i0);
-        D[q] = S[q];
+    if (init_copy) {
+        for (int q = threadIdx.x; q < nxy; q += blockDim.x) {
+            const int p1 = q / A.nx, p0 = q - p1 * A.nx;
+            const bool x_only = in_dir(Rx, 0, p0) && in_dir(Rx, 1, p1) && !in_dir(Ry, 1, p1);
+            dst[q] = x_only ? D[q] : S[q];
+        }
+    } else {
+        for (int e = threadIdx.x; e < nrx; e += blockDim.x) {
+            const int i1 = e / Rx.n[0], i0 = e - i1 * Rx.n[0];
+            const int p1 = Rx.lo[1] + Rx.step[1] * i1;
+            const int q = p1 * A.nx + Rx.lo[0] + Rx.step[0] * i0;
+            dst[q] = in_dir(Ry, 1, p1) ? S[q] : D[q];
+        }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nxy; i += blockDim.x) dst[i] = D[i];
+}
+
+// A pass along y or z done in place, one thread per column: each thread
+// copies its whole column (all points along the axis) into its own LDS slice
+// and then writes the outputs of region R on that column from those values,
+// which are the values before the pass, as the pass-then-copy-back version
+// reads them.  Columns of a wavefront are consecutive output positions of the
+// fastest remaining direction (coalesced).
+constexpr int WCOL_LANES = 64;
+__global__ __launch_bounds__(WCOL_LANES) void k_wcol(double *__restrict__ arr, const int32_t *__restrict__ ids,
+                                                     int wpb, ArrGeo A, Region R, int axis, int len, int astride,
+                                                     int g, TermTable T)
+{
+    extern __shared__ double col[];  // [position][lane]
+    // the column runs along `axis`; x and the remaining direction (z for a
+    // y pass, y for a z pass) number the columns
+    const bool ya = axis == 1;
+    const int n1 = ya ? R.n[2] : R.n[1], lo1 = ya ? R.lo[2] : R.lo[1], st1 = ya ? R.step[2] : R.step[1];
+    const int na = ya ? R.n[1] : R.n[2], loa = ya ? R.lo[1] : R.lo[2], sta = ya ? R.step[1] : R.step[2];
+    const PtIdx I = pt_index(wpb, A.neq);
+    if (I.t >= R.n[0] * n1) return;
+    const int i1 = I.t / R.n[0], i0 = I.t - i1 * R.n[0];
+    const int px = R.lo[0] + R.step[0] * i0, p1 = lo1 + st1 * i1;
+    const int q0 = ya ? p1 * A.ny * A.nx + px : p1 * A.nx + px;
+    double *base = arr + (size_t)ids[I.k] * A.blk_out + (size_t)I.c * A.cs + q0;
+    double *mine = col + threadIdx.x;
+    for (int p = 0; p < len; ++p) mine[p * WCOL_LANES] = base[p * astride];
+    for (int i = 0; i < na; ++i) {
+        const int p = loa + sta * i;
+        base[p * astride] = chain(T, (p - g) & 1, mine + p * WCOL_LANES, WCOL_LANES);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_copy_region(const double *__restrict__ in,
                                                      double *__restrict__ out,
-                                                     const int32_t *__restrict__ ids, int nb,
+                                                     const int32_t *__restrict__ ids, int wpb,
                                                      ArrGeo A, Region R)
 {
-    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
-    const long per_b = npts * A.neq;
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= per_b * nb) return;
-    const int k = (int)(e / per_b);
-    long r = e - (long)k * per_b;
-    const int c = (int)(r / npts);
-    r -= (long)c * npts;
-    const int i0 = (int)(r % R.n[0]), i1 = (int)((r / R.n[0]) % R.n[1]),
-              i2 = (int)(r / ((long)R.n[0] * R.n[1]));
-    const size_t q = ((size_t)(R.lo[2] + R.step[2] * i2) * A.ny + (R.lo[1] + R.step[1] * i1)) * A.nx +
-                     (R.lo[0] + R.step[0] * i0);
-    const int b = ids[k];
-    out[(size_t)b * A.blk_out + c * A.cs + q] = in[(size_t)b * A.blk_in + c * A.cs + q];
+    const PtIdx I = pt_index(wpb, A.neq);
+    if (I.t >= R.n[0] * R.n[1] * R.n[2]) return;
+    const int t1 = I.t / R.n[0];
+    const int i0 = I.t - t1 * R.n[0], i2 = t1 / R.n[1], i1 = t1 - i2 * R.n[1];
+    const int q = ((R.lo[2] + R.step[2] * i2) * A.ny + (R.lo[1] + R.step[1] * i1)) * A.nx +
+                  (R.lo[0] + R.step[0] * i0);
+    const size_t b = (size_t)ids[I.k];
+    out[b * A.blk_out + (size_t)I.c * A.cs + q] = in[b * A.blk_in + (size_t)I.c * A.cs + q];
 }
 
 // Linfty detail: per listed block and component, max over the interior of
@@ -351,29 +411,32 @@ __global__ __launch_bounds__(256) void k_detail(const double *__restrict__ arr,
                                                 const int32_t *__restrict__ ids, ArrGeo A,
                                                 DetailArgs D, double *__restrict__ out)
 {
-    __shared__ unsigned long long smax[8];
-    if (threadIdx.x < 8) smax[threadIdx.x] = 0ull;
+    __shared__ unsigned long long smax;
+    if (threadIdx.x == 0) smax = 0ull;
     __syncthreads();
-    const int b = ids[blockIdx.x];
-    const int npts = D.bs[0] * D.bs[1] * D.bs[2];
+    const int k = (int)blockIdx.x / D.neq, c = (int)blockIdx.x - k * D.neq;
+    const int b = ids[k];
+    const int ncol = D.bs[0] * D.bs[1], nz = D.bs[2];
     const int zoff = D.dim == 3 ? D.g : 0;
-    for (int c = 0; c < D.neq; ++c) {
-        double m = 0.0;
-        const double *u = arr + (size_t)b * A.blk_in + c * A.cs;
-        for (int e = threadIdx.x; e < npts; e += blockDim.x) {
-            const int ix = e % D.bs[0], iy = (e / D.bs[0]) % D.bs[1], iz = e / (D.bs[0] * D.bs[1]);
-            const bool sc = ((ix & 1) == 0) && ((iy & 1) == 0) && (D.dim == 2 || (iz & 1) == 0);
-            const double v = sc ? 0.0
-                                : u[((size_t)(zoff + iz) * A.ny + (D.g + iy)) * A.nx + (D.g + ix)];
-            const double a = D.groups[c] >= 2 ? sqrt(v * v) : fabs(v);
+    const bool sq = D.groups[c] >= 2;
+    const double *u = arr + (size_t)b * A.blk_in + (size_t)c * A.cs;
+    double m = 0.0;
+    for (int e = threadIdx.x; e < ncol; e += blockDim.x) {
+        const int iy = e / D.bs[0], ix = e - iy * D.bs[0];
+        const bool sc2 = ((ix & 1) == 0) && ((iy & 1) == 0);
+        const double *col = u + (size_t)(zoff * A.ny + D.g + iy) * A.nx + (D.g + ix);
+        const size_t zs = (size_t)A.nx * A.ny;
+        for (int iz = 0; iz < nz; ++iz) {
+            const bool sc = sc2 && (D.dim == 2 || (iz & 1) == 0);
+            const double v = sc ? 0.0 : col[iz * zs];
+            const double a = sq ? sqrt(v * v) : fabs(v);
             m = a > m ? a : m;
         }
-        // non-negative doubles order like their bit patterns
-        atomicMax(&smax[c], (unsigned long long)__double_as_longlong(m));
     }
+    // non-negative doubles order like their bit patterns
+    atomicMax(&smax, (unsigned long long)__double_as_longlong(m));
     __syncthreads();
-    if (threadIdx.x < D.neq)
-        out[(size_t)blockIdx.x * D.neq + threadIdx.x] = __longlong_as_double((long long)smax[threadIdx.x]);
+    if (threadIdx.x == 0) out[blockIdx.x] = __longlong_as_double((long long)smax);
 }
 
 // Per-box maxima (one workgroup per box): per component the max over the box
@@ -465,13 +528,14 @@ int check_ids(wh_pool *P, const int32_t *ids, int n)
 int launch_pass(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
                 const Region &R, int axis, int g, const TermTable &T)
 {
-    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
+    const int npts = R.n[0] * R.n[1] * R.n[2];
     if (npts <= 0 || nb <= 0) return WH_OK;
-    const long tot = npts * P->d.neq * nb;
+    const int wpb = (npts + 255) / 256;
+    WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
-    const long astride = axis == 0 ? 1 : (axis == 1 ? P->nx : (long)P->nx * P->ny);
-    hipLaunchKernelGGL(k_wpass, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, in, out,
-                       P->ws.cur_ids, nb, A, R, axis, astride, g, T);
+    const int astride = axis == 0 ? 1 : (axis == 1 ? P->nx : P->nx * P->ny);
+    hipLaunchKernelGGL(k_wpass, dim3((unsigned)((size_t)wpb * P->d.neq * nb)), dim3(256), 0, P->stream, in,
+                       out, P->ws.cur_ids, wpb, A, R, axis, astride, g, T);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -479,12 +543,13 @@ int launch_pass(wh_pool *P, const double *in, size_t blk_in, double *out, size_t
 int launch_copy_region(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
                        const Region &R)
 {
-    const long npts = (long)R.n[0] * R.n[1] * R.n[2];
+    const int npts = R.n[0] * R.n[1] * R.n[2];
     if (npts <= 0 || nb <= 0) return WH_OK;
-    const long tot = npts * P->d.neq * nb;
+    const int wpb = (npts + 255) / 256;
+    WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
-    hipLaunchKernelGGL(k_copy_region, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, in,
-                       out, P->ws.cur_ids, nb, A, R);
+    hipLaunchKernelGGL(k_copy_region, dim3((unsigned)((size_t)wpb * P->d.neq * nb)), dim3(256), 0, P->stream,
+                       in, out, P->ws.cur_ids, wpb, A, R);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -494,45 +559,103 @@ Region full_region(const wh_pool *P)
     return Region{{0, 0, 0}, {P->nx, P->ny, P->nz}, {1, 1, 1}};
 }
 
-// the fused 2D pair of passes, where a block plane pair fits in 64 KB of LDS
-bool sep2d_ok(const wh_pool *P) { return P->d.dim == 2 && 2 * sizeof(double) * P->nx * P->ny <= 64 * 1024; }
+// in-place pass along y (axis 1) or z (axis 2) on `arr` over region R
+int launch_wcol(wh_pool *P, double *arr, size_t blk, int nb, const Region &R, int axis, const TermTable &T)
+{
+    const int ncol = R.n[0] * (axis == 1 ? R.n[2] : R.n[1]);
+    if (ncol <= 0 || nb <= 0) return WH_OK;
+    const int wpb = (ncol + WCOL_LANES - 1) / WCOL_LANES;
+    WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
+    const int len = axis == 1 ? P->ny : P->nz;
+    const int astride = axis == 1 ? P->nx : P->nx * P->ny;
+    // every point a chain reads must lie in the thread's LDS column
+    for (int i = 0; i < R.n[axis]; ++i) {
+        const int p = R.lo[axis] + R.step[axis] * i, par = (p - P->d.g) & 1;
+        for (int k = 0; k < T.n[par]; ++k)
+            WH_CHECK(p + T.off[par][k] >= 0 && p + T.off[par][k] < len, WH_E_ARG,
+                     "wavelet pass: filter support leaves the block");
+    }
+    const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, P->d.neq};
+    const size_t lds = sizeof(double) * len * WCOL_LANES;
+    hipLaunchKernelGGL(k_wcol, dim3((unsigned)((size_t)wpb * P->d.neq * nb)), dim3(WCOL_LANES), lds, P->stream,
+                       arr, P->ws.cur_ids, wpb, A, R, axis, len, astride, P->d.g, T);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
 
-int launch_sep2d(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
+// the fused 2D pair of passes, where a block plane pair fits in 64 KB of LDS
+// the fused x/y passes, where two planes of a block fit in 64 KB of LDS
+bool sepxy_ok(const wh_pool *P) { return 2 * sizeof(double) * P->nx * P->ny + sizeof(TermTable) <= 64 * 1024; }
+
+// reach of the chains along a pass direction from output position p
+bool chain_in(const TermTable &T, int p, int g, int lo, int hi)
+{
+    const int par = (p - g) & 1;
+    for (int k = 0; k < T.n[par]; ++k)
+        if (p + T.off[par][k] < lo || p + T.off[par][k] > hi) return false;
+    return true;
+}
+
+bool region_in(const Region &R, const int ext[3])
+{
+    for (int d = 0; d < 3; ++d)
+        if (R.n[d] <= 0 || R.lo[d] < 0 || R.lo[d] + R.step[d] * (R.n[d] - 1) >= ext[d] ||
+            (R.step[d] != 1 && R.step[d] != 2))
+            return false;
+    return true;
+}
+
+int launch_sepxy(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
                  const Region &Rx, const Region &Ry, int init_copy, const TermTable &T)
 {
     if (nb <= 0) return WH_OK;
+    const int g = P->d.g;
+    const int ext[3] = {P->nx, P->ny, P->d.dim == 3 ? P->nz : 1};
+    // what the kernel relies on: both regions inside the block, the same
+    // planes, Ry's x positions are Rx's, the x chains stay in the row, the
+    // y chains read rows of Rx
+    bool ok = region_in(Rx, ext) && region_in(Ry, ext) && Rx.lo[2] == Ry.lo[2] && Rx.n[2] == Ry.n[2] &&
+              Rx.step[2] == Ry.step[2] && Rx.lo[0] == Ry.lo[0] && Rx.n[0] == Ry.n[0] && Rx.step[0] == Ry.step[0];
+    for (int i = 0; ok && i < Rx.n[0]; ++i) ok = chain_in(T, Rx.lo[0] + Rx.step[0] * i, g, 0, P->nx - 1);
+    for (int i = 0; ok && i < Ry.n[1]; ++i) {
+        const int p = Ry.lo[1] + Ry.step[1] * i, par = (p - g) & 1;
+        for (int k = 0; ok && k < T.n[par]; ++k) {
+            const int r = p + T.off[par][k] - Rx.lo[1];
+            ok = r >= 0 && r % Rx.step[1] == 0 && r / Rx.step[1] < Rx.n[1];
+        }
+    }
+    WH_CHECK(ok, WH_E_ARG, "wavelet pass: regions do not fit the fused x/y kernel");
+    const int nzp = P->d.dim == 3 ? P->nz : 1;
+    WH_CHECK((size_t)nb * P->d.neq * nzp < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
     const size_t lds = 2 * sizeof(double) * P->nx * P->ny;
-    hipLaunchKernelGGL(k_sep2d, dim3((unsigned)(nb * P->d.neq)), dim3(256), lds, P->stream, in, out,
-                       P->ws.cur_ids, A, Rx, Ry, P->d.g, init_copy, T);
+    hipLaunchKernelGGL(k_sepxy, dim3((unsigned)(nb * P->d.neq * nzp)), dim3(256), lds, P->stream, in, out,
+                       P->ws.cur_ids, A, Rx, Ry, nzp, g, init_copy, T);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
 
 // Three separable passes (x over an extended row range, then y, then z) of
-// the chains T: x reads `in` and writes `out`; y and z read `out` and go
-// through work slot 0 (in-place along the pass direction).
+// the chains T: x reads `in` and writes `out`; y and z work in place on
+// `out`, a column at a time.
 int separable(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
               const TermTable &T, int ext, bool init_copy = false)
 {
     const int g = P->d.g, dim = P->d.dim;
     const int B0 = P->d.bs[0], B1 = P->d.bs[1], B2 = dim == 3 ? P->d.bs[2] : 1;
     const int zlo = dim == 3 ? g - ext : 0, zn = dim == 3 ? B2 + 2 * ext : 1;
-    double *scratch = P->work;  // slot 0
     Region Rx{{g, g - ext, zlo}, {B0, B1 + 2 * ext, zn}, {1, 1, 1}};
     Region Ry{{g, g, zlo}, {B0, B1, zn}, {1, 1, 1}};
-    if (sep2d_ok(P)) return launch_sep2d(P, in, blk_in, out, blk_out, nb, Rx, Ry, init_copy ? 1 : 0, T);
+    Region Rz{{g, g, g}, {B0, B1, B2}, {1, 1, 1}};
+    if (sepxy_ok(P)) {
+        if (int rc = launch_sepxy(P, in, blk_in, out, blk_out, nb, Rx, Ry, init_copy ? 1 : 0, T)) return rc;
+        return dim == 3 ? launch_wcol(P, out, blk_out, nb, Rz, 2, T) : WH_OK;
+    }
     if (init_copy)
         if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
     if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
-    if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Ry, 1, g, T)) return rc;
-    if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Ry)) return rc;
-    if (dim == 3) {
-        Region Rz{{g, g, g}, {B0, B1, B2}, {1, 1, 1}};
-        if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Rz, 2, g, T)) return rc;
-        if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Rz)) return rc;
-    }
-    return WH_OK;
+    if (int rc = launch_wcol(P, out, blk_out, nb, Ry, 1, T)) return rc;
+    return dim == 3 ? launch_wcol(P, out, blk_out, nb, Rz, 2, T) : WH_OK;
 }
 
 // blockFilterXYZ_vct(HD, do_restriction): out = in, then the SC interior
@@ -546,22 +669,19 @@ int filter_passes(wh_pool *P, const double *in, size_t blk_in, double *out, size
     const TermTable T = filter_terms(w);
     const int lo = w.hd_lo, hi = w.hd_hi;
     const int zlo = dim == 3 ? g + lo : 0, zn = dim == 3 ? B2 - lo + hi : 1;
-    double *scratch = P->work;
     // the parity of every filtered point is even, so chain 0 applies
     Region Rx{{g, g + lo, zlo}, {B0 / 2, B1 - lo + hi, zn}, {2, 1, 1}};
     Region Ry{{g, g, zlo}, {B0 / 2, B1 / 2, zn}, {2, 2, 1}};
-    if (!trivial && sep2d_ok(P)) return launch_sep2d(P, in, blk_in, out, blk_out, nb, Rx, Ry, 1, T);
+    Region Rz{{g, g, g}, {B0 / 2, B1 / 2, B2 / 2}, {2, 2, 2}};
+    if (!trivial && sepxy_ok(P)) {
+        if (int rc = launch_sepxy(P, in, blk_in, out, blk_out, nb, Rx, Ry, 1, T)) return rc;
+        return dim == 3 ? launch_wcol(P, out, blk_out, nb, Rz, 2, T) : WH_OK;
+    }
     if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
     if (trivial) return WH_OK;
     if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
-    if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Ry, 1, g, T)) return rc;
-    if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Ry)) return rc;
-    if (dim == 3) {
-        Region Rz{{g, g, g}, {B0 / 2, B1 / 2, B2 / 2}, {2, 2, 2}};
-        if (int rc = launch_pass(P, out, blk_out, scratch, P->blk, nb, Rz, 2, g, T)) return rc;
-        if (int rc = launch_copy_region(P, scratch, P->blk, out, blk_out, nb, Rz)) return rc;
-    }
-    return WH_OK;
+    if (int rc = launch_wcol(P, out, blk_out, nb, Ry, 1, T)) return rc;
+    return dim == 3 ? launch_wcol(P, out, blk_out, nb, Rz, 2, T) : WH_OK;
 }
 
 }  // namespace
@@ -710,7 +830,8 @@ int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int
     for (int d = 0; d < 3; ++d) D.bs[d] = d < P->d.dim ? P->d.bs[d] : 1;
     for (int c = 0; c < 8; ++c) D.groups[c] = (groups && c < neq) ? groups[c] : 1;
     const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
-    hipLaunchKernelGGL(k_detail, dim3(n), dim3(256), 0, P->stream, arr, W.cur_ids, A, D, W.d_out);
+    hipLaunchKernelGGL(k_detail, dim3((unsigned)(n * neq)), dim3(256), 0, P->stream, arr, W.cur_ids, A, D,
+                       W.d_out);
     WH_HIP(hipGetLastError());
     WH_HIP(hipMemcpyAsync(detail.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost,
                           P->stream));
