@@ -160,6 +160,7 @@ def cpu_baseline_adaptive(w, state, seconds=20.0):
 
 
 def run_adaptive(args, w):
+    from wabbit_amd import lib as L
     from wabbit_amd import workloads
     from wabbit_amd.timestep import PATH_FUSED
     torch_sync = _torch_sync()
@@ -218,6 +219,9 @@ def run_adaptive(args, w):
 
     if getattr(s, "tuned", None) is not None:
         roofline["tuned_variant"] = s.tuned[0]
+        if w["dim"] == 3:
+            roofline["tuned_zsplit"] = s.pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT)
+            roofline["tuned_waves"] = s.pool.get_tuning(L.WH_TUNE_K3D_WAVES)
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
@@ -472,6 +476,9 @@ def main():
                     kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(k_ms, 5),
                     launches=k_n, bytes_per_block_stage=bpb, blocks_per_launch=forest.n,
                     sync_kernel_avg_ms=round(s_ms, 5), tuned_variant=tuned)
+    if dim == 3:
+        roofline["tuned_zsplit"] = pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT)
+        roofline["tuned_waves"] = pool.get_tuning(L.WH_TUNE_K3D_WAVES)
 
     paths = {"fused_graph": round(value, 1)}
     if not args.no_paths:

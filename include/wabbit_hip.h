@@ -338,22 +338,30 @@ int wh_adapt_tree(wh_pool *pool, wh_forest *forest, const wh_adapt_params *p);
 int wh_sync_tree(wh_pool *pool, wh_forest *forest, int array);
 
 /* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
- * stage kernel (each owns Bs2/ysplit rows of (x,y) columns and marches in z);
+ * stage kernel (each owns Bs1/ysplit rows of (x,y) columns and marches in z);
  * 0 selects the flat one-point-per-thread kernel, -1 automatic.
+ * WH_TUNE_K3D_ZSPLIT (default 1): the z range of a block is cut into zsplit
+ * pieces marched by separate workgroups (a split that does not divide Bs2
+ * into pieces of at least 2R planes, R the stencil reach, acts as 1).
+ * WH_TUNE_K3D_WAVES (default 1): 3 selects the build of the z-marching
+ * kernel whose register allocation allows 3 waves per SIMD.
  * WH_TUNE_K2D_STAGE: 2D stage kernel variant: 0 one workgroup per block;
  * 1 (default) the same, loading the mask and stage operands of each
  * thread's first point before the halo-tile barrier; 2, 3, 4 a persistent
  * kernel with 2, 4, 8 workgroups per compute unit that loads the next
  * block's halo tile while computing the current one (where the tile fits).
  * Every variant computes the same values bit for bit. */
-enum { WH_TUNE_K3D_YSPLIT = 1, WH_TUNE_K2D_STAGE = 2 };
+enum { WH_TUNE_K3D_YSPLIT = 1, WH_TUNE_K2D_STAGE = 2, WH_TUNE_K3D_ZSPLIT = 3, WH_TUNE_K3D_WAVES = 4 };
 int wh_set_tuning(wh_pool *pool, int key, int value);
 int wh_get_tuning(wh_pool *pool, int key, int *value);
 
 /* Autotuner: times every valid variant of the RHS/stage kernel for this
  * pool's (Bs, g, neq) and parameters on the current active blocks, keeps the
- * fastest and returns it (3D: the WH_TUNE_K3D_YSPLIT value, 2D: the
- * WH_TUNE_K2D_STAGE value).  Work slot 1 is used as scratch (overwritten). */
+ * fastest and returns it (3D: the WH_TUNE_K3D_YSPLIT value, with the
+ * WH_TUNE_K3D_ZSPLIT and WH_TUNE_K3D_WAVES values set as well; 2D: the
+ * WH_TUNE_K2D_STAGE value).  With three or more work slots the timed kernel
+ * is the fused RK stage kernel and work slots 0 and 1 are overwritten,
+ * otherwise the RHS kernel and work slot 1. */
 int wh_autotune(wh_pool *pool, const wh_acm_params *p, int reps, int *best_value, double *best_ms);
 
 /* HIP-event timing of kernel kinds (0: RHS or fused stage kernel, 1: ghost

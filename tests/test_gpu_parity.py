@@ -223,10 +223,12 @@ def test_divergence_guard_and_errors():
 
 
 @pytest.mark.parametrize("scheme", [4, 6])
-@pytest.mark.parametrize("ysplit", [0, 1, 2, 3])
-def test_3d_kernel_variants_identical(scheme, ysplit):
-    """Every tuning variant of the 3D stage kernel gives the oracle's RHS
-    bit for bit (the autotuner may pick any of them)."""
+@pytest.mark.parametrize("ysplit,zsplit,waves", [(0, 1, 1), (1, 1, 1), (2, 1, 1), (3, 1, 1), (2, 2, 1),
+                                                 (3, 4, 1), (2, 3, 3), (3, 1, 3)])
+def test_3d_kernel_variants_identical(scheme, ysplit, zsplit, waves):
+    """Every tuning variant of the 3D stage kernel (y-split, z-split, waves
+    per SIMD) gives the oracle's RHS bit for bit (the autotuner may pick any
+    of them)."""
     po = _oracle()
     forest, acm, hvy, mask = _setup(3, scheme, True, True, True, J=1, bs=24)
     hs = _oracle_sync(forest, acm, hvy, REACH[scheme])
@@ -235,6 +237,9 @@ def test_3d_kernel_variants_identical(scheme, ysplit):
     pool.upload(L.WH_BLOCK, 0, hs)
     pool.upload(L.WH_MASK, 0, mask)
     pool.set_tuning(L.WH_TUNE_K3D_YSPLIT, ysplit)
+    pool.set_tuning(L.WH_TUNE_K3D_ZSPLIT, zsplit)
+    pool.set_tuning(L.WH_TUNE_K3D_WAVES, waves)
+    assert pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT) == zsplit and pool.get_tuning(L.WH_TUNE_K3D_WAVES) == waves
     pool.rhs(L.acm_params_c(acm), 0.0, L.WH_BLOCK, 1)
     got = interior(pool.download(L.WH_WORK + 1, 0, forest.n), 24, forest.g, 3)
     p = po.make_params(acm)
@@ -244,6 +249,7 @@ def test_3d_kernel_variants_identical(scheme, ysplit):
     assert np.array_equal(got, want)
     best, ms = pool.autotune(L.acm_params_c(acm), reps=2)
     assert best in (0, 1, 2, 3, 4, 6, 8) and ms > 0
+    assert pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT) in (1, 2, 3, 4, 6)
     pool.close()
 
 

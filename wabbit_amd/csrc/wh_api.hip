@@ -860,7 +860,23 @@ extern "C" {
 int wh_set_tuning(wh_pool *P, int key, int value)
 {
     WH_CHECK(P, WH_E_ARG, "null argument");
-    WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE, WH_E_ARG, "unknown tuning key");
+    WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE || key == WH_TUNE_K3D_ZSPLIT ||
+                 key == WH_TUNE_K3D_WAVES,
+             WH_E_ARG, "unknown tuning key");
+    if (key == WH_TUNE_K3D_WAVES) {
+        WH_CHECK(value == 1 || value == 3, WH_E_ARG, "K3D_WAVES is 1 or 3");
+        P->k3d_waves = value;
+        P->topo_version++;
+        return WH_OK;
+    }
+    if (key == WH_TUNE_K3D_ZSPLIT) {
+        WH_CHECK(value >= 1 && value <= 64, WH_E_ARG, "zsplit out of range");
+        // a split that does not divide Bs2 into pieces of at least 2R planes
+        // falls back to 1 at the launch
+        P->k3d_zsplit = value;
+        P->topo_version++;
+        return WH_OK;
+    }
     if (key == WH_TUNE_K2D_STAGE) {
         WH_CHECK(value >= 0 && value <= 4, WH_E_ARG, "K2D_STAGE is 0..4");
         P->k2d_stage = value;
@@ -878,8 +894,13 @@ int wh_set_tuning(wh_pool *P, int key, int value)
 int wh_get_tuning(wh_pool *P, int key, int *value)
 {
     WH_CHECK(P && value, WH_E_ARG, "null argument");
-    WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE, WH_E_ARG, "unknown tuning key");
-    *value = key == WH_TUNE_K2D_STAGE ? P->k2d_stage : P->k3d_ysplit;
+    WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE || key == WH_TUNE_K3D_ZSPLIT ||
+                 key == WH_TUNE_K3D_WAVES,
+             WH_E_ARG, "unknown tuning key");
+    *value = key == WH_TUNE_K2D_STAGE   ? P->k2d_stage
+             : key == WH_TUNE_K3D_ZSPLIT ? P->k3d_zsplit
+             : key == WH_TUNE_K3D_WAVES  ? P->k3d_waves
+                                         : P->k3d_ysplit;
     return WH_OK;
 }
 
@@ -892,34 +913,56 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
     WH_CHECK(P->nact > 0, WH_E_STATE, "autotune needs an active list");
     WH_CHECK(!P->timing, WH_E_STATE, "disable kernel timing before autotuning");
     reps = std::max(1, std::min(reps, 100));
-    // 3D: the flat kernel and the z-marching splits; 2D: the load order of
-    // the mask and stage operands.  The variant value is reported in best_value.
-    std::vector<int> cand;
+    // 3D: the flat kernel and the z-marching (y-split, z-split) pairs; 2D:
+    // the load order of the mask and stage operands.  The variant value is
+    // reported in best_value (3D: the y-split; the z-split is set too).
+    struct Var {
+        int v, zs, waves;
+    };
+    std::vector<Var> cand;
     const bool three = P->d.dim == 3;
     if (three) {
-        cand.push_back(0);
+        cand.push_back({0, 1, 1});
         for (int ys = 1; ys <= 8; ++ys)
-            if (zm_valid(P->d.bs[0], P->d.bs[1], p->scheme, ys)) cand.push_back(ys);
+            for (int zs = 1; zs <= 8; ++zs)
+                if (zm_valid(P->d.bs[0], P->d.bs[1], p->scheme, ys) && zm_zvalid(P->d.bs[2], p->scheme, zs))
+                    for (int wv : {1, 3}) cand.push_back({ys, zs, wv});
     } else {
-        cand = {0, 1, 2, 3, 4};
+        for (int v = 0; v <= 4; ++v) cand.push_back({v, 1, 1});
     }
     int &knob = three ? P->k3d_ysplit : P->k2d_stage;
     const AcmDev a = make_acm_dev(p);
     double *out = array_ptr(P, WH_WORK + 1, nullptr, nullptr);
-    const int saved = knob;
-    int best = saved;
+    // with three work slots the timed kernel is the fused RK stage (a middle
+    // stage: reads the block array, the accumulator in work slot 0, writes
+    // work slots 0 and 1), otherwise the RHS; nothing else is written
+    const bool fused = P->d.n_work_slots >= 3;
+    double *acc = array_ptr(P, WH_WORK + 0, nullptr, nullptr);
+    auto launch = [&]() -> int {
+        if (fused) return launch_fused_stage(P, a, P->block, out, P->block, acc, nullptr, 1, 0.5, 1.0 / 3.0, nullptr);
+        return launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact);
+    };
+    const int saved = knob, saved_z = P->k3d_zsplit, saved_w = P->k3d_waves;
+    Var best{saved, saved_z, saved_w};
     double bms = 1e300;
-    for (int v : cand) {
-        knob = v;
-        int rc = launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact);  // warm
+    auto set = [&](const Var &v) {
+        knob = v.v;
+        if (three) {
+            P->k3d_zsplit = v.zs;
+            P->k3d_waves = v.waves;
+        }
+    };
+    for (const Var &v : cand) {
+        set(v);
+        int rc = launch();  // warm
         if (rc) {
-            knob = saved;
+            set({saved, saved_z, saved_w});
             return rc;
         }
         WH_HIP(hipEventRecord(P->ev_a, P->stream));
         for (int r = 0; r < reps; ++r)
-            if ((rc = launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact))) {
-                knob = saved;
+            if ((rc = launch())) {
+                set({saved, saved_z, saved_w});
                 return rc;
             }
         WH_HIP(hipEventRecord(P->ev_b, P->stream));
@@ -931,9 +974,9 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
             best = v;
         }
     }
-    knob = best;
+    set(best);
     P->topo_version++;
-    if (best_value) *best_value = best;
+    if (best_value) *best_value = best.v;
     if (best_ms) *best_ms = bms;
     return WH_OK;
 }

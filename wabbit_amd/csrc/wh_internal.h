@@ -138,6 +138,8 @@ struct wh_pool {
     wh::TimeDev graph_tc{};
     // tuning: y-split of the 3D stage kernel (-1 auto, 0 flat kernel)
     int k3d_ysplit = -1;
+    int k3d_zsplit = 1;
+    int k3d_waves = 1;  // 3: the z-marching kernel built for 3 waves per SIMD
     // tuning of the 2D stage kernel: 0 plain, 1 mask and epilogue operands
     // loaded before the tile barrier, 2..4 persistent pipelined kernel with
     // 2, 4, 8 workgroups per CU
@@ -210,5 +212,6 @@ int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<doub
 AcmDev make_acm_dev(const wh_acm_params *p);
 int reach_of(int scheme);
 bool zm_valid(int bs0, int bs1, int scheme, int ys);
+bool zm_zvalid(int bs2, int scheme, int zs);
 
 }  // namespace wh

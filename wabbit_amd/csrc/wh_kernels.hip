@@ -419,21 +419,28 @@ __global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, do
 // (interior from the threads' registers, the R-wide x/y ring from global
 // memory); the z stencil reads a per-thread register queue of 2R+1 planes.
 // The next plane's values are fetched before the current one is computed.
-// Every input value is read from memory once (plus the slab ring).
-template <int S, bool SKEW, bool FUSED>
-__global__ __launch_bounds__(512) void k_rhs3d_zm(const double *__restrict__ in,
+// Every input value is read from memory once (plus the slab ring), except
+// that with zsplit > 1 the z range of a block is cut into zsplit pieces,
+// each with its own workgroups, which read the 2R planes around their cut.
+// WPE: the minimum waves per SIMD the register allocation must allow (1: no
+// constraint; 3 may spill a few registers to scratch).
+template <int S, bool SKEW, bool FUSED, int WPE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rhs3d_zm(const double *__restrict__ in,
                                                   double *__restrict__ out,
                                                   const double *__restrict__ mask,
                                                   const int32_t *__restrict__ act,
                                                   const double *__restrict__ x0dx, Geo G, AcmDev a,
-                                                  StageArgs sa, int ysplit)
+                                                  StageArgs sa, int ysplit, int zsplit)
 {
     constexpr int R = Reach<S>::R;
     constexpr int W = 2 * R + 1;
     constexpr int MAXRING = 4;  // ring elements per thread held in registers
     extern __shared__ double plane[];
-    const int k = blockIdx.x / ysplit;
-    const int part = blockIdx.x - k * ysplit;
+    const int per_block = ysplit * zsplit;
+    const int k = blockIdx.x / per_block;
+    const int rem = blockIdx.x - k * per_block;
+    const int zpart = rem / ysplit, part = rem - zpart * ysplit;
+    const int tz0 = G.bs2 / zsplit, z0 = zpart * tz0, z1 = z0 + tz0;
     const int b = act[k];
     const int ty0 = G.bs1 / ysplit;
     const int y0 = part * ty0;
@@ -476,16 +483,16 @@ __global__ __launch_bounds__(512) void k_rhs3d_zm(const double *__restrict__ in,
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-        for (int j = 0; j < 2 * R; ++j) q[c][j] = u[c * G.cs + (size_t)(G.g - R + j) * zs + col];
+        for (int j = 0; j < 2 * R; ++j) q[c][j] = u[c * G.cs + (size_t)(G.g + z0 - R + j) * zs + col];
     double nq[4], nr[MAXRING];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) nq[c] = u[c * G.cs + (size_t)(G.g + R) * zs + col];
+    for (int c = 0; c < 4; ++c) nq[c] = u[c * G.cs + (size_t)(G.g + z0 + R) * zs + col];
 #pragma unroll
     for (int j = 0; j < MAXRING; ++j)
-        if (j < nring) nr[j] = u[(size_t)G.g * zs + r_glb[j]];
+        if (j < nring) nr[j] = u[(size_t)(G.g + z0) * zs + r_glb[j]];
 
     const int lc = (iyl + R) * tx + (ix + R);
-    for (int iz = 0; iz < G.bs2; ++iz) {
+    for (int iz = z0; iz < z1; ++iz) {
         const size_t zoff = (size_t)(G.g + iz) * zs;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -495,7 +502,7 @@ __global__ __launch_bounds__(512) void k_rhs3d_zm(const double *__restrict__ in,
 #pragma unroll
         for (int j = 0; j < MAXRING; ++j)
             if (j < nring) plane[r_lds[j]] = nr[j];
-        if (iz + 1 < G.bs2) {  // prefetch the next plane
+        if (iz + 1 < z1) {  // prefetch the next plane
             const size_t zn = zoff + zs;
 #pragma unroll
             for (int c = 0; c < 4; ++c) nq[c] = u[c * G.cs + zn + R * zs + col];
@@ -531,6 +538,13 @@ __global__ __launch_bounds__(512) void k_rhs3d_zm(const double *__restrict__ in,
 int reach_of(int scheme)
 {
     return scheme == WH_FD2 ? 1 : scheme == WH_FD4 ? 2 : 3;
+}
+
+// Is the z-marching kernel usable with this z-split?  It divides Bs2, and
+// each piece has at least 2R planes.
+bool zm_zvalid(int bs2, int scheme, int zs)
+{
+    return zs >= 1 && bs2 % zs == 0 && bs2 / zs >= 2 * reach_of(scheme);
 }
 
 // Is the z-marching kernel usable with this y-split?  Threads per workgroup
@@ -631,11 +645,18 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
 #undef L2D
     } else if (const int ys = zm_split(P, G, a.scheme); ys > 0) {
         const int R = reach_of(a.scheme);
+        const int zs = zm_zvalid(G.bs2, a.scheme, P->k3d_zsplit) ? P->k3d_zsplit : 1;
         const size_t lds = sizeof(double) * 4 * (G.bs0 + 2 * R) * (G.bs1 / ys + 2 * R);
-        dim3 grid(nact * ys), blockd(G.bs0 * (G.bs1 / ys));
-#define L3D(SC, SK)                                                                             \
-    hipLaunchKernelGGL((k_rhs3d_zm<SC, SK, FUSED>), grid, blockd, lds, P->stream, in, out, mask, \
-                       act, x0dx, G, a, sa, ys)
+        dim3 grid(nact * ys * zs), blockd(G.bs0 * (G.bs1 / ys));
+#define L3D(SC, SK)                                                                                    \
+    do {                                                                                                \
+        if (P->k3d_waves >= 3)                                                                          \
+            hipLaunchKernelGGL((k_rhs3d_zm<SC, SK, FUSED, 3>), grid, blockd, lds, P->stream, in, out, mask, \
+                               act, x0dx, G, a, sa, ys, zs);                                            \
+        else                                                                                            \
+            hipLaunchKernelGGL((k_rhs3d_zm<SC, SK, FUSED, 1>), grid, blockd, lds, P->stream, in, out, mask, \
+                               act, x0dx, G, a, sa, ys, zs);                                            \
+    } while (0)
         switch (a.scheme * 2 + a.skew) {
         case WH_FD2 * 2: L3D(WH_FD2, false); break;
         case WH_FD2 * 2 + 1: L3D(WH_FD2, true); break;

@@ -208,18 +208,39 @@ TermTable filter_terms(const wh_wavelet &w)
     return T;
 }
 
-__device__ __forceinline__ double chain(const TermTable &T, int par, const double *p, long stride)
+// Chain kinds: a plain left-to-right sum (no breaks), a sum from zero, or
+// the general form with breaks.  The plain and from-zero kinds evaluate the
+// same additions as the general form without its per-term selects.
+enum { CHAIN_PLAIN = 0, CHAIN_ZERO = 1, CHAIN_GENERAL = 2 };
+
+int chain_kind(const TermTable &T)
+{
+    if (T.from_zero) return CHAIN_ZERO;
+    for (int par = 0; par < 2; ++par) {
+        if (T.n[par] < 1) return CHAIN_GENERAL;
+        for (int k = 0; k < T.n[par]; ++k)
+            if (T.brk[par][k]) return CHAIN_GENERAL;
+    }
+    return CHAIN_PLAIN;
+}
+
+template <int KIND, class F>
+__device__ __forceinline__ double chain_f(const TermTable &T, int par, F at)
 {
     const int n = T.n[par];
-    if (T.from_zero) {
+    if constexpr (KIND == CHAIN_ZERO) {
         double s = 0.0;
-        for (int k = 0; k < n; ++k) s = s + p[T.off[par][k] * stride] * T.c[par][k];
+        for (int k = 0; k < n; ++k) s = s + at(T.off[par][k]) * T.c[par][k];
+        return s;
+    } else if constexpr (KIND == CHAIN_PLAIN) {
+        double s = at(T.off[par][0]) * T.c[par][0];
+        for (int k = 1; k < n; ++k) s = s + at(T.off[par][k]) * T.c[par][k];
         return s;
     }
     double total = 0.0, cur = 0.0;
     bool have = false;
     for (int k = 0; k < n; ++k) {
-        const double v = p[T.off[par][k] * stride] * T.c[par][k];
+        const double v = at(T.off[par][k]) * T.c[par][k];
         if (k == 0) {
             cur = v;
         } else if (T.brk[par][k]) {
@@ -232,6 +253,82 @@ __device__ __forceinline__ double chain(const TermTable &T, int par, const doubl
     }
     return have ? total + cur : cur;
 }
+
+// The plain and from-zero kinds unrolled to NT terms (n <= NT): all NT
+// points are read first (unused entries have offset 0, an in-range point),
+// then the n used ones are added in order -- the same additions as chain_f.
+template <int KIND, int NT, class F>
+__device__ __forceinline__ double chain_n(const TermTable &T, int par, F at)
+{
+    static_assert(KIND == CHAIN_PLAIN || KIND == CHAIN_ZERO, "unrolled chains have no breaks");
+    const int n = T.n[par];
+    double v[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) v[k] = at(T.off[par][k]);
+    double s = KIND == CHAIN_ZERO ? 0.0 : v[0] * T.c[par][0];
+#pragma unroll
+    for (int k = KIND == CHAIN_ZERO ? 0 : 1; k < NT; ++k)
+        if (k < n) s = s + v[k] * T.c[par][k];
+    return s;
+}
+
+// kernel variant of a chain table: kind and unroll length (0: loop)
+struct ChainVar {
+    int kind, nt;
+};
+ChainVar chain_var(const TermTable &T)
+{
+    ChainVar v{chain_kind(T), 0};
+    if (v.kind == CHAIN_GENERAL) return v;
+    const int n = std::max(T.n[0], T.n[1]);
+    v.nt = n <= 8 ? 8 : n <= 12 ? 12 : n <= 16 ? 16 : 0;
+    return v;
+}
+
+// KIND and NT select the evaluation (NT 0: the loop of chain_f)
+template <int KIND, int NT, class F>
+__device__ __forceinline__ double chain_v(const TermTable &T, int par, F at)
+{
+    if constexpr (NT == 0) return chain_f<KIND>(T, par, at);
+    else return chain_n<KIND, NT>(T, par, at);
+}
+
+// launch L(KIND, NT) for a ChainVar
+#define CHAIN_DISPATCH(VAR, L)                                \
+    do {                                                      \
+        const ChainVar cv_ = (VAR);                           \
+        if (cv_.kind == CHAIN_PLAIN && cv_.nt == 8) L(CHAIN_PLAIN, 8);        \
+        else if (cv_.kind == CHAIN_PLAIN && cv_.nt == 12) L(CHAIN_PLAIN, 12); \
+        else if (cv_.kind == CHAIN_PLAIN && cv_.nt == 16) L(CHAIN_PLAIN, 16); \
+        else if (cv_.kind == CHAIN_PLAIN) L(CHAIN_PLAIN, 0);                  \
+        else if (cv_.kind == CHAIN_ZERO && cv_.nt == 8) L(CHAIN_ZERO, 8);     \
+        else if (cv_.kind == CHAIN_ZERO && cv_.nt == 12) L(CHAIN_ZERO, 12);   \
+        else if (cv_.kind == CHAIN_ZERO && cv_.nt == 16) L(CHAIN_ZERO, 16);   \
+        else if (cv_.kind == CHAIN_ZERO) L(CHAIN_ZERO, 0);                    \
+        else L(CHAIN_GENERAL, 0);                                             \
+    } while (0)
+
+// the chain at p, p[off*stride] being the point at offset off
+template <int KIND>
+__device__ __forceinline__ double chain_k(const TermTable &T, int par, const double *p, long stride)
+{
+    return chain_f<KIND>(T, par, [&](int off) { return p[off * stride]; });
+}
+
+// n / d for 0 <= n, d < 2^16 by a multiply-high with m = ceil(2^32 / d):
+// n*m / 2^32 exceeds n/d by less than 2^-16 <= 1/d, so the floor is exact.
+struct FastDiv {
+    uint32_t m;
+    int one;
+};
+FastDiv fast_div(int d)
+{
+    FastDiv f;
+    f.one = d == 1;
+    f.m = d <= 1 ? 0u : (uint32_t)((0x100000000ull + (uint64_t)d - 1) / (uint64_t)d);
+    return f;
+}
+__device__ __forceinline__ int fdiv(int n, FastDiv f) { return f.one ? n : (int)__umulhi((unsigned)n, f.m); }
 
 // Output points of a pass: lo + step*i, i < n per direction.
 struct Region {
@@ -259,8 +356,9 @@ __device__ __forceinline__ PtIdx pt_index(int wpb, int neq)
     return r;
 }
 
-// One separable pass over the listed blocks: out = chain(in) along `axis`;
+// One separable pass over the listed blocks: out = chain_k(in) along `axis`;
 // the parity of the output point (relative to g) picks the SC/WC chain.
+template <int KIND>
 __global__ __launch_bounds__(256) void k_wpass(const double *__restrict__ in,
                                                double *__restrict__ out,
                                                const int32_t *__restrict__ ids, int wpb,
@@ -278,7 +376,7 @@ __global__ __launch_bounds__(256) void k_wpass(const double *__restrict__ in,
     const int par = (pa - g) & 1;
     const size_t b = (size_t)ids[I.k];
     out[b * A.blk_out + (size_t)I.c * A.cs + q] =
-        chain(T, par, in + b * A.blk_in + (size_t)I.c * A.cs + q, astride);
+        chain_k<KIND>(T, par, in + b * A.blk_in + (size_t)I.c * A.cs + q, astride);
 }
 
 // Region membership along one direction, for steps 1 and 2 (checked on the
@@ -289,6 +387,29 @@ __device__ __forceinline__ bool in_dir(const Region &R, int d, int p)
     return r >= 0 && (r & (R.step[d] - 1)) == 0 && (r >> (R.step[d] - 1)) < R.n[d];
 }
 
+// Output positions along a pass direction split by chain parity: class 0
+// holds i = 0, s, 2s, ... (parity par0), class 1 (steps of 1 only) i = 1, 3,
+// ...; one class-row of items per enumeration index, each class padded to a
+// whole number of wavefronts, so a wavefront evaluates one chain.
+struct ParSplit {
+    int par0, s, nA, nB;
+};
+ParSplit par_split(const Region &R, int d, int g)
+{
+    ParSplit P;
+    P.par0 = ((R.lo[d] - g) % 2 + 2) % 2;
+    if (R.step[d] % 2 == 0) {
+        P.s = 1;
+        P.nA = R.n[d];
+        P.nB = 0;
+    } else {
+        P.s = 2;
+        P.nA = (R.n[d] + 1) / 2;
+        P.nB = R.n[d] / 2;
+    }
+    return P;
+}
+
 // The x and y passes of one z-plane of one component of one block in one
 // workgroup (the whole pass in 2D).  The source plane is staged in LDS (S);
 // the x pass writes Rx into D; the y pass reads D along y and writes Ry into
@@ -296,14 +417,19 @@ __device__ __forceinline__ bool in_dir(const Region &R, int d, int p)
 // out (Ry points from S, the rest of Rx from D), and with init_copy the
 // points outside Rx take the source's values.  The y chains read only rows
 // of Rx and Ry's x positions are Rx's (checked on the host), so every value
-// is the one of the pass-by-pass version bit for bit.  The chain table is
-// copied to LDS: lanes of one wavefront alternate parity in the x pass.
+// is the one of the pass-by-pass version bit for bit.  Outputs are
+// enumerated by parity class (ParSplit), so each wavefront's chain is
+// uniform and its table is read through the scalar unit.
+struct SepDiv {
+    FastDiv xa, xb, ry, rx, nx;  // by Px.nA, Px.nB, Ry.n[0], Rx.n[0], A.nx
+};
+template <int KIND, int NT>
 __global__ __launch_bounds__(256) void k_sepxy(const double *__restrict__ in, double *__restrict__ out,
                                                const int32_t *__restrict__ ids, ArrGeo A, Region Rx,
-                                               Region Ry, int nzp, int g, int init_copy, TermTable T)
+                                               Region Ry, ParSplit Px, ParSplit Py, SepDiv V, int nzp,
+                                               int init_copy, TermTable T)
 {
     extern __shared__ double sp[];
-    __shared__ TermTable sT;
     const int nxy = A.nx * A.ny;
     double *S = sp, *D = sp + nxy;
     const int bc = (int)blockIdx.x / nzp, z = (int)blockIdx.x - bc * nzp;
@@ -316,38 +442,49 @@ __global__ __launch_bounds__(256) void k_sepxy(const double *__restrict__ in, do
             for (int i = threadIdx.x; i < nxy; i += blockDim.x) dst[i] = src[i];
         return;
     }
-    {
-        const int *tw = reinterpret_cast<const int *>(&T);
-        int *sw = reinterpret_cast<int *>(&sT);
-        for (int i = threadIdx.x; i < (int)(sizeof(TermTable) / sizeof(int)); i += blockDim.x) sw[i] = tw[i];
-    }
     for (int i = threadIdx.x; i < nxy; i += blockDim.x) S[i] = src[i];
     __syncthreads();
-    const int nrx = Rx.n[0] * Rx.n[1];
-    for (int e = threadIdx.x; e < nrx; e += blockDim.x) {
-        const int i1 = e / Rx.n[0], i0 = e - i1 * Rx.n[0];
-        const int p0 = Rx.lo[0] + Rx.step[0] * i0, p1 = Rx.lo[1] + Rx.step[1] * i1;
-        const int q = p1 * A.nx + p0;
-        D[q] = chain(sT, (p0 - g) & 1, S + q, 1);
+    {   // x pass: class items are (position class index j, row)
+        const int na = Px.nA * Rx.n[1], pa = (na + 63) & ~63, tot = pa + Px.nB * Rx.n[1];
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const bool cb = e >= pa;
+            const int r = cb ? e - pa : e, nc = cb ? Px.nB : Px.nA;
+            if (!cb && e >= na) continue;
+            const int i1 = fdiv(r, cb ? V.xb : V.xa), j = r - i1 * nc;
+            const int i0 = cb ? 1 + 2 * j : Px.s * j;
+            const int q = (Rx.lo[1] + Rx.step[1] * i1) * A.nx + Rx.lo[0] + Rx.step[0] * i0;
+            const int par = __builtin_amdgcn_readfirstlane(cb ? 1 - Px.par0 : Px.par0);
+            const double *sq = S + q;
+            D[q] = chain_v<KIND, NT>(T, par, [&](int off) { return sq[off]; });
+        }
     }
     __syncthreads();
-    const int nry = Ry.n[0] * Ry.n[1];
-    for (int e = threadIdx.x; e < nry; e += blockDim.x) {
-        const int i1 = e / Ry.n[0], i0 = e - i1 * Ry.n[0];
-        const int p0 = Ry.lo[0] + Ry.step[0] * i0, p1 = Ry.lo[1] + Ry.step[1] * i1;
-        const int q = p1 * A.nx + p0;
-        S[q] = chain(sT, (p1 - g) & 1, D + q, A.nx);
+    {   // y pass: class items are (column, row class index j)
+        const int na = Py.nA * Ry.n[0], pa = (na + 63) & ~63, tot = pa + Py.nB * Ry.n[0];
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const bool cb = e >= pa;
+            const int r = cb ? e - pa : e;
+            if (!cb && e >= na) continue;
+            const int j = fdiv(r, V.ry), i0 = r - j * Ry.n[0];
+            const int i1 = cb ? 1 + 2 * j : Py.s * j;
+            const int q = (Ry.lo[1] + Ry.step[1] * i1) * A.nx + Ry.lo[0] + Ry.step[0] * i0;
+            const int par = __builtin_amdgcn_readfirstlane(cb ? 1 - Py.par0 : Py.par0);
+            const double *dq = D + q;
+            const int nx = A.nx;
+            S[q] = chain_v<KIND, NT>(T, par, [&](int off) { return dq[off * nx]; });
+        }
     }
     __syncthreads();
     if (init_copy) {
         for (int q = threadIdx.x; q < nxy; q += blockDim.x) {
-            const int p1 = q / A.nx, p0 = q - p1 * A.nx;
+            const int p1 = fdiv(q, V.nx), p0 = q - p1 * A.nx;
             const bool x_only = in_dir(Rx, 0, p0) && in_dir(Rx, 1, p1) && !in_dir(Ry, 1, p1);
             dst[q] = x_only ? D[q] : S[q];
         }
     } else {
+        const int nrx = Rx.n[0] * Rx.n[1];
         for (int e = threadIdx.x; e < nrx; e += blockDim.x) {
-            const int i1 = e / Rx.n[0], i0 = e - i1 * Rx.n[0];
+            const int i1 = fdiv(e, V.rx), i0 = e - i1 * Rx.n[0];
             const int p1 = Rx.lo[1] + Rx.step[1] * i1;
             const int q = p1 * A.nx + Rx.lo[0] + Rx.step[0] * i0;
             dst[q] = in_dir(Ry, 1, p1) ? S[q] : D[q];
@@ -355,18 +492,20 @@ __global__ __launch_bounds__(256) void k_sepxy(const double *__restrict__ in, do
     }
 }
 
-// A pass along y or z done in place, one thread per column: each thread
-// copies its whole column (all points along the axis) into its own LDS slice
-// and then writes the outputs of region R on that column from those values,
-// which are the values before the pass, as the pass-then-copy-back version
-// reads them.  Columns of a wavefront are consecutive output positions of the
-// fastest remaining direction (coalesced).
-constexpr int WCOL_LANES = 64;
-__global__ __launch_bounds__(WCOL_LANES) void k_wcol(double *__restrict__ arr, const int32_t *__restrict__ ids,
-                                                     int wpb, ArrGeo A, Region R, int axis, int len, int astride,
-                                                     int g, TermTable T)
+// A pass along y or z done in place, one thread per column, marching along
+// it: a ring of WRING LDS slots per thread holds the column's points from
+// the lowest offset of the current output's chain to the highest; a point is
+// loaded once, before any output at or above it is written, so every chain
+// reads the values before the pass, as the pass-then-copy-back version does.
+// Columns of a wavefront are consecutive x positions (coalesced); the output
+// position, and with it the chain, is uniform.
+constexpr int WRING = 16, WRING_LANES = 256;
+template <int KIND, int NT>
+__global__ __launch_bounds__(WRING_LANES) void k_wring(double *__restrict__ arr, const int32_t *__restrict__ ids,
+                                                       int wpb, ArrGeo A, Region R, int axis, int len, int astride,
+                                                       int g, int lo_off, int hi_off, TermTable T)
 {
-    extern __shared__ double col[];  // [position][lane]
+    __shared__ double ring[WRING * WRING_LANES];  // [slot][lane]
     // the column runs along `axis`; x and the remaining direction (z for a
     // y pass, y for a z pass) number the columns
     const bool ya = axis == 1;
@@ -378,27 +517,44 @@ __global__ __launch_bounds__(WCOL_LANES) void k_wcol(double *__restrict__ arr, c
     const int px = R.lo[0] + R.step[0] * i0, p1 = lo1 + st1 * i1;
     const int q0 = ya ? p1 * A.ny * A.nx + px : p1 * A.nx + px;
     double *base = arr + (size_t)ids[I.k] * A.blk_out + (size_t)I.c * A.cs + q0;
-    double *mine = col + threadIdx.x;
-    for (int p = 0; p < len; ++p) mine[p * WCOL_LANES] = base[p * astride];
+    double *mine = ring + threadIdx.x;
+    // points next .. next+3 are in flight in f0..f3 (zero past the column's
+    // end, never read); each is loaded before any output at or above it is
+    // stored, and program order keeps a thread's load before its later store
+    // to the same address
+    auto ld = [&](int x) { return x < len ? base[x * astride] : 0.0; };
+    int next = max(loa + lo_off, 0);
+    double f0 = ld(next), f1 = ld(next + 1), f2 = ld(next + 2), f3 = ld(next + 3);
     for (int i = 0; i < na; ++i) {
         const int p = loa + sta * i;
-        base[p * astride] = chain(T, (p - g) & 1, mine + p * WCOL_LANES, WCOL_LANES);
+        const int need = min(p + hi_off, len - 1);
+        for (; next <= need; ++next) {
+            mine[(next & (WRING - 1)) * WRING_LANES] = f0;
+            f0 = f1;
+            f1 = f2;
+            f2 = f3;
+            f3 = ld(next + 4);
+        }
+        base[p * astride] =
+            chain_v<KIND, NT>(T, (p - g) & 1, [&](int off) { return mine[((p + off) & (WRING - 1)) * WRING_LANES]; });
     }
 }
 
-__global__ __launch_bounds__(256) void k_copy_region(const double *__restrict__ in,
-                                                     double *__restrict__ out,
-                                                     const int32_t *__restrict__ ids, int wpb,
-                                                     ArrGeo A, Region R)
+// Whole listed blocks (all components, contiguous per block) copied in
+// 16-byte words where every block offset and the length are even, in
+// doubles otherwise.
+template <typename W>
+__global__ __launch_bounds__(256) void k_copy_blocks(const double *__restrict__ in, size_t blk_in,
+                                                     double *__restrict__ out, size_t blk_out,
+                                                     const int32_t *__restrict__ ids, int wpb, int nw)
 {
-    const PtIdx I = pt_index(wpb, A.neq);
-    if (I.t >= R.n[0] * R.n[1] * R.n[2]) return;
-    const int t1 = I.t / R.n[0];
-    const int i0 = I.t - t1 * R.n[0], i2 = t1 / R.n[1], i1 = t1 - i2 * R.n[1];
-    const int q = ((R.lo[2] + R.step[2] * i2) * A.ny + (R.lo[1] + R.step[1] * i1)) * A.nx +
-                  (R.lo[0] + R.step[0] * i0);
-    const size_t b = (size_t)ids[I.k];
-    out[b * A.blk_out + (size_t)I.c * A.cs + q] = in[b * A.blk_in + (size_t)I.c * A.cs + q];
+    const int k = (int)blockIdx.x / wpb;
+    const int t = ((int)blockIdx.x - k * wpb) * 512 + (int)threadIdx.x;
+    const size_t b = (size_t)ids[k];
+    const W *src = reinterpret_cast<const W *>(in + b * blk_in);
+    W *dst = reinterpret_cast<W *>(out + b * blk_out);
+    if (t < nw) dst[t] = src[t];
+    if (t + 256 < nw) dst[t + 256] = src[t + 256];
 }
 
 // Linfty detail: per listed block and component, max over the interior of
@@ -534,58 +690,80 @@ int launch_pass(wh_pool *P, const double *in, size_t blk_in, double *out, size_t
     WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
     const int astride = axis == 0 ? 1 : (axis == 1 ? P->nx : P->nx * P->ny);
-    hipLaunchKernelGGL(k_wpass, dim3((unsigned)((size_t)wpb * P->d.neq * nb)), dim3(256), 0, P->stream, in,
-                       out, P->ws.cur_ids, wpb, A, R, axis, astride, g, T);
+    const dim3 grid((unsigned)((size_t)wpb * P->d.neq * nb));
+    switch (chain_kind(T)) {
+    case CHAIN_PLAIN:
+        hipLaunchKernelGGL(k_wpass<CHAIN_PLAIN>, grid, dim3(256), 0, P->stream, in, out, P->ws.cur_ids, wpb, A, R,
+                           axis, astride, g, T);
+        break;
+    case CHAIN_ZERO:
+        hipLaunchKernelGGL(k_wpass<CHAIN_ZERO>, grid, dim3(256), 0, P->stream, in, out, P->ws.cur_ids, wpb, A, R,
+                           axis, astride, g, T);
+        break;
+    default:
+        hipLaunchKernelGGL(k_wpass<CHAIN_GENERAL>, grid, dim3(256), 0, P->stream, in, out, P->ws.cur_ids, wpb, A,
+                           R, axis, astride, g, T);
+    }
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
 
-int launch_copy_region(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb,
-                       const Region &R)
+int launch_copy_blocks(wh_pool *P, const double *in, size_t blk_in, double *out, size_t blk_out, int nb)
 {
-    const int npts = R.n[0] * R.n[1] * R.n[2];
-    if (npts <= 0 || nb <= 0) return WH_OK;
-    const int wpb = (npts + 255) / 256;
-    WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
-    const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
-    hipLaunchKernelGGL(k_copy_region, dim3((unsigned)((size_t)wpb * P->d.neq * nb)), dim3(256), 0, P->stream,
-                       in, out, P->ws.cur_ids, wpb, A, R);
+    if (nb <= 0) return WH_OK;
+    const size_t len = P->cs * P->d.neq;
+    const bool wide = blk_in % 2 == 0 && blk_out % 2 == 0 && len % 2 == 0;
+    const size_t nw = wide ? len / 2 : len;
+    WH_CHECK(nw < (1u << 30), WH_E_ARG, "block copy: blocks too large");
+    const int wpb = (int)((nw + 511) / 512);
+    WH_CHECK((size_t)wpb * nb < (1u << 31), WH_E_ARG, "block copy: too many blocks in one launch");
+    const dim3 grid((unsigned)(wpb * nb));
+    if (wide)
+        hipLaunchKernelGGL(k_copy_blocks<double2>, grid, dim3(256), 0, P->stream, in, blk_in, out, blk_out,
+                           P->ws.cur_ids, wpb, (int)nw);
+    else
+        hipLaunchKernelGGL(k_copy_blocks<double>, grid, dim3(256), 0, P->stream, in, blk_in, out, blk_out,
+                           P->ws.cur_ids, wpb, (int)nw);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
 
-Region full_region(const wh_pool *P)
-{
-    return Region{{0, 0, 0}, {P->nx, P->ny, P->nz}, {1, 1, 1}};
-}
 
 // in-place pass along y (axis 1) or z (axis 2) on `arr` over region R
 int launch_wcol(wh_pool *P, double *arr, size_t blk, int nb, const Region &R, int axis, const TermTable &T)
 {
     const int ncol = R.n[0] * (axis == 1 ? R.n[2] : R.n[1]);
     if (ncol <= 0 || nb <= 0) return WH_OK;
-    const int wpb = (ncol + WCOL_LANES - 1) / WCOL_LANES;
+    const int wpb = (ncol + WRING_LANES - 1) / WRING_LANES;
     WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const int len = axis == 1 ? P->ny : P->nz;
     const int astride = axis == 1 ? P->nx : P->nx * P->ny;
-    // every point a chain reads must lie in the thread's LDS column
+    // every point a chain reads lies in the column, and the chains' offsets
+    // span at most the ring
+    int lo_off = 0, hi_off = 0;
     for (int i = 0; i < R.n[axis]; ++i) {
         const int p = R.lo[axis] + R.step[axis] * i, par = (p - P->d.g) & 1;
-        for (int k = 0; k < T.n[par]; ++k)
+        for (int k = 0; k < T.n[par]; ++k) {
             WH_CHECK(p + T.off[par][k] >= 0 && p + T.off[par][k] < len, WH_E_ARG,
                      "wavelet pass: filter support leaves the block");
+            lo_off = std::min(lo_off, T.off[par][k]);
+            hi_off = std::max(hi_off, T.off[par][k]);
+        }
     }
+    WH_CHECK(hi_off - lo_off + 1 <= WRING, WH_E_ARG, "wavelet pass: filter longer than the column ring");
     const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, P->d.neq};
-    const size_t lds = sizeof(double) * len * WCOL_LANES;
-    hipLaunchKernelGGL(k_wcol, dim3((unsigned)((size_t)wpb * P->d.neq * nb)), dim3(WCOL_LANES), lds, P->stream,
-                       arr, P->ws.cur_ids, wpb, A, R, axis, len, astride, P->d.g, T);
+    const dim3 grid((unsigned)((size_t)wpb * P->d.neq * nb));
+#define LWR(K, N)                                                                                             \
+    hipLaunchKernelGGL((k_wring<K, N>), grid, dim3(WRING_LANES), 0, P->stream, arr, P->ws.cur_ids, wpb, A, R, \
+                       axis, len, astride, P->d.g, lo_off, hi_off, T)
+    CHAIN_DISPATCH(chain_var(T), LWR);
+#undef LWR
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
 
-// the fused 2D pair of passes, where a block plane pair fits in 64 KB of LDS
 // the fused x/y passes, where two planes of a block fit in 64 KB of LDS
-bool sepxy_ok(const wh_pool *P) { return 2 * sizeof(double) * P->nx * P->ny + sizeof(TermTable) <= 64 * 1024; }
+bool sepxy_ok(const wh_pool *P) { return 2 * sizeof(double) * P->nx * P->ny <= 64 * 1024; }
 
 // reach of the chains along a pass direction from output position p
 bool chain_in(const TermTable &T, int p, int g, int lo, int hi)
@@ -629,8 +807,16 @@ int launch_sepxy(wh_pool *P, const double *in, size_t blk_in, double *out, size_
     WH_CHECK((size_t)nb * P->d.neq * nzp < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const ArrGeo A{blk_in, blk_out, P->cs, P->nx, P->ny, P->d.neq};
     const size_t lds = 2 * sizeof(double) * P->nx * P->ny;
-    hipLaunchKernelGGL(k_sepxy, dim3((unsigned)(nb * P->d.neq * nzp)), dim3(256), lds, P->stream, in, out,
-                       P->ws.cur_ids, A, Rx, Ry, nzp, g, init_copy, T);
+    const ParSplit Px = par_split(Rx, 0, g), Py = par_split(Ry, 1, g);
+    WH_CHECK(P->nx < (1 << 16) && P->nx * P->ny + 64 < (1 << 16), WH_E_ARG, "wavelet pass: plane too large");
+    const SepDiv V{fast_div(Px.nA), fast_div(std::max(Px.nB, 1)), fast_div(Ry.n[0]), fast_div(Rx.n[0]),
+                   fast_div(P->nx)};
+    const dim3 grid((unsigned)(nb * P->d.neq * nzp));
+#define LSX(K, N)                                                                                                 \
+    hipLaunchKernelGGL((k_sepxy<K, N>), grid, dim3(256), lds, P->stream, in, out, P->ws.cur_ids, A, Rx, Ry, Px, \
+                       Py, V, nzp, init_copy, T)
+    CHAIN_DISPATCH(chain_var(T), LSX);
+#undef LSX
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -652,7 +838,7 @@ int separable(wh_pool *P, const double *in, size_t blk_in, double *out, size_t b
         return dim == 3 ? launch_wcol(P, out, blk_out, nb, Rz, 2, T) : WH_OK;
     }
     if (init_copy)
-        if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
+        if (int rc = launch_copy_blocks(P, in, blk_in, out, blk_out, nb)) return rc;
     if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
     if (int rc = launch_wcol(P, out, blk_out, nb, Ry, 1, T)) return rc;
     return dim == 3 ? launch_wcol(P, out, blk_out, nb, Rz, 2, T) : WH_OK;
@@ -677,7 +863,7 @@ int filter_passes(wh_pool *P, const double *in, size_t blk_in, double *out, size
         if (int rc = launch_sepxy(P, in, blk_in, out, blk_out, nb, Rx, Ry, 1, T)) return rc;
         return dim == 3 ? launch_wcol(P, out, blk_out, nb, Rz, 2, T) : WH_OK;
     }
-    if (int rc = launch_copy_region(P, in, blk_in, out, blk_out, nb, full_region(P))) return rc;
+    if (int rc = launch_copy_blocks(P, in, blk_in, out, blk_out, nb)) return rc;
     if (trivial) return WH_OK;
     if (int rc = launch_pass(P, in, blk_in, out, blk_out, nb, Rx, 0, g, T)) return rc;
     if (int rc = launch_wcol(P, out, blk_out, nb, Ry, 1, T)) return rc;
@@ -769,7 +955,7 @@ int run_blocks_op(wh_pool *P, int op, int src_array, int dst_array, const std::v
     if (int rc = upload_ids(P, ids.data(), n)) return rc;
     const wh_wavelet &w = P->ws.w;
     WH_CHECK(op == WOP_COPY || P->d.g >= w.g, WH_E_ARG, "wavelet op: too few ghost nodes for the wavelet");
-    if (op == WOP_COPY) return launch_copy_region(P, in, blk_in, out, blk_out, n, full_region(P));
+    if (op == WOP_COPY) return launch_copy_blocks(P, in, blk_in, out, blk_out, n);
     if (op == WOP_FWT)
         return separable(P, in, blk_in, out, blk_out, n, fwt_terms(w), std::max(w.hd_hi, w.gd_hi));
     if (op == WOP_IWT)  // CDFX0 starts from a copy of the whole block

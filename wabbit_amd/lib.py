@@ -18,7 +18,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "wabbit_hip.h")
 
 WH_BLOCK, WH_TMP, WH_MASK, WH_WORK = 0, 1, 2, 16
 WH_PATCH_COPY, WH_PATCH_RESTRICT, WH_PATCH_PREDICT = 0, 1, 2
-WH_TUNE_K3D_YSPLIT, WH_TUNE_K2D_STAGE = 1, 2
+WH_TUNE_K3D_YSPLIT, WH_TUNE_K2D_STAGE, WH_TUNE_K3D_ZSPLIT, WH_TUNE_K3D_WAVES = 1, 2, 3, 4
 ERRORS = {-1: "WH_E_ARG", -2: "WH_E_HIP", -3: "WH_E_STATE", -4: "WH_E_DIVERGED", -5: "WH_E_NOMEM"}
 
 
@@ -282,6 +282,11 @@ class Pool:
 
     def set_tuning(self, key, value):
         check(load().wh_set_tuning(self.h, key, value))
+
+    def get_tuning(self, key):
+        v = ctypes.c_int()
+        check(load().wh_get_tuning(self.h, key, ctypes.byref(v)))
+        return v.value
 
     def autotune(self, acm_c, reps=5):
         v, ms = ctypes.c_int(), ctypes.c_double()
