@@ -161,6 +161,14 @@ typedef struct {
 
 int wh_time_set(wh_pool *pool, double time, int iteration);
 int wh_time_get(wh_pool *pool, double *time, int *iteration, double *dt_last);
+/* wh_time_fetch queues a copy of the device time, iteration and last dt to
+ * pinned host memory on the pool's stream and returns at once;
+ * wh_time_fetched waits for that copy only (not for later work) and returns
+ * the values: the host learns a device-side step's dt without a stream
+ * synchronisation of its own (main.f90's time bookkeeping, kept on the
+ * device). */
+int wh_time_fetch(wh_pool *pool);
+int wh_time_fetched(wh_pool *pool, double *time, int *iteration, double *dt_last);
 
 /* `nsteps` main-loop time steps with dt, its clamps and the time advance kept
  * on the device: per step the block-dt reduction, the clamps, a sync and the

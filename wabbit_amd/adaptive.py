@@ -79,7 +79,13 @@ class AdaptiveACM:
             domain=acm.domain_size, mask=mask)
         self.acm_c = L.acm_params_c(acm)
         self.butcher = butcher_rk4()
+        self.tp_c = L.time_params_c(tp)
         self.time, self.iteration = 0.0, 0
+        # the fused path keeps dt, its clamps and the time advance on the
+        # device (wh_run_steps) and learns them after adapt_tree without a
+        # synchronisation of its own; (time, iteration) last set on the device
+        self.device_time = path == PATH_FUSED
+        self._dev_time = None
         self.rk_blocks = 0
         self.phase_ms = None
         self._stale = True
@@ -167,13 +173,22 @@ class AdaptiveACM:
         if self.mask_p is not None:
             W.create_mask(self.pool, self.mask_p)
         t = self._tick(t, "mask")
-        dt = clamp_dt(self.pool.dt(self.acm_c), self.time, self.tp)
-        self.pool.rk_step(self.acm_c, self.butcher, self.time, dt, self.path)
+        if self.device_time:
+            if self._dev_time != (self.time, self.iteration):
+                self.pool.time_set(self.time, self.iteration)
+            self.pool.run_steps(self.acm_c, self.butcher, self.tp_c, 1, use_graph=False)
+            self.pool.time_fetch()
+        else:
+            dt = clamp_dt(self.pool.dt(self.acm_c), self.time, self.tp)
+            self.pool.rk_step(self.acm_c, self.butcher, self.time, dt, self.path)
+            self.time += dt
+            self.iteration += 1
         t = self._tick(t, "rk")
-        self.time += dt
-        self.iteration += 1
         W.adapt_tree(self.pool, self.forest, self.adapt_p)
         self._stale = True   # the next refine re-attaches; state()/mask() refresh
+        if self.device_time:
+            self.time, self.iteration, dt = self.pool.time_fetched()
+            self._dev_time = (self.time, self.iteration)
         self._tick(t, "adapt")
         return dt
 

@@ -358,6 +358,9 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
     if ((e = hipHostMalloc((void **)&P->h_scal, sizeof(double) * DS_COUNT, hipHostMallocDefault)) !=
         hipSuccess)
         return fail("hipHostMalloc", e);
+    if ((e = hipHostMalloc((void **)&P->h_time, sizeof(double) * DS_COUNT, hipHostMallocDefault)) !=
+        hipSuccess)
+        return fail("hipHostMalloc", e);
     if ((e = hipMemsetAsync(P->scal, 0, sizeof(double) * DS_COUNT, P->stream)) != hipSuccess)
         return fail("hipMemset", e);
     if ((e = hipMemsetAsync(P->block, 0, sizeof(double) * P->blk * d.nb_max, P->stream)) != hipSuccess)
@@ -369,6 +372,8 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
     if ((e = hipEventCreateWithFlags(&P->ev_tables, hipEventDisableTiming)) != hipSuccess)
         return fail("hipEventCreate", e);
     if ((e = hipEventCreate(&P->ev_b)) != hipSuccess) return fail("hipEventCreate", e);
+    if ((e = hipEventCreateWithFlags(&P->ev_time, hipEventDisableTiming)) != hipSuccess)
+        return fail("hipEventCreate", e);
     if ((e = hipStreamSynchronize(P->stream)) != hipSuccess) return fail("sync", e);
     *out = P;
     return WH_OK;
@@ -389,6 +394,8 @@ int wh_pool_destroy(wh_pool *P)
     if (P->d_dtb) (void)hipFree(P->d_dtb);
     if (P->d_dtg) (void)hipFree(P->d_dtg);
     if (P->h_scal) (void)hipHostFree(P->h_scal);
+    if (P->h_time) (void)hipHostFree(P->h_time);
+    if (P->ev_time) (void)hipEventDestroy(P->ev_time);
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
     if (P->ev_tables) (void)hipEventDestroy(P->ev_tables);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
@@ -772,6 +779,26 @@ int wh_time_get(wh_pool *P, double *time, int *iteration, double *dt_last)
     if (time) *time = P->h_scal[DS_TIME];
     if (iteration) *iteration = (int)P->h_scal[DS_ITER];
     if (dt_last) *dt_last = P->h_scal[DS_DT];
+    return WH_OK;
+}
+
+int wh_time_fetch(wh_pool *P)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    WH_HIP(hipMemcpyAsync(P->h_time, P->scal, sizeof(double) * DS_COUNT, hipMemcpyDeviceToHost, P->stream));
+    WH_HIP(hipEventRecord(P->ev_time, P->stream));
+    P->time_fetch_pending = true;
+    return WH_OK;
+}
+
+int wh_time_fetched(wh_pool *P, double *time, int *iteration, double *dt_last)
+{
+    WH_CHECK(P, WH_E_ARG, "null argument");
+    WH_CHECK(P->time_fetch_pending, WH_E_STATE, "wh_time_fetched: no wh_time_fetch before");
+    WH_HIP(hipEventSynchronize(P->ev_time));
+    if (time) *time = P->h_time[DS_TIME];
+    if (iteration) *iteration = (int)P->h_time[DS_ITER];
+    if (dt_last) *dt_last = P->h_time[DS_DT];
     return WH_OK;
 }
 

@@ -112,6 +112,9 @@ struct wh_pool {
     double *block = nullptr, *work = nullptr, *tmp = nullptr, *mask = nullptr;
     double *scal = nullptr;       // wh::DS_COUNT doubles
     double *h_scal = nullptr;     // pinned mirror
+    double *h_time = nullptr;     // pinned copy of the scalars by wh_time_fetch
+    hipEvent_t ev_time = nullptr; // recorded after that copy
+    bool time_fetch_pending = false;
     int32_t *d_active = nullptr;
     double *d_x0dx = nullptr;
     std::vector<int32_t> h_active;

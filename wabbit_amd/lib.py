@@ -99,6 +99,9 @@ def load(path: str = LIB_PATH):
         "wh_time_set": [_vp, ctypes.c_double, ctypes.c_int],
         "wh_time_get": [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                         ctypes.POINTER(ctypes.c_double)],
+        "wh_time_fetch": [_vp],
+        "wh_time_fetched": [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_double)],
         "wh_run_steps": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int,
                          ctypes.POINTER(TimeParamsC), ctypes.c_int, ctypes.c_int],
         "wh_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
@@ -270,6 +273,14 @@ class Pool:
     def time_get(self):
         t, it, dt = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
         check(load().wh_time_get(self.h, ctypes.byref(t), ctypes.byref(it), ctypes.byref(dt)))
+        return t.value, it.value, dt.value
+
+    def time_fetch(self):
+        check(load().wh_time_fetch(self.h))
+
+    def time_fetched(self):
+        t, it, dt = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        check(load().wh_time_fetched(self.h, ctypes.byref(t), ctypes.byref(it), ctypes.byref(dt)))
         return t.value, it.value, dt.value
 
     def run_steps(self, acm_c, butcher, tp_c, nsteps, use_graph=True):
