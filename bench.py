@@ -222,6 +222,7 @@ def run_adaptive(args, w):
         if w["dim"] == 3:
             roofline["tuned_zsplit"] = s.pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT)
             roofline["tuned_waves"] = s.pool.get_tuning(L.WH_TUNE_K3D_WAVES)
+            roofline["tuned_xcd"] = s.pool.get_tuning(L.WH_TUNE_K3D_XCD)
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
@@ -479,6 +480,7 @@ def main():
     if dim == 3:
         roofline["tuned_zsplit"] = pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT)
         roofline["tuned_waves"] = pool.get_tuning(L.WH_TUNE_K3D_WAVES)
+        roofline["tuned_xcd"] = pool.get_tuning(L.WH_TUNE_K3D_XCD)
 
     paths = {"fused_graph": round(value, 1)}
     if not args.no_paths:

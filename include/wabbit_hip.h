@@ -353,20 +353,29 @@ int wh_sync_tree(wh_pool *pool, wh_forest *forest, int array);
  * into pieces of at least 2R planes, R the stencil reach, acts as 1).
  * WH_TUNE_K3D_WAVES (default 1): 3 selects the build of the z-marching
  * kernel whose register allocation allows 3 waves per SIMD.
+ * WH_TUNE_K3D_XCD (default 0): 1 remaps the flat 3D kernel's workgroups so
+ * that those of one block run on one XCD (one L2).
  * WH_TUNE_K2D_STAGE: 2D stage kernel variant: 0 one workgroup per block;
  * 1 (default) the same, loading the mask and stage operands of each
  * thread's first point before the halo-tile barrier; 2, 3, 4 a persistent
  * kernel with 2, 4, 8 workgroups per compute unit that loads the next
  * block's halo tile while computing the current one (where the tile fits).
  * Every variant computes the same values bit for bit. */
-enum { WH_TUNE_K3D_YSPLIT = 1, WH_TUNE_K2D_STAGE = 2, WH_TUNE_K3D_ZSPLIT = 3, WH_TUNE_K3D_WAVES = 4 };
+enum {
+    WH_TUNE_K3D_YSPLIT = 1,
+    WH_TUNE_K2D_STAGE = 2,
+    WH_TUNE_K3D_ZSPLIT = 3,
+    WH_TUNE_K3D_WAVES = 4,
+    WH_TUNE_K3D_XCD = 5
+};
 int wh_set_tuning(wh_pool *pool, int key, int value);
 int wh_get_tuning(wh_pool *pool, int key, int *value);
 
 /* Autotuner: times every valid variant of the RHS/stage kernel for this
  * pool's (Bs, g, neq) and parameters on the current active blocks, keeps the
  * fastest and returns it (3D: the WH_TUNE_K3D_YSPLIT value, with the
- * WH_TUNE_K3D_ZSPLIT and WH_TUNE_K3D_WAVES values set as well; 2D: the
+ * WH_TUNE_K3D_ZSPLIT, WH_TUNE_K3D_WAVES and WH_TUNE_K3D_XCD values set as
+ * well; 2D: the
  * WH_TUNE_K2D_STAGE value).  With three or more work slots the timed kernel
  * is the fused RK stage kernel and work slots 0 and 1 are overwritten,
  * otherwise the RHS kernel and work slot 1. */

@@ -223,12 +223,13 @@ def test_divergence_guard_and_errors():
 
 
 @pytest.mark.parametrize("scheme", [4, 6])
-@pytest.mark.parametrize("ysplit,zsplit,waves", [(0, 1, 1), (1, 1, 1), (2, 1, 1), (3, 1, 1), (2, 2, 1),
-                                                 (3, 4, 1), (2, 3, 3), (3, 1, 3)])
-def test_3d_kernel_variants_identical(scheme, ysplit, zsplit, waves):
+@pytest.mark.parametrize("ysplit,zsplit,waves,xcd", [(0, 1, 1, 0), (0, 1, 1, 1), (1, 1, 1, 0), (2, 1, 1, 0),
+                                                     (3, 1, 1, 0), (2, 2, 1, 0), (3, 4, 1, 0), (2, 3, 3, 0),
+                                                     (3, 1, 3, 0)])
+def test_3d_kernel_variants_identical(scheme, ysplit, zsplit, waves, xcd):
     """Every tuning variant of the 3D stage kernel (y-split, z-split, waves
-    per SIMD) gives the oracle's RHS bit for bit (the autotuner may pick any
-    of them)."""
+    per SIMD, XCD swizzle of the flat kernel) gives the oracle's RHS bit for
+    bit (the autotuner may pick any of them)."""
     po = _oracle()
     forest, acm, hvy, mask = _setup(3, scheme, True, True, True, J=1, bs=24)
     hs = _oracle_sync(forest, acm, hvy, REACH[scheme])
@@ -239,6 +240,7 @@ def test_3d_kernel_variants_identical(scheme, ysplit, zsplit, waves):
     pool.set_tuning(L.WH_TUNE_K3D_YSPLIT, ysplit)
     pool.set_tuning(L.WH_TUNE_K3D_ZSPLIT, zsplit)
     pool.set_tuning(L.WH_TUNE_K3D_WAVES, waves)
+    pool.set_tuning(L.WH_TUNE_K3D_XCD, xcd)
     assert pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT) == zsplit and pool.get_tuning(L.WH_TUNE_K3D_WAVES) == waves
     pool.rhs(L.acm_params_c(acm), 0.0, L.WH_BLOCK, 1)
     got = interior(pool.download(L.WH_WORK + 1, 0, forest.n), 24, forest.g, 3)

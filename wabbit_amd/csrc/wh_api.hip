@@ -888,8 +888,14 @@ int wh_set_tuning(wh_pool *P, int key, int value)
 {
     WH_CHECK(P, WH_E_ARG, "null argument");
     WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE || key == WH_TUNE_K3D_ZSPLIT ||
-                 key == WH_TUNE_K3D_WAVES,
+                 key == WH_TUNE_K3D_WAVES || key == WH_TUNE_K3D_XCD,
              WH_E_ARG, "unknown tuning key");
+    if (key == WH_TUNE_K3D_XCD) {
+        WH_CHECK(value == 0 || value == 1, WH_E_ARG, "K3D_XCD is 0 or 1");
+        P->k3d_xcd = value;
+        P->topo_version++;
+        return WH_OK;
+    }
     if (key == WH_TUNE_K3D_WAVES) {
         WH_CHECK(value == 1 || value == 3, WH_E_ARG, "K3D_WAVES is 1 or 3");
         P->k3d_waves = value;
@@ -922,11 +928,12 @@ int wh_get_tuning(wh_pool *P, int key, int *value)
 {
     WH_CHECK(P && value, WH_E_ARG, "null argument");
     WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE || key == WH_TUNE_K3D_ZSPLIT ||
-                 key == WH_TUNE_K3D_WAVES,
+                 key == WH_TUNE_K3D_WAVES || key == WH_TUNE_K3D_XCD,
              WH_E_ARG, "unknown tuning key");
     *value = key == WH_TUNE_K2D_STAGE   ? P->k2d_stage
              : key == WH_TUNE_K3D_ZSPLIT ? P->k3d_zsplit
              : key == WH_TUNE_K3D_WAVES  ? P->k3d_waves
+             : key == WH_TUNE_K3D_XCD    ? P->k3d_xcd
                                          : P->k3d_ysplit;
     return WH_OK;
 }
@@ -944,18 +951,19 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
     // the load order of the mask and stage operands.  The variant value is
     // reported in best_value (3D: the y-split; the z-split is set too).
     struct Var {
-        int v, zs, waves;
+        int v, zs, waves, xcd;
     };
     std::vector<Var> cand;
     const bool three = P->d.dim == 3;
     if (three) {
-        cand.push_back({0, 1, 1});
+        cand.push_back({0, 1, 1, 0});
+        cand.push_back({0, 1, 1, 1});
         for (int ys = 1; ys <= 8; ++ys)
             for (int zs = 1; zs <= 8; ++zs)
                 if (zm_valid(P->d.bs[0], P->d.bs[1], p->scheme, ys) && zm_zvalid(P->d.bs[2], p->scheme, zs))
-                    for (int wv : {1, 3}) cand.push_back({ys, zs, wv});
+                    for (int wv : {1, 3}) cand.push_back({ys, zs, wv, 0});
     } else {
-        for (int v = 0; v <= 4; ++v) cand.push_back({v, 1, 1});
+        for (int v = 0; v <= 4; ++v) cand.push_back({v, 1, 1, 0});
     }
     int &knob = three ? P->k3d_ysplit : P->k2d_stage;
     const AcmDev a = make_acm_dev(p);
@@ -969,27 +977,28 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
         if (fused) return launch_fused_stage(P, a, P->block, out, P->block, acc, nullptr, 1, 0.5, 1.0 / 3.0, nullptr);
         return launch_rhs(P, a, P->block, out, P->d_active, P->d_x0dx, P->nact);
     };
-    const int saved = knob, saved_z = P->k3d_zsplit, saved_w = P->k3d_waves;
-    Var best{saved, saved_z, saved_w};
+    const Var saved{knob, P->k3d_zsplit, P->k3d_waves, P->k3d_xcd};
+    Var best = saved;
     double bms = 1e300;
     auto set = [&](const Var &v) {
         knob = v.v;
         if (three) {
             P->k3d_zsplit = v.zs;
             P->k3d_waves = v.waves;
+            P->k3d_xcd = v.xcd;
         }
     };
     for (const Var &v : cand) {
         set(v);
         int rc = launch();  // warm
         if (rc) {
-            set({saved, saved_z, saved_w});
+            set(saved);
             return rc;
         }
         WH_HIP(hipEventRecord(P->ev_a, P->stream));
         for (int r = 0; r < reps; ++r)
             if ((rc = launch())) {
-                set({saved, saved_z, saved_w});
+                set(saved);
                 return rc;
             }
         WH_HIP(hipEventRecord(P->ev_b, P->stream));

@@ -143,6 +143,7 @@ struct wh_pool {
     int k3d_ysplit = -1;
     int k3d_zsplit = 1;
     int k3d_waves = 1;  // 3: the z-marching kernel built for 3 waves per SIMD
+    int k3d_xcd = 0;    // 1: the flat 3D kernel's workgroups swizzled per XCD
     // tuning of the 2D stage kernel: 0 plain, 1 mask and epilogue operands
     // loaded before the tile barrier, 2..4 persistent pipelined kernel with
     // 2, 4, 8 workgroups per CU

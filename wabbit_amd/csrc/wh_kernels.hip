@@ -380,8 +380,21 @@ __global__ __launch_bounds__(256) void k_rhs2d_pipe(const double *__restrict__ i
     }
 }
 
-// 3D: 256 interior points per workgroup, stencil reads served by L1/L2.
-template <int S, bool SKEW, bool FUSED>
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin:
+// workgroup w runs on XCD w % 8.  The swizzle gives XCD x the consecutive
+// logical indices [start(x), start(x) + count(x)), so the workgroups of one
+// block share an L2 instead of being spread over all eight.  A permutation
+// of [0, nw) for any nw.
+__device__ __forceinline__ int xcd_swizzle(int w, int nw)
+{
+    constexpr int NX = 8;
+    const int q = nw / NX, r = nw - q * NX, x = w % NX;
+    return x * q + min(x, r) + w / NX;
+}
+
+// 3D: 256 interior points per workgroup, stencil reads served by L1/L2;
+// XCD: workgroup indices swizzled so that a block's workgroups share an L2.
+template <int S, bool SKEW, bool FUSED, bool XCD>
 __global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, double *__restrict__ out,
                                                const double *__restrict__ mask,
                                                const int32_t *__restrict__ act,
@@ -390,8 +403,9 @@ __global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, do
 {
     const int npts = G.bs0 * G.bs1 * G.bs2;
     const int nchunk = (npts + 255) / 256;
-    const int k = blockIdx.x / nchunk;
-    const int pnt = (blockIdx.x - k * nchunk) * 256 + threadIdx.x;
+    const int wg = XCD ? xcd_swizzle((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int k = wg / nchunk;
+    const int pnt = (wg - k * nchunk) * 256 + threadIdx.x;
     if (pnt >= npts) return;
     const int b = act[k];
     const int ix = pnt % G.bs0, iy = (pnt / G.bs0) % G.bs1, iz = pnt / (G.bs0 * G.bs1);
@@ -672,9 +686,15 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
     } else {
         const int npts = G.bs0 * G.bs1 * G.bs2;
         dim3 grid(nact * ((npts + 255) / 256)), blockd(256);
-#define L3D(SC, SK)                                                                           \
-    hipLaunchKernelGGL((k_rhs3d<SC, SK, FUSED>), grid, blockd, 0, P->stream, in, out, mask, act, \
-                       x0dx, G, a, sa)
+#define L3D(SC, SK)                                                                                  \
+    do {                                                                                              \
+        if (P->k3d_xcd)                                                                               \
+            hipLaunchKernelGGL((k_rhs3d<SC, SK, FUSED, true>), grid, blockd, 0, P->stream, in, out, mask, \
+                               act, x0dx, G, a, sa);                                                  \
+        else                                                                                          \
+            hipLaunchKernelGGL((k_rhs3d<SC, SK, FUSED, false>), grid, blockd, 0, P->stream, in, out,  \
+                               mask, act, x0dx, G, a, sa);                                            \
+    } while (0)
         switch (a.scheme * 2 + a.skew) {
         case WH_FD2 * 2: L3D(WH_FD2, false); break;
         case WH_FD2 * 2 + 1: L3D(WH_FD2, true); break;
@@ -909,11 +929,13 @@ __global__ __launch_bounds__(256) void k_refine3d(double *__restrict__ arr,
                               150.0 / 256.0, -25.0 / 256.0, 3.0 / 256.0};
     const double *cf = (N == 1) ? c2 : (N == 2) ? c4 : c6;
     // `wpc` workgroups per (pair, component): the pair and component are a
-    // scalar division of the workgroup index
+    // scalar division of the (XCD-swizzled: the daughters of a mother, listed
+    // together, read it through one L2) workgroup index
     const int ncol = nx * ny;
-    const int qc = (int)blockIdx.x / wpc;
+    const int wg = xcd_swizzle((int)blockIdx.x, (int)gridDim.x);
+    const int qc = wg / wpc;
     const int q = qc / neq, c = qc - q * neq;
-    const int r = ((int)blockIdx.x - qc * wpc) * (int)blockDim.x + (int)threadIdx.x;
+    const int r = (wg - qc * wpc) * (int)blockDim.x + (int)threadIdx.x;
     if (r >= ncol) return;
     const int j = r / nx, i = r - j * nx;
     const int mother = pairs[3 * q], daughter = pairs[3 * q + 1], dg = pairs[3 * q + 2];
