@@ -324,6 +324,7 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
     P->ny = d.bs[1] + 2 * d.g;
     P->nz = d.dim == 3 ? d.bs[2] + 2 * d.g : 1;
     P->cs = (size_t)P->nx * P->ny * P->nz;
+    P->sync_pad = d.dim == 3 ? 64 : 1;
     P->blk = P->cs * d.neq;
     P->tblk = P->cs * d.n_tmp_eqn;
     P->mblk = P->cs * d.n_mask;
@@ -564,7 +565,7 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
             ps.h_recv_ids.push_back(q.recv);
         }
         ps.max_stage = std::max(ps.max_stage, (int)q.stage);
-        ps.h_pstart[i + 1] = ps.h_pstart[i] + (int64_t)q.n[0] * q.n[1] * q.n[2];
+        ps.h_pstart[i + 1] = ps.h_pstart[i] + sync_slots(P, q);
     }
     {  // stage ranges over the (stage, receiver)-sorted table
         int i = 0;

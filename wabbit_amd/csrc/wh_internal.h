@@ -144,6 +144,7 @@ struct wh_pool {
     int k3d_zsplit = 1;
     int k3d_waves = 1;  // 3: the z-marching kernel built for 3 waves per SIMD
     int k3d_xcd = 0;    // 1: the flat 3D kernel's workgroups swizzled per XCD
+    int sync_pad = 1;   // ghost-fill slots per patch rounded up to this (1 or 64), sync_slots
     // tuning of the 2D stage kernel: 0 plain, 1 mask and epilogue operands
     // loaded before the tile barrier, 2..4 persistent pipelined kernel with
     // 2, 4, 8 workgroups per CU
@@ -183,6 +184,13 @@ int launch_dt_finalize(wh_pool *P, const TimeDev &tc);
 int launch_time_advance(wh_pool *P);
 int launch_set_scalar(wh_pool *P, int which, double v);
 
+// slots of a patch in the ghost fill's numbering: its point count, rounded
+// up to whole wavefronts where the pool pads (3D)
+inline int64_t sync_slots(const wh_pool *P, const wh_patch &q)
+{
+    const int64_t n = (int64_t)q.n[0] * q.n[1] * q.n[2];
+    return P->sync_pad > 1 ? (n + P->sync_pad - 1) / P->sync_pad * P->sync_pad : n;
+}
 int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const int64_t *d_pstart,
                      int p0, int p1, int64_t npts, const SyncSrc &S);
 

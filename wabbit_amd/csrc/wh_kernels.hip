@@ -789,9 +789,15 @@ __device__ __forceinline__ double pred_any(const double *u, int nx, int ny, int 
 }
 
 // Flat ghost fill: one thread per ghost point of the patches [p0, p1); the
-// patch of a point is found by binary search in the point-count prefix sum.
+// patch of a point is found by binary search in the prefix sum of the
+// patches' slot counts.  WAVE (slot counts padded to whole wavefronts of 64,
+// sync_slots): every wavefront lies in one patch, found once per wavefront
+// and read through the scalar unit; lanes past the patch's points idle.
+// Otherwise (slot count = point count) each lane searches between the
+// patches of its workgroup's first and last points.
 // Bits 8-9 of a patch's op select the array its sender is read from (0: the
 // synced array itself, 1/2: the alternates in S).
+template <bool WAVE>
 __global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
                                                    const wh_patch *__restrict__ pt,
                                                    const int64_t *__restrict__ pstart, int p0,
@@ -807,18 +813,30 @@ __global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
         }
         return lo;
     };
-    // the workgroup's first and last points and their patches (uniform),
-    // then each lane's patch between those two
-    const int64_t w0 = pstart[p0] + (int64_t)blockIdx.x * blockDim.x;
-    const int64_t w1 = min(w0 + (int64_t)blockDim.x, pstart[p1]) - 1;
-    const int pa = find(w0, p0, p1 - 1);
-    const int pb = find(w1, pa, p1 - 1);
-    const int64_t e = w0 + threadIdx.x;
-    if (e > w1) return;
-    const int lo = find(e, pa, pb);
+    int lo;
+    int64_t e;
+    if constexpr (WAVE) {
+        const int64_t wv = pstart[p0] + (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+        const int64_t ew = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(wv >> 32)) << 32) |
+                                     (uint32_t)__builtin_amdgcn_readfirstlane((int)wv));
+        if (ew >= pstart[p1]) return;
+        lo = find(ew, p0, p1 - 1);
+        e = ew + (threadIdx.x & 63u);
+    } else {
+        // the workgroup's first and last points and their patches (uniform),
+        // then each lane's patch between those two
+        const int64_t w0 = pstart[p0] + (int64_t)blockIdx.x * blockDim.x;
+        const int64_t w1 = min(w0 + (int64_t)blockDim.x, pstart[p1]) - 1;
+        const int pa = find(w0, p0, p1 - 1);
+        const int pb = find(w1, pa, p1 - 1);
+        e = w0 + threadIdx.x;
+        if (e > w1) return;
+        lo = find(e, pa, pb);
+    }
     const wh_patch P = pt[lo];
     const int r = (int)(e - pstart[lo]);
     const int n0 = P.n[0], n1 = P.n[1];
+    if (WAVE && r >= n0 * n1 * P.n[2]) return;  // padding slot
     const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
     const size_t ri =
         ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx + (P.recv_lo[0] + i);
@@ -844,8 +862,12 @@ int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const i
 {
     if (p1 <= p0 || npts <= 0) return WH_OK;
     const unsigned nblk = (unsigned)((npts + 255) / 256);
-    hipLaunchKernelGGL(k_sync_flat, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
-                       p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
+    if (P->sync_pad == 64)
+        hipLaunchKernelGGL(k_sync_flat<true>, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
+                           p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
+    else
+        hipLaunchKernelGGL(k_sync_flat<false>, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
+                           p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -1025,9 +1047,7 @@ int launch_sync(wh_pool *P, int sync_case, double *arr, int one_receiver)
         const int p0 = ps.stage_begin[stage], p1 = ps.stage_begin[stage + 1];
         if (p1 <= p0) continue;
         const int64_t npts = ps.h_pstart[p1] - ps.h_pstart[p0];
-        const unsigned nblk = (unsigned)((npts + 255) / 256);
-        hipLaunchKernelGGL(k_sync_flat, dim3(nblk), dim3(256), 0, P->stream, arr, ps.d_patches,
-                           ps.d_pstart, p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, SyncSrc{});
+        if (int rc = launch_sync_flat(P, arr, ps.d_patches, ps.d_pstart, p0, p1, npts, SyncSrc{})) return rc;
     }
     WH_HIP(hipGetLastError());
     return WH_OK;

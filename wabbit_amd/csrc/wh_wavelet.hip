@@ -1131,7 +1131,7 @@ int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<
         }
     }
     std::vector<int64_t> pstart(n + 1, 0);
-    for (int i = 0; i < n; ++i) pstart[i + 1] = pstart[i] + (int64_t)p[i].n[0] * p[i].n[1] * p[i].n[2];
+    for (int i = 0; i < n; ++i) pstart[i + 1] = pstart[i] + sync_slots(P, p[i]);
     if (int rc = stage_begin(P, sizeof(wh_patch) * n + sizeof(int64_t) * (n + 1))) return rc;
     const wh_patch *d_patches = (const wh_patch *)stage_put(P, p.data(), sizeof(wh_patch) * n);
     const int64_t *d_pstart = (const int64_t *)stage_put(P, pstart.data(), sizeof(int64_t) * (n + 1));
