@@ -1007,9 +1007,143 @@ __global__ __launch_bounds__(256) void k_refine3d(double *__restrict__ arr,
     }
 }
 
+// 3D refineBlock, one workgroup per (pair, component), marching through the
+// mother's coarse planes: per plane kz the coarse rows and columns the
+// daughter reaches are staged in LDS (A), the x interpolation of every
+// staged row at every fine x goes to LDS (B; pred_point's vx), and each
+// thread takes the y interpolation (vy) of its own fine (x, y) points into a
+// register window of the last 2N planes.  When the window ends at plane
+// cz + N, the fine planes 2cz and 2cz + 1 are complete: the even one is the
+// window's centre plane, the odd one the left-to-right z sum over the window
+// -- pred_point's arithmetic in its order.  Coarse points outside the mother
+// are staged as zero; the host's stencil-range check guarantees no output
+// reads them.  PPT: fine (x, y) points per thread.
+template <int N, int PPT>
+__global__ __launch_bounds__(256) void k_refine3d_lds(double *__restrict__ arr,
+                                                      const int32_t *__restrict__ pairs, int bs0, int bs1,
+                                                      int bs2, int g, int nx, int ny, int nz, size_t blk,
+                                                      size_t cs, int neq)
+{
+    constexpr double c2[2] = {0.5, 0.5};
+    constexpr double c4[4] = {-1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0};
+    constexpr double c6[6] = {3.0 / 256.0, -25.0 / 256.0, 150.0 / 256.0,
+                              150.0 / 256.0, -25.0 / 256.0, 3.0 / 256.0};
+    const double *cf = (N == 1) ? c2 : (N == 2) ? c4 : c6;
+    extern __shared__ double lds[];
+    const int wg = xcd_swizzle((int)blockIdx.x, (int)gridDim.x);
+    const int q = wg / neq, c = wg - q * neq;
+    const int mother = pairs[3 * q], daughter = pairs[3 * q + 1], dg = pairs[3 * q + 2];
+    const int fx0 = g + bs0 * ((dg >> 1) & 1), fy0 = g + bs1 * (dg & 1), fz0 = g + bs2 * ((dg >> 2) & 1);
+    // staged coarse window: columns [cxa, cxa + ncol), rows [cya, cya + nrow)
+    const int cxa = (fx0 >> 1) - N + 1, ncol = ((fx0 + nx - 1) >> 1) + N - cxa + 1;
+    const int cya = (fy0 >> 1) - N + 1, nrow = ((fy0 + ny - 1) >> 1) + N - cya + 1;
+    double *A = lds, *B = lds + nrow * ncol;
+    const double *u = arr + (size_t)mother * blk + (size_t)c * cs;
+    double *dst = arr + (size_t)daughter * blk + (size_t)c * cs;
+    const int nxy = nx * ny, tid = threadIdx.x;
+    double win[PPT][2 * N];
+    const int kz0 = (fz0 >> 1) - N + 1, kz1 = ((fz0 + nz - 1) >> 1) + N;
+    for (int kz = kz0; kz <= kz1; ++kz) {
+        const bool inz = kz >= 0 && kz < nz;
+        __syncthreads();  // the previous plane's B reads are done
+        for (int e = tid; e < nrow * ncol; e += 256) {
+            const int r = e / ncol, cc = e - r * ncol;
+            const int y = cya + r, x = cxa + cc;
+            A[e] = (inz && y >= 0 && y < ny && x >= 0 && x < nx) ? u[((size_t)kz * ny + y) * nx + x] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < nrow * nx; e += 256) {
+            const int r = e / nx, i = e - r * nx;
+            const int f0 = fx0 + i, cx = (f0 >> 1) - cxa;
+            const double *row = A + r * ncol;
+            double v;
+            if (f0 & 1) {
+                v = cf[0] * row[cx - N + 1];
+#pragma unroll
+                for (int t = 1; t < 2 * N; ++t) v = v + cf[t] * row[cx - N + 1 + t];
+            } else {
+                v = row[cx];
+            }
+            B[e] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pp = 0; pp < PPT; ++pp) {
+            const int pt = tid + 256 * pp;
+            double v = 0.0;
+            if (pt < nxy) {
+                const int j = pt / nx, i = pt - j * nx;
+                const int f1 = fy0 + j, cy = (f1 >> 1) - cya;
+                if (f1 & 1) {
+                    v = cf[0] * B[(cy - N + 1) * nx + i];
+#pragma unroll
+                    for (int t = 1; t < 2 * N; ++t) v = v + cf[t] * B[(cy - N + 1 + t) * nx + i];
+                } else {
+                    v = B[cy * nx + i];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 2 * N - 1; ++t) win[pp][t] = win[pp][t + 1];
+            win[pp][2 * N - 1] = v;
+        }
+        // fine planes whose window ends here
+        const int cz = kz - N;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int f2 = 2 * cz + h, k = f2 - fz0;
+            if (k < 0 || k >= nz) continue;
+#pragma unroll
+            for (int pp = 0; pp < PPT; ++pp) {
+                const int pt = tid + 256 * pp;
+                if (pt >= nxy) continue;
+                double v;
+                if (h) {
+                    v = cf[0] * win[pp][0];
+#pragma unroll
+                    for (int t = 1; t < 2 * N; ++t) v = v + cf[t] * win[pp][t];
+                } else {
+                    v = win[pp][N - 1];
+                }
+                dst[(size_t)k * nxy + pt] = v;
+            }
+        }
+    }
+}
+
+// LDS bytes of k_refine3d_lds for this pool and order, 0 if it does not apply
+static size_t refine3d_lds_bytes(const wh_pool *P, int N)
+{
+    if (P->nx * P->ny > 8 * 256) return 0;
+    const int ncol = P->nx / 2 + 2 * N + 2, nrow = P->ny / 2 + 2 * N + 2;  // upper bounds
+    const size_t b = sizeof(double) * ((size_t)nrow * ncol + (size_t)nrow * P->nx);
+    return b <= 64 * 1024 ? b : 0;
+}
+
 int launch_refine(wh_pool *P, double *arr, const int32_t *d_pairs, int npairs, int order)
 {
     if (npairs <= 0) return WH_OK;
+    if (P->d.dim == 3 && refine3d_lds_bytes(P, order / 2) > 0) {
+        const int N = order / 2, ppt = (P->nx * P->ny + 255) / 256;
+        const size_t lds = refine3d_lds_bytes(P, N);
+        const dim3 grid((unsigned)((size_t)P->d.neq * npairs));
+#define LRL(NN, PP)                                                                                    \
+    hipLaunchKernelGGL((k_refine3d_lds<NN, PP>), grid, dim3(256), lds, P->stream, arr, d_pairs, P->d.bs[0], \
+                       P->d.bs[1], P->d.bs[2], P->d.g, P->nx, P->ny, P->nz, P->blk, P->cs, P->d.neq)
+#define LRN(NN)                        \
+    do {                               \
+        if (ppt <= 2) LRL(NN, 2);      \
+        else if (ppt <= 4) LRL(NN, 4); \
+        else if (ppt <= 6) LRL(NN, 6); \
+        else LRL(NN, 8);               \
+    } while (0)
+        if (N == 2) LRN(2);
+        else if (N == 1) LRN(1);
+        else LRN(3);
+#undef LRN
+#undef LRL
+        WH_HIP(hipGetLastError());
+        return WH_OK;
+    }
     if (P->d.dim == 3) {
         const int wpc = (P->nx * P->ny + 255) / 256;
         const dim3 grid((unsigned)((size_t)wpc * P->d.neq * npairs));
