@@ -153,10 +153,14 @@ class AdaptiveACM:
             it += 1
         return it
 
-    def autotune_next_step(self):
+    def autotune_next_step(self, regrow: float = 2.0):
         """Pick the stage-kernel variant on the next step's refined grid
-        (the live blocks of a real RK stage; all variants are bit-identical)."""
+        (the live blocks of a real RK stage; all variants are bit-identical),
+        and again whenever the refined grid has grown by the factor
+        ``regrow`` since the last pick (the fastest variant depends on the
+        number of blocks per launch)."""
         self._tune = True
+        self._regrow = regrow
 
     def step(self):
         """One main-loop iteration; returns dt.  ``rk_blocks`` holds the
@@ -166,9 +170,11 @@ class AdaptiveACM:
         t = self._tick(t, "sync_tree")
         self.refine()
         self.rk_blocks = self.forest.n
-        if getattr(self, "_tune", False):
+        tuned_at = getattr(self, "_tuned_at", None)
+        if getattr(self, "_tune", False) or (tuned_at and self.rk_blocks >= self._regrow * tuned_at):
             self._tune = False
             self.tuned = self.pool.autotune(self.acm_c, reps=3)
+            self._tuned_at = self.rk_blocks
         t = self._tick(t, "refine")
         if self.mask_p is not None:
             W.create_mask(self.pool, self.mask_p)
