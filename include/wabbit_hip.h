@@ -352,7 +352,8 @@ int wh_sync_tree(wh_pool *pool, wh_forest *forest, int array);
  * pieces marched by separate workgroups (a split that does not divide Bs2
  * into pieces of at least 2R planes, R the stencil reach, acts as 1).
  * WH_TUNE_K3D_WAVES (default 1): 3 selects the build of the z-marching
- * kernel whose register allocation allows 3 waves per SIMD.
+ * kernel whose register allocation allows 3 waves per SIMD, 4 the build of
+ * the flat kernel allowing 4 (both may spill a few registers).
  * WH_TUNE_K3D_XCD (default 0): 1 remaps the flat 3D kernel's workgroups so
  * that those of one block run on one XCD (one L2).
  * WH_TUNE_K2D_STAGE: 2D stage kernel variant: 0 one workgroup per block;

@@ -223,7 +223,7 @@ def test_divergence_guard_and_errors():
 
 
 @pytest.mark.parametrize("scheme", [4, 6])
-@pytest.mark.parametrize("ysplit,zsplit,waves,xcd", [(0, 1, 1, 0), (0, 1, 1, 1), (1, 1, 1, 0), (2, 1, 1, 0),
+@pytest.mark.parametrize("ysplit,zsplit,waves,xcd", [(0, 1, 1, 0), (0, 1, 1, 1), (0, 1, 4, 1), (1, 1, 1, 0), (2, 1, 1, 0),
                                                      (3, 1, 1, 0), (2, 2, 1, 0), (3, 4, 1, 0), (2, 3, 3, 0),
                                                      (3, 1, 3, 0)])
 def test_3d_kernel_variants_identical(scheme, ysplit, zsplit, waves, xcd):

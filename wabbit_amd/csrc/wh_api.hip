@@ -898,7 +898,7 @@ int wh_set_tuning(wh_pool *P, int key, int value)
         return WH_OK;
     }
     if (key == WH_TUNE_K3D_WAVES) {
-        WH_CHECK(value == 1 || value == 3, WH_E_ARG, "K3D_WAVES is 1 or 3");
+        WH_CHECK(value == 1 || value == 3 || value == 4, WH_E_ARG, "K3D_WAVES is 1, 3 or 4");
         P->k3d_waves = value;
         P->topo_version++;
         return WH_OK;
@@ -957,8 +957,8 @@ int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, d
     std::vector<Var> cand;
     const bool three = P->d.dim == 3;
     if (three) {
-        cand.push_back({0, 1, 1, 0});
-        cand.push_back({0, 1, 1, 1});
+        for (int xcd : {0, 1})
+            for (int wv : {1, 4}) cand.push_back({0, 1, wv, xcd});
         for (int ys = 1; ys <= 8; ++ys)
             for (int zs = 1; zs <= 8; ++zs)
                 if (zm_valid(P->d.bs[0], P->d.bs[1], p->scheme, ys) && zm_zvalid(P->d.bs[2], p->scheme, zs))

@@ -142,7 +142,7 @@ struct wh_pool {
     // tuning: y-split of the 3D stage kernel (-1 auto, 0 flat kernel)
     int k3d_ysplit = -1;
     int k3d_zsplit = 1;
-    int k3d_waves = 1;  // 3: the z-marching kernel built for 3 waves per SIMD
+    int k3d_waves = 1;  // 3: the z-marching kernel built for 3 waves per SIMD, 4: the flat one for 4
     int k3d_xcd = 0;    // 1: the flat 3D kernel's workgroups swizzled per XCD
     int sync_pad = 1;   // ghost-fill slots per patch rounded up to this (1 or 64), sync_slots
     // tuning of the 2D stage kernel: 0 plain, 1 mask and epilogue operands

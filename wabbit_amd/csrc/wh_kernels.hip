@@ -393,9 +393,11 @@ __device__ __forceinline__ int xcd_swizzle(int w, int nw)
 }
 
 // 3D: 256 interior points per workgroup, stencil reads served by L1/L2;
-// XCD: workgroup indices swizzled so that a block's workgroups share an L2.
-template <int S, bool SKEW, bool FUSED, bool XCD>
-__global__ __launch_bounds__(256) void k_rhs3d(const double *__restrict__ in, double *__restrict__ out,
+// XCD: workgroup indices swizzled so that a block's workgroups share an L2;
+// WPE: minimum waves per SIMD of the register allocation (4 spills a few
+// registers to scratch).
+template <int S, bool SKEW, bool FUSED, bool XCD, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rhs3d(const double *__restrict__ in, double *__restrict__ out,
                                                const double *__restrict__ mask,
                                                const int32_t *__restrict__ act,
                                                const double *__restrict__ x0dx, Geo G, AcmDev a,
@@ -664,7 +666,7 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
         dim3 grid(nact * ys * zs), blockd(G.bs0 * (G.bs1 / ys));
 #define L3D(SC, SK)                                                                                    \
     do {                                                                                                \
-        if (P->k3d_waves >= 3)                                                                          \
+        if (P->k3d_waves == 3)                                                                          \
             hipLaunchKernelGGL((k_rhs3d_zm<SC, SK, FUSED, 3>), grid, blockd, lds, P->stream, in, out, mask, \
                                act, x0dx, G, a, sa, ys, zs);                                            \
         else                                                                                            \
@@ -686,14 +688,16 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
     } else {
         const int npts = G.bs0 * G.bs1 * G.bs2;
         dim3 grid(nact * ((npts + 255) / 256)), blockd(256);
-#define L3D(SC, SK)                                                                                  \
-    do {                                                                                              \
-        if (P->k3d_xcd)                                                                               \
-            hipLaunchKernelGGL((k_rhs3d<SC, SK, FUSED, true>), grid, blockd, 0, P->stream, in, out, mask, \
-                               act, x0dx, G, a, sa);                                                  \
-        else                                                                                          \
-            hipLaunchKernelGGL((k_rhs3d<SC, SK, FUSED, false>), grid, blockd, 0, P->stream, in, out,  \
-                               mask, act, x0dx, G, a, sa);                                            \
+#define L3F(SC, SK, X, W)                                                                          \
+    hipLaunchKernelGGL((k_rhs3d<SC, SK, FUSED, X, W>), grid, blockd, 0, P->stream, in, out, mask, act, \
+                       x0dx, G, a, sa)
+#define L3D(SC, SK)                                            \
+    do {                                                        \
+        const bool w4 = P->k3d_waves == 4;                      \
+        if (P->k3d_xcd && w4) L3F(SC, SK, true, 4);             \
+        else if (P->k3d_xcd) L3F(SC, SK, true, 1);              \
+        else if (w4) L3F(SC, SK, false, 4);                     \
+        else L3F(SC, SK, false, 1);                             \
     } while (0)
         switch (a.scheme * 2 + a.skew) {
         case WH_FD2 * 2: L3D(WH_FD2, false); break;
@@ -706,6 +710,7 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
         case WH_FD6 * 2 + 1: L3D(WH_FD6, true); break;
         default: set_error("unknown scheme"); return WH_E_ARG;
         }
+#undef L3F
 #undef L3D
     }
     WH_HIP(hipGetLastError());
