@@ -21,14 +21,11 @@ def _pool(dim, bs, g, neq, nb):
     return L.Pool(dim, [bs] * dim, g, neq, nb, n_work_slots=2, n_tmp_eqn=neq)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("wav", WAVS)
-def test_block_transforms_match_oracle(wav, dim):
+def _block_transforms(wav, dim, bs, n=5):
     w = W.setup(wav)
     wo = po.wavelet(wav)
     g = w.g
-    bs = 16 if dim == 2 else 12
-    n, neq = 5, 2
+    neq = 2
     rng = np.random.default_rng(11)
     nz = bs + 2 * g if dim == 3 else 1
     shape = (n, neq, nz, bs + 2 * g, bs + 2 * g)
@@ -36,7 +33,7 @@ def test_block_transforms_match_oracle(wav, dim):
     prior = rng.standard_normal(shape)
     pool = _pool(dim, bs, g, neq, n + 2)
     W.set_wavelet(pool, w)
-    ids = np.array([3, 0, 4, 1, 2], np.int32)   # a permuted list
+    ids = np.array([3, 0, 4, 1, 2] if n == 5 else list(range(n - 1, -1, -1)), np.int32)   # a permuted list
     for op, ofn in (("fwt", po.fwt_block), ("iwt", po.iwt_block), ("filter", None)):
         pool.upload(L.WH_TMP, 0, src)
         pool.upload(L.WH_BLOCK, 0, prior)
@@ -49,6 +46,25 @@ def test_block_transforms_match_oracle(wav, dim):
                 want = ofn(wo, dim, [bs] * dim, g, src[b], prior[b])
             assert np.array_equal(got[b], want), (op, b)
     pool.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("wav", WAVS + ["CDF64", "CDF66"])
+def test_block_transforms_match_oracle(wav, dim):
+    """FWT, IWT and the HD filter of a permuted block list, bit for bit.
+    CDF64/CDF66 are the wavelets whose reconstruction is the general chain
+    with breaks (two sums added), and whose decomposition chains are longer
+    than the unrolled kernel variants."""
+    _block_transforms(wav, dim, 16 if dim == 2 else 12)
+
+
+@pytest.mark.parametrize("wav,dim,bs", [("CDF20", 2, 64), ("CDF44", 2, 64), ("CDF66", 2, 60),
+                                        ("CDF44", 3, 54)])
+def test_block_transforms_large_blocks(wav, dim, bs):
+    """Blocks whose planes do not fit the fused x/y kernel's 64 KB of LDS
+    take the pass-per-launch path (x pass per output point, y and z passes
+    in place through the column ring): bit for bit as well."""
+    _block_transforms(wav, dim, bs, n=3 if dim == 3 else 5)
 
 
 @pytest.mark.parametrize("dim", [2, 3])

@@ -499,12 +499,14 @@ __global__ __launch_bounds__(256) void k_sepxy(const double *__restrict__ in, do
 // reads the values before the pass, as the pass-then-copy-back version does.
 // Columns of a wavefront are consecutive x positions (coalesced); the output
 // position, and with it the chain, is uniform.
-constexpr int WRING = 16, WRING_LANES = 256;
-template <int KIND, int NT>
-__global__ __launch_bounds__(WRING_LANES) void k_wring(double *__restrict__ arr, const int32_t *__restrict__ ids,
-                                                       int wpb, ArrGeo A, Region R, int axis, int len, int astride,
-                                                       int g, int lo_off, int hi_off, TermTable T)
+// WRING slots per column (16, or 32 for the longest chains), 4096 / WRING
+// columns per workgroup: 32 KB of LDS either way.
+template <int KIND, int NT, int WRING>
+__global__ __launch_bounds__(256) void k_wring(double *__restrict__ arr, const int32_t *__restrict__ ids,
+                                               int wpb, ArrGeo A, Region R, int axis, int len, int astride,
+                                               int g, int lo_off, int hi_off, TermTable T)
 {
+    constexpr int WRING_LANES = 4096 / WRING;
     __shared__ double ring[WRING * WRING_LANES];  // [slot][lane]
     // the column runs along `axis`; x and the remaining direction (z for a
     // y pass, y for a z pass) number the columns
@@ -734,8 +736,6 @@ int launch_wcol(wh_pool *P, double *arr, size_t blk, int nb, const Region &R, in
 {
     const int ncol = R.n[0] * (axis == 1 ? R.n[2] : R.n[1]);
     if (ncol <= 0 || nb <= 0) return WH_OK;
-    const int wpb = (ncol + WRING_LANES - 1) / WRING_LANES;
-    WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const int len = axis == 1 ? P->ny : P->nz;
     const int astride = axis == 1 ? P->nx : P->nx * P->ny;
     // every point a chain reads lies in the column, and the chains' offsets
@@ -750,12 +750,22 @@ int launch_wcol(wh_pool *P, double *arr, size_t blk, int nb, const Region &R, in
             hi_off = std::max(hi_off, T.off[par][k]);
         }
     }
-    WH_CHECK(hi_off - lo_off + 1 <= WRING, WH_E_ARG, "wavelet pass: filter longer than the column ring");
+    const int span = hi_off - lo_off + 1;
+    WH_CHECK(span <= 32, WH_E_ARG, "wavelet pass: filter longer than the column ring");
+    const int ring = span <= 16 ? 16 : 32, lanes = 4096 / ring;
+    const int wpb = (ncol + lanes - 1) / lanes;
+    WH_CHECK((size_t)wpb * P->d.neq * nb < (1u << 31), WH_E_ARG, "wavelet pass: too many blocks in one launch");
     const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, P->d.neq};
     const dim3 grid((unsigned)((size_t)wpb * P->d.neq * nb));
-#define LWR(K, N)                                                                                             \
-    hipLaunchKernelGGL((k_wring<K, N>), grid, dim3(WRING_LANES), 0, P->stream, arr, P->ws.cur_ids, wpb, A, R, \
-                       axis, len, astride, P->d.g, lo_off, hi_off, T)
+#define LWR(K, N)                                                                                               \
+    do {                                                                                                         \
+        if (ring == 16)                                                                                          \
+            hipLaunchKernelGGL((k_wring<K, N, 16>), grid, dim3(256), 0, P->stream, arr, P->ws.cur_ids, wpb, A, R, \
+                               axis, len, astride, P->d.g, lo_off, hi_off, T);                                  \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_wring<K, N, 32>), grid, dim3(128), 0, P->stream, arr, P->ws.cur_ids, wpb, A, R, \
+                               axis, len, astride, P->d.g, lo_off, hi_off, T);                                  \
+    } while (0)
     CHAIN_DISPATCH(chain_var(T), LWR);
 #undef LWR
     WH_HIP(hipGetLastError());
