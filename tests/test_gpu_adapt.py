@@ -133,3 +133,38 @@ def test_rk4_on_nonuniform_forest(dim, path):
     print("bit-identical:", np.array_equal(gi, wi))
     pool.close()
     f.close()
+
+
+@pytest.mark.parametrize("order,g,bs", [(2, 2, 8), (4, 3, 12), (6, 5, 12), (4, 6, 24)])
+def test_refine_blocks_3d_matches_oracle(order, g, bs):
+    """refineBlock in 3D (wh_refine_blocks, the LDS plane-marching kernel)
+    against the oracle's refineBlock (refinementExecute.f90:4-116): every
+    daughter of every mother bit for bit, for the three predictor orders,
+    daughters listed in a scrambled order of mothers."""
+    import ctypes
+    rng = np.random.default_rng(order * 100 + bs)
+    nm, neq, nd = 3, 2, 8
+    n = bs + 2 * g
+    moms = rng.standard_normal((nm, neq, n, n, n))
+    nb = nm * (nd + 1)
+    pool = L.Pool(3, [bs] * 3, g, neq, nb, n_work_slots=0)
+    pool.upload(L.WH_BLOCK, 0, moms)
+    tri = []
+    for m in (2, 0, 1):
+        for k in (3, 0, 7, 1, 6, 2, 5, 4):
+            tri.append((m, nm + m * nd + k, k))
+    t = np.ascontiguousarray(np.array(tri, np.int32))
+    L.check(L.load().wh_refine_blocks(pool.h, L.WH_BLOCK, t.ctypes.data, t.shape[0], order))
+    got = pool.download(L.WH_BLOCK, 0, nb)
+    pool.close()
+    lib = po.lib()
+    for m in range(nm):
+        daughters = [np.zeros((neq, n, n, n)) for _ in range(nd)]
+        arr = (ctypes.POINTER(ctypes.c_double) * nd)(
+            *[d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for d in daughters])
+        mm = np.ascontiguousarray(moms[m])
+        bsa = np.array([bs] * 3, np.int32)
+        lib.or_refine_block(3, bsa.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), g, neq,
+                            mm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), arr, order)
+        for k in range(nd):
+            assert np.array_equal(got[nm + m * nd + k], daughters[k]), (m, k)
