@@ -38,10 +38,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# bytes beyond L2 per block-stage of the fused 2D stage kernel on c2, from
-# the FETCH_SIZE / WRITE_SIZE passes scaled by the copy calibration
-# (profiles/c2_fused_stage_hbm_traffic.md)
-C2_STAGE_TRAFFIC_PER_BLOCK = 40638
+# bytes beyond L2 per block-stage of the fused 2D stage kernel on c2, per
+# stage-kernel variant (WH_TUNE_K2D_STAGE), from the FETCH_SIZE / WRITE_SIZE
+# passes scaled by the copy calibration (profiles/c2_fused_stage_hbm_traffic.md)
+C2_STAGE_TRAFFIC_PER_BLOCK = {1: 40638, 3: 40510}
 
 
 def b_min(q, bs, dim, g_r, m):
@@ -203,19 +203,28 @@ def run_adaptive(args, w):
     phases = {k: round(v / nprof, 4) for k, v in s.phase_ms.items()}
     s.profile(False)
     s.pool.timing(False)
+    # the stage kernel's launch duration: NREP RK4 stage sequences (4 NREP
+    # launches) back to back on the last step's refined grid, one HIP event
+    # pair around them (a per-launch event pair adds the dispatch gap to
+    # short kernels)
+    NREP = 5
+    ks_ms = s.pool.time_stage_kernel(s.acm_c, NREP)
     q, dim = w["dim"] + 1, w["dim"]
     # m: mask components the RHS reads (2D penalised + sponge: chi, u_s, colour, sponge)
     m = 5 if w["mask"] is not None else 0
     bpb = b_min(q, w["bs"], dim, 2, m)
-    achieved = bpb * st2 / (k_ms * k_n * 1e-3) / 1e9
-    bpl = st2 / max(k_n, 1)
-    traffic = round(C2_STAGE_TRAFFIC_PER_BLOCK * bpl) if w["kind"] == "adaptive" else None
+    bpl = s.rk_blocks
+    achieved = bpb * bpl / (ks_ms * 1e-3) / 1e9
+    variant = s.tuned[0] if getattr(s, "tuned", None) is not None else None
+    per_block = C2_STAGE_TRAFFIC_PER_BLOCK.get(variant) if w["kind"] == "adaptive" and dim == 2 else None
+    traffic = round(per_block * bpl) if per_block is not None else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(k_ms, 5), launches=k_n,
-                    bytes_per_block_stage=bpb, blocks_per_launch=round(bpl, 1))
+                    kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(ks_ms, 5), launches=4 * NREP,
+                    bytes_per_block_stage=bpb, blocks_per_launch=bpl,
+                    timing_pass_event_pair_ms=round(k_ms, 5), timing_pass_blocks_per_launch=round(st2 / max(k_n, 1), 1))
     if traffic is not None:
-        roofline["traffic_per_block_stage"] = C2_STAGE_TRAFFIC_PER_BLOCK
+        roofline["traffic_per_block_stage"] = per_block
 
     if getattr(s, "tuned", None) is not None:
         roofline["tuned_variant"] = s.tuned[0]
@@ -469,13 +478,18 @@ def main():
     k_ms, k_n = pool.kernel_time(0)
     s_ms, s_n = pool.kernel_time(1)
     pool.timing(False)
+    # the stage kernel back to back (NREP RK4 stage sequences, one HIP
+    # event pair around their 4 NREP launches)
+    NREP = 5
+    ks_ms = pool.time_stage_kernel(solver.acm_c, NREP)
     q = dim + 1
     bpb = b_min(q, w["bs"], dim, w["g_rhs"], 0)
-    achieved = bpb * forest.n / (k_ms * 1e-3) / 1e9
+    achieved = bpb * forest.n / (ks_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(k_ms, 5),
-                    launches=k_n, bytes_per_block_stage=bpb, blocks_per_launch=forest.n,
+                    kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(ks_ms, 5),
+                    launches=4 * NREP, bytes_per_block_stage=bpb, blocks_per_launch=forest.n,
+                    timing_pass_event_pair_ms=round(k_ms, 5),
                     sync_kernel_avg_ms=round(s_ms, 5), tuned_variant=tuned)
     if dim == 3:
         roofline["tuned_zsplit"] = pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT)

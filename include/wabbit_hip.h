@@ -382,6 +382,15 @@ int wh_get_tuning(wh_pool *pool, int key, int *value);
  * otherwise the RHS kernel and work slot 1. */
 int wh_autotune(wh_pool *pool, const wh_acm_params *p, int reps, int *best_value, double *best_ms);
 
+/* Average duration of one launch of the fused RK stage kernel (the tuned
+ * variant) over `reps` classic-RK4 stage sequences (first, two middle and
+ * last stage, 4*reps launches) back to back on the pool's stream, bracketed
+ * by one pair of HIP events, on the current active blocks: the stage
+ * kernel's time without the dispatch gaps a per-launch event pair adds.
+ * Reads the state, writes only work slots 0-2 (overwritten); needs three
+ * work slots.  No counterpart in the reference (measurement). */
+int wh_time_stage_kernel(wh_pool *pool, const wh_acm_params *p, int reps, double *ms);
+
 /* HIP-event timing of kernel kinds (0: RHS or fused stage kernel, 1: ghost
  * sync) on the pool's stream: average ms per launch since the last reset.
  * Timing serialises host and device; enable it only for measurement. */

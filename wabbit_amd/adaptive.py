@@ -171,7 +171,9 @@ class AdaptiveACM:
         self.refine()
         self.rk_blocks = self.forest.n
         tuned_at = getattr(self, "_tuned_at", None)
-        if getattr(self, "_tune", False) or (tuned_at and self.rk_blocks >= self._regrow * tuned_at):
+        regrown = (tuned_at and self.rk_blocks >= self._regrow * tuned_at
+                   and not getattr(self.pool, "timing_on", False))   # no re-tuning while kernels are timed
+        if getattr(self, "_tune", False) or regrown:
             self._tune = False
             self.tuned = self.pool.autotune(self.acm_c, reps=3)
             self._tuned_at = self.rk_blocks

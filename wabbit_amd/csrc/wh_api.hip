@@ -939,6 +939,42 @@ int wh_get_tuning(wh_pool *P, int key, int *value)
     return WH_OK;
 }
 
+int wh_time_stage_kernel(wh_pool *P, const wh_acm_params *p, int reps, double *ms)
+{
+    TablesUse tables_use{P};
+    WH_CHECK(P && p && ms, WH_E_ARG, "null argument");
+    if (int rc = check_acm(P, p)) return rc;
+    WH_CHECK(P->d.n_work_slots >= 3, WH_E_ARG, "stage-kernel timing needs 3 work slots");
+    WH_CHECK(P->nact > 0, WH_E_STATE, "stage-kernel timing needs an active list");
+    WH_CHECK(!P->timing, WH_E_STATE, "disable kernel timing first");
+    reps = std::max(1, std::min(reps, 1000));
+    const AcmDev a = make_acm_dev(p);
+    double *acc = array_ptr(P, WH_WORK + 0, nullptr, nullptr), *tout = array_ptr(P, WH_WORK + 1, nullptr, nullptr),
+           *sout = array_ptr(P, WH_WORK + 2, nullptr, nullptr);
+    // the four stages of classic RK4 (first, two middle, last), reading the
+    // state and writing only work slots
+    auto rk4 = [&]() -> int {
+        const double ca[4] = {0.5, 0.5, 1.0, 0.0}, cb[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+        const int mode[4] = {0, 1, 1, 2};
+        for (int j = 0; j < 4; ++j)
+            if (int rc = launch_fused_stage(P, a, P->block, mode[j] == 2 ? nullptr : tout,
+                                            mode[j] == 1 ? P->block : nullptr, acc, mode[j] == 2 ? sout : nullptr,
+                                            mode[j], ca[j], cb[j], nullptr))
+                return rc;
+        return WH_OK;
+    };
+    if (int rc = rk4()) return rc;  // warm
+    WH_HIP(hipEventRecord(P->ev_a, P->stream));
+    for (int r = 0; r < reps; ++r)
+        if (int rc = rk4()) return rc;
+    WH_HIP(hipEventRecord(P->ev_b, P->stream));
+    WH_HIP(hipEventSynchronize(P->ev_b));
+    float t = 0.f;
+    WH_HIP(hipEventElapsedTime(&t, P->ev_a, P->ev_b));
+    *ms = (double)t / (4.0 * reps);
+    return WH_OK;
+}
+
 int wh_autotune(wh_pool *P, const wh_acm_params *p, int reps, int *best_value, double *best_ms)
 {
     TablesUse tables_use{P};

@@ -100,6 +100,7 @@ def load(path: str = LIB_PATH):
         "wh_time_get": [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                         ctypes.POINTER(ctypes.c_double)],
         "wh_time_fetch": [_vp],
+        "wh_time_stage_kernel": [_vp, ctypes.POINTER(AcmParamsC), ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
         "wh_time_fetched": [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_double)],
         "wh_run_steps": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int,
@@ -275,6 +276,11 @@ class Pool:
         check(load().wh_time_get(self.h, ctypes.byref(t), ctypes.byref(it), ctypes.byref(dt)))
         return t.value, it.value, dt.value
 
+    def time_stage_kernel(self, acm_c, reps=20):
+        ms = ctypes.c_double()
+        check(load().wh_time_stage_kernel(self.h, ctypes.byref(acm_c), reps, ctypes.byref(ms)))
+        return ms.value
+
     def time_fetch(self):
         check(load().wh_time_fetch(self.h))
 
@@ -306,6 +312,7 @@ class Pool:
 
     def timing(self, on):
         check(load().wh_kernel_timing_enable(self.h, int(on)))
+        self.timing_on = bool(on)
         check(load().wh_kernel_timing(self.h, -1, None, None))
 
     def kernel_time(self, which):
