@@ -32,5 +32,6 @@ def test_fortran_interface_links_and_runs(tmp_path):
     assert f"sizeof(wh_wavelet)={ctypes.sizeof(W.WaveletC)}" in out
     assert f"sizeof(wh_adapt_params)={ctypes.sizeof(W.AdaptParamsC)}" in out
     assert f"sizeof(wh_mask_params)={ctypes.sizeof(W.MaskParamsC)}" in out
+    assert f"sizeof(wh_time_params)={ctypes.sizeof(L.TimeParamsC)}" in out
     assert "CDF44 g=6 g_rhs=2 lifted=1" in out
     assert "forest_n=4" in out

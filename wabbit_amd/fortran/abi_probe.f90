@@ -8,6 +8,7 @@ program abi_probe
     type(wh_wavelet) :: w
     type(wh_adapt_params) :: ap
     type(wh_mask_params) :: mp
+    type(wh_time_params) :: tp
     type(c_ptr) :: forest
     integer(c_int) :: bs(3), per(3)
     real(c_double) :: dom(3)
@@ -19,6 +20,7 @@ program abi_probe
     print '(A,I0)', 'sizeof(wh_wavelet)=', c_sizeof(w)
     print '(A,I0)', 'sizeof(wh_adapt_params)=', c_sizeof(ap)
     print '(A,I0)', 'sizeof(wh_mask_params)=', c_sizeof(mp)
+    print '(A,I0)', 'sizeof(wh_time_params)=', c_sizeof(tp)
     call wh_check(wh_wavelet_setup("CDF44" // c_null_char, 2, w), "setup_wavelet")
     print '(A,I0,A,I0,A,I0)', 'CDF44 g=', w%g, ' g_rhs=', w%g_rhs, ' lifted=', w%lifted
     bs = [16, 16, 1]

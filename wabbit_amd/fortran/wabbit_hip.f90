@@ -58,6 +58,13 @@ module wabbit_hip
         type(wh_mask_params) :: mask
     end type
 
+    type, bind(C) :: wh_time_params
+        real(c_double) :: time_max, dt_fixed, dt_max
+        real(c_double) :: write_time, write_time_first, tsave_stats, write_backup_time
+        integer(c_int) :: write_fixed_time
+        integer(c_int) :: write_backup_n
+    end type
+
     interface
         function wh_version() bind(C, name="wh_version") result(v)
             import :: c_int
@@ -246,6 +253,34 @@ module wabbit_hip
             type(wh_acm_params), intent(in) :: p
             real(c_double), intent(in) :: butcher(*), dt_groups(*)
             integer(c_int), value :: nrows, path
+            integer(c_int) :: rc
+        end function
+        function wh_time_set(pool, time, iteration) bind(C, name="wh_time_set") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: pool
+            real(c_double), value :: time
+            integer(c_int), value :: iteration
+            integer(c_int) :: rc
+        end function
+        function wh_run_steps(pool, p, butcher, nrows, tp, nsteps, use_graph) bind(C, name="wh_run_steps") result(rc)
+            import :: c_ptr, c_int, c_double, wh_acm_params, wh_time_params
+            type(c_ptr), value :: pool
+            type(wh_acm_params), intent(in) :: p
+            real(c_double), intent(in) :: butcher(*)
+            integer(c_int), value :: nrows, nsteps, use_graph
+            type(wh_time_params), intent(in) :: tp
+            integer(c_int) :: rc
+        end function
+        function wh_time_fetch(pool) bind(C, name="wh_time_fetch") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: pool
+            integer(c_int) :: rc
+        end function
+        function wh_time_fetched(pool, time, iteration, dt_last) bind(C, name="wh_time_fetched") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: pool
+            real(c_double), intent(out) :: time, dt_last
+            integer(c_int), intent(out) :: iteration
             integer(c_int) :: rc
         end function
         function wh_refine_blocks(pool, array, triples, ntriples, order) bind(C, name="wh_refine_blocks") result(rc)
