@@ -1113,7 +1113,7 @@ int or_tree_adapt(or_tree *T)
     }
     ensure_gradedness(T, stay);
     {
-        char *del = (char *)calloc((size_t)T->n, 1);
+        char *del = (char *)calloc((size_t)(T->n > 0 ? T->n : 1), 1);
         for (int b = 0; b < T->n; ++b) del[b] = T->status[b] == -1;
         delete_marked(T, del);
         free(del);
@@ -1127,7 +1127,7 @@ int or_tree_adapt(or_tree *T)
     }
     /* prune_fulltree2leafs */
     {
-        char *del = (char *)calloc((size_t)T->n, 1);
+        char *del = (char *)calloc((size_t)(T->n > 0 ? T->n : 1), 1);
         for (int b = 0; b < T->n; ++b) del[b] = !is_leaf(T, b, 0);
         delete_marked(T, del);
         free(del);
