@@ -295,14 +295,16 @@ static int wait_tables(wh_pool *P)
     return WH_OK;
 }
 
-// host -> pinned staging -> device buffer `dst`, ordered on the pool stream
+// host -> pinned staging -> device buffer `dst` (one host-to-device copy),
+// ordered on the pool stream; the pinned space is reused only after a stream
+// synchronisation
 static int upload_staged(wh_pool *P, void *dst, const void *src, size_t bytes)
 {
     if (!bytes) return WH_OK;
     if (int rc = stage_begin(P, bytes)) return rc;
-    void *d = stage_put(P, src, bytes);
-    if (!d) return WH_E_HIP;
-    WH_HIP(hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToDevice, P->stream));
+    void *h = stage_host(P, src, bytes);
+    if (!h) return WH_E_HIP;
+    WH_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, P->stream));
     return WH_OK;
 }
 

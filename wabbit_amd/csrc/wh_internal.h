@@ -75,6 +75,7 @@ struct WaveletWS {
     // reclaimed only after a stream synchronisation
     char *h_stage = nullptr, *d_stage = nullptr;
     size_t stage_cap = 0, stage_off = 0;
+    size_t pend_lo = ~(size_t)0;  // start of pieces put with defer, not yet copied
     const int32_t *cur_ids = nullptr;  // device block list of the current wavelet op
 };
 
@@ -202,7 +203,15 @@ void free_wavelet_ws(wh_pool *P);
 // returns its device address; stage_sync waits for the stream and frees all
 // staged space
 int stage_begin(wh_pool *P, size_t bytes);
-void *stage_put(wh_pool *P, const void *src, size_t bytes);
+// copies src into the reserved pinned space and returns the device address of
+// the same offset; the host-to-device copy is issued at once, or with defer
+// by the next stage_flush (one copy for consecutive deferred pieces of one
+// reservation)
+void *stage_put(wh_pool *P, const void *src, size_t bytes, bool defer = false);
+int stage_flush(wh_pool *P);
+// copies src into the reserved pinned space and returns that host address
+// (for a direct host-to-device copy into a buffer of the caller's)
+void *stage_host(wh_pool *P, const void *src, size_t bytes);
 int stage_sync(wh_pool *P);
 // device-side steps used by the adapt orchestration (wh_adapt.cpp); the
 // host vectors are uploaded on the pool's stream before use

@@ -164,8 +164,9 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
     for (int32_t h : ids) WH_CHECK(h >= 0 && h < P->d.nb_max, WH_E_ARG, "mask: hvy id out of range");
     WH_CHECK(x0dx.size() == (size_t)n * 6, WH_E_ARG, "mask: six origin/spacing values per block");
     if (int rc = stage_begin(P, sizeof(int32_t) * n + sizeof(double) * n * 6)) return rc;
-    const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n);
-    const double *d_x0dx = (const double *)stage_put(P, x0dx.data(), sizeof(double) * n * 6);
+    const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n, true);
+    const double *d_x0dx = (const double *)stage_put(P, x0dx.data(), sizeof(double) * n * 6, true);
+    if (int rc = stage_flush(P)) return rc;
     if (!d_ids || !d_x0dx) return WH_E_HIP;
     const long tot = (long)n * (long)P->cs;
     hipLaunchKernelGGL(k_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask, P->mblk,

@@ -897,6 +897,7 @@ int stage_sync(wh_pool *P)
 int stage_begin(wh_pool *P, size_t bytes)
 {
     WaveletWS &W = P->ws;
+    WH_CHECK(W.pend_lo == ~(size_t)0, WH_E_STATE, "staging: deferred pieces not flushed");
     const size_t need = stage_round(bytes) + 4 * STAGE_ALIGN;  // room for a few aligned pieces
     if (W.stage_off + need <= W.stage_cap) return WH_OK;
     if (int rc = stage_sync(P)) return rc;
@@ -912,9 +913,21 @@ int stage_begin(wh_pool *P, size_t bytes)
     return WH_OK;
 }
 
-void *stage_put(wh_pool *P, const void *src, size_t bytes)
+int stage_flush(wh_pool *P)
 {
     WaveletWS &W = P->ws;
+    if (W.pend_lo == ~(size_t)0) return WH_OK;
+    const size_t lo = W.pend_lo;
+    W.pend_lo = ~(size_t)0;
+    if (W.stage_off > lo)
+        WH_HIP(hipMemcpyAsync(W.d_stage + lo, W.h_stage + lo, W.stage_off - lo, hipMemcpyHostToDevice, P->stream));
+    return WH_OK;
+}
+
+void *stage_put(wh_pool *P, const void *src, size_t bytes, bool defer)
+{
+    WaveletWS &W = P->ws;
+    if (!defer && stage_flush(P) != WH_OK) return nullptr;
     const size_t r = stage_round(bytes ? bytes : 1);
     if (W.stage_off + r > W.stage_cap) {
         set_error("staging: stage_put beyond the reserved space");
@@ -923,13 +936,30 @@ void *stage_put(wh_pool *P, const void *src, size_t bytes)
     char *h = W.h_stage + W.stage_off, *d = W.d_stage + W.stage_off;
     if (bytes) {
         std::memcpy(h, src, bytes);
-        if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, P->stream) != hipSuccess) {
+        if (defer) {
+            if (W.pend_lo == ~(size_t)0) W.pend_lo = W.stage_off;
+        } else if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, P->stream) != hipSuccess) {
             set_error("staging: hipMemcpyAsync failed");
             return nullptr;
         }
     }
     W.stage_off += r;
     return d;
+}
+
+void *stage_host(wh_pool *P, const void *src, size_t bytes)
+{
+    WaveletWS &W = P->ws;
+    if (stage_flush(P) != WH_OK) return nullptr;
+    const size_t r = stage_round(bytes ? bytes : 1);
+    if (W.stage_off + r > W.stage_cap) {
+        set_error("staging: stage_host beyond the reserved space");
+        return nullptr;
+    }
+    char *h = W.h_stage + W.stage_off;
+    if (bytes) std::memcpy(h, src, bytes);
+    W.stage_off += r;
+    return h;
 }
 
 void free_wavelet_ws(wh_pool *P)
@@ -1143,8 +1173,9 @@ int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<
     std::vector<int64_t> pstart(n + 1, 0);
     for (int i = 0; i < n; ++i) pstart[i + 1] = pstart[i] + sync_slots(P, p[i]);
     if (int rc = stage_begin(P, sizeof(wh_patch) * n + sizeof(int64_t) * (n + 1))) return rc;
-    const wh_patch *d_patches = (const wh_patch *)stage_put(P, p.data(), sizeof(wh_patch) * n);
-    const int64_t *d_pstart = (const int64_t *)stage_put(P, pstart.data(), sizeof(int64_t) * (n + 1));
+    const wh_patch *d_patches = (const wh_patch *)stage_put(P, p.data(), sizeof(wh_patch) * n, true);
+    const int64_t *d_pstart = (const int64_t *)stage_put(P, pstart.data(), sizeof(int64_t) * (n + 1), true);
+    if (int rc = stage_flush(P)) return rc;
     if (!d_patches || !d_pstart) return WH_E_HIP;
     int s0 = 0;
     while (s0 < n) {
