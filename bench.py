@@ -416,7 +416,7 @@ def main():
         calibrate_copy(args.workload if args.workload in ("c1", "c3u") else "c1")
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               "(there is no multi-GPU decomposition)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c4":
