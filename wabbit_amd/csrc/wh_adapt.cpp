@@ -84,6 +84,7 @@ struct Tree {
 
     Tree(wh_forest *f, wh_pool *p) : F(f), P(p), w(p->ws.w), dim(f->dim), g(f->g), jmax(f->jmax)
     {
+        WH_PROF("tree_build");
         for (int d = 0; d < 3; ++d) bs[d] = f->bs[d];
         nd = dirs_of(dim, dirs);
         b.reserve(2 * f->blocks.size());
