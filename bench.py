@@ -420,7 +420,7 @@ def main():
               "(there is no multi-GPU decomposition)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c4":
-        args.lanes = args.lanes or 8
+        args.lanes = args.lanes or 12
         run_ensemble(args)
         return
     if args.workload == "c5":
