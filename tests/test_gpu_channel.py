@@ -62,3 +62,44 @@ def test_steps(case):
             _compare(s, tc, lv, hvy, 1e-12 if k == 0 else 1e-9)
     finally:
         s.close()
+
+
+def test_device_time_matches_host_dt(case):
+    """The fused step with dt, its clamps and the time advance on the device
+    (wh_run_steps + wh_time_fetch/wh_time_fetched) against the same solver
+    taking dt on the host (wh_dt_reduce + clamp_dt + wh_rk_step): identical
+    dt, time, iteration, grids and fields, bit for bit."""
+    out = {}
+    for dev in (True, False):
+        s = wl.channel_solver(case, path=PATH_FUSED, nb_max=8192)
+        s.device_time = dev
+        try:
+            dts = [s.step() for _ in range(5)]
+            out[dev] = (dts, s.time, s.iteration, s.state())
+        finally:
+            s.close()
+    (d1, t1, i1, (tc1, lv1, u1)), (d2, t2, i2, (tc2, lv2, u2)) = out[True], out[False]
+    assert d1 == d2 and t1 == t2 and i1 == i2 == 5
+    assert np.array_equal(tc1, tc2) and np.array_equal(lv1, lv2) and np.array_equal(u1, u2)
+
+
+def test_stage_kernel_timing_leaves_state(case):
+    """wh_time_stage_kernel times RK4 stage sequences on the live grid and
+    writes only work slots: the state and the next step are unchanged."""
+    from wabbit_amd import lib as L
+    ref = wl.channel_solver(case, path=PATH_FUSED, nb_max=8192)
+    s = wl.channel_solver(case, path=PATH_FUSED, nb_max=8192)
+    try:
+        for x in (ref, s):
+            x.step()
+        before = s.pool.download(L.WH_BLOCK, 0, s.nb_max)
+        ms = s.pool.time_stage_kernel(s.acm_c, 3)
+        assert ms > 0.0
+        assert np.array_equal(before, s.pool.download(L.WH_BLOCK, 0, s.nb_max))
+        for x in (ref, s):
+            x.step()
+        a, b = ref.state(), s.state()
+        assert all(np.array_equal(p, q) for p, q in zip(a, b))
+    finally:
+        ref.close()
+        s.close()
