@@ -900,13 +900,19 @@ int stage_begin(wh_pool *P, size_t bytes)
     WH_CHECK(W.pend_lo == ~(size_t)0, WH_E_STATE, "staging: deferred pieces not flushed");
     const size_t need = stage_round(bytes) + 4 * STAGE_ALIGN;  // room for a few aligned pieces
     if (W.stage_off + need <= W.stage_cap) return WH_OK;
+    // full: wait for the queued copies and kernels, then reuse the space; a
+    // buffer that fills between the orchestration's own synchronisations is
+    // doubled (up to 64 MB), so that this wait stops recurring
+    const bool filled = W.stage_cap > 0;
     if (int rc = stage_sync(P)) return rc;
-    if (need <= W.stage_cap) return WH_OK;
+    const size_t grow = filled && W.stage_cap < ((size_t)64 << 20) ? 2 * W.stage_cap : 0;
+    if (need <= W.stage_cap && !grow) return WH_OK;
+    const size_t old_cap = W.stage_cap;
     if (W.h_stage) WH_HIP(hipHostFree(W.h_stage));
     if (W.d_stage) WH_HIP(hipFree(W.d_stage));
     W.h_stage = W.d_stage = nullptr;
     W.stage_cap = 0;
-    const size_t cap = std::max<size_t>(need, std::max<size_t>(2 * W.stage_cap, (size_t)1 << 20));
+    const size_t cap = std::max<size_t>(need, std::max<size_t>(std::max(grow, 2 * old_cap), (size_t)1 << 20));
     WH_HIP(hipHostMalloc((void **)&W.h_stage, cap, hipHostMallocDefault));
     WH_HIP(hipMalloc((void **)&W.d_stage, cap));
     W.stage_cap = cap;
