@@ -47,11 +47,35 @@ struct wh_forest {
     mutable int pc_gm = -1, pc_gp = -1, pc_order = -1;
     mutable uint64_t pc_version = ~0ull;
 
+    // (level, grid index) -> index in blocks as a dense table per level with
+    // at most 2^DENSE_BITS cells (the hash map beyond); the entries set by the
+    // last rebuild are reset by the next, so a rebuild costs O(blocks)
+    static constexpr int DENSE_BITS = 22;
+    std::vector<std::vector<int>> dense;
+    std::vector<std::pair<int, size_t>> dense_set;
+    size_t lin(int level, const int q[3]) const
+    {
+        return ((size_t)q[2] << (2 * level)) | ((size_t)q[1] << level) | (size_t)q[0];
+    }
     void rebuild_map()
     {
         ++version;
         map.clear();
-        for (size_t i = 0; i < blocks.size(); ++i) map[wh::fkey(blocks[i].level, blocks[i].tc)] = (int)i;
+        for (const auto &e : dense_set) dense[e.first][e.second] = -1;
+        dense_set.clear();
+        for (size_t i = 0; i < blocks.size(); ++i) {
+            const wh::FBlock &b = blocks[i];
+            map[wh::fkey(b.level, b.tc)] = (int)i;
+            if (dim * b.level > DENSE_BITS) continue;
+            if ((int)dense.size() <= b.level) dense.resize(b.level + 1);
+            std::vector<int> &t = dense[b.level];
+            if (t.empty()) t.assign((size_t)1 << (dim * b.level), -1);
+            int q[3];
+            ijk_of(b.tc, b.level, q);
+            const size_t k = lin(b.level, q);
+            t[k] = (int)i;
+            dense_set.push_back({b.level, k});
+        }
     }
     int digit_shift(int level) const { return (jmax - level) * dim; }
     void ijk_of(int64_t tc, int level, int ijk[3]) const
@@ -91,6 +115,10 @@ struct wh_forest {
                 v = 0;
             }
             wrapped[d] = v;
+        }
+        if (dim * level <= DENSE_BITS) {
+            if (level >= (int)dense.size() || dense[level].empty()) return -1;  // no block on this level
+            return dense[level][lin(level, wrapped)];
         }
         auto it = map.find(wh::fkey(level, tc_of(wrapped, level)));
         return it == map.end() ? -1 : it->second;
