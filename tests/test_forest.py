@@ -43,7 +43,7 @@ def test_patch_tables_match_point_location(case, seed):
 
 def test_refine_bookkeeping():
     f = Forest(2, 4, [8, 8], 3, [1.0, 1.0], nb_max=40)
-    tcs, lev = random_graded_forest(2, 1, 1, 0, 0)
+    tcs, lev = random_graded_forest(2, 1, 4, 0, 0)
     f.set_blocks(tcs, lev, np.arange(4, dtype=np.int32))
     tri = f.refine(np.array([0, 2], np.int32))
     assert tri.shape == (8, 3)
@@ -60,7 +60,7 @@ def test_slot_base_and_errors():
     only slots of the range and reports running out of them."""
     from wabbit_amd import lib as L
     f = Forest(2, 4, [8, 8], 3, [1.0, 1.0], nb_max=8, slot_base=100)
-    tcs, lev = random_graded_forest(2, 1, 1, 0, 0)           # 4 blocks on level 1
+    tcs, lev = random_graded_forest(2, 1, 4, 0, 0)           # 4 blocks on level 1
     with pytest.raises(L.WabbitHipError):
         f.set_blocks(tcs, lev, np.arange(4, dtype=np.int32))      # below the range
     with pytest.raises(L.WabbitHipError):

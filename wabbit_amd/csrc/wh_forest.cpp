@@ -103,8 +103,7 @@ int wh_forest_set_blocks(wh_forest *F, int n, const int64_t *tc, const int32_t *
     F->free_hvy.clear();
     for (int h = F->nb_max - 1; h >= 0; --h)
         if (!used[h]) F->free_hvy.push_back(F->slot0 + h);
-    F->rebuild_map();
-    if ((int)F->map.size() != n) {
+    if (!F->rebuild_map()) {
         wh::set_error("wh_forest_set_blocks: duplicate (level, treecode)");
         return WH_E_ARG;
     }
@@ -350,23 +349,29 @@ int wh_forest_refine(wh_forest *F, int n, const int32_t *hvy_list, int32_t *pair
         wh::set_error("wh_forest_refine: pool out of memory (or output too small)");
         return WH_E_NOMEM;
     }
-    std::unordered_map<int, int> by_hvy;
-    for (size_t i = 0; i < F->blocks.size(); ++i) by_hvy[F->blocks[i].hvy] = (int)i;
+    // block index by pool slot (the forest's slots are [slot0, slot0 + nb_max))
+    std::vector<int> by_hvy((size_t)F->nb_max, -1);
+    for (size_t i = 0; i < F->blocks.size(); ++i) {
+        const int h = F->blocks[i].hvy - F->slot0;
+        if (h >= 0 && h < F->nb_max) by_hvy[h] = (int)i;
+    }
     std::vector<char> remove(F->blocks.size(), 0);
     std::vector<Block> added;
+    added.reserve((size_t)n * nd);
     int np = 0;
     for (int i = 0; i < n; ++i) {
-        auto it = by_hvy.find(hvy_list[i]);
-        if (it == by_hvy.end() || remove[it->second]) {
+        const int h = hvy_list[i] - F->slot0;
+        const int bi = (h >= 0 && h < F->nb_max) ? by_hvy[h] : -1;
+        if (bi < 0 || remove[bi]) {
             wh::set_error("wh_forest_refine: block not active");
             return WH_E_ARG;
         }
-        const Block m = F->blocks[it->second];
+        const Block m = F->blocks[bi];
         if (m.level >= F->jmax) {
             wh::set_error("wh_forest_refine: block already at Jmax");
             return WH_E_ARG;
         }
-        remove[it->second] = 1;
+        remove[bi] = 1;
         for (int k = 0; k < nd; ++k) {
             const int h = F->free_hvy.back();
             F->free_hvy.pop_back();

@@ -36,7 +36,7 @@ struct wh_forest {
     int nb_max = 0;
     int slot0 = 0;                          // the forest's pool slots: [slot0, slot0 + nb_max)
     std::vector<wh::FBlock> blocks;         // active blocks (leaf list)
-    std::unordered_map<uint64_t, int> map;  // (level, tc) -> index in blocks
+    std::unordered_map<uint64_t, int> map;  // (level, tc) -> index in blocks, levels beyond the dense tables
     std::vector<int> free_hvy;              // free pool slots
 
     // bumped by every change of the block list (all changes end in rebuild_map)
@@ -57,25 +57,34 @@ struct wh_forest {
     {
         return ((size_t)q[2] << (2 * level)) | ((size_t)q[1] << level) | (size_t)q[0];
     }
-    void rebuild_map()
+    // rebuilds the lookup; returns false if two blocks share (level, treecode)
+    bool rebuild_map()
     {
         ++version;
         map.clear();
         for (const auto &e : dense_set) dense[e.first][e.second] = -1;
         dense_set.clear();
+        bool unique = true;
         for (size_t i = 0; i < blocks.size(); ++i) {
             const wh::FBlock &b = blocks[i];
-            map[wh::fkey(b.level, b.tc)] = (int)i;
-            if (dim * b.level > DENSE_BITS) continue;
+            if (dim * b.level > DENSE_BITS) {
+                unique = map.emplace(wh::fkey(b.level, b.tc), (int)i).second && unique;
+                continue;
+            }
             if ((int)dense.size() <= b.level) dense.resize(b.level + 1);
             std::vector<int> &t = dense[b.level];
             if (t.empty()) t.assign((size_t)1 << (dim * b.level), -1);
             int q[3];
             ijk_of(b.tc, b.level, q);
             const size_t k = lin(b.level, q);
+            if (t[k] >= 0) {
+                unique = false;
+                continue;
+            }
             t[k] = (int)i;
             dense_set.push_back({b.level, k});
         }
+        return unique;
     }
     int digit_shift(int level) const { return (jmax - level) * dim; }
     void ijk_of(int64_t tc, int level, int ijk[3]) const
