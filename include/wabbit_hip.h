@@ -221,6 +221,10 @@ int wh_forest_destroy(wh_forest *forest);
 /* The forest's pool slots become [first, first + nb_max) (default first = 0),
  * so that several forests can share one pool; only while it has no blocks. */
 int wh_forest_set_slot_base(wh_forest *forest, int first);
+/* Replaces the block list.  WH_E_ARG for a slot id outside the forest's
+ * range or used twice, or a level outside [0, jmax] (the forest is left as it
+ * was), and for two blocks with one (level, treecode) (the forest is left
+ * empty). */
 int wh_forest_set_blocks(wh_forest *forest, int n, const int64_t *treecodes,
                          const int32_t *levels, const int32_t *hvy_ids);
 int wh_forest_size(const wh_forest *forest, int *n);

@@ -90,3 +90,32 @@ def test_empty_and_single_block_forests():
     pt = f.patches(3, 3, 4)
     assert len(pt) == 8 and np.all(pt["op"] == 0) and np.all(pt["send"] == 0) and np.all(pt["recv"] == 0)
     f.close()
+
+
+@pytest.mark.parametrize("level", [1, 12])
+def test_duplicate_block_rejected(level):
+    """Two blocks with one (level, treecode) are rejected by set_blocks, on a
+    level held in the dense per-level lookup tables (1) and on one beyond
+    them, held in the hash map (12 in 2D); distinct blocks are accepted.  A
+    list with a bad slot id leaves the forest unchanged, one with a duplicate
+    leaves it empty."""
+    from wabbit_amd import lib as L
+    from helpers import _tc
+    f = Forest(2, 12, [8, 8], 3, [1.0, 1.0], nb_max=8)
+    n = 2 ** level
+    cells = [(0, 0), (1, 0), (0, 1), (n - 1, n - 1)]
+    tcs = np.array([_tc(c, level, 12, 2) for c in cells], np.int64)
+    lev = np.full(4, level, np.int32)
+    f.set_blocks(tcs, lev, np.arange(4, dtype=np.int32))
+    assert f.n == 4
+    with pytest.raises(L.WabbitHipError):                          # bad hvy id: forest unchanged
+        f.set_blocks(tcs, lev, np.array([0, 1, 2, 99], np.int32))
+    assert f.n == 4 and sorted(f.blocks()[0].tolist()) == sorted(tcs.tolist())
+    dup = tcs.copy()
+    dup[3] = dup[1]
+    with pytest.raises(L.WabbitHipError):                          # duplicate: forest left empty
+        f.set_blocks(dup, lev, np.arange(4, dtype=np.int32))
+    assert f.n == 0
+    f.set_blocks(tcs[:3], lev[:3], np.arange(3, dtype=np.int32))   # usable again after the error
+    assert f.n == 3
+    f.close()

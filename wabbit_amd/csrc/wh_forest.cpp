@@ -89,8 +89,10 @@ int wh_forest_set_blocks(wh_forest *F, int n, const int64_t *tc, const int32_t *
         wh::set_error("wh_forest_set_blocks: bad argument");
         return WH_E_ARG;
     }
+    // validated before the forest changes: a rejected list leaves it as it was
     std::vector<char> used(F->nb_max, 0);
-    F->blocks.clear();
+    std::vector<Block> nb;
+    nb.reserve((size_t)n);
     for (int i = 0; i < n; ++i) {
         const int h = hvy[i] - F->slot0;
         if (h < 0 || h >= F->nb_max || used[h] || level[i] < 0 || level[i] > F->jmax) {
@@ -98,12 +100,20 @@ int wh_forest_set_blocks(wh_forest *F, int n, const int64_t *tc, const int32_t *
             return WH_E_ARG;
         }
         used[h] = 1;
-        F->blocks.push_back(Block{tc[i], level[i], hvy[i], 0});
+        nb.push_back(Block{tc[i], level[i], hvy[i], 0});
     }
-    F->free_hvy.clear();
-    for (int h = F->nb_max - 1; h >= 0; --h)
-        if (!used[h]) F->free_hvy.push_back(F->slot0 + h);
+    F->blocks.swap(nb);
+    auto free_all_but = [&](const std::vector<char> &u) {
+        F->free_hvy.clear();
+        for (int h = F->nb_max - 1; h >= 0; --h)
+            if (!u[h]) F->free_hvy.push_back(F->slot0 + h);
+    };
+    free_all_but(used);
     if (!F->rebuild_map()) {
+        // two blocks on one (level, treecode): the forest is left empty
+        F->blocks.clear();
+        free_all_but(std::vector<char>(F->nb_max, 0));
+        F->rebuild_map();
         wh::set_error("wh_forest_set_blocks: duplicate (level, treecode)");
         return WH_E_ARG;
     }
