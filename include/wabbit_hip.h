@@ -327,7 +327,7 @@ int wh_create_mask(wh_pool *pool, const wh_mask_params *mp, const int32_t *hvy_i
  * closing sync_ghosts_tree.  The forest's block list is replaced by the new
  * leaves.  WH_TMP must hold >= neq components; work slots 0 and 1 are used
  * as scratch.  The pool needs free slots for the mother blocks. */
-enum { WH_IND_EVERYWHERE = 0, WH_IND_THRESHOLD = 1 };
+enum { WH_IND_EVERYWHERE = 0, WH_IND_THRESHOLD = 1, WH_IND_EXTERNAL = 2 };
 typedef struct {
     int jmin, jmax;
     int indicator;                  /* WH_IND_* */
@@ -341,8 +341,37 @@ typedef struct {
     int security_zone;              /* addSecurityZone_CE_tree */
     double domain[3];
     wh_mask_params mask;
+    /* wavelet_decompose_full_tree's branch: 0 the reference's rule
+     * (leaf-first iff every Bs >= 3 max|HD tap|, adapt_tree.f90:321-322),
+     * 1 leaf-first, 2 level-wise; both give the same decomposition */
+    int decompose_branch;
 } wh_adapt_params;
 int wh_adapt_tree(wh_pool *pool, wh_forest *forest, const wh_adapt_params *p);
+
+/* adapt_tree with indicator "nothing (external)" (adapt_tree.f90:11-255,
+ * coarseningIndicator_tree.f90:55): the host keeps the refinement flags and
+ * passes them in, one per leaf in the forest's list order
+ * (wh_forest_get_blocks): -1 coarsen, 0 stay, 1 refine (a refine flag is
+ * kept as a status; adapt_tree never refines).  p->indicator must be
+ * WH_IND_EXTERNAL; the flags then go through respectJmaxJmin_tree and
+ * ensureGradedness_tree as the reference's do (no mask criterion, no
+ * security zone -- adapt_tree.f90:127 skips it for this indicator).  n must
+ * equal the forest's block count. */
+int wh_adapt_tree_flags(wh_pool *pool, wh_forest *forest, const wh_adapt_params *p,
+                        const int32_t *leaf_flags, int n);
+
+/* coarseningIndicator_block -> threshold_block on the listed blocks
+ * (coarseningIndicator_block.f90:12-108, threshold_block.f90:1-130):
+ * waveletDecomposition_optimized_block of src_array into dst_array, then per
+ * block the detail (max over the interior of |WC| per component, grouped by
+ * groups[] as in wh_detail_blocks) into detail_out[k*neq + c], and the flag
+ * flags_out[k] = -1 if every detail <= thresh[c] (eps * norm, Linfty), else
+ * 0.  The host decides what to do with the flags (wh_adapt_tree_flags).
+ * Ghost nodes of src_array must be synchronised.  Returns with the stream
+ * synchronised. */
+int wh_fwt_threshold(wh_pool *pool, int src_array, int dst_array, const int32_t *hvy_ids, int n,
+                     const double *thresh, const int32_t *groups, int32_t *flags_out,
+                     double *detail_out);
 
 /* sync_ghosts_tree (synchronize_ghosts_generic.f90:121-144) on a leaf forest
  * with all ghosts, restriction filtered with HD for lifted wavelets

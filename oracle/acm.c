@@ -24,8 +24,14 @@ static const double B_FD6[7] = {1.0 / 90.0, -3.0 / 20.0, 3.0 / 2.0, -49.0 / 18.0
 
 /* first derivative of f along stride s (rhs_ACM.f90:383-384 FD2, :540-542 FD4,
  * :460 TW4, :698 FD6); terms summed from the most negative offset up. */
-static double d1(int scheme, const double *f, ptrdiff_t s, double inv)
+static double d1(int scheme, const double *f, ptrdiff_t s, double inv, int plant)
 {
+    if ((plant & 2) && scheme == OR_FD6)
+        return (A_FD6[0] * f[-3 * s] + A_FD6[1] * f[-2 * s] + A_FD6[2] * f[-s] +
+                A_FD6[4] * f[s] + A_FD6[5] * f[2 * s]) * inv;
+    if ((plant & 4) && scheme == OR_FD4_TW)
+        return (A_TW4[0] * f[-3 * s] + A_TW4[1] * f[-2 * s] + A_TW4[2] * f[-s] +
+                A_TW4[4] * f[s] + A_TW4[5] * f[2 * s] - A_TW4[6] * f[3 * s]) * inv;
     switch (scheme) {
     case OR_FD2:
         return (f[s] - f[-s]) * inv * 0.5;
@@ -64,10 +70,11 @@ static double d1p(int scheme, const double *fa, const double *fb, ptrdiff_t s, d
 
 /* second derivative (rhs_ACM.f90:395-399 FD2, :554-558 FD4 -- also used by
  * the Tam-Webb scheme --, :712-716 FD6). */
-static double d2(int scheme, const double *f, ptrdiff_t s, double inv2)
+static double d2(int scheme, const double *f, ptrdiff_t s, double inv2, int plant)
 {
     switch (scheme) {
     case OR_FD2:
+        if (plant & 8) return (f[-s] - 1.0 * f[0] + f[s]) * inv2;
         return (f[-s] - 2.0 * f[0] + f[s]) * inv2;
     case OR_FD4:
     case OR_FD4_TW:
@@ -117,11 +124,11 @@ void or_rhs_acm_block(const or_acm_params *p, const int Bs[3], int g,
                 for (int c = 0; c <= dim; ++c) {
                     uc[c] = u[c * cs + i];
                     for (int d = 0; d < dim; ++d)
-                        du[c][d] = d1(sc, u + c * cs + i, s[d], inv[d]);
+                        du[c][d] = d1(sc, u + c * cs + i, s[d], inv[d], p->plant);
                 }
                 for (int c = 0; c < dim; ++c)
                     for (int d = 0; d < dim; ++d)
-                        dd[c][d] = d2(sc, u + c * cs + i, s[d], inv2[d]);
+                        dd[c][d] = d2(sc, u + c * cs + i, s[d], inv2[d], p->plant);
 
                 for (int c = 0; c < dim; ++c) {
                     double penal = 0.0;
@@ -144,8 +151,10 @@ void or_rhs_acm_block(const or_acm_params *p, const int Bs[3], int g,
                         for (int d = 0; d < dim; ++d) sum = sum + uc[d] * du[c][d];
                         t = -(0.5 * sum);
                     } else {
-                        t = -(uc[0] * du[c][0]);
-                        for (int d = 1; d < dim; ++d) t = t - uc[d] * du[c][d];
+                        /* plant bit 0 swaps the first two derivative directions */
+                        const int sw = (p->plant & 1) ? 1 : 0;
+                        t = -(uc[0] * du[c][sw]);
+                        for (int d = 1; d < dim; ++d) t = t - uc[d] * du[c][(sw && d == 1) ? 0 : d];
                     }
                     t = t - du[np][c];
                     t = t + p->nu * lap;

@@ -45,8 +45,15 @@ static void pred_coefs(int order, double *c)
  * (2nx-1, 2ny-1, 2nz-1) (nz = 1 in 2D).  Index (i,j,k) at i + nx*(j + ny*k). */
 void or_prediction(const double *coarse, int nx, int ny, int nz, double *fine, int order)
 {
+    or_prediction_zorder(coarse, nx, ny, nz, fine, order, order);
+}
+
+/* the same with a z pass of order zorder (zorder != order only as a planted
+ * mistake in tests) */
+void or_prediction_zorder(const double *coarse, int nx, int ny, int nz, double *fine, int order, int zorder)
+{
     const int fx = 2 * nx - 1, fy = 2 * ny - 1, fz = nz > 1 ? 2 * nz - 1 : 1;
-    const int nc = order, N = order / 2;
+    int nc = order, N = order / 2;
     double c[6];
     pred_coefs(order, c);
 #define F(i, j, k) fine[(i) + (size_t)fx * ((j) + (size_t)fy * (k))]
@@ -70,6 +77,9 @@ void or_prediction(const double *coarse, int nx, int ny, int nz, double *fine, i
                 F(i, j, k) = s;
             }
     }
+    pred_coefs(zorder, c);
+    nc = zorder;
+    N = zorder / 2;
     if (nz > 1)
         for (int k = 2 * N - 1; k <= fz - 2 * N; k += 2)
             for (int j = 0; j < fy; ++j)

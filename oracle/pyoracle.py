@@ -26,6 +26,7 @@ class OrAcmParams(ctypes.Structure):
         ("C_eta", ctypes.c_double), ("C_sponge", ctypes.c_double),
         ("u_mean_set", ctypes.c_double * 3),
         ("CFL", ctypes.c_double), ("CFL_nu", ctypes.c_double), ("CFL_eta", ctypes.c_double),
+        ("plant", ctypes.c_int),
     ]
 
 
@@ -229,7 +230,7 @@ class OrAdaptParams(ctypes.Structure):
                 ("norm", ctypes.c_double * 8), ("groups", ctypes.c_int * 8),
                 ("eps_normalized", ctypes.c_int), ("threshold_mask", ctypes.c_int),
                 ("security_zone", ctypes.c_int), ("domain", ctypes.c_double * 3),
-                ("mask", OrMaskParams)]
+                ("mask", OrMaskParams), ("decompose_branch", ctypes.c_int), ("plant", ctypes.c_int)]
 
 
 def mask_params(geometry=0, x_cntr=(0, 0, 0), R_cyl=0.5, smoothing_width=0.0, smoothing_safety=0.0,
@@ -275,6 +276,8 @@ def _wlib():
         for fn in ("or_tree_sync_full", "or_tree_refine_everywhere", "or_tree_adapt"):
             getattr(L, fn).argtypes = [vp]
             getattr(L, fn).restype = ctypes.c_int
+        L.or_tree_adapt_flags.argtypes = [vp, _ip]
+        L.or_tree_adapt_flags.restype = ctypes.c_int
         L.or_mask_block.argtypes = [ctypes.POINTER(OrMaskParams), ctypes.c_int, _ip, ctypes.c_int, _dp, _dp, _dp]
         L.or_tree_dt.argtypes = [vp, ctypes.POINTER(OrAcmParams)]
         L.or_tree_dt.restype = ctypes.c_double
@@ -331,8 +334,11 @@ class Tree:
 
     def __init__(self, w, dim, jmin, jmax, bs, g, treecodes, levels, hvy, indicator=0, use_ce=None,
                  eps=1e-3, norm=None, groups=None, force_maxlevel_dealiasing=False, eps_normalized=False,
-                 threshold_mask=False, security_zone=None, domain=(1.0, 1.0, 1.0), mask=None):
+                 threshold_mask=False, security_zone=None, domain=(1.0, 1.0, 1.0), mask=None,
+                 decompose_branch=0, plant=0):
         p = OrAdaptParams()
+        p.decompose_branch = int(decompose_branch)
+        p.plant = int(plant)
         p.dim, p.jmin, p.jmax, p.g = dim, jmin, jmax, g
         p.eps_normalized = int(eps_normalized)
         p.threshold_mask = int(threshold_mask)
@@ -379,6 +385,11 @@ class Tree:
 
     def adapt(self):
         self._ck(_wlib().or_tree_adapt(self.h), "or_tree_adapt")
+
+    def adapt_flags(self, leaf_flags):
+        """adapt_tree with external flags, one per leaf in get()'s order."""
+        f = np.ascontiguousarray(leaf_flags, np.int32)
+        self._ck(_wlib().or_tree_adapt_flags(self.h, _i(f)), "or_tree_adapt_flags")
 
     def size(self):
         return _wlib().or_tree_size(self.h)

@@ -549,8 +549,9 @@ static int tree_sync(or_tree *T, const sync_spec *S)
                             pred[cb] = (double *)malloc(sizeof(double) * (size_t)fx * fy * fz * T->neq);
                             const double *src = src_array(T, S, cb, 1);
                             for (int c = 0; c < T->neq; ++c)
-                                or_prediction(src + c * T->cs, nx, ny, nz, pred[cb] + (size_t)c * fx * fy * fz,
-                                              T->w.order);
+                                or_prediction_zorder(src + c * T->cs, nx, ny, nz,
+                                                     pred[cb] + (size_t)c * fx * fy * fz, T->w.order,
+                                                     (T->p.plant & 4) ? 2 : T->w.order);
                         }
                         int fi[3] = {0, 0, 0};
                         for (int d = 0; d < dim; ++d) {
@@ -683,6 +684,8 @@ static int decompose_full_tree(or_tree *T)
     int leaf_first = 1;
     for (int d = 0; d < T->dim; ++d)
         if (T->p.bs[d] < 3 * hmax) leaf_first = 0;
+    if (T->p.decompose_branch == 1) leaf_first = 1;
+    if (T->p.decompose_branch == 2) leaf_first = 0;
     int level = max_level(T);
     for (int b = 0; b < T->n; ++b) {
         if (T->level[b] == level || leaf_first) T->status[b] = -1;
@@ -703,7 +706,7 @@ static int decompose_full_tree(or_tree *T)
         S.src_ref = -1;
         S.filter = 1;
         S.gm = S.gp = g_this;
-        rc = tree_sync(T, &S);
+        if (!((T->p.plant & 2) && !leaf_first && it > 0)) rc = tree_sync(T, &S);
         if (rc) break;
         for (int b = 0; b < T->n; ++b) {
             if (T->status[b] != -1) continue;
@@ -717,6 +720,7 @@ static int decompose_full_tree(or_tree *T)
         for (int b = 0; b < T->n; ++b) {
             if (T->status[b] == -1) T->status[b] = 0;
             if ((T->status[b] == REF_TMP_EMPTY || !leaf_first) && T->level[b] == level - 1) T->status[b] = -1;
+            if ((T->p.plant & 1) && !leaf_first && b < n_leaf && T->status[b] == -1) T->status[b] = 0;
         }
         d2m(T, level);
         --level;
@@ -1077,16 +1081,39 @@ static void security_zone(or_tree *T, const double *norm)
     free(flag);
 }
 
-int or_tree_adapt(or_tree *T)
+int or_tree_adapt(or_tree *T) { return or_tree_adapt_flags(T, NULL); }
+
+int or_tree_adapt_flags(or_tree *T, const int *leaf_flags)
 {
     const int stay = REF_UNSIGNIFICANT_STAY;
+    if (T->p.indicator == OR_IND_EXTERNAL) {
+        if (!leaf_flags) return -10;
+        /* the flags come in or_tree_get's order: by level, then treecode */
+        int *ord = (int *)malloc(sizeof(int) * (T->n ? T->n : 1));
+        for (int b = 0; b < T->n; ++b) ord[b] = b;
+        for (int i = 1; i < T->n; ++i) {
+            const int v = ord[i];
+            int j = i - 1;
+            while (j >= 0 && cmp_idx_T_level_tc(T, ord[j], v) > 0) {
+                ord[j + 1] = ord[j];
+                --j;
+            }
+            ord[j + 1] = v;
+        }
+        for (int i = 0; i < T->n; ++i) T->status[ord[i]] = leaf_flags[i];
+        free(ord);
+    }
+    /* the decomposition hands the leaves' statuses back (its backup) */
     int rc = decompose_full_tree(T);
     if (rc) return rc;
-    /* coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331) */
-    for (int b = 0; b < T->n; ++b) T->status[b] = 0;
+    /* coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331); the
+     * external indicator returns at once (:55) */
+    if (T->p.indicator != OR_IND_EXTERNAL)
+        for (int b = 0; b < T->n; ++b) T->status[b] = 0;
     double norm[8];
     for (int c = 0; c < 8; ++c) norm[c] = T->p.norm[c];
-    if (T->p.indicator == OR_IND_EVERYWHERE) {
+    if (T->p.indicator == OR_IND_EXTERNAL) {
+    } else if (T->p.indicator == OR_IND_EVERYWHERE) {
         for (int b = 0; b < T->n; ++b)
             if (is_leaf(T, b, 0)) T->status[b] = -1;
     } else {

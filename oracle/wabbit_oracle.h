@@ -53,6 +53,12 @@ typedef struct {
     double C_eta, C_sponge;
     double u_mean_set[3];
     double CFL, CFL_nu, CFL_eta;
+    /* planted mistakes, for tests that show a check has teeth (0 in every
+     * real use): bit 0 the convective term takes du_c/dy where it needs
+     * du_c/dx and vice versa; bit 1 FD6's first derivative drops its +3
+     * term; bit 2 the Tam-Webb +3 coefficient has the wrong sign; bit 3
+     * the FD2 second derivative uses -1 for the centre weight */
+    int plant;
 } or_acm_params;
 
 /* RHS of one block: RHS_2D_acm / RHS_3D_acm (rhs_ACM.f90:292-922, 927-1779).
@@ -108,6 +114,7 @@ typedef struct {
 /* `prediction` (module_wavelets.f90:96-284) as whole-array passes:
  * coarse (nx,ny,nz) -> fine (2nx-1, 2ny-1, 2nz-1); nz = 1 in 2D. */
 void or_prediction(const double *coarse, int nx, int ny, int nz, double *fine, int order);
+void or_prediction_zorder(const double *coarse, int nx, int ny, int nz, double *fine, int order, int zorder);
 
 /* Ghost fill of depth gm/gp on a 2:1-balanced periodic leaf forest by point
  * location (see adapt.c).  Returns 0, -1 duplicate block, -2 not balanced,

@@ -102,7 +102,7 @@ void or_mask_block(const or_mask_params *mp, int dim, const int bs[3], int g, co
                    const double dx[3], double *mask);
 
 /* ---- adapt_tree on a periodic leaf forest ----------------------------- */
-enum { OR_IND_EVERYWHERE = 0, OR_IND_THRESHOLD = 1 };
+enum { OR_IND_EVERYWHERE = 0, OR_IND_THRESHOLD = 1, OR_IND_EXTERNAL = 2 };
 
 typedef struct {
     int dim, jmin, jmax, g, neq;
@@ -118,6 +118,17 @@ typedef struct {
     int security_zone;              /* addSecurityZone_CE_tree */
     double domain[3];
     or_mask_params mask;
+    /* wavelet_decompose_full_tree's branch: 0 the reference's rule
+     * (leaf-first iff every Bs >= 3 max|HD tap|, adapt_tree.f90:321-322),
+     * 1 leaf-first, 2 level-wise.  Forcing a branch is a test device: both
+     * must give the same decomposition. */
+    int decompose_branch;
+    /* planted mistakes, for tests that show a comparison has teeth (0 in
+     * every real use): bit 0 level-wise branch: leaves below the finest
+     * level are never decomposed; bit 1 level-wise branch: the ghost sync
+     * of iterations after the first is skipped; bit 2: the z pass of 3D
+     * prediction in ghost syncs is of order 2 */
+    int plant;
 } or_adapt_params;
 
 /* A forest with heavy data that the oracle owns.  n blocks in list order. */
@@ -136,6 +147,9 @@ int or_tree_sync_full(or_tree *T);
 int or_tree_refine_everywhere(or_tree *T);
 /* adapt_tree (adapt_tree.f90:11-255) without the final load balancing */
 int or_tree_adapt(or_tree *T);
+/* adapt_tree with indicator "nothing (external)": leaf_flags (-1/0/1) in
+ * or_tree_get's order replace coarseningIndicator_tree */
+int or_tree_adapt_flags(or_tree *T, const int *leaf_flags);
 /* the ACM time step on the leaves: min block dt (before the host clamps)
  * and one RungeKuttaGeneric step with sync_ghosts_RHS_tree of depth g_rhs
  * and the mask of every leaf rebuilt from mp (mp NULL: no mask) */

@@ -119,3 +119,41 @@ def test_duplicate_block_rejected(level):
     f.set_blocks(tcs[:3], lev[:3], np.arange(3, dtype=np.int32))   # usable again after the error
     assert f.n == 3
     f.close()
+
+
+def test_rejected_refine_leaves_forest_unchanged():
+    """A refine list with a bad entry after good ones (a block not active, a
+    block listed twice, one at Jmax) is rejected before any slot is taken:
+    the same valid refine succeeds afterwards, on the same free slots."""
+    from wabbit_amd import lib as L
+    f = Forest(2, 2, [8, 8], 3, [1.0, 1.0], nb_max=10)
+    tcs, lev = random_graded_forest(2, 1, 2, 0, 0)                # 4 blocks on level 1
+    f.set_blocks(tcs[:2], lev[:2], np.arange(2, dtype=np.int32))
+    for bad in ([0, 7], [0, 0]):
+        with pytest.raises(L.WabbitHipError):
+            f.refine(np.array(bad, np.int32))
+        assert f.n == 2
+    tri = f.refine(np.array([0], np.int32))                        # needs 4 of the 8 free slots
+    assert tri.shape == (4, 3) and f.n == 5
+    tc, lv, hv, _ = f.blocks()
+    with pytest.raises(L.WabbitHipError):                          # level 2 = Jmax, after a good entry
+        f.refine(np.array([1, int(hv[lv == 2][0])], np.int32))
+    assert f.n == 5
+    tri = f.refine(np.array([1], np.int32))                        # 4 free + the first mother's slot
+    assert tri.shape == (4, 3) and f.n == 8
+    f.close()
+
+
+def test_treecode_digits_beyond_level_rejected():
+    """A treecode with a non-zero digit below its block's level (or above
+    jmax) is rejected, and the forest is left as it was."""
+    from wabbit_amd import lib as L
+    from helpers import _tc
+    f = Forest(2, 4, [8, 8], 3, [1.0, 1.0], nb_max=8)
+    good = np.array([_tc((1, 0), 1, 4, 2)], np.int64)
+    f.set_blocks(good, np.array([1], np.int32), np.zeros(1, np.int32))
+    for bad in (good[0] | 1, good[0] | (1 << (4 * 2))):
+        with pytest.raises(L.WabbitHipError):
+            f.set_blocks(np.array([bad], np.int64), np.array([1], np.int32), np.zeros(1, np.int32))
+        assert f.n == 1 and f.blocks()[0][0] == good[0]
+    f.close()

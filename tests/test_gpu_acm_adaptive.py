@@ -26,10 +26,11 @@ def device_mask_params():
                          use_sponge=True, sponge_type=1, L_sponge=2.0, p_sponge=8.0, domain=(A.L, A.L, 1.0))
 
 
-def solver(path=2):
+def solver(path=2, decompose_branch=0):
     tc, lv, hvy, jmax_tc = A.initial_state()
     cfg = AdaptConfig(wavelet="CDF44", bs=BS, jmin=A.JMIN, jmax=A.JMAX, eps=1e-3, eps_normalized=True,
-                      groups=[1, 1, 1], threshold_mask=True, force_maxlevel_dealiasing=True)
+                      groups=[1, 1, 1], threshold_mask=True, force_maxlevel_dealiasing=True,
+                      decompose_branch=decompose_branch)
     return AdaptiveACM(A.acm_params(), A.time_params(), cfg, tc, lv, hvy, mask=device_mask_params(),
                        nb_max=2500, path=path, jmax_treecode=jmax_tc)
 
@@ -100,3 +101,51 @@ def test_adaptive_cylinder_run(path):
     assert s.iteration == int(ref10["iteration"]) == it_o
     _check(s, ref10, tree)
     s.close()
+
+
+def test_level_wise_branch_run():
+    """The level-wise decomposition forced on the device (Bs = 26 takes the
+    leaf-first branch): the reference's saved state at t = 0.05."""
+    s = solver(2, decompose_branch=2)
+    ref05 = load_npz("acm_cyl_t005.npz")
+    while s.time < 0.05 - 1e-12:
+        s.step()
+    assert s.iteration == int(ref05["iteration"])
+    _check(s, ref05)
+    s.close()
+
+
+def test_save_and_resume(tmp_path):
+    """save_data at t = 0.05 (one WABBIT HDF5 file per component), a fresh
+    solver resumed from the files (read_from_files), on to t = 0.1: the
+    same result as the uninterrupted run, bit for bit, and the reference's
+    saved state."""
+    from wabbit_amd import h5io
+    if not h5io.available():
+        pytest.skip("libhdf5 not found")
+    a = solver(2)
+    while a.time < 0.05 - 1e-12:
+        a.step()
+    paths = a.save(str(tmp_path))
+    assert [p.rsplit("/", 1)[1] for p in paths] == ["ux_000000050000.h5", "uy_000000050000.h5",
+                                                     "p_000000050000.h5"]
+    f = h5io.read_wabbit_file(paths[0])
+    ref05 = load_npz("acm_cyl_t005.npz")
+    key = {(int(x), int(y)): k for k, (x, y) in enumerate(zip(f.treecode, f.level))}
+    mine = np.stack([f.blocks[key[(int(x), int(y))]] for x, y in zip(ref05["treecode"], ref05["level"])])
+    assert _rel(mine[:, :BS, :BS], ref05["field_ux"][:, :BS, :BS]) <= 1e-12
+    b = solver(2)
+    b.load(paths)
+    assert (b.time, b.iteration) == (a.time, a.iteration)
+    while a.time < 0.1 - 1e-12:
+        a.step()
+        b.step()
+    assert (b.time, b.iteration) == (a.time, a.iteration)
+    ta, la, da = a.state()
+    tb, lb, db = b.state()
+    assert np.array_equal(ta, tb) and np.array_equal(la, lb)
+    inner = (slice(None), slice(None), 0, slice(G, G + BS), slice(G, G + BS))
+    assert np.array_equal(da[inner], db[inner])
+    _check(b, load_npz("acm_cyl_t010.npz"))
+    a.close()
+    b.close()

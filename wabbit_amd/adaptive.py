@@ -35,6 +35,7 @@ class AdaptConfig:
     use_ce: Optional[bool] = None          # default: lifted wavelet
     security_zone: Optional[bool] = None   # default: lifted wavelet
     refinement_indicator: str = "everywhere"
+    decompose_branch: int = 0              # 0 the reference's rule, 1 leaf-first, 2 level-wise
 
 
 def fd_reach(scheme: int) -> int:
@@ -76,7 +77,7 @@ class AdaptiveACM:
             force_maxlevel_dealiasing=cfg.force_maxlevel_dealiasing, eps_normalized=cfg.eps_normalized,
             threshold_mask=cfg.threshold_mask and mask is not None,
             security_zone=lifted if cfg.security_zone is None else cfg.security_zone,
-            domain=acm.domain_size, mask=mask)
+            domain=acm.domain_size, mask=mask, decompose_branch=cfg.decompose_branch)
         self.acm_c = L.acm_params_c(acm)
         self.butcher = butcher_rk4()
         self.tp_c = L.time_params_c(tp)
@@ -221,6 +222,40 @@ class AdaptiveACM:
         order = np.lexsort((tc, lv))
         data = np.stack([self.pool.download(L.WH_BLOCK, int(h), 1)[0] for h in hv[order]])
         return tc[order], lv[order], data
+
+    def save(self, directory: str, names=None) -> list:
+        """save_data (LIB/IO/save_data.f90:70-84): one WABBIT HDF5 file per
+        component, ``<name>_<timestr>.h5``, written after a sync_ghosts_tree
+        as saveHDF5_tree does (no_sync = .false.).  Returns the paths."""
+        import os
+        from . import h5io
+        if names is None:
+            names = ["ux", "uy", "uz"][:self.dim] + ["p"]
+        W.sync_tree(self.pool, self.forest)
+        tc, lv, data = self.state()
+        files = h5io.files_from_state(tc, lv, data, self.g, [self.cfg.bs] * self.dim, self.dim,
+                                      self.acm.domain_size, self.forest.jmax, self.time, self.iteration)
+        paths = []
+        for name, f in zip(names, files):
+            path = os.path.join(directory, f"{name}_{h5io.timestr(self.time)}.h5")
+            h5io.write_wabbit_file(path, f)
+            paths.append(path)
+        return paths
+
+    def load(self, paths) -> None:
+        """Resume from saved files (read_from_files = 1:
+        setInitialCondition_tree.f90:52-60 -> readHDF5vct_tree,
+        InputOutput.f90:322-764): the files' leaves and data replace the
+        current ones, time and iteration are taken from the files.  Ghost
+        nodes are filled by the next step's sync_ghosts_tree."""
+        from . import h5io
+        files = [h5io.read_wabbit_file(p) for p in paths]
+        tc, lv, data, time, it = h5io.state_from_files(files, self.g, max_level=self.forest.jmax)
+        if data.shape[1] != self.acm.neq:
+            raise ValueError(f"{data.shape[1]} component files for {self.acm.neq} equations")
+        self.set_blocks(tc, lv, data)
+        self.time, self.iteration = float(time), int(it)
+        self._dev_time = None
 
     def mask(self):
         self._fresh()

@@ -35,13 +35,38 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = True) -> str:
+    """Compile every source to an object in parallel, then link the library."""
+    from concurrent.futures import ThreadPoolExecutor
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     out = os.path.join(PKG, LIBNAME)
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        cmd = [hipcc()] + cflags + ["-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(compile_one, srcs))
+    objs = []
+    for obj, res in results:
+        _report(res, verbose)
+        objs.append(obj)
     tmp = out + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    _report(subprocess.run(cmd, capture_output=True, text=True), verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def _report(res, verbose):
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed")
@@ -51,8 +76,6 @@ def build(verbose: bool = True) -> str:
         lines = [ln for ln in res.stderr.splitlines() if "packed-fp32-ops" not in ln]
         if any(ln.strip() for ln in lines):
             sys.stderr.write("\n".join(lines) + "\n")
-    os.replace(tmp, out)
-    return out
 
 
 if __name__ == "__main__":

@@ -654,6 +654,8 @@ int decompose_full_tree(Tree &T, const wh_adapt_params &p)
     bool leaf_first = true;
     for (int d = 0; d < T.dim; ++d)
         if (T.bs[d] < 3 * hmax) leaf_first = false;
+    if (p.decompose_branch == 1) leaf_first = true;
+    if (p.decompose_branch == 2) leaf_first = false;
     int level = T.max_level();
     std::vector<int> all(n_leaf);
     for (int i = 0; i < n_leaf; ++i) {
@@ -1030,18 +1032,28 @@ int wh_sync_tree(wh_pool *P, wh_forest *F, int array)
     return tree_sync(T, S);
 }
 
-int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
+namespace wh {
+
+// adapt_tree (adapt_tree.f90:11-255); leaf_flags: the external statuses of
+// the leaves in forest list order (indicator "nothing (external)")
+static int adapt_tree_impl(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_t *leaf_flags)
 {
-    WH_PROF("api.adapt_tree");
-    using namespace wh;
     if (int rc = check_pool(P, F)) return rc;
     WH_CHECK(p && p->jmin >= 0 && p->jmax <= F->jmax && p->jmin <= p->jmax &&
-                 (p->indicator == WH_IND_EVERYWHERE || p->indicator == WH_IND_THRESHOLD),
+                 (p->indicator == WH_IND_EVERYWHERE || p->indicator == WH_IND_THRESHOLD ||
+                  (p->indicator == WH_IND_EXTERNAL && leaf_flags)) &&
+                 p->decompose_branch >= 0 && p->decompose_branch <= 2,
              WH_E_ARG, "wh_adapt_tree: bad parameters");
     for (const FBlock &b : F->blocks)
         WH_CHECK(b.level >= p->jmin, WH_E_ARG, "wh_adapt_tree: a block is below jmin");
     const int stay = REF_UNSIGNIFICANT_STAY;
     Tree T(F, P);
+    if (leaf_flags)  // the leaves are T.b[0, n) in forest list order
+        for (size_t i = 0; i < F->blocks.size(); ++i) {
+            const int f = leaf_flags[i];
+            WH_CHECK(f == -1 || f == 0 || f == 1, WH_E_ARG, "wh_adapt_tree_flags: a flag is not -1, 0 or 1");
+            T.b[i].status = f;
+        }
     const std::vector<int> free0 = F->free_hvy;
     auto fail = [&](int rc) {
         // restore the forest's slots; device data of the leaves is not restored
@@ -1049,10 +1061,15 @@ int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
         (void)stage_sync(P);
         return rc;
     };
+    // the decomposition hands the leaves' statuses back (its backup), the
+    // new mothers get 0
     if (int rc = decompose_full_tree(T, *p)) return fail(rc);
-    // coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331)
-    for (TNode &x : T.b) x.status = 0;
-    if (p->indicator == WH_IND_EVERYWHERE) {
+    // coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331); with
+    // the external indicator it returns at once (:55)
+    if (p->indicator != WH_IND_EXTERNAL)
+        for (TNode &x : T.b) x.status = 0;
+    if (p->indicator == WH_IND_EXTERNAL) {
+    } else if (p->indicator == WH_IND_EVERYWHERE) {
         for (int i = 0; i < (int)T.b.size(); ++i)
             if (T.is_leaf(i, false)) T.b[i].status = -1;
     } else {
@@ -1111,4 +1128,22 @@ int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
     return tree_sync(T, S);
 }
 
-}  // extern "C"
+}  // namespace wh
+
+int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
+{
+    WH_PROF("api.adapt_tree");
+    WH_CHECK(p && p->indicator != WH_IND_EXTERNAL, WH_E_ARG,
+             "wh_adapt_tree: the external indicator needs wh_adapt_tree_flags");
+    return wh::adapt_tree_impl(P, F, p, nullptr);
+}
+
+int wh_adapt_tree_flags(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_t *leaf_flags, int n)
+{
+    WH_PROF("api.adapt_tree");
+    WH_CHECK(P && F && p && p->indicator == WH_IND_EXTERNAL && leaf_flags && n == (int)F->blocks.size(),
+             WH_E_ARG, "wh_adapt_tree_flags: needs WH_IND_EXTERNAL and one flag per leaf");
+    return wh::adapt_tree_impl(P, F, p, leaf_flags);
+}
+
+}  // extern "C""

@@ -99,6 +99,14 @@ int wh_forest_set_blocks(wh_forest *F, int n, const int64_t *tc, const int32_t *
             wh::set_error("wh_forest_set_blocks: hvy id / level out of range or duplicate");
             return WH_E_ARG;
         }
+        // the digits of the levels below the block's own must be zero
+        // (module_treelib.f90: a block's treecode holds level[i] digits)
+        const int low_bits = (F->jmax - level[i]) * F->dim;
+        if (tc[i] < 0 || (low_bits > 0 && (tc[i] & ((int64_t(1) << low_bits) - 1)) != 0) ||
+            (F->jmax * F->dim < 63 && (tc[i] >> (F->jmax * F->dim)) != 0)) {
+            wh::set_error("wh_forest_set_blocks: treecode has digits beyond its level or beyond jmax");
+            return WH_E_ARG;
+        }
         used[h] = 1;
         nb.push_back(Block{tc[i], level[i], hvy[i], 0});
     }
@@ -365,23 +373,27 @@ int wh_forest_refine(wh_forest *F, int n, const int32_t *hvy_list, int32_t *pair
         const int h = F->blocks[i].hvy - F->slot0;
         if (h >= 0 && h < F->nb_max) by_hvy[h] = (int)i;
     }
+    // every entry is checked before a slot is taken: a rejected list leaves
+    // the forest (blocks and free slots) as it was
     std::vector<char> remove(F->blocks.size(), 0);
-    std::vector<Block> added;
-    added.reserve((size_t)n * nd);
-    int np = 0;
     for (int i = 0; i < n; ++i) {
         const int h = hvy_list[i] - F->slot0;
         const int bi = (h >= 0 && h < F->nb_max) ? by_hvy[h] : -1;
         if (bi < 0 || remove[bi]) {
-            wh::set_error("wh_forest_refine: block not active");
+            wh::set_error("wh_forest_refine: block not active or listed twice");
             return WH_E_ARG;
         }
-        const Block m = F->blocks[bi];
-        if (m.level >= F->jmax) {
+        if (F->blocks[bi].level >= F->jmax) {
             wh::set_error("wh_forest_refine: block already at Jmax");
             return WH_E_ARG;
         }
         remove[bi] = 1;
+    }
+    std::vector<Block> added;
+    added.reserve((size_t)n * nd);
+    int np = 0;
+    for (int i = 0; i < n; ++i) {
+        const Block m = F->blocks[by_hvy[hvy_list[i] - F->slot0]];
         for (int k = 0; k < nd; ++k) {
             const int h = F->free_hvy.back();
             F->free_hvy.pop_back();

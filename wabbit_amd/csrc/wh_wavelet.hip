@@ -1249,6 +1249,26 @@ int wh_detail_blocks(wh_pool *pool, int array, const int32_t *ids, int n, const 
     return WH_OK;
 }
 
+int wh_fwt_threshold(wh_pool *pool, int src_array, int dst_array, const int32_t *ids, int n, const double *thresh,
+                     const int32_t *groups, int32_t *flags_out, double *detail_out)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || (ids && thresh && flags_out && detail_out)), WH_E_ARG,
+             "wh_fwt_threshold: bad argument");
+    const std::vector<int32_t> list(ids, ids + n);
+    if (int rc = wh::run_blocks_op(pool, wh::WOP_FWT, src_array, dst_array, list)) return rc;
+    std::vector<double> d;
+    if (int rc = wh::run_detail(pool, dst_array, list, groups, d)) return rc;
+    const int neq = pool->d.neq;
+    for (int k = 0; k < n; ++k) {
+        bool all = true;  // threshold_block.f90:119-124
+        for (int c = 0; c < neq; ++c)
+            if (!(d[(size_t)k * neq + c] <= thresh[c])) all = false;
+        flags_out[k] = all ? -1 : 0;
+    }
+    std::copy(d.begin(), d.end(), detail_out);
+    return WH_OK;
+}
+
 int wh_apply_boxes(wh_pool *pool, int array, int src_array, const wh_box *boxes, int n)
 {
     WH_CHECK(pool && n >= 0 && (n == 0 || boxes), WH_E_ARG, "wh_apply_boxes: bad argument");

@@ -29,7 +29,7 @@ module wabbit_hip
     end type
 
     integer(c_int), parameter :: WH_WAVELET_TAPS = 33
-    integer(c_int), parameter :: WH_IND_EVERYWHERE = 0, WH_IND_THRESHOLD = 1
+    integer(c_int), parameter :: WH_IND_EVERYWHERE = 0, WH_IND_THRESHOLD = 1, WH_IND_EXTERNAL = 2
 
     type, bind(C) :: wh_wavelet
         integer(c_int) :: X, Y, lifted, specialised
@@ -56,6 +56,7 @@ module wabbit_hip
         integer(c_int) :: eps_normalized, threshold_mask, security_zone
         real(c_double) :: domain(3)
         type(wh_mask_params) :: mask
+        integer(c_int) :: decompose_branch
     end type
 
     type, bind(C) :: wh_time_params
@@ -215,6 +216,25 @@ module wabbit_hip
             import :: c_ptr, c_int, wh_adapt_params
             type(c_ptr), value :: pool, forest
             type(wh_adapt_params), intent(in) :: p
+            integer(c_int) :: rc
+        end function
+        function wh_adapt_tree_flags(pool, forest, p, leaf_flags, n) bind(C, name="wh_adapt_tree_flags") result(rc)
+            import :: c_ptr, c_int, c_int32_t, wh_adapt_params
+            type(c_ptr), value :: pool, forest
+            type(wh_adapt_params), intent(in) :: p
+            integer(c_int32_t), intent(in) :: leaf_flags(*)
+            integer(c_int), value :: n
+            integer(c_int) :: rc
+        end function
+        function wh_fwt_threshold(pool, src_array, dst_array, hvy_ids, n, thresh, groups, flags_out, detail_out) &
+                bind(C, name="wh_fwt_threshold") result(rc)
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: pool
+            integer(c_int), value :: src_array, dst_array, n
+            integer(c_int32_t), intent(in) :: hvy_ids(*), groups(*)
+            real(c_double), intent(in) :: thresh(*)
+            integer(c_int32_t), intent(out) :: flags_out(*)
+            real(c_double), intent(out) :: detail_out(*)
             integer(c_int) :: rc
         end function
         function wh_forest_refine(forest, n, hvy_ids, triples, cap, ntriples) &

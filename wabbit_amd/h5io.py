@@ -86,6 +86,15 @@ def _lib():
     return lib
 
 
+def available() -> bool:
+    """True if a libhdf5 shared library can be loaded."""
+    try:
+        _lib()
+        return True
+    except OSError:
+        return False
+
+
 def _type(name):
     return _hid_t.in_dll(_lib(), name).value
 
@@ -257,3 +266,72 @@ def write_wabbit_file(path: str, f: WabbitFile) -> None:
         _write_attr(fid, "blocks", "dim", np.array([f.dim], np.int32))
     finally:
         lib.H5Fclose(fid)
+
+
+def files_from_state(treecodes, levels, data, g, bs, dim, domain, max_level, time, iteration):
+    """One WabbitFile per component of a leaf state (data: (n, neq, nz, ny,
+    nx) with synchronised ghost nodes) in saveHDF5_tree's layout: each block
+    stores its interior plus the first upper ghost point (InputOutput.f90:
+    174-211).  Origins and spacings follow get_block_spacing_origin."""
+    from .forest import ijk_from_treecode, spacing_origin
+    tc = np.asarray(treecodes, np.int64)
+    lv = np.asarray(levels, np.int32)
+    n = len(tc)
+    bsl = [int(b) for b in (list(bs) + [1] * 3)[:3]]
+    org = np.zeros((n, dim))
+    spc = np.zeros((n, dim))
+    for k in range(n):
+        ijk = ijk_from_treecode(int(tc[k]), int(lv[k]), max_level, dim)
+        x0, dx = spacing_origin(ijk, int(lv[k]), domain, bsl[:dim], dim)
+        org[k], spc[k] = x0[:dim], dx[:dim]
+    if dim == 2:
+        sl = (slice(None), 0, slice(g, g + bsl[1] + 1), slice(g, g + bsl[0] + 1))
+    else:
+        sl = (slice(None), slice(g, g + bsl[2] + 1), slice(g, g + bsl[1] + 1), slice(g, g + bsl[0] + 1))
+    out = []
+    for c in range(data.shape[1]):
+        blocks = np.ascontiguousarray(data[:, c][sl])
+        out.append(WabbitFile(blocks=blocks, treecode=tc.copy(), level=lv.copy(),
+                              refinement_status=np.zeros(n, np.int32), origin=org, spacing=spc,
+                              time=float(time), iteration=int(iteration), dim=dim,
+                              domain_size=np.asarray(list(domain)[:dim], np.float64),
+                              bs=np.asarray(bsl[:dim], np.int32), max_level=int(max_level)))
+    return out
+
+
+def state_from_files(files, g, max_level=None):
+    """readHDF5vct_tree (InputOutput.f90:322-764) into a leaf state: (treecodes,
+    levels, data (n, neq, nz, ny, nx), time, iteration).  Each file is one
+    component; the stored interior (plus first upper ghost point) goes to
+    the block's interior, every other ghost node is left 0 for the caller's
+    sync_ghosts_tree.  Treecodes are re-based to ``max_level`` digits if
+    given (the digit of level l sits at bit (max_level - l)*dim)."""
+    f0 = files[0]
+    for f in files[1:]:
+        if not (np.array_equal(f.treecode, f0.treecode) and np.array_equal(f.level, f0.level)):
+            raise ValueError("component files hold different grids")
+    dim = f0.dim
+    bsl = [int(b) for b in f0.bs] + [1] * (3 - dim)
+    n = len(f0.treecode)
+    nx, ny = bsl[0] + 2 * g, bsl[1] + 2 * g
+    nz = bsl[2] + 2 * g if dim == 3 else 1
+    data = np.zeros((n, len(files), nz, ny, nx))
+    for c, f in enumerate(files):
+        if dim == 2:
+            data[:, c, 0, g:g + bsl[1] + 1, g:g + bsl[0] + 1] = f.blocks
+        else:
+            data[:, c, g:g + bsl[2] + 1, g:g + bsl[1] + 1, g:g + bsl[0] + 1] = f.blocks
+    tc = f0.treecode.astype(np.int64)
+    if max_level is not None and max_level != f0.max_level:
+        sh = (int(max_level) - int(f0.max_level)) * dim
+        tc = tc << sh if sh >= 0 else tc >> (-sh)
+    return tc, f0.level.astype(np.int32), data, f0.time, f0.iteration
+
+
+def timestr(time: float) -> str:
+    """The 12-digit time stamp of the reference's file names (module_helpers.f90
+    timestr: whole part i6.6, then the fraction in millionths i6.6)."""
+    import math
+    whole = int(time + 1e-12)
+    frac = int(round(max((time - math.floor(time + 1e-12)) * 1e6, 0.0)))
+    return f"{whole:06d}{frac:06d}"

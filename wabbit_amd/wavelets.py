@@ -19,7 +19,7 @@ _vp = ctypes.c_void_p
 _BOUND = False
 
 WH_BOX_ZERO_WC, WH_BOX_COPY_SC = 0, 1
-WH_IND_EVERYWHERE, WH_IND_THRESHOLD = 0, 1
+WH_IND_EVERYWHERE, WH_IND_THRESHOLD, WH_IND_EXTERNAL = 0, 1, 2
 
 
 class WaveletC(ctypes.Structure):
@@ -59,7 +59,7 @@ class AdaptParamsC(ctypes.Structure):
                 ("eps", ctypes.c_double), ("norm", ctypes.c_double * 8),
                 ("groups", ctypes.c_int32 * 8), ("eps_normalized", ctypes.c_int),
                 ("threshold_mask", ctypes.c_int), ("security_zone", ctypes.c_int),
-                ("domain", ctypes.c_double * 3), ("mask", MaskParamsC)]
+                ("domain", ctypes.c_double * 3), ("mask", MaskParamsC), ("decompose_branch", ctypes.c_int)]
 
 
 def mask_params(geometry=0, x_cntr=(0.0, 0.0, 0.0), R_cyl=0.5, smoothing_width=0.0, smoothing_safety=0.0,
@@ -110,6 +110,8 @@ def _bind():
         "wh_apply_boxes": [_vp, ip, ip, _vp, ip],
         "wh_run_patches": [_vp, ip, ip, ip, _vp, ip],
         "wh_adapt_tree": [_vp, _vp, ctypes.POINTER(AdaptParamsC)],
+        "wh_adapt_tree_flags": [_vp, _vp, ctypes.POINTER(AdaptParamsC), _vp, ip],
+        "wh_fwt_threshold": [_vp, ip, ip, _vp, ip, _vp, _vp, _vp, _vp],
         "wh_sync_tree": [_vp, _vp, ip],
         "wh_create_mask": [_vp, ctypes.POINTER(MaskParamsC), _vp, ip, _vp],
     }
@@ -184,12 +186,13 @@ def run_patches(pool, array, patches, alt1=-1, alt2=-1):
 def adapt_params(jmin, jmax, indicator=WH_IND_THRESHOLD, use_ce=True, eps=1e-3, norm=None,
                  groups=None, force_maxlevel_dealiasing=False, neq=1, eps_normalized=False,
                  threshold_mask=False, security_zone=None, domain=(1.0, 1.0, 1.0),
-                 mask=None) -> AdaptParamsC:
+                 mask=None, decompose_branch=0) -> AdaptParamsC:
     """adapt_tree settings; security_zone defaults to use_ce, as the
     reference's useSecurityZone defaults to the lifted-wavelet flag
     (ini_file_to_params.f90:543-546)."""
     p = AdaptParamsC()
     p.jmin, p.jmax, p.indicator = jmin, jmax, indicator
+    p.decompose_branch = int(decompose_branch)
     p.use_ce = int(use_ce)
     p.eps_normalized = int(eps_normalized)
     p.threshold_mask = int(threshold_mask)
@@ -211,6 +214,29 @@ def adapt_tree(pool, forest, params: AdaptParamsC):
     becomes the new leaf list (re-attach the pool's tables afterwards)."""
     _bind()
     L.check(L.load().wh_adapt_tree(pool.h, forest.h, ctypes.byref(params)))
+
+
+def adapt_tree_flags(pool, forest, params: AdaptParamsC, leaf_flags):
+    """adapt_tree with the host's refinement flags (indicator "nothing
+    (external)"): one flag per leaf in the forest's list order."""
+    _bind()
+    f = np.ascontiguousarray(leaf_flags, np.int32)
+    L.check(L.load().wh_adapt_tree_flags(pool.h, forest.h, ctypes.byref(params), f.ctypes.data if f.size else None,
+                                         f.size))
+
+
+def fwt_threshold(pool, src, dst, ids, thresh, groups):
+    """coarseningIndicator_block / threshold_block of the listed blocks:
+    returns (flags (n,) in {-1, 0}, detail (n, neq))."""
+    _bind()
+    a = _ids(ids)
+    th = np.ascontiguousarray(thresh, np.float64)
+    g = np.ascontiguousarray(groups, np.int32)
+    flags = np.zeros(a.size, np.int32)
+    det = np.zeros((a.size, pool.neq))
+    L.check(L.load().wh_fwt_threshold(pool.h, src, dst, a.ctypes.data if a.size else None, a.size, th.ctypes.data,
+                                      g.ctypes.data, flags.ctypes.data, det.ctypes.data))
+    return flags, det
 
 
 def sync_tree(pool, forest, array=L.WH_BLOCK):
