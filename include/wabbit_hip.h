@@ -200,6 +200,49 @@ int wh_dt_reduce_groups(wh_pool *pool, const wh_acm_params *p, double *dt_out);
 int wh_rk_step_groups(wh_pool *pool, const wh_acm_params *p, const double *butcher, int nrows,
                       const double *dt_groups, int path);
 
+/* Physics constants per block group (independent forests of one pool may
+ * differ in c_0, nu, gamma_p, C_eta, C_sponge, u_mean_set, CFL numbers,
+ * penalization and sponge; they share dim, scheme and skew_symmetry, which
+ * select the kernels).  params[ngroups], ngroups = the count given to
+ * wh_set_block_groups (which, like wh_set_active, clears them); ngroups = 0
+ * clears.  Used by wh_dt_reduce_groups and wh_batch_rk_step in place of the
+ * constants of their `p` argument (type_params_acm, module_ACM.f90:56-149,
+ * one per forest in the reference). */
+int wh_set_group_params(wh_pool *pool, const wh_acm_params *params, int ngroups);
+
+/* ---- batches: RungeKuttaGeneric over several pools at once -------------
+ * The reference steps one forest per process (runge_kutta_generic.f90:1-156,
+ * main.f90:350).  A batch steps the block groups of several pools of one
+ * dimension (any block sizes, e.g. one pool per Bs class and host thread)
+ * together: per RK stage one ghost fill per sync stage over the patch tables
+ * of all pools and ONE fused stage kernel over the active blocks of all
+ * pools (persistent kernels over device-resident tables of entries and
+ * patches; the fused path's stage sequence, wh_rk_step path 2).  Each pool
+ * needs its active list, patch table (case 0) and block groups set, three
+ * work slots, and a mask where a group is penalised.  Per block the
+ * arithmetic is that of wh_rk_step_groups, so each forest's result does not
+ * depend on the batch.  The batch has its own stream, ordered after the work
+ * queued on every pool's stream and before their later work. */
+typedef struct wh_batch wh_batch;
+int wh_batch_create(wh_batch **out, wh_pool *const *pools, int npools);
+int wh_batch_destroy(wh_batch *batch);
+/* One step of every group of every pool: dt_groups[] holds one dt per group,
+ * the groups of pool 0 first, then those of pool 1, ...  The constants come
+ * from a pool's wh_set_group_params where set, else from *p.  use_graph = 1
+ * replays the step from a hipGraph captured once (re-captured only when a
+ * table buffer grows or the tableau, scheme or pool list changes); the
+ * tables are uploaded before each replay (no memset nodes). */
+int wh_batch_rk_step(wh_batch *batch, const wh_acm_params *p, const double *butcher, int nrows,
+                     const double *dt_groups, int use_graph);
+/* Average duration of one launch of the batch's fused stage kernel over
+ * `reps` RK stage sequences back to back on the last step's tables, one HIP
+ * event pair around them; the last stage writes a work slot, not the state.
+ * *blocks: the blocks per launch.  No counterpart in the reference
+ * (measurement). */
+int wh_batch_time_stage_kernel(wh_batch *batch, const wh_acm_params *p, const double *butcher, int nrows,
+                               int reps, double *ms, int *blocks);
+int wh_batch_synchronize(wh_batch *batch);
+
 /* refineBlock (LIB/MESH/refinementExecute.f90:4-116) for a list of
  * (mother, daughter, digit) triples: daughter `digit` of a mother takes its
  * quadrant/octant of the mother's interpolated grid (prediction of order

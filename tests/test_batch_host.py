@@ -22,7 +22,10 @@ def test_pool_keys_split_shapes_and_physics():
     s24 = wl.channel_spec(2, 4, 24, 1, 8)
     s16b = wl.channel_spec(3, 5, 16, 1, 8)
     assert b.key(s16) != b.key(s24)
-    assert b.key(s16) == b.key(s16b)          # same shape and physics, different geometry and Jmax
+    assert b.key(s16) == b.key(s16b)          # same shape, different geometry and Jmax
+    s16c = wl.channel_spec(4, 4, 16, 1, 8)
+    s16c.acm.c_0, s16c.acm.nu = 10.0, 2e-3
+    assert b.key(s16) == b.key(s16c)          # physics constants travel per group, not per pool
 
 
 def test_ensemble_members():

@@ -2,17 +2,25 @@
 
 The reference runs one forest per process (LIB/MAIN/main.f90); here a batch of
 independent solves -- an ensemble, or the requests of a serving loop -- share
-one block pool per block shape (dim, Bs, wavelet, physics constants) and one
-launch per RK-stage operation: every solve's blocks form one active list
-with a group index per block (wh_set_block_groups), the ghost-patch tables of
-all solves are concatenated (patches never cross solves), the block-dt
-reduction runs per group (wh_dt_reduce_groups) and each solve steps with its
-own dt (wh_rk_step_groups).  Mesh work (sync_ghosts_tree, refine_tree,
-createMask_tree, adapt_tree) runs per solve on the shared pool, each solve
-inside its own range of pool slots (wh_forest_set_slot_base).
+one block pool per block shape (dim, Bs, wavelet, scheme, masked) and host
+thread ("lane").  Per main-loop iteration:
 
-Each solve's arithmetic is exactly that of AdaptiveACM on its own, so a
-solve's result does not depend on what else is in the batch.
+  A. per solve, on its lane's pool (lanes in parallel): sync_ghosts_tree,
+     refine_tree "everywhere", createMask_tree, the RHS ghost patches; per
+     pool the union active list with a group index per block
+     (wh_set_block_groups), each group's physics constants
+     (wh_set_group_params) and the block-dt minimum per group
+     (wh_dt_reduce_groups), clamped per solve on the host;
+  B. ONE RungeKuttaGeneric step for all pools of a dimension
+     (wh_batch_rk_step): per RK stage one ghost fill per sync stage and one
+     fused stage kernel over every block of every pool, replayed from a
+     hipGraph;
+  C. per solve (lanes in parallel): the time advance and adapt_tree.
+
+Mesh work runs per solve, each inside its own range of pool slots
+(wh_forest_set_slot_base).  Each solve's arithmetic is exactly that of
+AdaptiveACM on its own, so a solve's result does not depend on what else is
+in the batch.
 """
 from __future__ import annotations
 
@@ -70,9 +78,11 @@ class SlotRanges:
         return sum(self.used.values())
 
 
-def _acm_key(a: ACMParams):
-    return (a.dim, a.scheme, bool(a.skew_symmetry), bool(a.penalization), bool(a.use_sponge), a.c_0, a.nu,
-            a.gamma_p, a.C_eta, a.C_sponge, tuple(a.u_mean_set), a.CFL, a.CFL_nu, a.CFL_eta)
+def pool_key(spec: "SolveSpec"):
+    """What a pool is shared by: the block shape and the kernels' template
+    choices; physics constants travel per group."""
+    return (spec.acm.dim, spec.cfg.bs, spec.cfg.wavelet, spec.mask is not None, spec.acm.scheme,
+            bool(spec.acm.skew_symmetry))
 
 
 class Solve:
@@ -84,6 +94,7 @@ class Solve:
         self.rk_blocks = 0
         self.error: Optional[str] = None   # set when the solve failed (e.g. out of its slots)
         cfg, acm = spec.cfg, spec.acm
+        self.acm_c = L.acm_params_c(acm)
         self.forest = Forest(acm.dim, cfg.jmax, [cfg.bs] * acm.dim, group.w.g, acm.domain_size[:acm.dim],
                              spec.slots, slot_base=base)
         self.mask_p = W.mask_params(**spec.mask) if spec.mask is not None else None
@@ -135,21 +146,24 @@ class Solve:
 
 
 class PoolGroup:
-    """One shared pool for the solves of one block shape and physics."""
+    """One shared pool for the solves of one block shape (pool_key)."""
 
-    def __init__(self, acm: ACMParams, bs: int, wavelet: str, n_slots: int, masked: bool,
-                 n_work_slots: int = 5):
+    def __init__(self, key, n_slots: int, n_work_slots: int = 3):
         """n_work_slots: 3 suffice for the fused path and adapt_tree, the
         per-block path needs 5 (RK4)."""
-        self.acm = acm
-        self.w = W.setup(wavelet, fd_reach(acm.scheme))
-        self.pool = L.Pool(acm.dim, [bs] * acm.dim, self.w.g, acm.neq, n_slots, n_work_slots=n_work_slots,
-                           n_tmp_eqn=acm.neq, n_mask=6 if masked else 0, mask_per_block=True)
+        dim, bs, wavelet, masked, scheme, skew = key
+        self.key = key
+        self.dim = dim
+        self.w = W.setup(wavelet, fd_reach(scheme))
+        neq = dim + 1
+        self.pool = L.Pool(dim, [bs] * dim, self.w.g, neq, n_slots, n_work_slots=n_work_slots,
+                           n_tmp_eqn=neq, n_mask=6 if masked else 0, mask_per_block=True)
         W.set_wavelet(self.pool, self.w)
         self.slots = SlotRanges(n_slots)
-        self.acm_c = L.acm_params_c(acm)
         self.solves: List[Solve] = []
         self.last_stages = 0
+        self._live: List[Solve] = []
+        self._dts = np.zeros(0)
 
     def admit(self, rid, spec: SolveSpec) -> Optional[Solve]:
         base = self.slots.alloc(spec.slots)
@@ -170,14 +184,11 @@ class PoolGroup:
         s.close()
         self.slots.free(s.base)
 
-    def step(self, path=PATH_FUSED, butcher=None):
-        """One main-loop iteration of every solve of the group: per solve
-        sync_ghosts_tree + refine everywhere + mask, then one RK step of all
-        solves together (one launch per stage operation), then per solve
-        the time advance and adapt_tree.  Returns the block-RK-stages done."""
-        if not self.solves:
-            return 0
-        bt = butcher_rk4() if butcher is None else butcher
+    def prepare(self):
+        """Phase A: per solve sync_ghosts_tree + refine everywhere + mask,
+        then the pool's union active list, patches, groups, group constants
+        and clamped dt per group.  A pool without live solves gets empty
+        tables."""
         pool, w = self.pool, self.w
         hvs, xs, grp, pats, live = [], [], [], [], []
         for s in self.solves:
@@ -202,27 +213,48 @@ class PoolGroup:
             grp.append(np.full(len(hv), len(live), np.int32))
             pats.append(pat)
             live.append(s)
-        self.last_stages = 0
+        self._live = live
         if not live:
-            return 0
+            pool.set_active(np.zeros(0, np.int32), np.zeros((0, 6)))
+            pool.set_patches(0, np.zeros(0, L.PATCH_DTYPE))
+            self._dts = np.zeros(0)
+            return
         pool.set_active(np.concatenate(hvs), np.concatenate(xs))
         pool.set_patches(0, np.concatenate(pats))
         pool.set_block_groups(np.concatenate(grp), len(live))
-        dts = pool.dt_groups(self.acm_c)
-        dts = np.array([clamp_dt(d, s.time, s.spec.tp) for d, s in zip(dts, live)])
-        pool.rk_step_groups(self.acm_c, bt, dts, path)
+        pool.set_group_params([s.acm_c for s in live])
+        dts = pool.dt_groups(live[0].acm_c)
+        self._dts = np.array([clamp_dt(d, s.time, s.spec.tp) for d, s in zip(dts, live)])
+
+    def finish(self):
+        """Phase C: time advance and adapt_tree per solve; returns the
+        block-RK-stages of the step."""
         stages = 0
-        for s, dt in zip(live, dts):
+        for s, dt in zip(self._live, self._dts):
             s.time += float(dt)
             s.iteration += 1
             stages += 4 * s.rk_blocks
         self.last_stages = stages
-        for s in live:
+        for s in self._live:
             try:
-                W.adapt_tree(pool, s.forest, s.adapt_p)
+                W.adapt_tree(self.pool, s.forest, s.adapt_p)
             except L.WabbitHipError as e:
                 s.error = str(e)
+        self._live = []
         return stages
+
+    def step_alone(self, path=PATH_FUSED, butcher=None):
+        """One main-loop iteration of the group's solves with the pool's own
+        RK launches (path 1: one launch per block; path 2: the fused stages
+        of this pool only)."""
+        self.prepare()
+        if self._live:
+            bt = butcher_rk4() if butcher is None else butcher
+            if any(_differs(s.acm_c, self._live[0].acm_c) for s in self._live):
+                raise ValueError("solves with different constants in one pool need the batch step")
+            self.pool.set_group_params([])
+            self.pool.rk_step_groups(self._live[0].acm_c, bt, self._dts, path)
+        return self.finish()
 
     def close(self):
         for s in list(self.solves):
@@ -231,37 +263,58 @@ class PoolGroup:
         self.pool.close()
 
 
+def _differs(a, b):
+    return bytes(a) != bytes(b)
+
+
 class Batch:
     """Independent solves on shared pools, one pool per block shape and lane.
 
     With lanes > 1 the solves are spread over that many independent sets of
-    pools, each with its own stream, stepped by one host thread per lane:
-    the per-solve mesh work runs in the library with the interpreter lock
-    released, so the lanes' host orchestration and their kernels overlap."""
+    pools, each with its own stream, whose per-solve mesh work runs on one
+    host thread per lane (in the library, with the interpreter lock
+    released); the RK stages of all pools of a dimension run as one launch
+    per stage (wh_batch_rk_step) between the two mesh phases.
 
-    def __init__(self, slots_per_pool: Dict[tuple, int], n_work_slots: int = 5, lanes: int = 1):
+    prealloc: the pool keys (pool_key) to create at start-up in every lane;
+    then admission never creates a pool, and a request whose shape has no
+    pool is refused."""
+
+    def __init__(self, slots_per_pool: Dict[tuple, int], n_work_slots: int = 5, lanes: int = 1,
+                 prealloc: Optional[List[tuple]] = None, use_graph: bool = True):
         """slots_per_pool: {(dim, Bs): pool slots} -- the device memory set
         aside per shape (the reference's --memory), split evenly over the
         lanes."""
         self.slots_per_pool = dict(slots_per_pool)
         self.n_work_slots = n_work_slots
         self.lanes = max(1, int(lanes))
+        self.use_graph = use_graph
         self.groups: Dict[tuple, PoolGroup] = {}
         self.next_rid = 0
         self._exec = None
+        self._steppers: Dict[int, L.BatchStepper] = {}
+        self._stepper_pools: Dict[int, tuple] = {}
+        self.preallocated = prealloc is not None
         if self.lanes > 1:
             from concurrent.futures import ThreadPoolExecutor
             self._exec = ThreadPoolExecutor(self.lanes)
+        for k in prealloc or []:
+            for lane in range(self.lanes):
+                self._create(k, lane)
+
+    def _create(self, pkey, lane):
+        n = self.slots_per_pool[(pkey[0], pkey[1])] // self.lanes
+        self.groups[(lane,) + tuple(pkey)] = PoolGroup(pkey, n, self.n_work_slots)
 
     def key(self, spec: SolveSpec, lane: int = 0):
-        return (lane, spec.acm.dim, spec.cfg.bs, spec.cfg.wavelet, spec.mask is not None, _acm_key(spec.acm))
+        return (lane,) + pool_key(spec)
 
     def group_for(self, spec: SolveSpec, lane: int = 0) -> PoolGroup:
         k = self.key(spec, lane)
         if k not in self.groups:
-            n = self.slots_per_pool[(spec.acm.dim, spec.cfg.bs)] // self.lanes
-            self.groups[k] = PoolGroup(spec.acm, spec.cfg.bs, spec.cfg.wavelet, n, spec.mask is not None,
-                                       self.n_work_slots)
+            if self.preallocated:
+                raise ValueError(f"no pool for {pool_key(spec)}: the batch was preallocated without it")
+            self._create(pool_key(spec), lane)
         return self.groups[k]
 
     def admit(self, spec: SolveSpec, rid=None) -> Optional[Solve]:
@@ -284,15 +337,55 @@ class Batch:
     def solves(self):
         return [s for g in self.groups.values() for s in g.solves]
 
-    def step(self, path=PATH_FUSED):
-        """One iteration of every admitted solve; returns block-RK-stages."""
-        if self._exec is None:
-            return sum(g.step(path) for g in self.groups.values())
+    def _by_lane(self):
         by_lane: Dict[int, List[PoolGroup]] = {}
         for k, g in self.groups.items():
             by_lane.setdefault(k[0], []).append(g)
-        futs = [self._exec.submit(lambda gs: sum(g.step(path) for g in gs), gs) for gs in by_lane.values()]
-        return sum(f.result() for f in futs)
+        return list(by_lane.values())
+
+    def _run_lanes(self, fn):
+        lanes = self._by_lane()
+        if self._exec is None:
+            return [fn(gs) for gs in lanes]
+        futs = [self._exec.submit(fn, gs) for gs in lanes]
+        return [f.result() for f in futs]
+
+    def _stepper(self, dim, groups):
+        pools = tuple(g.pool for g in groups)
+        ids = tuple(id(p) for p in pools)
+        if self._stepper_pools.get(dim) != ids:
+            if dim in self._steppers:
+                self._steppers[dim].close()
+            self._steppers[dim] = L.BatchStepper(pools)
+            self._stepper_pools[dim] = ids
+        return self._steppers[dim]
+
+    def step(self, path=PATH_FUSED):
+        """One iteration of every admitted solve; returns block-RK-stages."""
+        if path != PATH_FUSED:
+            return sum(self._run_lanes(lambda gs: sum(g.step_alone(path) for g in gs)))
+        self._run_lanes(lambda gs: [g.prepare() for g in gs])
+        bt = butcher_rk4()
+        for dim in (2, 3):
+            groups = [g for g in self.groups.values() if g.dim == dim]
+            if not any(g._live for g in groups):
+                continue
+            live = [g for g in groups if g._live]
+            st = self._stepper(dim, groups)
+            dts = np.concatenate([g._dts for g in groups]) if groups else np.zeros(0)
+            st.rk_step(live[0]._live[0].acm_c, bt, dts, use_graph=self.use_graph)
+        return sum(self._run_lanes(lambda gs: sum(g.finish() for g in gs)))
+
+    def stage_kernel_roofline(self, reps=5):
+        """{dim: (ms per launch, blocks per launch)} of the batched fused
+        stage kernel on the last step's union grid."""
+        out = {}
+        bt = butcher_rk4()
+        for dim, st in self._steppers.items():
+            acm = next((s.acm_c for g in self.groups.values() if g.dim == dim for s in g.solves), None)
+            if acm is not None:
+                out[dim] = st.time_stage_kernel(acm, bt, reps)
+        return out
 
     def retire_done(self, keep_state=True):
         """Remove finished (or failed) solves; returns [(rid, tag, state or
@@ -306,6 +399,10 @@ class Batch:
         return out
 
     def close(self):
+        for st in self._steppers.values():
+            st.close()
+        self._steppers = {}
+        self._stepper_pools = {}
         for g in self.groups.values():
             g.close()
         self.groups = {}

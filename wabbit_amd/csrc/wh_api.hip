@@ -344,6 +344,7 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
         if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
             return fail("hipDeviceGetAttribute", e);
         P->n_cu = ncu > 0 ? ncu : 256;
+        P->device = dev;
     }
     if ((e = hipMalloc(&P->block, sizeof(double) * P->blk * d.nb_max)) != hipSuccess)
         return fail("hipMalloc(hvy_block)", e);
@@ -396,6 +397,7 @@ int wh_pool_destroy(wh_pool *P)
     if (P->d_grp) (void)hipFree(P->d_grp);
     if (P->d_dtb) (void)hipFree(P->d_dtb);
     if (P->d_dtg) (void)hipFree(P->d_dtg);
+    if (P->d_gacm) (void)hipFree(P->d_gacm);
     if (P->h_scal) (void)hipHostFree(P->h_scal);
     if (P->h_time) (void)hipHostFree(P->h_time);
     if (P->ev_time) (void)hipEventDestroy(P->ev_time);
@@ -506,6 +508,8 @@ int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
     if (int rc = upload_staged(P, P->d_x0dx, x0dx, sizeof(double) * 6 * n)) return rc;
     P->nact = n;
     P->ngrp = 0;  // block groups refer to the previous active list
+    P->ngacm = 0;
+    P->h_gparams.clear();
     P->topo_version++;
     return WH_OK;
 }
@@ -719,6 +723,8 @@ int wh_set_block_groups(wh_pool *P, const int32_t *group, int n, int ngroups)
     P->h_grp.assign(group, group + n);
     if (int rc = upload_staged(P, P->d_grp, group, sizeof(int32_t) * n)) return rc;
     P->ngrp = ngroups;
+    P->ngacm = 0;  // new groups: their constants are set again (wh_set_group_params)
+    P->h_gparams.clear();
     return WH_OK;
 }
 
@@ -745,6 +751,7 @@ int wh_rk_step_groups(wh_pool *P, const wh_acm_params *p, const double *bt, int 
     WH_CHECK(P && p && bt && dt_groups, WH_E_ARG, "null argument");
     WH_CHECK(nrows >= 2 && nrows <= 9, WH_E_ARG, "butcher tableau size");
     WH_CHECK(P->ngrp > 0, WH_E_STATE, "no block groups set");
+    WH_CHECK(P->ngacm == 0, WH_E_STATE, "per-group constants are stepped by wh_batch_rk_step");
     for (int i = 0; i < P->ngrp; ++i)
         WH_CHECK(dt_groups[i] >= 0.0, WH_E_ARG, "negative dt");
     if (int rc = check_acm(P, p)) return rc;

@@ -49,6 +49,29 @@ struct AcmDev {
     int dim, scheme, skew, penal, sponge;
 };
 
+// Block geometry of a pool as the kernels see it.
+struct Geo {
+    int bs0, bs1, bs2, g, nx, ny;
+    size_t blk, cs, mblk;
+    int mask_per_block;
+};
+
+// One pool of a multi-pool launch: its arrays (0 hvy_block, 1..3 work slots
+// 0..2), its mask (or null) and geometry.
+struct MPoolDev {
+    double *arr[4];
+    const double *mask;
+    Geo G;
+    int neq, dim;
+};
+
+// The RK stage of a multi-pool fused stage launch: input / output array
+// selectors into MPoolDev::arr, the epilogue mode and tableau coefficients.
+struct MStage {
+    int in_sel, out_sel, sout_sel, mode, use_a, use_b;
+    double a, b;
+};
+
 // Alternate source arrays of explicit patch lists (bits 8-9 of wh_patch::op).
 struct SyncSrc {
     const double *alt[2] = {nullptr, nullptr};
@@ -151,6 +174,7 @@ struct wh_pool {
     // 2, 4, 8 workgroups per CU
     int k2d_stage = 1;
     int n_cu = 256;  // compute units of the pool's device
+    int device = 0;  // the device the pool lives on
     // wavelet transforms / adapt_tree
     wh::WaveletWS ws;
     // block groups (independent forests in one pool): group of each active
@@ -160,6 +184,10 @@ struct wh_pool {
     int32_t *d_grp = nullptr;
     double *d_dtb = nullptr;
     unsigned long long *d_dtg = nullptr;
+    // per-group physics constants (wh_set_group_params); ngacm = 0: none
+    int ngacm = 0, gacm_cap = 0;
+    std::vector<wh_acm_params> h_gparams;
+    wh::AcmDev *d_gacm = nullptr;
 };
 
 namespace wh {
@@ -231,6 +259,14 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
 int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<double> &stats);
 
 AcmDev make_acm_dev(const wh_acm_params *p);
+Geo make_geo(const wh_pool *P);
+// multi-pool launches (independent forests of a batch, wh_batch.hip)
+int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
+                       const int2 *d_ent, const double *d_x0dx, const double *d_dt, const AcmDev *d_gacm,
+                       const int32_t *d_egrp, const int *d_nent, const MStage &ms, size_t lds, int grid,
+                       int nchunk);
+int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int sel, const wh_patch *d_pt,
+                      const int32_t *d_ppool, const int64_t *d_pstart, const int *d_rng, int grid);
 int reach_of(int scheme);
 bool zm_valid(int bs0, int bs1, int scheme, int ys);
 bool zm_zvalid(int bs2, int scheme, int zs);

@@ -157,12 +157,6 @@ struct StridedAcc {
     }
 };
 
-struct Geo {
-    int bs0, bs1, bs2, g, nx, ny;
-    size_t blk, cs, mblk;
-    int mask_per_block;
-};
-
 // Runge-Kutta epilogue of the fused stage kernel.
 //   mode 0 (first stage): u0 = input state; acc = u0 + (dt*b)*k; tout = u0 + (dt*a)*k
 //   mode 1 (middle):      acc = acc + (dt*b)*k;  tout = u0 + (dt*a)*k
@@ -234,21 +228,19 @@ __device__ __forceinline__ void stage_epilogue_pre(const StageArgs &sa, int k, s
 // With PRE each thread loads the mask and, for the fused stages, the acc /
 // u0 values of its first point into registers before the tile barrier, so
 // that these reads overlap the tile load instead of following it.
+// rhs2d_block is the work of one workgroup on block b (active position k):
+// shared by the one-pool kernel and the multi-pool kernel.  mk: the mask of
+// block b or null.
 template <int S, bool SKEW, bool FUSED, bool PRE>
-__global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, double *__restrict__ out,
-                                               const double *__restrict__ mask,
-                                               const int32_t *__restrict__ act,
-                                               const double *__restrict__ x0dx, Geo G, AcmDev a,
-                                               StageArgs sa)
+__device__ __forceinline__ void rhs2d_block(double *tile, const double *__restrict__ in, double *__restrict__ out,
+                                            const double *__restrict__ mk, int b, int k,
+                                            const double *__restrict__ x0dx, const Geo &G, const AcmDev &a,
+                                            const StageArgs &sa)
 {
     constexpr int R = Reach<S>::R;
-    extern __shared__ double tile[];
-    const int k = blockIdx.x;
-    const int b = act[k];
     const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R, tcs = tx * ty;
     const int npts = G.bs0 * G.bs1;
     const double *u = in + (size_t)b * G.blk;
-    const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
     const size_t base = (size_t)b * G.blk;
     const int p0 = threadIdx.x;
     double mpre[6] = {0, 0, 0, 0, 0, 0}, apre[3] = {0, 0, 0}, upre[3] = {0, 0, 0};
@@ -300,6 +292,60 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
 #pragma unroll
             for (int c = 0; c < 3; ++c) out[o + c * G.cs] = r[c];
         }
+    }
+}
+
+template <int S, bool SKEW, bool FUSED, bool PRE>
+__global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, double *__restrict__ out,
+                                               const double *__restrict__ mask,
+                                               const int32_t *__restrict__ act,
+                                               const double *__restrict__ x0dx, Geo G, AcmDev a,
+                                               StageArgs sa)
+{
+    extern __shared__ double tile[];
+    const int k = blockIdx.x;
+    const int b = act[k];
+    const double *mk = mask ? mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
+    rhs2d_block<S, SKEW, FUSED, PRE>(tile, in, out, mk, b, k, x0dx, G, a, sa);
+}
+
+// ---------------------------------------------------------------------------
+// Several pools in one launch (independent forests of a batch, any block
+// size): entry k of the launch is block ent[k].y of pool ent[k].x, with its
+// own origin/spacing (x0dx), dt and group constants (gacm[egrp[k]]).  The
+// stage's input and output arrays are selected per pool from arr[]:
+// 0 hvy_block, 1 work slot 0 (the RK accumulator), 2 / 3 work slots 1 / 2.
+// The kernels are persistent (grid-stride over the entries, their count read
+// from device memory), so one captured launch serves any number of blocks.
+// ---------------------------------------------------------------------------
+template <int S, bool SKEW>
+__global__ __launch_bounds__(256) void k_rhs2d_multi(const MPoolDev *__restrict__ pools, const int2 *__restrict__ ent,
+                                                     const double *__restrict__ x0dx, const double *__restrict__ dt,
+                                                     const AcmDev *__restrict__ gacm,
+                                                     const int32_t *__restrict__ egrp, const int *__restrict__ nent_p,
+                                                     MStage st)
+{
+    extern __shared__ double tile[];
+    const int nent = *nent_p;
+    for (int k = blockIdx.x; k < nent; k += gridDim.x) {
+        const int2 e = ent[k];
+        const MPoolDev &P = pools[e.x];
+        const Geo G = P.G;
+        const AcmDev a = gacm[egrp[k]];
+        StageArgs sa{};
+        sa.u0 = P.arr[0];
+        sa.acc = P.arr[1];
+        sa.tout = P.arr[st.out_sel];
+        sa.sout = P.arr[st.sout_sel];
+        sa.dtb = dt;
+        sa.a = st.a;
+        sa.b = st.b;
+        sa.mode = st.mode;
+        sa.use_a = st.use_a;
+        sa.use_b = st.use_b;
+        const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)e.y * G.mblk : 0) : nullptr;
+        rhs2d_block<S, SKEW, true, true>(tile, P.arr[st.in_sel], nullptr, mk, e.y, k, x0dx, G, a, sa);
+        __syncthreads();  // the tile is reused by the next entry
     }
 }
 
@@ -426,6 +472,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
+    }
+}
+
+// 3D, several pools in one launch (same arithmetic per point as k_rhs3d):
+// work item t = (entry, chunk of 256 points), nchunk chunks per entry (the
+// largest block of the launch; points past a smaller block's end idle);
+// persistent over the items, their count nent * nchunk from device memory.
+// With XCD the item index is swizzled so that a block's chunks run on one
+// XCD (the grid is a multiple of 8, so item t runs on XCD t % 8).
+template <int S, bool SKEW, bool XCD>
+__global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict__ pools, const int2 *__restrict__ ent,
+                                                     const double *__restrict__ x0dx, const double *__restrict__ dt,
+                                                     const AcmDev *__restrict__ gacm,
+                                                     const int32_t *__restrict__ egrp, const int *__restrict__ nent_p,
+                                                     MStage st, int nchunk)
+{
+    const int total = *nent_p * nchunk;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int w = XCD ? xcd_swizzle(t, total) : t;
+        const int k = w / nchunk;
+        const int pnt = (w - k * nchunk) * 256 + threadIdx.x;
+        const int2 e = ent[k];
+        const MPoolDev &P = pools[e.x];
+        const Geo G = P.G;
+        const int npts = G.bs0 * G.bs1 * G.bs2;
+        if (pnt >= npts) continue;
+        const AcmDev a = gacm[egrp[k]];
+        StageArgs sa{};
+        sa.u0 = P.arr[0];
+        sa.acc = P.arr[1];
+        sa.tout = P.arr[st.out_sel];
+        sa.sout = P.arr[st.sout_sel];
+        sa.dtb = dt;
+        sa.a = st.a;
+        sa.b = st.b;
+        sa.mode = st.mode;
+        sa.use_a = st.use_a;
+        sa.use_b = st.use_b;
+        const int b = e.y;
+        const int ix = pnt % G.bs0, iy = (pnt / G.bs0) % G.bs1, iz = pnt / (G.bs0 * G.bs1);
+        const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4], dx2 = x0dx[6 * k + 5];
+        const double inv[3] = {1.0 / dx0, 1.0 / dx1, 1.0 / dx2};
+        const double inv2[3] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1), 1.0 / (dx2 * dx2)};
+        const size_t gi = ((size_t)(G.g + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
+        const double *uc = P.arr[st.in_sel] + (size_t)b * G.blk + gi;
+        const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) + gi : nullptr;
+        double r[4];
+        const StridedAcc<3> A{uc, (ptrdiff_t)G.cs, {1, G.nx, (ptrdiff_t)G.nx * G.ny}};
+        acm_point<3, S, SKEW>(A, inv, inv2, a, mk, G.cs, r);
+        stage_epilogue<4>(sa, k, (size_t)b * G.blk + gi, G.cs, r, uc, G.cs);
     }
 }
 
@@ -585,7 +681,7 @@ static int zm_split(const wh_pool *P, const Geo &G, int scheme)
     return 0;
 }
 
-static Geo make_geo(const wh_pool *P)
+Geo make_geo(const wh_pool *P)
 {
     Geo G;
     G.bs0 = P->d.bs[0];
@@ -873,6 +969,117 @@ int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const i
     else
         hipLaunchKernelGGL(k_sync_flat<false>, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
                            p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+// Ghost fill over the patches of several pools (one sync stage): patch q
+// belongs to pool ppool[q], fills array `sel` of that pool; one thread per
+// ghost point, persistent over the points [pstart[p0], pstart[p1]) with the
+// range (p0, p1) read from device memory.  Per point the copy / decimation
+// / prediction of k_sync_flat.
+__global__ __launch_bounds__(256) void k_sync_multi(const MPoolDev *__restrict__ pools, int sel,
+                                                    const wh_patch *__restrict__ pt,
+                                                    const int32_t *__restrict__ ppool,
+                                                    const int64_t *__restrict__ pstart,
+                                                    const int *__restrict__ rng)
+{
+    const int p0 = rng[0], p1 = rng[1];
+    if (p1 <= p0) return;
+    auto find = [&](int64_t x, int lo, int hi) {
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pstart[mid] <= x) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    const int64_t end = pstart[p1];
+    for (int64_t w0 = pstart[p0] + (int64_t)blockIdx.x * blockDim.x; w0 < end;
+         w0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t w1 = min(w0 + (int64_t)blockDim.x, end) - 1;
+        const int pa = find(w0, p0, p1 - 1);
+        const int pb = find(w1, pa, p1 - 1);
+        const int64_t e = w0 + threadIdx.x;
+        if (e > w1) continue;
+        const int lo = find(e, pa, pb);
+        const wh_patch P = pt[lo];
+        const MPoolDev &M = pools[ppool[lo]];
+        const Geo &G = M.G;
+        const int r = (int)(e - pstart[lo]);
+        const int n0 = P.n[0], n1 = P.n[1];
+        const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
+        double *arr = M.arr[sel];
+        const size_t ri = ((size_t)(P.recv_lo[2] + kk) * G.ny + (P.recv_lo[1] + j)) * G.nx + (P.recv_lo[0] + i);
+        double *dst = arr + (size_t)P.recv * G.blk + ri;
+        const double *sbase = arr + (size_t)P.send * G.blk;
+        const int op = P.op & 0xff;
+        if (op == WH_PATCH_PREDICT) {
+            const int f[3] = {P.send_lo[0] + i, P.send_lo[1] + j, P.send_lo[2] + kk};
+            for (int c = 0; c < M.neq; ++c)
+                dst[c * G.cs] = pred_any(sbase + c * G.cs, G.nx, G.ny, M.dim, f, P.aux);
+            continue;
+        }
+        const int stp = (op == WH_PATCH_RESTRICT) ? 2 : 1;
+        const size_t si = ((size_t)(P.send_lo[2] + stp * kk) * G.ny + (P.send_lo[1] + stp * j)) * G.nx +
+                          (P.send_lo[0] + stp * i);
+        for (int c = 0; c < M.neq; ++c) dst[c * G.cs] = sbase[si + c * G.cs];
+    }
+}
+
+int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int sel, const wh_patch *d_pt,
+                      const int32_t *d_ppool, const int64_t *d_pstart, const int *d_rng, int grid)
+{
+    hipLaunchKernelGGL(k_sync_multi, dim3(grid), dim3(256), 0, stream, d_pools, sel, d_pt, d_ppool, d_pstart, d_rng);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
+                       const int2 *d_ent, const double *d_x0dx, const double *d_dt, const AcmDev *d_gacm,
+                       const int32_t *d_egrp, const int *d_nent, const MStage &ms, size_t lds, int grid,
+                       int nchunk)
+{
+    if (dim == 2) {
+        WH_CHECK(lds <= 160 * 1024, WH_E_ARG, "2D block tile exceeds LDS");
+#define M2D(SC, SK)                                                                                         \
+    hipLaunchKernelGGL((k_rhs2d_multi<SC, SK>), dim3(grid), dim3(256), lds, stream, d_pools, d_ent, d_x0dx, d_dt, \
+                       d_gacm, d_egrp, d_nent, ms)
+        switch (scheme * 2 + skew) {
+        case WH_FD2 * 2: M2D(WH_FD2, false); break;
+        case WH_FD2 * 2 + 1: M2D(WH_FD2, true); break;
+        case WH_FD4 * 2: M2D(WH_FD4, false); break;
+        case WH_FD4 * 2 + 1: M2D(WH_FD4, true); break;
+        case WH_FD4_TW * 2: M2D(WH_FD4_TW, false); break;
+        case WH_FD4_TW * 2 + 1: M2D(WH_FD4_TW, true); break;
+        case WH_FD6 * 2: M2D(WH_FD6, false); break;
+        case WH_FD6 * 2 + 1: M2D(WH_FD6, true); break;
+        default: set_error("unknown scheme"); return WH_E_ARG;
+        }
+#undef M2D
+    } else {
+#define M3D(SC, SK)                                                                                           \
+    do {                                                                                                      \
+        if (xcd)                                                                                              \
+            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, true>), dim3(grid), dim3(256), 0, stream, d_pools, d_ent,   \
+                               d_x0dx, d_dt, d_gacm, d_egrp, d_nent, ms, nchunk);                               \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, false>), dim3(grid), dim3(256), 0, stream, d_pools, d_ent,  \
+                               d_x0dx, d_dt, d_gacm, d_egrp, d_nent, ms, nchunk);                               \
+    } while (0)
+        switch (scheme * 2 + skew) {
+        case WH_FD2 * 2: M3D(WH_FD2, false); break;
+        case WH_FD2 * 2 + 1: M3D(WH_FD2, true); break;
+        case WH_FD4 * 2: M3D(WH_FD4, false); break;
+        case WH_FD4 * 2 + 1: M3D(WH_FD4, true); break;
+        case WH_FD4_TW * 2: M3D(WH_FD4_TW, false); break;
+        case WH_FD4_TW * 2 + 1: M3D(WH_FD4_TW, true); break;
+        case WH_FD6 * 2: M3D(WH_FD6, false); break;
+        case WH_FD6 * 2 + 1: M3D(WH_FD6, true); break;
+        default: set_error("unknown scheme"); return WH_E_ARG;
+        }
+#undef M3D
+    }
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -1267,10 +1474,12 @@ __device__ double block_max(double v)
 
 __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ in,
                                             const int32_t *__restrict__ act,
-                                            const double *__restrict__ x0dx, Geo G, AcmDev a,
-                                            unsigned long long *dtmin, const int32_t *__restrict__ grp)
+                                            const double *__restrict__ x0dx, Geo G, AcmDev a0,
+                                            unsigned long long *dtmin, const int32_t *__restrict__ grp,
+                                            const AcmDev *__restrict__ gacm)
 {
     const int k = blockIdx.x, b = act[k];
+    const AcmDev a = gacm ? gacm[grp[k]] : a0;  // per-group constants of independent forests
     const int npts = G.bs0 * G.bs1 * G.bs2;
     const int zoff = (G.bs2 > 1) ? G.g : 0;
     const double *u = in + (size_t)b * G.blk;
@@ -1328,7 +1537,7 @@ int launch_dt(wh_pool *P, const AcmDev &a)
         const Geo G = make_geo(P);
         hipLaunchKernelGGL(k_dt, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
                            P->d_x0dx, G, a, (unsigned long long *)(P->scal + DS_DTMIN_BITS),
-                           (const int32_t *)nullptr);
+                           (const int32_t *)nullptr, (const AcmDev *)nullptr);
     }
     WH_HIP(hipGetLastError());
     return WH_OK;
@@ -1351,7 +1560,8 @@ int launch_dt_groups(wh_pool *P, const AcmDev &a)
     if (P->nact > 0) {
         const Geo G = make_geo(P);
         hipLaunchKernelGGL(k_dt, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
-                           P->d_x0dx, G, a, P->d_dtg, (const int32_t *)P->d_grp);
+                           P->d_x0dx, G, a, P->d_dtg, (const int32_t *)P->d_grp,
+                           (const AcmDev *)(P->ngacm == P->ngrp ? P->d_gacm : nullptr));
     }
     WH_HIP(hipGetLastError());
     return WH_OK;

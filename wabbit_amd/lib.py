@@ -115,6 +115,13 @@ def load(path: str = LIB_PATH):
         "wh_set_block_groups": [_vp, _vp, ctypes.c_int, ctypes.c_int],
         "wh_dt_reduce_groups": [_vp, ctypes.POINTER(AcmParamsC), _vp],
         "wh_rk_step_groups": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int, _vp, ctypes.c_int],
+        "wh_set_group_params": [_vp, _vp, ctypes.c_int],
+        "wh_batch_create": [pp, _vp, ctypes.c_int],
+        "wh_batch_destroy": [_vp],
+        "wh_batch_rk_step": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int, _vp, ctypes.c_int],
+        "wh_batch_time_stage_kernel": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
+        "wh_batch_synchronize": [_vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -264,6 +271,13 @@ class Pool:
         check(load().wh_rk_step_groups(self.h, ctypes.byref(acm_c), bt.ctypes.data, bt.shape[0],
                                        d.ctypes.data, path))
 
+    def set_group_params(self, params):
+        """Physics constants per block group (a list of AcmParamsC, one per
+        group); an empty list clears them."""
+        arr = (AcmParamsC * max(1, len(params)))(*params) if params else None
+        check(load().wh_set_group_params(self.h, ctypes.cast(arr, ctypes.c_void_p) if arr is not None else None,
+                                         len(params)))
+
     def rk_step(self, acm_c, butcher, time, dt, path=0):
         bt = np.ascontiguousarray(butcher, np.float64)
         check(load().wh_rk_step(self.h, ctypes.byref(acm_c), bt.ctypes.data, bt.shape[0], time, dt, path))
@@ -319,3 +333,44 @@ class Pool:
         ms, n = ctypes.c_double(), ctypes.c_int()
         check(load().wh_kernel_timing(self.h, which, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+class BatchStepper:
+    """RungeKuttaGeneric over the block groups of several pools of one
+    dimension in one launch per RK stage (wh_batch_*): each pool's active
+    list, patch table and block groups must be set before each step."""
+
+    def __init__(self, pools):
+        self.pools = list(pools)
+        arr = (ctypes.c_void_p * len(self.pools))(*[p.h.value for p in self.pools])
+        h = ctypes.c_void_p()
+        check(load().wh_batch_create(ctypes.byref(h), ctypes.cast(arr, ctypes.c_void_p), len(self.pools)))
+        self.h = h
+
+    def rk_step(self, acm_c, butcher, dt_groups, use_graph=True):
+        bt = np.ascontiguousarray(butcher, np.float64)
+        d = np.ascontiguousarray(dt_groups, np.float64)
+        check(load().wh_batch_rk_step(self.h, ctypes.byref(acm_c), bt.ctypes.data, bt.shape[0],
+                                      d.ctypes.data if d.size else None, int(use_graph)))
+
+    def time_stage_kernel(self, acm_c, butcher, reps=5):
+        """(ms per launch, blocks per launch) of the fused stage kernel."""
+        bt = np.ascontiguousarray(butcher, np.float64)
+        ms, nb = ctypes.c_double(), ctypes.c_int()
+        check(load().wh_batch_time_stage_kernel(self.h, ctypes.byref(acm_c), bt.ctypes.data, bt.shape[0], reps,
+                                                ctypes.byref(ms), ctypes.byref(nb)))
+        return ms.value, nb.value
+
+    def synchronize(self):
+        check(load().wh_batch_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().wh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -59,8 +59,10 @@ class Server:
     """Continuous batching over a Batch of shared pools."""
 
     def __init__(self, slots_per_pool: Dict[tuple, int], n_work_slots: int = 3, path: int = PATH_FUSED,
-                 keep_states: bool = False, lanes: int = 1):
-        self.batch = Batch(slots_per_pool, n_work_slots=n_work_slots, lanes=lanes)
+                 keep_states: bool = False, lanes: int = 1, prealloc=None):
+        """prealloc: pool keys (batch.pool_key) created at start-up; then
+        admission never creates a pool."""
+        self.batch = Batch(slots_per_pool, n_work_slots=n_work_slots, lanes=lanes, prealloc=prealloc)
         self.path = path
         self.keep_states = keep_states
 
