@@ -1,28 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the block time step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3u]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c2|c3|c5|c1|c3u]
 
 A "step" is one time step of the workload's main loop.  The metric is
 block-RK-stages/s: the RK4 stages times the blocks each stage swept, summed
 over the K timed steps, divided by the wall time of those steps.  Inputs are
 resident in HBM before the timed region starts.
 
-c2 (default): the adaptive penalised-cylinder channel (2D ACM, Bs=16,
-Jmin=2, Jmax=6, CDF44, eps=1e-3, adapt every step, 500 steps).  A step is
-the reference's main-loop iteration: sync_ghosts_tree, refine everywhere,
-the mask, dt, four fused RHS+RK stages, adapt_tree.  The blocks of a step
-are those of the refined grid the RK stages run on.
+c4 (default): the batched ensemble of 64 independent adaptive penalised
+channel solves (2D ACM, CDF44, eps=1e-3, seeded cylinders, Jmax in
+{4,5,6,7}, Bs in {16,24}), the largest configuration.  A step is one
+main-loop iteration of every member: per member sync_ghosts_tree, refine
+everywhere, the mask and dt; one RK4 step of all members (one fused stage
+launch per RK stage over every block of every pool, replayed from a
+hipGraph); per member adapt_tree.
 
+c2: the adaptive penalised-cylinder channel alone (Bs=16, Jmin=2, Jmax=6).
+c3: the adaptive 3D periodic workload.  c5: the serving stream.
 c1 / c3u: uniform periodic grids (no adaptivity); a step is device-side
 dt + four ghost syncs and fused RHS+RK stages, replayed from a hipGraph.
 
 The workloads are defined in wabbit_amd/workloads.py.
 
 Printed: one JSON line with the contract fields plus "roofline" (the fused
-RHS+RK stage kernel, HIP-event timed on the library's stream) and
-"cpu_baseline" (the oracle's C restatement on the host cores, a bounded
-sample of the same workload).
+RHS+RK stage kernel timed back to back with HIP events on the library's
+stream) and "cpu_baseline" (the oracle's C restatement on the host cores, a
+bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -262,36 +266,68 @@ def run_adaptive(args, w):
     s.close()
 
 
-def _group_roofline(groups, stages_by_group):
-    """Fused stage kernel of the batched path: algorithmic bytes of all
-    timed launches / their summed durations (HIP events per pool stream)."""
-    tot_bytes, tot_ms, launches = 0.0, 0.0, 0
-    for g, st in zip(groups, stages_by_group):
-        ms, n = g.pool.kernel_time(0)
-        if n == 0:
-            continue
-        a = g.acm
-        m = 5 if a.penalization or a.use_sponge else 0
-        tot_bytes += b_min(a.dim + 1, g.pool.bs[0], a.dim, 2, m) * st
-        tot_ms += ms * n
-        launches += n
-    achieved = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
-    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_rhs2d<FD4,fused>",
-                launches=launches, avg_launch_ms=round(tot_ms / max(launches, 1), 5),
-                blocks_per_launch=round(sum(stages_by_group) / max(launches, 1), 1))
+def cpu_baseline_ensemble(specs, states, seconds=15.0, threads=None):
+    """The oracle's tree path (serial C per member) on the host cores: one
+    host thread per member (the calls release the interpreter lock), each
+    running its member's main loop from the member's initial grid until
+    `seconds` have passed."""
+    import threading
+
+    from oracle import pyoracle as po
+    from wabbit_amd.params import butcher_rk4, clamp_dt
+    threads = threads or min(16, os.cpu_count() or 1, len(specs))
+    stages = [0] * threads
+    steps = [0] * threads
+    t_end = [0.0]
+
+    def work(i):
+        spec, (tc, lv, hvy) = specs[i], states[i]
+        cfg = spec.cfg
+        wo = po.wavelet(cfg.wavelet, fd_max_size=2)
+        mp = po.mask_params(**spec.mask)
+        tree = po.Tree(wo, 2, cfg.jmin, cfg.jmax, [cfg.bs] * 2, wo.g, tc, lv, hvy, indicator=1, eps=cfg.eps,
+                       groups=cfg.groups, eps_normalized=cfg.eps_normalized, threshold_mask=cfg.threshold_mask,
+                       domain=spec.acm.domain_size, mask=mp)
+        ap = po.make_params(spec.acm)
+        t = 0.0
+        while time.perf_counter() < t_end[0]:
+            tree.sync()
+            tree.refine_everywhere()
+            nb = tree.size()
+            dt = clamp_dt(tree.dt(ap), t, spec.tp)
+            tree.rk_step(ap, mp, butcher_rk4(), dt, 2)
+            t += dt
+            tree.adapt()
+            stages[i] += 4 * nb
+            steps[i] += 1
+        tree.close()
+
+    t0 = time.perf_counter()
+    t_end[0] = t0 + seconds
+    th = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    el = time.perf_counter() - t0
+    return dict(value=sum(stages) / el, unit="block-RK-stages/s", cores=threads, kind="port",
+                sample=f"members 0-{threads - 1} of the ensemble from their initial grids, one host thread each "
+                       f"running the oracle's serial C main loop for {el:.1f} s ({sum(steps)} member-steps)")
 
 
 def run_ensemble(args):
-    """c4: 64 independent channel solves (seeded cylinders, Jmax in
-    {4,5,6,7}, Bs in {16,24}) stepped together: per pool one launch per RK
-    stage operation over all members' blocks; W untimed + K timed steps."""
+    """c4: the 64-member ensemble of channel solves (seeded cylinders, Jmax
+    in {4,5,6,7}, Bs in {16,24}).  Per step: per member sync_ghosts_tree,
+    refine everywhere, mask (lanes of pools in parallel), then ONE RK4 step
+    of all members (one fused stage launch per RK stage over every block of
+    every pool, replayed from a hipGraph), then per member adapt_tree."""
     from wabbit_amd import workloads
     from wabbit_amd.batch import Batch
+    from wabbit_amd.params import butcher_rk4
     torch_sync = _torch_sync()
     steps = args.steps if args.steps is not None else 200
     warmup = args.warmup
-    specs = workloads.ensemble(args.members, nsteps=warmup + steps + 2)
+    specs = workloads.ensemble(args.members, nsteps=warmup + steps + 4)
     lanes = args.lanes
     tot, big = {}, {}
     for sp in specs:
@@ -302,64 +338,84 @@ def run_ensemble(args):
     slots = {k: lanes * (-(-tot[k] // lanes) + big[k]) for k in tot}
     t0 = time.perf_counter()
     b = Batch(slots, n_work_slots=3, lanes=lanes)
+    solves = []
     for sp in sorted(specs, key=lambda x: -x.slots):      # largest first: packs the lanes
-        if b.admit(sp) is None:
+        s = b.admit(sp)
+        if s is None:
             raise SystemExit(f"ensemble member {sp.tag} does not fit")
-    print(f"# ensemble: {len(specs)} members admitted, {sum(len(s.forest.blocks()[0]) for s in b.solves)} "
-          f"blocks ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        solves.append(s)
+    by_tag = {s.spec.tag: s for s in solves}
+    ncpu = 0 if args.no_cpu else min(16, os.cpu_count() or 1)
+    cpu_states = [by_tag[sp.tag].state() for sp in specs[:ncpu]]
+    print(f"# ensemble: {len(specs)} members admitted in {lanes} lanes, "
+          f"{sum(len(s.forest.blocks()[0]) for s in solves)} blocks ({time.perf_counter() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
     for _ in range(warmup):
         b.step()
+    for g in b.groups.values():
+        g.pool.synchronize()
     torch_sync()
     t0 = time.perf_counter()
     stages = 0
+    nb_min, nb_max = 1 << 40, 0
     for k in range(steps):
         stages += b.step()
-        if (k + 1) % 50 == 0:
-            print(f"# step {k + 1}: {sum(s.rk_blocks for s in b.solves)} blocks", file=sys.stderr, flush=True)
+        nb = sum(s.rk_blocks for s in solves)
+        nb_min, nb_max = min(nb_min, nb), max(nb_max, nb)
+        if (k + 1) % 10 == 0:
+            print(f"# step {k + 1}: {nb} blocks", file=sys.stderr, flush=True)
     for g in b.groups.values():
         g.pool.synchronize()
     torch_sync()
     el = time.perf_counter() - t0
     value = stages / el
-    groups = list(b.groups.values())
-    for g in groups:
-        g.pool.timing(True)
-    st_by = [0] * len(groups)
-    for _ in range(2):
-        b.step()
-        for i, g in enumerate(groups):
-            st_by[i] += g.last_stages
-    roofline = _group_roofline(groups, st_by)
-    for g in groups:
-        g.pool.timing(False)
-    nblocks = sum(s.rk_blocks for s in b.solves)
+
+    # the batched fused stage kernel alone: RK4 stage sequences back to back
+    # on the last step's union grid (every block of every pool in one launch)
+    NREP = 5
+    ms, nblk = b.stage_kernel_roofline(NREP)[2]
+    per_launch = sum(g.last_nact * b_min(3, g.key[1], 2, 2, 5 if g.key[3] else 0) for g in b.groups.values())
+    achieved = per_launch / (ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_rhs2d_multi<FD4,non-skew>", avg_launch_ms=round(ms, 5), launches=4 * NREP,
+                    blocks_per_launch=nblk, bytes_per_launch=int(per_launch),
+                    bytes_per_block_stage={f"Bs{bs}": b_min(3, bs, 2, 2, 5) for bs in (16, 24)},
+                    working_set_bytes=int(per_launch))
+    nblocks = sum(s.rk_blocks for s in solves)
     b.close()
 
-    # the plain path (one launch per block, as the reference loops over
-    # blocks) against the batched fused path on the same 4 members, 2 steps
+    # the plain path (one launch per block and operation, as the reference
+    # loops over blocks) against the batched path on the same 8 members
     plain = {}
-    sub = workloads.ensemble(4, nsteps=4)
-    for path in (1, 2):
-        bb = Batch({(2, 16): 40000, (2, 24): 40000}, n_work_slots=5)
-        for sp in sub:
-            bb.admit(sp)
-        bb.step(path)
-        for g in bb.groups.values():
-            g.pool.synchronize()
-        t1 = time.perf_counter()
-        st = bb.step(path) + bb.step(path)
-        for g in bb.groups.values():
-            g.pool.synchronize()
-        plain["fused" if path == 2 else "per_block"] = round(st / (time.perf_counter() - t1), 1)
-        bb.close()
+    if not args.no_paths:
+        sub = workloads.ensemble(8, nsteps=5)
+        for name, path in (("batched", 2), ("per_block", 1)):
+            bb = Batch({(2, 16): 40000, (2, 24): 40000}, n_work_slots=5)
+            for sp in sub:
+                bb.admit(sp)
+            bb.step(path)
+            for g in bb.groups.values():
+                g.pool.synchronize()
+            t1 = time.perf_counter()
+            st = bb.step(path) + bb.step(path)
+            for g in bb.groups.values():
+                g.pool.synchronize()
+            plain[name] = round(st / (time.perf_counter() - t1), 1)
+            bb.close()
+    cpu = None
+    if ncpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_ensemble(specs[:ncpu], cpu_states, seconds=15.0, threads=ncpu)
     out = {
         "metric": "block-RK-stages/s", "value": round(value, 1), "unit": "block-RK-stages/s", "n_gpus": 1,
         "steps": steps, "warmup": warmup, "ms_per_step": round(el / steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"c4-ensemble-{len(specs)}x-channel-J4-7-Bs16-24", "members": len(specs),
-                   "lanes": lanes, "blocks_at_end": nblocks},
-        "roofline": roofline, "cpu_baseline": None,
-        "paths_block_RK_stages_per_s_4_members": plain,
+                   "lanes": lanes, "blocks_per_step_min": nb_min, "blocks_per_step_max": nb_max,
+                   "blocks_at_end": nblocks, "wavelet": "CDF44", "scheme": "FD_4th_central", "rk": "RK4"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "paths_block_RK_stages_per_s_8_members": plain,
     }
     print(json.dumps(out), flush=True)
 
@@ -399,8 +455,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c1", "c3u"])
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c3", "c5", "c1", "c3u"])
     ap.add_argument("--members", type=int, default=64, help="c4: ensemble members")
     ap.add_argument("--requests", type=int, default=24, help="c5: requests taken from the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (1/s)")

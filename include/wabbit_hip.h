@@ -350,14 +350,15 @@ int wh_run_patches(wh_pool *pool, int array, int alt1, int alt2, const wh_patch 
 
 /* ---- penalisation mask (create_mask_2D_ACM, LIB/EQUATION/ACMnew/create_mask.f90:183-320) */
 typedef struct {
-    int geometry;                    /* 0 none, 1 cylinder (a circle in 2D) */
+    int geometry;                    /* 0 none, 1 cylinder (2D: draw_circle), 2 sphere-fixed (3D: draw_sphere) */
     double x_cntr[3], R_cyl;
     double smoothing_width, smoothing_safety;  /* cosine smoothing (module_ACM.f90:467-471) */
-    int use_sponge, sponge_type;     /* sponge_type 0: "rect", 1: "p-norm" (sponge.f90:4-58) */
+    int use_sponge, sponge_type;     /* sponge_type 0: "rect", 1: "p-norm" (sponge.f90:4-233) */
     double L_sponge, p_sponge;
     double domain[3];
 } wh_mask_params;
-/* CREATE_MASK_meta "all-parts" (module_physics_metamodule.f90:38-84) for the
+/* CREATE_MASK_meta "all-parts" (module_physics_metamodule.f90:38-84; 2D
+ * create_mask_2D_ACM create_mask.f90:183-320, 3D create_mask_3D_acm :6-174) for the
  * listed blocks into WH_MASK (6 components: chi, u_s, colour, sponge); x0dx
  * per listed block as in wh_set_active.  n < 0: all active blocks with the
  * pool's own origins and spacings. */

@@ -75,3 +75,79 @@ def test_cylinder_on_channel():
     k = np.arange(1, 40)
     assert np.allclose(chi[jc, ic + k], chi[jc, ic - k], rtol=0, atol=1e-15)
     assert np.allclose(chi[jc + k, ic], chi[jc - k, ic], rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# 3D: create_mask_3D_acm "sphere-fixed" (draw_sphere, module_geometry.f90:
+# 49-117) and sponge_3D (sponge.f90:61-233).  The plane z = z_c through the
+# sphere's centre must equal the 2D circle of the same centre and radius bit
+# for bit (the z term adds an exact 0), and for the p-norm sponge the plane
+# z = L/2 must equal the 2D p-norm sponge (pinned by the reference's saved
+# masks); the rect sponge equals the 2D one on planes farther than L_sponge
+# from the z walls and is symmetric under z -> L - z.
+# ---------------------------------------------------------------------------
+BS3, G3 = 12, 4
+
+
+def _block3(mp, x0, dx):
+    return po.mask_block(mp, 3, [BS3] * 3, G3, x0, dx)
+
+
+def _block2(mp, x0, dx):
+    return po.mask_block(mp, 2, [BS3] * 2, G3, x0[:2], dx[:2])
+
+
+def test_sphere_centre_plane_is_the_circle():
+    dx = (1.0 / 48, 1.0 / 48, 1.0 / 48)
+    c = (0.5, 0.5, 0.5)
+    R, h = 0.15, 1.5 * dx[0]
+    x0 = (0.375, 0.375, 0.375)            # the centre plane z = 0.5 is interior index G3 + 6
+    kz = G3 + int(round((c[2] - x0[2]) / dx[2]))
+    mp3 = po.mask_params(geometry=2, x_cntr=c, R_cyl=R, smoothing_width=h, smoothing_safety=h,
+                         domain=(1.0, 1.0, 1.0))
+    mp2 = po.mask_params(geometry=1, x_cntr=c, R_cyl=R, smoothing_width=h, smoothing_safety=h,
+                         domain=(1.0, 1.0, 1.0))
+    m3 = _block3(mp3, x0, dx)
+    m2 = _block2(mp2, x0, dx)
+    assert np.array_equal(m3[0, kz], m2[0, 0]) and np.array_equal(m3[4, kz], m2[4, 0])
+    assert np.any((m3[0] > 0) & (m3[0] < 1)) and np.any(m3[0] == 1.0)
+    # radial: chi depends on |x - c| only -- the octant mirror images agree
+    n = BS3 + 2 * G3
+    ar = (np.arange(n) - G3) * dx[0] + x0[0]
+    Z, Y, X = np.meshgrid(ar, ar, ar, indexing="ij")
+    r = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2)
+    inner = (slice(G3, G3 + BS3 + 1),) * 3
+    chi, rr = m3[0][inner], r[inner]
+    assert np.all(chi[rr <= R - h] == 1.0) and np.all(chi[rr >= R + h] == 0.0)
+    ic = G3 + 6
+    for k in range(1, 6):
+        a = m3[0, ic + k, ic, ic]
+        assert a == m3[0, ic - k, ic, ic] == m3[0, ic, ic + k, ic] == m3[0, ic, ic, ic - k]
+    # no sponge: component 5 is 0; u_s is 0; colour 1
+    assert np.all(m3[5] == 0.0) and np.all(m3[1:4] == 0.0) and np.all(m3[4] == 1.0)
+
+
+def test_sponge_3d_planes_are_the_2d_sponges():
+    dx = (1.0 / 64, 1.0 / 64, 1.0 / 64)
+    L = 1.0
+    # p-norm on the plane z = L/2
+    mp = po.mask_params(use_sponge=True, sponge_type=1, L_sponge=0.2, p_sponge=8.0, domain=(L, L, L))
+    for x0 in ((0.0, 0.0, 0.40625), (0.8125, 0.625, 0.40625)):
+        kz = G3 + int(round((0.5 - x0[2]) / dx[2]))
+        m3 = _block3(mp, x0, dx)
+        m2 = _block2(mp, x0, dx)
+        assert np.array_equal(m3[5, kz], m2[5, 0])
+        assert np.any(m3[5] > 0.0)
+    # rect: planes farther than L_sponge from both z walls equal the 2D
+    # sponge of a domain with the same x, y lengths
+    mp = po.mask_params(use_sponge=True, sponge_type=0, L_sponge=0.125, domain=(L, L, L))
+    x0 = (0.0, 0.8125, 0.375)
+    m3 = _block3(mp, x0, dx)
+    m2 = _block2(mp, x0, dx)
+    for k in range(G3, G3 + BS3):
+        assert np.array_equal(m3[5, k], m2[5, 0])
+    # symmetric under z -> L - z: the blocks at z0 and L - z0 - Bs dx mirror
+    a = _block3(mp, (0.0, 0.0, 0.0), dx)[5, G3:G3 + BS3]
+    b = _block3(mp, (0.0, 0.0, L - BS3 * dx[2]), dx)[5, G3:G3 + BS3]
+    assert np.array_equal(a[1:], b[::-1][:-1])
+    assert np.all(a[0][G3:G3 + BS3, G3:G3 + BS3] == 1.0)          # on the wall z = 0

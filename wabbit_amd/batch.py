@@ -214,11 +214,13 @@ class PoolGroup:
             pats.append(pat)
             live.append(s)
         self._live = live
+        self.last_nact = 0
         if not live:
             pool.set_active(np.zeros(0, np.int32), np.zeros((0, 6)))
             pool.set_patches(0, np.zeros(0, L.PATCH_DTYPE))
             self._dts = np.zeros(0)
             return
+        self.last_nact = int(sum(len(h) for h in hvs))
         pool.set_active(np.concatenate(hvs), np.concatenate(xs))
         pool.set_patches(0, np.concatenate(pats))
         pool.set_block_groups(np.concatenate(grp), len(live))

@@ -1,6 +1,7 @@
 // wh_mask.hip -- the ACM penalisation mask of listed blocks on the device
 // (CREATE_MASK_meta "all-parts", LIB/EQUATION/module_physics_metamodule.f90:
-// 38-84, with create_mask_2D_ACM, LIB/EQUATION/ACMnew/create_mask.f90:183-320):
+// 38-84, with create_mask_2D_ACM, LIB/EQUATION/ACMnew/create_mask.f90:183-320,
+// and in 3D create_mask_3D_acm, create_mask.f90:6-174, see k_mask3d):
 //   component 0: chi of the cylinder (draw_circle, LIB/EQUATION/insects/
 //                module_geometry.f90:315-381), written over the interior and
 //                the first upper ghost point within the circle's bounding box
@@ -20,7 +21,7 @@ namespace {
 
 struct MaskDev {
     int geometry, use_sponge, sponge_type;
-    double cx, cy, R, h, safety, Lh, p, pinv, off, Lx, Ly, pi;
+    double cx, cy, cz, R, h, safety, Lh, p, pinv, off, Lx, Ly, Lz, pi;
 };
 
 __device__ __forceinline__ double step_cos(double x, double t, double h, double pi)
@@ -83,19 +84,81 @@ __global__ __launch_bounds__(256) void k_mask(double *__restrict__ mask, size_t 
     m[5 * cs] = sp;
 }
 
+// 3D (create_mask_3D_acm, create_mask.f90:6-174): "sphere-fixed" by
+// draw_sphere (module_geometry.f90:49-117: interior plus first upper ghost
+// point within the sphere's box widened by the smoothing safety, dist =
+// |x - c| - R as norm2_3d sums it), sponge_3D (sponge.f90:61-233) "rect"
+// (min of the six wall distances) or "p-norm" on the interior.
+__global__ __launch_bounds__(256) void k_mask3d(double *__restrict__ mask, size_t mblk, size_t cs,
+                                                const int32_t *__restrict__ ids,
+                                                const double *__restrict__ x0dx, int nb, int nx, int ny,
+                                                int nz, int g, int bs0, int bs1, int bs2, MaskDev M)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nb * (long)cs) return;
+    const int k = (int)(e / (long)cs);
+    const int q = (int)(e - (long)k * cs);
+    const int ix = q % nx, iy = (q / nx) % ny, iz = q / (nx * ny);
+    const double x0 = x0dx[6 * k + 0], y0 = x0dx[6 * k + 1], z0 = x0dx[6 * k + 2];
+    const double dx = x0dx[6 * k + 3], dy = x0dx[6 * k + 4], dz = x0dx[6 * k + 5];
+    double *m = mask + (size_t)ids[k] * mblk + q;
+    double chi = 0.0;
+    if (M.geometry == 2) {
+        const int nsafe = (int)ceil(M.safety / fmin(fmin(dx, dy), dz));
+        const int bxmin = (int)((M.cx - M.R - x0) / dx) - nsafe + (g + 1);
+        const int bymin = (int)((M.cy - M.R - y0) / dy) - nsafe + (g + 1);
+        const int bzmin = (int)((M.cz - M.R - z0) / dz) - nsafe + (g + 1);
+        const int bxmax = (int)ceil((M.cx + M.R - x0) / dx) + nsafe + (g + 1);
+        const int bymax = (int)ceil((M.cy + M.R - y0) / dy) + nsafe + (g + 1);
+        const int bzmax = (int)ceil((M.cz + M.R - z0) / dz) + nsafe + (g + 1);
+        const int i1 = ix + 1, j1 = iy + 1, k1 = iz + 1;
+        if (i1 >= max(bxmin, g + 1) && i1 <= min(bxmax, nx - g + 1) && j1 >= max(bymin, g + 1) &&
+            j1 <= min(bymax, ny - g + 1) && k1 >= max(bzmin, g + 1) && k1 <= min(bzmax, nz - g + 1)) {
+            const double z = (double)(k1 - (g + 1)) * dz + z0;
+            const double y = (double)(j1 - (g + 1)) * dy + y0;
+            const double x = (double)(i1 - (g + 1)) * dx + x0;
+            const double ax = x - M.cx, ay = y - M.cy, az = z - M.cz;
+            const double dist = sqrt(ax * ax + ay * ay + az * az) - M.R;
+            chi = step_cos(dist, 0.0, M.h, M.pi);
+        }
+    }
+    double sp = 0.0;
+    if (M.use_sponge && ix >= g && ix < g + bs0 && iy >= g && iy < g + bs1 && iz >= g && iz < g + bs2) {
+        double tmp;
+        if (M.sponge_type == 0) {
+            const double x = (double)(ix - g) * dx + x0, y = (double)(iy - g) * dy + y0,
+                         z = (double)(iz - g) * dz + z0;
+            tmp = fmin(fmin(fmin(fmin(fmin(x, y), z), -(x - M.Lx)), -(y - M.Ly)), -(z - M.Lz));
+        } else {
+            const double z = pow((double)(iz - g) * dz + z0 - M.off, M.p);
+            const double y = pow((double)(iy - g) * dy + y0 - M.off, M.p);
+            const double x = pow((double)(ix - g) * dx + x0 - M.off, M.p);
+            tmp = -(pow(x + y + z, M.pinv) - M.off);
+        }
+        sp = step_cos(tmp, M.Lh, M.Lh, M.pi);
+    }
+    m[0] = chi;
+    m[cs] = 0.0;
+    m[2 * cs] = 0.0;
+    m[3 * cs] = 0.0;
+    m[4 * cs] = 1.0;
+    m[5 * cs] = sp;
+}
+
 // chi statistics over the interior of listed blocks: [between, max, min]
 __global__ __launch_bounds__(256) void k_mask_stats(const double *__restrict__ mask, size_t mblk, size_t cs,
-                                                    const int32_t *__restrict__ ids, int nx, int g, int bs0,
-                                                    int bs1, double *__restrict__ out)
+                                                    const int32_t *__restrict__ ids, int nx, int ny, int g,
+                                                    int bs0, int bs1, int bs2, double *__restrict__ out)
 {
     __shared__ double smax[256], smin[256];
     __shared__ int sbetw[256];
     const double *m = mask + (size_t)ids[blockIdx.x] * mblk;
     double mx = -1.0e300, mn = 1.0e300;
     int betw = 0;
-    for (int e = threadIdx.x; e < bs0 * bs1; e += blockDim.x) {
-        const int ix = g + e % bs0, iy = g + e / bs0;
-        const double v = m[(size_t)iy * nx + ix];
+    for (int e = threadIdx.x; e < bs0 * bs1 * bs2; e += blockDim.x) {
+        const int ix = g + e % bs0, iy = g + (e / bs0) % bs1;
+        const int iz = bs2 > 1 ? g + e / (bs0 * bs1) : 0;
+        const double v = m[((size_t)iz * ny + iy) * nx + ix];
         if (v > 1.0e-12 && v < 1.0 - 1.0e-12) betw = 1;
         mx = fmax(mx, v);
         mn = fmin(mn, v);
@@ -127,6 +190,7 @@ MaskDev make_mask_dev(const wh_mask_params &mp)
     M.sponge_type = mp.sponge_type;
     M.cx = mp.x_cntr[0];
     M.cy = mp.x_cntr[1];
+    M.cz = mp.x_cntr[2];
     M.R = mp.R_cyl;
     M.h = mp.smoothing_width;
     M.safety = mp.smoothing_safety;
@@ -136,6 +200,7 @@ MaskDev make_mask_dev(const wh_mask_params &mp)
     M.off = 0.5 * mp.domain[0];
     M.Lx = mp.domain[0];
     M.Ly = mp.domain[1];
+    M.Lz = mp.domain[2];
     M.pi = 4.0 * std::atan(1.0);
     return M;
 }
@@ -144,7 +209,6 @@ MaskDev make_mask_dev(const wh_mask_params &mp)
 
 int check_mask_pool(wh_pool *P)
 {
-    WH_CHECK(P->d.dim == 2, WH_E_ARG, "mask: only the 2D ACM mask is implemented");
     WH_CHECK(P->mask && P->d.n_mask >= 6 && P->d.mask_per_block, WH_E_ARG,
              "mask: the pool needs a per-block mask with 6 components");
     return WH_OK;
@@ -154,10 +218,14 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
              const std::vector<double> &x0dx)
 {
     if (int rc = check_mask_pool(P)) return rc;
-    WH_CHECK(mp.geometry == 0 || mp.geometry == 1, WH_E_ARG, "mask: unknown geometry");
+    const int dim = P->d.dim;
+    WH_CHECK(mp.geometry == 0 || (dim == 2 && mp.geometry == 1) || (dim == 3 && mp.geometry == 2), WH_E_ARG,
+             "mask: unknown geometry (2D: 1 cylinder; 3D: 2 sphere-fixed)");
     WH_CHECK(mp.sponge_type == 0 || mp.sponge_type == 1, WH_E_ARG, "mask: unknown sponge type");
-    WH_CHECK(!(mp.use_sponge && mp.sponge_type == 1) || std::fabs(mp.domain[1] - mp.domain[0]) <= 1.0e-10,
-             WH_E_ARG, "mask: the p-norm sponge needs a square domain");
+    WH_CHECK(!(mp.use_sponge && mp.sponge_type == 1) ||
+                 (std::fabs(mp.domain[1] - mp.domain[0]) <= 1.0e-10 &&
+                  (dim == 2 || std::fabs(mp.domain[2] - mp.domain[0]) <= 1.0e-10)),
+             WH_E_ARG, "mask: the p-norm sponge needs a square (cubic) domain");
     WH_CHECK(mp.geometry == 0 || (mp.smoothing_width > 0.0), WH_E_ARG, "mask: smoothing width must be > 0");
     const int n = (int)ids.size();
     if (n == 0) return WH_OK;
@@ -169,9 +237,14 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
     if (int rc = stage_flush(P)) return rc;
     if (!d_ids || !d_x0dx) return WH_E_HIP;
     const long tot = (long)n * (long)P->cs;
-    hipLaunchKernelGGL(k_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask, P->mblk,
-                       P->cs, d_ids, d_x0dx, n, P->nx, P->ny, P->d.g, P->d.bs[0], P->d.bs[1],
-                       make_mask_dev(mp));
+    if (dim == 3)
+        hipLaunchKernelGGL(k_mask3d, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask,
+                           P->mblk, P->cs, d_ids, d_x0dx, n, P->nx, P->ny, P->nz, P->d.g, P->d.bs[0], P->d.bs[1],
+                           P->d.bs[2], make_mask_dev(mp));
+    else
+        hipLaunchKernelGGL(k_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask,
+                           P->mblk, P->cs, d_ids, d_x0dx, n, P->nx, P->ny, P->d.g, P->d.bs[0], P->d.bs[1],
+                           make_mask_dev(mp));
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
@@ -194,7 +267,7 @@ int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<doub
     const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n);
     if (!d_ids) return WH_E_HIP;
     hipLaunchKernelGGL(k_mask_stats, dim3(n), dim3(256), 0, P->stream, P->mask, P->mblk, P->cs, d_ids, P->nx,
-                       P->d.g, P->d.bs[0], P->d.bs[1], W.d_out);
+                       P->ny, P->d.g, P->d.bs[0], P->d.bs[1], P->d.dim == 3 ? P->d.bs[2] : 1, W.d_out);
     WH_HIP(hipGetLastError());
     WH_HIP(hipMemcpyAsync(stats.data(), W.d_out, sizeof(double) * n * 3, hipMemcpyDeviceToHost, P->stream));
     return stage_sync(P);

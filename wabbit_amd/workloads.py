@@ -10,7 +10,8 @@ c2   2D ACM, adaptive, penalised cylinder in a channel: domain [0,4]x[0,2]
      adapt every step.  Cylinder of seeded radius in
      [0.1,0.2] and seeded centre, C_eta=1e-3, mask smoothed over 1.5 dx_min.
      Inflow u=(1,0) plus a seeded Fourier perturbation (|k|<=8, |k|^-2,
-     u_rms=0.05), p=0; nu=1e-3, c0=20, CFL 1.0, RK4, 500 steps.
+     u_rms=0.05), p=0; nu=1e-3, c0=20, CFL 0.5 (the reference's default,
+     ini_file_to_params.f90:624), RK4, 500 steps.
 c3u  3D ACM, periodic unit cube, uniform level 4 (4096 blocks), Bs=24,
      CDF44 ghosts (g=6, g_rhs=2), FD_4th_central, c0=15, nu=1e-3; seeded
      solenoidal velocity (1<=|k|<=6, E(k)~k^-5/3, u_rms=1), p=0.
@@ -68,7 +69,7 @@ def channel(seed: int = 1234, jmax: int = 6, bs: int = 16) -> dict:
                 use_sponge=True, sponge_type=0, L_sponge=0.25, p_sponge=20.0, domain=domain)
     acm = ACMParams(dim=2, scheme=4, skew_symmetry=False, penalization=True, use_sponge=True,
                     c_0=20.0, nu=1e-3, gamma_p=1.0, C_eta=1e-3, C_sponge=1e-2,
-                    u_mean_set=[1.0, 0.0, 0.0], CFL=1.0, CFL_eta=0.99,
+                    u_mean_set=[1.0, 0.0, 0.0], CFL=0.5, CFL_eta=0.99,
                     domain_size=list(domain))
     return dict(name=f"c2-2d-channel-cylinder-J{jmin}-{jmax}-Bs{bs}", kind="adaptive", dim=2, bs=bs,
                 jmin=jmin, jmax=jmax, wavelet="CDF44", eps=1e-3, domain=domain, acm=acm,
