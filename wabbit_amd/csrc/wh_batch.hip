@@ -304,7 +304,11 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     }
     // device buffers (grown, never shrunk; growing re-captures the graph)
     uint64_t &ver = B->buf_version;
-    const bool need = (size_t)nent > B->cap_ent || gacm.size() > B->cap_gacm || pt.size() + 1 > B->cap_pstart;
+    // every buffer against its own capacity: they grow by different factors
+    const size_t npt = pt.size() + 1;
+    const bool need = (size_t)nent > B->cap_ent || (size_t)nent > B->cap_egrp || (size_t)nent > B->cap_dt ||
+                      (size_t)nent * 6 > B->cap_x0dx || gacm.size() > B->cap_gacm || npt > B->cap_pt ||
+                      npt > B->cap_ppool || npt > B->cap_pstart;
     if (need) {
         WH_HIP(hipStreamSynchronize(B->stream));  // queued work may still read the old buffers
         int rc = grow(B->d_ent, B->cap_ent, nent, ver);
