@@ -417,6 +417,22 @@ int wh_fwt_threshold(wh_pool *pool, int src_array, int dst_array, const int32_t 
                      const double *thresh, const int32_t *groups, int32_t *flags_out,
                      double *detail_out);
 
+/* wh_fwt_threshold without allocating anything.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and checks as wh_fwt_threshold; the
+ * block ids and the per-block details pass through a fixed device array of
+ * the library (one per GPU, held until the process ends) instead of the
+ * pool's staging and result buffers, so n is limited to
+ * WH_NOALLOC_MAX_BLOCKS and neq to 8.  The array's address on a GPU is set
+ * up when the first pool on that GPU is created.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several host threads)
+ * race.  Returns with the pool's stream synchronised. */
+#define WH_NOALLOC_MAX_BLOCKS 16384
+int wh_fwt_threshold_noalloc(wh_pool *pool, int src_array, int dst_array, const int32_t *hvy_ids,
+                             int n, const double *thresh, const int32_t *groups,
+                             int32_t *flags_out, double *detail_out);
+
 /* sync_ghosts_tree (synchronize_ghosts_generic.f90:121-144) on a leaf forest
  * with all ghosts, restriction filtered with HD for lifted wavelets
  * (restrict_copy_at_CE, restrict_predict_data.f90:101-170). */

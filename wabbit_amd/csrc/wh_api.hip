@@ -345,6 +345,7 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
             return fail("hipDeviceGetAttribute", e);
         P->n_cu = ncu > 0 ? ncu : 256;
         P->device = dev;
+        wh::noalloc_setup(dev);
     }
     if ((e = hipMalloc(&P->block, sizeof(double) * P->blk * d.nb_max)) != hipSuccess)
         return fail("hipMalloc(hvy_block)", e);

@@ -246,6 +246,9 @@ int stage_sync(wh_pool *P);
 int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<wh_patch> &patches);
 int run_blocks_op(wh_pool *P, int op, int src_array, int dst_array, const std::vector<int32_t> &ids);
 int run_boxes(wh_pool *P, int array, int src_array, const std::vector<wh_box> &boxes);
+// resolve the device address of the no-allocation entry point's scratch
+// array on device dev (the current device); once per device
+void noalloc_setup(int dev);
 int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
                std::vector<double> &detail);
 enum { WOP_FWT = 0, WOP_IWT = 1, WOP_FILTER = 2, WOP_COPY = 3 };

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "wh_internal.h"
 #include "wh_prof.h"
@@ -1039,6 +1040,49 @@ int run_boxes(wh_pool *P, int array, int src_array, const std::vector<wh_box> &b
     return WH_OK;
 }
 
+// k_detail over n listed blocks whose ids are at d_ids on the device, into
+// d_out[k*neq + c] (every entry written: nothing needs clearing).
+int launch_detail(wh_pool *P, const double *arr, size_t blk, const int32_t *d_ids, int n, const DetailArgs &D,
+                  double *d_out)
+{
+    const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, D.neq};
+    hipLaunchKernelGGL(k_detail, dim3((unsigned)(n * D.neq)), dim3(256), 0, P->stream, arr, d_ids, A, D, d_out);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+DetailArgs detail_args(const wh_pool *P, const int32_t *groups)
+{
+    DetailArgs D{};
+    D.g = P->d.g;
+    D.dim = P->d.dim;
+    D.neq = P->d.neq;
+    for (int d = 0; d < 3; ++d) D.bs[d] = d < P->d.dim ? P->d.bs[d] : 1;
+    for (int c = 0; c < 8; ++c) D.groups[c] = (groups && c < D.neq) ? groups[c] : 1;
+    return D;
+}
+
+// groups: the max over the group's components (threshold_block.f90:86-91),
+// group 0 gives 0; in place on the host
+void apply_detail_groups(double *detail, int n, const DetailArgs &D)
+{
+    const int neq = D.neq;
+    int maxg = 0;
+    for (int c = 0; c < neq; ++c) maxg = std::max(maxg, D.groups[c]);
+    for (int k = 0; k < n; ++k) {
+        double *dk = detail + (size_t)k * neq;
+        for (int l = 2; l <= maxg; ++l) {
+            double m = -1.0;
+            for (int c = 0; c < neq; ++c)
+                if (D.groups[c] == l) m = std::max(m, dk[c]);
+            for (int c = 0; c < neq; ++c)
+                if (D.groups[c] == l) dk[c] = m;
+        }
+        for (int c = 0; c < neq; ++c)
+            if (D.groups[c] == 0) dk[c] = 0.0;
+    }
+}
+
 int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
                std::vector<double> &detail)
 {
@@ -1055,34 +1099,12 @@ int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int
     if (int rc = upload_ids(P, ids.data(), n)) return rc;
     WaveletWS &W = P->ws;
     if (int rc = ensure(W.d_out, W.out_cap, (size_t)n * neq)) return rc;
-    DetailArgs D{};
-    D.g = P->d.g;
-    D.dim = P->d.dim;
-    D.neq = neq;
-    for (int d = 0; d < 3; ++d) D.bs[d] = d < P->d.dim ? P->d.bs[d] : 1;
-    for (int c = 0; c < 8; ++c) D.groups[c] = (groups && c < neq) ? groups[c] : 1;
-    const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
-    hipLaunchKernelGGL(k_detail, dim3((unsigned)(n * neq)), dim3(256), 0, P->stream, arr, W.cur_ids, A, D,
-                       W.d_out);
-    WH_HIP(hipGetLastError());
+    const DetailArgs D = detail_args(P, groups);
+    if (int rc = launch_detail(P, arr, blk, W.cur_ids, n, D, W.d_out)) return rc;
     WH_HIP(hipMemcpyAsync(detail.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost,
                           P->stream));
     if (int rc = stage_sync(P)) return rc;
-    // groups: the max over the group's components (threshold_block.f90:86-91)
-    int maxg = 0;
-    for (int c = 0; c < neq; ++c) maxg = std::max(maxg, D.groups[c]);
-    for (int k = 0; k < n; ++k) {
-        double *dk = &detail[(size_t)k * neq];
-        for (int l = 2; l <= maxg; ++l) {
-            double m = -1.0;
-            for (int c = 0; c < neq; ++c)
-                if (D.groups[c] == l) m = std::max(m, dk[c]);
-            for (int c = 0; c < neq; ++c)
-                if (D.groups[c] == l) dk[c] = m;
-        }
-        for (int c = 0; c < neq; ++c)
-            if (D.groups[c] == 0) dk[c] = 0.0;
-    }
+    apply_detail_groups(detail.data(), n, D);
     return WH_OK;
 }
 
@@ -1199,6 +1221,58 @@ int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Device scratch of wh_fwt_threshold_noalloc: the listed block ids, then the
+// per-block details (k_detail writes every entry it later copies, so nothing
+// needs clearing between calls).  One copy per GPU, held until the process
+// ends.  In host code `scratch` only names the symbol; its device address on
+// each GPU comes from hipGetSymbolAddress (noalloc_setup) and is the only
+// form host code passes on.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t NOALLOC_IDS_BYTES = ((size_t)WH_NOALLOC_MAX_BLOCKS * sizeof(int32_t) + 255) / 256 * 256;
+constexpr size_t NOALLOC_DETAIL_BYTES = (size_t)WH_NOALLOC_MAX_BLOCKS * 8 * sizeof(double);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_IDS_BYTES + NOALLOC_DETAIL_BYTES];
+
+namespace wh {
+namespace {
+std::mutex noalloc_mutex;
+bool noalloc_done[NOALLOC_MAX_DEVICES];
+hipError_t noalloc_err[NOALLOC_MAX_DEVICES];
+unsigned char *noalloc_ptr[NOALLOC_MAX_DEVICES];
+}  // namespace
+
+void noalloc_setup(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (noalloc_done[dev]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    noalloc_err[dev] = e;
+    noalloc_ptr[dev] = e == hipSuccess ? static_cast<unsigned char *>(p) : nullptr;
+    noalloc_done[dev] = true;
+}
+
+namespace {
+int noalloc_scratch(int dev, unsigned char **out)
+{
+    WH_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, WH_E_ARG, "no-alloc scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    WH_CHECK(noalloc_done[dev], WH_E_STATE, "no-alloc scratch: not set up on this device (create a pool on it)");
+    if (noalloc_err[dev] != hipSuccess) {
+        set_error(std::string("no-alloc scratch: hipGetSymbolAddress: ") + hipGetErrorString(noalloc_err[dev]));
+        return WH_E_HIP;
+    }
+    *out = noalloc_ptr[dev];
+    return WH_OK;
+}
+}  // namespace
+}  // namespace wh
+
 extern "C" {
 
 int wh_pool_set_wavelet(wh_pool *pool, const wh_wavelet *w)
@@ -1279,6 +1353,56 @@ int wh_run_patches(wh_pool *pool, int array, int alt1, int alt2, const wh_patch 
 {
     WH_CHECK(pool && n >= 0 && (n == 0 || patches), WH_E_ARG, "wh_run_patches: bad argument");
     return wh::run_patch_list(pool, array, alt1, alt2, std::vector<wh_patch>(patches, patches + n));
+}
+
+int wh_fwt_threshold_noalloc(wh_pool *pool, int src_array, int dst_array, const int32_t *ids, int n,
+                             const double *thresh, const int32_t *groups, int32_t *flags_out, double *detail_out)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || (ids && thresh && flags_out && detail_out)), WH_E_ARG,
+             "wh_fwt_threshold_noalloc: bad argument");
+    WH_CHECK(n <= WH_NOALLOC_MAX_BLOCKS, WH_E_ARG,
+             "wh_fwt_threshold_noalloc: more blocks than WH_NOALLOC_MAX_BLOCKS (16384) in one call");
+    wh_pool *P = pool;
+    const int neq = P->d.neq;
+    WH_CHECK(neq <= 8, WH_E_ARG, "wh_fwt_threshold_noalloc: at most 8 components");
+    unsigned char *base = nullptr;
+    if (int rc = wh::noalloc_scratch(P->device, &base)) return rc;
+    if (n == 0) return WH_OK;
+    // the checks of run_blocks_op (FWT) and run_detail
+    size_t blk_in = 0, blk_out = 0;
+    int nc_in = 0, nc_out = 0;
+    const double *in = wh::array_ptr(P, src_array, &blk_in, &nc_in);
+    double *out = wh::array_ptr(P, dst_array, &blk_out, &nc_out);
+    WH_CHECK(in && out, WH_E_ARG, "wh_fwt_threshold_noalloc: unknown array");
+    WH_CHECK(src_array != dst_array, WH_E_ARG, "wh_fwt_threshold_noalloc: source and destination must differ");
+    WH_CHECK(nc_in >= neq && nc_out >= neq, WH_E_ARG, "wh_fwt_threshold_noalloc: arrays need neq components");
+    WH_CHECK(P->ws.set, WH_E_STATE, "wh_fwt_threshold_noalloc: no wavelet set on the pool");
+    WH_CHECK(P->d.n_work_slots >= 1 && src_array != WH_WORK && dst_array != WH_WORK, WH_E_ARG,
+             "wh_fwt_threshold_noalloc: work slot 0 is scratch and needed");
+    const wh_wavelet &w = P->ws.w;
+    WH_CHECK(P->d.g >= w.g, WH_E_ARG, "wh_fwt_threshold_noalloc: too few ghost nodes for the wavelet");
+    if (int rc = wh::check_ids(P, ids, n)) return rc;
+    if (int rc = wh::stage_flush(P)) return rc;
+    int32_t *d_ids = reinterpret_cast<int32_t *>(base);
+    double *d_det = reinterpret_cast<double *>(base + NOALLOC_IDS_BYTES);
+    WH_HIP(hipMemcpyAsync(d_ids, ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, P->stream));
+    const int32_t *saved = P->ws.cur_ids;
+    P->ws.cur_ids = d_ids;
+    const wh::DetailArgs D = wh::detail_args(P, groups);
+    int rc = wh::separable(P, in, blk_in, out, blk_out, n, wh::fwt_terms(w), std::max(w.hd_hi, w.gd_hi));
+    if (!rc) rc = wh::launch_detail(P, out, blk_out, d_ids, n, D, d_det);
+    P->ws.cur_ids = saved;
+    if (rc) return rc;
+    WH_HIP(hipMemcpyAsync(detail_out, d_det, sizeof(double) * n * neq, hipMemcpyDeviceToHost, P->stream));
+    WH_HIP(hipStreamSynchronize(P->stream));
+    wh::apply_detail_groups(detail_out, n, D);
+    for (int k = 0; k < n; ++k) {
+        bool all = true;  // threshold_block.f90:119-124
+        for (int c = 0; c < neq; ++c)
+            if (!(detail_out[(size_t)k * neq + c] <= thresh[c])) all = false;
+        flags_out[k] = all ? -1 : 0;
+    }
+    return WH_OK;
 }
 
 }  // extern "C"

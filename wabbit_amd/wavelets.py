@@ -112,6 +112,7 @@ def _bind():
         "wh_adapt_tree": [_vp, _vp, ctypes.POINTER(AdaptParamsC)],
         "wh_adapt_tree_flags": [_vp, _vp, ctypes.POINTER(AdaptParamsC), _vp, ip],
         "wh_fwt_threshold": [_vp, ip, ip, _vp, ip, _vp, _vp, _vp, _vp],
+        "wh_fwt_threshold_noalloc": [_vp, ip, ip, _vp, ip, _vp, _vp, _vp, _vp],
         "wh_sync_tree": [_vp, _vp, ip],
         "wh_create_mask": [_vp, ctypes.POINTER(MaskParamsC), _vp, ip, _vp],
     }
@@ -236,6 +237,21 @@ def fwt_threshold(pool, src, dst, ids, thresh, groups):
     det = np.zeros((a.size, pool.neq))
     L.check(L.load().wh_fwt_threshold(pool.h, src, dst, a.ctypes.data if a.size else None, a.size, th.ctypes.data,
                                       g.ctypes.data, flags.ctypes.data, det.ctypes.data))
+    return flags, det
+
+
+def fwt_threshold_noalloc(pool, src, dst, ids, thresh, groups):
+    """fwt_threshold through the entry point that allocates nothing (ids and
+    details pass through the library's fixed device array; at most
+    WH_NOALLOC_MAX_BLOCKS blocks per call)."""
+    _bind()
+    a = _ids(ids)
+    th = np.ascontiguousarray(thresh, np.float64)
+    g = np.ascontiguousarray(groups, np.int32)
+    flags = np.zeros(a.size, np.int32)
+    det = np.zeros((a.size, pool.neq))
+    L.check(L.load().wh_fwt_threshold_noalloc(pool.h, src, dst, a.ctypes.data if a.size else None, a.size,
+                                              th.ctypes.data, g.ctypes.data, flags.ctypes.data, det.ctypes.data))
     return flags, det
 
 
