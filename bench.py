@@ -355,6 +355,7 @@ def run_ensemble(args):
     for g in b.groups.values():
         g.pool.synchronize()
     torch_sync()
+    ph0 = dict(b.phase_s)
     t0 = time.perf_counter()
     stages = 0
     nb_min, nb_max = 1 << 40, 0
@@ -369,6 +370,7 @@ def run_ensemble(args):
     torch_sync()
     el = time.perf_counter() - t0
     value = stages / el
+    phases = {k: round((b.phase_s.get(k, 0.0) - ph0.get(k, 0.0)) / steps * 1e3, 3) for k in b.phase_s}
 
     # the batched fused stage kernel alone: RK4 stage sequences back to back
     # on the last step's union grid (every block of every pool in one launch)
@@ -416,6 +418,7 @@ def run_ensemble(args):
                    "blocks_at_end": nblocks, "wavelet": "CDF44", "scheme": "FD_4th_central", "rk": "RK4"},
         "roofline": roofline, "cpu_baseline": cpu,
         "paths_block_RK_stages_per_s_8_members": plain,
+        "phase_ms_per_step": phases,
     }
     print(json.dumps(out), flush=True)
 

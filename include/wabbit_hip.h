@@ -216,22 +216,25 @@ int wh_set_group_params(wh_pool *pool, const wh_acm_params *params, int ngroups)
  * dimension (any block sizes, e.g. one pool per Bs class and host thread)
  * together: per RK stage one ghost fill per sync stage over the patch tables
  * of all pools and ONE fused stage kernel over the active blocks of all
- * pools (persistent kernels over device-resident tables of entries and
- * patches; the fused path's stage sequence, wh_rk_step path 2).  Each pool
- * needs its active list, patch table (case 0) and block groups set, three
- * work slots, and a mask where a group is penalised.  Per block the
- * arithmetic is that of wh_rk_step_groups, so each forest's result does not
- * depend on the batch.  The batch has its own stream, ordered after the work
- * queued on every pool's stream and before their later work. */
+ * pools (persistent kernels that read every pool's own device tables --
+ * active list, origins/spacings, block groups, patch table -- through one
+ * small descriptor per pool; the fused path's stage sequence, wh_rk_step
+ * path 2).  Each pool needs its active list, patch table (case 0) and block
+ * groups set, three work slots, and a mask where a group is penalised.  Per
+ * block the arithmetic is that of wh_rk_step_groups, so each forest's result
+ * does not depend on the batch.  The batch has its own stream, ordered after
+ * the work queued on every pool's stream and before their later work; a
+ * pool's next table change waits until the step has read its tables. */
 typedef struct wh_batch wh_batch;
 int wh_batch_create(wh_batch **out, wh_pool *const *pools, int npools);
 int wh_batch_destroy(wh_batch *batch);
 /* One step of every group of every pool: dt_groups[] holds one dt per group,
  * the groups of pool 0 first, then those of pool 1, ...  The constants come
  * from a pool's wh_set_group_params where set, else from *p.  use_graph = 1
- * replays the step from a hipGraph captured once (re-captured only when a
- * table buffer grows or the tableau, scheme or pool list changes); the
- * tables are uploaded before each replay (no memset nodes). */
+ * replays the step from a hipGraph captured once (re-captured only when the
+ * per-group arrays grow or the tableau, scheme or block sizes change); the
+ * pool descriptors, per-group dt and constants are uploaded before each
+ * replay (no memset nodes). */
 int wh_batch_rk_step(wh_batch *batch, const wh_acm_params *p, const double *butcher, int nrows,
                      const double *dt_groups, int use_graph);
 /* Average duration of one launch of the batch's fused stage kernel over

@@ -24,6 +24,7 @@ in the batch.
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
 
@@ -296,6 +297,7 @@ class Batch:
         self._exec = None
         self._steppers: Dict[int, L.BatchStepper] = {}
         self._stepper_pools: Dict[int, tuple] = {}
+        self.phase_s: Dict[str, float] = {}
         self.preallocated = prealloc is not None
         if self.lanes > 1:
             from concurrent.futures import ThreadPoolExecutor
@@ -363,10 +365,15 @@ class Batch:
         return self._steppers[dim]
 
     def step(self, path=PATH_FUSED):
-        """One iteration of every admitted solve; returns block-RK-stages."""
+        """One iteration of every admitted solve; returns block-RK-stages.
+        Wall time of the three phases accumulates in phase_s (prepare: the
+        mesh work before the RK step, rk: the batched RK launches, finish:
+        adapt_tree)."""
         if path != PATH_FUSED:
             return sum(self._run_lanes(lambda gs: sum(g.step_alone(path) for g in gs)))
+        t0 = time.perf_counter()
         self._run_lanes(lambda gs: [g.prepare() for g in gs])
+        t1 = time.perf_counter()
         bt = butcher_rk4()
         for dim in (2, 3):
             groups = [g for g in self.groups.values() if g.dim == dim]
@@ -376,7 +383,12 @@ class Batch:
             st = self._stepper(dim, groups)
             dts = np.concatenate([g._dts for g in groups]) if groups else np.zeros(0)
             st.rk_step(live[0]._live[0].acm_c, bt, dts, use_graph=self.use_graph)
-        return sum(self._run_lanes(lambda gs: sum(g.finish() for g in gs)))
+        t2 = time.perf_counter()
+        stages = sum(self._run_lanes(lambda gs: sum(g.finish() for g in gs)))
+        t3 = time.perf_counter()
+        for k, v in (("prepare", t1 - t0), ("rk", t2 - t1), ("finish", t3 - t2)):
+            self.phase_s[k] = self.phase_s.get(k, 0.0) + v
+        return stages
 
     def stage_kernel_roofline(self, reps=5):
         """{dim: (ms per launch, blocks per launch)} of the batched fused

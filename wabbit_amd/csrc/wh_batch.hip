@@ -4,15 +4,16 @@
 //
 // The reference runs one forest per process (LIB/MAIN/main.f90) and loops
 // over its blocks (runge_kutta_generic.f90:1-156).  Here every pool keeps its
-// own active list, ghost-patch table and block groups (set as for
-// wh_rk_step_groups); the batch gathers them into device-resident tables --
-// one entry per active block (pool, hvy id, origin/spacing, dt, group
-// constants) and the concatenated patch lists with their pool -- and runs
-// each RK stage as one ghost fill per sync stage plus one fused stage kernel
-// over all entries.  The kernels are persistent and read their counts from
-// device memory, so the captured launch sequence is replayed unchanged while
-// the tables change from step to step (re-captured only when a table buffer
-// must grow or the tableau / scheme changes).  No memset is captured.
+// own active list, origin/spacing, block groups and RHS ghost-patch table on
+// the device (set as for wh_rk_step_groups); the batch passes the kernels
+// one small descriptor per pool (where those tables are, and where the pool's
+// entries and ghost slots start in the launch) plus dt and constants per
+// block group, and runs each RK stage as one ghost fill per sync stage plus
+// one fused stage kernel over the blocks of all pools.  The kernels are
+// persistent and read their counts and the descriptors from device memory,
+// so the captured launch sequence is replayed unchanged while the grids
+// change from step to step (re-captured only when the group arrays must grow
+// or the tableau / scheme / block sizes change).  No memset is captured.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -33,18 +34,13 @@ struct wh_batch {
     bool up_recorded = false;
     std::vector<hipEvent_t> ev_pool;
     int n_cu = 256;
-    // device tables (capacities in elements)
+    // device tables: one descriptor per pool, dt and constants per block
+    // group (capacity cap_grp), counts ([0] entries, [s] slots of sync stage s)
     MPoolDev *d_pools = nullptr;
-    int2 *d_ent = nullptr;
-    double *d_x0dx = nullptr, *d_dt = nullptr;
-    int32_t *d_egrp = nullptr;
+    double *d_dtg = nullptr;
     AcmDev *d_gacm = nullptr;
-    wh_patch *d_pt = nullptr;
-    int32_t *d_ppool = nullptr;
-    int64_t *d_pstart = nullptr;
-    int *d_cnt = nullptr;  // [0] entries, [1 + 2(s-1)], [2 + 2(s-1)]: patch range of sync stage s
-    size_t cap_ent = 0, cap_x0dx = 0, cap_dt = 0, cap_egrp = 0, cap_gacm = 0, cap_pt = 0, cap_ppool = 0,
-           cap_pstart = 0;
+    int64_t *d_tot = nullptr;
+    size_t cap_grp = 0;
     // pinned host copies of the tables
     char *h_buf = nullptr;
     size_t h_cap = 0;
@@ -88,10 +84,10 @@ void free_graph(wh_batch *B)
 int launch_sequence(wh_batch *B, const wh_acm_params &p, const double *bt, int nrows)
 {
     const int s = nrows - 1;
+    const int np = (int)B->pools.size();
     auto sync = [&](int sel) {
         for (int st = 1; st <= 3; ++st)
-            if (int rc = wh::launch_sync_multi(B->stream, B->d_pools, sel, B->d_pt, B->d_ppool, B->d_pstart,
-                                               B->d_cnt + 1 + 2 * (st - 1), B->grid_sync))
+            if (int rc = wh::launch_sync_multi(B->stream, B->d_pools, np, sel, st, B->d_tot, B->grid_sync))
                 return rc;
         return (int)WH_OK;
     };
@@ -105,9 +101,8 @@ int launch_sequence(wh_batch *B, const wh_acm_params &p, const double *bt, int n
         ms.use_a = std::abs(a) >= 1.0e-8;
         ms.use_b = std::abs(b) >= 1.0e-8;
         ms.sout_sel = mode == 2 ? B->sout_sel : 0;
-        return wh::launch_stage_multi(B->stream, B->dim, p.scheme, p.skew_symmetry ? 1 : 0, 1, B->d_pools,
-                                      B->d_ent, B->d_x0dx, B->d_dt, B->d_gacm, B->d_egrp, B->d_cnt, ms, B->lds,
-                                      B->grid_stage, B->nchunk);
+        return wh::launch_stage_multi(B->stream, B->dim, p.scheme, p.skew_symmetry ? 1 : 0, 1, B->d_pools, np,
+                                      B->d_dtg, B->d_gacm, B->d_tot, ms, B->lds, B->grid_stage, B->nchunk);
     };
     // arrays: 0 state, 1 accumulator (work 0), 2 / 3 ping-pong stage inputs
     if (int rc = sync(0)) return rc;
@@ -197,7 +192,7 @@ int wh_batch_create(wh_batch **out, wh_pool *const *pools, int npools)
     B->ev_pool.resize(npools, nullptr);
     for (auto &ev : B->ev_pool)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&B->d_cnt, sizeof(int) * 8)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&B->d_tot, sizeof(int64_t) * 4)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&B->d_pools, sizeof(MPoolDev) * npools)) != hipSuccess) return fail(e);
     *out = B;
     return WH_OK;
@@ -208,8 +203,7 @@ int wh_batch_destroy(wh_batch *B)
     if (!B) return WH_OK;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     free_graph(B);
-    for (void *p : {(void *)B->d_pools, (void *)B->d_ent, (void *)B->d_x0dx, (void *)B->d_dt, (void *)B->d_egrp,
-                    (void *)B->d_gacm, (void *)B->d_pt, (void *)B->d_ppool, (void *)B->d_pstart, (void *)B->d_cnt})
+    for (void *p : {(void *)B->d_pools, (void *)B->d_dtg, (void *)B->d_gacm, (void *)B->d_tot})
         if (p) (void)hipFree(p);
     if (B->h_buf) (void)hipHostFree(B->h_buf);
     for (hipEvent_t ev : B->ev_pool)
@@ -233,16 +227,16 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
                      "wh_batch_rk_step: the fused stages need a tableau with sub-diagonal stage rows");
     WH_CHECK(p->dim == B->dim, WH_E_ARG, "wh_batch_rk_step: params dim does not match the pools");
     const int npools = (int)B->pools.size();
-    // gather: entries, group constants, patches
-    std::vector<int2> ent;
-    std::vector<double> x0dx, dt;
-    std::vector<int32_t> egrp;
+    // per pool: where its device tables are and where its entries, groups
+    // and ghost slots start in the launch; per group: dt and constants
     std::vector<AcmDev> gacm;
+    std::vector<double> dtg;
     std::vector<MPoolDev> mp(npools);
     size_t lds = 0;
     int nchunk = 1;
     const int R = p->scheme == WH_FD2 ? 1 : p->scheme == WH_FD4 ? 2 : 3;
-    int goff = 0;
+    int goff = 0, nent = 0;
+    int64_t tot[4] = {0, 0, 0, 0};
     for (int i = 0; i < npools; ++i) {
         wh_pool *P = B->pools[i];
         WH_CHECK(P->nact == (int)P->h_active.size() && (P->nact == 0 || P->ngrp > 0), WH_E_STATE,
@@ -262,7 +256,9 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
                 masked = true;
             }
             gacm.push_back(wh::make_acm_dev(&q));
+            dtg.push_back(dt_groups[goff + gi]);
         }
+        const wh::PatchSet &ps = P->ps[0];
         MPoolDev &m = mp[i];
         m.arr[0] = P->block;
         for (int s = 0; s < 3; ++s) m.arr[1 + s] = wh::array_ptr(P, WH_WORK + s, nullptr, nullptr);
@@ -270,65 +266,51 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
         m.G = wh::make_geo(P);
         m.neq = P->d.neq;
         m.dim = P->d.dim;
-        if (B->dim == 2)
-            lds = std::max(lds, sizeof(double) * 3 * (P->d.bs[0] + 2 * R) * (P->d.bs[1] + 2 * R));
-        else
-            nchunk = std::max(nchunk, (P->d.bs[0] * P->d.bs[1] * P->d.bs[2] + 255) / 256);
-        for (int k = 0; k < P->nact; ++k) {
-            ent.push_back(int2{i, P->h_active[k]});
-            x0dx.insert(x0dx.end(), P->h_x0dx.begin() + 6 * k, P->h_x0dx.begin() + 6 * k + 6);
-            const int g = goff + P->h_grp[k];
-            egrp.push_back(g);
-            dt.push_back(dt_groups[g]);
+        m.act = P->d_active;
+        m.x0dx = P->d_x0dx;
+        m.grp = P->d_grp;
+        m.pt = ps.d_patches;
+        m.pstart = ps.d_pstart;
+        m.ent0 = nent;
+        m.goff = goff;
+        for (int st = 1; st <= 3; ++st) {
+            const int a = ps.n ? ps.stage_begin[st] : 0, b = ps.n ? ps.stage_begin[st + 1] : 0;
+            m.p0[st - 1] = a;
+            m.p1[st - 1] = b;
+            m.s0[st - 1] = tot[st];
+            if (P->nact > 0 && b > a) tot[st] += ps.h_pstart[b] - ps.h_pstart[a];
+            else m.p1[st - 1] = a;  // a pool without blocks fills no ghosts
         }
+        if (P->nact > 0) {
+            if (B->dim == 2)
+                lds = std::max(lds, sizeof(double) * 3 * (P->d.bs[0] + 2 * R) * (P->d.bs[1] + 2 * R));
+            else
+                nchunk = std::max(nchunk, (P->d.bs[0] * P->d.bs[1] * P->d.bs[2] + 255) / 256);
+        }
+        nent += P->nact;
         goff += P->ngrp;
     }
-    const int nent = (int)ent.size();
-    if (gacm.empty()) gacm.push_back(wh::make_acm_dev(p));
-    std::vector<wh_patch> pt;
-    std::vector<int32_t> ppool;
-    std::vector<int64_t> pstart(1, 0);
-    int cnt[8] = {nent, 0, 0, 0, 0, 0, 0, 0};
-    for (int st = 1; st <= 3; ++st) {
-        cnt[1 + 2 * (st - 1)] = (int)pt.size();
-        for (int i = 0; i < npools; ++i) {
-            const wh::PatchSet &ps = B->pools[i]->ps[0];
-            for (int q = ps.stage_begin[st]; q < ps.stage_begin[st + 1]; ++q) {
-                const wh_patch &w = ps.host[q];
-                pt.push_back(w);
-                ppool.push_back(i);
-                pstart.push_back(pstart.back() + (int64_t)w.n[0] * w.n[1] * w.n[2]);
-            }
-        }
-        cnt[2 + 2 * (st - 1)] = (int)pt.size();
+    tot[0] = nent;
+    if (gacm.empty()) {
+        gacm.push_back(wh::make_acm_dev(p));
+        dtg.push_back(0.0);
     }
-    // device buffers (grown, never shrunk; growing re-captures the graph)
+    // group arrays (grown, never shrunk; growing re-captures the graph)
     uint64_t &ver = B->buf_version;
-    // every buffer against its own capacity: they grow by different factors
-    const size_t npt = pt.size() + 1;
-    const bool need = (size_t)nent > B->cap_ent || (size_t)nent > B->cap_egrp || (size_t)nent > B->cap_dt ||
-                      (size_t)nent * 6 > B->cap_x0dx || gacm.size() > B->cap_gacm || npt > B->cap_pt ||
-                      npt > B->cap_ppool || npt > B->cap_pstart;
-    if (need) {
+    if (gacm.size() > B->cap_grp) {
         WH_HIP(hipStreamSynchronize(B->stream));  // queued work may still read the old buffers
-        int rc = grow(B->d_ent, B->cap_ent, nent, ver);
-        if (!rc) rc = grow(B->d_egrp, B->cap_egrp, nent, ver);
-        if (!rc) rc = grow(B->d_dt, B->cap_dt, nent, ver);
-        if (!rc) rc = grow(B->d_x0dx, B->cap_x0dx, (size_t)nent * 6, ver);
-        if (!rc) rc = grow(B->d_gacm, B->cap_gacm, gacm.size(), ver);
-        if (!rc) rc = grow(B->d_pt, B->cap_pt, pt.size() + 1, ver);
-        if (!rc) rc = grow(B->d_ppool, B->cap_ppool, pt.size() + 1, ver);
-        if (!rc) rc = grow(B->d_pstart, B->cap_pstart, pt.size() + 1, ver);
+        size_t c = B->cap_grp;
+        int rc = grow(B->d_dtg, c, gacm.size(), ver);
+        c = B->cap_grp;
+        if (!rc) rc = grow(B->d_gacm, c, gacm.size(), ver);
         if (rc) return rc;
+        B->cap_grp = c;
     }
     // pinned copies: the previous step's uploads must have left the buffer
-    const size_t bytes[8] = {sizeof(MPoolDev) * npools, sizeof(int2) * nent, sizeof(double) * 6 * nent,
-                             sizeof(double) * nent, sizeof(int32_t) * nent, sizeof(AcmDev) * gacm.size(),
-                             sizeof(wh_patch) * pt.size(), sizeof(int32_t) * ppool.size()};
-    const size_t pbytes = sizeof(int64_t) * pstart.size();
-    size_t total = sizeof(cnt) + pbytes;
+    const size_t bytes[4] = {sizeof(MPoolDev) * npools, sizeof(double) * dtg.size(), sizeof(AcmDev) * gacm.size(),
+                             sizeof(tot)};
+    size_t total = 0;
     for (size_t b : bytes) total += (b + 255) & ~(size_t)255;
-    total += 512;
     if (B->up_recorded) WH_HIP(hipEventSynchronize(B->ev_up));
     if (total > B->h_cap) {
         if (B->h_buf) WH_HIP(hipHostFree(B->h_buf));
@@ -352,15 +334,9 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     };
     int rc = WH_OK;
     if (!rc) rc = put(B->d_pools, mp.data(), bytes[0]);
-    if (!rc) rc = put(B->d_ent, ent.data(), bytes[1]);
-    if (!rc) rc = put(B->d_x0dx, x0dx.data(), bytes[2]);
-    if (!rc) rc = put(B->d_dt, dt.data(), bytes[3]);
-    if (!rc) rc = put(B->d_egrp, egrp.data(), bytes[4]);
-    if (!rc) rc = put(B->d_gacm, gacm.data(), bytes[5]);
-    if (!rc) rc = put(B->d_pt, pt.data(), bytes[6]);
-    if (!rc) rc = put(B->d_ppool, ppool.data(), bytes[7]);
-    if (!rc) rc = put(B->d_pstart, pstart.data(), pbytes);
-    if (!rc) rc = put(B->d_cnt, cnt, sizeof(cnt));
+    if (!rc) rc = put(B->d_dtg, dtg.data(), bytes[1]);
+    if (!rc) rc = put(B->d_gacm, gacm.data(), bytes[2]);
+    if (!rc) rc = put(B->d_tot, tot, bytes[3]);
     if (rc) return rc;
     WH_HIP(hipEventRecord(B->ev_up, B->stream));
     B->up_recorded = true;
@@ -397,9 +373,14 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     } else {
         if (int lrc = launch_sequence(B, *p, bt, nrows)) return lrc;
     }
-    // the pools' later work (adapt_tree, downloads) follows the step
+    // the pools' later work (adapt_tree, downloads) follows the step, and a
+    // pool replaces its tables only after the step has read them
     WH_HIP(hipEventRecord(B->ev_done, B->stream));
-    for (wh_pool *P : B->pools) WH_HIP(hipStreamWaitEvent(P->stream, B->ev_done, 0));
+    for (wh_pool *P : B->pools) {
+        WH_HIP(hipStreamWaitEvent(P->stream, B->ev_done, 0));
+        WH_HIP(hipEventRecord(P->ev_tables, P->stream));
+        P->tables_recorded = true;
+    }
     return WH_OK;
 }
 
@@ -407,7 +388,7 @@ int wh_batch_time_stage_kernel(wh_batch *B, const wh_acm_params *p, const double
                                double *ms, int *blocks)
 {
     WH_CHECK(B && p && bt && ms && reps >= 1 && reps <= 1000, WH_E_ARG, "wh_batch_time_stage_kernel: bad argument");
-    WH_CHECK(B->d_cnt && B->grid_stage > 0, WH_E_STATE, "wh_batch_time_stage_kernel: no step run yet");
+    WH_CHECK(B->d_tot && B->grid_stage > 0, WH_E_STATE, "wh_batch_time_stage_kernel: no step run yet");
     const int s = nrows - 1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     WH_HIP(hipEventCreate(&e0));
@@ -423,8 +404,8 @@ int wh_batch_time_stage_kernel(wh_batch *B, const wh_acm_params *p, const double
         m.use_a = std::abs(a) >= 1.0e-8;
         m.use_b = std::abs(b) >= 1.0e-8;
         return wh::launch_stage_multi(B->stream, B->dim, p->scheme, p->skew_symmetry ? 1 : 0, 1, B->d_pools,
-                                      B->d_ent, B->d_x0dx, B->d_dt, B->d_gacm, B->d_egrp, B->d_cnt, m, B->lds,
-                                      B->grid_stage, B->nchunk);
+                                      (int)B->pools.size(), B->d_dtg, B->d_gacm, B->d_tot, m, B->lds, B->grid_stage,
+                                      B->nchunk);
     };
     int rc = WH_OK;
     WH_HIP(hipEventRecord(e0, B->stream));

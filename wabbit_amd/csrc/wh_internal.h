@@ -57,12 +57,26 @@ struct Geo {
 };
 
 // One pool of a multi-pool launch: its arrays (0 hvy_block, 1..3 work slots
-// 0..2), its mask (or null) and geometry.
+// 0..2), its mask (or null) and geometry, and where the launch finds the
+// pool's own device tables -- active list, origin/spacing and block group
+// per active position, the RHS ghost-patch table with its slot prefix sum --
+// and the pool's place in the launch: its first entry (entries are the
+// pools' active lists one after the other), its first block group, and per
+// sync stage s its patch range [p0[s], p1[s]) and first slot s0[s] in the
+// stage's slots (the pools' slot ranges one after the other).
 struct MPoolDev {
     double *arr[4];
     const double *mask;
     Geo G;
     int neq, dim;
+    const int32_t *act;
+    const double *x0dx;
+    const int32_t *grp;
+    const wh_patch *pt;
+    const int64_t *pstart;
+    int ent0, goff;
+    int p0[3], p1[3];
+    int64_t s0[3];
 };
 
 // The RK stage of a multi-pool fused stage launch: input / output array
@@ -264,12 +278,13 @@ int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<doub
 AcmDev make_acm_dev(const wh_acm_params *p);
 Geo make_geo(const wh_pool *P);
 // multi-pool launches (independent forests of a batch, wh_batch.hip)
+// d_tot: [0] entries of the launch, [s] slots of sync stage s (1..3); dtg /
+// gacm: dt and constants per block group of the launch
 int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
-                       const int2 *d_ent, const double *d_x0dx, const double *d_dt, const AcmDev *d_gacm,
-                       const int32_t *d_egrp, const int *d_nent, const MStage &ms, size_t lds, int grid,
-                       int nchunk);
-int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int sel, const wh_patch *d_pt,
-                      const int32_t *d_ppool, const int64_t *d_pstart, const int *d_rng, int grid);
+                       int npools, const double *d_dtg, const AcmDev *d_gacm, const int64_t *d_tot,
+                       const MStage &ms, size_t lds, int grid, int nchunk);
+int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, int sel, int stage,
+                      const int64_t *d_tot, int grid);
 int reach_of(int scheme);
 bool zm_valid(int bs0, int bs1, int scheme, int ys);
 bool zm_zvalid(int bs2, int scheme, int zs);

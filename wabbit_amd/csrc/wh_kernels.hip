@@ -311,40 +311,63 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
 
 // ---------------------------------------------------------------------------
 // Several pools in one launch (independent forests of a batch, any block
-// size): entry k of the launch is block ent[k].y of pool ent[k].x, with its
-// own origin/spacing (x0dx), dt and group constants (gacm[egrp[k]]).  The
+// size): entry k of the launch is position k - ent0 of the active list of
+// the pool whose entry range holds k (pools[] in entry order), with that
+// position's origin/spacing and block group g = goff + grp[..] (dt dtg[g],
+// constants gacm[g]), all read from the pool's own device tables.  The
 // stage's input and output arrays are selected per pool from arr[]:
 // 0 hvy_block, 1 work slot 0 (the RK accumulator), 2 / 3 work slots 1 / 2.
 // The kernels are persistent (grid-stride over the entries, their count read
 // from device memory), so one captured launch serves any number of blocks.
 // ---------------------------------------------------------------------------
+// the last pool whose first entry (or first slot of stage s, s >= 0) is <= x
+__device__ __forceinline__ int pool_of(const MPoolDev *__restrict__ pools, int npools, int64_t x, int s)
+{
+    int lo = 0, hi = npools - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t first = s < 0 ? (int64_t)pools[mid].ent0 : pools[mid].s0[s];
+        if (first <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ StageArgs multi_stage_args(const MPoolDev &P, const MStage &st, const double *dtp)
+{
+    StageArgs sa{};
+    sa.u0 = P.arr[0];
+    sa.acc = P.arr[1];
+    sa.tout = P.arr[st.out_sel];
+    sa.sout = P.arr[st.sout_sel];
+    sa.dtb = dtp;  // the entry's dt at index 0
+    sa.a = st.a;
+    sa.b = st.b;
+    sa.mode = st.mode;
+    sa.use_a = st.use_a;
+    sa.use_b = st.use_b;
+    return sa;
+}
+
 template <int S, bool SKEW>
-__global__ __launch_bounds__(256) void k_rhs2d_multi(const MPoolDev *__restrict__ pools, const int2 *__restrict__ ent,
-                                                     const double *__restrict__ x0dx, const double *__restrict__ dt,
+__global__ __launch_bounds__(256) void k_rhs2d_multi(const MPoolDev *__restrict__ pools, int npools,
+                                                     const double *__restrict__ dtg,
                                                      const AcmDev *__restrict__ gacm,
-                                                     const int32_t *__restrict__ egrp, const int *__restrict__ nent_p,
-                                                     MStage st)
+                                                     const int64_t *__restrict__ tot, MStage st)
 {
     extern __shared__ double tile[];
-    const int nent = *nent_p;
+    const int nent = (int)tot[0];
     for (int k = blockIdx.x; k < nent; k += gridDim.x) {
-        const int2 e = ent[k];
-        const MPoolDev &P = pools[e.x];
+        const MPoolDev &P = pools[pool_of(pools, npools, k, -1)];
+        const int j = k - P.ent0;
         const Geo G = P.G;
-        const AcmDev a = gacm[egrp[k]];
-        StageArgs sa{};
-        sa.u0 = P.arr[0];
-        sa.acc = P.arr[1];
-        sa.tout = P.arr[st.out_sel];
-        sa.sout = P.arr[st.sout_sel];
-        sa.dtb = dt;
-        sa.a = st.a;
-        sa.b = st.b;
-        sa.mode = st.mode;
-        sa.use_a = st.use_a;
-        sa.use_b = st.use_b;
-        const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)e.y * G.mblk : 0) : nullptr;
-        rhs2d_block<S, SKEW, true, true>(tile, P.arr[st.in_sel], nullptr, mk, e.y, k, x0dx, G, a, sa);
+        const int b = P.act[j];
+        const int g = P.goff + P.grp[j];
+        const AcmDev a = gacm[g];
+        const StageArgs sa = multi_stage_args(P, st, dtg + g);
+        const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
+        rhs2d_block<S, SKEW, true, true>(tile, P.arr[st.in_sel], nullptr, mk, b, 0, P.x0dx + 6 * (size_t)j, G, a,
+                                         sa);
         __syncthreads();  // the tile is reused by the next entry
     }
 }
@@ -482,37 +505,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // With XCD the item index is swizzled so that a block's chunks run on one
 // XCD (the grid is a multiple of 8, so item t runs on XCD t % 8).
 template <int S, bool SKEW, bool XCD>
-__global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict__ pools, const int2 *__restrict__ ent,
-                                                     const double *__restrict__ x0dx, const double *__restrict__ dt,
+__global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict__ pools, int npools,
+                                                     const double *__restrict__ dtg,
                                                      const AcmDev *__restrict__ gacm,
-                                                     const int32_t *__restrict__ egrp, const int *__restrict__ nent_p,
-                                                     MStage st, int nchunk)
+                                                     const int64_t *__restrict__ tot, MStage st, int nchunk)
 {
-    const int total = *nent_p * nchunk;
+    const int total = (int)tot[0] * nchunk;
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int w = XCD ? xcd_swizzle(t, total) : t;
         const int k = w / nchunk;
         const int pnt = (w - k * nchunk) * 256 + threadIdx.x;
-        const int2 e = ent[k];
-        const MPoolDev &P = pools[e.x];
+        const MPoolDev &P = pools[pool_of(pools, npools, k, -1)];
         const Geo G = P.G;
         const int npts = G.bs0 * G.bs1 * G.bs2;
         if (pnt >= npts) continue;
-        const AcmDev a = gacm[egrp[k]];
-        StageArgs sa{};
-        sa.u0 = P.arr[0];
-        sa.acc = P.arr[1];
-        sa.tout = P.arr[st.out_sel];
-        sa.sout = P.arr[st.sout_sel];
-        sa.dtb = dt;
-        sa.a = st.a;
-        sa.b = st.b;
-        sa.mode = st.mode;
-        sa.use_a = st.use_a;
-        sa.use_b = st.use_b;
-        const int b = e.y;
+        const int j = k - P.ent0;
+        const int b = P.act[j];
+        const int g = P.goff + P.grp[j];
+        const AcmDev a = gacm[g];
+        const StageArgs sa = multi_stage_args(P, st, dtg + g);
+        const double *xd = P.x0dx + 6 * (size_t)j;
         const int ix = pnt % G.bs0, iy = (pnt / G.bs0) % G.bs1, iz = pnt / (G.bs0 * G.bs1);
-        const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4], dx2 = x0dx[6 * k + 5];
+        const double dx0 = xd[3], dx1 = xd[4], dx2 = xd[5];
         const double inv[3] = {1.0 / dx0, 1.0 / dx1, 1.0 / dx2};
         const double inv2[3] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1), 1.0 / (dx2 * dx2)};
         const size_t gi = ((size_t)(G.g + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
@@ -521,7 +535,7 @@ __global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict_
         double r[4];
         const StridedAcc<3> A{uc, (ptrdiff_t)G.cs, {1, G.nx, (ptrdiff_t)G.nx * G.ny}};
         acm_point<3, S, SKEW>(A, inv, inv2, a, mk, G.cs, r);
-        stage_epilogue<4>(sa, k, (size_t)b * G.blk + gi, G.cs, r, uc, G.cs);
+        stage_epilogue<4>(sa, 0, (size_t)b * G.blk + gi, G.cs, r, uc, G.cs);
     }
 }
 
@@ -973,41 +987,44 @@ int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const i
     return WH_OK;
 }
 
-// Ghost fill over the patches of several pools (one sync stage): patch q
-// belongs to pool ppool[q], fills array `sel` of that pool; one thread per
-// ghost point, persistent over the points [pstart[p0], pstart[p1]) with the
-// range (p0, p1) read from device memory.  Per point the copy / decimation
-// / prediction of k_sync_flat.
-__global__ __launch_bounds__(256) void k_sync_multi(const MPoolDev *__restrict__ pools, int sel,
-                                                    const wh_patch *__restrict__ pt,
-                                                    const int32_t *__restrict__ ppool,
-                                                    const int64_t *__restrict__ pstart,
-                                                    const int *__restrict__ rng)
+// Ghost fill over the patches of several pools (one sync stage s): the
+// stage's slots are the pools' slot ranges one after the other (pool p from
+// s0[s]); slot e of pool p is slot e - s0[s] + pstart[p0[s]] of that pool's
+// own patch table, inside patch q = the last with pstart[q] <= it (padding
+// slots past a patch's points idle).  One thread per slot, persistent over
+// the slots, their count read from device memory.  Per point the copy /
+// decimation / prediction of k_sync_flat, into array `sel` of the pool.
+__global__ __launch_bounds__(256) void k_sync_multi(const MPoolDev *__restrict__ pools, int npools, int sel,
+                                                    int stage, const int64_t *__restrict__ tot)
 {
-    const int p0 = rng[0], p1 = rng[1];
-    if (p1 <= p0) return;
-    auto find = [&](int64_t x, int lo, int hi) {
+    const int s = stage - 1;
+    const int64_t end = tot[stage];
+    auto find = [](const int64_t *ps, int64_t x, int lo, int hi) {
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (pstart[mid] <= x) lo = mid;
+            if (ps[mid] <= x) lo = mid;
             else hi = mid - 1;
         }
         return lo;
     };
-    const int64_t end = pstart[p1];
-    for (int64_t w0 = pstart[p0] + (int64_t)blockIdx.x * blockDim.x; w0 < end;
-         w0 += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x; w0 < end; w0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t w1 = min(w0 + (int64_t)blockDim.x, end) - 1;
-        const int pa = find(w0, p0, p1 - 1);
-        const int pb = find(w1, pa, p1 - 1);
+        const int pa = pool_of(pools, npools, w0, s), pb = pool_of(pools, npools, w1, s);
         const int64_t e = w0 + threadIdx.x;
         if (e > w1) continue;
-        const int lo = find(e, pa, pb);
-        const wh_patch P = pt[lo];
-        const MPoolDev &M = pools[ppool[lo]];
+        const MPoolDev &M = pools[pa == pb ? pa : pool_of(pools, npools, e, s)];
+        const int64_t shift = M.pstart[M.p0[s]] - M.s0[s];  // pool slot of launch slot e: e + shift
+        int lo = M.p0[s], hi = M.p1[s] - 1;
+        if (pa == pb) {  // the workgroup's slots lie in one pool: narrow the search once
+            lo = find(M.pstart, w0 + shift, lo, hi);
+            hi = find(M.pstart, w1 + shift, lo, hi);
+        }
+        const int q = find(M.pstart, e + shift, lo, hi);
+        const wh_patch P = M.pt[q];
         const Geo &G = M.G;
-        const int r = (int)(e - pstart[lo]);
+        const int r = (int)(e + shift - M.pstart[q]);
         const int n0 = P.n[0], n1 = P.n[1];
+        if (r >= n0 * n1 * P.n[2]) continue;  // padding slot
         const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
         double *arr = M.arr[sel];
         const size_t ri = ((size_t)(P.recv_lo[2] + kk) * G.ny + (P.recv_lo[1] + j)) * G.nx + (P.recv_lo[0] + i);
@@ -1027,24 +1044,23 @@ __global__ __launch_bounds__(256) void k_sync_multi(const MPoolDev *__restrict__
     }
 }
 
-int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int sel, const wh_patch *d_pt,
-                      const int32_t *d_ppool, const int64_t *d_pstart, const int *d_rng, int grid)
+int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, int sel, int stage,
+                      const int64_t *d_tot, int grid)
 {
-    hipLaunchKernelGGL(k_sync_multi, dim3(grid), dim3(256), 0, stream, d_pools, sel, d_pt, d_ppool, d_pstart, d_rng);
+    hipLaunchKernelGGL(k_sync_multi, dim3(grid), dim3(256), 0, stream, d_pools, npools, sel, stage, d_tot);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
 
 int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
-                       const int2 *d_ent, const double *d_x0dx, const double *d_dt, const AcmDev *d_gacm,
-                       const int32_t *d_egrp, const int *d_nent, const MStage &ms, size_t lds, int grid,
-                       int nchunk)
+                       int npools, const double *d_dtg, const AcmDev *d_gacm, const int64_t *d_tot,
+                       const MStage &ms, size_t lds, int grid, int nchunk)
 {
     if (dim == 2) {
         WH_CHECK(lds <= 160 * 1024, WH_E_ARG, "2D block tile exceeds LDS");
 #define M2D(SC, SK)                                                                                         \
-    hipLaunchKernelGGL((k_rhs2d_multi<SC, SK>), dim3(grid), dim3(256), lds, stream, d_pools, d_ent, d_x0dx, d_dt, \
-                       d_gacm, d_egrp, d_nent, ms)
+    hipLaunchKernelGGL((k_rhs2d_multi<SC, SK>), dim3(grid), dim3(256), lds, stream, d_pools, npools, d_dtg, \
+                       d_gacm, d_tot, ms)
         switch (scheme * 2 + skew) {
         case WH_FD2 * 2: M2D(WH_FD2, false); break;
         case WH_FD2 * 2 + 1: M2D(WH_FD2, true); break;
@@ -1061,11 +1077,11 @@ int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xc
 #define M3D(SC, SK)                                                                                           \
     do {                                                                                                      \
         if (xcd)                                                                                              \
-            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, true>), dim3(grid), dim3(256), 0, stream, d_pools, d_ent,   \
-                               d_x0dx, d_dt, d_gacm, d_egrp, d_nent, ms, nchunk);                               \
+            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, true>), dim3(grid), dim3(256), 0, stream, d_pools, npools, \
+                               d_dtg, d_gacm, d_tot, ms, nchunk);                                              \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, false>), dim3(grid), dim3(256), 0, stream, d_pools, d_ent,  \
-                               d_x0dx, d_dt, d_gacm, d_egrp, d_nent, ms, nchunk);                               \
+            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, false>), dim3(grid), dim3(256), 0, stream, d_pools, npools, \
+                               d_dtg, d_gacm, d_tot, ms, nchunk);                                              \
     } while (0)
         switch (scheme * 2 + skew) {
         case WH_FD2 * 2: M3D(WH_FD2, false); break;
