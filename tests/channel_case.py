@@ -57,6 +57,28 @@ def oracle_run(w, tc, lv, hvy, nsteps, on_step=None):
     return tree, time
 
 
+def oracle_steps(w, tc, lv, hvy):
+    """The oracle's main loop as a generator of (time, treecodes, levels,
+    hvy) after each iteration, so a test can compare step by step without
+    holding every state."""
+    tree = oracle_tree(w, tc, lv, hvy)
+    ap = po.make_params(w["acm"])
+    mp = po.mask_params(**w["mask"])
+    bt = butcher_rk4()
+    time = 0.0
+    try:
+        while True:
+            tree.sync()
+            tree.refine_everywhere()
+            dt = clamp_dt(tree.dt(ap), time, w["tp"])
+            tree.rk_step(ap, mp, bt, dt, GRHS)
+            time += dt
+            tree.adapt()
+            yield (time,) + tuple(tree.get())
+    finally:
+        tree.close()
+
+
 def sorted_keys(tc, lv):
     return sorted((int(a), int(b)) for a, b in zip(tc, lv))
 

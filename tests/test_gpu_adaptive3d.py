@@ -1,7 +1,7 @@
 """GPU: the adaptive 3D periodic workload (CDF44, eps=1e-3, solenoidal
-Fourier velocity) at reduced size (Bs=16, Jmin=2, Jmax=3) on the device
-against the oracle: the initial grid and two main-loop iterations (sync,
-refine everywhere, RK4, adapt_tree).
+Fourier velocity) at reduced Jmax (Jmin=2, Jmax=3) on the device against
+the oracle, at Bs=16 and at config 3's Bs=24: the initial grid and two
+main-loop iterations (sync, refine everywhere, RK4, adapt_tree).
 
 Bars: block lists identical after every adapt_tree; fields within relative
 L2 1e-12 after the first step and 1e-9 after the second."""
@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import periodic3d_case as C
+from wabbit_amd import wavelets as W
 from wabbit_amd import workloads as wl
 
 pytestmark = pytest.mark.gpu
@@ -16,10 +17,16 @@ pytestmark = pytest.mark.gpu
 NSTEPS = 2
 
 
-def test_adaptive_3d_matches_oracle():
-    w = wl.periodic3d(jmax=3, bs=16)
+# Bs=16 takes adapt_tree's level-wise decomposition, Bs=24 (config 3's block
+# size) the leaf-first one: CDF44's HD reaches 6 points, and the leaf-first
+# branch needs Bs >= 3*6 (adapt_tree.f90:321-322)
+@pytest.mark.parametrize("bs", [16, 24])
+def test_adaptive_3d_matches_oracle(bs):
+    w = wl.periodic3d(jmax=3, bs=bs)
+    wv = W.setup(w["wavelet"])
+    assert (bs >= 3 * max(-wv.hd_lo, wv.hd_hi)) == (bs == 24)
     tc_o, lv_o, hvy_o = C.oracle_initial_grid(w)
-    print(f"initial grid {len(tc_o)} blocks", flush=True)
+    print(f"Bs {bs}: initial grid {len(tc_o)} blocks", flush=True)
     states = []
     C.oracle_run(w, tc_o, lv_o, hvy_o, NSTEPS, lambda tree, t, it: states.append((t,) + tree.get()))
     s = wl.adaptive_solver(w, nb_max=4096)

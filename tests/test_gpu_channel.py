@@ -16,7 +16,7 @@ from wabbit_amd.timestep import PATH_FUSED
 
 pytestmark = pytest.mark.gpu
 
-NSTEPS = 12
+NSTEPS = 500
 
 
 @pytest.fixture(scope="module")
@@ -51,16 +51,18 @@ def test_initial_grid(case):
 
 def test_steps(case):
     tc_o, lv_o, hvy_o = C.oracle_initial_grid(case)
-    states = []
-    C.oracle_run(case, tc_o, lv_o, hvy_o, NSTEPS, lambda tree, t, it: states.append((t,) + tree.get()))
+    oracle = C.oracle_steps(case, tc_o, lv_o, hvy_o)
     s = wl.channel_solver(case, path=PATH_FUSED, nb_max=8192)
     try:
         for k in range(NSTEPS):
             s.step()
-            t, tc, lv, hvy = states[k]
+            t, tc, lv, hvy = next(oracle)
+            if (k + 1) % 20 == 0:
+                print(f"step {k + 1}: {len(tc)} blocks, t={t:.6f}", flush=True)
             assert abs(s.time - t) <= 1e-12 * t
             _compare(s, tc, lv, hvy, 1e-12 if k == 0 else 1e-9)
     finally:
+        oracle.close()
         s.close()
 
 
