@@ -429,14 +429,28 @@ def run_serving(args):
     --requests of the stream."""
     from wabbit_amd import workloads
     from wabbit_amd.serve import Request, Server
+    from wabbit_amd.batch import budget_slots, pool_key, slot_bytes
     mix = workloads.serving_mix(n=args.requests, rate=args.rate)
     reqs = [Request(rid, spec, t) for rid, spec, t in mix]
-    # pools: 2D Bs16, 2D Bs24, 3D Bs24 -- about 55% of the 288 GB; each of the
-    # lanes gets 1/lanes of every pool, enough for the largest request (2D
-    # Jmax 7: 12288 slots, 3D Jmax 5: 4096 slots)
+    # every pool created at start-up within 60% of the device memory (the
+    # reference's --memory): one pool per block shape and lane, so admission
+    # only takes free slots; the 3D pool gets most of the budget (a 3D Bs=24
+    # slot holds ~7.5 MB, a 2D one 0.13-0.22 MB)
     lanes = args.lanes
-    slots = {(2, 16): 16000 * lanes, (2, 24): 16000 * lanes, (3, 24): 4500 * lanes}
-    srv = Server(slots, n_work_slots=3, lanes=lanes)
+    keys = sorted({pool_key(r.spec) for r in reqs})
+    from wabbit_amd import lib as L
+    total = L.device_memory()[1]
+    weights = {k: (0.78 if k[0] == 3 else 0.13 if k[1] == 24 else 0.09) for k in keys}
+    slots = budget_slots(keys, weights, 0.6 * total, lanes)
+    for k in keys:
+        need = max(r.spec.slots for r in reqs if pool_key(r.spec) == k)
+        if slots[(k[0], k[1])] // lanes < need:
+            raise SystemExit(f"c5: a lane's share of the {k[0]}D Bs{k[1]} pool ({slots[(k[0], k[1])] // lanes} slots) "
+                             f"is smaller than its largest request ({need}); use fewer lanes")
+    pool_bytes = sum(slots[(k[0], k[1])] * slot_bytes(k) for k in keys)
+    print(f"# c5: {len(keys)} pool shapes x {lanes} lanes, {pool_bytes / 1e9:.1f} GB of {total / 1e9:.1f} GB "
+          f"({100 * pool_bytes / total:.0f} %) preallocated", file=sys.stderr, flush=True)
+    srv = Server(slots, n_work_slots=3, lanes=lanes, prealloc=keys)
     st = srv.run(reqs, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     srv.close()
     out = {
@@ -444,7 +458,8 @@ def run_serving(args):
         "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"c5-serving-{len(reqs)}req-rate{args.rate}", "requests": len(reqs),
-                   "lanes": args.lanes, "pool_slots": {f"{k[0]}d-Bs{k[1]}": v for k, v in slots.items()}},
+                   "lanes": args.lanes, "pool_slots": {f"{k[0]}d-Bs{k[1]}": v for k, v in slots.items()},
+                   "preallocated_bytes": int(pool_bytes), "device_bytes": int(total)},
         "p50_latency_s": round(st.p50, 4), "p99_latency_s": round(st.p99, 4),
         "mean_queue_wait_s": round(st.mean_wait, 4), "completed": st.completed, "failed": st.failed,
         "wall_s": round(st.wall, 3), "block_RK_stages_per_s": round(st.block_rk_stages / st.wall, 1),
@@ -461,11 +476,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c4", choices=["c4", "c2", "c3", "c5", "c1", "c3u"])
     ap.add_argument("--members", type=int, default=64, help="c4: ensemble members")
-    ap.add_argument("--requests", type=int, default=24, help="c5: requests taken from the stream")
+    ap.add_argument("--requests", type=int, default=256, help="c5: requests taken from the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (1/s)")
     ap.add_argument("--lanes", type=int, default=None,
                     help="c4/c5: host threads (pool sets) stepping solves (default c4: 12, c5: 4 -- c5's "
-                         "3D pool share per lane must hold a Jmax-5 request within ~55%% of HBM)")
+                         "3D pool share per lane must hold a Jmax-5 request within 60%% of HBM)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -93,6 +93,10 @@ typedef struct {
 
 const char *wh_last_error(void);
 int wh_version(void);
+/* Free and total memory of the current device, through the library's own
+ * HIP runtime: what a host sizes its pools from, as allocate_forest turns
+ * --memory into a block count (LIB/MESH/allocate_forest.f90:160-196). */
+int wh_device_memory(size_t *free_bytes, size_t *total_bytes);
 
 /* ---- pool: allocate_forest (LIB/MESH/allocate_forest.f90:1-381) -------- */
 int wh_pool_create(wh_pool **out, const wh_pool_desc *desc);

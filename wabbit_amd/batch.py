@@ -86,6 +86,29 @@ def pool_key(spec: "SolveSpec"):
             bool(spec.acm.skew_symmetry))
 
 
+def slot_bytes(key, n_work_slots: int = 3) -> int:
+    """Device bytes one pool slot of pool_key `key` holds: hvy_block, the
+    work slots and hvy_tmp (neq components each) plus the 6-component mask
+    of penalised pools."""
+    dim, bs, wavelet, masked, scheme, _ = key
+    g = W.setup(wavelet, fd_reach(scheme)).g
+    cs = (bs + 2 * g) ** dim
+    neq = dim + 1
+    return 8 * cs * (neq * (2 + n_work_slots) + (6 if masked else 0))
+
+
+def budget_slots(keys, weights, budget_bytes: float, lanes: int = 1, n_work_slots: int = 3) -> Dict[tuple, int]:
+    """Slots per (dim, Bs) for pools created at start-up within budget_bytes:
+    key k gets weights[k] of the budget, rounded down to a multiple of
+    lanes (every lane's pool set holds 1/lanes of each pool)."""
+    out = {}
+    tot = sum(weights[k] for k in keys)
+    for k in keys:
+        n = int(budget_bytes * weights[k] / tot // slot_bytes(k, n_work_slots))
+        out[(k[0], k[1])] = n - n % lanes
+    return out
+
+
 class Solve:
     """A solve admitted to a pool group."""
 

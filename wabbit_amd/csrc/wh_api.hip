@@ -310,6 +310,13 @@ static int upload_staged(wh_pool *P, void *dst, const void *src, size_t bytes)
 
 int wh_version(void) { return 100; }
 
+int wh_device_memory(size_t *free_bytes, size_t *total_bytes)
+{
+    WH_CHECK(free_bytes && total_bytes, WH_E_ARG, "wh_device_memory: null argument");
+    WH_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return WH_OK;
+}
+
 int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
 {
     WH_CHECK(out && desc, WH_E_ARG, "null argument");

@@ -122,6 +122,7 @@ def load(path: str = LIB_PATH):
         "wh_batch_time_stage_kernel": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
         "wh_batch_synchronize": [_vp],
+        "wh_device_memory": [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -129,6 +130,15 @@ def load(path: str = LIB_PATH):
         fn.restype = ctypes.c_int
     _LIB = L
     return L
+
+
+def device_memory():
+    """(free, total) bytes of the current device, from the library's own HIP
+    runtime (torch bundles another one; initialising that one first leaves
+    this one without a device)."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    check(load().wh_device_memory(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 def check(rc):
