@@ -881,20 +881,22 @@ wh_box interior_box(const Tree &T, int i)
 }
 
 // componentWiseNorm_tree(hvy_tmp, "Linfty") over the leaves
-// (OPERATORS/componentWiseNorm_tree.f90), then norm <= 1e-9 -> 1
+// (OPERATORS/componentWiseNorm_tree.f90): the leaves' interior boxes, and
+// the norm from their per-box maxima m, then norm <= 1e-9 -> 1
 // (coarseningIndicator_tree.f90:160-163)
-int leaf_linfty(Tree &T, const int32_t *groups, double *norm)
+std::vector<wh_box> leaf_boxes(const Tree &T)
 {
-    WH_PROF("leaf_linfty");
-    const int neq = T.P->d.neq;
     std::vector<wh_box> boxes;
     for (int i = 0; i < (int)T.b.size(); ++i)
         if (T.is_leaf(i, false)) boxes.push_back(interior_box(T, i));
-    std::vector<double> m;
-    if (int rc = run_region_max(T.P, A_T, boxes, false, false, groups, m)) return rc;
+    return boxes;
+}
+
+void leaf_linfty(size_t nbox, int neq, const int32_t *groups, const double *m, double *norm)
+{
     for (int c = 0; c < neq; ++c) norm[c] = -1.0;
-    for (size_t k = 0; k < boxes.size(); ++k) {
-        double *v = &m[k * neq];
+    for (size_t k = 0; k < nbox; ++k) {
+        const double *v = &m[k * neq];
         for (int c = 0; c < neq; ++c)
             if (groups[c] == 1) norm[c] = std::max(norm[c], v[c]);
         int maxg = 0;
@@ -909,18 +911,16 @@ int leaf_linfty(Tree &T, const int32_t *groups, double *norm)
     }
     for (int c = 0; c < neq; ++c)
         if (norm[c] <= 1.0e-9) norm[c] = 1.0;
-    return WH_OK;
 }
 
 // coarseningIndicatorMask_tree (coarseningIndicator_tree.f90:338-422): the
-// mask of every block of the full tree; a block whose chi is partly inside
-// (0,1) or not constant stays
-int mask_criterion(Tree &T, const wh_adapt_params &p)
+// mask of every block of the full tree (mask_blocks lists them and launches
+// the mask); a block whose chi is partly inside (0,1) or not constant stays
+// (mask_statuses, from [between, max, min] per listed block)
+int mask_blocks(Tree &T, const wh_adapt_params &p, std::vector<int> &idx, std::vector<int32_t> &ids)
 {
     WH_PROF("mask_criterion");
-    std::vector<int32_t> ids;
     std::vector<double> x0dx;
-    std::vector<int> idx;
     for (int i = 0; i < (int)T.b.size(); ++i) {
         if (p.force_maxlevel_dealiasing && T.b[i].level == p.jmax) continue;
         idx.push_back(i);
@@ -929,16 +929,17 @@ int mask_criterion(Tree &T, const wh_adapt_params &p)
         T.F->x0dx(FBlock{T.b[i].tc, T.b[i].level, T.b[i].hvy, 0}, xd);
         x0dx.insert(x0dx.end(), xd, xd + 6);
     }
-    if (int rc = run_mask(T.P, p.mask, ids, x0dx)) return rc;
-    std::vector<double> st;
-    if (int rc = run_mask_stats(T.P, ids, st)) return rc;
+    return run_mask(T.P, p.mask, ids, x0dx);
+}
+
+void mask_statuses(Tree &T, const std::vector<int> &idx, const double *st)
+{
     for (size_t k = 0; k < idx.size(); ++k) {
         const bool stay = st[3 * k] != 0.0 || (st[3 * k + 1] - st[3 * k + 2]) > 1.0e-12;
         const int ms = stay ? 0 : -1;
         int &s = T.b[idx[k]].status;
         s = std::max(s, ms);
     }
-    return WH_OK;
 }
 
 // addSecurityZone_CE_tree (securityZone_tree.f90): a block that would coarsen
@@ -1073,26 +1074,52 @@ static int adapt_tree_impl(wh_pool *P, wh_forest *F, const wh_adapt_params *p, c
         for (int i = 0; i < (int)T.b.size(); ++i)
             if (T.is_leaf(i, false)) T.b[i].status = -1;
     } else {
+        // the three reductions that need no host decision -- the leaves'
+        // norms, the blocks' details, the mask statistics -- go out together
+        // and come back in one round trip; the host logic below then runs in
+        // the reference's order
+        const int neq = P->d.neq;
         double norm[8];
         for (int c = 0; c < 8; ++c) norm[c] = p->norm[c];
-        if (p->eps_normalized)
-            if (int rc = leaf_linfty(T, p->groups, norm)) return fail(rc);
+        std::vector<wh_box> nboxes;
+        if (p->eps_normalized) nboxes = leaf_boxes(T);
         std::vector<int> idx;
         for (int i = 0; i < (int)T.b.size(); ++i) {
             if (p->force_maxlevel_dealiasing && T.b[i].level == p->jmax) T.b[i].status = -1;
             else idx.push_back(i);
         }
-        std::vector<double> det;
-        if (int rc = run_detail(P, A_U, hvy_of(T, idx), p->groups, det)) return fail(rc);
-        const int neq = P->d.neq;
+        // the mask criterion lists the same blocks as the details
+        std::vector<int> midx;
+        std::vector<int32_t> mids;
+        const size_t o_det = nboxes.size() * neq, o_mask = o_det + idx.size() * neq;
+        const size_t n_res = o_mask + (p->threshold_mask ? idx.size() * 3 : 0);
+        double *d_res = nullptr;
+        std::vector<double> res;
+        {
+            WH_PROF("indicator_reductions");
+            if (int rc = results_reserve(P, n_res, &d_res)) return fail(rc);
+            if (int rc = launch_region_max(P, A_T, nboxes, false, false, p->groups, d_res)) return fail(rc);
+            if (int rc = launch_detail_list(P, A_U, hvy_of(T, idx), p->groups, d_res + o_det)) return fail(rc);
+            if (p->threshold_mask) {
+                if (int rc = mask_blocks(T, *p, midx, mids)) return fail(rc);
+                if (mids.size() != idx.size()) {
+                    set_error("adapt: mask and detail block lists differ");
+                    return fail(WH_E_STATE);
+                }
+                if (int rc = launch_mask_stats(P, mids, d_res + o_mask)) return fail(rc);
+            }
+            if (int rc = results_fetch(P, n_res, res)) return fail(rc);
+        }
+        if (p->eps_normalized) leaf_linfty(nboxes.size(), neq, p->groups, res.data(), norm);
+        double *det = res.data() + o_det;
+        detail_groups(P, p->groups, det, (int)idx.size());
         for (size_t k = 0; k < idx.size(); ++k) {
             bool all = true;
             for (int c = 0; c < neq; ++c)
                 if (!(det[k * neq + c] <= p->eps * norm[c])) all = false;
             T.b[idx[k]].status = all ? -1 : 0;
         }
-        if (p->threshold_mask)
-            if (int rc = mask_criterion(T, *p)) return fail(rc);
+        if (p->threshold_mask) mask_statuses(T, midx, res.data() + o_mask);
         if (p->security_zone)
             if (int rc = security_zone(T, *p, norm)) return fail(rc);
     }

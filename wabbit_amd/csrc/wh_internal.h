@@ -268,6 +268,20 @@ int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int
 enum { WOP_FWT = 0, WOP_IWT = 1, WOP_FILTER = 2, WOP_COPY = 3 };
 // per-box component maxima (|v|, SCs zeroed with zero_sc, sqrt(v*v) for
 // grouped components with sqrt_groups): out[k*neq + c]
+// Reductions launched into a caller's device range d_out (no copy, no
+// synchronisation), so that several of them come back in one round trip:
+// results_reserve sizes the pool's result buffer for n doubles (before any
+// of the launches), results_fetch copies them back and synchronises.
+int launch_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
+                      const int32_t *groups, double *d_out);
+int launch_detail_list(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
+                       double *d_out);
+int launch_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, double *d_out);
+int results_reserve(wh_pool *P, size_t n, double **d_out);
+int results_fetch(wh_pool *P, size_t n, std::vector<double> &out);
+// threshold_block's grouping of detail components (threshold_block.f90:86-91),
+// in place on n blocks of neq values
+void detail_groups(const wh_pool *P, const int32_t *groups, double *detail, int n);
 int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
                    const int32_t *groups, std::vector<double> &out);
 // penalisation mask (wh_mask.hip)

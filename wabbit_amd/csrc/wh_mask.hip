@@ -249,28 +249,31 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
     return WH_OK;
 }
 
+int launch_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, double *d_out)
+{
+    if (int rc = check_mask_pool(P)) return rc;
+    const int n = (int)ids.size();
+    if (n == 0) return WH_OK;
+    for (int32_t h : ids) WH_CHECK(h >= 0 && h < P->d.nb_max, WH_E_ARG, "mask stats: hvy id out of range");
+    if (int rc = stage_begin(P, sizeof(int32_t) * n)) return rc;
+    const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n);
+    if (!d_ids) return WH_E_HIP;
+    hipLaunchKernelGGL(k_mask_stats, dim3(n), dim3(256), 0, P->stream, P->mask, P->mblk, P->cs, d_ids, P->nx,
+                       P->ny, P->d.g, P->d.bs[0], P->d.bs[1], P->d.dim == 3 ? P->d.bs[2] : 1, d_out);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
 int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<double> &stats)
 {
     if (int rc = check_mask_pool(P)) return rc;
     const int n = (int)ids.size();
     stats.assign((size_t)n * 3, 0.0);
     if (n == 0) return WH_OK;
-    WaveletWS &W = P->ws;
-    if (W.out_cap < (size_t)n * 3) {
-        // the buffer is only used by downloads, which synchronise: nothing in flight reads it
-        if (W.d_out) WH_HIP(hipFree(W.d_out));
-        W.d_out = nullptr;
-        WH_HIP(hipMalloc(&W.d_out, sizeof(double) * n * 3));
-        W.out_cap = (size_t)n * 3;
-    }
-    if (int rc = stage_begin(P, sizeof(int32_t) * n)) return rc;
-    const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n);
-    if (!d_ids) return WH_E_HIP;
-    hipLaunchKernelGGL(k_mask_stats, dim3(n), dim3(256), 0, P->stream, P->mask, P->mblk, P->cs, d_ids, P->nx,
-                       P->ny, P->d.g, P->d.bs[0], P->d.bs[1], P->d.dim == 3 ? P->d.bs[2] : 1, W.d_out);
-    WH_HIP(hipGetLastError());
-    WH_HIP(hipMemcpyAsync(stats.data(), W.d_out, sizeof(double) * n * 3, hipMemcpyDeviceToHost, P->stream));
-    return stage_sync(P);
+    double *d_out = nullptr;
+    if (int rc = results_reserve(P, (size_t)n * 3, &d_out)) return rc;
+    if (int rc = launch_mask_stats(P, ids, d_out)) return rc;
+    return results_fetch(P, (size_t)n * 3, stats);
 }
 
 }  // namespace wh

@@ -1083,6 +1083,11 @@ void apply_detail_groups(double *detail, int n, const DetailArgs &D)
     }
 }
 
+void detail_groups(const wh_pool *P, const int32_t *groups, double *detail, int n)
+{
+    apply_detail_groups(detail, n, detail_args(P, groups));
+}
+
 int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
                std::vector<double> &detail)
 {
@@ -1090,29 +1095,20 @@ int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int
     const int n = (int)ids.size(), neq = P->d.neq;
     detail.assign((size_t)n * neq, 0.0);
     if (n == 0) return WH_OK;
-    WH_CHECK(neq <= 8, WH_E_ARG, "detail: at most 8 components");
-    size_t blk = 0;
-    int nc = 0;
-    const double *arr = array_ptr(P, array, &blk, &nc);
-    WH_CHECK(arr && nc >= neq, WH_E_ARG, "detail: bad array");
-    if (int rc = check_ids(P, ids.data(), n)) return rc;
-    if (int rc = upload_ids(P, ids.data(), n)) return rc;
     WaveletWS &W = P->ws;
     if (int rc = ensure(W.d_out, W.out_cap, (size_t)n * neq)) return rc;
-    const DetailArgs D = detail_args(P, groups);
-    if (int rc = launch_detail(P, arr, blk, W.cur_ids, n, D, W.d_out)) return rc;
+    if (int rc = launch_detail_list(P, array, ids, groups, W.d_out)) return rc;
     WH_HIP(hipMemcpyAsync(detail.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost,
                           P->stream));
     if (int rc = stage_sync(P)) return rc;
-    apply_detail_groups(detail.data(), n, D);
+    apply_detail_groups(detail.data(), n, detail_args(P, groups));
     return WH_OK;
 }
 
-int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
-                   const int32_t *groups, std::vector<double> &out)
+int launch_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
+                      const int32_t *groups, double *d_out)
 {
     const int n = (int)boxes.size(), neq = P->d.neq;
-    out.assign((size_t)n * neq, 0.0);
     if (n == 0) return WH_OK;
     WH_CHECK(neq <= 8, WH_E_ARG, "region max: at most 8 components");
     size_t blk = 0;
@@ -1125,8 +1121,6 @@ int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool
         for (int d = 0; d < 3; ++d)
             WH_CHECK(b.lo[d] >= 0 && b.hi[d] < lim[d] && b.lo[d] <= b.hi[d], WH_E_ARG, "region max: bad box");
     }
-    WaveletWS &W = P->ws;
-    if (int rc = ensure(W.d_out, W.out_cap, (size_t)n * neq)) return rc;
     if (int rc = stage_begin(P, sizeof(wh_box) * n)) return rc;
     const wh_box *d_boxes = (const wh_box *)stage_put(P, boxes.data(), sizeof(wh_box) * n);
     if (!d_boxes) return WH_E_HIP;
@@ -1138,10 +1132,52 @@ int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool
     R.sqrt_groups = sqrt_groups;
     for (int c = 0; c < 8; ++c) R.groups[c] = (groups && c < neq) ? groups[c] : 1;
     const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
-    hipLaunchKernelGGL(k_region_max, dim3(n), dim3(256), 0, P->stream, arr, d_boxes, A, R, W.d_out);
+    hipLaunchKernelGGL(k_region_max, dim3(n), dim3(256), 0, P->stream, arr, d_boxes, A, R, d_out);
     WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
+                   const int32_t *groups, std::vector<double> &out)
+{
+    const int n = (int)boxes.size(), neq = P->d.neq;
+    out.assign((size_t)n * neq, 0.0);
+    if (n == 0) return WH_OK;
+    WaveletWS &W = P->ws;
+    if (int rc = ensure(W.d_out, W.out_cap, (size_t)n * neq)) return rc;
+    if (int rc = launch_region_max(P, array, boxes, zero_sc, sqrt_groups, groups, W.d_out)) return rc;
     WH_HIP(hipMemcpyAsync(out.data(), W.d_out, sizeof(double) * n * neq, hipMemcpyDeviceToHost, P->stream));
     return stage_sync(P);
+}
+
+int results_reserve(wh_pool *P, size_t n, double **d_out)
+{
+    WaveletWS &W = P->ws;
+    if (int rc = ensure(W.d_out, W.out_cap, std::max<size_t>(n, 1))) return rc;
+    *d_out = W.d_out;
+    return WH_OK;
+}
+
+int results_fetch(wh_pool *P, size_t n, std::vector<double> &out)
+{
+    out.assign(n, 0.0);
+    if (n) WH_HIP(hipMemcpyAsync(out.data(), P->ws.d_out, sizeof(double) * n, hipMemcpyDeviceToHost, P->stream));
+    return stage_sync(P);
+}
+
+int launch_detail_list(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
+                       double *d_out)
+{
+    const int n = (int)ids.size(), neq = P->d.neq;
+    if (n == 0) return WH_OK;
+    WH_CHECK(neq <= 8, WH_E_ARG, "detail: at most 8 components");
+    size_t blk = 0;
+    int nc = 0;
+    const double *arr = array_ptr(P, array, &blk, &nc);
+    WH_CHECK(arr && nc >= neq, WH_E_ARG, "detail: bad array");
+    if (int rc = check_ids(P, ids.data(), n)) return rc;
+    if (int rc = upload_ids(P, ids.data(), n)) return rc;
+    return launch_detail(P, arr, blk, P->ws.cur_ids, n, detail_args(P, groups), d_out);
 }
 
 int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<wh_patch> &patches_in)
