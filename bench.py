@@ -479,7 +479,7 @@ def main():
     ap.add_argument("--requests", type=int, default=256, help="c5: requests taken from the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (1/s)")
     ap.add_argument("--lanes", type=int, default=None,
-                    help="c4/c5: host threads (pool sets) stepping solves (default c4: 12, c5: 4 -- c5's "
+                    help="c4/c5: host threads (pool sets) stepping solves (default c4: 4, c5: 4 -- c5's "
                          "3D pool share per lane must hold a Jmax-5 request within 60%% of HBM)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
@@ -494,7 +494,7 @@ def main():
               "(there is no multi-GPU decomposition)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c4":
-        args.lanes = args.lanes or 12
+        args.lanes = args.lanes or 4
         run_ensemble(args)
         return
     if args.workload == "c5":

@@ -445,6 +445,20 @@ int wh_fwt_threshold_noalloc(wh_pool *pool, int src_array, int dst_array, const 
  * (restrict_copy_at_CE, restrict_predict_data.f90:101-170). */
 int wh_sync_tree(wh_pool *pool, wh_forest *forest, int array);
 
+/* sync_ghosts_tree and adapt_tree for n forests that share the pool
+ * (independent solves, each in its own slot range): per forest the same
+ * sequence of steps as wh_sync_tree / wh_adapt_tree, with every device
+ * operation launched once for all forests over their concatenated lists and
+ * the coarsening indicator's reductions of all forests read back together.
+ * No counterpart in the reference (one forest per process there).
+ * wh_adapt_trees: params[i] and status[i] per forest; status[i] is forest
+ * i's own error (its checks, its slots), which leaves its block list
+ * unchanged while the others go on; the return value is an error only when
+ * an operation shared by all forests failed. */
+int wh_sync_trees(wh_pool *pool, wh_forest *const *forests, int n, int array);
+int wh_adapt_trees(wh_pool *pool, wh_forest *const *forests, int n, const wh_adapt_params *params,
+                   int *status);
+
 /* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
  * stage kernel (each owns Bs1/ysplit rows of (x,y) columns and marches in z);
  * 0 selects the flat one-point-per-thread kernel, -1 automatic.

@@ -36,7 +36,7 @@ from .adaptive import AdaptConfig, fd_reach
 from .forest import equidistant_treecodes, refined_treecodes
 from .params import ACMParams, TimeParams, butcher_rk4, clamp_dt
 from .timestep import PATH_FUSED
-from .tree import Forest, refine_on_device
+from .tree import Forest, refine_blocks, refine_on_device
 
 
 @dataclass
@@ -215,20 +215,42 @@ class PoolGroup:
         tables."""
         pool, w = self.pool, self.w
         hvs, xs, grp, pats, live = [], [], [], [], []
-        for s in self.solves:
-            if s.done:
-                continue
+        cand = [s for s in self.solves if not s.done]
+        # sync_ghosts_tree of every solve, one launch per operation for all of
+        # them; should that fail, each solve alone, so a failure stays its own
+        try:
+            W.sync_trees(pool, [s.forest for s in cand])
+        except L.WabbitHipError:
+            for s in cand:
+                try:
+                    W.sync_tree(pool, s.forest)
+                except L.WabbitHipError as e:
+                    s.error = str(e)
+            cand = [s for s in cand if s.error is None]
+        # refine everywhere: every solve's new blocks on the host, one
+        # refinement launch for all (a solve whose free slots cannot take all
+        # its daughters at once refines in batches of its own)
+        tris = []
+        for s in list(cand):
             try:
-                W.sync_tree(pool, s.forest)
                 tc, lv, hv, _ = s.forest.blocks()
-                ids = hv[lv < s.spec.cfg.jmax]
-                if ids.size:
+                ids = np.asarray(hv[lv < s.spec.cfg.jmax], np.int32)
+                if ids.size and s.forest.nb_max - s.forest.n >= (1 << s.forest.dim) * ids.size:
+                    tris.append(s.forest.refine(ids))
+                elif ids.size:
                     refine_on_device(pool, s.forest, ids, w.order, w.g_rhs, presync=None, attach=False)
+            except L.WabbitHipError as e:    # the solve alone fails (e.g. out of its slots)
+                s.error = str(e)
+                cand.remove(s)
+        if tris:
+            refine_blocks(pool, np.concatenate(tris), w.order)
+        for s in cand:
+            try:
                 tc, lv, hv, x = s.forest.blocks()
                 if s.mask_p is not None:
                     W.create_mask(pool, s.mask_p, hv, x)
                 pat = s.forest.patches(w.g_rhs, w.g_rhs, w.order)
-            except L.WabbitHipError as e:    # the solve alone fails (e.g. out of its slots)
+            except L.WabbitHipError as e:    # the solve alone fails
                 s.error = str(e)
                 continue
             s.rk_blocks = len(hv)
@@ -261,11 +283,15 @@ class PoolGroup:
             s.iteration += 1
             stages += 4 * s.rk_blocks
         self.last_stages = stages
-        for s in self._live:
-            try:
-                W.adapt_tree(self.pool, s.forest, s.adapt_p)
-            except L.WabbitHipError as e:
-                s.error = str(e)
+        # adapt_tree of every solve, their device operations launched together
+        live = [s for s in self._live if s.error is None]
+        try:
+            res = W.adapt_trees(self.pool, [s.forest for s in live], [s.adapt_p for s in live])
+        except L.WabbitHipError as e:   # an operation shared by all of them failed
+            res = [(e.code, str(e))] * len(live)
+        for s, (code, msg) in zip(live, res):
+            if code:
+                s.error = f"{L.ERRORS.get(code, code)}: {msg}"
         self._live = []
         return stages
 

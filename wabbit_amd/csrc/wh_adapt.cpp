@@ -70,7 +70,25 @@ int dirs_of(int dim, int dirs[26][3])
     return nd;
 }
 
+// The device operations of the orchestration: launched at once, or logged
+// per tree (Tree::log) and replayed for several trees of one pool together
+// (replay): round r launches the r-th operation of every tree, operations of
+// one kind and arguments as one launch over their concatenated lists.  The
+// trees' blocks are disjoint slots of the pool's arrays, so only each tree's
+// own order matters.
+enum OpKind { OP_BLOCKS = 0, OP_BOXES = 1, OP_PATCHES = 2 };
+struct DevOp {
+    int kind = 0, a = 0, b = 0, c = 0;  // BLOCKS: op, src, dst; BOXES: array, src; PATCHES: array, alt1, alt2
+    std::vector<int32_t> ids;
+    std::vector<wh_box> boxes;
+    std::vector<wh_patch> patches;
+};
+struct OpLog {
+    std::vector<DevOp> ops;
+};
+
 struct Tree {
+    OpLog *log = nullptr;
     wh_forest *F;
     wh_pool *P;
     wh_wavelet w;
@@ -336,6 +354,83 @@ struct Tree {
 };
 
 
+int dev_blocks(Tree &T, int op, int src, int dst, std::vector<int32_t> ids)
+{
+    if (!T.log) return run_blocks_op(T.P, op, src, dst, ids);
+    if (ids.empty()) return WH_OK;
+    DevOp o;
+    o.kind = OP_BLOCKS;
+    o.a = op;
+    o.b = src;
+    o.c = dst;
+    o.ids = std::move(ids);
+    T.log->ops.push_back(std::move(o));
+    return WH_OK;
+}
+
+int dev_boxes(Tree &T, int array, int src, std::vector<wh_box> boxes)
+{
+    if (!T.log) return run_boxes(T.P, array, src, boxes);
+    if (boxes.empty()) return WH_OK;
+    DevOp o;
+    o.kind = OP_BOXES;
+    o.a = array;
+    o.b = src;
+    o.boxes = std::move(boxes);
+    T.log->ops.push_back(std::move(o));
+    return WH_OK;
+}
+
+int dev_patches(Tree &T, int array, int alt1, int alt2, std::vector<wh_patch> patches)
+{
+    if (!T.log) return run_patch_list(T.P, array, alt1, alt2, patches);
+    if (patches.empty()) return WH_OK;
+    DevOp o;
+    o.kind = OP_PATCHES;
+    o.a = array;
+    o.b = alt1;
+    o.c = alt2;
+    o.patches = std::move(patches);
+    T.log->ops.push_back(std::move(o));
+    return WH_OK;
+}
+
+int replay(wh_pool *P, const std::vector<OpLog *> &logs)
+{
+    WH_PROF("replay");
+    size_t rounds = 0;
+    for (const OpLog *l : logs) rounds = std::max(rounds, l->ops.size());
+    int rc = WH_OK;
+    for (size_t r = 0; r < rounds && !rc; ++r) {
+        std::vector<DevOp> merged;
+        for (OpLog *l : logs) {
+            if (r >= l->ops.size()) continue;
+            DevOp &o = l->ops[r];
+            DevOp *m = nullptr;
+            for (DevOp &x : merged)
+                if (x.kind == o.kind && x.a == o.a && x.b == o.b && x.c == o.c) {
+                    m = &x;
+                    break;
+                }
+            if (!m) {
+                merged.push_back(std::move(o));
+                continue;
+            }
+            m->ids.insert(m->ids.end(), o.ids.begin(), o.ids.end());
+            m->boxes.insert(m->boxes.end(), o.boxes.begin(), o.boxes.end());
+            m->patches.insert(m->patches.end(), o.patches.begin(), o.patches.end());
+        }
+        for (const DevOp &m : merged) {
+            if (m.kind == OP_BLOCKS) rc = run_blocks_op(P, m.a, m.b, m.c, m.ids);
+            else if (m.kind == OP_BOXES) rc = run_boxes(P, m.a, m.b, m.boxes);
+            else rc = run_patch_list(P, m.a, m.b, m.c, m.patches);
+            if (rc) break;
+        }
+    }
+    for (OpLog *l : logs) l->ops.clear();
+    return rc;
+}
+
 // ---- coarse extension boxes ------------------------------------------------
 // get_indices_of_modify_patch (neighborhood.f90) with lvl_diff = +1: the
 // first Nl / last Nr interior points toward direction o, full interior across
@@ -413,7 +508,7 @@ int ce_modify(Tree &T, CeCase ce, int s_level, int s_ref, bool clear_wc, bool co
             }
         }
     }
-    return run_boxes(T.P, A_U, A_T, boxes);
+    return dev_boxes(T, A_U, A_T, std::move(boxes));
 }
 
 // ---- ghost syncs -------------------------------------------------------------
@@ -549,7 +644,7 @@ int tree_sync(Tree &T, const SyncSpec &S)
         }
     }
     gen.reset();
-    if (int rc = run_patch_list(T.P, A_U, A_T, A_FILT, st1)) return rc;
+    if (int rc = dev_patches(T, A_U, A_T, A_FILT, std::move(st1))) return rc;
     if (filt && !restrict_senders.empty()) {
         // restrict_copy_at_CE: HD-filtered copy of every restriction sender,
         // original SC points put back near level jumps of leaf senders
@@ -574,11 +669,11 @@ int tree_sync(Tree &T, const SyncSpec &S)
                     boxes.push_back(bx);
                 }
             }
-            if (int rc = run_blocks_op(T.P, WOP_FILTER, arr, A_FILT, ids)) return rc;
-            if (int rc = run_boxes(T.P, A_FILT, arr, boxes)) return rc;
+            if (int rc = dev_blocks(T, WOP_FILTER, arr, A_FILT, std::move(ids))) return rc;
+            if (int rc = dev_boxes(T, A_FILT, arr, std::move(boxes))) return rc;
         }
     }
-    return run_patch_list(T.P, A_U, A_T, A_FILT, st23);
+    return dev_patches(T, A_U, A_T, A_FILT, std::move(st23));
 }
 
 // ---- adapt_tree pieces ----------------------------------------------------------
@@ -631,7 +726,7 @@ int d2m(Tree &T, int level)
         x.stage = 1;
         p.push_back(x);
     }
-    return run_patch_list(T.P, A_U, -1, -1, p);
+    return dev_patches(T, A_U, -1, -1, std::move(p));
 }
 
 std::vector<int32_t> hvy_of(const Tree &T, const std::vector<int> &idx)
@@ -662,7 +757,7 @@ int decompose_full_tree(Tree &T, const wh_adapt_params &p)
         all[i] = i;
         if (T.b[i].level == level || leaf_first) T.b[i].status = -1;
     }
-    if (int rc = run_blocks_op(T.P, WOP_COPY, A_U, A_T, hvy_of(T, all))) return rc;
+    if (int rc = dev_blocks(T, WOP_COPY, A_U, A_T, hvy_of(T, all))) return rc;
     if (int rc = init_full_tree(T, p.jmin)) return rc;
     const int g_this = std::max(std::abs(w.hd_lo), w.gd_hi);
     int it = 0;
@@ -683,9 +778,9 @@ int decompose_full_tree(Tree &T, const wh_adapt_params &p)
         }
         std::vector<int> both(dec);
         both.insert(both.end(), dea.begin(), dea.end());
-        if (int rc = run_blocks_op(T.P, WOP_COPY, A_U, A_T, hvy_of(T, both))) return rc;
-        if (int rc = run_blocks_op(T.P, WOP_FWT, A_T, A_U, hvy_of(T, dec))) return rc;
-        if (int rc = run_blocks_op(T.P, WOP_FILTER, A_T, A_U, hvy_of(T, dea))) return rc;
+        if (int rc = dev_blocks(T, WOP_COPY, A_U, A_T, hvy_of(T, both))) return rc;
+        if (int rc = dev_blocks(T, WOP_FWT, A_T, A_U, hvy_of(T, dec))) return rc;
+        if (int rc = dev_blocks(T, WOP_FILTER, A_T, A_U, hvy_of(T, dea))) return rc;
         if (p.use_ce)
             if (int rc = ce_modify(T, CE_REF, 0, -1, true, true)) return rc;
         for (TNode &x : T.b) {
@@ -830,8 +925,8 @@ int reconstruct_tree(Tree &T, const wh_adapt_params &p)
             all.push_back(i);
             if (T.b[i].status == -1) rec.push_back(i);
         }
-        if (int rc = run_blocks_op(T.P, WOP_IWT, A_U, A_T, hvy_of(T, rec))) return rc;
-        return run_blocks_op(T.P, WOP_COPY, A_T, A_U, hvy_of(T, all));
+        if (int rc = dev_blocks(T, WOP_IWT, A_U, A_T, hvy_of(T, rec))) return rc;
+        return dev_blocks(T, WOP_COPY, A_T, A_U, hvy_of(T, all));
     };
     if (leaf_only) {
         if (int rc = tree_sync(T, S)) return rc;
@@ -944,15 +1039,15 @@ void mask_statuses(Tree &T, const std::vector<int> &idx, const double *st)
 
 // addSecurityZone_CE_tree (securityZone_tree.f90): a block that would coarsen
 // stays if a same-level neighbour that stays has significant details within
-// Nwc of their common boundary
-int security_zone(Tree &T, const wh_adapt_params &p, const double *norm)
+// Nwc of their common boundary.  zone_boxes lists the boxes whose maxima
+// decide it (a block that stays, toward a neighbour that would coarsen);
+// zone_apply applies the rule given their per-box maxima m.
+void zone_boxes(const Tree &T, const wh_adapt_params &p, std::vector<wh_box> &boxes,
+                std::vector<std::pair<int, int>> &who)
 {
     WH_PROF("security_zone");
     int dirs[26][3];
     const int nd = dirs_of(T.dim, dirs);
-    const int neq = T.P->d.neq;
-    std::vector<wh_box> boxes;
-    std::vector<std::pair<int, int>> who;  // (block, direction)
     for (int i = 0; i < (int)T.b.size(); ++i) {
         if (T.b[i].level == p.jmax && p.force_maxlevel_dealiasing) continue;
         if (T.b[i].status != 0) continue;
@@ -966,10 +1061,16 @@ int security_zone(Tree &T, const wh_adapt_params &p, const double *norm)
             who.push_back({i, q});
         }
     }
-    std::vector<double> m;
-    if (int rc = run_region_max(T.P, A_U, boxes, true, true, p.groups, m)) return rc;
+}
+
+void zone_apply(Tree &T, const wh_adapt_params &p, const double *norm, const std::vector<std::pair<int, int>> &who,
+                double *m)
+{
+    int dirs[26][3];
+    const int nd = dirs_of(T.dim, dirs);
+    const int neq = T.P->d.neq;
     std::vector<char> flag(T.b.size() * 26, 0);
-    for (size_t k = 0; k < boxes.size(); ++k) {
+    for (size_t k = 0; k < who.size(); ++k) {
         double *v = &m[k * neq];
         combine_groups(v, neq, p.groups);
         bool all = true;
@@ -991,7 +1092,6 @@ int security_zone(Tree &T, const wh_adapt_params &p, const double *norm)
     }
     for (size_t i = 0; i < T.b.size(); ++i)
         if (keep[i]) T.b[i].status = 0;
-    return WH_OK;
 }
 
 // leaves of the tree back into the forest's block list
@@ -1019,25 +1119,11 @@ int check_pool(wh_pool *P, wh_forest *F)
 }  // namespace
 }  // namespace wh
 
-extern "C" {
-
-int wh_sync_tree(wh_pool *P, wh_forest *F, int array)
-{
-    WH_PROF("api.sync_tree");
-    using namespace wh;
-    if (int rc = check_pool(P, F)) return rc;
-    WH_CHECK(array == WH_BLOCK, WH_E_ARG, "wh_sync_tree: only WH_BLOCK is supported");
-    Tree T(F, P);
-    SyncSpec S;
-    S.gm = S.gp = T.g;
-    return tree_sync(T, S);
-}
-
 namespace wh {
+namespace {
 
-// adapt_tree (adapt_tree.f90:11-255); leaf_flags: the external statuses of
-// the leaves in forest list order (indicator "nothing (external)")
-static int adapt_tree_impl(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_t *leaf_flags)
+// adapt_tree's checks of one forest (adapt_tree.f90:11-255 preconditions)
+int check_adapt(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_t *leaf_flags)
 {
     if (int rc = check_pool(P, F)) return rc;
     WH_CHECK(p && p->jmin >= 0 && p->jmax <= F->jmax && p->jmin <= p->jmax &&
@@ -1047,122 +1133,325 @@ static int adapt_tree_impl(wh_pool *P, wh_forest *F, const wh_adapt_params *p, c
              WH_E_ARG, "wh_adapt_tree: bad parameters");
     for (const FBlock &b : F->blocks)
         WH_CHECK(b.level >= p->jmin, WH_E_ARG, "wh_adapt_tree: a block is below jmin");
+    if (leaf_flags)
+        for (size_t i = 0; i < F->blocks.size(); ++i)
+            WH_CHECK(leaf_flags[i] == -1 || leaf_flags[i] == 0 || leaf_flags[i] == 1, WH_E_ARG,
+                     "wh_adapt_tree_flags: a flag is not -1, 0 or 1");
+    return WH_OK;
+}
+
+bool same_groups(const wh_adapt_params &a, const wh_adapt_params &b, int neq)
+{
+    for (int c = 0; c < neq; ++c)
+        if (a.groups[c] != b.groups[c]) return false;
+    return true;
+}
+
+// One forest of a multi-forest adapt_tree.
+struct TreeRun {
+    wh_forest *F = nullptr;
+    const wh_adapt_params *p = nullptr;
+    std::unique_ptr<Tree> T;
+    std::vector<int> free0;
+    int rc = WH_OK;
+    OpLog log;
+    double norm[8];
+    std::vector<int> idx, midx;
+    std::vector<int32_t> mids;
+    std::vector<wh_box> nboxes, zboxes;
+    std::vector<std::pair<int, int>> zwho;
+    bool alive() const { return rc == WH_OK && T; }
+};
+
+// Region maxima of the listed boxes of several forests: consecutive forests
+// with equal groups in one launch; forest t's maxima at d_out + off(t), the
+// forests' ranges one after the other in forest order.
+int region_max_runs(wh_pool *P, std::vector<TreeRun *> &R, std::vector<wh_box> TreeRun::*boxes, int array,
+                    bool zero_sc, bool sqrt_groups, double *d_out)
+{
+    const int neq = P->d.neq;
+    size_t off = 0;
+    for (size_t a = 0; a < R.size();) {
+        size_t b = a + 1;
+        while (b < R.size() && same_groups(*R[a]->p, *R[b]->p, neq)) ++b;
+        std::vector<wh_box> all;
+        for (size_t t = a; t < b; ++t) all.insert(all.end(), (R[t]->*boxes).begin(), (R[t]->*boxes).end());
+        if (int rc = launch_region_max(P, array, all, zero_sc, sqrt_groups, R[a]->p->groups, d_out + off)) return rc;
+        off += all.size() * neq;
+        a = b;
+    }
+    return WH_OK;
+}
+
+// adapt_tree (adapt_tree.f90:11-255) for n forests of one pool: per forest
+// the reference's sequence of steps, with the device operations of all
+// forests launched together (op logs, replay) and the indicator's reductions
+// of all forests read back in one round trip (one more for the security
+// zone).  flags[i]: forest i's external leaf statuses (indicator "nothing
+// (external)") or null.  status[i]: forest i's result; an error of one
+// forest (its checks, its slots) leaves its block list unchanged and the
+// others go on; the return value is an error only when an operation shared
+// by all forests failed.
+int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_params *ps,
+                     const int32_t *const *flags, int *status)
+{
     const int stay = REF_UNSIGNIFICANT_STAY;
-    Tree T(F, P);
-    if (leaf_flags)  // the leaves are T.b[0, n) in forest list order
-        for (size_t i = 0; i < F->blocks.size(); ++i) {
-            const int f = leaf_flags[i];
-            WH_CHECK(f == -1 || f == 0 || f == 1, WH_E_ARG, "wh_adapt_tree_flags: a flag is not -1, 0 or 1");
-            T.b[i].status = f;
-        }
-    const std::vector<int> free0 = F->free_hvy;
-    auto fail = [&](int rc) {
-        // restore the forest's slots; device data of the leaves is not restored
-        F->free_hvy = free0;
-        (void)stage_sync(P);
-        return rc;
+    const int neq = P->d.neq;
+    std::vector<TreeRun> R(n);
+    auto fail = [&](TreeRun &t, int rc) {
+        if (t.rc != WH_OK) return;
+        if (t.T) t.F->free_hvy = t.free0;  // restore the forest's slots; device data of the leaves is not
+        t.rc = rc;
     };
-    // the decomposition hands the leaves' statuses back (its backup), the
-    // new mothers get 0
-    if (int rc = decompose_full_tree(T, *p)) return fail(rc);
-    // coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331); with
-    // the external indicator it returns at once (:55)
-    if (p->indicator != WH_IND_EXTERNAL)
-        for (TNode &x : T.b) x.status = 0;
-    if (p->indicator == WH_IND_EXTERNAL) {
-    } else if (p->indicator == WH_IND_EVERYWHERE) {
-        for (int i = 0; i < (int)T.b.size(); ++i)
-            if (T.is_leaf(i, false)) T.b[i].status = -1;
-    } else {
-        // the three reductions that need no host decision -- the leaves'
-        // norms, the blocks' details, the mask statistics -- go out together
-        // and come back in one round trip; the host logic below then runs in
-        // the reference's order
-        const int neq = P->d.neq;
-        double norm[8];
-        for (int c = 0; c < 8; ++c) norm[c] = p->norm[c];
-        std::vector<wh_box> nboxes;
-        if (p->eps_normalized) nboxes = leaf_boxes(T);
-        std::vector<int> idx;
-        for (int i = 0; i < (int)T.b.size(); ++i) {
-            if (p->force_maxlevel_dealiasing && T.b[i].level == p->jmax) T.b[i].status = -1;
-            else idx.push_back(i);
+    for (int i = 0; i < n; ++i) {
+        TreeRun &t = R[i];
+        t.F = Fs[i];
+        t.p = &ps[i];
+        const int32_t *lf = flags ? flags[i] : nullptr;
+        if (int rc = check_adapt(P, t.F, t.p, lf)) {
+            t.rc = rc;
+            continue;
         }
-        // the mask criterion lists the same blocks as the details
-        std::vector<int> midx;
-        std::vector<int32_t> mids;
-        const size_t o_det = nboxes.size() * neq, o_mask = o_det + idx.size() * neq;
-        const size_t n_res = o_mask + (p->threshold_mask ? idx.size() * 3 : 0);
+        t.T.reset(new Tree(t.F, P));
+        if (lf)  // the leaves are T.b[0, n) in forest list order
+            for (size_t k = 0; k < t.F->blocks.size(); ++k) t.T->b[k].status = lf[k];
+        t.free0 = t.F->free_hvy;
+        // one forest launches as it goes (its host work overlaps the device);
+        // several log their operations and launch them together
+        if (n > 1) t.T->log = &t.log;
+    }
+    int shared = WH_OK;
+    auto live = [&]() {
+        std::vector<TreeRun *> v;
+        for (TreeRun &t : R)
+            if (t.alive()) v.push_back(&t);
+        return v;
+    };
+    auto run_logs = [&]() {
+        std::vector<OpLog *> logs;
+        for (TreeRun *t : live()) logs.push_back(&t->log);
+        if (int rc = replay(P, logs)) {
+            for (TreeRun *t : live()) fail(*t, rc);
+            shared = rc;
+        }
+    };
+    // wavelet_decompose_full_tree; it hands the leaves' statuses back (its
+    // backup), the new mothers get 0
+    for (TreeRun *t : live())
+        if (int rc = decompose_full_tree(*t->T, *t->p)) fail(*t, rc);
+    run_logs();
+    // coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331); with the
+    // external indicator it returns at once (:55).  The reductions that need
+    // no host decision (leaf norms, details, mask statistics) of all forests
+    // go out together: results [norms of all][details of all][mask of all]
+    std::vector<TreeRun *> thr;
+    for (TreeRun *t : live()) {
+        Tree &T = *t->T;
+        const wh_adapt_params &p = *t->p;
+        if (p.indicator != WH_IND_EXTERNAL)
+            for (TNode &x : T.b) x.status = 0;
+        if (p.indicator == WH_IND_EVERYWHERE) {
+            for (int i = 0; i < (int)T.b.size(); ++i)
+                if (T.is_leaf(i, false)) T.b[i].status = -1;
+        } else if (p.indicator == WH_IND_THRESHOLD) {
+            for (int c = 0; c < 8; ++c) t->norm[c] = p.norm[c];
+            if (p.eps_normalized) t->nboxes = leaf_boxes(T);
+            for (int i = 0; i < (int)T.b.size(); ++i) {
+                if (p.force_maxlevel_dealiasing && T.b[i].level == p.jmax) T.b[i].status = -1;
+                else t->idx.push_back(i);
+            }
+            thr.push_back(t);
+        }
+    }
+    if (!thr.empty() && !shared) {
+        WH_PROF("indicator_reductions");
+        size_t n_norm = 0, n_det = 0, n_mask = 0;
+        for (TreeRun *t : thr) {
+            n_norm += t->nboxes.size() * neq;
+            n_det += t->idx.size() * neq;
+            if (t->p->threshold_mask) n_mask += t->idx.size() * 3;
+        }
         double *d_res = nullptr;
         std::vector<double> res;
-        {
-            WH_PROF("indicator_reductions");
-            if (int rc = results_reserve(P, n_res, &d_res)) return fail(rc);
-            if (int rc = launch_region_max(P, A_T, nboxes, false, false, p->groups, d_res)) return fail(rc);
-            if (int rc = launch_detail_list(P, A_U, hvy_of(T, idx), p->groups, d_res + o_det)) return fail(rc);
-            if (p->threshold_mask) {
-                if (int rc = mask_blocks(T, *p, midx, mids)) return fail(rc);
-                if (mids.size() != idx.size()) {
-                    set_error("adapt: mask and detail block lists differ");
-                    return fail(WH_E_STATE);
-                }
-                if (int rc = launch_mask_stats(P, mids, d_res + o_mask)) return fail(rc);
+        int rc = results_reserve(P, n_norm + n_det + n_mask, &d_res);
+        if (!rc) rc = region_max_runs(P, thr, &TreeRun::nboxes, A_T, false, false, d_res);
+        size_t off = n_norm;
+        for (size_t a = 0; a < thr.size() && !rc;) {  // details: consecutive forests with equal groups
+            size_t b = a + 1;
+            while (b < thr.size() && same_groups(*thr[a]->p, *thr[b]->p, neq)) ++b;
+            std::vector<int32_t> ids;
+            for (size_t t = a; t < b; ++t) {
+                const std::vector<int32_t> h = hvy_of(*thr[t]->T, thr[t]->idx);
+                ids.insert(ids.end(), h.begin(), h.end());
             }
-            if (int rc = results_fetch(P, n_res, res)) return fail(rc);
+            rc = launch_detail_list(P, A_U, ids, thr[a]->p->groups, d_res + off);
+            off += ids.size() * neq;
+            a = b;
         }
-        if (p->eps_normalized) leaf_linfty(nboxes.size(), neq, p->groups, res.data(), norm);
-        double *det = res.data() + o_det;
-        detail_groups(P, p->groups, det, (int)idx.size());
-        for (size_t k = 0; k < idx.size(); ++k) {
-            bool all = true;
-            for (int c = 0; c < neq; ++c)
-                if (!(det[k * neq + c] <= p->eps * norm[c])) all = false;
-            T.b[idx[k]].status = all ? -1 : 0;
+        std::vector<int32_t> mids;
+        for (TreeRun *t : thr) {  // the masks: one launch per forest (its own geometry)
+            if (rc || !t->p->threshold_mask) continue;
+            rc = mask_blocks(*t->T, *t->p, t->midx, t->mids);
+            if (!rc && t->mids.size() != t->idx.size()) {
+                set_error("adapt: mask and detail block lists differ");
+                rc = WH_E_STATE;
+            }
+            mids.insert(mids.end(), t->mids.begin(), t->mids.end());
         }
-        if (p->threshold_mask) mask_statuses(T, midx, res.data() + o_mask);
-        if (p->security_zone)
-            if (int rc = security_zone(T, *p, norm)) return fail(rc);
+        if (!rc && !mids.empty()) rc = launch_mask_stats(P, mids, d_res + n_norm + n_det);
+        if (!rc) rc = results_fetch(P, n_norm + n_det + n_mask, res);
+        if (rc) {
+            for (TreeRun *t : thr) fail(*t, rc);
+            shared = rc;
+        } else {
+            size_t o_norm = 0, o_det = n_norm, o_mask = n_norm + n_det;
+            for (TreeRun *t : thr) {
+                Tree &T = *t->T;
+                const wh_adapt_params &p = *t->p;
+                if (p.eps_normalized) leaf_linfty(t->nboxes.size(), neq, p.groups, res.data() + o_norm, t->norm);
+                o_norm += t->nboxes.size() * neq;
+                double *det = res.data() + o_det;
+                detail_groups(P, p.groups, det, (int)t->idx.size());
+                for (size_t k = 0; k < t->idx.size(); ++k) {
+                    bool all = true;
+                    for (int c = 0; c < neq; ++c)
+                        if (!(det[k * neq + c] <= p.eps * t->norm[c])) all = false;
+                    T.b[t->idx[k]].status = all ? -1 : 0;
+                }
+                o_det += t->idx.size() * neq;
+                if (p.threshold_mask) {
+                    mask_statuses(T, t->midx, res.data() + o_mask);
+                    o_mask += t->idx.size() * 3;
+                }
+            }
+        }
     }
-    // respectJmaxJmin_tree (respectJmaxJmin_tree.f90:1-39)
-    for (TNode &x : T.b) {
-        if (x.status == 1 && x.level >= p->jmax) x.status = stay;
-        if (x.status == -1 && x.level <= p->jmin) x.status = stay;
+    // the security zone of all forests in one more round trip
+    std::vector<TreeRun *> zone;
+    for (TreeRun *t : thr)
+        if (t->alive() && t->p->security_zone) {
+            zone_boxes(*t->T, *t->p, t->zboxes, t->zwho);
+            zone.push_back(t);
+        }
+    if (!zone.empty() && !shared) {
+        size_t nz = 0;
+        for (TreeRun *t : zone) nz += t->zboxes.size() * neq;
+        double *d_res = nullptr;
+        std::vector<double> res;
+        int rc = results_reserve(P, nz, &d_res);
+        if (!rc) rc = region_max_runs(P, zone, &TreeRun::zboxes, A_U, true, true, d_res);
+        if (!rc) rc = results_fetch(P, nz, res);
+        if (rc) {
+            for (TreeRun *t : zone) fail(*t, rc);
+            shared = rc;
+        } else {
+            size_t off = 0;
+            for (TreeRun *t : zone) {
+                zone_apply(*t->T, *t->p, t->norm, t->zwho, res.data() + off);
+                off += t->zboxes.size() * neq;
+            }
+        }
     }
-    ensure_gradedness(T, stay);
-    {
+    for (TreeRun *t : live()) {
+        Tree &T = *t->T;
+        const wh_adapt_params &p = *t->p;
+        // respectJmaxJmin_tree (respectJmaxJmin_tree.f90:1-39)
+        for (TNode &x : T.b) {
+            if (x.status == 1 && x.level >= p.jmax) x.status = stay;
+            if (x.status == -1 && x.level <= p.jmin) x.status = stay;
+        }
+        ensure_gradedness(T, stay);
         std::vector<char> del(T.b.size(), 0);
         for (size_t i = 0; i < T.b.size(); ++i) del[i] = T.b[i].status == -1;
         T.remove_marked(del);
+        if (p.use_ce) {
+            int rc = ce_modify(T, CE_TREE, 0, 0, true, false);
+            if (!rc) rc = reconstruct_tree(T, p);
+            if (rc) {
+                fail(*t, rc);
+                t->log.ops.clear();
+            }
+        } else {
+            std::vector<int> all(T.b.size());
+            for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+            if (int rc = dev_blocks(T, WOP_COPY, A_T, A_U, hvy_of(T, all))) fail(*t, rc);
+        }
     }
-    if (p->use_ce) {
-        if (int rc = ce_modify(T, CE_TREE, 0, 0, true, false)) return fail(rc);
-        if (int rc = reconstruct_tree(T, *p)) return fail(rc);
-    } else {
-        std::vector<int> all(T.b.size());
-        for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
-        if (int rc = run_blocks_op(P, WOP_COPY, A_T, A_U, hvy_of(T, all))) return fail(rc);
-    }
-    // prune_fulltree2leafs
-    {
+    run_logs();
+    for (TreeRun *t : live()) {
+        Tree &T = *t->T;
+        // prune_fulltree2leafs
         std::vector<char> del(T.b.size(), 0);
         for (size_t i = 0; i < T.b.size(); ++i) del[i] = !T.is_leaf((int)i, false);
         T.remove_marked(del);
+        for (TNode &x : T.b) x.status = 0;
+        store_leaves(T);
+        SyncSpec S;
+        S.gm = S.gp = T.g;
+        if (int rc = tree_sync(T, S)) fail(*t, rc);
     }
-    for (TNode &x : T.b) x.status = 0;
-    store_leaves(T);
-    SyncSpec S;
-    S.gm = S.gp = T.g;
-    return tree_sync(T, S);
+    run_logs();
+    bool any_failed = false;
+    for (int i = 0; i < n; ++i) {
+        status[i] = R[i].rc;
+        if (R[i].rc) any_failed = true;
+    }
+    if (any_failed) (void)stage_sync(P);
+    return shared;
 }
 
+// sync_ghosts_tree of n forests of one pool, their ghost patches in one
+// launch per operation
+int sync_trees_impl(wh_pool *P, wh_forest *const *Fs, int n)
+{
+    std::vector<std::unique_ptr<Tree>> T;
+    std::vector<OpLog> logs(n);
+    std::vector<OpLog *> lp;
+    for (int i = 0; i < n; ++i) {
+        if (int rc = check_pool(P, Fs[i])) return rc;
+        T.emplace_back(new Tree(Fs[i], P));
+        if (n > 1) T.back()->log = &logs[i];
+        SyncSpec S;
+        S.gm = S.gp = T.back()->g;
+        if (int rc = tree_sync(*T.back(), S)) return rc;
+        lp.push_back(&logs[i]);
+    }
+    return replay(P, lp);
+}
+
+}  // namespace
 }  // namespace wh
+
+extern "C" {
+
+int wh_sync_tree(wh_pool *P, wh_forest *F, int array)
+{
+    WH_PROF("api.sync_tree");
+    WH_CHECK(P && F, WH_E_ARG, "wh_sync_tree: null argument");
+    WH_CHECK(array == WH_BLOCK, WH_E_ARG, "wh_sync_tree: only WH_BLOCK is supported");
+    return wh::sync_trees_impl(P, &F, 1);
+}
+
+int wh_sync_trees(wh_pool *P, wh_forest *const *forests, int n, int array)
+{
+    WH_PROF("api.sync_tree");
+    WH_CHECK(P && n >= 0 && (n == 0 || forests), WH_E_ARG, "wh_sync_trees: null argument");
+    WH_CHECK(array == WH_BLOCK, WH_E_ARG, "wh_sync_trees: only WH_BLOCK is supported");
+    for (int i = 0; i < n; ++i) {
+        WH_CHECK(forests[i], WH_E_ARG, "wh_sync_trees: null forest");
+        for (int j = 0; j < i; ++j) WH_CHECK(forests[j] != forests[i], WH_E_ARG, "wh_sync_trees: a forest listed twice");
+    }
+    return wh::sync_trees_impl(P, forests, n);
+}
 
 int wh_adapt_tree(wh_pool *P, wh_forest *F, const wh_adapt_params *p)
 {
     WH_PROF("api.adapt_tree");
-    WH_CHECK(p && p->indicator != WH_IND_EXTERNAL, WH_E_ARG,
+    WH_CHECK(P && F && p && p->indicator != WH_IND_EXTERNAL, WH_E_ARG,
              "wh_adapt_tree: the external indicator needs wh_adapt_tree_flags");
-    return wh::adapt_tree_impl(P, F, p, nullptr);
+    int st = WH_OK;
+    const int rc = wh::adapt_trees_impl(P, &F, 1, p, nullptr, &st);
+    return rc ? rc : st;
 }
 
 int wh_adapt_tree_flags(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_t *leaf_flags, int n)
@@ -1170,7 +1459,21 @@ int wh_adapt_tree_flags(wh_pool *P, wh_forest *F, const wh_adapt_params *p, cons
     WH_PROF("api.adapt_tree");
     WH_CHECK(P && F && p && p->indicator == WH_IND_EXTERNAL && leaf_flags && n == (int)F->blocks.size(),
              WH_E_ARG, "wh_adapt_tree_flags: needs WH_IND_EXTERNAL and one flag per leaf");
-    return wh::adapt_tree_impl(P, F, p, leaf_flags);
+    int st = WH_OK;
+    const int rc = wh::adapt_trees_impl(P, &F, 1, p, &leaf_flags, &st);
+    return rc ? rc : st;
 }
 
-}  // extern "C""
+int wh_adapt_trees(wh_pool *P, wh_forest *const *forests, int n, const wh_adapt_params *params, int *status)
+{
+    WH_PROF("api.adapt_trees");
+    WH_CHECK(P && n >= 0 && (n == 0 || (forests && params && status)), WH_E_ARG, "wh_adapt_trees: null argument");
+    for (int i = 0; i < n; ++i) {
+        WH_CHECK(forests[i] && params[i].indicator != WH_IND_EXTERNAL, WH_E_ARG,
+                 "wh_adapt_trees: null forest or external indicator");
+        for (int j = 0; j < i; ++j) WH_CHECK(forests[j] != forests[i], WH_E_ARG, "wh_adapt_trees: a forest listed twice");
+    }
+    return wh::adapt_trees_impl(P, forests, n, params, nullptr, status);
+}
+
+}  // extern "C"

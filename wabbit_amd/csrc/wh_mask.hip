@@ -251,9 +251,9 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
 
 int launch_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, double *d_out)
 {
-    if (int rc = check_mask_pool(P)) return rc;
     const int n = (int)ids.size();
     if (n == 0) return WH_OK;
+    if (int rc = check_mask_pool(P)) return rc;
     for (int32_t h : ids) WH_CHECK(h >= 0 && h < P->d.nb_max, WH_E_ARG, "mask stats: hvy id out of range");
     if (int rc = stage_begin(P, sizeof(int32_t) * n)) return rc;
     const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n);

@@ -128,6 +128,15 @@ class Forest:
         return tc, lv, hv, x
 
 
+def refine_blocks(pool, triples, order, array=L.WH_BLOCK):
+    """refineBlock on the device for (mother, daughter, digit) triples of
+    Forest.refine (of one or several forests sharing the pool)."""
+    _bind()
+    t = np.ascontiguousarray(triples, np.int32).reshape(-1, 3)
+    if t.shape[0]:
+        L.check(L.load().wh_refine_blocks(pool.h, array, t.ctypes.data, t.shape[0], order))
+
+
 def refine_on_device(pool, forest, hvy_ids, order, g_rhs, array=L.WH_BLOCK, presync="table", attach=True,
                      full=True):
     """refine_tree's execution step for the given blocks: sync the full ghost

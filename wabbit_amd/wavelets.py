@@ -114,6 +114,8 @@ def _bind():
         "wh_fwt_threshold": [_vp, ip, ip, _vp, ip, _vp, _vp, _vp, _vp],
         "wh_fwt_threshold_noalloc": [_vp, ip, ip, _vp, ip, _vp, _vp, _vp, _vp],
         "wh_sync_tree": [_vp, _vp, ip],
+        "wh_sync_trees": [_vp, _vp, ip, ip],
+        "wh_adapt_trees": [_vp, _vp, ip, _vp, _vp],
         "wh_create_mask": [_vp, ctypes.POINTER(MaskParamsC), _vp, ip, _vp],
     }
     for name, args in sig.items():
@@ -259,3 +261,32 @@ def sync_tree(pool, forest, array=L.WH_BLOCK):
     """sync_ghosts_tree with filtered restriction for lifted wavelets."""
     _bind()
     L.check(L.load().wh_sync_tree(pool.h, forest.h, array))
+
+
+def sync_trees(pool, forests, array=L.WH_BLOCK):
+    """sync_tree of several forests sharing the pool, one launch per
+    operation for all of them."""
+    _bind()
+    if not forests:
+        return
+    arr = (ctypes.c_void_p * len(forests))(*[f.h.value for f in forests])
+    L.check(L.load().wh_sync_trees(pool.h, ctypes.cast(arr, ctypes.c_void_p), len(forests), array))
+
+
+def adapt_trees(pool, forests, params):
+    """adapt_tree of several forests sharing the pool (params: one
+    AdaptParamsC per forest), their device operations launched together.
+    Returns one (code, message) per forest: code 0, or the error that forest
+    alone hit (its block list is then unchanged).  An error of an operation
+    shared by all forests raises."""
+    _bind()
+    n = len(forests)
+    if n == 0:
+        return []
+    arr = (ctypes.c_void_p * n)(*[f.h.value for f in forests])
+    prm = (AdaptParamsC * n)(*params)
+    st = (ctypes.c_int * n)()
+    L.check(L.load().wh_adapt_trees(pool.h, ctypes.cast(arr, ctypes.c_void_p), n, ctypes.cast(prm, ctypes.c_void_p),
+                                    ctypes.cast(st, ctypes.c_void_p)))
+    msg = L.load().wh_last_error().decode(errors="replace") if any(st) else ""
+    return [(int(c), msg if c else "") for c in st]
