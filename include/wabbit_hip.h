@@ -249,6 +249,15 @@ int wh_batch_rk_step(wh_batch *batch, const wh_acm_params *p, const double *butc
 int wh_batch_time_stage_kernel(wh_batch *batch, const wh_acm_params *p, const double *butcher, int nrows,
                                int reps, double *ms, int *blocks);
 int wh_batch_synchronize(wh_batch *batch);
+/* The batch's 2D stage kernel (no counterpart in the reference):
+ * WH_TUNE_K2D_STAGE 0 = one block per workgroup at a time, the first
+ * point's operands loaded before the tile barrier; 1 = software pipelined
+ * (the next block's metadata and halo tile in flight while a block is
+ * computed; used where every tile is at most 32 x 32, i.e. Bs <= 28 with
+ * FD4); 2 = 0 without the early loads; 3 / 4 = 2 with its registers bounded
+ * for 6 / 8 waves per SIMD.  All give the same results bit for bit. */
+int wh_batch_set_tuning(wh_batch *batch, int key, int value);
+int wh_batch_get_tuning(wh_batch *batch, int key, int *value);
 
 /* refineBlock (LIB/MESH/refinementExecute.f90:4-116) for a list of
  * (mother, daughter, digit) triples: daughter `digit` of a mother takes its

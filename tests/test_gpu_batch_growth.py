@@ -6,7 +6,8 @@ the sequence 1024 -> 1104 -> 1552 blocks grows them once and then needs
 more origin/spacing room than the first growth gave (6 doubles per entry)
 while the entry table still fits.  Every step must equal the one-pool
 grouped step (wh_rk_step_groups, fused path) on an identical pool, bit for
-bit, with and without graph replay.
+bit, with and without graph replay and for every variant of the batch's 2D
+stage kernel.
 """
 import numpy as np
 import pytest
@@ -39,8 +40,8 @@ def _tables(levels):
             np.concatenate(pts), len(levels))
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_step_with_growing_tables(use_graph):
+@pytest.mark.parametrize("use_graph,variant", [(False, 0), (True, 0), (True, 1), (True, 2), (True, 3), (True, 4)])
+def test_batch_step_with_growing_tables(use_graph, variant):
     acm = ACMParams(dim=2, scheme=4, skew_symmetry=False, penalization=False, c_0=20.0, nu=1e-3,
                     gamma_p=1.0, CFL=0.5, domain_size=[1.0, 1.0, 1.0])
     acm_c = L.acm_params_c(acm)
@@ -49,6 +50,7 @@ def test_batch_step_with_growing_tables(use_graph):
     a = L.Pool(2, [BS, BS], G, 3, NB, n_work_slots=3)
     b = L.Pool(2, [BS, BS], G, 3, NB, n_work_slots=3)
     st = L.BatchStepper([a])
+    st.set_tuning(L.WH_TUNE_K2D_STAGE, variant)   # every 2D stage-kernel variant: the same bits
     try:
         data = rng.standard_normal((NB,) + a.block_shape()) * 0.1
         a.upload(L.WH_BLOCK, 0, data)

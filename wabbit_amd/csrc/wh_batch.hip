@@ -54,6 +54,13 @@ struct wh_batch {
     int nchunk = 1, grid_stage = 0, grid_sync = 0;
     int n_ent_last = 0;
     int sout_sel = 0;  // the last stage's output array (0: the state; 3 while timing)
+    // 2D stage kernel (WH_TUNE_K2D_STAGE): 0 one block per workgroup at a
+    // time with the first point's operands loaded before the tile barrier;
+    // 1 software-pipelined (used where every tile is at most 32 x 32); 2 as
+    // 0 without the early loads; 3 / 4 as 2 with registers for 6 / 8 waves
+    // per SIMD
+    int k2d = 0;
+    bool pipe_ok = false;  // the last step's tiles fit the pipelined kernel
 };
 
 namespace {
@@ -79,6 +86,15 @@ void free_graph(wh_batch *B)
     B->graph = nullptr;
 }
 
+bool pipe_on(const wh_batch *B) { return B->dim == 2 && B->k2d == 1 && B->pipe_ok; }
+
+// the launcher's kernel choice (0: the default kernel)
+int k2d_sel(const wh_batch *B) { return B->dim != 2 ? 0 : B->k2d == 1 ? (pipe_on(B) ? 1 : 0) : B->k2d; }
+
+// persistent stage-kernel grids, multiples of 8 (one per XCD); the pipelined
+// kernel holds more registers, so its grid stays resident at 4 per CU
+int stage_grid(const wh_batch *B) { return B->dim == 2 ? (pipe_on(B) ? 4 : 8) * B->n_cu : 16 * B->n_cu; }
+
 // the RK stage sequence of the fused path (runge_kutta_generic.f90:63-154 with
 // the stage-j RHS and its accumulation fused, as wh_rk_step path 2)
 int launch_sequence(wh_batch *B, const wh_acm_params &p, const double *bt, int nrows)
@@ -102,7 +118,8 @@ int launch_sequence(wh_batch *B, const wh_acm_params &p, const double *bt, int n
         ms.use_b = std::abs(b) >= 1.0e-8;
         ms.sout_sel = mode == 2 ? B->sout_sel : 0;
         return wh::launch_stage_multi(B->stream, B->dim, p.scheme, p.skew_symmetry ? 1 : 0, 1, B->d_pools, np,
-                                      B->d_dtg, B->d_gacm, B->d_tot, ms, B->lds, B->grid_stage, B->nchunk);
+                                      B->d_dtg, B->d_gacm, B->d_tot, ms, B->lds, B->grid_stage, B->nchunk,
+                                      k2d_sel(B));
     };
     // arrays: 0 state, 1 accumulator (work 0), 2 / 3 ping-pong stage inputs
     if (int rc = sync(0)) return rc;
@@ -237,6 +254,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     const int R = p->scheme == WH_FD2 ? 1 : p->scheme == WH_FD4 ? 2 : 3;
     int goff = 0, nent = 0;
     int64_t tot[4] = {0, 0, 0, 0};
+    bool pipe_ok = true;
     for (int i = 0; i < npools; ++i) {
         wh_pool *P = B->pools[i];
         WH_CHECK(P->nact == (int)P->h_active.size() && (P->nact == 0 || P->ngrp > 0), WH_E_STATE,
@@ -282,6 +300,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
             else m.p1[st - 1] = a;  // a pool without blocks fills no ghosts
         }
         if (P->nact > 0) {
+            if (std::max(P->d.bs[0], P->d.bs[1]) + 2 * R > 32) pipe_ok = false;
             if (B->dim == 2)
                 lds = std::max(lds, sizeof(double) * 3 * (P->d.bs[0] + 2 * R) * (P->d.bs[1] + 2 * R));
             else
@@ -340,10 +359,11 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     if (rc) return rc;
     WH_HIP(hipEventRecord(B->ev_up, B->stream));
     B->up_recorded = true;
-    // launch geometry: persistent grids, multiples of 8 (one per XCD)
+    // launch geometry
     B->lds = lds;
     B->nchunk = nchunk;
-    B->grid_stage = B->dim == 2 ? 8 * B->n_cu : 16 * B->n_cu;
+    B->pipe_ok = pipe_ok;
+    B->grid_stage = stage_grid(B);
     B->grid_sync = 4 * B->n_cu;
     B->n_ent_last = nent;
     if (use_graph) {
@@ -353,6 +373,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
         key.push_back((double)lds);
         key.push_back(nchunk);
         key.push_back(npools);
+        key.push_back(k2d_sel(B));
         if (!B->gexec || B->graph_version != ver || key != B->graph_key) {
             free_graph(B);
             WH_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
@@ -404,8 +425,8 @@ int wh_batch_time_stage_kernel(wh_batch *B, const wh_acm_params *p, const double
         m.use_a = std::abs(a) >= 1.0e-8;
         m.use_b = std::abs(b) >= 1.0e-8;
         return wh::launch_stage_multi(B->stream, B->dim, p->scheme, p->skew_symmetry ? 1 : 0, 1, B->d_pools,
-                                      (int)B->pools.size(), B->d_dtg, B->d_gacm, B->d_tot, m, B->lds, B->grid_stage,
-                                      B->nchunk);
+                                      (int)B->pools.size(), B->d_dtg, B->d_gacm, B->d_tot, m, B->lds,
+                                      stage_grid(B), B->nchunk, k2d_sel(B));
     };
     int rc = WH_OK;
     WH_HIP(hipEventRecord(e0, B->stream));
@@ -429,6 +450,22 @@ int wh_batch_time_stage_kernel(wh_batch *B, const wh_acm_params *p, const double
     if (rc) return rc;
     *ms = (double)t / (reps * s);
     if (blocks) *blocks = B->n_ent_last;
+    return WH_OK;
+}
+
+int wh_batch_set_tuning(wh_batch *B, int key, int value)
+{
+    WH_CHECK(B, WH_E_ARG, "wh_batch_set_tuning: null argument");
+    WH_CHECK(key == WH_TUNE_K2D_STAGE && value >= 0 && value <= 4, WH_E_ARG,
+             "wh_batch_set_tuning: WH_TUNE_K2D_STAGE takes 0 to 4");
+    B->k2d = value;
+    return WH_OK;
+}
+
+int wh_batch_get_tuning(wh_batch *B, int key, int *value)
+{
+    WH_CHECK(B && value && key == WH_TUNE_K2D_STAGE, WH_E_ARG, "wh_batch_get_tuning: bad argument");
+    *value = B->k2d;
     return WH_OK;
 }
 

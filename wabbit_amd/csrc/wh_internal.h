@@ -293,10 +293,11 @@ AcmDev make_acm_dev(const wh_acm_params *p);
 Geo make_geo(const wh_pool *P);
 // multi-pool launches (independent forests of a batch, wh_batch.hip)
 // d_tot: [0] entries of the launch, [s] slots of sync stage s (1..3); dtg /
-// gacm: dt and constants per block group of the launch
+// gacm: dt and constants per block group of the launch; pipe: the 2D
+// software-pipelined kernel (tile widths and heights <= 32)
 int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
                        int npools, const double *d_dtg, const AcmDev *d_gacm, const int64_t *d_tot,
-                       const MStage &ms, size_t lds, int grid, int nchunk);
+                       const MStage &ms, size_t lds, int grid, int nchunk, int pipe = 0);
 int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, int sel, int stage,
                       const int64_t *d_tot, int grid);
 int reach_of(int scheme);

@@ -231,16 +231,17 @@ __device__ __forceinline__ void stage_epilogue_pre(const StageArgs &sa, int k, s
 // rhs2d_block is the work of one workgroup on block b (active position k):
 // shared by the one-pool kernel and the multi-pool kernel.  mk: the mask of
 // block b or null.
-template <int S, bool SKEW, bool FUSED, bool PRE>
+// LOADED: the caller has already put the tile in LDS (and synchronised).
+template <int S, bool SKEW, bool FUSED, bool PRE, bool LOADED = false>
 __device__ __forceinline__ void rhs2d_block(double *tile, const double *__restrict__ in, double *__restrict__ out,
                                             const double *__restrict__ mk, int b, int k,
                                             const double *__restrict__ x0dx, const Geo &G, const AcmDev &a,
                                             const StageArgs &sa)
 {
+    static_assert(!(PRE && LOADED), "the operand prefetch belongs to the tile load");
     constexpr int R = Reach<S>::R;
     const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R, tcs = tx * ty;
     const int npts = G.bs0 * G.bs1;
-    const double *u = in + (size_t)b * G.blk;
     const size_t base = (size_t)b * G.blk;
     const int p0 = threadIdx.x;
     double mpre[6] = {0, 0, 0, 0, 0, 0}, apre[3] = {0, 0, 0}, upre[3] = {0, 0, 0};
@@ -263,12 +264,16 @@ __device__ __forceinline__ void rhs2d_block(double *tile, const double *__restri
             }
         }
     }
-    for (int i = threadIdx.x; i < 3 * tcs; i += blockDim.x) {
-        const int c = i / tcs, rr = i - c * tcs;
-        const int iy = rr / tx, ix = rr - iy * tx;
-        tile[i] = u[c * G.cs + (size_t)(G.g - R + iy) * G.nx + (G.g - R + ix)];
+    if constexpr (!LOADED) {
+        const double *u = in + (size_t)b * G.blk;
+        for (int i = threadIdx.x; i < 3 * tcs; i += blockDim.x) {
+            const int c = i / tcs, rr = i - c * tcs;
+            const int iy = rr / tx, ix = rr - iy * tx;
+            tile[i] = u[c * G.cs + (size_t)(G.g - R + iy) * G.nx + (G.g - R + ix)];
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    (void)ty;
     const double dx0 = x0dx[6 * k + 3], dx1 = x0dx[6 * k + 4];
     const double inv[2] = {1.0 / dx0, 1.0 / dx1};
     const double inv2[2] = {1.0 / (dx0 * dx0), 1.0 / (dx1 * dx1)};
@@ -349,26 +354,101 @@ __device__ __forceinline__ StageArgs multi_stage_args(const MPoolDev &P, const M
     return sa;
 }
 
-template <int S, bool SKEW>
-__global__ __launch_bounds__(256) void k_rhs2d_multi(const MPoolDev *__restrict__ pools, int npools,
-                                                     const double *__restrict__ dtg,
-                                                     const AcmDev *__restrict__ gacm,
-                                                     const int64_t *__restrict__ tot, MStage st)
+// PRE: the first point's mask and operands loaded before the tile barrier;
+// WPE: minimum waves per SIMD the register allocation must allow.
+template <int S, bool SKEW, bool PRE, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rhs2d_multi(
+    const MPoolDev *__restrict__ pools, int npools, const double *__restrict__ dtg, const AcmDev *__restrict__ gacm,
+    const int64_t *__restrict__ tot, MStage st)
 {
     extern __shared__ double tile[];
     const int nent = (int)tot[0];
     for (int k = blockIdx.x; k < nent; k += gridDim.x) {
-        const MPoolDev &P = pools[pool_of(pools, npools, k, -1)];
+        // the block's indices are the same in every lane (readfirstlane), so
+        // the descriptor and the constants are read through scalar loads
+        const MPoolDev &P = pools[__builtin_amdgcn_readfirstlane(pool_of(pools, npools, k, -1))];
         const int j = k - P.ent0;
-        const Geo G = P.G;
-        const int b = P.act[j];
-        const int g = P.goff + P.grp[j];
-        const AcmDev a = gacm[g];
+        const Geo &G = P.G;
+        const int b = __builtin_amdgcn_readfirstlane(P.act[j]);
+        const int g = __builtin_amdgcn_readfirstlane(P.goff + P.grp[j]);
+        const AcmDev &a = gacm[g];
         const StageArgs sa = multi_stage_args(P, st, dtg + g);
         const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
-        rhs2d_block<S, SKEW, true, true>(tile, P.arr[st.in_sel], nullptr, mk, b, 0, P.x0dx + 6 * (size_t)j, G, a,
-                                         sa);
+        rhs2d_block<S, SKEW, true, PRE>(tile, P.arr[st.in_sel], nullptr, mk, b, 0, P.x0dx + 6 * (size_t)j, G, a,
+                                        sa);
         __syncthreads();  // the tile is reused by the next entry
+    }
+}
+
+// 2D, several pools, software-pipelined: while block k is computed from
+// the LDS tile, the workgroup already holds the next entry's metadata and
+// has its halo tile in flight into registers; after the compute the
+// registers go to LDS.  The tile is loaded by rows: lane (col, row0) =
+// (t % 32, t / 32) takes column col of rows row0, row0 + 8, ... of every
+// component, so a wavefront reads two contiguous row segments and no index
+// is divided.  Needs tile widths <= 32 and heights <= 8 * MP_ROWS (Bs up to
+// 28 with FD4); per block the arithmetic is that of k_rhs2d_multi.
+constexpr int MP_ROWS = 4;
+template <int S, bool SKEW>
+__global__ __launch_bounds__(256) void k_rhs2d_multi_pipe(const MPoolDev *__restrict__ pools, int npools,
+                                                          const double *__restrict__ dtg,
+                                                          const AcmDev *__restrict__ gacm,
+                                                          const int64_t *__restrict__ tot, MStage st)
+{
+    constexpr int R = Reach<S>::R;
+    extern __shared__ double tile[];
+    const int nent = (int)tot[0];
+    int k = blockIdx.x;
+    if (k >= nent) return;  // uniform over the workgroup
+    const int col = threadIdx.x & 31, row0 = threadIdx.x >> 5;
+    double reg[3][MP_ROWS];
+    int cp, cj, cb;  // the current entry: pool, active position, hvy id
+    auto fetch = [&](int kk, int &p, int &j, int &b) {
+        p = pool_of(pools, npools, kk, -1);
+        const MPoolDev &P = pools[p];
+        j = kk - P.ent0;
+        b = P.act[j];
+        const Geo &G = P.G;
+        const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R;
+        const double *u = P.arr[st.in_sel] + (size_t)b * G.blk + (size_t)(G.g - R) * G.nx + (G.g - R);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < MP_ROWS; ++r) {
+                const int row = row0 + 8 * r;
+                if (col < tx && row < ty) reg[c][r] = u[c * G.cs + (size_t)row * G.nx + col];
+            }
+    };
+    fetch(k, cp, cj, cb);
+    while (true) {
+        __syncthreads();  // the previous block is done with the tile
+        {
+            const Geo &G = pools[cp].G;
+            const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R, tcs = tx * ty;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < MP_ROWS; ++r) {
+                    const int row = row0 + 8 * r;
+                    if (col < tx && row < ty) tile[c * tcs + row * tx + col] = reg[c][r];
+                }
+        }
+        __syncthreads();
+        const int kn = k + (int)gridDim.x;
+        int np = 0, nj = 0, nb = 0;
+        if (kn < nent) fetch(kn, np, nj, nb);  // in flight while block k is computed
+        const MPoolDev &P = pools[cp];
+        const Geo G = P.G;
+        const int g = P.goff + P.grp[cj];
+        const AcmDev a = gacm[g];
+        const StageArgs sa = multi_stage_args(P, st, dtg + g);
+        const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)cb * G.mblk : 0) : nullptr;
+        rhs2d_block<S, SKEW, true, false, true>(tile, nullptr, nullptr, mk, cb, 0, P.x0dx + 6 * (size_t)cj, G, a, sa);
+        if (kn >= nent) break;  // uniform
+        k = kn;
+        cp = np;
+        cj = nj;
+        cb = nb;
     }
 }
 
@@ -515,14 +595,14 @@ __global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict_
         const int w = XCD ? xcd_swizzle(t, total) : t;
         const int k = w / nchunk;
         const int pnt = (w - k * nchunk) * 256 + threadIdx.x;
-        const MPoolDev &P = pools[pool_of(pools, npools, k, -1)];
-        const Geo G = P.G;
+        const MPoolDev &P = pools[__builtin_amdgcn_readfirstlane(pool_of(pools, npools, k, -1))];
+        const Geo &G = P.G;
         const int npts = G.bs0 * G.bs1 * G.bs2;
         if (pnt >= npts) continue;
         const int j = k - P.ent0;
-        const int b = P.act[j];
-        const int g = P.goff + P.grp[j];
-        const AcmDev a = gacm[g];
+        const int b = __builtin_amdgcn_readfirstlane(P.act[j]);
+        const int g = __builtin_amdgcn_readfirstlane(P.goff + P.grp[j]);
+        const AcmDev &a = gacm[g];
         const StageArgs sa = multi_stage_args(P, st, dtg + g);
         const double *xd = P.x0dx + 6 * (size_t)j;
         const int ix = pnt % G.bs0, iy = (pnt / G.bs0) % G.bs1, iz = pnt / (G.bs0 * G.bs1);
@@ -1054,13 +1134,28 @@ int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, i
 
 int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
                        int npools, const double *d_dtg, const AcmDev *d_gacm, const int64_t *d_tot,
-                       const MStage &ms, size_t lds, int grid, int nchunk)
+                       const MStage &ms, size_t lds, int grid, int nchunk, int pipe)
 {
     if (dim == 2) {
         WH_CHECK(lds <= 160 * 1024, WH_E_ARG, "2D block tile exceeds LDS");
-#define M2D(SC, SK)                                                                                         \
-    hipLaunchKernelGGL((k_rhs2d_multi<SC, SK>), dim3(grid), dim3(256), lds, stream, d_pools, npools, d_dtg, \
-                       d_gacm, d_tot, ms)
+#define M2D(SC, SK)                                                                                              \
+    do {                                                                                                         \
+        if (pipe == 1)                                                                                           \
+            hipLaunchKernelGGL((k_rhs2d_multi_pipe<SC, SK>), dim3(grid), dim3(256), lds, stream, d_pools, npools, \
+                               d_dtg, d_gacm, d_tot, ms);                                                        \
+        else if (pipe == 2)                                                                                      \
+            hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, false, 1>), dim3(grid), dim3(256), lds, stream, d_pools,    \
+                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+        else if (pipe == 3)                                                                                      \
+            hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, false, 6>), dim3(grid), dim3(256), lds, stream, d_pools,    \
+                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+        else if (pipe == 4)                                                                                      \
+            hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, false, 8>), dim3(grid), dim3(256), lds, stream, d_pools,    \
+                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, true, 1>), dim3(grid), dim3(256), lds, stream, d_pools,     \
+                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+    } while (0)
         switch (scheme * 2 + skew) {
         case WH_FD2 * 2: M2D(WH_FD2, false); break;
         case WH_FD2 * 2 + 1: M2D(WH_FD2, true); break;

@@ -122,6 +122,8 @@ def load(path: str = LIB_PATH):
         "wh_batch_time_stage_kernel": [_vp, ctypes.POINTER(AcmParamsC), _vp, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
         "wh_batch_synchronize": [_vp],
+        "wh_batch_set_tuning": [_vp, ctypes.c_int, ctypes.c_int],
+        "wh_batch_get_tuning": [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
         "wh_device_memory": [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
     }
     for name, args in sig.items():
@@ -373,6 +375,14 @@ class BatchStepper:
 
     def synchronize(self):
         check(load().wh_batch_synchronize(self.h))
+
+    def set_tuning(self, key, value):
+        check(load().wh_batch_set_tuning(self.h, key, value))
+
+    def get_tuning(self, key):
+        v = ctypes.c_int()
+        check(load().wh_batch_get_tuning(self.h, key, ctypes.byref(v)))
+        return v.value
 
     def close(self):
         if getattr(self, "h", None):
