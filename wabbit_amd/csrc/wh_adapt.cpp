@@ -13,9 +13,12 @@
 // reference's xfer_block_data.f90:321-437 picks (hvy_block or hvy_tmp).
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstdint>
 #include <cstdlib>
 #include <memory>
+#include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1122,6 +1125,39 @@ int check_pool(wh_pool *P, wh_forest *F)
 namespace wh {
 namespace {
 
+// Host threads for the per-forest work of the multi-forest calls (the
+// forests are independent until their device operations are merged):
+// WH_HOST_THREADS, default 4.  fn(i) for i in [0, n); device calls stay on
+// the calling thread.
+int host_threads()
+{
+    static const int n = [] {
+        const char *e = std::getenv("WH_HOST_THREADS");
+        const int v = e ? std::atoi(e) : 4;
+        return std::max(1, std::min(v, 64));
+    }();
+    return n;
+}
+
+template <class F>
+void parallel_for(int n, F &&fn)
+{
+    const int nt = std::min(host_threads(), n);
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) fn(i);
+    };
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (int k = 1; k < nt; ++k) th.emplace_back(work);
+    work();
+    for (std::thread &x : th) x.join();
+}
+
 // adapt_tree's checks of one forest (adapt_tree.f90:11-255 preconditions)
 int check_adapt(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_t *leaf_flags)
 {
@@ -1160,6 +1196,7 @@ struct TreeRun {
     std::vector<int32_t> mids;
     std::vector<wh_box> nboxes, zboxes;
     std::vector<std::pair<int, int>> zwho;
+    std::string err;  // the message of rc (set on the thread that failed)
     bool alive() const { return rc == WH_OK && T; }
 };
 
@@ -1202,6 +1239,14 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
         if (t.rc != WH_OK) return;
         if (t.T) t.F->free_hvy = t.free0;  // restore the forest's slots; device data of the leaves is not
         t.rc = rc;
+        t.err = wh_last_error();
+    };
+    // fn(t) for every live forest, on the host threads
+    auto each_live = [&](auto &&fn) {
+        std::vector<TreeRun *> v;
+        for (TreeRun &t : R)
+            if (t.alive()) v.push_back(&t);
+        parallel_for((int)v.size(), [&](int i) { fn(*v[i]); });
     };
     for (int i = 0; i < n; ++i) {
         TreeRun &t = R[i];
@@ -1237,8 +1282,9 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
     };
     // wavelet_decompose_full_tree; it hands the leaves' statuses back (its
     // backup), the new mothers get 0
-    for (TreeRun *t : live())
-        if (int rc = decompose_full_tree(*t->T, *t->p)) fail(*t, rc);
+    each_live([&](TreeRun &t) {
+        if (int rc = decompose_full_tree(*t.T, *t.p)) fail(t, rc);
+    });
     run_logs();
     // coarseningIndicator_tree (coarseningIndicator_tree.f90:4-331); with the
     // external indicator it returns at once (:55).  The reductions that need
@@ -1304,13 +1350,22 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
             for (TreeRun *t : thr) fail(*t, rc);
             shared = rc;
         } else {
-            size_t o_norm = 0, o_det = n_norm, o_mask = n_norm + n_det;
-            for (TreeRun *t : thr) {
+            std::vector<size_t> o_norm(thr.size()), o_det(thr.size()), o_mask(thr.size());
+            size_t a = 0, b = n_norm, c = n_norm + n_det;
+            for (size_t i = 0; i < thr.size(); ++i) {
+                o_norm[i] = a;
+                o_det[i] = b;
+                o_mask[i] = c;
+                a += thr[i]->nboxes.size() * neq;
+                b += thr[i]->idx.size() * neq;
+                if (thr[i]->p->threshold_mask) c += thr[i]->idx.size() * 3;
+            }
+            parallel_for((int)thr.size(), [&](int i) {
+                TreeRun *t = thr[i];
                 Tree &T = *t->T;
                 const wh_adapt_params &p = *t->p;
-                if (p.eps_normalized) leaf_linfty(t->nboxes.size(), neq, p.groups, res.data() + o_norm, t->norm);
-                o_norm += t->nboxes.size() * neq;
-                double *det = res.data() + o_det;
+                if (p.eps_normalized) leaf_linfty(t->nboxes.size(), neq, p.groups, res.data() + o_norm[i], t->norm);
+                double *det = res.data() + o_det[i];
                 detail_groups(P, p.groups, det, (int)t->idx.size());
                 for (size_t k = 0; k < t->idx.size(); ++k) {
                     bool all = true;
@@ -1318,21 +1373,15 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
                         if (!(det[k * neq + c] <= p.eps * t->norm[c])) all = false;
                     T.b[t->idx[k]].status = all ? -1 : 0;
                 }
-                o_det += t->idx.size() * neq;
-                if (p.threshold_mask) {
-                    mask_statuses(T, t->midx, res.data() + o_mask);
-                    o_mask += t->idx.size() * 3;
-                }
-            }
+                if (p.threshold_mask) mask_statuses(T, t->midx, res.data() + o_mask[i]);
+            });
         }
     }
     // the security zone of all forests in one more round trip
     std::vector<TreeRun *> zone;
     for (TreeRun *t : thr)
-        if (t->alive() && t->p->security_zone) {
-            zone_boxes(*t->T, *t->p, t->zboxes, t->zwho);
-            zone.push_back(t);
-        }
+        if (t->alive() && t->p->security_zone) zone.push_back(t);
+    parallel_for((int)zone.size(), [&](int i) { zone_boxes(*zone[i]->T, *zone[i]->p, zone[i]->zboxes, zone[i]->zwho); });
     if (!zone.empty() && !shared) {
         size_t nz = 0;
         for (TreeRun *t : zone) nz += t->zboxes.size() * neq;
@@ -1345,14 +1394,19 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
             for (TreeRun *t : zone) fail(*t, rc);
             shared = rc;
         } else {
-            size_t off = 0;
-            for (TreeRun *t : zone) {
-                zone_apply(*t->T, *t->p, t->norm, t->zwho, res.data() + off);
-                off += t->zboxes.size() * neq;
+            std::vector<size_t> off(zone.size());
+            size_t o = 0;
+            for (size_t i = 0; i < zone.size(); ++i) {
+                off[i] = o;
+                o += zone[i]->zboxes.size() * neq;
             }
+            parallel_for((int)zone.size(), [&](int i) {
+                zone_apply(*zone[i]->T, *zone[i]->p, zone[i]->norm, zone[i]->zwho, res.data() + off[i]);
+            });
         }
     }
-    for (TreeRun *t : live()) {
+    each_live([&](TreeRun &tr) {
+        TreeRun *t = &tr;
         Tree &T = *t->T;
         const wh_adapt_params &p = *t->p;
         // respectJmaxJmin_tree (respectJmaxJmin_tree.f90:1-39)
@@ -1376,9 +1430,10 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
             for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
             if (int rc = dev_blocks(T, WOP_COPY, A_T, A_U, hvy_of(T, all))) fail(*t, rc);
         }
-    }
+    });
     run_logs();
-    for (TreeRun *t : live()) {
+    each_live([&](TreeRun &tr) {
+        TreeRun *t = &tr;
         Tree &T = *t->T;
         // prune_fulltree2leafs
         std::vector<char> del(T.b.size(), 0);
@@ -1389,11 +1444,12 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
         SyncSpec S;
         S.gm = S.gp = T.g;
         if (int rc = tree_sync(T, S)) fail(*t, rc);
-    }
+    });
     run_logs();
     bool any_failed = false;
     for (int i = 0; i < n; ++i) {
         status[i] = R[i].rc;
+        if (R[i].rc && !any_failed && !R[i].err.empty()) set_error(R[i].err);  // the first forest's message
         if (R[i].rc) any_failed = true;
     }
     if (any_failed) (void)stage_sync(P);
@@ -1404,18 +1460,31 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
 // launch per operation
 int sync_trees_impl(wh_pool *P, wh_forest *const *Fs, int n)
 {
-    std::vector<std::unique_ptr<Tree>> T;
-    std::vector<OpLog> logs(n);
-    std::vector<OpLog *> lp;
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n; ++i)
         if (int rc = check_pool(P, Fs[i])) return rc;
-        T.emplace_back(new Tree(Fs[i], P));
-        if (n > 1) T.back()->log = &logs[i];
+    std::vector<OpLog> logs(n);
+    std::vector<int> rcs(n, WH_OK);
+    std::vector<std::string> errs(n);
+    parallel_for(n > 1 ? n : 0, [&](int i) {  // several forests: patch lists on the host threads
+        Tree T(Fs[i], P);
+        T.log = &logs[i];
         SyncSpec S;
-        S.gm = S.gp = T.back()->g;
-        if (int rc = tree_sync(*T.back(), S)) return rc;
-        lp.push_back(&logs[i]);
+        S.gm = S.gp = T.g;
+        if ((rcs[i] = tree_sync(T, S))) errs[i] = wh_last_error();
+    });
+    if (n == 1) {  // one forest launches as it goes
+        Tree T(Fs[0], P);
+        SyncSpec S;
+        S.gm = S.gp = T.g;
+        return tree_sync(T, S);
     }
+    for (int i = 0; i < n; ++i)
+        if (rcs[i]) {
+            set_error(errs[i]);
+            return rcs[i];
+        }
+    std::vector<OpLog *> lp;
+    for (OpLog &l : logs) lp.push_back(&l);
     return replay(P, lp);
 }
 
