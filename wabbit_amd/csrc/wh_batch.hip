@@ -37,6 +37,7 @@ struct wh_batch {
     // device tables: one descriptor per pool, dt and constants per block
     // group (capacity cap_grp), counts ([0] entries, [s] slots of sync stage s)
     MPoolDev *d_pools = nullptr;
+    wh::MEnt *d_ents = nullptr;  // entry records, one per slot of the pools
     double *d_dtg = nullptr;
     AcmDev *d_gacm = nullptr;
     int64_t *d_tot = nullptr;
@@ -61,6 +62,7 @@ struct wh_batch {
     // per SIMD
     int k2d = 0;
     bool pipe_ok = false;  // the last step's tiles fit the pipelined kernel
+    int full_rows = 0;     // the last step's stages write their outputs in whole rows
 };
 
 namespace {
@@ -101,6 +103,8 @@ int launch_sequence(wh_batch *B, const wh_acm_params &p, const double *bt, int n
 {
     const int s = nrows - 1;
     const int np = (int)B->pools.size();
+    // the step's entry records, read by every stage
+    if (int rc = wh::launch_entries(B->stream, B->d_pools, np, B->d_tot, B->d_ents, 2 * B->n_cu)) return rc;
     auto sync = [&](int sel) {
         for (int st = 1; st <= 3; ++st)
             if (int rc = wh::launch_sync_multi(B->stream, B->d_pools, np, sel, st, B->d_tot, B->grid_sync))
@@ -117,9 +121,10 @@ int launch_sequence(wh_batch *B, const wh_acm_params &p, const double *bt, int n
         ms.use_a = std::abs(a) >= 1.0e-8;
         ms.use_b = std::abs(b) >= 1.0e-8;
         ms.sout_sel = mode == 2 ? B->sout_sel : 0;
+        ms.full_rows = B->full_rows;
         return wh::launch_stage_multi(B->stream, B->dim, p.scheme, p.skew_symmetry ? 1 : 0, 1, B->d_pools, np,
-                                      B->d_dtg, B->d_gacm, B->d_tot, ms, B->lds, B->grid_stage, B->nchunk,
-                                      k2d_sel(B));
+                                      B->d_ents, B->d_dtg, B->d_gacm, B->d_tot, ms, B->lds, B->grid_stage,
+                                      B->nchunk, k2d_sel(B));
     };
     // arrays: 0 state, 1 accumulator (work 0), 2 / 3 ping-pong stage inputs
     if (int rc = sync(0)) return rc;
@@ -211,6 +216,9 @@ int wh_batch_create(wh_batch **out, wh_pool *const *pools, int npools)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&B->d_tot, sizeof(int64_t) * 4)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&B->d_pools, sizeof(MPoolDev) * npools)) != hipSuccess) return fail(e);
+    size_t slots = 0;
+    for (int i = 0; i < npools; ++i) slots += (size_t)pools[i]->d.nb_max;
+    if ((e = hipMalloc(&B->d_ents, sizeof(wh::MEnt) * slots)) != hipSuccess) return fail(e);
     *out = B;
     return WH_OK;
 }
@@ -220,7 +228,7 @@ int wh_batch_destroy(wh_batch *B)
     if (!B) return WH_OK;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     free_graph(B);
-    for (void *p : {(void *)B->d_pools, (void *)B->d_dtg, (void *)B->d_gacm, (void *)B->d_tot})
+    for (void *p : {(void *)B->d_pools, (void *)B->d_ents, (void *)B->d_dtg, (void *)B->d_gacm, (void *)B->d_tot})
         if (p) (void)hipFree(p);
     if (B->h_buf) (void)hipHostFree(B->h_buf);
     for (hipEvent_t ev : B->ev_pool)
@@ -253,6 +261,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     int nchunk = 1;
     const int R = p->scheme == WH_FD2 ? 1 : p->scheme == WH_FD4 ? 2 : 3;
     int goff = 0, nent = 0;
+    double step_bytes = 0.0;  // one block of every entry
     int64_t tot[4] = {0, 0, 0, 0};
     bool pipe_ok = true;
     for (int i = 0; i < npools; ++i) {
@@ -308,6 +317,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
         }
         nent += P->nact;
         goff += P->ngrp;
+        step_bytes += (double)P->nact * sizeof(double) * (double)m.G.blk;
     }
     tot[0] = nent;
     if (gacm.empty()) {
@@ -363,6 +373,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     B->lds = lds;
     B->nchunk = nchunk;
     B->pipe_ok = pipe_ok;
+    B->full_rows = wh::stage_full_rows(1.0, step_bytes) ? 1 : 0;
     B->grid_stage = stage_grid(B);
     B->grid_sync = 4 * B->n_cu;
     B->n_ent_last = nent;
@@ -374,6 +385,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
         key.push_back(nchunk);
         key.push_back(npools);
         key.push_back(k2d_sel(B));
+        key.push_back(B->full_rows);
         if (!B->gexec || B->graph_version != ver || key != B->graph_key) {
             free_graph(B);
             WH_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
@@ -424,11 +436,13 @@ int wh_batch_time_stage_kernel(wh_batch *B, const wh_acm_params *p, const double
         m.b = b;
         m.use_a = std::abs(a) >= 1.0e-8;
         m.use_b = std::abs(b) >= 1.0e-8;
+        m.full_rows = B->full_rows;
         return wh::launch_stage_multi(B->stream, B->dim, p->scheme, p->skew_symmetry ? 1 : 0, 1, B->d_pools,
-                                      (int)B->pools.size(), B->d_dtg, B->d_gacm, B->d_tot, m, B->lds,
+                                      (int)B->pools.size(), B->d_ents, B->d_dtg, B->d_gacm, B->d_tot, m, B->lds,
                                       stage_grid(B), B->nchunk, k2d_sel(B));
     };
-    int rc = WH_OK;
+    // the last step's entry records (rebuilt from its descriptors)
+    int rc = wh::launch_entries(B->stream, B->d_pools, (int)B->pools.size(), B->d_tot, B->d_ents, 2 * B->n_cu);
     WH_HIP(hipEventRecord(e0, B->stream));
     for (int r = 0; r < reps && !rc; ++r) {
         // one RK4 stage sequence that reads the state and writes work slots only

@@ -79,12 +79,30 @@ struct MPoolDev {
     int64_t s0[3];
 };
 
+// Entry k of a multi-pool launch, resolved once per step (launch_entries):
+// its pool, hvy id, block group in the launch (goff + grp) and position in
+// the pool's active list.  The stage kernels read one record per block
+// instead of searching the pools and then reading the active list and the
+// group table one after the other.
+struct MEnt {
+    int32_t pool, b, g, j;
+};
+
 // The RK stage of a multi-pool fused stage launch: input / output array
 // selectors into MPoolDev::arr, the epilogue mode and tableau coefficients.
 struct MStage {
     int in_sel, out_sel, sout_sel, mode, use_a, use_b;
+    int full_rows;  // the next stage input written in whole rows (stage_full_rows)
     double a, b;
 };
+
+// Whether a fused RK stage over nblk blocks of blk_bytes each writes the
+// next stage input in whole rows (zeros in the ghost columns of the interior
+// rows): worth it where the stage's arrays (input, accumulator, state,
+// output) exceed the 256 MB last-level cache, so that partially written
+// cache lines would go to HBM and be merged there with their old contents;
+// below that the lines stay in the cache and the extra bytes only cost.
+inline bool stage_full_rows(double nblk, double blk_bytes) { return 4.0 * nblk * blk_bytes > 256.0 * (1 << 20); }
 
 // Alternate source arrays of explicit patch lists (bits 8-9 of wh_patch::op).
 struct SyncSrc {
@@ -295,9 +313,12 @@ Geo make_geo(const wh_pool *P);
 // d_tot: [0] entries of the launch, [s] slots of sync stage s (1..3); dtg /
 // gacm: dt and constants per block group of the launch; pipe: the 2D
 // software-pipelined kernel (tile widths and heights <= 32)
+// d_ents: the launch's entry records (launch_entries, capacity >= d_tot[0])
+int launch_entries(hipStream_t stream, const MPoolDev *d_pools, int npools, const int64_t *d_tot, MEnt *d_ents,
+                   int grid);
 int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
-                       int npools, const double *d_dtg, const AcmDev *d_gacm, const int64_t *d_tot,
-                       const MStage &ms, size_t lds, int grid, int nchunk, int pipe = 0);
+                       int npools, const MEnt *d_ents, const double *d_dtg, const AcmDev *d_gacm,
+                       const int64_t *d_tot, const MStage &ms, size_t lds, int grid, int nchunk, int pipe = 0);
 int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, int sel, int stage,
                       const int64_t *d_tot, int grid);
 int reach_of(int scheme);

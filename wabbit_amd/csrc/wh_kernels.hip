@@ -163,6 +163,10 @@ struct StridedAcc {
 //   mode 2 (last):        sout = acc + (dt*b)*k
 // b == 0 skips the acc term as the reference skips |coef| < 1e-8
 // (runge_kutta_generic.f90:99,144).
+// The accumulator lives only through one RK step and holds block interiors
+// only: point p (x fastest) of component c of block b at b*NC*npts + c*npts
+// + p, so its reads and writes cover whole cache lines.  u0, tout and sout
+// are in the ghost-including block layout.
 struct StageArgs {
     const double *u0;
     double *acc, *tout, *sout;
@@ -170,57 +174,109 @@ struct StageArgs {
     const double *dtb;  // per active-list position dt (independent forests), or null
     double a, b;
     int mode, use_b, use_a;
+    int full_rows;  // tout written in whole rows (stage_ghost_cols)
 };
 
 struct NoEpilogue {};
 
+// The arrays of one block of a fused stage: base pointers of block b (the
+// same in every lane, so they stay in scalar registers); the epilogue
+// addresses points by 32-bit offsets inside the block.
+struct StageBlk {
+    const double *u0;
+    double *acc, *tout, *sout;
+};
+
 template <int NC>
-__device__ __forceinline__ void stage_epilogue(const StageArgs &sa, int k, size_t o, size_t cs,
-                                               const double *r, const double *ucen,
-                                               ptrdiff_t ucs)
+__device__ __forceinline__ StageBlk stage_blk(const StageArgs &sa, int b, size_t blk, int npts)
+{
+    const size_t o = (size_t)b * blk;
+    return StageBlk{sa.u0 ? sa.u0 + o : nullptr, sa.acc + (size_t)b * NC * npts, sa.tout ? sa.tout + o : nullptr,
+                    sa.sout ? sa.sout + o : nullptr};
+}
+
+// point p of the interior (offset gi in the block, components cs apart)
+template <int NC>
+__device__ __forceinline__ void stage_epilogue(const StageArgs &sa, const StageBlk &B, int k, int gi, int cs, int p,
+                                               int npts, const double *r, const double *ucen, ptrdiff_t ucs)
 {
     const double dt = sa.dtb ? sa.dtb[k] : sa.scal[DS_DT];
     const double ca = dt * sa.a, cb = dt * sa.b;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const size_t i = o + c * cs;
+        const int i = gi + c * cs, ia = p + c * npts;
         if (sa.mode == 0) {
             const double u0 = ucen[c * ucs];
-            sa.acc[i] = sa.use_b ? u0 + cb * r[c] : u0;
-            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+            B.acc[ia] = sa.use_b ? u0 + cb * r[c] : u0;
+            if (sa.use_a) B.tout[i] = u0 + ca * r[c];
         } else if (sa.mode == 1) {
-            const double u0 = sa.u0[i];
-            if (sa.use_b) sa.acc[i] = sa.acc[i] + cb * r[c];
-            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+            const double u0 = B.u0[i];
+            if (sa.use_b) B.acc[ia] = B.acc[ia] + cb * r[c];
+            if (sa.use_a) B.tout[i] = u0 + ca * r[c];
         } else {
-            sa.sout[i] = sa.use_b ? sa.acc[i] + cb * r[c] : sa.acc[i];
+            B.sout[i] = sa.use_b ? B.acc[ia] + cb * r[c] : B.acc[ia];
         }
     }
 }
 
 // the fused epilogue with the acc / u0 values of the point already loaded
-// (accv, u0v: components npts apart)
+// (accv, u0v: one value per component)
 template <int NC>
-__device__ __forceinline__ void stage_epilogue_pre(const StageArgs &sa, int k, size_t o, size_t cs,
-                                                   const double *r, const double *ucen, ptrdiff_t ucs,
-                                                   const double *accv, const double *u0v, int npts)
+__device__ __forceinline__ void stage_epilogue_pre(const StageArgs &sa, const StageBlk &B, int k, int gi, int cs,
+                                                   int p, int npts, const double *r, const double *ucen,
+                                                   ptrdiff_t ucs, const double *accv, const double *u0v)
 {
     const double dt = sa.dtb ? sa.dtb[k] : sa.scal[DS_DT];
     const double ca = dt * sa.a, cb = dt * sa.b;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const size_t i = o + c * cs;
+        const int i = gi + c * cs, ia = p + c * npts;
         if (sa.mode == 0) {
             const double u0 = ucen[c * ucs];
-            sa.acc[i] = sa.use_b ? u0 + cb * r[c] : u0;
-            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+            B.acc[ia] = sa.use_b ? u0 + cb * r[c] : u0;
+            if (sa.use_a) B.tout[i] = u0 + ca * r[c];
         } else if (sa.mode == 1) {
-            const double u0 = u0v[c * npts];
-            if (sa.use_b) sa.acc[i] = accv[c * npts] + cb * r[c];
-            if (sa.use_a) sa.tout[i] = u0 + ca * r[c];
+            if (sa.use_b) B.acc[ia] = accv[c] + cb * r[c];
+            if (sa.use_a) B.tout[i] = u0v[c] + ca * r[c];
         } else {
-            sa.sout[i] = sa.use_b ? accv[c * npts] + cb * r[c] : accv[c * npts];
+            B.sout[i] = sa.use_b ? accv[c] + cb * r[c] : accv[c];
         }
+    }
+}
+
+// Zeros in the ghost columns (x < g and x >= g + bs0) of one interior row of
+// the next stage input, written by the threads of the row's first 2g points
+// (where sa.full_rows: stages whose arrays exceed the last-level cache):
+// with the interior written point by point, every cache line of the row is
+// then written whole, so no line is merged with its old contents in memory.
+// Before the next stage the ghost fill overwrites every ghost value its
+// stencil reads (the stencils run along the axes, within g_rhs of the
+// interior); the other ghost values of a stage input are not read.  row: the
+// offset of the row's first (x = 0) element in the block.
+template <int NC>
+__device__ __forceinline__ void stage_ghost_cols(const StageArgs &sa, const StageBlk &B, int row, int cs, int ix,
+                                                 int bs0, int g)
+{
+    if (!sa.full_rows || sa.mode == 2 || !sa.use_a) return;
+    for (int x = ix; x < 2 * g; x += bs0) {
+        const int i = row + (x < g ? x : bs0 + x);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) B.tout[i + c * cs] = 0.0;
+    }
+}
+
+// stage_ghost_cols for every interior row of a 2D block, by the whole
+// workgroup after its points (no per-point registers live meanwhile)
+template <int NC>
+__device__ __forceinline__ void stage_ghost_cols_2d(const StageArgs &sa, const StageBlk &B, const Geo &G)
+{
+    if (!sa.full_rows || sa.mode == 2 || !sa.use_a) return;
+    const int w = 2 * G.g, cs = (int)G.cs;
+    for (int i = threadIdx.x; i < G.bs1 * w; i += blockDim.x) {
+        const int r = i / w, x = i - r * w;
+        const int o = (G.g + r) * G.nx + (x < G.g ? x : G.bs0 + x);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) B.tout[o + c * cs] = 0.0;
     }
 }
 
@@ -244,11 +300,13 @@ __device__ __forceinline__ void rhs2d_block(double *tile, const double *__restri
     const int npts = G.bs0 * G.bs1;
     const size_t base = (size_t)b * G.blk;
     const int p0 = threadIdx.x;
+    const int cs = (int)G.cs;
+    const StageBlk SB = FUSED ? stage_blk<3>(sa, b, G.blk, npts) : StageBlk{};
     double mpre[6] = {0, 0, 0, 0, 0, 0}, apre[3] = {0, 0, 0}, upre[3] = {0, 0, 0};
     if constexpr (PRE) {
         if (p0 < npts) {
             const int iy = p0 / G.bs0, ix = p0 - iy * G.bs0;
-            const size_t gi = (size_t)(G.g + iy) * G.nx + (G.g + ix);
+            const int gi = (G.g + iy) * G.nx + (G.g + ix);
             if (mk) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c)
@@ -256,10 +314,10 @@ __device__ __forceinline__ void rhs2d_block(double *tile, const double *__restri
             }
             if (FUSED && sa.mode >= 1) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) apre[c] = sa.acc[base + c * G.cs + gi];
+                for (int c = 0; c < 3; ++c) apre[c] = SB.acc[c * npts + p0];
                 if (sa.mode == 1) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) upre[c] = sa.u0[base + c * G.cs + gi];
+                    for (int c = 0; c < 3; ++c) upre[c] = SB.u0[c * cs + gi];
                 }
             }
         }
@@ -280,24 +338,24 @@ __device__ __forceinline__ void rhs2d_block(double *tile, const double *__restri
     for (int pnt = threadIdx.x; pnt < npts; pnt += blockDim.x) {
         const int iy = pnt / G.bs0, ix = pnt - iy * G.bs0;
         const double *uc = tile + (iy + R) * tx + (ix + R);
-        const size_t gi = (size_t)(G.g + iy) * G.nx + (G.g + ix);
+        const int gi = (G.g + iy) * G.nx + (G.g + ix);
         double r[3];
         const StridedAcc<2> A{uc, tcs, {1, tx, 0}};
-        const size_t o = base + gi;
         if (PRE && pnt == p0) {
             // separate calls (not a pointer select) keep mpre in registers
             if (mk) acm_point<2, S, SKEW>(A, inv, inv2, a, mpre, 1, r);
             else acm_point<2, S, SKEW>(A, inv, inv2, a, nullptr, 1, r);
-            if constexpr (FUSED) stage_epilogue_pre<3>(sa, k, o, G.cs, r, uc, tcs, apre, upre, 1);
+            if constexpr (FUSED) stage_epilogue_pre<3>(sa, SB, k, gi, cs, pnt, npts, r, uc, tcs, apre, upre);
         } else {
             acm_point<2, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
-            if constexpr (FUSED) stage_epilogue<3>(sa, k, o, G.cs, r, uc, tcs);
+            if constexpr (FUSED) stage_epilogue<3>(sa, SB, k, gi, cs, pnt, npts, r, uc, tcs);
         }
         if constexpr (!FUSED) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) out[o + c * G.cs] = r[c];
+            for (int c = 0; c < 3; ++c) out[base + gi + c * G.cs] = r[c];
         }
     }
+    if constexpr (FUSED) stage_ghost_cols_2d<3>(sa, SB, G);
 }
 
 template <int S, bool SKEW, bool FUSED, bool PRE>
@@ -319,7 +377,9 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
 // size): entry k of the launch is position k - ent0 of the active list of
 // the pool whose entry range holds k (pools[] in entry order), with that
 // position's origin/spacing and block group g = goff + grp[..] (dt dtg[g],
-// constants gacm[g]), all read from the pool's own device tables.  The
+// constants gacm[g]), all read from the pool's own device tables; once per
+// step k_entries resolves every entry into a record (pool, hvy id, group,
+// position) that the stage kernels read with one load per block.  The
 // stage's input and output arrays are selected per pool from arr[]:
 // 0 hvy_block, 1 work slot 0 (the RK accumulator), 2 / 3 work slots 1 / 2.
 // The kernels are persistent (grid-stride over the entries, their count read
@@ -351,26 +411,46 @@ __device__ __forceinline__ StageArgs multi_stage_args(const MPoolDev &P, const M
     sa.mode = st.mode;
     sa.use_a = st.use_a;
     sa.use_b = st.use_b;
+    sa.full_rows = st.full_rows;
     return sa;
+}
+
+// the entry records of a launch: one thread per entry
+__global__ __launch_bounds__(256) void k_entries(const MPoolDev *__restrict__ pools, int npools,
+                                                 const int64_t *__restrict__ tot, MEnt *__restrict__ ents)
+{
+    const int n = (int)tot[0];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int p = pool_of(pools, npools, k, -1);
+        const MPoolDev &P = pools[p];
+        const int j = k - P.ent0;
+        ents[k] = MEnt{p, P.act[j], P.goff + P.grp[j], j};
+    }
+}
+
+// entry k's record with every field wave-uniform, so that the descriptor and
+// the group's constants are read through scalar loads
+__device__ __forceinline__ MEnt entry_of(const MEnt *__restrict__ ents, int k)
+{
+    const MEnt e = ents[k];
+    return MEnt{__builtin_amdgcn_readfirstlane(e.pool), __builtin_amdgcn_readfirstlane(e.b),
+                __builtin_amdgcn_readfirstlane(e.g), __builtin_amdgcn_readfirstlane(e.j)};
 }
 
 // PRE: the first point's mask and operands loaded before the tile barrier;
 // WPE: minimum waves per SIMD the register allocation must allow.
 template <int S, bool SKEW, bool PRE, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_rhs2d_multi(
-    const MPoolDev *__restrict__ pools, int npools, const double *__restrict__ dtg, const AcmDev *__restrict__ gacm,
-    const int64_t *__restrict__ tot, MStage st)
+    const MPoolDev *__restrict__ pools, const MEnt *__restrict__ ents, const double *__restrict__ dtg,
+    const AcmDev *__restrict__ gacm, const int64_t *__restrict__ tot, MStage st)
 {
     extern __shared__ double tile[];
     const int nent = (int)tot[0];
     for (int k = blockIdx.x; k < nent; k += gridDim.x) {
-        // the block's indices are the same in every lane (readfirstlane), so
-        // the descriptor and the constants are read through scalar loads
-        const MPoolDev &P = pools[__builtin_amdgcn_readfirstlane(pool_of(pools, npools, k, -1))];
-        const int j = k - P.ent0;
+        const MEnt e = entry_of(ents, k);
+        const MPoolDev &P = pools[e.pool];
+        const int j = e.j, b = e.b, g = e.g;
         const Geo &G = P.G;
-        const int b = __builtin_amdgcn_readfirstlane(P.act[j]);
-        const int g = __builtin_amdgcn_readfirstlane(P.goff + P.grp[j]);
         const AcmDev &a = gacm[g];
         const StageArgs sa = multi_stage_args(P, st, dtg + g);
         const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)b * G.mblk : 0) : nullptr;
@@ -390,7 +470,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // 28 with FD4); per block the arithmetic is that of k_rhs2d_multi.
 constexpr int MP_ROWS = 4;
 template <int S, bool SKEW>
-__global__ __launch_bounds__(256) void k_rhs2d_multi_pipe(const MPoolDev *__restrict__ pools, int npools,
+__global__ __launch_bounds__(256) void k_rhs2d_multi_pipe(const MPoolDev *__restrict__ pools,
+                                                          const MEnt *__restrict__ ents,
                                                           const double *__restrict__ dtg,
                                                           const AcmDev *__restrict__ gacm,
                                                           const int64_t *__restrict__ tot, MStage st)
@@ -402,12 +483,14 @@ __global__ __launch_bounds__(256) void k_rhs2d_multi_pipe(const MPoolDev *__rest
     if (k >= nent) return;  // uniform over the workgroup
     const int col = threadIdx.x & 31, row0 = threadIdx.x >> 5;
     double reg[3][MP_ROWS];
-    int cp, cj, cb;  // the current entry: pool, active position, hvy id
-    auto fetch = [&](int kk, int &p, int &j, int &b) {
-        p = pool_of(pools, npools, kk, -1);
+    int cp, cj, cb, cg;  // the current entry: pool, active position, hvy id, group
+    auto fetch = [&](int kk, int &p, int &j, int &b, int &g) {
+        const MEnt e = entry_of(ents, kk);
+        p = e.pool;
+        j = e.j;
+        b = e.b;
+        g = e.g;
         const MPoolDev &P = pools[p];
-        j = kk - P.ent0;
-        b = P.act[j];
         const Geo &G = P.G;
         const int tx = G.bs0 + 2 * R, ty = G.bs1 + 2 * R;
         const double *u = P.arr[st.in_sel] + (size_t)b * G.blk + (size_t)(G.g - R) * G.nx + (G.g - R);
@@ -419,7 +502,7 @@ __global__ __launch_bounds__(256) void k_rhs2d_multi_pipe(const MPoolDev *__rest
                 if (col < tx && row < ty) reg[c][r] = u[c * G.cs + (size_t)row * G.nx + col];
             }
     };
-    fetch(k, cp, cj, cb);
+    fetch(k, cp, cj, cb, cg);
     while (true) {
         __syncthreads();  // the previous block is done with the tile
         {
@@ -435,13 +518,12 @@ __global__ __launch_bounds__(256) void k_rhs2d_multi_pipe(const MPoolDev *__rest
         }
         __syncthreads();
         const int kn = k + (int)gridDim.x;
-        int np = 0, nj = 0, nb = 0;
-        if (kn < nent) fetch(kn, np, nj, nb);  // in flight while block k is computed
+        int np = 0, nj = 0, nb = 0, ng = 0;
+        if (kn < nent) fetch(kn, np, nj, nb, ng);  // in flight while block k is computed
         const MPoolDev &P = pools[cp];
         const Geo G = P.G;
-        const int g = P.goff + P.grp[cj];
-        const AcmDev a = gacm[g];
-        const StageArgs sa = multi_stage_args(P, st, dtg + g);
+        const AcmDev a = gacm[cg];
+        const StageArgs sa = multi_stage_args(P, st, dtg + cg);
         const double *mk = P.mask ? P.mask + (G.mask_per_block ? (size_t)cb * G.mblk : 0) : nullptr;
         rhs2d_block<S, SKEW, true, false, true>(tile, nullptr, nullptr, mk, cb, 0, P.x0dx + 6 * (size_t)cj, G, a, sa);
         if (kn >= nent) break;  // uniform
@@ -449,6 +531,7 @@ __global__ __launch_bounds__(256) void k_rhs2d_multi_pipe(const MPoolDev *__rest
         cp = np;
         cj = nj;
         cb = nb;
+        cg = ng;
     }
 }
 
@@ -515,12 +598,14 @@ __global__ __launch_bounds__(256) void k_rhs2d_pipe(const double *__restrict__ i
             acm_point<2, S, SKEW>(A, inv, inv2, a, mk ? mk + gi : nullptr, G.cs, r);
             const size_t o = (size_t)b * G.blk + gi;
             if constexpr (FUSED) {
-                stage_epilogue<3>(sa, k, o, G.cs, r, uc, tcs);
+                const StageBlk SB = stage_blk<3>(sa, b, G.blk, npts);
+                stage_epilogue<3>(sa, SB, k, (int)gi, (int)G.cs, pnt, npts, r, uc, tcs);
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) out[o + c * G.cs] = r[c];
             }
         }
+        if constexpr (FUSED) stage_ghost_cols_2d<3>(sa, stage_blk<3>(sa, b, G.blk, npts), G);
         if (kn >= nact) break;  // uniform
         store(sm + (cur ^ 1) * ntile);
         __syncthreads();
@@ -571,7 +656,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     acm_point<3, S, SKEW>(A, inv, inv2, a, mk, G.cs, r);
     const size_t o = (size_t)b * G.blk + gi;
     if constexpr (FUSED) {
-        stage_epilogue<4>(sa, k, o, G.cs, r, uc, G.cs);
+        const StageBlk SB = stage_blk<4>(sa, b, G.blk, npts);
+        stage_epilogue<4>(sa, SB, k, (int)gi, (int)G.cs, pnt, npts, r, uc, G.cs);
+        stage_ghost_cols<4>(sa, SB, (int)gi - (G.g + ix), (int)G.cs, ix, G.bs0, G.g);
     } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
@@ -585,7 +672,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // With XCD the item index is swizzled so that a block's chunks run on one
 // XCD (the grid is a multiple of 8, so item t runs on XCD t % 8).
 template <int S, bool SKEW, bool XCD>
-__global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict__ pools, int npools,
+__global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict__ pools,
+                                                     const MEnt *__restrict__ ents,
                                                      const double *__restrict__ dtg,
                                                      const AcmDev *__restrict__ gacm,
                                                      const int64_t *__restrict__ tot, MStage st, int nchunk)
@@ -595,13 +683,12 @@ __global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict_
         const int w = XCD ? xcd_swizzle(t, total) : t;
         const int k = w / nchunk;
         const int pnt = (w - k * nchunk) * 256 + threadIdx.x;
-        const MPoolDev &P = pools[__builtin_amdgcn_readfirstlane(pool_of(pools, npools, k, -1))];
+        const MEnt e = entry_of(ents, k);
+        const MPoolDev &P = pools[e.pool];
         const Geo &G = P.G;
         const int npts = G.bs0 * G.bs1 * G.bs2;
         if (pnt >= npts) continue;
-        const int j = k - P.ent0;
-        const int b = __builtin_amdgcn_readfirstlane(P.act[j]);
-        const int g = __builtin_amdgcn_readfirstlane(P.goff + P.grp[j]);
+        const int j = e.j, b = e.b, g = e.g;
         const AcmDev &a = gacm[g];
         const StageArgs sa = multi_stage_args(P, st, dtg + g);
         const double *xd = P.x0dx + 6 * (size_t)j;
@@ -615,7 +702,9 @@ __global__ __launch_bounds__(256) void k_rhs3d_multi(const MPoolDev *__restrict_
         double r[4];
         const StridedAcc<3> A{uc, (ptrdiff_t)G.cs, {1, G.nx, (ptrdiff_t)G.nx * G.ny}};
         acm_point<3, S, SKEW>(A, inv, inv2, a, mk, G.cs, r);
-        stage_epilogue<4>(sa, 0, (size_t)b * G.blk + gi, G.cs, r, uc, G.cs);
+        const StageBlk SB = stage_blk<4>(sa, b, G.blk, npts);
+        stage_epilogue<4>(sa, SB, 0, (int)gi, (int)G.cs, pnt, npts, r, uc, G.cs);
+        stage_ghost_cols<4>(sa, SB, (int)gi - (G.g + ix), (int)G.cs, ix, G.bs0, G.g);
     }
 }
 
@@ -728,7 +817,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const size_t o = (size_t)b * G.blk + gi;
         if constexpr (FUSED) {
             const double u0c[4] = {q[0][R], q[1][R], q[2][R], q[3][R]};
-            stage_epilogue<4>(sa, k, o, G.cs, r, u0c, 1);
+            const int npts = G.bs0 * G.bs1 * G.bs2;
+            const StageBlk SB = stage_blk<4>(sa, b, G.blk, npts);
+            stage_epilogue<4>(sa, SB, k, (int)gi, (int)G.cs, (iz * G.bs1 + iy) * G.bs0 + ix, npts, r, u0c, 1);
+            stage_ghost_cols<4>(sa, SB, (int)gi - (G.g + ix), (int)G.cs, ix, G.bs0, G.g);
         } else {
 #pragma unroll
             for (int c = 0; c < 4; ++c) out[o + c * G.cs] = r[c];
@@ -930,6 +1022,7 @@ int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *t
     sa.mode = mode;
     sa.use_a = std::abs(acoef) >= 1.0e-8;
     sa.use_b = std::abs(bcoef) >= 1.0e-8;
+    sa.full_rows = stage_full_rows(P->nact, sizeof(double) * (double)make_geo(P).blk);
     return dispatch_rhs<true>(P, a, tin, nullptr, P->d_active, P->d_x0dx, P->nact, sa);
 }
 
@@ -1132,29 +1225,38 @@ int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, i
     return WH_OK;
 }
 
-int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
-                       int npools, const double *d_dtg, const AcmDev *d_gacm, const int64_t *d_tot,
-                       const MStage &ms, size_t lds, int grid, int nchunk, int pipe)
+int launch_entries(hipStream_t stream, const MPoolDev *d_pools, int npools, const int64_t *d_tot, MEnt *d_ents,
+                   int grid)
 {
+    hipLaunchKernelGGL(k_entries, dim3(grid), dim3(256), 0, stream, d_pools, npools, d_tot, d_ents);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xcd, const MPoolDev *d_pools,
+                       int npools, const MEnt *d_ents, const double *d_dtg, const AcmDev *d_gacm,
+                       const int64_t *d_tot, const MStage &ms, size_t lds, int grid, int nchunk, int pipe)
+{
+    (void)npools;
     if (dim == 2) {
         WH_CHECK(lds <= 160 * 1024, WH_E_ARG, "2D block tile exceeds LDS");
 #define M2D(SC, SK)                                                                                              \
     do {                                                                                                         \
         if (pipe == 1)                                                                                           \
-            hipLaunchKernelGGL((k_rhs2d_multi_pipe<SC, SK>), dim3(grid), dim3(256), lds, stream, d_pools, npools, \
+            hipLaunchKernelGGL((k_rhs2d_multi_pipe<SC, SK>), dim3(grid), dim3(256), lds, stream, d_pools, d_ents, \
                                d_dtg, d_gacm, d_tot, ms);                                                        \
         else if (pipe == 2)                                                                                      \
             hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, false, 1>), dim3(grid), dim3(256), lds, stream, d_pools,    \
-                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+                               d_ents, d_dtg, d_gacm, d_tot, ms);                                                \
         else if (pipe == 3)                                                                                      \
             hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, false, 6>), dim3(grid), dim3(256), lds, stream, d_pools,    \
-                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+                               d_ents, d_dtg, d_gacm, d_tot, ms);                                                \
         else if (pipe == 4)                                                                                      \
             hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, false, 8>), dim3(grid), dim3(256), lds, stream, d_pools,    \
-                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+                               d_ents, d_dtg, d_gacm, d_tot, ms);                                                \
         else                                                                                                     \
             hipLaunchKernelGGL((k_rhs2d_multi<SC, SK, true, 1>), dim3(grid), dim3(256), lds, stream, d_pools,     \
-                               npools, d_dtg, d_gacm, d_tot, ms);                                                \
+                               d_ents, d_dtg, d_gacm, d_tot, ms);                                                \
     } while (0)
         switch (scheme * 2 + skew) {
         case WH_FD2 * 2: M2D(WH_FD2, false); break;
@@ -1172,10 +1274,10 @@ int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xc
 #define M3D(SC, SK)                                                                                           \
     do {                                                                                                      \
         if (xcd)                                                                                              \
-            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, true>), dim3(grid), dim3(256), 0, stream, d_pools, npools, \
+            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, true>), dim3(grid), dim3(256), 0, stream, d_pools, d_ents, \
                                d_dtg, d_gacm, d_tot, ms, nchunk);                                              \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, false>), dim3(grid), dim3(256), 0, stream, d_pools, npools, \
+            hipLaunchKernelGGL((k_rhs3d_multi<SC, SK, false>), dim3(grid), dim3(256), 0, stream, d_pools, d_ents, \
                                d_dtg, d_gacm, d_tot, ms, nchunk);                                              \
     } while (0)
         switch (scheme * 2 + skew) {
