@@ -468,6 +468,34 @@ int wh_sync_trees(wh_pool *pool, wh_forest *const *forests, int n, int array);
 int wh_adapt_trees(wh_pool *pool, wh_forest *const *forests, int n, const wh_adapt_params *params,
                    int *status);
 
+/* The rest of a main-loop iteration's mesh work for n forests that share the
+ * pool (independent solves; no counterpart in the reference, which runs one
+ * forest per process), each call made once per pool instead of once per
+ * forest.  status[i] is forest i's own error; the return value is an error
+ * only when an operation shared by all forests failed.
+ * wh_refine_trees: refine_tree with the "everywhere" indicator
+ * (LIB/MESH/refine_tree.f90, refinementExecute.f90:4-116): every leaf of
+ * forest i below level jmax[i] gets its 2^dim daughters (block lists on the
+ * host, then one wh_refine_blocks launch for all forests); a forest without
+ * the free slots for all its daughters at once is left unchanged with
+ * WH_E_NOMEM.
+ * wh_create_masks: createMask_tree (LIB/PHYSICS/create_mask.f90:6-320) on
+ * the leaves of each forest with its own params[i].
+ * wh_set_tables_from_forests: the tables of the RK step -- active list and
+ * origins/spacings of every forest's leaves (forest after forest, in leaf
+ * order), block group k for the k-th forest taken, and the RHS ghost-patch
+ * table of depth g_rhs with predictors of `order`
+ * (synchronize_ghosts_generic.f90:155 sync_ghosts_RHS_tree) -- as
+ * wh_set_active + wh_set_block_groups + wh_set_patches(pool, 0, ...) would
+ * set them from wh_forest_get_blocks / wh_forest_patches; a forest whose
+ * patches cannot be generated is left out (status[i], nblocks[i] = 0);
+ * nblocks[i]: forest i's leaves in the tables. */
+int wh_refine_trees(wh_pool *pool, wh_forest *const *forests, int n, const int32_t *jmax, int order, int array,
+                    int *status);
+int wh_create_masks(wh_pool *pool, wh_forest *const *forests, int n, const wh_mask_params *params, int *status);
+int wh_set_tables_from_forests(wh_pool *pool, wh_forest *const *forests, int n, int g_rhs, int order, int *status,
+                               int *nblocks);
+
 /* Kernel tuning knobs.  WH_TUNE_K3D_YSPLIT: workgroups per block of the 3D
  * stage kernel (each owns Bs1/ysplit rows of (x,y) columns and marches in z);
  * 0 selects the flat one-point-per-thread kernel, -1 automatic.

@@ -36,7 +36,7 @@ from .adaptive import AdaptConfig, fd_reach
 from .forest import equidistant_treecodes, refined_treecodes
 from .params import ACMParams, TimeParams, butcher_rk4, clamp_dt
 from .timestep import PATH_FUSED
-from .tree import Forest, refine_blocks, refine_on_device
+from .tree import Forest, refine_on_device
 
 
 @dataclass
@@ -209,12 +209,11 @@ class PoolGroup:
         self.slots.free(s.base)
 
     def prepare(self):
-        """Phase A: per solve sync_ghosts_tree + refine everywhere + mask,
-        then the pool's union active list, patches, groups, group constants
-        and clamped dt per group.  A pool without live solves gets empty
-        tables."""
+        """Phase A: sync_ghosts_tree + refine everywhere + mask of every
+        solve (each a call for all solves of the pool), then the pool's union
+        active list, patches, groups, group constants and clamped dt per
+        group.  A pool without live solves gets empty tables."""
         pool, w = self.pool, self.w
-        hvs, xs, grp, pats, live = [], [], [], [], []
         cand = [s for s in self.solves if not s.done]
         # sync_ghosts_tree of every solve, one launch per operation for all of
         # them; should that fail, each solve alone, so a failure stays its own
@@ -227,49 +226,46 @@ class PoolGroup:
                 except L.WabbitHipError as e:
                     s.error = str(e)
             cand = [s for s in cand if s.error is None]
-        # refine everywhere: every solve's new blocks on the host, one
-        # refinement launch for all (a solve whose free slots cannot take all
-        # its daughters at once refines in batches of its own)
-        tris = []
-        for s in list(cand):
-            try:
-                tc, lv, hv, _ = s.forest.blocks()
-                ids = np.asarray(hv[lv < s.spec.cfg.jmax], np.int32)
-                if ids.size and s.forest.nb_max - s.forest.n >= (1 << s.forest.dim) * ids.size:
-                    tris.append(s.forest.refine(ids))
-                elif ids.size:
-                    refine_on_device(pool, s.forest, ids, w.order, w.g_rhs, presync=None, attach=False)
-            except L.WabbitHipError as e:    # the solve alone fails (e.g. out of its slots)
-                s.error = str(e)
+        # refine everywhere below Jmax: every solve's new blocks on the host
+        # threads and one refinement launch for all, in one call; a solve
+        # whose free slots cannot take all its daughters at once refines in
+        # batches of its own
+        def fail(s, code, msg):
+            s.error = f"{L.ERRORS.get(code, code)}: {msg}"
+        res = W.refine_trees(pool, [s.forest for s in cand], [s.spec.cfg.jmax for s in cand], w.order)
+        for s, (code, msg) in zip(list(cand), res):
+            if code == L.WH_E_NOMEM:
+                try:
+                    tc, lv, hv, _ = s.forest.blocks()
+                    refine_on_device(pool, s.forest, np.asarray(hv[lv < s.spec.cfg.jmax], np.int32), w.order,
+                                     w.g_rhs, presync=None, attach=False)
+                    continue
+                except L.WabbitHipError as e:   # the solve alone fails
+                    code, msg = e.code, str(e)
+            if code:
+                fail(s, code, msg)
                 cand.remove(s)
-        if tris:
-            refine_blocks(pool, np.concatenate(tris), w.order)
-        for s in cand:
-            try:
-                tc, lv, hv, x = s.forest.blocks()
-                if s.mask_p is not None:
-                    W.create_mask(pool, s.mask_p, hv, x)
-                pat = s.forest.patches(w.g_rhs, w.g_rhs, w.order)
-            except L.WabbitHipError as e:    # the solve alone fails
-                s.error = str(e)
+        masked = [s for s in cand if s.mask_p is not None]
+        for s, (code, msg) in zip(masked, W.create_masks(pool, [s.forest for s in masked],
+                                                         [s.mask_p for s in masked])):
+            if code:
+                fail(s, code, msg)
+                cand.remove(s)
+        # the RK step's tables from the forests (active list, origins and
+        # spacings, one block group per solve, RHS ghost patches)
+        res, nbs = W.set_tables_from_forests(pool, [s.forest for s in cand], w.g_rhs, w.order)
+        live = []
+        for s, (code, msg), nb in zip(cand, res, nbs):
+            if code:
+                fail(s, code, msg)
                 continue
-            s.rk_blocks = len(hv)
-            hvs.append(hv)
-            xs.append(x)
-            grp.append(np.full(len(hv), len(live), np.int32))
-            pats.append(pat)
+            s.rk_blocks = nb
             live.append(s)
         self._live = live
-        self.last_nact = 0
+        self.last_nact = int(sum(nbs))
         if not live:
-            pool.set_active(np.zeros(0, np.int32), np.zeros((0, 6)))
-            pool.set_patches(0, np.zeros(0, L.PATCH_DTYPE))
             self._dts = np.zeros(0)
             return
-        self.last_nact = int(sum(len(h) for h in hvs))
-        pool.set_active(np.concatenate(hvs), np.concatenate(xs))
-        pool.set_patches(0, np.concatenate(pats))
-        pool.set_block_groups(np.concatenate(grp), len(live))
         pool.set_group_params([s.acm_c for s in live])
         dts = pool.dt_groups(live[0].acm_c)
         self._dts = np.array([clamp_dt(d, s.time, s.spec.tp) for d, s in zip(dts, live)])

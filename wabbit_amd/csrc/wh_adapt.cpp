@@ -1488,6 +1488,128 @@ int sync_trees_impl(wh_pool *P, wh_forest *const *Fs, int n)
     return replay(P, lp);
 }
 
+// the forest's blocks fit the pool (dimension, block size, ghost nodes,
+// slot range)
+int check_fits(wh_pool *P, const wh_forest *F)
+{
+    WH_CHECK(F->dim == P->d.dim && F->g == P->d.g && F->slot0 + F->nb_max <= P->d.nb_max, WH_E_ARG,
+             "forest and pool disagree on dim / ghost nodes / capacity");
+    for (int d = 0; d < F->dim; ++d) WH_CHECK(F->bs[d] == P->d.bs[d], WH_E_ARG, "forest and pool block sizes differ");
+    return WH_OK;
+}
+
+// refine_tree with the "everywhere" indicator (refine_tree.f90: every leaf
+// below jmax[i] gets 2^dim daughters, which keeps a graded grid graded) for
+// n forests of one pool: the block lists on the host threads, then one
+// refinement launch (refineBlock, refinementExecute.f90:4-116) for the new
+// blocks of all of them.  A forest without the free slots for all its
+// daughters at once is left unchanged and reports WH_E_NOMEM.
+int refine_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const int *jmax, int order, int array, int *status)
+{
+    std::vector<std::vector<int32_t>> tri(n);
+    std::vector<std::string> errs(n);
+    parallel_for(n, [&](int i) {
+        wh_forest *F = Fs[i];
+        status[i] = check_fits(P, F);
+        if (status[i]) {
+            errs[i] = wh_last_error();
+            return;
+        }
+        std::vector<int32_t> ids;
+        for (const FBlock &b : F->blocks)
+            if (b.level < jmax[i]) ids.push_back(b.hvy);
+        if (ids.empty()) return;
+        const int nd = 1 << F->dim, cap = (int)ids.size() * nd;
+        if ((int)F->free_hvy.size() < cap) {
+            status[i] = WH_E_NOMEM;
+            errs[i] = "refine_trees: the forest has too few free slots to refine all its blocks at once";
+            return;
+        }
+        tri[i].resize(3 * (size_t)cap);
+        int np = 0;
+        if ((status[i] = wh_forest_refine(F, (int)ids.size(), ids.data(), tri[i].data(), cap, &np))) {
+            errs[i] = wh_last_error();
+            tri[i].clear();
+            return;
+        }
+        tri[i].resize(3 * (size_t)np);
+    });
+    std::vector<int32_t> all;
+    for (int i = 0; i < n; ++i) all.insert(all.end(), tri[i].begin(), tri[i].end());
+    for (int i = 0; i < n; ++i)
+        if (status[i]) set_error(errs[i]);  // the last failing forest's message
+    if (all.empty()) return WH_OK;
+    return wh_refine_blocks(P, array, all.data(), (int)(all.size() / 3), order);
+}
+
+// The RK tables of a pool from n forests' leaves: the active list and
+// origin/spacing (forest after forest, each in its leaf order), block group
+// k for the k-th forest taken, and the RHS ghost-patch table (depth g_rhs,
+// predictor order; sync_ghosts_RHS_tree) -- what wh_set_active,
+// wh_set_block_groups and wh_set_patches(0) set from the forests' own lists,
+// with the patch lists generated on the host threads and moved into the
+// pool's table directly.  A forest whose patches cannot be generated is left
+// out (status); nblocks[i]: its leaves in the tables (0 if left out).
+int set_tables_impl(wh_pool *P, wh_forest *const *Fs, int n, int g_rhs, int order, int *status, int *nblocks)
+{
+    std::vector<std::string> errs(n);
+    parallel_for(n, [&](int i) {
+        int np = 0;
+        status[i] = check_fits(P, Fs[i]);
+        if (!status[i]) status[i] = wh_forest_patches(Fs[i], g_rhs, g_rhs, order, nullptr, 0, &np);
+        if (status[i]) errs[i] = wh_last_error();
+    });
+    size_t nact = 0;
+    for (int i = 0; i < n; ++i) {
+        nblocks[i] = status[i] ? 0 : (int)Fs[i]->blocks.size();
+        nact += nblocks[i];
+        if (status[i]) set_error(errs[i]);
+    }
+    WH_CHECK(nact <= (size_t)P->d.nb_max, WH_E_ARG, "set_tables: more leaves than pool slots");
+    std::vector<int32_t> act(nact), grp(nact);
+    std::vector<double> x0dx(6 * nact);
+    std::vector<PatchRun> runs;
+    size_t k = 0;
+    int ng = 0;
+    for (int i = 0; i < n; ++i) {
+        if (status[i]) continue;
+        const wh_forest *F = Fs[i];
+        for (const FBlock &b : F->blocks) {
+            act[k] = b.hvy;
+            grp[k] = ng;
+            F->x0dx(b, &x0dx[6 * k]);
+            ++k;
+        }
+        // the list wh_forest_patches just generated (the forest's cache)
+        runs.push_back(PatchRun{F->pcache.data(), (int)F->pcache.size()});
+        ++ng;
+    }
+    if (int rc = wh_set_active(P, act.data(), (int)nact, x0dx.data())) return rc;
+    if (int rc = set_patch_table(P, 0, runs.data(), (int)runs.size())) return rc;
+    if (ng > 0)
+        if (int rc = wh_set_block_groups(P, grp.data(), (int)nact, ng)) return rc;
+    return WH_OK;
+}
+
+// createMask_tree (create_mask.f90:6-320) on the leaves of n forests of one
+// pool, each with its own parameters, one mask launch per forest
+int create_masks_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_mask_params *mp, int *status)
+{
+    for (int i = 0; i < n; ++i) {
+        const wh_forest *F = Fs[i];
+        if ((status[i] = check_fits(P, F))) continue;
+        std::vector<int32_t> ids(F->blocks.size());
+        std::vector<double> x0dx(6 * F->blocks.size());
+        for (size_t j = 0; j < F->blocks.size(); ++j) {
+            ids[j] = F->blocks[j].hvy;
+            F->x0dx(F->blocks[j], &x0dx[6 * j]);
+        }
+        status[i] = run_mask(P, mp[i], ids, x0dx);
+        if (status[i] == WH_E_HIP) return status[i];  // the device failed: every forest fails
+    }
+    return WH_OK;
+}
+
 }  // namespace
 }  // namespace wh
 
@@ -1545,4 +1667,42 @@ int wh_adapt_trees(wh_pool *P, wh_forest *const *forests, int n, const wh_adapt_
     return wh::adapt_trees_impl(P, forests, n, params, nullptr, status);
 }
 
+int wh_refine_trees(wh_pool *P, wh_forest *const *forests, int n, const int32_t *jmax, int order, int array,
+                    int *status)
+{
+    WH_PROF("api.refine_trees");
+    WH_CHECK(P && n >= 0 && (n == 0 || (forests && jmax && status)), WH_E_ARG, "wh_refine_trees: null argument");
+    WH_CHECK(order == 2 || order == 4 || order == 6, WH_E_ARG, "wh_refine_trees: order must be 2, 4 or 6");
+    for (int i = 0; i < n; ++i) {
+        WH_CHECK(forests[i], WH_E_ARG, "wh_refine_trees: null forest");
+        for (int j = 0; j < i; ++j) WH_CHECK(forests[j] != forests[i], WH_E_ARG, "wh_refine_trees: a forest listed twice");
+    }
+    return wh::refine_trees_impl(P, forests, n, jmax, order, array, status);
+}
+
+int wh_set_tables_from_forests(wh_pool *P, wh_forest *const *forests, int n, int g_rhs, int order, int *status,
+                               int *nblocks)
+{
+    WH_PROF("api.set_tables_from_forests");
+    WH_CHECK(P && n >= 0 && (n == 0 || (forests && status && nblocks)), WH_E_ARG,
+             "wh_set_tables_from_forests: null argument");
+    WH_CHECK(g_rhs >= 0 && g_rhs <= P->d.g && (order == 2 || order == 4 || order == 6), WH_E_ARG,
+             "wh_set_tables_from_forests: g_rhs beyond the ghost nodes or order not 2, 4, 6");
+    for (int i = 0; i < n; ++i) {
+        WH_CHECK(forests[i], WH_E_ARG, "wh_set_tables_from_forests: null forest");
+        for (int j = 0; j < i; ++j)
+            WH_CHECK(forests[j] != forests[i], WH_E_ARG, "wh_set_tables_from_forests: a forest listed twice");
+    }
+    return wh::set_tables_impl(P, forests, n, g_rhs, order, status, nblocks);
+}
+
+int wh_create_masks(wh_pool *P, wh_forest *const *forests, int n, const wh_mask_params *params, int *status)
+{
+    WH_PROF("api.create_masks");
+    WH_CHECK(P && n >= 0 && (n == 0 || (forests && params && status)), WH_E_ARG, "wh_create_masks: null argument");
+    for (int i = 0; i < n; ++i) WH_CHECK(forests[i], WH_E_ARG, "wh_create_masks: null forest");
+    return wh::create_masks_impl(P, forests, n, params, status);
+}
+
 }  // extern "C"
+

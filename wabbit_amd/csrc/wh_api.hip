@@ -522,40 +522,14 @@ int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
     return WH_OK;
 }
 
-int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatch)
+// The patch table of a sync case from runs of patches (taken one after the
+// other), without checks: stable counting sort by stage (1..3; within a
+// stage the receivers keep their order), per-receiver rows, slot prefix
+// sums, upload.
+static int set_patch_runs(wh_pool *P, int sync_case, const PatchRun *runs, int nruns)
 {
-    WH_PROF("wh_set_patches");
-    WH_CHECK(P && (npatch == 0 || patches), WH_E_ARG, "null argument");
-    WH_CHECK(sync_case == 0 || sync_case == 1, WH_E_ARG, "sync_case must be 0 or 1");
-    WH_CHECK(npatch >= 0, WH_E_ARG, "negative patch count");
-    const int ext[3] = {P->nx, P->ny, P->nz};
-    for (int i = 0; i < npatch; ++i) {
-        const wh_patch &q = patches[i];
-        WH_CHECK(q.recv >= 0 && q.recv < P->d.nb_max && q.send >= 0 && q.send < P->d.nb_max,
-                 WH_E_ARG, "patch block id out of pool range");
-        WH_CHECK(q.op == WH_PATCH_COPY || q.op == WH_PATCH_RESTRICT || q.op == WH_PATCH_PREDICT,
-                 WH_E_ARG, "unknown patch op");
-        WH_CHECK(q.stage >= 1 && q.stage <= 3, WH_E_ARG, "patch stage must be 1..3");
-        WH_CHECK(q.op != WH_PATCH_PREDICT || q.aux == 2 || q.aux == 4 || q.aux == 6, WH_E_ARG,
-                 "predict patch order must be 2, 4 or 6");
-        const int st = q.op == WH_PATCH_RESTRICT ? 2 : 1;
-        for (int d = 0; d < 3; ++d) {
-            WH_CHECK(q.n[d] >= 1, WH_E_ARG, "empty patch");
-            WH_CHECK(q.recv_lo[d] >= 0 && q.recv_lo[d] + q.n[d] <= ext[d], WH_E_ARG,
-                     "receiver box outside the block");
-            if (q.op == WH_PATCH_PREDICT) {
-                // coarse stencil of every point must stay inside the sender block
-                const int N = q.aux / 2;
-                const int fmin = q.send_lo[d], fmax = q.send_lo[d] + q.n[d] - 1;
-                const bool interp = (d < P->d.dim);
-                WH_CHECK(interp ? pred_range_ok(fmin, fmax, N, ext[d]) : (fmin == 0 && fmax == 0),
-                         WH_E_ARG, "prediction stencil outside the sender block");
-            } else {
-                WH_CHECK(q.send_lo[d] >= 0 && q.send_lo[d] + st * (q.n[d] - 1) < ext[d], WH_E_ARG,
-                         "sender box outside the block");
-            }
-        }
-    }
+    int npatch = 0;
+    for (int r = 0; r < nruns; ++r) npatch += runs[r].n;
     if (int rc = wait_tables(P)) return rc;  // nothing queued reads the old table any more
     PatchSet &ps = P->ps[sync_case];
     ps.max_stage = 0;
@@ -563,10 +537,12 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
     // within a stage (rows of the per-receiver table are runs of one receiver)
     {
         int cnt[5] = {0, 0, 0, 0, 0};
-        for (int i = 0; i < npatch; ++i) ++cnt[patches[i].stage + 1];
+        for (int r = 0; r < nruns; ++r)
+            for (int i = 0; i < runs[r].n; ++i) ++cnt[runs[r].p[i].stage + 1];
         for (int st = 1; st < 5; ++st) cnt[st] += cnt[st - 1];
         ps.host.resize(npatch);
-        for (int i = 0; i < npatch; ++i) ps.host[cnt[patches[i].stage]++] = patches[i];
+        for (int r = 0; r < nruns; ++r)
+            for (int i = 0; i < runs[r].n; ++i) ps.host[cnt[runs[r].p[i].stage]++] = runs[r].p[i];
     }
     ps.h_recv_off.clear();
     ps.h_recv_ids.clear();
@@ -610,6 +586,44 @@ int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatc
     }
     P->topo_version++;
     return WH_OK;
+}
+
+int wh_set_patches(wh_pool *P, int sync_case, const wh_patch *patches, int npatch)
+{
+    WH_PROF("wh_set_patches");
+    WH_CHECK(P && (npatch == 0 || patches), WH_E_ARG, "null argument");
+    WH_CHECK(sync_case == 0 || sync_case == 1, WH_E_ARG, "sync_case must be 0 or 1");
+    WH_CHECK(npatch >= 0, WH_E_ARG, "negative patch count");
+    const int ext[3] = {P->nx, P->ny, P->nz};
+    for (int i = 0; i < npatch; ++i) {
+        const wh_patch &q = patches[i];
+        WH_CHECK(q.recv >= 0 && q.recv < P->d.nb_max && q.send >= 0 && q.send < P->d.nb_max,
+                 WH_E_ARG, "patch block id out of pool range");
+        WH_CHECK(q.op == WH_PATCH_COPY || q.op == WH_PATCH_RESTRICT || q.op == WH_PATCH_PREDICT,
+                 WH_E_ARG, "unknown patch op");
+        WH_CHECK(q.stage >= 1 && q.stage <= 3, WH_E_ARG, "patch stage must be 1..3");
+        WH_CHECK(q.op != WH_PATCH_PREDICT || q.aux == 2 || q.aux == 4 || q.aux == 6, WH_E_ARG,
+                 "predict patch order must be 2, 4 or 6");
+        const int st = q.op == WH_PATCH_RESTRICT ? 2 : 1;
+        for (int d = 0; d < 3; ++d) {
+            WH_CHECK(q.n[d] >= 1, WH_E_ARG, "empty patch");
+            WH_CHECK(q.recv_lo[d] >= 0 && q.recv_lo[d] + q.n[d] <= ext[d], WH_E_ARG,
+                     "receiver box outside the block");
+            if (q.op == WH_PATCH_PREDICT) {
+                // coarse stencil of every point must stay inside the sender block
+                const int N = q.aux / 2;
+                const int fmin = q.send_lo[d], fmax = q.send_lo[d] + q.n[d] - 1;
+                const bool interp = (d < P->d.dim);
+                WH_CHECK(interp ? pred_range_ok(fmin, fmax, N, ext[d]) : (fmin == 0 && fmax == 0),
+                         WH_E_ARG, "prediction stencil outside the sender block");
+            } else {
+                WH_CHECK(q.send_lo[d] >= 0 && q.send_lo[d] + st * (q.n[d] - 1) < ext[d], WH_E_ARG,
+                         "sender box outside the block");
+            }
+        }
+    }
+    const PatchRun run{patches, npatch};
+    return set_patch_runs(P, sync_case, &run, 1);
 }
 
 int wh_sync_ghosts(wh_pool *P, int array, int sync_case)
@@ -1114,3 +1128,10 @@ extern "C" int wh_refine_blocks(wh_pool *P, int array, const int32_t *triples, i
     if (!d_tr) return WH_E_HIP;
     return launch_refine(P, arr, d_tr, ntriples, order);
 }
+
+namespace wh {
+int set_patch_table(wh_pool *P, int sync_case, const PatchRun *runs, int nruns)
+{
+    return set_patch_runs(P, sync_case, runs, nruns);
+}
+}  // namespace wh

@@ -313,6 +313,14 @@ Geo make_geo(const wh_pool *P);
 // d_tot: [0] entries of the launch, [s] slots of sync stage s (1..3); dtg /
 // gacm: dt and constants per block group of the launch; pipe: the 2D
 // software-pipelined kernel (tile widths and heights <= 32)
+// a run of ghost patches (set_patch_table takes several, one after the other)
+struct PatchRun {
+    const wh_patch *p;
+    int n;
+};
+// the patch table of sync case 0/1 from runs of patches, without the checks
+// of wh_set_patches (for lists the library generated itself)
+int set_patch_table(wh_pool *P, int sync_case, const PatchRun *runs, int nruns);
 // d_ents: the launch's entry records (launch_entries, capacity >= d_tot[0])
 int launch_entries(hipStream_t stream, const MPoolDev *d_pools, int npools, const int64_t *d_tot, MEnt *d_ents,
                    int grid);

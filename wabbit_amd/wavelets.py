@@ -116,6 +116,9 @@ def _bind():
         "wh_sync_tree": [_vp, _vp, ip],
         "wh_sync_trees": [_vp, _vp, ip, ip],
         "wh_adapt_trees": [_vp, _vp, ip, _vp, _vp],
+        "wh_refine_trees": [_vp, _vp, ip, _vp, ip, ip, _vp],
+        "wh_create_masks": [_vp, _vp, ip, _vp, _vp],
+        "wh_set_tables_from_forests": [_vp, _vp, ip, ip, ip, _vp, _vp],
         "wh_create_mask": [_vp, ctypes.POINTER(MaskParamsC), _vp, ip, _vp],
     }
     for name, args in sig.items():
@@ -290,3 +293,57 @@ def adapt_trees(pool, forests, params):
                                     ctypes.cast(st, ctypes.c_void_p)))
     msg = L.load().wh_last_error().decode(errors="replace") if any(st) else ""
     return [(int(c), msg if c else "") for c in st]
+
+
+def _status(st):
+    msg = L.load().wh_last_error().decode(errors="replace") if any(st) else ""
+    return [(int(c), msg if c else "") for c in st]
+
+
+def refine_trees(pool, forests, jmax, order, array=L.WH_BLOCK):
+    """refine_tree "everywhere" below jmax[i] for several forests sharing the
+    pool, one refinement launch for all.  Returns one (code, message) per
+    forest: code WH_E_NOMEM leaves a forest without the free slots for all its
+    daughters at once unchanged."""
+    _bind()
+    n = len(forests)
+    if n == 0:
+        return []
+    arr = (ctypes.c_void_p * n)(*[f.h.value for f in forests])
+    jm = (ctypes.c_int32 * n)(*[int(j) for j in jmax])
+    st = (ctypes.c_int * n)()
+    L.check(L.load().wh_refine_trees(pool.h, ctypes.cast(arr, ctypes.c_void_p), n, ctypes.cast(jm, ctypes.c_void_p),
+                                     order, array, ctypes.cast(st, ctypes.c_void_p)))
+    return _status(st)
+
+
+def create_masks(pool, forests, params):
+    """createMask_tree on the leaves of several forests sharing the pool, each
+    with its own MaskParamsC.  Returns one (code, message) per forest."""
+    _bind()
+    n = len(forests)
+    if n == 0:
+        return []
+    arr = (ctypes.c_void_p * n)(*[f.h.value for f in forests])
+    prm = (MaskParamsC * n)(*params)
+    st = (ctypes.c_int * n)()
+    L.check(L.load().wh_create_masks(pool.h, ctypes.cast(arr, ctypes.c_void_p), n, ctypes.cast(prm, ctypes.c_void_p),
+                                     ctypes.cast(st, ctypes.c_void_p)))
+    return _status(st)
+
+
+def set_tables_from_forests(pool, forests, g_rhs, order):
+    """The RK step's tables of the pool (active list, origins/spacings, one
+    block group per forest, RHS ghost patches) from several forests' leaves.
+    Returns ([(code, message)] per forest, [leaves in the tables] per forest);
+    a forest with a non-zero code is left out."""
+    _bind()
+    n = len(forests)
+    arr = (ctypes.c_void_p * max(n, 1))(*[f.h.value for f in forests])
+    st = (ctypes.c_int * max(n, 1))()
+    nb = (ctypes.c_int * max(n, 1))()
+    L.check(L.load().wh_set_tables_from_forests(pool.h, ctypes.cast(arr, ctypes.c_void_p), n, g_rhs, order,
+                                                ctypes.cast(st, ctypes.c_void_p), ctypes.cast(nb, ctypes.c_void_p)))
+    pool.ngroups = sum(1 for c in st[:n] if c == 0)  # one block group per forest taken
+    return _status(st[:n]), [int(x) for x in nb[:n]]
+
