@@ -479,8 +479,9 @@ def main():
     ap.add_argument("--requests", type=int, default=256, help="c5: requests taken from the stream")
     ap.add_argument("--rate", type=float, default=4.0, help="c5: Poisson arrival rate (1/s)")
     ap.add_argument("--lanes", type=int, default=None,
-                    help="c4/c5: host threads (pool sets) stepping solves (default c4: 4, c5: 4 -- c5's "
-                         "3D pool share per lane must hold a Jmax-5 request within 60%% of HBM)")
+                    help="c4/c5: independent pool sets (every pool steps its solves on a host thread of its "
+                         "own; default 2 -- c5's 3D pool share per lane must hold a Jmax-5 request within "
+                         "60%% of HBM)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -494,11 +495,11 @@ def main():
               "(there is no multi-GPU decomposition)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c4":
-        args.lanes = args.lanes or 4
+        args.lanes = args.lanes or 2
         run_ensemble(args)
         return
     if args.workload == "c5":
-        args.lanes = args.lanes or 4
+        args.lanes = args.lanes or 2
         run_serving(args)
         return
     w = workload(args.workload)

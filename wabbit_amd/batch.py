@@ -319,10 +319,10 @@ class Batch:
     """Independent solves on shared pools, one pool per block shape and lane.
 
     With lanes > 1 the solves are spread over that many independent sets of
-    pools, each with its own stream, whose per-solve mesh work runs on one
-    host thread per lane (in the library, with the interpreter lock
-    released); the RK stages of all pools of a dimension run as one launch
-    per stage (wh_batch_rk_step) between the two mesh phases.
+    pools.  Every pool has its own stream, and the mesh work of its solves
+    runs on a host thread of its own (in the library, with the interpreter
+    lock released); the RK stages of all pools of a dimension run as one
+    launch per stage (wh_batch_rk_step) between the two mesh phases.
 
     prealloc: the pool keys (pool_key) to create at start-up in every lane;
     then admission never creates a pool, and a request whose shape has no
@@ -340,13 +340,11 @@ class Batch:
         self.groups: Dict[tuple, PoolGroup] = {}
         self.next_rid = 0
         self._exec = None
+        self._exec_n = 0
         self._steppers: Dict[int, L.BatchStepper] = {}
         self._stepper_pools: Dict[int, tuple] = {}
         self.phase_s: Dict[str, float] = {}
         self.preallocated = prealloc is not None
-        if self.lanes > 1:
-            from concurrent.futures import ThreadPoolExecutor
-            self._exec = ThreadPoolExecutor(self.lanes)
         for k in prealloc or []:
             for lane in range(self.lanes):
                 self._create(k, lane)
@@ -386,17 +384,20 @@ class Batch:
     def solves(self):
         return [s for g in self.groups.values() for s in g.solves]
 
-    def _by_lane(self):
-        by_lane: Dict[int, List[PoolGroup]] = {}
-        for k, g in self.groups.items():
-            by_lane.setdefault(k[0], []).append(g)
-        return list(by_lane.values())
-
-    def _run_lanes(self, fn):
-        lanes = self._by_lane()
-        if self._exec is None:
-            return [fn(gs) for gs in lanes]
-        futs = [self._exec.submit(fn, gs) for gs in lanes]
+    def _run_groups(self, fn):
+        """fn(group) for every pool group, each on a host thread of its own
+        (the groups share nothing but the device); the results in group
+        order."""
+        gs = list(self.groups.values())
+        if len(gs) <= 1:
+            return [fn(g) for g in gs]
+        if self._exec_n < len(gs):
+            from concurrent.futures import ThreadPoolExecutor
+            if self._exec is not None:
+                self._exec.shutdown()
+            self._exec = ThreadPoolExecutor(len(gs))
+            self._exec_n = len(gs)
+        futs = [self._exec.submit(fn, g) for g in gs]
         return [f.result() for f in futs]
 
     def _stepper(self, dim, groups):
@@ -415,9 +416,9 @@ class Batch:
         mesh work before the RK step, rk: the batched RK launches, finish:
         adapt_tree)."""
         if path != PATH_FUSED:
-            return sum(self._run_lanes(lambda gs: sum(g.step_alone(path) for g in gs)))
+            return sum(self._run_groups(lambda g: g.step_alone(path)))
         t0 = time.perf_counter()
-        self._run_lanes(lambda gs: [g.prepare() for g in gs])
+        self._run_groups(lambda g: g.prepare())
         t1 = time.perf_counter()
         bt = butcher_rk4()
         for dim in (2, 3):
@@ -429,7 +430,7 @@ class Batch:
             dts = np.concatenate([g._dts for g in groups]) if groups else np.zeros(0)
             st.rk_step(live[0]._live[0].acm_c, bt, dts, use_graph=self.use_graph)
         t2 = time.perf_counter()
-        stages = sum(self._run_lanes(lambda gs: sum(g.finish() for g in gs)))
+        stages = sum(self._run_groups(lambda g: g.finish()))
         t3 = time.perf_counter()
         for k, v in (("prepare", t1 - t0), ("rk", t2 - t1), ("finish", t3 - t2)):
             self.phase_s[k] = self.phase_s.get(k, 0.0) + v
