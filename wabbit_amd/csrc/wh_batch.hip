@@ -35,7 +35,8 @@ struct wh_batch {
     std::vector<hipEvent_t> ev_pool;
     int n_cu = 256;
     // device tables: one descriptor per pool, dt and constants per block
-    // group (capacity cap_grp), counts ([0] entries, [s] slots of sync stage s)
+    // group (capacity cap_grp), counts ([0] entries, [s] slots and [3 + s]
+    // patches of sync stage s)
     MPoolDev *d_pools = nullptr;
     wh::MEnt *d_ents = nullptr;  // entry records, one per slot of the pools
     double *d_dtg = nullptr;
@@ -107,7 +108,7 @@ int launch_sequence(wh_batch *B, const wh_acm_params &p, const double *bt, int n
     if (int rc = wh::launch_entries(B->stream, B->d_pools, np, B->d_tot, B->d_ents, 2 * B->n_cu)) return rc;
     auto sync = [&](int sel) {
         for (int st = 1; st <= 3; ++st)
-            if (int rc = wh::launch_sync_multi(B->stream, B->d_pools, np, sel, st, B->d_tot, B->grid_sync))
+            if (int rc = wh::launch_sync_multi(B->stream, B->d_pools, np, sel, st, B->d_tot, B->grid_sync, B->dim))
                 return rc;
         return (int)WH_OK;
     };
@@ -214,7 +215,7 @@ int wh_batch_create(wh_batch **out, wh_pool *const *pools, int npools)
     B->ev_pool.resize(npools, nullptr);
     for (auto &ev : B->ev_pool)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e);
-    if ((e = hipMalloc(&B->d_tot, sizeof(int64_t) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&B->d_tot, sizeof(int64_t) * 7)) != hipSuccess) return fail(e);
     if ((e = hipMalloc(&B->d_pools, sizeof(MPoolDev) * npools)) != hipSuccess) return fail(e);
     size_t slots = 0;
     for (int i = 0; i < npools; ++i) slots += (size_t)pools[i]->d.nb_max;
@@ -262,7 +263,7 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
     const int R = p->scheme == WH_FD2 ? 1 : p->scheme == WH_FD4 ? 2 : 3;
     int goff = 0, nent = 0;
     double step_bytes = 0.0;  // one block of every entry
-    int64_t tot[4] = {0, 0, 0, 0};
+    int64_t tot[7] = {0, 0, 0, 0, 0, 0, 0};
     bool pipe_ok = true;
     for (int i = 0; i < npools; ++i) {
         wh_pool *P = B->pools[i];
@@ -305,8 +306,13 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
             m.p0[st - 1] = a;
             m.p1[st - 1] = b;
             m.s0[st - 1] = tot[st];
-            if (P->nact > 0 && b > a) tot[st] += ps.h_pstart[b] - ps.h_pstart[a];
-            else m.p1[st - 1] = a;  // a pool without blocks fills no ghosts
+            m.q0[st - 1] = tot[3 + st];
+            if (P->nact > 0 && b > a) {
+                tot[st] += ps.h_pstart[b] - ps.h_pstart[a];
+                tot[3 + st] += b - a;
+            } else {
+                m.p1[st - 1] = a;  // a pool without blocks fills no ghosts
+            }
         }
         if (P->nact > 0) {
             if (std::max(P->d.bs[0], P->d.bs[1]) + 2 * R > 32) pipe_ok = false;

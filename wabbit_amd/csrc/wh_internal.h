@@ -62,8 +62,9 @@ struct Geo {
 // per active position, the RHS ghost-patch table with its slot prefix sum --
 // and the pool's place in the launch: its first entry (entries are the
 // pools' active lists one after the other), its first block group, and per
-// sync stage s its patch range [p0[s], p1[s]) and first slot s0[s] in the
-// stage's slots (the pools' slot ranges one after the other).
+// sync stage s its patch range [p0[s], p1[s]), its first slot s0[s] in the
+// stage's slots (the pools' slot ranges one after the other) and its first
+// patch q0[s] in the stage's patches (the pools' ranges one after the other).
 struct MPoolDev {
     double *arr[4];
     const double *mask;
@@ -76,7 +77,7 @@ struct MPoolDev {
     const int64_t *pstart;
     int ent0, goff;
     int p0[3], p1[3];
-    int64_t s0[3];
+    int64_t s0[3], q0[3];
 };
 
 // Entry k of a multi-pool launch, resolved once per step (launch_entries):
@@ -310,7 +311,8 @@ int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<doub
 AcmDev make_acm_dev(const wh_acm_params *p);
 Geo make_geo(const wh_pool *P);
 // multi-pool launches (independent forests of a batch, wh_batch.hip)
-// d_tot: [0] entries of the launch, [s] slots of sync stage s (1..3); dtg /
+// d_tot: [0] entries of the launch, [s] slots of sync stage s (1..3), [3 + s]
+// patches of sync stage s; dtg /
 // gacm: dt and constants per block group of the launch; pipe: the 2D
 // software-pipelined kernel (tile widths and heights <= 32)
 // a run of ghost patches (set_patch_table takes several, one after the other)
@@ -328,7 +330,7 @@ int launch_stage_multi(hipStream_t stream, int dim, int scheme, int skew, int xc
                        int npools, const MEnt *d_ents, const double *d_dtg, const AcmDev *d_gacm,
                        const int64_t *d_tot, const MStage &ms, size_t lds, int grid, int nchunk, int pipe = 0);
 int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, int sel, int stage,
-                      const int64_t *d_tot, int grid);
+                      const int64_t *d_tot, int grid, int dim);
 int reach_of(int scheme);
 bool zm_valid(int bs0, int bs1, int scheme, int ys);
 bool zm_zvalid(int bs2, int scheme, int zs);

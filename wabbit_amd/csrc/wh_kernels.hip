@@ -385,13 +385,14 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
 // The kernels are persistent (grid-stride over the entries, their count read
 // from device memory), so one captured launch serves any number of blocks.
 // ---------------------------------------------------------------------------
-// the last pool whose first entry (or first slot of stage s, s >= 0) is <= x
+// the last pool whose first entry (s < 0), first slot of sync stage s (0..2) or
+// first patch of sync stage s - 3 (3..5) is <= x
 __device__ __forceinline__ int pool_of(const MPoolDev *__restrict__ pools, int npools, int64_t x, int s)
 {
     int lo = 0, hi = npools - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int64_t first = s < 0 ? (int64_t)pools[mid].ent0 : pools[mid].s0[s];
+        const int64_t first = s < 0 ? (int64_t)pools[mid].ent0 : s < 3 ? pools[mid].s0[s] : pools[mid].q0[s - 3];
         if (first <= x) lo = mid;
         else hi = mid - 1;
     }
@@ -1145,10 +1146,61 @@ __global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
     for (int c = 0; c < neq; ++c) dst[c * cs] = src[c * cs];
 }
 
+// The points of one ghost patch, taken by the 64 lanes of a wavefront in
+// steps of 64 (x fastest): per point the copy / decimation / prediction of
+// k_sync_flat.  P is the same in every lane (read through the scalar unit);
+// sbase: block P.send of the sender array.
+__device__ __forceinline__ void sync_patch(double *__restrict__ arr, const wh_patch &P,
+                                           const double *__restrict__ sbase, size_t blk, size_t cs, int nx, int ny,
+                                           int neq, int dim, int lane)
+{
+    const int n0 = P.n[0], n1 = P.n[1], np = n0 * n1 * P.n[2];
+    double *dbase = arr + (size_t)P.recv * blk;
+    const int op = P.op & 0xff;
+    const int st = (op == WH_PATCH_RESTRICT) ? 2 : 1;
+    for (int r = lane; r < np; r += 64) {
+        const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
+        double *dst = dbase + ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx + (P.recv_lo[0] + i);
+        if (op == WH_PATCH_PREDICT) {
+            const int f[3] = {P.send_lo[0] + i, P.send_lo[1] + j, P.send_lo[2] + kk};
+            for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(sbase + c * cs, nx, ny, dim, f, P.aux);
+        } else {
+            const double *src = sbase + ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
+                                (P.send_lo[0] + st * i);
+            for (int c = 0; c < neq; ++c) dst[c * cs] = src[c * cs];
+        }
+    }
+}
+
+// 2D ghost fill of the patches [p0, p1): one wavefront per patch (patch
+// p0 + w for wavefront w of the launch), its record read once through the
+// scalar unit -- no search; a 2D ghost patch has a few dozen points, so the
+// wavefront takes them in one or two steps (3D patches are far larger and
+// use the slot kernel, whose wavefronts split a patch).  Bits 8-9 of a
+// patch's op select the sender's array as in k_sync_flat.
+__global__ __launch_bounds__(256) void k_sync_patches(double *__restrict__ arr, const wh_patch *__restrict__ pt,
+                                                      int p0, int p1, size_t blk, size_t cs, int nx, int ny,
+                                                      int neq, SyncSrc S)
+{
+    const int q = p0 + (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (q >= p1) return;
+    const wh_patch P = pt[q];
+    const int sel = (P.op >> 8) & 3;
+    const double *sbase = sel == 0 ? arr + (size_t)P.send * blk
+                                   : S.alt[sel - 1] + (size_t)P.send * S.blk_alt[sel - 1];
+    sync_patch(arr, P, sbase, blk, cs, nx, ny, neq, 2, (int)(threadIdx.x & 63u));
+}
+
 int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const int64_t *d_pstart,
                      int p0, int p1, int64_t npts, const SyncSrc &S)
 {
     if (p1 <= p0 || npts <= 0) return WH_OK;
+    if (P->d.dim == 2) {
+        hipLaunchKernelGGL(k_sync_patches, dim3((unsigned)((p1 - p0 + 3) / 4)), dim3(256), 0, P->stream, arr,
+                           d_patches, p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
+        WH_HIP(hipGetLastError());
+        return WH_OK;
+    }
     const unsigned nblk = (unsigned)((npts + 255) / 256);
     if (P->sync_pad == 64)
         hipLaunchKernelGGL(k_sync_flat<true>, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
@@ -1217,10 +1269,36 @@ __global__ __launch_bounds__(256) void k_sync_multi(const MPoolDev *__restrict__
     }
 }
 
-int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, int sel, int stage,
-                      const int64_t *d_tot, int grid)
+// 2D ghost fill over the patches of several pools (one sync stage s): the
+// stage's patches are the pools' patch ranges [p0[s], p1[s]) one after the
+// other (pool p's from q0[s] on); wavefront w of the launch fills patch
+// w - q0[s] + p0[s] of the pool whose range holds w, as k_sync_patches.
+// Persistent over the patches, their count tot[4 + s] read from device
+// memory.
+__global__ __launch_bounds__(256) void k_sync_multi_patches(const MPoolDev *__restrict__ pools, int npools,
+                                                            int sel, int stage, const int64_t *__restrict__ tot)
 {
-    hipLaunchKernelGGL(k_sync_multi, dim3(grid), dim3(256), 0, stream, d_pools, npools, sel, stage, d_tot);
+    const int s = stage - 1;
+    const int64_t nq = tot[4 + s];
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int64_t w = (int64_t)blockIdx.x * 4 + wv; w < nq; w += (int64_t)gridDim.x * 4) {
+        const MPoolDev &M = pools[__builtin_amdgcn_readfirstlane(pool_of(pools, npools, w, 3 + s))];
+        const wh_patch P = M.pt[M.p0[s] + (int)(w - M.q0[s])];
+        double *arr = M.arr[sel];
+        const Geo &G = M.G;
+        sync_patch(arr, P, arr + (size_t)P.send * G.blk, G.blk, G.cs, G.nx, G.ny, M.neq, 2,
+                   (int)(threadIdx.x & 63u));
+    }
+}
+
+int launch_sync_multi(hipStream_t stream, const MPoolDev *d_pools, int npools, int sel, int stage,
+                      const int64_t *d_tot, int grid, int dim)
+{
+    if (dim == 2)
+        hipLaunchKernelGGL(k_sync_multi_patches, dim3(grid), dim3(256), 0, stream, d_pools, npools, sel, stage,
+                           d_tot);
+    else
+        hipLaunchKernelGGL(k_sync_multi, dim3(grid), dim3(256), 0, stream, d_pools, npools, sel, stage, d_tot);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
