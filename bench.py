@@ -337,7 +337,7 @@ def run_ensemble(args):
     # per lane: an even share plus room for the largest member (first-fit slack)
     slots = {k: lanes * (-(-tot[k] // lanes) + big[k]) for k in tot}
     t0 = time.perf_counter()
-    b = Batch(slots, n_work_slots=3, lanes=lanes)
+    b = Batch(slots, n_work_slots=3, lanes=lanes, parallel=not args.serial)
     solves = []
     for sp in sorted(specs, key=lambda x: -x.slots):      # largest first: packs the lanes
         s = b.admit(sp)
@@ -414,7 +414,8 @@ def run_ensemble(args):
         "steps": steps, "warmup": warmup, "ms_per_step": round(el / steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"c4-ensemble-{len(specs)}x-channel-J4-7-Bs16-24", "members": len(specs),
-                   "lanes": lanes, "blocks_per_step_min": nb_min, "blocks_per_step_max": nb_max,
+                   "lanes": lanes, "host_threads": "serial" if args.serial else "one per pool",
+                   "blocks_per_step_min": nb_min, "blocks_per_step_max": nb_max,
                    "blocks_at_end": nblocks, "wavelet": "CDF44", "scheme": "FD_4th_central", "rk": "RK4"},
         "roofline": roofline, "cpu_baseline": cpu,
         "paths_block_RK_stages_per_s_8_members": plain,
@@ -483,6 +484,9 @@ def main():
                          "own; default 2 -- c5's 3D pool share per lane must hold a Jmax-5 request within "
                          "60%% of HBM)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--serial", action="store_true",
+                    help="c4: step the pools one after the other on the calling thread (the same device work; "
+                         "for kernel tracers)")
     ap.add_argument("--no-paths", action="store_true", help="skip the per-path comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain interior copy of known size (counter calibration)")

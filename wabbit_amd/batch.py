@@ -329,14 +329,17 @@ class Batch:
     pool is refused."""
 
     def __init__(self, slots_per_pool: Dict[tuple, int], n_work_slots: int = 5, lanes: int = 1,
-                 prealloc: Optional[List[tuple]] = None, use_graph: bool = True):
+                 prealloc: Optional[List[tuple]] = None, use_graph: bool = True, parallel: bool = True):
         """slots_per_pool: {(dim, Bs): pool slots} -- the device memory set
         aside per shape (the reference's --memory), split evenly over the
-        lanes."""
+        lanes.  parallel=False steps the pools one after the other on the
+        calling thread (the same device work; for tracing tools that do not
+        follow several host threads calling the runtime)."""
         self.slots_per_pool = dict(slots_per_pool)
         self.n_work_slots = n_work_slots
         self.lanes = max(1, int(lanes))
         self.use_graph = use_graph
+        self.parallel = parallel
         self.groups: Dict[tuple, PoolGroup] = {}
         self.next_rid = 0
         self._exec = None
@@ -389,7 +392,7 @@ class Batch:
         (the groups share nothing but the device); the results in group
         order."""
         gs = list(self.groups.values())
-        if len(gs) <= 1:
+        if len(gs) <= 1 or not self.parallel:
             return [fn(g) for g in gs]
         if self._exec_n < len(gs):
             from concurrent.futures import ThreadPoolExecutor

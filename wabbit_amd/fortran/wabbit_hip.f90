@@ -310,6 +310,84 @@ module wabbit_hip
             integer(c_int32_t), intent(in) :: triples(*)
             integer(c_int) :: rc
         end function
+        ! several forests sharing one pool (forests(1:n): wh_forest handles)
+        function wh_sync_trees(pool, forests, n, array) bind(C, name="wh_sync_trees") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: pool
+            type(c_ptr), intent(in) :: forests(*)
+            integer(c_int), value :: n, array
+            integer(c_int) :: rc
+        end function
+        function wh_adapt_trees(pool, forests, n, params, status) bind(C, name="wh_adapt_trees") result(rc)
+            import :: c_ptr, c_int, wh_adapt_params
+            type(c_ptr), value :: pool
+            type(c_ptr), intent(in) :: forests(*)
+            integer(c_int), value :: n
+            type(wh_adapt_params), intent(in) :: params(*)
+            integer(c_int), intent(out) :: status(*)
+            integer(c_int) :: rc
+        end function
+        function wh_refine_trees(pool, forests, n, jmax, order, array, status) bind(C, name="wh_refine_trees") result(rc)
+            import :: c_ptr, c_int, c_int32_t
+            type(c_ptr), value :: pool
+            type(c_ptr), intent(in) :: forests(*)
+            integer(c_int), value :: n, order, array
+            integer(c_int32_t), intent(in) :: jmax(*)
+            integer(c_int), intent(out) :: status(*)
+            integer(c_int) :: rc
+        end function
+        function wh_create_masks(pool, forests, n, params, status) bind(C, name="wh_create_masks") result(rc)
+            import :: c_ptr, c_int, wh_mask_params
+            type(c_ptr), value :: pool
+            type(c_ptr), intent(in) :: forests(*)
+            integer(c_int), value :: n
+            type(wh_mask_params), intent(in) :: params(*)
+            integer(c_int), intent(out) :: status(*)
+            integer(c_int) :: rc
+        end function
+        function wh_set_tables_from_forests(pool, forests, n, g_rhs, order, status, nblocks) &
+                bind(C, name="wh_set_tables_from_forests") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: pool
+            type(c_ptr), intent(in) :: forests(*)
+            integer(c_int), value :: n, g_rhs, order
+            integer(c_int), intent(out) :: status(*), nblocks(*)
+            integer(c_int) :: rc
+        end function
+        function wh_set_group_params(pool, params, ngroups) bind(C, name="wh_set_group_params") result(rc)
+            import :: c_ptr, c_int, wh_acm_params
+            type(c_ptr), value :: pool
+            type(wh_acm_params), intent(in) :: params(*)
+            integer(c_int), value :: ngroups
+            integer(c_int) :: rc
+        end function
+        ! RungeKuttaGeneric over several pools of one dimension at once
+        function wh_batch_create(batch, pools, npools) bind(C, name="wh_batch_create") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), intent(out) :: batch
+            type(c_ptr), intent(in) :: pools(*)
+            integer(c_int), value :: npools
+            integer(c_int) :: rc
+        end function
+        function wh_batch_destroy(batch) bind(C, name="wh_batch_destroy") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: batch
+            integer(c_int) :: rc
+        end function
+        function wh_batch_rk_step(batch, p, butcher, nrows, dt_groups, use_graph) &
+                bind(C, name="wh_batch_rk_step") result(rc)
+            import :: c_ptr, c_int, c_double, wh_acm_params
+            type(c_ptr), value :: batch
+            type(wh_acm_params), intent(in) :: p
+            real(c_double), intent(in) :: butcher(*), dt_groups(*)
+            integer(c_int), value :: nrows, use_graph
+            integer(c_int) :: rc
+        end function
+        function wh_batch_synchronize(batch) bind(C, name="wh_batch_synchronize") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: batch
+            integer(c_int) :: rc
+        end function
     end interface
 
 contains
