@@ -259,7 +259,7 @@ static void free_patchset(PatchSet &ps)
     if (ps.d_patches) (void)hipFree(ps.d_patches);
     if (ps.d_recv_off) (void)hipFree(ps.d_recv_off);
     if (ps.d_recv_ids) (void)hipFree(ps.d_recv_ids);
-    if (ps.d_pstart) (void)hipFree(ps.d_pstart);
+    if (ps.d_chunks) (void)hipFree(ps.d_chunks);
     ps = PatchSet{};
 }
 
@@ -333,7 +333,6 @@ int wh_pool_create(wh_pool **out, const wh_pool_desc *desc)
     P->ny = d.bs[1] + 2 * d.g;
     P->nz = d.dim == 3 ? d.bs[2] + 2 * d.g : 1;
     P->cs = (size_t)P->nx * P->ny * P->nz;
-    P->sync_pad = d.dim == 3 ? 64 : 1;
     P->blk = P->cs * d.neq;
     P->tblk = P->cs * d.n_tmp_eqn;
     P->mblk = P->cs * d.n_mask;
@@ -524,8 +523,8 @@ int wh_set_active(wh_pool *P, const int32_t *act, int n, const double *x0dx)
 
 // The patch table of a sync case from runs of patches (taken one after the
 // other), without checks: stable counting sort by stage (1..3; within a
-// stage the receivers keep their order), per-receiver rows, slot prefix
-// sums, upload.
+// stage the receivers keep their order), per-receiver rows, the chunks of
+// the 3D ghost fill, upload.
 static int set_patch_runs(wh_pool *P, int sync_case, const PatchRun *runs, int nruns)
 {
     int npatch = 0;
@@ -546,7 +545,7 @@ static int set_patch_runs(wh_pool *P, int sync_case, const PatchRun *runs, int n
     }
     ps.h_recv_off.clear();
     ps.h_recv_ids.clear();
-    ps.h_pstart.assign(npatch + 1, 0);
+    ps.h_chunks.clear();
     for (int s = 0; s < 5; ++s) ps.stage_begin[s] = 0;
     for (int i = 0; i < npatch; ++i) {
         const wh_patch &q = ps.host[i];
@@ -555,28 +554,33 @@ static int set_patch_runs(wh_pool *P, int sync_case, const PatchRun *runs, int n
             ps.h_recv_ids.push_back(q.recv);
         }
         ps.max_stage = std::max(ps.max_stage, (int)q.stage);
-        ps.h_pstart[i + 1] = ps.h_pstart[i] + sync_slots(P, q);
+        if (P->d.dim == 3) add_chunks(ps.h_chunks, i, q);  // 2D: one wavefront per patch, no chunks
     }
-    {  // stage ranges over the (stage, receiver)-sorted table
-        int i = 0;
+    {  // stage ranges over the (stage, receiver)-sorted table, in patches and in chunks
+        int i = 0, c = 0;
         for (int s = 1; s <= 3; ++s) {
             ps.stage_begin[s] = i;
-            while (i < npatch && ps.host[i].stage == s) ++i;
+            ps.chunk_begin[s] = c;
+            while (i < npatch && ps.host[i].stage == s) {
+                const wh_patch &q = ps.host[i++];
+                if (P->d.dim == 3) c += (q.n[0] * q.n[1] * q.n[2] + SYNC_CHUNK - 1) / SYNC_CHUNK;
+            }
         }
         ps.stage_begin[4] = npatch;
+        ps.chunk_begin[4] = c;
     }
     ps.h_recv_off.push_back(npatch);
     ps.n = npatch;
     ps.nrecv = (int)ps.h_recv_ids.size();
     if (npatch) {
         if ((size_t)npatch > ps.cap_patches || ps.h_recv_off.size() > ps.cap_recv_off ||
-            ps.h_recv_ids.size() > ps.cap_recv_ids || ps.h_pstart.size() > ps.cap_pstart)
+            ps.h_recv_ids.size() > ps.cap_recv_ids || ps.h_chunks.size() > ps.cap_chunks)
             WH_HIP(hipStreamSynchronize(P->stream));  // before freeing buffers queued work may read
         if (int rc = grow_dev(ps.d_patches, ps.cap_patches, (size_t)npatch)) return rc;
         if (int rc = grow_dev(ps.d_recv_off, ps.cap_recv_off, ps.h_recv_off.size())) return rc;
         if (int rc = grow_dev(ps.d_recv_ids, ps.cap_recv_ids, ps.h_recv_ids.size())) return rc;
-        if (int rc = grow_dev(ps.d_pstart, ps.cap_pstart, ps.h_pstart.size())) return rc;
-        if (int rc = upload_staged(P, ps.d_pstart, ps.h_pstart.data(), sizeof(int64_t) * ps.h_pstart.size()))
+        if (int rc = grow_dev(ps.d_chunks, ps.cap_chunks, ps.h_chunks.size())) return rc;
+        if (int rc = upload_staged(P, ps.d_chunks, ps.h_chunks.data(), sizeof(SyncChunk) * ps.h_chunks.size()))
             return rc;
         if (int rc = upload_staged(P, ps.d_patches, ps.host.data(), sizeof(wh_patch) * npatch)) return rc;
         if (int rc = upload_staged(P, ps.d_recv_off, ps.h_recv_off.data(), sizeof(int32_t) * ps.h_recv_off.size()))

@@ -35,7 +35,7 @@ struct wh_batch {
     std::vector<hipEvent_t> ev_pool;
     int n_cu = 256;
     // device tables: one descriptor per pool, dt and constants per block
-    // group (capacity cap_grp), counts ([0] entries, [s] slots and [3 + s]
+    // group (capacity cap_grp), counts ([0] entries, [s] chunks and [3 + s]
     // patches of sync stage s)
     MPoolDev *d_pools = nullptr;
     wh::MEnt *d_ents = nullptr;  // entry records, one per slot of the pools
@@ -298,20 +298,24 @@ int wh_batch_rk_step(wh_batch *B, const wh_acm_params *p, const double *bt, int 
         m.x0dx = P->d_x0dx;
         m.grp = P->d_grp;
         m.pt = ps.d_patches;
-        m.pstart = ps.d_pstart;
+        m.ch = ps.d_chunks;
         m.ent0 = nent;
         m.goff = goff;
         for (int st = 1; st <= 3; ++st) {
             const int a = ps.n ? ps.stage_begin[st] : 0, b = ps.n ? ps.stage_begin[st + 1] : 0;
+            const int ca = ps.n ? ps.chunk_begin[st] : 0, cb = ps.n ? ps.chunk_begin[st + 1] : 0;
             m.p0[st - 1] = a;
             m.p1[st - 1] = b;
+            m.c0[st - 1] = ca;
+            m.c1[st - 1] = cb;
             m.s0[st - 1] = tot[st];
             m.q0[st - 1] = tot[3 + st];
             if (P->nact > 0 && b > a) {
-                tot[st] += ps.h_pstart[b] - ps.h_pstart[a];
+                tot[st] += cb - ca;
                 tot[3 + st] += b - a;
-            } else {
-                m.p1[st - 1] = a;  // a pool without blocks fills no ghosts
+            } else {  // a pool without blocks fills no ghosts
+                m.p1[st - 1] = a;
+                m.c1[st - 1] = ca;
             }
         }
         if (P->nact > 0) {

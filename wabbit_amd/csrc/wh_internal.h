@@ -56,15 +56,25 @@ struct Geo {
     int mask_per_block;
 };
 
+// A 3D ghost patch holds up to a few thousand points; the ghost fill cuts
+// every patch into chunks of SYNC_CHUNK points (its first point `first`),
+// one wavefront per chunk (four points per lane), so that the work spreads
+// evenly and a wavefront finds its patch with one load.
+constexpr int SYNC_CHUNK = 256;
+struct SyncChunk {
+    int32_t patch, first;
+};
+
 // One pool of a multi-pool launch: its arrays (0 hvy_block, 1..3 work slots
 // 0..2), its mask (or null) and geometry, and where the launch finds the
 // pool's own device tables -- active list, origin/spacing and block group
 // per active position, the RHS ghost-patch table with its slot prefix sum --
 // and the pool's place in the launch: its first entry (entries are the
 // pools' active lists one after the other), its first block group, and per
-// sync stage s its patch range [p0[s], p1[s]), its first slot s0[s] in the
-// stage's slots (the pools' slot ranges one after the other) and its first
-// patch q0[s] in the stage's patches (the pools' ranges one after the other).
+// sync stage s its patch range [p0[s], p1[s]) and chunk range [c0[s],
+// c1[s]), and where these start in the launch: its first chunk s0[s] among
+// the stage's chunks and its first patch q0[s] among the stage's patches
+// (the pools' ranges one after the other).
 struct MPoolDev {
     double *arr[4];
     const double *mask;
@@ -74,9 +84,9 @@ struct MPoolDev {
     const double *x0dx;
     const int32_t *grp;
     const wh_patch *pt;
-    const int64_t *pstart;
+    const SyncChunk *ch;
     int ent0, goff;
-    int p0[3], p1[3];
+    int p0[3], p1[3], c0[3], c1[3];
     int64_t s0[3], q0[3];
 };
 
@@ -119,8 +129,6 @@ struct WaveletWS {
     size_t ids_cap = 0;
     wh_patch *d_patches = nullptr;
     size_t patch_cap = 0;
-    int64_t *d_pstart = nullptr;
-    size_t pstart_cap = 0;
     wh_box *d_boxes = nullptr;
     size_t box_cap = 0;
     double *d_out = nullptr;
@@ -147,14 +155,15 @@ struct PatchSet {
     wh_patch *d_patches = nullptr;
     int32_t *d_recv_off = nullptr;  // CSR over (stage, receiver) rows
     int32_t *d_recv_ids = nullptr;  // receiver hvy id per CSR row
-    int64_t *d_pstart = nullptr;    // prefix sum of patch point counts
+    SyncChunk *d_chunks = nullptr;   // the patches cut into wavefront-sized chunks
     std::vector<int32_t> h_recv_off, h_recv_ids;
-    std::vector<int64_t> h_pstart;
+    std::vector<SyncChunk> h_chunks;
     int stage_begin[5] = {0, 0, 0, 0, 0};  // stage s (1..3): patches [stage_begin[s], stage_begin[s+1])
+    int chunk_begin[5] = {0, 0, 0, 0, 0};  // and chunks [chunk_begin[s], chunk_begin[s+1])
     int n = 0, nrecv = 0;
     int max_stage = 0;
     // device capacities (element counts); buffers are reused until they must grow
-    size_t cap_patches = 0, cap_recv_off = 0, cap_recv_ids = 0, cap_pstart = 0;
+    size_t cap_patches = 0, cap_recv_off = 0, cap_recv_ids = 0, cap_chunks = 0;
 };
 
 }  // namespace wh
@@ -201,7 +210,6 @@ struct wh_pool {
     int k3d_zsplit = 1;
     int k3d_waves = 1;  // 3: the z-marching kernel built for 3 waves per SIMD, 4: the flat one for 4
     int k3d_xcd = 0;    // 1: the flat 3D kernel's workgroups swizzled per XCD
-    int sync_pad = 1;   // ghost-fill slots per patch rounded up to this (1 or 64), sync_slots
     // tuning of the 2D stage kernel: 0 plain, 1 mask and epilogue operands
     // loaded before the tile barrier, 2..4 persistent pipelined kernel with
     // 2, 4, 8 workgroups per CU
@@ -246,15 +254,18 @@ int launch_dt_finalize(wh_pool *P, const TimeDev &tc);
 int launch_time_advance(wh_pool *P);
 int launch_set_scalar(wh_pool *P, int which, double v);
 
-// slots of a patch in the ghost fill's numbering: its point count, rounded
-// up to whole wavefronts where the pool pads (3D)
-inline int64_t sync_slots(const wh_pool *P, const wh_patch &q)
+// the chunks of patch i of a list (its points cut into pieces of
+// SYNC_CHUNK), appended in point order
+inline void add_chunks(std::vector<SyncChunk> &out, int i, const wh_patch &q)
 {
-    const int64_t n = (int64_t)q.n[0] * q.n[1] * q.n[2];
-    return P->sync_pad > 1 ? (n + P->sync_pad - 1) / P->sync_pad * P->sync_pad : n;
+    const int n = q.n[0] * q.n[1] * q.n[2];
+    for (int r = 0; r < n; r += SYNC_CHUNK) out.push_back(SyncChunk{i, r});
 }
-int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const int64_t *d_pstart,
-                     int p0, int p1, int64_t npts, const SyncSrc &S);
+// ghost fill of one sync stage of a device patch list: 2D the patches
+// [p0, p1), one wavefront per patch; 3D the chunks [c0, c1), one wavefront
+// per chunk
+int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const SyncChunk *d_chunks, int p0, int p1,
+                     int c0, int c1, const SyncSrc &S);
 
 // wavelet workspace (wh_wavelet.hip)
 void free_wavelet_ws(wh_pool *P);
@@ -311,7 +322,7 @@ int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<doub
 AcmDev make_acm_dev(const wh_acm_params *p);
 Geo make_geo(const wh_pool *P);
 // multi-pool launches (independent forests of a batch, wh_batch.hip)
-// d_tot: [0] entries of the launch, [s] slots of sync stage s (1..3), [3 + s]
+// d_tot: [0] entries of the launch, [s] chunks of sync stage s (1..3), [3 + s]
 // patches of sync stage s; dtg /
 // gacm: dt and constants per block group of the launch; pipe: the 2D
 // software-pipelined kernel (tile widths and heights <= 32)

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void k_rhs2d(const double *__restrict__ in, do
 // The kernels are persistent (grid-stride over the entries, their count read
 // from device memory), so one captured launch serves any number of blocks.
 // ---------------------------------------------------------------------------
-// the last pool whose first entry (s < 0), first slot of sync stage s (0..2) or
+// the last pool whose first entry (s < 0), first chunk of sync stage s (0..2) or
 // first patch of sync stage s - 3 (3..5) is <= x
 __device__ __forceinline__ int pool_of(const MPoolDev *__restrict__ pools, int npools, int64_t x, int s)
 {
@@ -1077,88 +1077,21 @@ __device__ __forceinline__ double pred_any(const double *u, int nx, int ny, int 
     return pred_point<3>(u, nx, ny, dim, f);
 }
 
-// Flat ghost fill: one thread per ghost point of the patches [p0, p1); the
-// patch of a point is found by binary search in the prefix sum of the
-// patches' slot counts.  WAVE (slot counts padded to whole wavefronts of 64,
-// sync_slots): every wavefront lies in one patch, found once per wavefront
-// and read through the scalar unit; lanes past the patch's points idle.
-// Otherwise (slot count = point count) each lane searches between the
-// patches of its workgroup's first and last points.
-// Bits 8-9 of a patch's op select the array its sender is read from (0: the
-// synced array itself, 1/2: the alternates in S).
-template <bool WAVE>
-__global__ __launch_bounds__(256) void k_sync_flat(double *__restrict__ arr,
-                                                   const wh_patch *__restrict__ pt,
-                                                   const int64_t *__restrict__ pstart, int p0,
-                                                   int p1, size_t blk, size_t cs, int nx, int ny,
-                                                   int neq, SyncSrc S)
-{
-    // last patch in [lo, hi] with pstart <= x
-    auto find = [&](int64_t x, int lo, int hi) {
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pstart[mid] <= x) lo = mid;
-            else hi = mid - 1;
-        }
-        return lo;
-    };
-    int lo;
-    int64_t e;
-    if constexpr (WAVE) {
-        const int64_t wv = pstart[p0] + (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-        const int64_t ew = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(wv >> 32)) << 32) |
-                                     (uint32_t)__builtin_amdgcn_readfirstlane((int)wv));
-        if (ew >= pstart[p1]) return;
-        lo = find(ew, p0, p1 - 1);
-        e = ew + (threadIdx.x & 63u);
-    } else {
-        // the workgroup's first and last points and their patches (uniform),
-        // then each lane's patch between those two
-        const int64_t w0 = pstart[p0] + (int64_t)blockIdx.x * blockDim.x;
-        const int64_t w1 = min(w0 + (int64_t)blockDim.x, pstart[p1]) - 1;
-        const int pa = find(w0, p0, p1 - 1);
-        const int pb = find(w1, pa, p1 - 1);
-        e = w0 + threadIdx.x;
-        if (e > w1) return;
-        lo = find(e, pa, pb);
-    }
-    const wh_patch P = pt[lo];
-    const int r = (int)(e - pstart[lo]);
-    const int n0 = P.n[0], n1 = P.n[1];
-    if (WAVE && r >= n0 * n1 * P.n[2]) return;  // padding slot
-    const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
-    const size_t ri =
-        ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx + (P.recv_lo[0] + i);
-    double *dst = arr + (size_t)P.recv * blk + ri;
-    const int op = P.op & 0xff, sel = (P.op >> 8) & 3;
-    const double *sbase = sel == 0 ? arr + (size_t)P.send * blk
-                                   : S.alt[sel - 1] + (size_t)P.send * S.blk_alt[sel - 1];
-    if (op == WH_PATCH_PREDICT) {
-        const int f[3] = {P.send_lo[0] + i, P.send_lo[1] + j, P.send_lo[2] + kk};
-        const int dim = (cs == (size_t)nx * ny) ? 2 : 3;
-        for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(sbase + c * cs, nx, ny, dim, f, P.aux);
-        return;
-    }
-    const int st = (op == WH_PATCH_RESTRICT) ? 2 : 1;
-    const size_t si = ((size_t)(P.send_lo[2] + st * kk) * ny + (P.send_lo[1] + st * j)) * nx +
-                      (P.send_lo[0] + st * i);
-    const double *src = sbase + si;
-    for (int c = 0; c < neq; ++c) dst[c * cs] = src[c * cs];
-}
-
-// The points of one ghost patch, taken by the 64 lanes of a wavefront in
-// steps of 64 (x fastest): per point the copy / decimation / prediction of
-// k_sync_flat.  P is the same in every lane (read through the scalar unit);
-// sbase: block P.send of the sender array.
+// The points [r0, r_end) of one ghost patch (x fastest), taken by the 64
+// lanes of a wavefront in steps of 64 (lane l starts at r0 + l): per point
+// the copy of a same-level sender, the decimation of a finer one (every
+// other point) or the prediction of a coarser one (pred_any), every
+// component, into the receiver's ghost nodes.  P is the same in every lane
+// (read through the scalar unit); sbase: block P.send of the sender array.
 __device__ __forceinline__ void sync_patch(double *__restrict__ arr, const wh_patch &P,
                                            const double *__restrict__ sbase, size_t blk, size_t cs, int nx, int ny,
-                                           int neq, int dim, int lane)
+                                           int neq, int dim, int r0, int r_end = 1 << 30)
 {
-    const int n0 = P.n[0], n1 = P.n[1], np = n0 * n1 * P.n[2];
+    const int n0 = P.n[0], n1 = P.n[1], np = min(n0 * n1 * P.n[2], r_end);
     double *dbase = arr + (size_t)P.recv * blk;
     const int op = P.op & 0xff;
     const int st = (op == WH_PATCH_RESTRICT) ? 2 : 1;
-    for (int r = lane; r < np; r += 64) {
+    for (int r = r0; r < np; r += 64) {
         const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
         double *dst = dbase + ((size_t)(P.recv_lo[2] + kk) * ny + (P.recv_lo[1] + j)) * nx + (P.recv_lo[0] + i);
         if (op == WH_PATCH_PREDICT) {
@@ -1176,8 +1109,9 @@ __device__ __forceinline__ void sync_patch(double *__restrict__ arr, const wh_pa
 // p0 + w for wavefront w of the launch), its record read once through the
 // scalar unit -- no search; a 2D ghost patch has a few dozen points, so the
 // wavefront takes them in one or two steps (3D patches are far larger and
-// use the slot kernel, whose wavefronts split a patch).  Bits 8-9 of a
-// patch's op select the sender's array as in k_sync_flat.
+// are cut into chunks, k_sync_chunks).  Bits 8-9 of a patch's op select the
+// array its sender is read from (0: the synced array itself, 1/2: the
+// alternates in S).
 __global__ __launch_bounds__(256) void k_sync_patches(double *__restrict__ arr, const wh_patch *__restrict__ pt,
                                                       int p0, int p1, size_t blk, size_t cs, int nx, int ny,
                                                       int neq, SyncSrc S)
@@ -1191,81 +1125,59 @@ __global__ __launch_bounds__(256) void k_sync_patches(double *__restrict__ arr, 
     sync_patch(arr, P, sbase, blk, cs, nx, ny, neq, 2, (int)(threadIdx.x & 63u));
 }
 
-int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const int64_t *d_pstart,
-                     int p0, int p1, int64_t npts, const SyncSrc &S)
+// 3D ghost fill of the chunks [c0, c1): one wavefront per chunk (chunk c0 + w
+// for wavefront w of the launch), chunk and patch records read once through
+// the scalar unit, lane l takes points first + l, first + l + 64, ... of the
+// chunk.
+__global__ __launch_bounds__(256) void k_sync_chunks(double *__restrict__ arr, const wh_patch *__restrict__ pt,
+                                                     const SyncChunk *__restrict__ ch, int c0, int c1, size_t blk,
+                                                     size_t cs, int nx, int ny, int neq, SyncSrc S)
 {
-    if (p1 <= p0 || npts <= 0) return WH_OK;
+    const int q = c0 + (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (q >= c1) return;
+    const SyncChunk c = ch[q];
+    const wh_patch P = pt[c.patch];
+    const int sel = (P.op >> 8) & 3;
+    const double *sbase = sel == 0 ? arr + (size_t)P.send * blk
+                                   : S.alt[sel - 1] + (size_t)P.send * S.blk_alt[sel - 1];
+    sync_patch(arr, P, sbase, blk, cs, nx, ny, neq, 3, c.first + (int)(threadIdx.x & 63u), c.first + SYNC_CHUNK);
+}
+
+int launch_sync_flat(wh_pool *P, double *arr, const wh_patch *d_patches, const SyncChunk *d_chunks, int p0, int p1,
+                     int c0, int c1, const SyncSrc &S)
+{
     if (P->d.dim == 2) {
-        hipLaunchKernelGGL(k_sync_patches, dim3((unsigned)((p1 - p0 + 3) / 4)), dim3(256), 0, P->stream, arr,
-                           d_patches, p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
-        WH_HIP(hipGetLastError());
-        return WH_OK;
+        if (p1 > p0)
+            hipLaunchKernelGGL(k_sync_patches, dim3((unsigned)((p1 - p0 + 3) / 4)), dim3(256), 0, P->stream, arr,
+                               d_patches, p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
+    } else if (c1 > c0) {
+        hipLaunchKernelGGL(k_sync_chunks, dim3((unsigned)((c1 - c0 + 3) / 4)), dim3(256), 0, P->stream, arr,
+                           d_patches, d_chunks, c0, c1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
     }
-    const unsigned nblk = (unsigned)((npts + 255) / 256);
-    if (P->sync_pad == 64)
-        hipLaunchKernelGGL(k_sync_flat<true>, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
-                           p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
-    else
-        hipLaunchKernelGGL(k_sync_flat<false>, dim3(nblk), dim3(256), 0, P->stream, arr, d_patches, d_pstart,
-                           p0, p1, P->blk, P->cs, P->nx, P->ny, P->d.neq, S);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
 
-// Ghost fill over the patches of several pools (one sync stage s): the
-// stage's slots are the pools' slot ranges one after the other (pool p from
-// s0[s]); slot e of pool p is slot e - s0[s] + pstart[p0[s]] of that pool's
-// own patch table, inside patch q = the last with pstart[q] <= it (padding
-// slots past a patch's points idle).  One thread per slot, persistent over
-// the slots, their count read from device memory.  Per point the copy /
-// decimation / prediction of k_sync_flat, into array `sel` of the pool.
+// 3D ghost fill over the chunks of several pools (one sync stage s): the
+// stage's chunks are the pools' chunk ranges [c0[s], c1[s]) one after the
+// other (pool p's from s0[s] on); wavefront w of the launch fills chunk
+// w - s0[s] + c0[s] of the pool whose range holds w, as k_sync_chunks.
+// Persistent over the chunks, their count tot[stage] read from device
+// memory; into array `sel` of the pool.
 __global__ __launch_bounds__(256) void k_sync_multi(const MPoolDev *__restrict__ pools, int npools, int sel,
                                                     int stage, const int64_t *__restrict__ tot)
 {
     const int s = stage - 1;
-    const int64_t end = tot[stage];
-    auto find = [](const int64_t *ps, int64_t x, int lo, int hi) {
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (ps[mid] <= x) lo = mid;
-            else hi = mid - 1;
-        }
-        return lo;
-    };
-    for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x; w0 < end; w0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t w1 = min(w0 + (int64_t)blockDim.x, end) - 1;
-        const int pa = pool_of(pools, npools, w0, s), pb = pool_of(pools, npools, w1, s);
-        const int64_t e = w0 + threadIdx.x;
-        if (e > w1) continue;
-        const MPoolDev &M = pools[pa == pb ? pa : pool_of(pools, npools, e, s)];
-        const int64_t shift = M.pstart[M.p0[s]] - M.s0[s];  // pool slot of launch slot e: e + shift
-        int lo = M.p0[s], hi = M.p1[s] - 1;
-        if (pa == pb) {  // the workgroup's slots lie in one pool: narrow the search once
-            lo = find(M.pstart, w0 + shift, lo, hi);
-            hi = find(M.pstart, w1 + shift, lo, hi);
-        }
-        const int q = find(M.pstart, e + shift, lo, hi);
-        const wh_patch P = M.pt[q];
-        const Geo &G = M.G;
-        const int r = (int)(e + shift - M.pstart[q]);
-        const int n0 = P.n[0], n1 = P.n[1];
-        if (r >= n0 * n1 * P.n[2]) continue;  // padding slot
-        const int i = r % n0, j = (r / n0) % n1, kk = r / (n0 * n1);
+    const int64_t nc = tot[stage];
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int64_t w = (int64_t)blockIdx.x * 4 + wv; w < nc; w += (int64_t)gridDim.x * 4) {
+        const MPoolDev &M = pools[__builtin_amdgcn_readfirstlane(pool_of(pools, npools, w, s))];
+        const SyncChunk c = M.ch[M.c0[s] + (int)(w - M.s0[s])];
+        const wh_patch P = M.pt[c.patch];
         double *arr = M.arr[sel];
-        const size_t ri = ((size_t)(P.recv_lo[2] + kk) * G.ny + (P.recv_lo[1] + j)) * G.nx + (P.recv_lo[0] + i);
-        double *dst = arr + (size_t)P.recv * G.blk + ri;
-        const double *sbase = arr + (size_t)P.send * G.blk;
-        const int op = P.op & 0xff;
-        if (op == WH_PATCH_PREDICT) {
-            const int f[3] = {P.send_lo[0] + i, P.send_lo[1] + j, P.send_lo[2] + kk};
-            for (int c = 0; c < M.neq; ++c)
-                dst[c * G.cs] = pred_any(sbase + c * G.cs, G.nx, G.ny, M.dim, f, P.aux);
-            continue;
-        }
-        const int stp = (op == WH_PATCH_RESTRICT) ? 2 : 1;
-        const size_t si = ((size_t)(P.send_lo[2] + stp * kk) * G.ny + (P.send_lo[1] + stp * j)) * G.nx +
-                          (P.send_lo[0] + stp * i);
-        for (int c = 0; c < M.neq; ++c) dst[c * G.cs] = sbase[si + c * G.cs];
+        const Geo &G = M.G;
+        sync_patch(arr, P, arr + (size_t)P.send * G.blk, G.blk, G.cs, G.nx, G.ny, M.neq, 3,
+                   c.first + (int)(threadIdx.x & 63u), c.first + SYNC_CHUNK);
     }
 }
 
@@ -1683,8 +1595,9 @@ int launch_sync(wh_pool *P, int sync_case, double *arr, int one_receiver)
     for (int stage = 1; stage <= 3; ++stage) {
         const int p0 = ps.stage_begin[stage], p1 = ps.stage_begin[stage + 1];
         if (p1 <= p0) continue;
-        const int64_t npts = ps.h_pstart[p1] - ps.h_pstart[p0];
-        if (int rc = launch_sync_flat(P, arr, ps.d_patches, ps.d_pstart, p0, p1, npts, SyncSrc{})) return rc;
+        if (int rc = launch_sync_flat(P, arr, ps.d_patches, ps.d_chunks, p0, p1, ps.chunk_begin[stage],
+                                      ps.chunk_begin[stage + 1], SyncSrc{}))
+            return rc;
     }
     WH_HIP(hipGetLastError());
     return WH_OK;

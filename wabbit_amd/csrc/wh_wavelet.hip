@@ -976,7 +976,6 @@ void free_wavelet_ws(wh_pool *P)
     if (W.d_stage) (void)hipFree(W.d_stage);
     if (W.d_ids) (void)hipFree(W.d_ids);
     if (W.d_patches) (void)hipFree(W.d_patches);
-    if (W.d_pstart) (void)hipFree(W.d_pstart);
     if (W.d_boxes) (void)hipFree(W.d_boxes);
     if (W.d_out) (void)hipFree(W.d_out);
     W = WaveletWS{};
@@ -1234,19 +1233,25 @@ int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<
             }
         }
     }
-    std::vector<int64_t> pstart(n + 1, 0);
-    for (int i = 0; i < n; ++i) pstart[i + 1] = pstart[i] + sync_slots(P, p[i]);
-    if (int rc = stage_begin(P, sizeof(wh_patch) * n + sizeof(int64_t) * (n + 1))) return rc;
+    // 3D: the patches cut into wavefront-sized chunks (cstart[i]: patch i's first)
+    std::vector<SyncChunk> chunks;
+    std::vector<int> cstart(n + 1, 0);
+    if (P->d.dim == 3)
+        for (int i = 0; i < n; ++i) {
+            add_chunks(chunks, i, p[i]);
+            cstart[i + 1] = (int)chunks.size();
+        }
+    if (int rc = stage_begin(P, sizeof(wh_patch) * n + sizeof(SyncChunk) * chunks.size())) return rc;
     const wh_patch *d_patches = (const wh_patch *)stage_put(P, p.data(), sizeof(wh_patch) * n, true);
-    const int64_t *d_pstart = (const int64_t *)stage_put(P, pstart.data(), sizeof(int64_t) * (n + 1), true);
+    const SyncChunk *d_chunks =
+        chunks.empty() ? nullptr : (const SyncChunk *)stage_put(P, chunks.data(), sizeof(SyncChunk) * chunks.size(), true);
     if (int rc = stage_flush(P)) return rc;
-    if (!d_patches || !d_pstart) return WH_E_HIP;
+    if (!d_patches || (!chunks.empty() && !d_chunks)) return WH_E_HIP;
     int s0 = 0;
     while (s0 < n) {
         int s1 = s0;
         while (s1 < n && p[s1].stage == p[s0].stage) ++s1;
-        if (int rc = launch_sync_flat(P, arr, d_patches, d_pstart, s0, s1, pstart[s1] - pstart[s0], S))
-            return rc;
+        if (int rc = launch_sync_flat(P, arr, d_patches, d_chunks, s0, s1, cstart[s0], cstart[s1], S)) return rc;
         s0 = s1;
     }
     return WH_OK;
