@@ -1,0 +1,113 @@
+"""GPU: the multi-forest mesh calls of a batch step against the same work
+done forest by forest through the single-forest entry points.
+
+- wh_set_tables_from_forests: the RK step on tables built from three forests
+  (active list, origins/spacings, one block group per forest, RHS ghost
+  patches) equals, bit for bit, the step on tables set from each forest's
+  wh_forest_get_blocks / wh_forest_patches lists concatenated on the host.
+- wh_refine_trees: refining every leaf below each forest's Jmax in one call
+  gives the block lists and block data of wh_forest_refine + wh_refine_blocks
+  per forest; a forest without the free slots for all its daughters reports
+  WH_E_NOMEM and is left unchanged, while the others are refined.
+"""
+import numpy as np
+import pytest
+
+from helpers import random_block_data, random_graded_forest
+from wabbit_amd import lib as L
+from wabbit_amd import wavelets as W
+from wabbit_amd.params import ACMParams, butcher_rk4
+from wabbit_amd.tree import Forest, refine_blocks
+
+pytestmark = pytest.mark.gpu
+
+BS, G, G_RHS, ORDER, JMAX = 16, 6, 2, 4, 5
+SEEDS = (3, 4, 5)
+
+
+def _forests(slots_each, seeds=SEEDS, short=None):
+    """Forests on disjoint slot ranges [i * slots_each, ...) of one pool; a
+    forest listed in `short` gets only as many slots as it has leaves."""
+    out = []
+    for i, sd in enumerate(seeds):
+        tcs, lev = random_graded_forest(2, 2, JMAX, sd, nref=6)
+        n = tcs.shape[0]
+        nb = n if short is not None and i in short else slots_each
+        f = Forest(2, JMAX, [BS, BS], G, [1.0, 1.0], nb_max=nb, slot_base=i * slots_each)
+        f.set_blocks(tcs, lev, i * slots_each + np.arange(n, dtype=np.int32))
+        out.append(f)
+    return out
+
+
+def _pool(nslots, data):
+    p = L.Pool(2, [BS, BS], G, 3, nslots, n_work_slots=3)
+    p.upload(L.WH_BLOCK, 0, data)
+    return p
+
+
+def test_set_tables_from_forests_equals_host_tables():
+    slots = 256
+    nslots = slots * len(SEEDS)
+    data = 0.1 * random_block_data(nslots, 3, 2, BS, G, 11)
+    acm = ACMParams(dim=2, scheme=4, skew_symmetry=False, penalization=False, c_0=10.0, nu=1e-2,
+                    gamma_p=1.0, CFL=0.5, domain_size=[1.0, 1.0, 1.0])
+    ac = L.acm_params_c(acm)
+    dts = 1e-4 * (1.0 + np.arange(len(SEEDS)))
+    fa, fb = _forests(slots), _forests(slots)
+    pa, pb = _pool(nslots, data), _pool(nslots, data)
+    try:
+        res, nbs = W.set_tables_from_forests(pa, fa, G_RHS, ORDER)
+        assert [c for c, _ in res] == [0] * len(SEEDS)
+        assert nbs == [f.n for f in fa]
+        hv, x, grp, pts = [], [], [], []
+        for k, f in enumerate(fb):
+            _, _, h, xx = f.blocks()
+            hv.append(h)
+            x.append(xx)
+            grp.append(np.full(h.size, k, np.int32))
+            pts.append(f.patches(G_RHS, G_RHS, ORDER))
+        pb.set_active(np.concatenate(hv), np.concatenate(x))
+        pb.set_patches(0, np.concatenate(pts))
+        pb.set_block_groups(np.concatenate(grp), len(SEEDS))
+        print(f"{sum(nbs)} blocks in {len(SEEDS)} forests, {sum(p.size for p in pts)} RHS patches", flush=True)
+        for p in (pa, pb):
+            p.rk_step_groups(ac, butcher_rk4(), dts, path=2)
+        got, want = pa.download(L.WH_BLOCK, 0, nslots), pb.download(L.WH_BLOCK, 0, nslots)
+        act = np.concatenate(hv)
+        inner = (act, slice(None), 0, slice(G, G + BS), slice(G, G + BS))
+        assert not np.array_equal(got[inner], data[inner])
+        assert np.array_equal(got[inner], want[inner])
+    finally:
+        for o in fa + fb + [pa, pb]:
+            o.close()
+
+
+def test_refine_trees_matches_per_forest_refine():
+    slots = 512
+    nslots = slots * len(SEEDS)
+    data = 0.1 * random_block_data(nslots, 3, 2, BS, G, 12)
+    fa, fb = _forests(slots, short={2}), _forests(slots, short={2})
+    pa, pb = _pool(nslots, data), _pool(nslots, data)
+    try:
+        before = [f.blocks()[:3] for f in fa]
+        res = W.refine_trees(pa, fa, [JMAX] * len(SEEDS), ORDER)
+        codes = [c for c, _ in res]
+        print("refine_trees status:", codes, flush=True)
+        assert codes == [0, 0, L.WH_E_NOMEM]
+        after = fa[2].blocks()[:3]
+        assert all(np.array_equal(a, b) for a, b in zip(before[2], after))
+        tris = []
+        for f in fb[:2]:
+            tc, lv, hv, _ = f.blocks()
+            tris.append(f.refine(np.asarray(hv[lv < JMAX], np.int32)))
+        refine_blocks(pb, np.concatenate(tris), ORDER)
+        for a, b in zip(fa, fb):
+            ba, bb = a.blocks(), b.blocks()
+            assert all(np.array_equal(x, y) for x, y in zip(ba, bb))
+        got, want = pa.download(L.WH_BLOCK, 0, nslots), pb.download(L.WH_BLOCK, 0, nslots)
+        ids = np.concatenate([f.blocks()[2] for f in fa])
+        assert np.array_equal(got[ids], want[ids])
+        assert sum(f.n for f in fa[:2]) > sum(b[0].size for b in before[:2])
+    finally:
+        for o in fa + fb + [pa, pb]:
+            o.close()
