@@ -42,10 +42,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# bytes beyond L2 per block-stage of the fused 2D stage kernel on c2, per
-# stage-kernel variant (WH_TUNE_K2D_STAGE), from the FETCH_SIZE / WRITE_SIZE
-# passes scaled by the copy calibration (profiles/c2_fused_stage_hbm_traffic.md)
-C2_STAGE_TRAFFIC_PER_BLOCK = {1: 40638, 3: 40510}
+LLC_BYTES = 256 << 20  # MI355X last-level cache (Infinity Cache)
+
+
+def frac_kind(working_set):
+    """What the roofline fraction measures: an HBM fraction only where the
+    stage's arrays exceed the last-level cache."""
+    return "hbm" if working_set > LLC_BYTES else "effective bandwidth (working set within the 256 MB LLC)"
+
+
+def stage_working_set(nblocks, bs, g, dim, q):
+    """Bytes of the arrays a fused stage touches: input, accumulator, state
+    and output, ghost-including blocks of q components."""
+    return 4 * nblocks * q * (bs + 2 * g) ** dim * 8
 
 
 def b_min(q, bs, dim, g_r, m):
@@ -219,16 +228,14 @@ def run_adaptive(args, w):
     bpb = b_min(q, w["bs"], dim, 2, m)
     bpl = s.rk_blocks
     achieved = bpb * bpl / (ks_ms * 1e-3) / 1e9
-    variant = s.tuned[0] if getattr(s, "tuned", None) is not None else None
-    per_block = C2_STAGE_TRAFFIC_PER_BLOCK.get(variant) if w["kind"] == "adaptive" and dim == 2 else None
-    traffic = round(per_block * bpl) if per_block is not None else None
+    ws = stage_working_set(bpl, w["bs"], s.g, dim, q)
+    # traffic: the counter passes of the committed profile (they cannot be read inside the timed run)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, frac_of=frac_kind(ws),
+                    working_set_bytes=ws,
                     kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(ks_ms, 5), launches=4 * NREP,
                     bytes_per_block_stage=bpb, blocks_per_launch=bpl,
                     timing_pass_event_pair_ms=round(k_ms, 5), timing_pass_blocks_per_launch=round(st2 / max(k_n, 1), 1))
-    if traffic is not None:
-        roofline["traffic_per_block_stage"] = per_block
 
     if getattr(s, "tuned", None) is not None:
         roofline["tuned_variant"] = s.tuned[0]
@@ -261,6 +268,8 @@ def run_adaptive(args, w):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phase_ms_per_step": phases,
+        # the RK step alone (the profiled pass's rk phase, synchronised)
+        "stepper_only_block_RK_stages_per_s": round(st2 / nprof / (phases["rk"] * 1e-3), 1) if phases.get("rk") else None,
     }
     print(json.dumps(out), flush=True)
     s.close()
@@ -383,7 +392,7 @@ def run_ensemble(args):
                     kernel="k_rhs2d_multi<FD4,non-skew>", avg_launch_ms=round(ms, 5), launches=4 * NREP,
                     blocks_per_launch=nblk, bytes_per_launch=int(per_launch),
                     bytes_per_block_stage={f"Bs{bs}": b_min(3, bs, 2, 2, 5) for bs in (16, 24)},
-                    working_set_bytes=int(per_launch))
+                    working_set_bytes=int(per_launch), frac_of=frac_kind(per_launch))
     nblocks = sum(s.rk_blocks for s in solves)
     b.close()
 
@@ -564,8 +573,10 @@ def main():
     q = dim + 1
     bpb = b_min(q, w["bs"], dim, w["g_rhs"], 0)
     achieved = bpb * forest.n / (ks_ms * 1e-3) / 1e9
+    ws = stage_working_set(forest.n, w["bs"], w["g"], dim, q)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, frac_of=frac_kind(ws),
+                    working_set_bytes=ws,
                     kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(ks_ms, 5),
                     launches=4 * NREP, bytes_per_block_stage=bpb, blocks_per_launch=forest.n,
                     timing_pass_event_pair_ms=round(k_ms, 5),
