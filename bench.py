@@ -580,7 +580,11 @@ def main():
                     kernel="k_rhs%dd<FD4,fused>" % dim, avg_launch_ms=round(ks_ms, 5),
                     launches=4 * NREP, bytes_per_block_stage=bpb, blocks_per_launch=forest.n,
                     timing_pass_event_pair_ms=round(k_ms, 5),
-                    sync_kernel_avg_ms=round(s_ms, 5), tuned_variant=tuned)
+                    sync_kernel_avg_ms=round(s_ms, 5) if s_n else None, tuned_variant=tuned)
+    if dim == 2:
+        # 1: the stages gather their halos from the neighbours' interiors
+        # (no ghost-fill launches: sync_kernel_avg_ms is then null)
+        roofline["halo_gather"] = pool.get_tuning(L.WH_TUNE_GATHER)
     if dim == 3:
         roofline["tuned_zsplit"] = pool.get_tuning(L.WH_TUNE_K3D_ZSPLIT)
         roofline["tuned_waves"] = pool.get_tuning(L.WH_TUNE_K3D_WAVES)

@@ -512,13 +512,20 @@ int wh_set_tables_from_forests(wh_pool *pool, wh_forest *const *forests, int n, 
  * thread's first point before the halo-tile barrier; 2, 3, 4 a persistent
  * kernel with 2, 4, 8 workgroups per compute unit that loads the next
  * block's halo tile while computing the current one (where the tile fits).
- * Every variant computes the same values bit for bit. */
+ * Every variant computes the same values bit for bit.
+ * WH_TUNE_GATHER (default 1): where the RHS ghost-patch table of a 2D pool
+ * consists of same-level copies only (a uniform grid) that cover the halo
+ * of every active block, the fused RK stages read each block's halo tile
+ * straight from its neighbours' interiors and the step launches no ghost
+ * fill (the values are the same copies; the state's own ghost nodes are
+ * then not refreshed by the step); 0 fills the ghost nodes first. */
 enum {
     WH_TUNE_K3D_YSPLIT = 1,
     WH_TUNE_K2D_STAGE = 2,
     WH_TUNE_K3D_ZSPLIT = 3,
     WH_TUNE_K3D_WAVES = 4,
-    WH_TUNE_K3D_XCD = 5
+    WH_TUNE_K3D_XCD = 5,
+    WH_TUNE_GATHER = 6
 };
 int wh_set_tuning(wh_pool *pool, int key, int value);
 int wh_get_tuning(wh_pool *pool, int key, int *value);

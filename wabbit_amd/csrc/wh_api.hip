@@ -94,8 +94,32 @@ static bool fused_ok(const double *bt, int nrows)
     return true;
 }
 
+// The halo-gather table of the fused 2D stages (WH_TUNE_GATHER), or null:
+// the RHS patch table holds same-level copies only, at least as deep as the
+// stencil reach, and every active block has the senders of all 8
+// directions (checked once per topology version).
+static const int32_t *gather_table(wh_pool *P, const AcmDev &a)
+{
+    const PatchSet &ps = P->ps[0];
+    if (!P->gather || P->d.dim != 2 || !ps.gather_ok || !ps.d_gnb || ps.gather_depth < reach_of(a.scheme))
+        return nullptr;
+    if (P->gather_topo != P->topo_version) {
+        bool all = P->nact > 0;
+        for (int k = 0; k < P->nact && all; ++k) {
+            const int32_t *row = &ps.h_gnb[(size_t)P->h_active[k] * 9];
+            for (int d = 0; d < 9 && all; ++d)
+                if (d != 4 && row[d] < 0) all = false;
+        }
+        P->gather_all = all;
+        P->gather_topo = P->topo_version;
+    }
+    return P->gather_all ? ps.d_gnb : nullptr;
+}
+
 // one step of the fused path; dt is taken from the device scalar DS_DT, or
-// per block from dtb (block groups)
+// per block from dtb (block groups).  With a halo-gather table no ghost
+// fill is launched: every stage reads its blocks' halos from the
+// neighbours' interiors of its input array.
 static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows,
                       const double *dtb = nullptr)
 {
@@ -104,8 +128,17 @@ static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows,
     double *acc = array_ptr(P, WH_WORK + 0, nullptr, nullptr);
     double *T[2] = {array_ptr(P, WH_WORK + 1, nullptr, nullptr),
                     array_ptr(P, WH_WORK + 2, nullptr, nullptr)};
+    const int32_t *gnb = gather_table(P, a);
+    // the ghost fill (timed as a sync launch); nothing at all with gather
+    auto sync = [&](double *arr) {
+        return gnb ? (int)WH_OK : timed(P, 1, [&] { return wh::launch_sync(P, 0, arr, -1); });
+    };
+    auto launch_fused_stage = [&](wh_pool *Q, const AcmDev &aa, const double *tin, double *tout, const double *u0,
+                                  double *ac, double *sout, int mode, double acoef, double bcoef, const double *db) {
+        return wh::launch_fused_stage(Q, aa, tin, tout, u0, ac, sout, mode, acoef, bcoef, db, gnb);
+    };
     int rc;
-    if ((rc = timed(P, 1, [&] { return launch_sync(P, 0, S, -1); }))) return rc;
+    if ((rc = sync(S))) return rc;
     // stage 1: input S, writes acc and T[0]
     if ((rc = timed(P, 0, [&] {
              return launch_fused_stage(P, a, S, T[0], nullptr, acc, nullptr, 0, bt[1 * nrows + 1],
@@ -115,7 +148,7 @@ static int fused_step(wh_pool *P, const AcmDev &a, const double *bt, int nrows,
     int cur = 0;
     for (int j = 2; j <= s; ++j) {
         double *tin = T[cur];
-        if ((rc = timed(P, 1, [&] { return launch_sync(P, 0, tin, -1); }))) return rc;
+        if ((rc = sync(tin))) return rc;
         if (j < s) {
             double *tout = T[cur ^ 1];
             if ((rc = timed(P, 0, [&] {
@@ -260,6 +293,7 @@ static void free_patchset(PatchSet &ps)
     if (ps.d_recv_off) (void)hipFree(ps.d_recv_off);
     if (ps.d_recv_ids) (void)hipFree(ps.d_recv_ids);
     if (ps.d_chunks) (void)hipFree(ps.d_chunks);
+    if (ps.d_gnb) (void)hipFree(ps.d_gnb);
     ps = PatchSet{};
 }
 
@@ -571,6 +605,36 @@ static int set_patch_runs(wh_pool *P, int sync_case, const PatchRun *runs, int n
     }
     ps.h_recv_off.push_back(npatch);
     ps.n = npatch;
+    // 2D, same-level copies only: the senders of every receiver's 8
+    // directions (for the fused stages' halo gather)
+    ps.gather_ok = sync_case == 0 && P->d.dim == 2 && npatch > 0;
+    ps.gather_depth = -1;
+    if (ps.gather_ok) {
+        ps.h_gnb.assign((size_t)P->d.nb_max * 9, -1);
+        const int g = P->d.g, b0 = P->d.bs[0], b1 = P->d.bs[1];
+        auto dir = [&](int lo, int n, int bs, int &w) {  // -1 / 0 / +1, or 2: not a face or corner box
+            if (lo + n == g && lo >= 0) return w = n, -1;
+            if (lo == g + bs) return w = n, 1;
+            if (lo == g && n == bs) return 0;
+            return 2;
+        };
+        for (int i = 0; i < npatch && ps.gather_ok; ++i) {
+            const wh_patch &q = ps.host[i];
+            int wx = -1, wy = -1;
+            const int dx = dir(q.recv_lo[0], q.n[0], b0, wx), dy = dir(q.recv_lo[1], q.n[1], b1, wy);
+            const int w = wx >= 0 ? wx : wy;
+            if ((q.op & 0x3ff) != WH_PATCH_COPY || q.stage != 1 || dx == 2 || dy == 2 || (dx == 0 && dy == 0) ||
+                (wx >= 0 && wy >= 0 && wx != wy) || (ps.gather_depth >= 0 && w != ps.gather_depth) ||
+                q.send_lo[0] != q.recv_lo[0] - dx * b0 || q.send_lo[1] != q.recv_lo[1] - dy * b1) {
+                ps.gather_ok = false;
+                break;
+            }
+            ps.gather_depth = w;
+            int32_t &slot = ps.h_gnb[(size_t)q.recv * 9 + (dy + 1) * 3 + (dx + 1)];
+            if (slot >= 0) ps.gather_ok = false;  // a direction filled twice
+            slot = q.send;
+        }
+    }
     ps.nrecv = (int)ps.h_recv_ids.size();
     if (npatch) {
         if ((size_t)npatch > ps.cap_patches || ps.h_recv_off.size() > ps.cap_recv_off ||
@@ -582,6 +646,11 @@ static int set_patch_runs(wh_pool *P, int sync_case, const PatchRun *runs, int n
         if (int rc = grow_dev(ps.d_chunks, ps.cap_chunks, ps.h_chunks.size())) return rc;
         if (int rc = upload_staged(P, ps.d_chunks, ps.h_chunks.data(), sizeof(SyncChunk) * ps.h_chunks.size()))
             return rc;
+        if (ps.gather_ok) {
+            if (ps.h_gnb.size() > ps.cap_gnb) WH_HIP(hipStreamSynchronize(P->stream));
+            if (int rc = grow_dev(ps.d_gnb, ps.cap_gnb, ps.h_gnb.size())) return rc;
+            if (int rc = upload_staged(P, ps.d_gnb, ps.h_gnb.data(), sizeof(int32_t) * ps.h_gnb.size())) return rc;
+        }
         if (int rc = upload_staged(P, ps.d_patches, ps.host.data(), sizeof(wh_patch) * npatch)) return rc;
         if (int rc = upload_staged(P, ps.d_recv_off, ps.h_recv_off.data(), sizeof(int32_t) * ps.h_recv_off.size()))
             return rc;
@@ -924,8 +993,14 @@ int wh_set_tuning(wh_pool *P, int key, int value)
 {
     WH_CHECK(P, WH_E_ARG, "null argument");
     WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE || key == WH_TUNE_K3D_ZSPLIT ||
-                 key == WH_TUNE_K3D_WAVES || key == WH_TUNE_K3D_XCD,
+                 key == WH_TUNE_K3D_WAVES || key == WH_TUNE_K3D_XCD || key == WH_TUNE_GATHER,
              WH_E_ARG, "unknown tuning key");
+    if (key == WH_TUNE_GATHER) {
+        WH_CHECK(value == 0 || value == 1, WH_E_ARG, "GATHER is 0 or 1");
+        P->gather = value;
+        P->topo_version++;
+        return WH_OK;
+    }
     if (key == WH_TUNE_K3D_XCD) {
         WH_CHECK(value == 0 || value == 1, WH_E_ARG, "K3D_XCD is 0 or 1");
         P->k3d_xcd = value;
@@ -964,9 +1039,10 @@ int wh_get_tuning(wh_pool *P, int key, int *value)
 {
     WH_CHECK(P && value, WH_E_ARG, "null argument");
     WH_CHECK(key == WH_TUNE_K3D_YSPLIT || key == WH_TUNE_K2D_STAGE || key == WH_TUNE_K3D_ZSPLIT ||
-                 key == WH_TUNE_K3D_WAVES || key == WH_TUNE_K3D_XCD,
+                 key == WH_TUNE_K3D_WAVES || key == WH_TUNE_K3D_XCD || key == WH_TUNE_GATHER,
              WH_E_ARG, "unknown tuning key");
     *value = key == WH_TUNE_K2D_STAGE   ? P->k2d_stage
+             : key == WH_TUNE_GATHER    ? P->gather
              : key == WH_TUNE_K3D_ZSPLIT ? P->k3d_zsplit
              : key == WH_TUNE_K3D_WAVES  ? P->k3d_waves
              : key == WH_TUNE_K3D_XCD    ? P->k3d_xcd
@@ -987,14 +1063,16 @@ int wh_time_stage_kernel(wh_pool *P, const wh_acm_params *p, int reps, double *m
     double *acc = array_ptr(P, WH_WORK + 0, nullptr, nullptr), *tout = array_ptr(P, WH_WORK + 1, nullptr, nullptr),
            *sout = array_ptr(P, WH_WORK + 2, nullptr, nullptr);
     // the four stages of classic RK4 (first, two middle, last), reading the
-    // state and writing only work slots
+    // state and writing only work slots; with the halo-gather table the
+    // steps use, the launches gather their halos the same way
+    const int32_t *gnb = gather_table(P, a);
     auto rk4 = [&]() -> int {
         const double ca[4] = {0.5, 0.5, 1.0, 0.0}, cb[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
         const int mode[4] = {0, 1, 1, 2};
         for (int j = 0; j < 4; ++j)
             if (int rc = launch_fused_stage(P, a, P->block, mode[j] == 2 ? nullptr : tout,
                                             mode[j] == 1 ? P->block : nullptr, acc, mode[j] == 2 ? sout : nullptr,
-                                            mode[j], ca[j], cb[j], nullptr))
+                                            mode[j], ca[j], cb[j], nullptr, gnb))
                 return rc;
         return WH_OK;
     };

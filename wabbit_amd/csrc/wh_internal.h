@@ -158,6 +158,14 @@ struct PatchSet {
     SyncChunk *d_chunks = nullptr;   // the patches cut into wavefront-sized chunks
     std::vector<int32_t> h_recv_off, h_recv_ids;
     std::vector<SyncChunk> h_chunks;
+    // a 2D table of same-level copies only: per receiver hvy id the senders of
+    // its 9 directions (x fastest, centre unused), all copying `gather_depth`
+    // ghost layers; gather_ok false otherwise (set_patch_runs)
+    bool gather_ok = false;
+    int gather_depth = 0;
+    std::vector<int32_t> h_gnb;
+    int32_t *d_gnb = nullptr;
+    size_t cap_gnb = 0;
     int stage_begin[5] = {0, 0, 0, 0, 0};  // stage s (1..3): patches [stage_begin[s], stage_begin[s+1])
     int chunk_begin[5] = {0, 0, 0, 0, 0};  // and chunks [chunk_begin[s], chunk_begin[s+1])
     int n = 0, nrecv = 0;
@@ -214,6 +222,11 @@ struct wh_pool {
     // loaded before the tile barrier, 2..4 persistent pipelined kernel with
     // 2, 4, 8 workgroups per CU
     int k2d_stage = 1;
+    // WH_TUNE_GATHER, and whether the current tables allow it (checked once
+    // per topology version: every active block has its 8 senders)
+    int gather = 1;
+    uint64_t gather_topo = ~0ull;
+    bool gather_all = false;
     int n_cu = 256;  // compute units of the pool's device
     int device = 0;  // the device the pool lives on
     // wavelet transforms / adapt_tree
@@ -245,9 +258,10 @@ int launch_combine(wh_pool *P, double *out, const double *base, int nterms,
 int launch_dt(wh_pool *P, const AcmDev &a);
 int launch_maxabs(wh_pool *P, const double *in);
 // dtb: per active-list position dt (block groups), or null for the pool's DS_DT
+// gnb: the halo-gather table of a 2D pool (the ghost fill skipped), or null
 int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *tout,
                        const double *u0, double *acc, double *sout, int mode, double acoef,
-                       double bcoef, const double *dtb);
+                       double bcoef, const double *dtb, const int32_t *gnb = nullptr);
 // per-group minimum block dt into P->d_dtg (bit patterns) for the P->ngrp groups
 int launch_dt_groups(wh_pool *P, const AcmDev &a);
 int launch_dt_finalize(wh_pool *P, const TimeDev &tc);

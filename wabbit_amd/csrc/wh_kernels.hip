@@ -174,7 +174,8 @@ struct StageArgs {
     const double *dtb;  // per active-list position dt (independent forests), or null
     double a, b;
     int mode, use_b, use_a;
-    int full_rows;  // tout written in whole rows (stage_ghost_cols)
+    int full_rows;      // tout written in whole rows (stage_ghost_cols)
+    const int32_t *gnb;  // 2D halo gather: senders of each block's 9 directions (by hvy id), or null
 };
 
 struct NoEpilogue {};
@@ -324,10 +325,24 @@ __device__ __forceinline__ void rhs2d_block(double *tile, const double *__restri
     }
     if constexpr (!LOADED) {
         const double *u = in + (size_t)b * G.blk;
-        for (int i = threadIdx.x; i < 3 * tcs; i += blockDim.x) {
-            const int c = i / tcs, rr = i - c * tcs;
-            const int iy = rr / tx, ix = rr - iy * tx;
-            tile[i] = u[c * G.cs + (size_t)(G.g - R + iy) * G.nx + (G.g - R + ix)];
+        if (FUSED && sa.gnb) {
+            // the halo from the neighbours' interiors (same-level senders, the
+            // values the ghost fill would copy): tile point (x, y) relative to
+            // the interior lies in direction (dx, dy) of the block
+            const int32_t *nb = sa.gnb + (size_t)b * 9;
+            for (int i = threadIdx.x; i < 3 * tcs; i += blockDim.x) {
+                const int c = i / tcs, rr = i - c * tcs;
+                const int y = rr / tx - R, x = rr - (y + R) * tx - R;
+                const int dx = x < 0 ? -1 : (x >= G.bs0 ? 1 : 0), dy = y < 0 ? -1 : (y >= G.bs1 ? 1 : 0);
+                const double *src = (dx | dy) ? in + (size_t)nb[(dy + 1) * 3 + dx + 1] * G.blk : u;
+                tile[i] = src[c * G.cs + (size_t)(G.g + y - dy * G.bs1) * G.nx + (G.g + x - dx * G.bs0)];
+            }
+        } else {
+            for (int i = threadIdx.x; i < 3 * tcs; i += blockDim.x) {
+                const int c = i / tcs, rr = i - c * tcs;
+                const int iy = rr / tx, ix = rr - iy * tx;
+                tile[i] = u[c * G.cs + (size_t)(G.g - R + iy) * G.nx + (G.g - R + ix)];
+            }
         }
         __syncthreads();
     }
@@ -898,7 +913,8 @@ static int dispatch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *o
         const int pre = P->k2d_stage == 1;
         // the persistent pipelined variant: two tiles in LDS, each thread
         // holds at most PIPE_MAXL tile elements of the next block
-        const int wpc = P->k2d_stage == 2 ? 2 : P->k2d_stage == 3 ? 4 : P->k2d_stage == 4 ? 8 : 0;
+        // (the halo gather reads through the one-workgroup-per-block kernel)
+        const int wpc = (FUSED && sa.gnb) ? 0 : P->k2d_stage == 2 ? 2 : P->k2d_stage == 3 ? 4 : P->k2d_stage == 4 ? 8 : 0;
         if (wpc && 3 * (G.bs0 + 2 * R) * (G.bs1 + 2 * R) <= PIPE_MAXL * 256 && 2 * lds <= 160 * 1024) {
             const int nwg = std::min(nact, wpc * P->n_cu);
             dim3 pgrid(nwg), pblock(256);
@@ -1009,9 +1025,10 @@ int launch_rhs(wh_pool *P, const AcmDev &a, const double *in, double *out, const
 
 int launch_fused_stage(wh_pool *P, const AcmDev &a, const double *tin, double *tout,
                        const double *u0, double *acc, double *sout, int mode, double acoef,
-                       double bcoef, const double *dtb)
+                       double bcoef, const double *dtb, const int32_t *gnb)
 {
     StageArgs sa{};
+    sa.gnb = gnb;
     sa.dtb = dtb;
     sa.u0 = u0;
     sa.acc = acc;

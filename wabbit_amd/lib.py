@@ -19,6 +19,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "wabbit_hip.h")
 WH_BLOCK, WH_TMP, WH_MASK, WH_WORK = 0, 1, 2, 16
 WH_PATCH_COPY, WH_PATCH_RESTRICT, WH_PATCH_PREDICT = 0, 1, 2
 WH_TUNE_K3D_YSPLIT, WH_TUNE_K2D_STAGE, WH_TUNE_K3D_ZSPLIT, WH_TUNE_K3D_WAVES, WH_TUNE_K3D_XCD = 1, 2, 3, 4, 5
+WH_TUNE_GATHER = 6
 ERRORS = {-1: "WH_E_ARG", -2: "WH_E_HIP", -3: "WH_E_STATE", -4: "WH_E_DIVERGED", -5: "WH_E_NOMEM"}
 WH_OK, WH_E_ARG, WH_E_HIP, WH_E_STATE, WH_E_DIVERGED, WH_E_NOMEM = 0, -1, -2, -3, -4, -5
 
