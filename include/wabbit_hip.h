@@ -347,6 +347,28 @@ int wh_copy_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *h
 int wh_detail_blocks(wh_pool *pool, int array, const int32_t *hvy_ids, int n,
                      const int32_t *groups, double *detail);
 
+/* params%eps_norm (ini_file_to_params.f90:494; "Linfty" by default) */
+enum { WH_NORM_LINFTY = 0, WH_NORM_L1 = 1, WH_NORM_L2 = 2, WH_NORM_H1 = 3 };
+/* threshold_block's detail (threshold_block.f90:69-124) with the
+ * wavelet_renorm_block scaling of eps_norm (module_wavelets.f90:1848-1960)
+ * for blocks on levels[k] (level_ref 0, as threshold_block calls it): L1
+ * 2^(-(J+1)dim) and L2 2^(-(J+1)dim/2) on every WC, then /4 (L1) or /2 (L2)
+ * per direction in which the point lies on an SC position; H1 in 3D
+ * 2^(J/2) and 2^(2/3) per such direction, in 2D as Linfty.  WH_NORM_LINFTY
+ * is wh_detail_blocks. */
+int wh_detail_blocks_norm(wh_pool *pool, int array, const int32_t *hvy_ids, const int32_t *levels, int n,
+                          const int32_t *groups, int eps_norm, double *detail);
+/* componentWiseNorm_tree(params, hvy_block, tree_ID, eps_norm, norm)
+ * (OPERATORS/componentWiseNorm_tree.f90:1-200, norm_case default) over the
+ * listed blocks (the caller passes the leaves): per component of group 1
+ * the sum over the blocks of dx^dim * sum(u^2) (L2, H1) or dx^dim *
+ * sum(|u|) (L1) over the interior, the components of a group l >= 2
+ * together, then the square root for L2/H1; Linfty: the max of |u|.  dx of
+ * block k is 2^-levels[k] * domain[d] / Bs[d]; groups NULL: all 1.  No
+ * clamp (coarseningIndicator_tree's norm <= 1e-9 -> 1 is the caller's). */
+int wh_component_norm(wh_pool *pool, int array, const int32_t *hvy_ids, const int32_t *levels, int n,
+                      const int32_t *groups, int eps_norm, const double *domain, double *norm);
+
 /* Coarse-extension boxes (coarseExtensionManipulateWC/SC_block,
  * module_wavelets.f90:877-1027): inclusive ghost-inclusive index box of block
  * hvy; WH_BOX_ZERO_WC zeroes the points that are not SC in every direction,
@@ -396,7 +418,7 @@ typedef struct {
     double eps;
     double norm[8];                 /* per component: threshold eps * norm */
     int32_t groups[8];              /* threshold_state_vector_component */
-    int eps_normalized;             /* norm = Linfty of the leaves (componentWiseNorm_tree) */
+    int eps_normalized;             /* norm = eps_norm of the leaves (componentWiseNorm_tree) */
     int threshold_mask;             /* coarseningIndicatorMask_tree, with `mask` */
     int security_zone;              /* addSecurityZone_CE_tree */
     double domain[3];
@@ -405,6 +427,7 @@ typedef struct {
      * (leaf-first iff every Bs >= 3 max|HD tap|, adapt_tree.f90:321-322),
      * 1 leaf-first, 2 level-wise; both give the same decomposition */
     int decompose_branch;
+    int eps_norm;                   /* WH_NORM_*: the details' renormalisation and the norm */
 } wh_adapt_params;
 int wh_adapt_tree(wh_pool *pool, wh_forest *forest, const wh_adapt_params *p);
 

@@ -230,7 +230,11 @@ class OrAdaptParams(ctypes.Structure):
                 ("norm", ctypes.c_double * 8), ("groups", ctypes.c_int * 8),
                 ("eps_normalized", ctypes.c_int), ("threshold_mask", ctypes.c_int),
                 ("security_zone", ctypes.c_int), ("domain", ctypes.c_double * 3),
-                ("mask", OrMaskParams), ("decompose_branch", ctypes.c_int), ("plant", ctypes.c_int)]
+                ("mask", OrMaskParams), ("decompose_branch", ctypes.c_int), ("plant", ctypes.c_int),
+                ("eps_norm", ctypes.c_int)]
+
+# params%eps_norm
+NORM_LINFTY, NORM_L1, NORM_L2, NORM_H1 = 0, 1, 2, 3
 
 
 def mask_params(geometry=0, x_cntr=(0, 0, 0), R_cyl=0.5, smoothing_width=0.0, smoothing_safety=0.0,
@@ -267,6 +271,12 @@ def _wlib():
             getattr(L, fn).argtypes = [ctypes.POINTER(OrWavelet), ctypes.c_int, _ip, ctypes.c_int,
                                        ctypes.c_int, _dp, _dp]
         L.or_detail_block.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
+        L.or_detail_block_norm.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int,
+                                           ctypes.c_int, _dp]
+        L.or_renorm_value.argtypes = [ctypes.c_int] * 7 + [ctypes.c_double]
+        L.or_renorm_value.restype = ctypes.c_double
+        L.or_component_norm.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(_dp), _dp, _dp]
         L.or_tree_create.argtypes = [ctypes.POINTER(OrWavelet), ctypes.POINTER(OrAdaptParams), ctypes.c_int,
                                      vp, _ip, _dp]
         L.or_tree_create.restype = vp
@@ -329,15 +339,47 @@ def detail_block(dim, bs, g, u, groups):
     return out
 
 
+def detail_block_norm(dim, bs, g, u, groups, level, eps_norm):
+    """threshold_block's detail with wavelet_renorm_block of eps_norm for a
+    block on `level` (threshold_block.f90:69-124, module_wavelets.f90:1848-1960)."""
+    u = np.ascontiguousarray(u, np.float64)
+    gr = np.ascontiguousarray(groups, np.int32)
+    out = np.zeros(u.shape[0])
+    _wlib().or_detail_block_norm(dim, _i(_blk(dim, bs)), g, u.shape[0], _i(gr), _d(u), int(level), int(eps_norm),
+                                 _d(out))
+    return out
+
+
+def renorm_value(eps_norm, dim, level, ix, iy, iz, g, v):
+    """wavelet_renorm_block for one value at 0-based point (ix, iy, iz)."""
+    return _wlib().or_renorm_value(int(eps_norm), dim, int(level), ix, iy, iz, g, float(v))
+
+
+def component_norm(dim, bs, g, blocks, dx, groups, eps_norm):
+    """componentWiseNorm_tree over the given blocks (each (neq, nz, ny, nx)),
+    dx[k] the spacing of block k (componentWiseNorm_tree.f90:1-200)."""
+    blocks = [np.ascontiguousarray(b, np.float64) for b in blocks]
+    neq = blocks[0].shape[0]
+    ptrs = (_dp * len(blocks))(*[_d(b) for b in blocks])
+    dxa = np.zeros((len(blocks), 3))
+    dxa[:, :dim] = np.asarray(dx, np.float64).reshape(len(blocks), -1)[:, :dim]
+    gr = np.ascontiguousarray(groups, np.int32)
+    out = np.zeros(neq)
+    _wlib().or_component_norm(dim, _i(_blk(dim, bs)), g, neq, _i(gr), int(eps_norm), len(blocks), ptrs,
+                              _d(dxa), _d(out))
+    return out
+
+
 class Tree:
     """A periodic forest owned by the oracle (adapt_tree, syncs, refinement)."""
 
     def __init__(self, w, dim, jmin, jmax, bs, g, treecodes, levels, hvy, indicator=0, use_ce=None,
                  eps=1e-3, norm=None, groups=None, force_maxlevel_dealiasing=False, eps_normalized=False,
                  threshold_mask=False, security_zone=None, domain=(1.0, 1.0, 1.0), mask=None,
-                 decompose_branch=0, plant=0):
+                 decompose_branch=0, plant=0, eps_norm=NORM_LINFTY):
         p = OrAdaptParams()
         p.decompose_branch = int(decompose_branch)
+        p.eps_norm = int(eps_norm)
         p.plant = int(plant)
         p.dim, p.jmin, p.jmax, p.g = dim, jmin, jmax, g
         p.eps_normalized = int(eps_normalized)

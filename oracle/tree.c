@@ -943,9 +943,11 @@ static void block_x0dx(const or_tree *T, int b, double x0[3], double dx[3])
     }
 }
 
-/* Linfty detail of a box (threshold_block with indices), SCs zeroed by the
- * global parity, grouped components as in or_detail_block */
-static void detail_box(const or_tree *T, const double *u, const int lo[3], const int hi[3], double *det)
+/* detail of a box (threshold_block with indices) of a block on `level`,
+ * SCs zeroed by the global parity, renormed (wavelet_renorm_block), grouped
+ * components as in or_detail_block */
+static void detail_box(const or_tree *T, const double *u, int level, const int lo[3], const int hi[3],
+                       double *det)
 {
     const int g = T->g;
     int maxgrp = 0;
@@ -957,7 +959,8 @@ static void detail_box(const or_tree *T, const double *u, const int lo[3], const
             for (int iy = lo[1]; iy <= hi[1]; ++iy)
                 for (int ix = lo[0]; ix <= hi[0]; ++ix) {
                     if (is_sc(T, ix, iy, iz)) continue;
-                    const double v = u[c * T->cs + ((size_t)iz * T->ny + iy) * T->nx + ix];
+                    const double v = or_renorm_value(T->p.eps_norm, T->dim, level, ix, iy, iz, T->g,
+                                                     u[c * T->cs + ((size_t)iz * T->ny + iy) * T->nx + ix]);
                     const double a = T->p.groups[c] >= 2 ? sqrt(v * v) : fabs(v);
                     if (a > m[c]) m[c] = a;
                 }
@@ -1009,6 +1012,30 @@ static void leaf_linfty(const or_tree *T, double *norm)
         if (norm[c] <= 1.0e-9) norm[c] = 1.0;
 }
 
+/* componentWiseNorm_tree(hvy_tmp, eps_norm) over the leaves in list order,
+ * then norm <= 1e-9 -> 1 (coarseningIndicator_tree.f90:146-163) */
+static void leaf_norm(const or_tree *T, double *norm)
+{
+    if (T->p.eps_norm == OR_NORM_LINFTY) {
+        leaf_linfty(T, norm);
+        return;
+    }
+    const double **u = (const double **)malloc(sizeof(double *) * (T->n ? T->n : 1));
+    double *dx = (double *)malloc(sizeof(double) * 3 * (T->n ? T->n : 1));
+    int n = 0;
+    for (int b = 0; b < T->n; ++b) {
+        if (!is_leaf(T, b, 0)) continue;
+        double x0[3];
+        block_x0dx(T, b, x0, dx + 3 * n);
+        u[n++] = T->t[b];
+    }
+    or_component_norm(T->dim, T->p.bs, T->g, T->neq, T->p.groups, T->p.eps_norm, n, u, dx, norm);
+    for (int c = 0; c < T->neq; ++c)
+        if (norm[c] <= 1.0e-9) norm[c] = 1.0;
+    free(u);
+    free(dx);
+}
+
 /* coarseningIndicatorMask_tree (coarseningIndicator_tree.f90:338-422): a
  * block whose chi is partly in (0,1) or not constant stays */
 static void mask_criterion(or_tree *T)
@@ -1055,7 +1082,7 @@ static void security_zone(or_tree *T, const double *norm)
             int lo[3], hi[3];
             modify_box(T, dirs[q], T->w.Nwcl, T->w.Nwcr, lo, hi);
             double det[8];
-            detail_box(T, T->u[b], lo, hi, det);
+            detail_box(T, T->u[b], T->level[b], lo, hi, det);
             int all = 1;
             for (int c = 0; c < T->neq; ++c)
                 if (!(det[c] <= T->p.eps * norm[c])) all = 0;
@@ -1117,14 +1144,15 @@ int or_tree_adapt_flags(or_tree *T, const int *leaf_flags)
         for (int b = 0; b < T->n; ++b)
             if (is_leaf(T, b, 0)) T->status[b] = -1;
     } else {
-        if (T->p.eps_normalized) leaf_linfty(T, norm);
+        if (T->p.eps_normalized) leaf_norm(T, norm);
         double det[8];
         for (int b = 0; b < T->n; ++b) {
             if (T->p.force_maxlevel_dealiasing && T->level[b] == T->p.jmax) {
                 T->status[b] = -1;
                 continue;
             }
-            or_detail_block(T->dim, T->p.bs, T->g, T->neq, T->p.groups, T->u[b], det);
+            or_detail_block_norm(T->dim, T->p.bs, T->g, T->neq, T->p.groups, T->u[b], T->level[b], T->p.eps_norm,
+                                 det);
             int all = 1;
             for (int c = 0; c < T->neq; ++c)
                 if (!(det[c] <= T->p.eps * norm[c])) all = 0;

@@ -317,8 +317,85 @@ void or_filter_block(const or_wavelet *w, int dim, const int bs[3], int g, int n
     free(tmp);
 }
 
+double or_renorm_value(int eps_norm, int dim, int level, int ix, int iy, int iz, int g, double v)
+{
+    /* the patch's copy of the block, then the level shift, then the SC
+     * rows of x, of y, of z (module_wavelets.f90:1909-1953); Jref = 0 */
+    const int Jref = 0;
+    const int sc[3] = {((ix - g) & 1) == 0, ((iy - g) & 1) == 0, dim == 3 && ((iz - g) & 1) == 0};
+    double r = v;
+    if (eps_norm == OR_NORM_L1) {
+        r = r * pow(2.0, (double)((Jref - level - 1) * dim));
+        for (int d = 0; d < dim; ++d)
+            if (sc[d]) r = r / 4.0;
+    } else if (eps_norm == OR_NORM_L2) {
+        r = r * pow(2.0, (double)((Jref - level - 1) * dim) / 2.0);
+        for (int d = 0; d < dim; ++d)
+            if (sc[d]) r = r / 2.0;
+    } else if (eps_norm == OR_NORM_H1 && dim == 3) {
+        r = r * pow(2.0, (double)(Jref - level) * (2.0 - (double)dim) / 2.0);
+        for (int d = 0; d < dim; ++d)
+            if (sc[d]) r = r * pow(2.0, 2.0 * ((double)dim - 2.0) / 3.0);
+    }
+    return r;
+}
+
 void or_detail_block(int dim, const int bs[3], int g, int neq, const int *groups, const double *u,
                      double *detail)
+{
+    or_detail_block_norm(dim, bs, g, neq, groups, u, 0, OR_NORM_LINFTY, detail);
+}
+
+void or_component_norm(int dim, const int bs[3], int g, int neq, const int *groups, int eps_norm, int nblk,
+                       const double *const *u, const double *dx, double *norm)
+{
+    const int nx = bs[0] + 2 * g, ny = bs[1] + 2 * g, nz = dim == 3 ? bs[2] + 2 * g : 1;
+    const long cs = (long)nx * ny * nz;
+    const int z0 = dim == 3 ? g : 0, z1 = dim == 3 ? g + bs[2] : 1;
+    const int inf = eps_norm == OR_NORM_LINFTY;
+    int maxgrp = 0;
+    for (int c = 0; c < neq; ++c) maxgrp = groups[c] > maxgrp ? groups[c] : maxgrp;
+    for (int c = 0; c < neq; ++c) norm[c] = inf ? -1.0 : 0.0;
+    /* sum over the points of components cl[0..ncl) (Fortran order: x, y,
+     * z, then component) of u^2, |u|, or the max of |u| */
+    double nb;
+#define NORM_SECTION(COND)                                                                  \
+    do {                                                                                   \
+        nb = 0.0;                                                                          \
+        for (int c = 0; c < neq; ++c) {                                                    \
+            if (!(COND)) continue;                                                         \
+            for (int iz = z0; iz < z1; ++iz)                                               \
+                for (int iy = g; iy < g + bs[1]; ++iy)                                     \
+                    for (int ix = g; ix < g + bs[0]; ++ix) {                               \
+                        const double v = ub[c * cs + ((long)iz * ny + iy) * nx + ix];      \
+                        if (inf) nb = fabs(v) > nb ? fabs(v) : nb;                         \
+                        else if (eps_norm == OR_NORM_L1) nb = nb + fabs(v);                \
+                        else nb = nb + v * v;                                              \
+                    }                                                                      \
+        }                                                                                  \
+    } while (0)
+    for (int k = 0; k < nblk; ++k) {
+        const double *ub = u[k];
+        double vol = 1.0;
+        for (int d = 0; d < dim; ++d) vol = vol * dx[k * 3 + d];
+        for (int p = 0; p < neq; ++p) {
+            if (groups[p] != 1) continue;
+            NORM_SECTION(c == p);
+            norm[p] = inf ? (nb > norm[p] ? nb : norm[p]) : norm[p] + vol * nb;
+        }
+        for (int l = 2; l <= maxgrp; ++l) {
+            NORM_SECTION(groups[c] == l);
+            for (int c = 0; c < neq; ++c)
+                if (groups[c] == l) norm[c] = inf ? (nb > norm[c] ? nb : norm[c]) : norm[c] + vol * nb;
+        }
+    }
+#undef NORM_SECTION
+    if (eps_norm == OR_NORM_L2 || eps_norm == OR_NORM_H1)
+        for (int c = 0; c < neq; ++c) norm[c] = sqrt(norm[c]);
+}
+
+void or_detail_block_norm(int dim, const int bs[3], int g, int neq, const int *groups, const double *u,
+                          int level, int eps_norm, double *detail)
 {
     const int nx = bs[0] + 2 * g, ny = bs[1] + 2 * g, nz = dim == 3 ? bs[2] + 2 * g : 1;
     const long cs = (long)nx * ny * nz;
@@ -326,10 +403,12 @@ void or_detail_block(int dim, const int bs[3], int g, int neq, const int *groups
     int maxgrp = 0;
     for (int c = 0; c < neq; ++c) maxgrp = groups[c] > maxgrp ? groups[c] : maxgrp;
     for (int c = 0; c < neq; ++c) detail[c] = -1.0;
-    /* value with SCs zeroed (wavelet_renorm_block, Linfty: no scaling) */
+    /* value with SCs zeroed, renormed (wavelet_renorm_block) */
 #define WCV(c, iz, iy, ix)                                                                    \
-    ((((ix - g) & 1) == 0 && ((iy - g) & 1) == 0 && (dim == 2 || ((iz - g) & 1) == 0)) ? 0.0 \
-                                                                                         : u[(c) * cs + ((long)(iz) * ny + (iy)) * nx + (ix)])
+    ((((ix - g) & 1) == 0 && ((iy - g) & 1) == 0 && (dim == 2 || ((iz - g) & 1) == 0))        \
+         ? 0.0                                                                                \
+         : or_renorm_value(eps_norm, dim, level, ix, iy, iz, g,                              \
+                           u[(c) * cs + ((long)(iz) * ny + (iy)) * nx + (ix)]))
     for (int l = 2; l <= maxgrp; ++l) {
         double m = -1.0;
         int any = 0;

@@ -28,6 +28,15 @@
  *                        the same acm_CDF44 run (every step refines, syncs
  *                        and steps the whole tree).
  *   or_tree_dt           same run (the dt sequence fixes the iteration count).
+ *   or_renorm_value / or_detail_block_norm / or_component_norm (eps_norm
+ *                        L1, L2, H1): no reference fixture uses them; Linfty
+ *                        and 2D H1 leave details unchanged, the level
+ *                        scaling of the renormed details equals that of the
+ *                        norm of the same data, the norms of constants,
+ *                        vector groups and a sine in closed form
+ *                        (tests/test_oracle_norm.py).  Parity unpinned: the
+ *                        per-direction SC-to-WC factors, the -1 of the level
+ *                        shift and H1 in 3D follow the reference's lines only.
  *   or_mask_block        the run's saved masks (p-norm sponge, cylinder);
  *                        the rect sponge by symmetry/support properties on a
  *                        non-square domain (tests/test_oracle_mask.py).
@@ -85,6 +94,33 @@ void or_filter_block(const or_wavelet *w, int dim, const int bs[3], int g, int n
 void or_detail_block(int dim, const int bs[3], int g, int neq, const int *groups,
                      const double *u, double *detail);
 
+/* params%eps_norm (ini_file_to_params.f90:494) */
+enum { OR_NORM_LINFTY = 0, OR_NORM_L1 = 1, OR_NORM_L2 = 2, OR_NORM_H1 = 3 };
+
+/* wavelet_renorm_block (module_wavelets.f90:1848-1960) for one value v at a
+ * point of a block on `level`, level_ref 0 (threshold_block passes none,
+ * threshold_block.f90:72-74): the level shift, then per direction d (x, y,
+ * z in that order) in which the point lies on a scaling-coefficient
+ * position (ix - g even, 0-based) the SC-to-WC factor.  Linfty and H1 in
+ * 2D leave v as it is. */
+double or_renorm_value(int eps_norm, int dim, int level, int ix, int iy, int iz, int g, double v);
+
+/* or_detail_block with the renormalisation of eps_norm for a block on
+ * `level` (threshold_block.f90:69-124); eps_norm OR_NORM_LINFTY is
+ * or_detail_block. */
+void or_detail_block_norm(int dim, const int bs[3], int g, int neq, const int *groups, const double *u,
+                          int level, int eps_norm, double *detail);
+
+/* componentWiseNorm_tree (componentWiseNorm_tree.f90:1-200) over the
+ * listed blocks (the leaves, in the order given), norm_case default,
+ * threshold_state_vector on: per component group 1 the sum over the blocks
+ * of dx^dim * sum(u^2) (L2, H1) or of dx^dim * sum(|u|) (L1) over the
+ * interior, the components of a group l >= 2 together; L2/H1 take the
+ * square root; Linfty the max of |u| (grouped as threshold_block does).
+ * dx[k*3 + d] is block k's spacing.  Group 0 components stay 0. */
+void or_component_norm(int dim, const int bs[3], int g, int neq, const int *groups, int eps_norm, int nblk,
+                       const double *const *u, const double *dx, double *norm);
+
 /* ---- penalisation mask (create_mask_2D_ACM, create_mask.f90:183-320) --- */
 typedef struct {
     int geometry;                   /* 0 none, 1 cylinder (a circle in 2D) */
@@ -129,6 +165,7 @@ typedef struct {
      * of iterations after the first is skipped; bit 2: the z pass of 3D
      * prediction in ghost syncs is of order 2 */
     int plant;
+    int eps_norm;                   /* OR_NORM_*: params%eps_norm */
 } or_adapt_params;
 
 /* A forest with heavy data that the oracle owns.  n blocks in list order. */

@@ -301,3 +301,31 @@ def random_block_data(n, neq, dim, bs, g, seed):
     nx = bs + 2 * g
     nz = nx if dim == 3 else 1
     return rng.standard_normal((n, neq, nz, nx, nx))
+
+
+def smooth_blocks(tcs, lev, jmax, dim, bs, g, seed, neq):
+    """Smooth periodic field with a sharp feature, sampled on every block
+    (ghosts included) so the threshold keeps some blocks and drops others."""
+    rng = np.random.default_rng(seed)
+    n = tcs.shape[0]
+    nz = bs + 2 * g if dim == 3 else 1
+    out = np.zeros((n, neq, nz, bs + 2 * g, bs + 2 * g))
+    cen = rng.uniform(0.2, 0.8, size=3)
+    for k in range(n):
+        l = int(lev[k])
+        ijk = [0, 0, 0]
+        for lv in range(1, l + 1):
+            d = (int(tcs[k]) >> ((jmax - lv) * dim)) & ((1 << dim) - 1)
+            ijk[0] = 2 * ijk[0] + ((d >> 1) & 1)
+            ijk[1] = 2 * ijk[1] + (d & 1)
+            ijk[2] = 2 * ijk[2] + ((d >> 2) & 1 if dim == 3 else 0)
+        dx = 2.0 ** -l / bs
+        xs = [(ijk[d] * bs + np.arange(bs + 2 * g) - g) * dx for d in range(3)]
+        if dim == 2:
+            xs[2] = np.zeros(1)
+        Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+        r2 = (np.sin(np.pi * (X - cen[0])) ** 2 + np.sin(np.pi * (Y - cen[1])) ** 2 +
+              (np.sin(np.pi * (Z - cen[2])) ** 2 if dim == 3 else 0.0))
+        for c in range(neq):
+            out[k, c] = np.exp(-r2 / (0.004 * (c + 1))) + 0.1 * np.cos(2 * np.pi * (X + c * Y))
+    return out

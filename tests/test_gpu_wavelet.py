@@ -6,7 +6,7 @@ own; see tests/test_oracle_wavelet.py)."""
 import numpy as np
 import pytest
 
-from helpers import fixture_to_hvy, load_npz, random_graded_forest
+from helpers import fixture_to_hvy, load_npz, random_graded_forest, smooth_blocks
 from oracle import pyoracle as po
 from wabbit_amd import lib as L
 from wabbit_amd import wavelets as W
@@ -197,34 +197,6 @@ def test_coarsen_everywhere_gpu(wav):
         assert np.linalg.norm(out - want) / np.linalg.norm(want) <= 1e-14
 
 
-def _smooth_blocks(tcs, lev, jmax, dim, bs, g, seed, neq):
-    """Smooth periodic field with a sharp feature, sampled on every block
-    (ghosts included) so the threshold keeps some blocks and drops others."""
-    rng = np.random.default_rng(seed)
-    n = tcs.shape[0]
-    nz = bs + 2 * g if dim == 3 else 1
-    out = np.zeros((n, neq, nz, bs + 2 * g, bs + 2 * g))
-    cen = rng.uniform(0.2, 0.8, size=3)
-    for k in range(n):
-        l = int(lev[k])
-        ijk = [0, 0, 0]
-        for lv in range(1, l + 1):
-            d = (int(tcs[k]) >> ((jmax - lv) * dim)) & ((1 << dim) - 1)
-            ijk[0] = 2 * ijk[0] + ((d >> 1) & 1)
-            ijk[1] = 2 * ijk[1] + (d & 1)
-            ijk[2] = 2 * ijk[2] + ((d >> 2) & 1 if dim == 3 else 0)
-        dx = 2.0 ** -l / bs
-        xs = [(ijk[d] * bs + np.arange(bs + 2 * g) - g) * dx for d in range(3)]
-        if dim == 2:
-            xs[2] = np.zeros(1)
-        Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
-        r2 = (np.sin(np.pi * (X - cen[0])) ** 2 + np.sin(np.pi * (Y - cen[1])) ** 2 +
-              (np.sin(np.pi * (Z - cen[2])) ** 2 if dim == 3 else 0.0))
-        for c in range(neq):
-            out[k, c] = np.exp(-r2 / (0.004 * (c + 1))) + 0.1 * np.cos(2 * np.pi * (X + c * Y))
-    return out
-
-
 @pytest.mark.parametrize("wav,dim,bs,eps", [("CDF44", 2, 16, 1e-3), ("CDF44", 2, 24, 1e-3),
                                             ("CDF22", 2, 16, 1e-2), ("CDF40", 2, 16, 1e-3),
                                             ("CDF62", 2, 24, 1e-3), ("CDF44", 3, 16, 1e-2)])
@@ -238,7 +210,7 @@ def test_adapt_threshold_matches_oracle(wav, dim, bs, eps):
     tcs, lev = random_graded_forest(dim, jmin, jmax, 3, 8 if dim == 2 else 3)
     n = tcs.shape[0]
     neq = 2
-    hvy = _smooth_blocks(tcs, lev, jmax, dim, bs, g, 4, neq)
+    hvy = smooth_blocks(tcs, lev, jmax, dim, bs, g, 4, neq)
     t = po.Tree(po.wavelet(wav), dim, jmin, jmax, [bs] * dim, g, tcs, lev, hvy, indicator=1, eps=eps,
                 groups=[1, 1])
     t.sync()

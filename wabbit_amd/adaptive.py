@@ -36,6 +36,7 @@ class AdaptConfig:
     security_zone: Optional[bool] = None   # default: lifted wavelet
     refinement_indicator: str = "everywhere"
     decompose_branch: int = 0              # 0 the reference's rule, 1 leaf-first, 2 level-wise
+    eps_norm: int = 0                      # WH_NORM_*: params%eps_norm (Linfty by default)
 
 
 def fd_reach(scheme: int) -> int:
@@ -77,7 +78,7 @@ class AdaptiveACM:
             force_maxlevel_dealiasing=cfg.force_maxlevel_dealiasing, eps_normalized=cfg.eps_normalized,
             threshold_mask=cfg.threshold_mask and mask is not None,
             security_zone=lifted if cfg.security_zone is None else cfg.security_zone,
-            domain=acm.domain_size, mask=mask, decompose_branch=cfg.decompose_branch)
+            domain=acm.domain_size, mask=mask, decompose_branch=cfg.decompose_branch, eps_norm=cfg.eps_norm)
         self.acm_c = L.acm_params_c(acm)
         self.butcher = butcher_rk4()
         self.tp_c = L.time_params_c(tp)

@@ -978,10 +978,10 @@ wh_box interior_box(const Tree &T, int i)
     return b;
 }
 
-// componentWiseNorm_tree(hvy_tmp, "Linfty") over the leaves
-// (OPERATORS/componentWiseNorm_tree.f90): the leaves' interior boxes, and
-// the norm from their per-box maxima m, then norm <= 1e-9 -> 1
-// (coarseningIndicator_tree.f90:160-163)
+// componentWiseNorm_tree(hvy_tmp, eps_norm) over the leaves
+// (OPERATORS/componentWiseNorm_tree.f90): the leaves' interior boxes (their
+// per-box maxima or sums are reduced on the device, accumulate_norm makes
+// the norm of them, then norm <= 1e-9 -> 1, coarseningIndicator_tree.f90:160-163)
 std::vector<wh_box> leaf_boxes(const Tree &T)
 {
     std::vector<wh_box> boxes;
@@ -990,25 +990,13 @@ std::vector<wh_box> leaf_boxes(const Tree &T)
     return boxes;
 }
 
-void leaf_linfty(size_t nbox, int neq, const int32_t *groups, const double *m, double *norm)
+// the levels of the leaf boxes, in leaf_boxes' order
+std::vector<int> leaf_levels(const Tree &T)
 {
-    for (int c = 0; c < neq; ++c) norm[c] = -1.0;
-    for (size_t k = 0; k < nbox; ++k) {
-        const double *v = &m[k * neq];
-        for (int c = 0; c < neq; ++c)
-            if (groups[c] == 1) norm[c] = std::max(norm[c], v[c]);
-        int maxg = 0;
-        for (int c = 0; c < neq; ++c) maxg = std::max(maxg, (int)groups[c]);
-        for (int l = 2; l <= maxg; ++l) {
-            double mm = 0.0;
-            for (int c = 0; c < neq; ++c)
-                if (groups[c] == l) mm = std::max(mm, v[c]);
-            for (int c = 0; c < neq; ++c)
-                if (groups[c] == l) norm[c] = std::max(norm[c], mm);
-        }
-    }
-    for (int c = 0; c < neq; ++c)
-        if (norm[c] <= 1.0e-9) norm[c] = 1.0;
+    std::vector<int> lv;
+    for (int i = 0; i < (int)T.b.size(); ++i)
+        if (T.is_leaf(i, false)) lv.push_back(T.b[i].level);
+    return lv;
 }
 
 // coarseningIndicatorMask_tree (coarseningIndicator_tree.f90:338-422): the
@@ -1165,7 +1153,8 @@ int check_adapt(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_
     WH_CHECK(p && p->jmin >= 0 && p->jmax <= F->jmax && p->jmin <= p->jmax &&
                  (p->indicator == WH_IND_EVERYWHERE || p->indicator == WH_IND_THRESHOLD ||
                   (p->indicator == WH_IND_EXTERNAL && leaf_flags)) &&
-                 p->decompose_branch >= 0 && p->decompose_branch <= 2,
+                 p->decompose_branch >= 0 && p->decompose_branch <= 2 && p->eps_norm >= WH_NORM_LINFTY &&
+                 p->eps_norm <= WH_NORM_H1,
              WH_E_ARG, "wh_adapt_tree: bad parameters");
     for (const FBlock &b : F->blocks)
         WH_CHECK(b.level >= p->jmin, WH_E_ARG, "wh_adapt_tree: a block is below jmin");
@@ -1176,8 +1165,10 @@ int check_adapt(wh_pool *P, wh_forest *F, const wh_adapt_params *p, const int32_
     return WH_OK;
 }
 
+// forests whose reductions can share a launch: equal groups and norm
 bool same_groups(const wh_adapt_params &a, const wh_adapt_params &b, int neq)
 {
+    if (a.eps_norm != b.eps_norm) return false;
     for (int c = 0; c < neq; ++c)
         if (a.groups[c] != b.groups[c]) return false;
     return true;
@@ -1195,7 +1186,11 @@ struct TreeRun {
     std::vector<int> idx, midx;
     std::vector<int32_t> mids;
     std::vector<wh_box> nboxes, zboxes;
+    std::vector<int> nlev;
     std::vector<std::pair<int, int>> zwho;
+    // results per detail or zone box: one, or one per point type where the
+    // norm renormalises (the host scales the typed maxima)
+    int ntyp(int dim) const { return p->eps_norm != WH_NORM_LINFTY ? (1 << dim) - 1 : 1; }
     std::string err;  // the message of rc (set on the thread that failed)
     bool alive() const { return rc == WH_OK && T; }
 };
@@ -1204,7 +1199,7 @@ struct TreeRun {
 // with equal groups in one launch; forest t's maxima at d_out + off(t), the
 // forests' ranges one after the other in forest order.
 int region_max_runs(wh_pool *P, std::vector<TreeRun *> &R, std::vector<wh_box> TreeRun::*boxes, int array,
-                    bool zero_sc, bool sqrt_groups, double *d_out)
+                    bool zone, double *d_out)
 {
     const int neq = P->d.neq;
     size_t off = 0;
@@ -1213,8 +1208,9 @@ int region_max_runs(wh_pool *P, std::vector<TreeRun *> &R, std::vector<wh_box> T
         while (b < R.size() && same_groups(*R[a]->p, *R[b]->p, neq)) ++b;
         std::vector<wh_box> all;
         for (size_t t = a; t < b; ++t) all.insert(all.end(), (R[t]->*boxes).begin(), (R[t]->*boxes).end());
-        if (int rc = launch_region_max(P, array, all, zero_sc, sqrt_groups, R[a]->p->groups, d_out + off)) return rc;
-        off += all.size() * neq;
+        if (int rc = launch_region_norm(P, array, all, R[a]->p->eps_norm, zone, R[a]->p->groups, d_out + off))
+            return rc;
+        off += all.size() * neq * (zone ? R[a]->ntyp(P->d.dim) : 1);
         a = b;
     }
     return WH_OK;
@@ -1233,7 +1229,7 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
                      const int32_t *const *flags, int *status)
 {
     const int stay = REF_UNSIGNIFICANT_STAY;
-    const int neq = P->d.neq;
+    const int neq = P->d.neq, dim = P->d.dim;
     std::vector<TreeRun> R(n);
     auto fail = [&](TreeRun &t, int rc) {
         if (t.rc != WH_OK) return;
@@ -1301,7 +1297,10 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
                 if (T.is_leaf(i, false)) T.b[i].status = -1;
         } else if (p.indicator == WH_IND_THRESHOLD) {
             for (int c = 0; c < 8; ++c) t->norm[c] = p.norm[c];
-            if (p.eps_normalized) t->nboxes = leaf_boxes(T);
+            if (p.eps_normalized) {
+                t->nboxes = leaf_boxes(T);
+                t->nlev = leaf_levels(T);
+            }
             for (int i = 0; i < (int)T.b.size(); ++i) {
                 if (p.force_maxlevel_dealiasing && T.b[i].level == p.jmax) T.b[i].status = -1;
                 else t->idx.push_back(i);
@@ -1314,13 +1313,13 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
         size_t n_norm = 0, n_det = 0, n_mask = 0;
         for (TreeRun *t : thr) {
             n_norm += t->nboxes.size() * neq;
-            n_det += t->idx.size() * neq;
+            n_det += t->idx.size() * neq * t->ntyp(dim);
             if (t->p->threshold_mask) n_mask += t->idx.size() * 3;
         }
         double *d_res = nullptr;
         std::vector<double> res;
         int rc = results_reserve(P, n_norm + n_det + n_mask, &d_res);
-        if (!rc) rc = region_max_runs(P, thr, &TreeRun::nboxes, A_T, false, false, d_res);
+        if (!rc) rc = region_max_runs(P, thr, &TreeRun::nboxes, A_T, false, d_res);
         size_t off = n_norm;
         for (size_t a = 0; a < thr.size() && !rc;) {  // details: consecutive forests with equal groups
             size_t b = a + 1;
@@ -1330,8 +1329,9 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
                 const std::vector<int32_t> h = hvy_of(*thr[t]->T, thr[t]->idx);
                 ids.insert(ids.end(), h.begin(), h.end());
             }
-            rc = launch_detail_list(P, A_U, ids, thr[a]->p->groups, d_res + off);
-            off += ids.size() * neq;
+            const int nt = thr[a]->ntyp(dim);
+            rc = launch_detail_list(P, A_U, ids, thr[a]->p->groups, d_res + off, nt > 1);
+            off += ids.size() * neq * nt;
             a = b;
         }
         std::vector<int32_t> mids;
@@ -1357,15 +1357,28 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
                 o_det[i] = b;
                 o_mask[i] = c;
                 a += thr[i]->nboxes.size() * neq;
-                b += thr[i]->idx.size() * neq;
+                b += thr[i]->idx.size() * neq * thr[i]->ntyp(dim);
                 if (thr[i]->p->threshold_mask) c += thr[i]->idx.size() * 3;
             }
             parallel_for((int)thr.size(), [&](int i) {
                 TreeRun *t = thr[i];
                 Tree &T = *t->T;
                 const wh_adapt_params &p = *t->p;
-                if (p.eps_normalized) leaf_linfty(t->nboxes.size(), neq, p.groups, res.data() + o_norm[i], t->norm);
+                if (p.eps_normalized) {
+                    accumulate_norm(res.data() + o_norm[i], (int)t->nboxes.size(), neq, p.groups, t->nlev.data(), dim,
+                                    T.bs, p.domain, p.eps_norm, t->norm);
+                    for (int c = 0; c < neq; ++c)
+                        if (t->norm[c] <= 1.0e-9) t->norm[c] = 1.0;
+                }
                 double *det = res.data() + o_det[i];
+                std::vector<double> renormed;
+                if (t->ntyp(dim) > 1) {  // wavelet_renorm_block of the typed maxima
+                    std::vector<int> lv(t->idx.size());
+                    for (size_t k = 0; k < t->idx.size(); ++k) lv[k] = T.b[t->idx[k]].level;
+                    renormed.resize(t->idx.size() * neq);
+                    renorm_typed(det, (int)t->idx.size(), neq, dim, p.eps_norm, lv.data(), p.groups, renormed.data());
+                    det = renormed.data();
+                }
                 detail_groups(P, p.groups, det, (int)t->idx.size());
                 for (size_t k = 0; k < t->idx.size(); ++k) {
                     bool all = true;
@@ -1384,11 +1397,11 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
     parallel_for((int)zone.size(), [&](int i) { zone_boxes(*zone[i]->T, *zone[i]->p, zone[i]->zboxes, zone[i]->zwho); });
     if (!zone.empty() && !shared) {
         size_t nz = 0;
-        for (TreeRun *t : zone) nz += t->zboxes.size() * neq;
+        for (TreeRun *t : zone) nz += t->zboxes.size() * neq * t->ntyp(dim);
         double *d_res = nullptr;
         std::vector<double> res;
         int rc = results_reserve(P, nz, &d_res);
-        if (!rc) rc = region_max_runs(P, zone, &TreeRun::zboxes, A_U, true, true, d_res);
+        if (!rc) rc = region_max_runs(P, zone, &TreeRun::zboxes, A_U, true, d_res);
         if (!rc) rc = results_fetch(P, nz, res);
         if (rc) {
             for (TreeRun *t : zone) fail(*t, rc);
@@ -1398,10 +1411,21 @@ int adapt_trees_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_adapt_par
             size_t o = 0;
             for (size_t i = 0; i < zone.size(); ++i) {
                 off[i] = o;
-                o += zone[i]->zboxes.size() * neq;
+                o += zone[i]->zboxes.size() * neq * zone[i]->ntyp(dim);
             }
             parallel_for((int)zone.size(), [&](int i) {
-                zone_apply(*zone[i]->T, *zone[i]->p, zone[i]->norm, zone[i]->zwho, res.data() + off[i]);
+                TreeRun *t = zone[i];
+                double *m = res.data() + off[i];
+                std::vector<double> renormed;
+                if (t->ntyp(dim) > 1) {  // wavelet_renorm_block of the typed maxima, the zone block's level
+                    const int nb = (int)t->zboxes.size();
+                    std::vector<int> lv(nb);
+                    for (int k = 0; k < nb; ++k) lv[k] = t->T->b[t->zwho[k].first].level;
+                    renormed.resize((size_t)nb * neq);
+                    renorm_typed(m, nb, neq, dim, t->p->eps_norm, lv.data(), t->p->groups, renormed.data());
+                    m = renormed.data();
+                }
+                zone_apply(*t->T, *t->p, t->norm, t->zwho, m);
             });
         }
     }

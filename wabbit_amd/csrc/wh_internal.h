@@ -319,7 +319,25 @@ enum { WOP_FWT = 0, WOP_IWT = 1, WOP_FILTER = 2, WOP_COPY = 3 };
 int launch_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
                       const int32_t *groups, double *d_out);
 int launch_detail_list(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
-                       double *d_out);
+                       double *d_out, bool typed = false);
+// eps_norm (WH_NORM_*) reductions of boxes: Linfty as launch_region_max
+// (zone: zero_sc and sqrt_groups); other norms: zone -- the per-type maxima
+// of |v| (2^dim - 1 per box and component), else the per-box sums of |v|
+// (L1) or v*v (L2, H1)
+int launch_region_norm(wh_pool *P, int array, const std::vector<wh_box> &boxes, int kind, bool zone,
+                       const int32_t *groups, double *d_out);
+// wavelet_renorm_block (module_wavelets.f90:1848-1960, level_ref 0) of the
+// max m of |v| over the points of type t (bit d: SC position in direction
+// d) of a block on `level`, in the reference's order of operations
+double renorm_max(double m, int t, int level, int dim, int kind);
+// typed maxima of n blocks -> their renormed details out[k*neq + c]
+// (grouped components as sqrt(r*r), threshold_block.f90:86-91)
+void renorm_typed(const double *typed, int n, int neq, int dim, int kind, const int *levels,
+                  const int32_t *groups, double *out);
+// componentWiseNorm_tree (componentWiseNorm_tree.f90:63-200) from per-block
+// values v[k*neq + c] (maxima for Linfty, sums otherwise), blocks in order
+void accumulate_norm(const double *v, int n, int neq, const int32_t *groups, const int *levels, int dim,
+                     const int bs[3], const double domain[3], int kind, double *norm);
 int launch_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, double *d_out);
 int results_reserve(wh_pool *P, size_t n, double **d_out);
 int results_fetch(wh_pool *P, size_t n, std::vector<double> &out);

@@ -598,6 +598,111 @@ __global__ __launch_bounds__(256) void k_detail(const double *__restrict__ arr,
     if (threadIdx.x == 0) out[blockIdx.x] = __longlong_as_double((long long)smax);
 }
 
+// Point types: bit d set where the point lies on a scaling-coefficient
+// position in direction d (index - g even); the all-SC type (2^dim - 1) is
+// the zeroed SC itself.  wavelet_renorm_block scales all points of one
+// type of one block by one factor, and rounding is monotone, so the max of
+// the renormed values is the renormed max of |v| per type: the typed
+// kernels below return those maxima (2^dim - 1 per block and component)
+// and the host applies the level and direction factors.
+__device__ inline void typed_max(double (&m)[7], int t, double a)
+{
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if (q == t) m[q] = a > m[q] ? a : m[q];
+}
+
+// Per listed block and component: max over the interior of |v| per point
+// type, out[(k*neq + c)*(2^dim - 1) + t].
+__global__ __launch_bounds__(256) void k_detail_typed(const double *__restrict__ arr,
+                                                      const int32_t *__restrict__ ids, ArrGeo A,
+                                                      DetailArgs D, double *__restrict__ out)
+{
+    __shared__ unsigned long long smax[7];
+    if (threadIdx.x < 7) smax[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int k = (int)blockIdx.x / D.neq, c = (int)blockIdx.x - k * D.neq;
+    const int b = ids[k];
+    const int ncol = D.bs[0] * D.bs[1], nz = D.bs[2];
+    const int zoff = D.dim == 3 ? D.g : 0;
+    const int nt = (1 << D.dim) - 1;
+    const double *u = arr + (size_t)b * A.blk_in + (size_t)c * A.cs;
+    double m[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int e = threadIdx.x; e < ncol; e += blockDim.x) {
+        const int iy = e / D.bs[0], ix = e - iy * D.bs[0];
+        const int t2 = ((ix & 1) == 0 ? 1 : 0) | ((iy & 1) == 0 ? 2 : 0);
+        const double *col = u + (size_t)(zoff * A.ny + D.g + iy) * A.nx + (D.g + ix);
+        const size_t zs = (size_t)A.nx * A.ny;
+        for (int iz = 0; iz < nz; ++iz) {
+            const int t = t2 | ((D.dim == 3 && (iz & 1) == 0) ? 4 : 0);
+            if (t != nt) typed_max(m, t, fabs(col[iz * zs]));
+        }
+    }
+    for (int q = 0; q < nt; ++q) atomicMax(&smax[q], (unsigned long long)__double_as_longlong(m[q]));
+    __syncthreads();
+    if (threadIdx.x < nt) out[(size_t)blockIdx.x * nt + threadIdx.x] = __longlong_as_double((long long)smax[threadIdx.x]);
+}
+
+// Per box and component: max over the box of |v| per point type (the
+// security zone's threshold_block with indices), out[(k*neq + c)*nt + t].
+__global__ __launch_bounds__(256) void k_region_max_typed(const double *__restrict__ arr,
+                                                          const wh_box *__restrict__ boxes, ArrGeo A, int g,
+                                                          int dim, int neq, double *__restrict__ out)
+{
+    __shared__ unsigned long long smax[8 * 7];
+    if (threadIdx.x < 8 * 7) smax[threadIdx.x] = 0ull;
+    __syncthreads();
+    const wh_box B = boxes[blockIdx.x];
+    const int n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1, n2 = B.hi[2] - B.lo[2] + 1;
+    const int npts = n0 * n1 * n2;
+    const int nt = (1 << dim) - 1;
+    for (int c = 0; c < neq; ++c) {
+        double m[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const double *u = arr + (size_t)B.hvy * A.blk_in + c * A.cs;
+        for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+            const int ix = B.lo[0] + e % n0, iy = B.lo[1] + (e / n0) % n1, iz = B.lo[2] + e / (n0 * n1);
+            const int t = (((ix - g) & 1) == 0 ? 1 : 0) | (((iy - g) & 1) == 0 ? 2 : 0) |
+                          ((dim == 3 && ((iz - g) & 1) == 0) ? 4 : 0);
+            if (t != nt) typed_max(m, t, fabs(u[((size_t)iz * A.ny + iy) * A.nx + ix]));
+        }
+        for (int q = 0; q < nt; ++q) atomicMax(&smax[c * 7 + q], (unsigned long long)__double_as_longlong(m[q]));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < neq * nt; i += blockDim.x) {
+        const int c = i / nt, q = i - c * nt;
+        out[(size_t)blockIdx.x * neq * nt + i] = __longlong_as_double((long long)smax[c * 7 + q]);
+    }
+}
+
+// Per box and component: the sum over the box of |v| (l1) or v*v (else),
+// componentWiseNorm_tree's per-block sums; a fixed reduction order (each
+// thread's strided partial sum, then a tree over the workgroup).
+__global__ __launch_bounds__(256) void k_region_sum(const double *__restrict__ arr, const wh_box *__restrict__ boxes,
+                                                    ArrGeo A, int neq, int l1, double *__restrict__ out)
+{
+    __shared__ double part[256];
+    const wh_box B = boxes[blockIdx.x];
+    const int n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1, n2 = B.hi[2] - B.lo[2] + 1;
+    const int npts = n0 * n1 * n2;
+    for (int c = 0; c < neq; ++c) {
+        const double *u = arr + (size_t)B.hvy * A.blk_in + c * A.cs;
+        double s = 0.0;
+        for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+            const int ix = B.lo[0] + e % n0, iy = B.lo[1] + (e / n0) % n1, iz = B.lo[2] + e / (n0 * n1);
+            const double v = u[((size_t)iz * A.ny + iy) * A.nx + ix];
+            s = s + (l1 ? fabs(v) : v * v);
+        }
+        part[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * neq + c] = part[0];
+        __syncthreads();
+    }
+}
+
 // Per-box maxima (one workgroup per box): per component the max over the box
 // of |v| (zero_sc: SC points count as 0; sqrt_groups: components of group
 // >= 2 take sqrt(v*v)), as threshold_block with `indices` and the Linfty
@@ -1104,25 +1209,115 @@ int run_detail(wh_pool *P, int array, const std::vector<int32_t> &ids, const int
     return WH_OK;
 }
 
+// checks the boxes against the pool and stages them on the device
+static int stage_boxes(wh_pool *P, int array, const std::vector<wh_box> &boxes, const double **arr, size_t *blk,
+                       const wh_box **d_boxes)
+{
+    const int n = (int)boxes.size(), neq = P->d.neq;
+    WH_CHECK(neq <= 8, WH_E_ARG, "region reduction: at most 8 components");
+    int nc = 0;
+    *arr = array_ptr(P, array, blk, &nc);
+    WH_CHECK(*arr && nc >= neq, WH_E_ARG, "region reduction: bad array");
+    const int lim[3] = {P->nx, P->ny, P->nz};
+    for (const wh_box &b : boxes) {
+        WH_CHECK(b.hvy >= 0 && b.hvy < P->d.nb_max, WH_E_ARG, "region reduction: bad hvy id");
+        for (int d = 0; d < 3; ++d)
+            WH_CHECK(b.lo[d] >= 0 && b.hi[d] < lim[d] && b.lo[d] <= b.hi[d], WH_E_ARG,
+                     "region reduction: bad box");
+    }
+    if (int rc = stage_begin(P, sizeof(wh_box) * n)) return rc;
+    *d_boxes = (const wh_box *)stage_put(P, boxes.data(), sizeof(wh_box) * n);
+    return *d_boxes ? (int)WH_OK : (int)WH_E_HIP;
+}
+
+int launch_region_norm(wh_pool *P, int array, const std::vector<wh_box> &boxes, int kind, bool zone,
+                       const int32_t *groups, double *d_out)
+{
+    if (kind == WH_NORM_LINFTY) return launch_region_max(P, array, boxes, zone, zone, groups, d_out);
+    const int n = (int)boxes.size(), neq = P->d.neq;
+    if (n == 0) return WH_OK;
+    const double *arr = nullptr;
+    size_t blk = 0;
+    const wh_box *d_boxes = nullptr;
+    if (int rc = stage_boxes(P, array, boxes, &arr, &blk, &d_boxes)) return rc;
+    const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
+    if (zone)
+        hipLaunchKernelGGL(k_region_max_typed, dim3(n), dim3(256), 0, P->stream, arr, d_boxes, A, P->d.g, P->d.dim,
+                           neq, d_out);
+    else
+        hipLaunchKernelGGL(k_region_sum, dim3(n), dim3(256), 0, P->stream, arr, d_boxes, A, neq,
+                           kind == WH_NORM_L1 ? 1 : 0, d_out);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
+}
+
+double renorm_max(double m, int t, int level, int dim, int kind)
+{
+    const int jref = 0;
+    double r = m;
+    if (kind == WH_NORM_L1) {
+        r = r * std::pow(2.0, (double)((jref - level - 1) * dim));
+        for (int d = 0; d < dim; ++d)
+            if ((t >> d) & 1) r = r / 4.0;
+    } else if (kind == WH_NORM_L2) {
+        r = r * std::pow(2.0, (double)((jref - level - 1) * dim) / 2.0);
+        for (int d = 0; d < dim; ++d)
+            if ((t >> d) & 1) r = r / 2.0;
+    } else if (kind == WH_NORM_H1 && dim == 3) {
+        r = r * std::pow(2.0, (double)(jref - level) * (2.0 - (double)dim) / 2.0);
+        const double f = std::pow(2.0, 2.0 * ((double)dim - 2.0) / 3.0);
+        for (int d = 0; d < dim; ++d)
+            if ((t >> d) & 1) r = r * f;
+    }
+    return r;
+}
+
+void renorm_typed(const double *typed, int n, int neq, int dim, int kind, const int *levels,
+                  const int32_t *groups, double *out)
+{
+    const int nt = (1 << dim) - 1;
+    for (int k = 0; k < n; ++k)
+        for (int c = 0; c < neq; ++c) {
+            double r = 0.0;
+            for (int t = 0; t < nt; ++t) r = std::max(r, renorm_max(typed[((size_t)k * neq + c) * nt + t], t, levels[k], dim, kind));
+            out[(size_t)k * neq + c] = (groups && groups[c] >= 2) ? std::sqrt(r * r) : r;
+        }
+}
+
+void accumulate_norm(const double *v, int n, int neq, const int32_t *groups, const int *levels, int dim,
+                     const int bs[3], const double domain[3], int kind, double *norm)
+{
+    const bool inf = kind == WH_NORM_LINFTY;
+    int maxg = 0;
+    for (int c = 0; c < neq; ++c) maxg = std::max(maxg, (int)groups[c]);
+    for (int c = 0; c < neq; ++c) norm[c] = inf ? -1.0 : 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double *vk = v + (size_t)k * neq;
+        double vol = 1.0;
+        for (int d = 0; d < dim; ++d) vol = vol * (std::ldexp(1.0, -levels[k]) * domain[d] / (double)bs[d]);
+        for (int c = 0; c < neq; ++c)
+            if (groups[c] == 1) norm[c] = inf ? std::max(norm[c], vk[c]) : norm[c] + vol * vk[c];
+        for (int l = 2; l <= maxg; ++l) {
+            double nb = 0.0;
+            for (int c = 0; c < neq; ++c)
+                if (groups[c] == l) nb = inf ? std::max(nb, vk[c]) : nb + vk[c];
+            for (int c = 0; c < neq; ++c)
+                if (groups[c] == l) norm[c] = inf ? std::max(norm[c], nb) : norm[c] + vol * nb;
+        }
+    }
+    if (kind == WH_NORM_L2 || kind == WH_NORM_H1)
+        for (int c = 0; c < neq; ++c) norm[c] = std::sqrt(norm[c]);
+}
+
 int launch_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool zero_sc, bool sqrt_groups,
                       const int32_t *groups, double *d_out)
 {
     const int n = (int)boxes.size(), neq = P->d.neq;
     if (n == 0) return WH_OK;
-    WH_CHECK(neq <= 8, WH_E_ARG, "region max: at most 8 components");
+    const double *arr = nullptr;
     size_t blk = 0;
-    int nc = 0;
-    const double *arr = array_ptr(P, array, &blk, &nc);
-    WH_CHECK(arr && nc >= neq, WH_E_ARG, "region max: bad array");
-    const int lim[3] = {P->nx, P->ny, P->nz};
-    for (const wh_box &b : boxes) {
-        WH_CHECK(b.hvy >= 0 && b.hvy < P->d.nb_max, WH_E_ARG, "region max: bad hvy id");
-        for (int d = 0; d < 3; ++d)
-            WH_CHECK(b.lo[d] >= 0 && b.hi[d] < lim[d] && b.lo[d] <= b.hi[d], WH_E_ARG, "region max: bad box");
-    }
-    if (int rc = stage_begin(P, sizeof(wh_box) * n)) return rc;
-    const wh_box *d_boxes = (const wh_box *)stage_put(P, boxes.data(), sizeof(wh_box) * n);
-    if (!d_boxes) return WH_E_HIP;
+    const wh_box *d_boxes = nullptr;
+    if (int rc = stage_boxes(P, array, boxes, &arr, &blk, &d_boxes)) return rc;
     RegionArgs R{};
     R.g = P->d.g;
     R.dim = P->d.dim;
@@ -1165,7 +1360,7 @@ int results_fetch(wh_pool *P, size_t n, std::vector<double> &out)
 }
 
 int launch_detail_list(wh_pool *P, int array, const std::vector<int32_t> &ids, const int32_t *groups,
-                       double *d_out)
+                       double *d_out, bool typed)
 {
     const int n = (int)ids.size(), neq = P->d.neq;
     if (n == 0) return WH_OK;
@@ -1176,7 +1371,12 @@ int launch_detail_list(wh_pool *P, int array, const std::vector<int32_t> &ids, c
     WH_CHECK(arr && nc >= neq, WH_E_ARG, "detail: bad array");
     if (int rc = check_ids(P, ids.data(), n)) return rc;
     if (int rc = upload_ids(P, ids.data(), n)) return rc;
-    return launch_detail(P, arr, blk, P->ws.cur_ids, n, detail_args(P, groups), d_out);
+    if (!typed) return launch_detail(P, arr, blk, P->ws.cur_ids, n, detail_args(P, groups), d_out);
+    const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, neq};
+    hipLaunchKernelGGL(k_detail_typed, dim3((unsigned)(n * neq)), dim3(256), 0, P->stream, arr,
+                       (const int32_t *)P->ws.cur_ids, A, detail_args(P, groups), d_out);
+    WH_HIP(hipGetLastError());
+    return WH_OK;
 }
 
 int run_patch_list(wh_pool *P, int array, int alt1, int alt2, const std::vector<wh_patch> &patches_in)
@@ -1352,6 +1552,61 @@ int wh_copy_blocks(wh_pool *pool, int src_array, int dst_array, const int32_t *i
 {
     WH_CHECK(pool && n >= 0 && (n == 0 || ids), WH_E_ARG, "wh_copy_blocks: bad argument");
     return wh::run_blocks_op(pool, wh::WOP_COPY, src_array, dst_array, std::vector<int32_t>(ids, ids + n));
+}
+
+int wh_detail_blocks_norm(wh_pool *pool, int array, const int32_t *ids, const int32_t *levels, int n,
+                          const int32_t *groups, int eps_norm, double *detail)
+{
+    WH_CHECK(pool && n >= 0 && (n == 0 || (ids && levels && detail)), WH_E_ARG,
+             "wh_detail_blocks_norm: bad argument");
+    WH_CHECK(eps_norm >= WH_NORM_LINFTY && eps_norm <= WH_NORM_H1, WH_E_ARG, "wh_detail_blocks_norm: unknown norm");
+    wh_pool *P = pool;
+    const int neq = P->d.neq, dim = P->d.dim, nt = (1 << dim) - 1;
+    if (eps_norm == WH_NORM_LINFTY) return wh_detail_blocks(pool, array, ids, n, groups, detail);
+    if (n == 0) return WH_OK;
+    WH_CHECK(neq <= 8, WH_E_ARG, "wh_detail_blocks_norm: at most 8 components");
+    std::vector<double> typed;
+    double *d_out = nullptr;
+    if (int rc = wh::results_reserve(P, (size_t)n * neq * nt, &d_out)) return rc;
+    if (int rc = wh::launch_detail_list(P, array, std::vector<int32_t>(ids, ids + n), groups, d_out, true)) return rc;
+    if (int rc = wh::results_fetch(P, (size_t)n * neq * nt, typed)) return rc;
+    std::vector<int> lv(levels, levels + n);
+    wh::renorm_typed(typed.data(), n, neq, dim, eps_norm, lv.data(), groups, detail);
+    wh::detail_groups(P, groups, detail, n);
+    return WH_OK;
+}
+
+int wh_component_norm(wh_pool *pool, int array, const int32_t *ids, const int32_t *levels, int n,
+                      const int32_t *groups, int eps_norm, const double *domain, double *norm)
+{
+    WH_CHECK(pool && n >= 0 && norm && domain && (n == 0 || (ids && levels)), WH_E_ARG,
+             "wh_component_norm: bad argument");
+    WH_CHECK(eps_norm >= WH_NORM_LINFTY && eps_norm <= WH_NORM_H1, WH_E_ARG, "wh_component_norm: unknown norm");
+    wh_pool *P = pool;
+    const int neq = P->d.neq, dim = P->d.dim;
+    WH_CHECK(neq <= 8, WH_E_ARG, "wh_component_norm: at most 8 components");
+    int32_t gr[8];
+    for (int c = 0; c < 8; ++c) gr[c] = (groups && c < neq) ? groups[c] : 1;
+    std::vector<wh_box> boxes((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        wh_box &b = boxes[k];
+        b = wh_box{};
+        b.hvy = ids[k];
+        for (int d = 0; d < 3; ++d) {
+            b.lo[d] = d < dim ? P->d.g : 0;
+            b.hi[d] = d < dim ? P->d.g + P->d.bs[d] - 1 : 0;
+        }
+    }
+    std::vector<double> v;
+    if (n > 0) {
+        double *d_out = nullptr;
+        if (int rc = wh::results_reserve(P, (size_t)n * neq, &d_out)) return rc;
+        if (int rc = wh::launch_region_norm(P, array, boxes, eps_norm, false, gr, d_out)) return rc;
+        if (int rc = wh::results_fetch(P, (size_t)n * neq, v)) return rc;
+    }
+    std::vector<int> lv(levels, levels + n);
+    wh::accumulate_norm(v.data(), n, neq, gr, lv.data(), dim, P->d.bs, domain, eps_norm, norm);
+    return WH_OK;
 }
 
 int wh_detail_blocks(wh_pool *pool, int array, const int32_t *ids, int n, const int32_t *groups,

@@ -30,6 +30,7 @@ module wabbit_hip
 
     integer(c_int), parameter :: WH_WAVELET_TAPS = 33
     integer(c_int), parameter :: WH_IND_EVERYWHERE = 0, WH_IND_THRESHOLD = 1, WH_IND_EXTERNAL = 2
+    integer(c_int), parameter :: WH_NORM_LINFTY = 0, WH_NORM_L1 = 1, WH_NORM_L2 = 2, WH_NORM_H1 = 3
 
     type, bind(C) :: wh_wavelet
         integer(c_int) :: X, Y, lifted, specialised
@@ -57,6 +58,7 @@ module wabbit_hip
         real(c_double) :: domain(3)
         type(wh_mask_params) :: mask
         integer(c_int) :: decompose_branch
+        integer(c_int) :: eps_norm
     end type
 
     type, bind(C) :: wh_time_params
@@ -235,6 +237,25 @@ module wabbit_hip
             real(c_double), intent(in) :: thresh(*)
             integer(c_int32_t), intent(out) :: flags_out(*)
             real(c_double), intent(out) :: detail_out(*)
+            integer(c_int) :: rc
+        end function
+        function wh_detail_blocks_norm(pool, array, hvy_ids, levels, n, groups, eps_norm, detail) &
+                bind(C, name="wh_detail_blocks_norm") result(rc)
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: pool
+            integer(c_int), value :: array, n, eps_norm
+            integer(c_int32_t), intent(in) :: hvy_ids(*), levels(*), groups(*)
+            real(c_double), intent(out) :: detail(*)
+            integer(c_int) :: rc
+        end function
+        function wh_component_norm(pool, array, hvy_ids, levels, n, groups, eps_norm, domain, norm) &
+                bind(C, name="wh_component_norm") result(rc)
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: pool
+            integer(c_int), value :: array, n, eps_norm
+            integer(c_int32_t), intent(in) :: hvy_ids(*), levels(*), groups(*)
+            real(c_double), intent(in) :: domain(3)
+            real(c_double), intent(out) :: norm(*)
             integer(c_int) :: rc
         end function
         function wh_forest_refine(forest, n, hvy_ids, triples, cap, ntriples) &

@@ -20,6 +20,7 @@ _BOUND = False
 
 WH_BOX_ZERO_WC, WH_BOX_COPY_SC = 0, 1
 WH_IND_EVERYWHERE, WH_IND_THRESHOLD, WH_IND_EXTERNAL = 0, 1, 2
+WH_NORM_LINFTY, WH_NORM_L1, WH_NORM_L2, WH_NORM_H1 = 0, 1, 2, 3
 
 
 class WaveletC(ctypes.Structure):
@@ -59,7 +60,8 @@ class AdaptParamsC(ctypes.Structure):
                 ("eps", ctypes.c_double), ("norm", ctypes.c_double * 8),
                 ("groups", ctypes.c_int32 * 8), ("eps_normalized", ctypes.c_int),
                 ("threshold_mask", ctypes.c_int), ("security_zone", ctypes.c_int),
-                ("domain", ctypes.c_double * 3), ("mask", MaskParamsC), ("decompose_branch", ctypes.c_int)]
+                ("domain", ctypes.c_double * 3), ("mask", MaskParamsC), ("decompose_branch", ctypes.c_int),
+                ("eps_norm", ctypes.c_int)]
 
 
 def mask_params(geometry=0, x_cntr=(0.0, 0.0, 0.0), R_cyl=0.5, smoothing_width=0.0, smoothing_safety=0.0,
@@ -107,6 +109,8 @@ def _bind():
         "wh_filter_blocks": [_vp, ip, ip, _vp, ip],
         "wh_copy_blocks": [_vp, ip, ip, _vp, ip],
         "wh_detail_blocks": [_vp, ip, _vp, ip, _vp, _vp],
+        "wh_detail_blocks_norm": [_vp, ip, _vp, _vp, ip, _vp, ip, _vp],
+        "wh_component_norm": [_vp, ip, _vp, _vp, ip, _vp, ip, _vp, _vp],
         "wh_apply_boxes": [_vp, ip, ip, _vp, ip],
         "wh_run_patches": [_vp, ip, ip, ip, _vp, ip],
         "wh_adapt_tree": [_vp, _vp, ctypes.POINTER(AdaptParamsC)],
@@ -177,6 +181,36 @@ def detail_blocks(pool, array, ids, groups, neq):
     return out
 
 
+def detail_blocks_norm(pool, array, ids, levels, groups, neq, eps_norm):
+    """threshold_block's detail with wavelet_renorm_block of eps_norm
+    (WH_NORM_*) for blocks on the given levels."""
+    _bind()
+    a = _ids(ids)
+    lv = np.ascontiguousarray(levels, np.int32)
+    g = np.ascontiguousarray(groups, np.int32)
+    assert lv.size == a.size
+    out = np.zeros((a.size, neq))
+    L.check(L.load().wh_detail_blocks_norm(pool.h, array, a.ctypes.data if a.size else None,
+                                           lv.ctypes.data if a.size else None, a.size, g.ctypes.data, int(eps_norm),
+                                           out.ctypes.data))
+    return out
+
+
+def component_norm(pool, array, ids, levels, groups, neq, eps_norm, domain):
+    """componentWiseNorm_tree over the listed blocks (eps_norm WH_NORM_*)."""
+    _bind()
+    a = _ids(ids)
+    lv = np.ascontiguousarray(levels, np.int32)
+    g = np.ascontiguousarray(groups, np.int32)
+    dom = np.ascontiguousarray(list(domain) + [1.0] * (3 - len(domain)), np.float64)
+    assert lv.size == a.size
+    out = np.zeros(neq)
+    L.check(L.load().wh_component_norm(pool.h, array, a.ctypes.data if a.size else None,
+                                       lv.ctypes.data if a.size else None, a.size, g.ctypes.data, int(eps_norm),
+                                       dom.ctypes.data, out.ctypes.data))
+    return out
+
+
 def apply_boxes(pool, array, src_array, boxes):
     _bind()
     b = np.ascontiguousarray(boxes, BOX_DTYPE)
@@ -192,13 +226,14 @@ def run_patches(pool, array, patches, alt1=-1, alt2=-1):
 def adapt_params(jmin, jmax, indicator=WH_IND_THRESHOLD, use_ce=True, eps=1e-3, norm=None,
                  groups=None, force_maxlevel_dealiasing=False, neq=1, eps_normalized=False,
                  threshold_mask=False, security_zone=None, domain=(1.0, 1.0, 1.0),
-                 mask=None, decompose_branch=0) -> AdaptParamsC:
+                 mask=None, decompose_branch=0, eps_norm=0) -> AdaptParamsC:
     """adapt_tree settings; security_zone defaults to use_ce, as the
     reference's useSecurityZone defaults to the lifted-wavelet flag
     (ini_file_to_params.f90:543-546)."""
     p = AdaptParamsC()
     p.jmin, p.jmax, p.indicator = jmin, jmax, indicator
     p.decompose_branch = int(decompose_branch)
+    p.eps_norm = int(eps_norm)
     p.use_ce = int(use_ce)
     p.eps_normalized = int(eps_normalized)
     p.threshold_mask = int(threshold_mask)
