@@ -481,9 +481,62 @@ void ce_ghost_box(const Tree &T, int i, const int o[3], wh_box &bx)
 
 enum CeCase { CE_TREE = 0, CE_LEVEL = 1, CE_REF = 2 };
 // coarse_extension_modify (reconstruction_step.f90:3-100) on WH_BLOCK
+// The cache of generated lists (TreeSyncCache) of a tree's forest: the key
+// is tag and parameters (spec) followed by every node's treecode, level and
+// status in order; the lists name blocks by node index.
+void tree_key(const Tree &T, std::initializer_list<int64_t> spec, std::vector<int64_t> &key)
+{
+    key.assign(spec.begin(), spec.end());
+    const int64_t geo[] = {T.w.order, T.w.lifted, T.w.Nscl, T.w.Nscr, T.w.Nwcl, T.w.Nwcr, T.dim, T.g,
+                           T.bs[0], T.bs[1], T.bs[2], T.P->nx, T.P->ny, T.P->nz, (int64_t)T.b.size()};
+    key.insert(key.end(), geo, geo + sizeof(geo) / sizeof(geo[0]));
+    key.reserve(key.size() + 2 * T.b.size());
+    for (const TNode &x : T.b) {
+        key.push_back(x.tc);
+        key.push_back(((int64_t)x.level << 32) | (int64_t)(uint32_t)x.status);
+    }
+}
+
+TreeSyncCache *cache_find(Tree &T, const std::vector<int64_t> &key)
+{
+    for (TreeSyncCache &c : T.F->tscache)
+        if (c.key == key) {
+            c.used = ++T.F->ts_clock;
+            return &c;
+        }
+    return nullptr;
+}
+
+// stores an entry (the least recently used one dropped beyond TS_CACHE)
+TreeSyncCache &cache_put(Tree &T, TreeSyncCache &&c)
+{
+    std::vector<TreeSyncCache> &cache = T.F->tscache;
+    c.used = ++T.F->ts_clock;
+    if ((int)cache.size() < wh_forest::TS_CACHE) {
+        cache.push_back(std::move(c));
+        return cache.back();
+    }
+    TreeSyncCache *lru = &cache[0];
+    for (TreeSyncCache &e : cache)
+        if (e.used < lru->used) lru = &e;
+    *lru = std::move(c);
+    return *lru;
+}
+
+// node indices of cached boxes -> the nodes' pool slots
+std::vector<wh_box> boxes_of(const Tree &T, const std::vector<wh_box> &src)
+{
+    std::vector<wh_box> out(src);
+    for (wh_box &bx : out) bx.hvy = T.b[bx.hvy].hvy;
+    return out;
+}
+
 int ce_modify(Tree &T, CeCase ce, int s_level, int s_ref, bool clear_wc, bool copy_sc)
 {
     WH_PROF("ce_modify");
+    std::vector<int64_t> key;
+    tree_key(T, {2, (int64_t)ce, s_level, s_ref, clear_wc, copy_sc}, key);
+    if (const TreeSyncCache *c = cache_find(T, key)) return dev_boxes(T, A_U, A_T, boxes_of(T, c->fboxes[0]));
     int dirs[26][3];
     const int nd = dirs_of(T.dim, dirs);
     std::vector<wh_box> boxes;
@@ -496,7 +549,7 @@ int ce_modify(Tree &T, CeCase ce, int s_level, int s_ref, bool clear_wc, bool co
             if (T.nb_coarse(i, o) < 0) continue;
             if (T.valid(T.nb_same(i, o)) || T.any_finer(i, o, true)) continue;
             wh_box bx{};
-            bx.hvy = T.b[i].hvy;
+            bx.hvy = i;  // node index, mapped to its slot on use
             if (clear_wc) {
                 bx.op = WH_BOX_ZERO_WC;
                 modify_box(T, o, T.w.Nwcl, T.w.Nwcr, bx);
@@ -511,7 +564,10 @@ int ce_modify(Tree &T, CeCase ce, int s_level, int s_ref, bool clear_wc, bool co
             }
         }
     }
-    return dev_boxes(T, A_U, A_T, std::move(boxes));
+    TreeSyncCache e;
+    e.key = std::move(key);
+    e.fboxes[0] = std::move(boxes);
+    return dev_boxes(T, A_U, A_T, boxes_of(T, cache_put(T, std::move(e)).fboxes[0]));
 }
 
 // ---- ghost syncs -------------------------------------------------------------
@@ -522,9 +578,42 @@ int src_sel(const Tree &T, const SyncSpec &S, int sender, bool res_pred)
     return 0;
 }
 
+// launches a cached sync: its node indices mapped to the nodes' pool slots
+int tree_sync_run(Tree &T, const TreeSyncCache &c)
+{
+    auto patches = [&](const std::vector<wh_patch> &src) {
+        std::vector<wh_patch> out(src);
+        for (wh_patch &p : out) {
+            p.recv = T.b[p.recv].hvy;
+            p.send = T.b[p.send].hvy;
+        }
+        return out;
+    };
+    if (int rc = dev_patches(T, A_U, A_T, A_FILT, patches(*c.st1))) return rc;
+    // restrict_copy_at_CE's filtered senders, where the sync has any
+    if (!c.fids[0].empty() || !c.fids[1].empty()) {
+        for (int k = 0; k < 2; ++k) {
+            const int arr = k == 0 ? A_U : A_T;
+            std::vector<int32_t> ids(c.fids[k]);
+            for (int32_t &i : ids) i = T.b[i].hvy;
+            if (int rc = dev_blocks(T, WOP_FILTER, arr, A_FILT, std::move(ids))) return rc;
+            if (int rc = dev_boxes(T, A_FILT, arr, boxes_of(T, c.fboxes[k]))) return rc;
+        }
+    }
+    return dev_patches(T, A_U, A_T, A_FILT, patches(*c.st23));
+}
+
 int tree_sync(Tree &T, const SyncSpec &S)
 {
     WH_PROF("tree_sync");
+    std::vector<int64_t> key;
+    tree_key(T, {1, S.same, S.restrict_, S.predict, S.recv_ref_on, S.recv_level_on, S.recv_ref, S.recv_level,
+                 (int64_t)S.src, S.src_ref, S.filter, S.gm, S.gp},
+             key);
+    if (const TreeSyncCache *c = cache_find(T, key)) {
+        WH_PROF("tree_sync.hit");
+        return tree_sync_run(T, *c);
+    }
     const int dim = T.dim, g = T.g, order = T.w.order;
     const bool filt = S.filter && T.w.lifted;
     int dirs[26][3];
@@ -567,8 +656,8 @@ int tree_sync(Tree &T, const SyncSpec &S)
             if (T.valid(ns)) {
                 if (!S.same) continue;
                 wh_patch p{};
-                p.recv = R.hvy;
-                p.send = T.b[ns].hvy;
+                p.recv = r;  // node indices; tree_sync_run maps them to slots
+                p.send = ns;
                 for (int d = 0; d < 3; ++d) {
                     p.recv_lo[d] = rlo[d];
                     p.n[d] = rn[d];
@@ -590,8 +679,8 @@ int tree_sync(Tree &T, const SyncSpec &S)
                     const auto &c = *cp;
                     const int f = c.first;
                     wh_patch p{};
-                    p.recv = R.hvy;
-                    p.send = T.b[f].hvy;
+                    p.recv = r;
+                    p.send = f;
                     for (int d = 0; d < 3; ++d) {
                         if (d >= dim) {
                             p.recv_lo[d] = 0;
@@ -628,8 +717,8 @@ int tree_sync(Tree &T, const SyncSpec &S)
             const int c = T.cover(r, o);
             if (!T.valid(c) || !T.is_leaf(c, true)) continue;
             wh_patch p{};
-            p.recv = R.hvy;
-            p.send = T.b[c].hvy;
+            p.recv = r;
+            p.send = c;
             for (int d = 0; d < 3; ++d) {
                 p.recv_lo[d] = rlo[d];
                 p.n[d] = rn[d];
@@ -646,8 +735,10 @@ int tree_sync(Tree &T, const SyncSpec &S)
             st23.push_back(p);
         }
     }
-    gen.reset();
-    if (int rc = dev_patches(T, A_U, A_T, A_FILT, std::move(st1))) return rc;
+    TreeSyncCache c;
+    c.key = std::move(key);
+    c.st1 = std::make_shared<const std::vector<wh_patch>>(std::move(st1));
+    c.st23 = std::make_shared<const std::vector<wh_patch>>(std::move(st23));
     if (filt && !restrict_senders.empty()) {
         // restrict_copy_at_CE: HD-filtered copy of every restriction sender,
         // original SC points put back near level jumps of leaf senders
@@ -655,28 +746,27 @@ int tree_sync(Tree &T, const SyncSpec &S)
         restrict_senders.erase(std::unique(restrict_senders.begin(), restrict_senders.end()),
                                restrict_senders.end());
         for (int arr : {A_U, A_T}) {
-            std::vector<int32_t> ids;
-            std::vector<wh_box> boxes;
+            std::vector<int32_t> &ids = c.fids[arr == A_U ? 0 : 1];
+            std::vector<wh_box> &boxes = c.fboxes[arr == A_U ? 0 : 1];
             for (int s : restrict_senders) {
                 if ((src_sel(T, S, s, true) == 1 ? A_T : A_U) != arr) continue;
-                ids.push_back(T.b[s].hvy);
+                ids.push_back(s);
                 if (!T.is_leaf(s, false)) continue;
                 for (int k = 0; k < nd; ++k) {
                     const int *o = dirs[k];
                     if (T.valid(T.nb_same(s, o))) continue;
                     if (T.nb_coarse(s, o) < 0 && !T.any_finer(s, o, false)) continue;
                     wh_box bx{};
-                    bx.hvy = T.b[s].hvy;
+                    bx.hvy = s;
                     bx.op = WH_BOX_COPY_SC;
                     modify_box(T, o, T.w.Nscl, T.w.Nscr, bx);
                     boxes.push_back(bx);
                 }
             }
-            if (int rc = dev_blocks(T, WOP_FILTER, arr, A_FILT, std::move(ids))) return rc;
-            if (int rc = dev_boxes(T, A_FILT, arr, std::move(boxes))) return rc;
         }
     }
-    return dev_patches(T, A_U, A_T, A_FILT, std::move(st23));
+    gen.reset();
+    return tree_sync_run(T, cache_put(T, std::move(c)));
 }
 
 // ---- adapt_tree pieces ----------------------------------------------------------

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -26,6 +27,21 @@ struct FBlock {
 };
 
 inline uint64_t fkey(int level, int64_t tc) { return ((uint64_t)level << 58) ^ (uint64_t)tc; }
+
+// The generated lists of one full-tree ghost sync of adapt_tree / sync_tree
+// with blocks named by their node index in the tree (recv, send, box and
+// filter ids), keyed by everything the generation reads: the sync's
+// parameters, the pool's geometry and wavelet, and the tree's nodes
+// (treecode, level, status) in order.  A step whose tree equals an earlier
+// one's reuses the lists (mapped to the nodes' current pool slots) instead
+// of generating them again.
+struct TreeSyncCache {
+    std::vector<int64_t> key;
+    std::shared_ptr<const std::vector<wh_patch>> st1, st23;
+    std::vector<int32_t> fids[2];  // HD-filtered senders per source array
+    std::vector<wh_box> fboxes[2];
+    uint64_t used = 0;
+};
 }  // namespace wh
 
 struct wh_forest {
@@ -46,6 +62,12 @@ struct wh_forest {
     mutable std::vector<wh_patch> pcache;
     mutable int pc_gm = -1, pc_gp = -1, pc_order = -1;
     mutable uint64_t pc_version = ~0ull;
+    // generated lists of adapt_tree's full-tree ghost syncs and coarse
+    // extension boxes (least recently used dropped beyond TS_CACHE; a step
+    // of adapt_tree keys about two dozen)
+    static constexpr int TS_CACHE = 32;
+    std::vector<wh::TreeSyncCache> tscache;
+    uint64_t ts_clock = 0;
 
     // (level, grid index) -> index in blocks as a dense table per level with
     // at most 2^DENSE_BITS cells (the hash map beyond); the entries set by the
