@@ -157,3 +157,38 @@ def test_treecode_digits_beyond_level_rejected():
             f.set_blocks(np.array([bad], np.int64), np.array([1], np.int32), np.zeros(1, np.int32))
         assert f.n == 1 and f.blocks()[0][0] == good[0]
     f.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_patch_lists_reused_for_a_repeated_block_list(dim):
+    """The patch generation is reused when the block list repeats in
+    treecodes and levels (as after a refine/coarsen cycle that lands on the
+    same grid) but sits in other pool slots: the lists name the new slots,
+    equal to a fresh forest's, and a changed block list is generated anew."""
+    bs, g, jmax = (8, 3, 5) if dim == 2 else (6, 3, 3)
+    tcs, lev = random_graded_forest(dim, 1, jmax, 4, nref=6 if dim == 2 else 3)
+    n = tcs.shape[0]
+    nb = 3 * n
+    rng = np.random.default_rng(9)
+    f = Forest(dim, jmax, [bs] * dim, g, [1.0] * dim, nb_max=nb)
+    try:
+        f.set_blocks(tcs, lev, np.arange(n, dtype=np.int32))
+        first = f.patches(g, g, 4)
+        for trial in range(3):
+            slots = rng.permutation(nb)[:n].astype(np.int32)
+            f.set_blocks(tcs, lev, slots)
+            got = f.patches(g, g, 4)
+            fresh = Forest(dim, jmax, [bs] * dim, g, [1.0] * dim, nb_max=nb)
+            fresh.set_blocks(tcs, lev, slots)
+            want = fresh.patches(g, g, 4)
+            fresh.close()
+            assert np.array_equal(got, want), trial
+            assert got.size == first.size
+        # a different grid (one block fewer) is not taken for the cached one
+        f.set_blocks(tcs[:-1], lev[:-1], np.arange(n - 1, dtype=np.int32))
+        fresh = Forest(dim, jmax, [bs] * dim, g, [1.0] * dim, nb_max=nb)
+        fresh.set_blocks(tcs[:-1], lev[:-1], np.arange(n - 1, dtype=np.int32))
+        assert np.array_equal(f.patches(g, g, 4), fresh.patches(g, g, 4))
+        fresh.close()
+    finally:
+        f.close()

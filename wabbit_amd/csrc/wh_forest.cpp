@@ -182,6 +182,38 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
     }
     const int dim = F->dim, g = F->g;
     const int N = order / 2;
+    // the lists name blocks by their index in the block list: a block list
+    // equal in treecodes and levels to the last generated one reuses them
+    std::vector<int64_t> key = {gminus, gplus, order, dim, g, F->bs[0], F->bs[1], F->bs[2], F->jmax,
+                                F->periodic[0], F->periodic[1], F->periodic[2], (int64_t)F->blocks.size()};
+    key.reserve(key.size() + 2 * F->blocks.size());
+    for (const Block &b : F->blocks) {
+        key.push_back(b.tc);
+        key.push_back(b.level);
+    }
+    auto finish = [&](const std::vector<wh_patch> &idx) -> int {
+        std::vector<wh_patch> P(idx);
+        for (wh_patch &p : P) {
+            p.recv = F->blocks[p.recv].hvy;
+            p.send = F->blocks[p.send].hvy;
+        }
+        *npatch = (int)P.size();
+        if (out) {
+            if (cap < (int)P.size()) {
+                wh::set_error("wh_forest_patches: capacity too small");
+                return WH_E_ARG;
+            }
+            std::copy(P.begin(), P.end(), out);
+        }
+        F->pcache.swap(P);
+        F->pc_gm = gminus;
+        F->pc_gp = gplus;
+        F->pc_order = order;
+        F->pc_version = F->version;
+        return WH_OK;
+    };
+    if (key == F->pc_key) return finish(F->pc_idx);
+    F->pc_key.clear();
     std::vector<wh_patch> P;
     std::vector<wh_patch> fp;
     int dirs[27][3], nd = 0;
@@ -195,7 +227,8 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
                 ++nd;
             }
     const int nxyz[3] = {F->bs[0] + 2 * g, F->bs[1] + 2 * g, dim == 3 ? F->bs[2] + 2 * g : 1};
-    for (const Block &b : F->blocks) {
+    for (int bi = 0; bi < (int)F->blocks.size(); ++bi) {
+        const Block &b = F->blocks[bi];
         int ijk[3];
         F->ijk_of(b.tc, b.level, ijk);
         for (int q = 0; q < nd; ++q) {
@@ -208,8 +241,8 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
             int s = F->find(b.level, nb, w);
             if (s >= 0) {
                 wh_patch p{};
-                p.recv = b.hvy;
-                p.send = F->blocks[s].hvy;
+                p.recv = bi;  // block indices, mapped to hvy ids by finish
+                p.send = s;
                 for (int d = 0; d < 3; ++d) {
                     p.recv_lo[d] = rlo[d];
                     p.n[d] = rn[d];
@@ -253,8 +286,8 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
                                 break;
                             }
                             wh_patch p{};
-                            p.recv = b.hvy;
-                            p.send = F->blocks[f].hvy;
+                            p.recv = bi;
+                            p.send = f;
                             bool ok = true;
                             for (int d = 0; d < 3; ++d) {
                                 if (d >= dim) {
@@ -301,8 +334,8 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
                 const int c = F->find(b.level - 1, cij, cw);
                 if (c >= 0) {
                     wh_patch p{};
-                    p.recv = b.hvy;
-                    p.send = F->blocks[c].hvy;
+                    p.recv = bi;
+                    p.send = c;
                     bool ok = true;
                     for (int d = 0; d < 3; ++d) {
                         p.recv_lo[d] = rlo[d];
@@ -337,20 +370,9 @@ int wh_forest_patches(const wh_forest *F, int gminus, int gplus, int order, wh_p
             // no cover: non-periodic domain boundary, left to boundary conditions
         }
     }
-    *npatch = (int)P.size();
-    if (out) {
-        if (cap < (int)P.size()) {
-            wh::set_error("wh_forest_patches: capacity too small");
-            return WH_E_ARG;
-        }
-        std::copy(P.begin(), P.end(), out);
-    }
-    F->pcache.swap(P);
-    F->pc_gm = gminus;
-    F->pc_gp = gplus;
-    F->pc_order = order;
-    F->pc_version = F->version;
-    return WH_OK;
+    F->pc_idx.swap(P);
+    F->pc_key.swap(key);
+    return finish(F->pc_idx);
 }
 
 // Refine the listed blocks (refinementExecute.f90:4-116 bookkeeping): each

@@ -29,8 +29,9 @@ struct FBlock {
 inline uint64_t fkey(int level, int64_t tc) { return ((uint64_t)level << 58) ^ (uint64_t)tc; }
 
 // The generated lists of one full-tree ghost sync of adapt_tree / sync_tree
-// with blocks named by their node index in the tree (recv, send, box and
-// filter ids), keyed by everything the generation reads: the sync's
+// (or of coarse-extension boxes) with blocks
+// named by their node index in the tree (recv, send, box and filter ids),
+// keyed by everything the generation reads: the sync's
 // parameters, the pool's geometry and wavelet, and the tree's nodes
 // (treecode, level, status) in order.  A step whose tree equals an earlier
 // one's reuses the lists (mapped to the nodes' current pool slots) instead
@@ -62,6 +63,11 @@ struct wh_forest {
     mutable std::vector<wh_patch> pcache;
     mutable int pc_gm = -1, pc_gp = -1, pc_order = -1;
     mutable uint64_t pc_version = ~0ull;
+    // the last generation with blocks named by block-list index, and its
+    // key (ghost widths, order, geometry, every block's treecode and level):
+    // a block list that repeats in new pool slots reuses it
+    mutable std::vector<wh_patch> pc_idx;
+    mutable std::vector<int64_t> pc_key;
     // generated lists of adapt_tree's full-tree ghost syncs and coarse
     // extension boxes (least recently used dropped beyond TS_CACHE; a step
     // of adapt_tree keys about two dozen)
