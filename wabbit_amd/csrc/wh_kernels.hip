@@ -1693,27 +1693,34 @@ __device__ double block_max(double v)
     return v;
 }
 
+// GET_DT_BLOCK_ACM of active block k.  WAVES 1: one wavefront per block
+// (four blocks per workgroup; the 2D blocks' few hundred points), 0: the
+// whole workgroup per block.  The max and min are exact whatever the order.
+template <int WAVES>
 __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ in,
                                             const int32_t *__restrict__ act,
                                             const double *__restrict__ x0dx, Geo G, AcmDev a0,
                                             unsigned long long *dtmin, const int32_t *__restrict__ grp,
-                                            const AcmDev *__restrict__ gacm)
+                                            const AcmDev *__restrict__ gacm, int nact)
 {
-    const int k = blockIdx.x, b = act[k];
+    const int lane = WAVES ? (int)(threadIdx.x & 63) : (int)threadIdx.x, step = WAVES ? 64 : (int)blockDim.x;
+    const int k = WAVES ? (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6) : (int)blockIdx.x;
+    if (WAVES && k >= nact) return;  // whole wavefronts leave; no barrier follows in this form
+    const int b = act[k];
     const AcmDev a = gacm ? gacm[grp[k]] : a0;  // per-group constants of independent forests
     const int npts = G.bs0 * G.bs1 * G.bs2;
     const int zoff = (G.bs2 > 1) ? G.g : 0;
     const double *u = in + (size_t)b * G.blk;
     double m = -1.0;
-    for (int p = threadIdx.x; p < npts; p += blockDim.x) {
+    for (int p = lane; p < npts; p += step) {
         const int ix = p % G.bs0, iy = (p / G.bs0) % G.bs1, iz = p / (G.bs0 * G.bs1);
         const size_t i = ((size_t)(zoff + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
         double v = u[i] * u[i] + u[G.cs + i] * u[G.cs + i];
         if (a.dim == 3) v = v + u[2 * G.cs + i] * u[2 * G.cs + i];
         m = fmax(m, v);
     }
-    m = block_max(m);
-    if (threadIdx.x == 0) {
+    m = WAVES ? wave_max(m) : block_max(m);
+    if (lane == 0) {
         double dxmin = x0dx[6 * k + 3];
         for (int d = 1; d < a.dim; ++d) dxmin = fmin(dxmin, x0dx[6 * k + 3 + d]);
         const double ueig = sqrt(m) + sqrt(a.c_0 * a.c_0 + m);
@@ -1756,9 +1763,14 @@ int launch_dt(wh_pool *P, const AcmDev &a)
                        9.0e9);
     if (P->nact > 0) {
         const Geo G = make_geo(P);
-        hipLaunchKernelGGL(k_dt, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
-                           P->d_x0dx, G, a, (unsigned long long *)(P->scal + DS_DTMIN_BITS),
-                           (const int32_t *)nullptr, (const AcmDev *)nullptr);
+        if (P->d.dim == 2)
+            hipLaunchKernelGGL(k_dt<1>, dim3((P->nact + 3) / 4), dim3(256), 0, P->stream, P->block, P->d_active,
+                               P->d_x0dx, G, a, (unsigned long long *)(P->scal + DS_DTMIN_BITS),
+                               (const int32_t *)nullptr, (const AcmDev *)nullptr, P->nact);
+        else
+            hipLaunchKernelGGL(k_dt<0>, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
+                               P->d_x0dx, G, a, (unsigned long long *)(P->scal + DS_DTMIN_BITS),
+                               (const int32_t *)nullptr, (const AcmDev *)nullptr, P->nact);
     }
     WH_HIP(hipGetLastError());
     return WH_OK;
@@ -1780,9 +1792,13 @@ int launch_dt_groups(wh_pool *P, const AcmDev &a)
                        bits);
     if (P->nact > 0) {
         const Geo G = make_geo(P);
-        hipLaunchKernelGGL(k_dt, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
-                           P->d_x0dx, G, a, P->d_dtg, (const int32_t *)P->d_grp,
-                           (const AcmDev *)(P->ngacm == P->ngrp ? P->d_gacm : nullptr));
+        const AcmDev *gacm = P->ngacm == P->ngrp ? P->d_gacm : nullptr;
+        if (P->d.dim == 2)
+            hipLaunchKernelGGL(k_dt<1>, dim3((P->nact + 3) / 4), dim3(256), 0, P->stream, P->block, P->d_active,
+                               P->d_x0dx, G, a, P->d_dtg, (const int32_t *)P->d_grp, gacm, P->nact);
+        else
+            hipLaunchKernelGGL(k_dt<0>, dim3(P->nact), dim3(256), 0, P->stream, P->block, P->d_active,
+                               P->d_x0dx, G, a, P->d_dtg, (const int32_t *)P->d_grp, gacm, P->nact);
     }
     WH_HIP(hipGetLastError());
     return WH_OK;

@@ -598,6 +598,41 @@ __global__ __launch_bounds__(256) void k_detail(const double *__restrict__ arr,
     if (threadIdx.x == 0) out[blockIdx.x] = __longlong_as_double((long long)smax);
 }
 
+// k_detail with one wavefront per (block, component), four per workgroup:
+// the 2D blocks' few hundred points; the wavefront's lanes reduce by
+// shuffles (the max is exact in any order)
+__global__ __launch_bounds__(256) void k_detail_wave(const double *__restrict__ arr,
+                                                     const int32_t *__restrict__ ids, ArrGeo A,
+                                                     DetailArgs D, int nitems, double *__restrict__ out)
+{
+    const int item = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    if (item >= nitems) return;  // whole wavefronts; no barrier in this kernel
+    const int k = item / D.neq, c = item - k * D.neq;
+    const int b = ids[k];
+    const int ncol = D.bs[0] * D.bs[1], nz = D.bs[2];
+    const int zoff = D.dim == 3 ? D.g : 0;
+    const bool sq = D.groups[c] >= 2;
+    const double *u = arr + (size_t)b * A.blk_in + (size_t)c * A.cs;
+    double m = 0.0;
+    for (int e = lane; e < ncol; e += 64) {
+        const int iy = e / D.bs[0], ix = e - iy * D.bs[0];
+        const bool sc2 = ((ix & 1) == 0) && ((iy & 1) == 0);
+        const double *col = u + (size_t)(zoff * A.ny + D.g + iy) * A.nx + (D.g + ix);
+        const size_t zs = (size_t)A.nx * A.ny;
+        for (int iz = 0; iz < nz; ++iz) {
+            const bool sc = sc2 && (D.dim == 2 || (iz & 1) == 0);
+            const double v = sc ? 0.0 : col[iz * zs];
+            const double a = sq ? sqrt(v * v) : fabs(v);
+            m = a > m ? a : m;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(m, o, 64);
+        m = t > m ? t : m;
+    }
+    if (lane == 0) out[item] = m;
+}
+
 // Point types: bit d set where the point lies on a scaling-coefficient
 // position in direction d (index - g even); the all-SC type (2^dim - 1) is
 // the zeroed SC itself.  wavelet_renorm_block scales all points of one
@@ -1150,7 +1185,12 @@ int launch_detail(wh_pool *P, const double *arr, size_t blk, const int32_t *d_id
                   double *d_out)
 {
     const ArrGeo A{blk, blk, P->cs, P->nx, P->ny, D.neq};
-    hipLaunchKernelGGL(k_detail, dim3((unsigned)(n * D.neq)), dim3(256), 0, P->stream, arr, d_ids, A, D, d_out);
+    const int nitems = n * D.neq;
+    if (D.dim == 2)
+        hipLaunchKernelGGL(k_detail_wave, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, P->stream, arr, d_ids, A,
+                           D, nitems, d_out);
+    else
+        hipLaunchKernelGGL(k_detail, dim3((unsigned)nitems), dim3(256), 0, P->stream, arr, d_ids, A, D, d_out);
     WH_HIP(hipGetLastError());
     return WH_OK;
 }
