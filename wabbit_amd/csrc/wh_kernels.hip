@@ -1363,6 +1363,32 @@ __global__ __launch_bounds__(256) void k_refine(double *__restrict__ arr,
     for (int c = 0; c < neq; ++c) dst[c * cs] = pred_any(src + c * cs, nx, ny, dim, f, order);
 }
 
+// 2D refineBlock with the mother's component plane staged in LDS: one
+// workgroup per (pair, component), each daughter point pred_point of the
+// staged plane -- k_refine's arithmetic, its stencil reads from LDS
+// instead of up to (2N)^2 cached global loads per point.  The host's
+// stencil-range check keeps every read inside the plane.
+template <int N>
+__global__ __launch_bounds__(256) void k_refine2d_lds(double *__restrict__ arr, const int32_t *__restrict__ pairs,
+                                                      int bs0, int bs1, int g, int nx, int ny, size_t blk, size_t cs,
+                                                      int neq)
+{
+    extern __shared__ double plane[];
+    const int q = (int)blockIdx.x / neq, c = (int)blockIdx.x - q * neq;
+    const int mother = pairs[3 * q], daughter = pairs[3 * q + 1], dg = pairs[3 * q + 2];
+    const int npt = nx * ny;
+    const double *src = arr + (size_t)mother * blk + (size_t)c * cs;
+    for (int e = threadIdx.x; e < npt; e += blockDim.x) plane[e] = src[e];
+    __syncthreads();
+    double *dst = arr + (size_t)daughter * blk + (size_t)c * cs;
+    const int o0 = g + bs0 * ((dg >> 1) & 1), o1 = g + bs1 * (dg & 1);
+    for (int r = threadIdx.x; r < npt; r += blockDim.x) {
+        const int j = r / nx, i = r - j * nx;
+        const int f[3] = {o0 + i, o1 + j, 0};
+        dst[r] = pred_point<N>(plane, nx, ny, 2, f);
+    }
+}
+
 // 3D refineBlock, one thread per (x, y) column of a daughter component,
 // marching in z: the x-then-y interpolated values of the mother's coarse
 // planes (pred_point's vy) pass through a register window of 2N planes, and
@@ -1587,6 +1613,19 @@ int launch_refine(wh_pool *P, double *arr, const int32_t *d_pairs, int npairs, i
         else if (order == 2) LR3(1);
         else LR3(3);
 #undef LR3
+        WH_HIP(hipGetLastError());
+        return WH_OK;
+    }
+    const size_t plane = sizeof(double) * (size_t)P->nx * P->ny;
+    if (P->d.dim == 2 && plane <= 64 * 1024) {
+        const dim3 grid((unsigned)((size_t)P->d.neq * npairs));
+#define LR2(NN)                                                                                              \
+    hipLaunchKernelGGL(k_refine2d_lds<NN>, grid, dim3(256), plane, P->stream, arr, d_pairs, P->d.bs[0], P->d.bs[1], \
+                       P->d.g, P->nx, P->ny, P->blk, P->cs, P->d.neq)
+        if (order == 4) LR2(2);
+        else if (order == 2) LR2(1);
+        else LR2(3);
+#undef LR2
         WH_HIP(hipGetLastError());
         return WH_OK;
     }
