@@ -589,7 +589,7 @@ int tree_sync_run(Tree &T, const TreeSyncCache &c)
         }
         return out;
     };
-    if (int rc = dev_patches(T, A_U, A_T, A_FILT, patches(*c.st1))) return rc;
+    if (int rc = dev_patches(T, A_U, A_T, A_FILT, patches(c.st1))) return rc;
     // restrict_copy_at_CE's filtered senders, where the sync has any
     if (!c.fids[0].empty() || !c.fids[1].empty()) {
         for (int k = 0; k < 2; ++k) {
@@ -600,7 +600,7 @@ int tree_sync_run(Tree &T, const TreeSyncCache &c)
             if (int rc = dev_boxes(T, A_FILT, arr, boxes_of(T, c.fboxes[k]))) return rc;
         }
     }
-    return dev_patches(T, A_U, A_T, A_FILT, patches(*c.st23));
+    return dev_patches(T, A_U, A_T, A_FILT, patches(c.st23));
 }
 
 int tree_sync(Tree &T, const SyncSpec &S)
@@ -737,8 +737,8 @@ int tree_sync(Tree &T, const SyncSpec &S)
     }
     TreeSyncCache c;
     c.key = std::move(key);
-    c.st1 = std::make_shared<const std::vector<wh_patch>>(std::move(st1));
-    c.st23 = std::make_shared<const std::vector<wh_patch>>(std::move(st23));
+    c.st1 = std::move(st1);
+    c.st23 = std::move(st23);
     if (filt && !restrict_senders.empty()) {
         // restrict_copy_at_CE: HD-filtered copy of every restriction sender,
         // original SC points put back near level jumps of leaf senders

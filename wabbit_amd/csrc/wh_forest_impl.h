@@ -9,7 +9,6 @@
 
 #include <cmath>
 #include <cstdint>
-#include <memory>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -38,7 +37,7 @@ inline uint64_t fkey(int level, int64_t tc) { return ((uint64_t)level << 58) ^ (
 // of generating them again.
 struct TreeSyncCache {
     std::vector<int64_t> key;
-    std::shared_ptr<const std::vector<wh_patch>> st1, st23;
+    std::vector<wh_patch> st1, st23;  // sync stage 1, stages 2-3
     std::vector<int32_t> fids[2];  // HD-filtered senders per source array
     std::vector<wh_box> fboxes[2];
     uint64_t used = 0;
