@@ -1734,7 +1734,8 @@ __device__ double block_max(double v)
 
 // GET_DT_BLOCK_ACM of active block k.  WAVES 1: one wavefront per block
 // (four blocks per workgroup; the 2D blocks' few hundred points), 0: the
-// whole workgroup per block.  The max and min are exact whatever the order.
+// whole workgroup per block.  The max and min are exact whatever the order,
+// so the per-group minimum may be combined in any grouping.
 template <int WAVES>
 __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ in,
                                             const int32_t *__restrict__ act,
@@ -1743,32 +1744,58 @@ __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ in,
                                             const AcmDev *__restrict__ gacm, int nact)
 {
     const int lane = WAVES ? (int)(threadIdx.x & 63) : (int)threadIdx.x, step = WAVES ? 64 : (int)blockDim.x;
-    const int k = WAVES ? (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6) : (int)blockIdx.x;
-    if (WAVES && k >= nact) return;  // whole wavefronts leave; no barrier follows in this form
-    const int b = act[k];
-    const AcmDev a = gacm ? gacm[grp[k]] : a0;  // per-group constants of independent forests
-    const int npts = G.bs0 * G.bs1 * G.bs2;
-    const int zoff = (G.bs2 > 1) ? G.g : 0;
-    const double *u = in + (size_t)b * G.blk;
-    double m = -1.0;
-    for (int p = lane; p < npts; p += step) {
-        const int ix = p % G.bs0, iy = (p / G.bs0) % G.bs1, iz = p / (G.bs0 * G.bs1);
-        const size_t i = ((size_t)(zoff + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
-        double v = u[i] * u[i] + u[G.cs + i] * u[G.cs + i];
-        if (a.dim == 3) v = v + u[2 * G.cs + i] * u[2 * G.cs + i];
-        m = fmax(m, v);
+    const int w = (int)(threadIdx.x >> 6);
+    const int k = WAVES ? (int)blockIdx.x * 4 + w : (int)blockIdx.x;
+    const bool live = !WAVES || k < nact;  // wave-uniform
+    double dt = 0.0;
+    if (live) {
+        const int b = act[k];
+        const int npts = G.bs0 * G.bs1 * G.bs2;
+        const int zoff = (G.bs2 > 1) ? G.g : 0;
+        const double *u = in + (size_t)b * G.blk;
+        double m = -1.0;
+        for (int p = lane; p < npts; p += step) {
+            const int ix = p % G.bs0, iy = (p / G.bs0) % G.bs1, iz = p / (G.bs0 * G.bs1);
+            const size_t i = ((size_t)(zoff + iz) * G.ny + (G.g + iy)) * G.nx + (G.g + ix);
+            double v = u[i] * u[i] + u[G.cs + i] * u[G.cs + i];
+            if (G.bs2 > 1) v = v + u[2 * G.cs + i] * u[2 * G.cs + i];
+            m = fmax(m, v);
+        }
+        m = WAVES ? wave_max(m) : block_max(m);
+        if (lane == 0) {
+            const AcmDev &a = gacm ? gacm[grp[k]] : a0;  // per-group constants of independent forests
+            double dxmin = x0dx[6 * k + 3];
+            for (int d = 1; d < a.dim; ++d) dxmin = fmin(dxmin, x0dx[6 * k + 3 + d]);
+            const double ueig = sqrt(m) + sqrt(a.c_0 * a.c_0 + m);
+            dt = (ueig >= 1.0e-6) ? a.CFL * dxmin / ueig : 1.0e-2;
+            if (a.nu > 1.0e-13) dt = fmin(dt, a.CFL_nu * (dxmin * dxmin) / a.nu);
+            if (a.gamma_p > 0.0) dt = fmin(dt, a.CFL_eta * a.gamma_p);
+            if (a.penal) dt = fmin(dt, a.CFL_eta * a.C_eta);
+            if (a.sponge) dt = fmin(dt, a.CFL_eta * a.C_sponge);
+            if (!WAVES) atomicMin(grp ? dtmin + grp[k] : dtmin, (unsigned long long)__double_as_longlong(dt));
+        }
     }
-    m = WAVES ? wave_max(m) : block_max(m);
-    if (lane == 0) {
-        double dxmin = x0dx[6 * k + 3];
-        for (int d = 1; d < a.dim; ++d) dxmin = fmin(dxmin, x0dx[6 * k + 3 + d]);
-        const double ueig = sqrt(m) + sqrt(a.c_0 * a.c_0 + m);
-        double dt = (ueig >= 1.0e-6) ? a.CFL * dxmin / ueig : 1.0e-2;
-        if (a.nu > 1.0e-13) dt = fmin(dt, a.CFL_nu * (dxmin * dxmin) / a.nu);
-        if (a.gamma_p > 0.0) dt = fmin(dt, a.CFL_eta * a.gamma_p);
-        if (a.penal) dt = fmin(dt, a.CFL_eta * a.C_eta);
-        if (a.sponge) dt = fmin(dt, a.CFL_eta * a.C_sponge);
-        atomicMin(grp ? dtmin + grp[k] : dtmin, (unsigned long long)__double_as_longlong(dt));
+    if (WAVES) {
+        // the workgroup's four blocks (consecutive active entries: mostly one
+        // group) combined first, one atomic per distinct group
+        __shared__ double sdt[4];
+        __shared__ int sgr[4];
+        if (lane == 0) {
+            sdt[w] = dt;
+            sgr[w] = live ? (grp ? grp[k] : 0) : -1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int x = 0; x < 4; ++x) {
+                if (sgr[x] < 0) continue;
+                bool first = true;
+                for (int y = 0; y < x; ++y) first = first && sgr[y] != sgr[x];
+                if (!first) continue;
+                double v = sdt[x];
+                for (int y = x + 1; y < 4; ++y)
+                    if (sgr[y] == sgr[x]) v = fmin(v, sdt[y]);
+                atomicMin(dtmin + sgr[x], (unsigned long long)__double_as_longlong(v));
+            }
     }
 }
 
