@@ -111,3 +111,43 @@ def test_refine_trees_matches_per_forest_refine():
     finally:
         for o in fa + fb + [pa, pb]:
             o.close()
+
+
+def test_create_masks_per_forest_geometry():
+    """wh_create_masks over three forests with different cylinders and
+    sponges (one launch, a geometry per block) equals wh_create_mask of each
+    forest on its own, bit for bit; a forest with an invalid geometry
+    reports WH_E_ARG and leaves its blocks' masks as they were."""
+    slots = 256
+    nslots = slots * len(SEEDS)
+    geo = [dict(geometry=1, x_cntr=(0.4 + 0.1 * i, 0.5, 0.0), R_cyl=0.1 + 0.03 * i, smoothing_width=0.01,
+                smoothing_safety=0.01, use_sponge=True, sponge_type=i % 2, L_sponge=0.1, p_sponge=20.0,
+                domain=(1.0, 1.0, 1.0)) for i in range(len(SEEDS))]
+    fa, fb = _forests(slots), _forests(slots)
+    pa = L.Pool(2, [BS, BS], G, 3, nslots, n_work_slots=1, n_mask=6)
+    pb = L.Pool(2, [BS, BS], G, 3, nslots, n_work_slots=1, n_mask=6)
+    try:
+        print("create_masks over three forests", flush=True)
+        res = W.create_masks(pa, fa, [W.mask_params(**g) for g in geo])
+        assert [c for c, _ in res] == [0] * len(SEEDS)
+        for f, g in zip(fb, geo):
+            _, _, hv, x0dx = f.blocks()
+            W.create_mask(pb, W.mask_params(**g), hv, x0dx)
+        got, want = pa.download(L.WH_MASK, 0, nslots), pb.download(L.WH_MASK, 0, nslots)
+        used = np.concatenate([f.blocks()[2] for f in fa])
+        assert np.array_equal(got[used], want[used])
+        # the middle forest's geometry is invalid (no smoothing width): it alone fails
+        bad = [W.mask_params(**g) for g in geo]
+        bad[1] = W.mask_params(**dict(geo[1], smoothing_width=0.0))
+        before = pa.download(L.WH_MASK, 0, nslots)
+        res = W.create_masks(pa, fa, bad)
+        assert [c for c, _ in res] == [0, L.WH_E_ARG, 0]
+        after = pa.download(L.WH_MASK, 0, nslots)
+        mid = fa[1].blocks()[2]
+        assert np.array_equal(after[mid], before[mid])
+        assert np.array_equal(after[used], want[used])
+    finally:
+        for f in fa + fb:
+            f.close()
+        pa.close()
+        pb.close()

@@ -1709,19 +1709,31 @@ int set_tables_impl(wh_pool *P, wh_forest *const *Fs, int n, int g_rhs, int orde
 // pool, each with its own parameters, one mask launch per forest
 int create_masks_impl(wh_pool *P, wh_forest *const *Fs, int n, const wh_mask_params *mp, int *status)
 {
+    // every forest that fits and whose geometry is valid, in one launch
+    // (its blocks take their forest's geometry through a per-block index)
+    std::vector<wh_mask_params> geos;
+    std::vector<int32_t> gidx, ids;
+    std::vector<double> x0dx;
+    std::vector<int> in;
     for (int i = 0; i < n; ++i) {
         const wh_forest *F = Fs[i];
         if ((status[i] = check_fits(P, F))) continue;
-        std::vector<int32_t> ids(F->blocks.size());
-        std::vector<double> x0dx(6 * F->blocks.size());
-        for (size_t j = 0; j < F->blocks.size(); ++j) {
-            ids[j] = F->blocks[j].hvy;
-            F->x0dx(F->blocks[j], &x0dx[6 * j]);
+        if ((status[i] = check_mask_params(P, mp[i]))) continue;
+        in.push_back(i);
+        const int q = (int)geos.size();
+        geos.push_back(mp[i]);
+        for (const FBlock &b : F->blocks) {
+            ids.push_back(b.hvy);
+            gidx.push_back(q);
+            x0dx.resize(x0dx.size() + 6);
+            F->x0dx(b, &x0dx[x0dx.size() - 6]);
         }
-        status[i] = run_mask(P, mp[i], ids, x0dx);
-        if (status[i] == WH_E_HIP) return status[i];  // the device failed: every forest fails
     }
-    return WH_OK;
+    if (geos.empty()) return WH_OK;
+    if (geos.size() == 1) gidx.clear();
+    const int rc = run_masks(P, geos, gidx, ids, x0dx);
+    for (int i : in) status[i] = rc;
+    return rc == WH_E_HIP ? rc : (int)WH_OK;  // the device failed: every forest fails
 }
 
 }  // namespace

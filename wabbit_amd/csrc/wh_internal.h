@@ -349,6 +349,12 @@ int run_region_max(wh_pool *P, int array, const std::vector<wh_box> &boxes, bool
 // penalisation mask (wh_mask.hip)
 int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &ids,
              const std::vector<double> &x0dx);
+// the masks of several forests' blocks in one launch: mps[gidx[k]] is
+// block k's geometry (gidx empty: one geometry for all); the parameters'
+// checks of run_mask for every geometry
+int check_mask_params(wh_pool *P, const wh_mask_params &mp);
+int run_masks(wh_pool *P, const std::vector<wh_mask_params> &mps, const std::vector<int32_t> &gidx,
+              const std::vector<int32_t> &ids, const std::vector<double> &x0dx);
 int run_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, std::vector<double> &stats);
 
 AcmDev make_acm_dev(const wh_acm_params *p);

@@ -35,12 +35,14 @@ __device__ __forceinline__ double step_cos(double x, double t, double h, double 
 __global__ __launch_bounds__(256) void k_mask(double *__restrict__ mask, size_t mblk, size_t cs,
                                               const int32_t *__restrict__ ids,
                                               const double *__restrict__ x0dx, int nb, int nx, int ny,
-                                              int g, int bs0, int bs1, MaskDev M)
+                                              int g, int bs0, int bs1, const MaskDev *__restrict__ geos,
+                                              const int32_t *__restrict__ gidx)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long)nb * (long)cs) return;
     const int k = (int)(e / (long)cs);
     const int q = (int)(e - (long)k * cs);
+    const MaskDev M = geos[gidx ? gidx[k] : 0];  // the listed block's forest's geometry
     const int ix = q % nx, iy = q / nx;  // 2D: nz = 1
     const double x0 = x0dx[6 * k + 0], y0 = x0dx[6 * k + 1], dx = x0dx[6 * k + 3], dy = x0dx[6 * k + 4];
     double *m = mask + (size_t)ids[k] * mblk + q;
@@ -92,12 +94,14 @@ __global__ __launch_bounds__(256) void k_mask(double *__restrict__ mask, size_t 
 __global__ __launch_bounds__(256) void k_mask3d(double *__restrict__ mask, size_t mblk, size_t cs,
                                                 const int32_t *__restrict__ ids,
                                                 const double *__restrict__ x0dx, int nb, int nx, int ny,
-                                                int nz, int g, int bs0, int bs1, int bs2, MaskDev M)
+                                                int nz, int g, int bs0, int bs1, int bs2,
+                                                const MaskDev *__restrict__ geos, const int32_t *__restrict__ gidx)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long)nb * (long)cs) return;
     const int k = (int)(e / (long)cs);
     const int q = (int)(e - (long)k * cs);
+    const MaskDev M = geos[gidx ? gidx[k] : 0];
     const int ix = q % nx, iy = (q / nx) % ny, iz = q / (nx * ny);
     const double x0 = x0dx[6 * k + 0], y0 = x0dx[6 * k + 1], z0 = x0dx[6 * k + 2];
     const double dx = x0dx[6 * k + 3], dy = x0dx[6 * k + 4], dz = x0dx[6 * k + 5];
@@ -214,10 +218,8 @@ int check_mask_pool(wh_pool *P)
     return WH_OK;
 }
 
-int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &ids,
-             const std::vector<double> &x0dx)
+int check_mask_params(wh_pool *P, const wh_mask_params &mp)
 {
-    if (int rc = check_mask_pool(P)) return rc;
     const int dim = P->d.dim;
     WH_CHECK(mp.geometry == 0 || (dim == 2 && mp.geometry == 1) || (dim == 3 && mp.geometry == 2), WH_E_ARG,
              "mask: unknown geometry (2D: 1 cylinder; 3D: 2 sphere-fixed)");
@@ -227,26 +229,51 @@ int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &i
                   (dim == 2 || std::fabs(mp.domain[2] - mp.domain[0]) <= 1.0e-10)),
              WH_E_ARG, "mask: the p-norm sponge needs a square (cubic) domain");
     WH_CHECK(mp.geometry == 0 || (mp.smoothing_width > 0.0), WH_E_ARG, "mask: smoothing width must be > 0");
-    const int n = (int)ids.size();
+    return WH_OK;
+}
+
+int run_masks(wh_pool *P, const std::vector<wh_mask_params> &mps, const std::vector<int32_t> &gidx,
+              const std::vector<int32_t> &ids, const std::vector<double> &x0dx)
+{
+    if (int rc = check_mask_pool(P)) return rc;
+    for (const wh_mask_params &mp : mps)
+        if (int rc = check_mask_params(P, mp)) return rc;
+    const int n = (int)ids.size(), ng = (int)mps.size();
     if (n == 0) return WH_OK;
+    WH_CHECK(ng > 0 && (ng == 1 ? gidx.empty() : gidx.size() == (size_t)n), WH_E_ARG,
+             "mask: one geometry index per block");
+    for (int32_t q : gidx) WH_CHECK(q >= 0 && q < ng, WH_E_ARG, "mask: geometry index out of range");
     for (int32_t h : ids) WH_CHECK(h >= 0 && h < P->d.nb_max, WH_E_ARG, "mask: hvy id out of range");
     WH_CHECK(x0dx.size() == (size_t)n * 6, WH_E_ARG, "mask: six origin/spacing values per block");
-    if (int rc = stage_begin(P, sizeof(int32_t) * n + sizeof(double) * n * 6)) return rc;
+    std::vector<MaskDev> geos(ng);
+    for (int i = 0; i < ng; ++i) geos[i] = make_mask_dev(mps[i]);
+    if (int rc = stage_begin(P, sizeof(int32_t) * (n + gidx.size()) + sizeof(double) * n * 6 +
+                                    sizeof(MaskDev) * ng))
+        return rc;
     const int32_t *d_ids = (const int32_t *)stage_put(P, ids.data(), sizeof(int32_t) * n, true);
     const double *d_x0dx = (const double *)stage_put(P, x0dx.data(), sizeof(double) * n * 6, true);
+    const MaskDev *d_geos = (const MaskDev *)stage_put(P, geos.data(), sizeof(MaskDev) * ng, true);
+    const int32_t *d_gidx =
+        gidx.empty() ? nullptr : (const int32_t *)stage_put(P, gidx.data(), sizeof(int32_t) * gidx.size(), true);
     if (int rc = stage_flush(P)) return rc;
-    if (!d_ids || !d_x0dx) return WH_E_HIP;
+    if (!d_ids || !d_x0dx || !d_geos || (!gidx.empty() && !d_gidx)) return WH_E_HIP;
     const long tot = (long)n * (long)P->cs;
-    if (dim == 3)
+    if (P->d.dim == 3)
         hipLaunchKernelGGL(k_mask3d, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask,
                            P->mblk, P->cs, d_ids, d_x0dx, n, P->nx, P->ny, P->nz, P->d.g, P->d.bs[0], P->d.bs[1],
-                           P->d.bs[2], make_mask_dev(mp));
+                           P->d.bs[2], d_geos, d_gidx);
     else
         hipLaunchKernelGGL(k_mask, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, P->stream, P->mask,
-                           P->mblk, P->cs, d_ids, d_x0dx, n, P->nx, P->ny, P->d.g, P->d.bs[0], P->d.bs[1],
-                           make_mask_dev(mp));
+                           P->mblk, P->cs, d_ids, d_x0dx, n, P->nx, P->ny, P->d.g, P->d.bs[0], P->d.bs[1], d_geos,
+                           d_gidx);
     WH_HIP(hipGetLastError());
     return WH_OK;
+}
+
+int run_mask(wh_pool *P, const wh_mask_params &mp, const std::vector<int32_t> &ids,
+             const std::vector<double> &x0dx)
+{
+    return run_masks(P, std::vector<wh_mask_params>{mp}, std::vector<int32_t>{}, ids, x0dx);
 }
 
 int launch_mask_stats(wh_pool *P, const std::vector<int32_t> &ids, double *d_out)
