@@ -273,6 +273,7 @@ def _wlib():
         L.or_detail_block.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
         L.or_detail_block_norm.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int,
                                            ctypes.c_int, _dp]
+        L.or_set_norm_plant.argtypes = [ctypes.c_int]
         L.or_renorm_value.argtypes = [ctypes.c_int] * 7 + [ctypes.c_double]
         L.or_renorm_value.restype = ctypes.c_double
         L.or_component_norm.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int,
@@ -353,6 +354,11 @@ def detail_block_norm(dim, bs, g, u, groups, level, eps_norm):
 def renorm_value(eps_norm, dim, level, ix, iy, iz, g, v):
     """wavelet_renorm_block for one value at 0-based point (ix, iy, iz)."""
     return _wlib().or_renorm_value(int(eps_norm), dim, int(level), ix, iy, iz, g, float(v))
+
+
+def set_norm_plant(bits):
+    """Planted mistakes of the eps_norm functions (tests only; 0 restores)."""
+    _wlib().or_set_norm_plant(int(bits))
 
 
 def component_norm(dim, bs, g, blocks, dx, groups, eps_norm):

@@ -317,19 +317,27 @@ void or_filter_block(const or_wavelet *w, int dim, const int bs[3], int g, int n
     free(tmp);
 }
 
+/* planted mistakes of the eps_norm functions (or_set_norm_plant), for the
+ * tests that show their pins have teeth; 0 in every real use */
+static int norm_plant = 0;
+void or_set_norm_plant(int bits) { norm_plant = bits; }
+
 double or_renorm_value(int eps_norm, int dim, int level, int ix, int iy, int iz, int g, double v)
 {
     /* the patch's copy of the block, then the level shift, then the SC
      * rows of x, of y, of z (module_wavelets.f90:1909-1953); Jref = 0 */
     const int Jref = 0;
     const int sc[3] = {((ix - g) & 1) == 0, ((iy - g) & 1) == 0, dim == 3 && ((iz - g) & 1) == 0};
+    /* plant bit 2: the level enters the shift with the wrong sign */
+    const int lv = (norm_plant & 4) ? -level - 2 : level;
     double r = v;
     if (eps_norm == OR_NORM_L1) {
-        r = r * pow(2.0, (double)((Jref - level - 1) * dim));
+        r = r * pow(2.0, (double)((Jref - lv - 1) * dim));
         for (int d = 0; d < dim; ++d)
             if (sc[d]) r = r / 4.0;
     } else if (eps_norm == OR_NORM_L2) {
-        r = r * pow(2.0, (double)((Jref - level - 1) * dim) / 2.0);
+        /* plant bit 0: the L2 shift without its square root (L1's) */
+        r = r * pow(2.0, (double)((Jref - lv - 1) * dim) / ((norm_plant & 1) ? 1.0 : 2.0));
         for (int d = 0; d < dim; ++d)
             if (sc[d]) r = r / 2.0;
     } else if (eps_norm == OR_NORM_H1 && dim == 3) {
@@ -378,6 +386,7 @@ void or_component_norm(int dim, const int bs[3], int g, int neq, const int *grou
         const double *ub = u[k];
         double vol = 1.0;
         for (int d = 0; d < dim; ++d) vol = vol * dx[k * 3 + d];
+        if (norm_plant & 2) vol = 1.0; /* plant bit 1: the cell volume dropped */
         for (int p = 0; p < neq; ++p) {
             if (groups[p] != 1) continue;
             NORM_SECTION(c == p);

@@ -105,6 +105,12 @@ enum { OR_NORM_LINFTY = 0, OR_NORM_L1 = 1, OR_NORM_L2 = 2, OR_NORM_H1 = 3 };
  * 2D leave v as it is. */
 double or_renorm_value(int eps_norm, int dim, int level, int ix, int iy, int iz, int g, double v);
 
+/* Planted mistakes of the eps_norm functions, for tests that show a pin has
+ * teeth (0 in every real use): bit 0 the L2 level shift without its square
+ * root, bit 1 componentWiseNorm without the cell volume, bit 2 the level
+ * with the wrong sign in the shift. */
+void or_set_norm_plant(int bits);
+
 /* or_detail_block with the renormalisation of eps_norm for a block on
  * `level` (threshold_block.f90:69-124); eps_norm OR_NORM_LINFTY is
  * or_detail_block. */

@@ -45,28 +45,28 @@ def test_linfty_and_h1_2d_leave_details(dim):
         assert po.renorm_value(po.NORM_LINFTY, dim, 3, 4, 5, 6, g, v) == v
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("name", ["L1", "L2"])
-def test_detail_scales_with_level_like_the_norm(dim, name):
+def _level_scaling_ok(dim, kind):
     """One level finer: renormed details and componentWiseNorm of the same
     data shrink by the same factor, 2^(-dim/p) for Lp."""
-    kind = NORMS[name]
     bs, g = 8, 3
     u = _block(dim, bs, g, 2, 7)
+    expect = 2.0 ** (-dim / (1.0 if kind == po.NORM_L1 else 2.0))
     for lev in (0, 2, 5):
         d0 = po.detail_block_norm(dim, [bs] * dim, g, u, [1, 1], lev, kind)
         d1 = po.detail_block_norm(dim, [bs] * dim, g, u, [1, 1], lev + 1, kind)
         dx0 = np.full(dim, 2.0 ** -lev / bs)
         n0 = po.component_norm(dim, [bs] * dim, g, [u], [dx0], [1, 1], kind)
         n1 = po.component_norm(dim, [bs] * dim, g, [u], [dx0 / 2], [1, 1], kind)
-        expect = 2.0 ** (-dim / (1.0 if kind == po.NORM_L1 else 2.0))
-        assert np.allclose(n1 / n0, expect, rtol=1e-14, atol=0)
-        assert np.allclose(d1 / d0, n1 / n0, rtol=1e-14, atol=0), (lev, d1 / d0, n1 / n0)
-    # the check has teeth: L1's detail scaling is not L2's norm scaling
-    other = po.NORM_L2 if kind == po.NORM_L1 else po.NORM_L1
-    d0 = po.detail_block_norm(dim, [bs] * dim, g, u, [1, 1], 2, other)
-    d1 = po.detail_block_norm(dim, [bs] * dim, g, u, [1, 1], 3, other)
-    assert not np.allclose(d1 / d0, expect, rtol=1e-6)
+        if not (np.allclose(n1 / n0, expect, rtol=1e-14, atol=0) and
+                np.allclose(d1 / d0, n1 / n0, rtol=1e-14, atol=0)):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("name", ["L1", "L2"])
+def test_detail_scales_with_level_like_the_norm(dim, name):
+    assert _level_scaling_ok(dim, NORMS[name])
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -91,8 +91,7 @@ def _covering(dim, bs, L):
     return dxs
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_component_norm_closed_forms(dim):
+def _closed_forms_ok(dim):
     bs, g = 6, 2
     L = [1.0, 2.0, 0.5][:dim]
     area = float(np.prod(L))
@@ -103,16 +102,40 @@ def test_component_norm_closed_forms(dim):
     blk[0], blk[1], blk[2] = a, b, c
     blocks = [blk] * len(dxs)
     l2 = po.component_norm(dim, [bs] * dim, g, blocks, dxs, [1, 1, 1], po.NORM_L2)
-    assert np.allclose(l2, np.abs([a, b, c]) * np.sqrt(area), rtol=1e-14, atol=0)
     h1 = po.component_norm(dim, [bs] * dim, g, blocks, dxs, [1, 1, 1], po.NORM_H1)
-    assert np.array_equal(h1, l2)        # componentWiseNorm takes H1 as L2
     l1 = po.component_norm(dim, [bs] * dim, g, blocks, dxs, [1, 1, 1], po.NORM_L1)
-    assert np.allclose(l1, np.abs([a, b, c]) * area, rtol=1e-14, atol=0)
     grp = po.component_norm(dim, [bs] * dim, g, blocks, dxs, [2, 2, 0], po.NORM_L2)
-    assert np.allclose(grp[:2], np.hypot(a, b) * np.sqrt(area), rtol=1e-14, atol=0)
-    assert grp[2] == 0.0
     inf = po.component_norm(dim, [bs] * dim, g, blocks, dxs, [2, 2, 1], po.NORM_LINFTY)
-    assert np.array_equal(inf, [max(abs(a), abs(b))] * 2 + [abs(c)])
+    return (np.allclose(l2, np.abs([a, b, c]) * np.sqrt(area), rtol=1e-14, atol=0) and
+            np.array_equal(h1, l2) and   # componentWiseNorm takes H1 as L2
+            np.allclose(l1, np.abs([a, b, c]) * area, rtol=1e-14, atol=0) and
+            np.allclose(grp[:2], np.hypot(a, b) * np.sqrt(area), rtol=1e-14, atol=0) and grp[2] == 0.0 and
+            np.array_equal(inf, [max(abs(a), abs(b))] * 2 + [abs(c)]))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_component_norm_closed_forms(dim):
+    assert _closed_forms_ok(dim)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_planted_mistakes_fail_the_pins(dim):
+    """Each planted mistake of the oracle's eps_norm functions fails a pin:
+    the L2 shift without its square root and the level with the wrong sign
+    fail the level scaling, the norm without the cell volume fails the
+    closed forms."""
+    try:
+        po.set_norm_plant(1)
+        assert not _level_scaling_ok(dim, po.NORM_L2)
+        assert _level_scaling_ok(dim, po.NORM_L1)      # the plant touches L2 only
+        po.set_norm_plant(4)
+        assert not _level_scaling_ok(dim, po.NORM_L1)
+        assert not _level_scaling_ok(dim, po.NORM_L2)
+        po.set_norm_plant(2)
+        assert not _closed_forms_ok(dim)
+    finally:
+        po.set_norm_plant(0)
+    assert _level_scaling_ok(dim, po.NORM_L2) and _closed_forms_ok(dim)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
